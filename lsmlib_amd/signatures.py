@@ -1,0 +1,122 @@
+"""Argument layout of every LSM3D_* entry point the library exports.
+
+LSMLIB's toolbox passes every argument by reference (Fortran-77 calling
+convention, C prototypes in src/toolbox/*/lsm_*3d.h).  Each entry below spells
+one prototype as a sequence of argument kinds:
+
+  A  LSMLIB_REAL*      array (column-major, i fastest)
+  B  6 x const int*    ghost/fill/include box  ilo, ihi, jlo, jhi, klo, khi
+  R  const LSMLIB_REAL* scalar input
+  O  LSMLIB_REAL*      scalar output
+  I  const int*        integer input
+
+The same table drives the ctypes marshalling for the product library and lets
+tests check that every declared symbol is exported.
+"""
+
+# (symbol, kinds, reference prototype file:line)
+_TABLE = [
+    # spatial derivatives -- lsm_spatial_derivatives3d.h
+    ("lsm3dhjeno1_", "AAAB AAAB AB AB B RRR", "lsm_spatial_derivatives3d.h:83"),
+    ("lsm3dhjeno2_", "AAAB AAAB AB AB AB B RRR", "lsm_spatial_derivatives3d.h:159"),
+    ("lsm3dhjeno3_", "AAAB AAAB AB AB AB AB B RRR", "lsm_spatial_derivatives3d.h:243"),
+    ("lsm3dhjweno5_", "AAAB AAAB AB AB B RRR", "lsm_spatial_derivatives3d.h:332"),
+    ("lsm3dupwindhjeno1_", "AAAB AB AAAB AB B RRR", "lsm_spatial_derivatives3d.h:395"),
+    ("lsm3dupwindhjeno2_", "AAAB AB AAAB AB AB B RRR", "lsm_spatial_derivatives3d.h:459"),
+    ("lsm3dupwindhjeno3_", "AAAB AB AAAB AB AB AB B RRR", "lsm_spatial_derivatives3d.h:531"),
+    ("lsm3dupwindhjweno5_", "AAAB AB AAAB AB B RRR", "lsm_spatial_derivatives3d.h:608"),
+    # level set equation RHS -- lsm_level_set_evolution3d.h
+    ("lsm3dzerooutlevelseteqnrhs_", "AB", "lsm_level_set_evolution3d.h:64"),
+    ("lsm3daddadvectiontermtolserhs_", "AB AAAB AAAB B", "lsm_level_set_evolution3d.h:94"),
+    ("lsm3daddnormalveltermtolserhs_", "AB AAAB AAAB AB B", "lsm_level_set_evolution3d.h:151"),
+    ("lsm3daddconstnormalveltermtolserhs_", "AB AAAB AAAB R B", "lsm_level_set_evolution3d.h:217"),
+    ("lsm3daddexternalandnormalveltermtolserhs_", "AB AAAB AAAB AAAAB B",
+     "lsm_level_set_evolution3d.h:398"),
+    # TVD Runge-Kutta -- lsm_tvd_runge_kutta3d.h
+    ("lsm3drk1step_", "AB AB AB B R", "lsm_tvd_runge_kutta3d.h:59"),
+    ("lsm3dtvdrk2stage1_", "AB AB AB B R", "lsm_tvd_runge_kutta3d.h:108"),
+    ("lsm3dtvdrk2stage2_", "AB AB AB AB B R", "lsm_tvd_runge_kutta3d.h:154"),
+    ("lsm3dtvdrk3stage1_", "AB AB AB B R", "lsm_tvd_runge_kutta3d.h:210"),
+    ("lsm3dtvdrk3stage2_", "AB AB AB AB B R", "lsm_tvd_runge_kutta3d.h:256"),
+    ("lsm3dtvdrk3stage3_", "AB AB AB AB B R", "lsm_tvd_runge_kutta3d.h:309"),
+    # reinitialization -- lsm_reinitialization3d.h
+    ("lsm3dcomputereinitializationeqnrhs_", "AB AB AB AAAB AAAB B RRR I",
+     "lsm_reinitialization3d.h:77"),
+    ("lsm3dcomputeorthogonalizationeqnrhs_", "AB AAAB AAAB AB AAAB AAAB B RRR",
+     "lsm_reinitialization3d.h:163"),
+    ("lsm3dcomputedistanceforsubcellfix_", "AAB B RRR", "lsm_reinitialization3d.h:247"),
+    ("lsm3dcomputereinitializationeqnrhssubcellfixorder1_", "AB AB AAB AAAB AAAB B RRR",
+     "lsm_reinitialization3d.h:295"),
+    # reductions -- lsm_utilities3d.h, lsm_geometry3d.h
+    ("lsm3dmaxnormdiff_", "O AB AB B", "lsm_utilities3d.h:87"),
+    ("lsm3dmaxnormdiffcontrolvolume_", "O AB AB AB I B", "lsm_utilities3d.h:533"),
+    ("lsm3dcomputestableadvectiondt_", "O AAAB B RRR R", "lsm_utilities3d.h:221"),
+    ("lsm3dcomputestableadvectiondtcontrolvolume_", "O AAAB AB I B RRR R",
+     "lsm_utilities3d.h:585"),
+    ("lsm3dcomputestablenormalveldt_", "O AB AAAB AAAB B RRR R", "lsm_utilities3d.h:268"),
+    ("lsm3dcomputestableconstnormalveldt_", "O R AAAB AAAB B RRR R", "lsm_utilities3d.h:331"),
+    ("lsm3dvolumeintegralphilessthanzero_", "O AB AB B RRR R", "lsm_utilities3d.h:383"),
+    ("lsm3dvolumeintegralphigreaterthanzero_", "O AB AB B RRR R", "lsm_utilities3d.h:430"),
+    ("lsm3dvolumeregionphilessthanzero_", "O AB B RRR R", "lsm_geometry3d.h:180"),
+    ("lsm3dvolumeregionphigreaterthanzero_", "O AB B RRR R", "lsm_geometry3d.h:217"),
+    # boundary conditions -- lsm_boundary_conditions3d.h
+    ("lsm3dlinearextrapolation_", "AB B I", "lsm_boundary_conditions3d.h:71"),
+    ("lsm3dsignedlinearextrapolation_", "AB B I", "lsm_boundary_conditions3d.h:111"),
+    ("lsm3dcopyextrapolation_", "AB B I", "lsm_boundary_conditions3d.h:135"),
+    ("lsm3dhomogeneousneumanneno1_", "AB B I", "lsm_boundary_conditions3d.h:161"),
+    ("lsm3dhomogeneousneumanneno2_", "AB B I", "lsm_boundary_conditions3d.h:186"),
+    ("lsm3dhomogeneousneumanneno3_", "AB B I", "lsm_boundary_conditions3d.h:211"),
+    ("lsm3dhomogeneousneumannweno5_", "AB B I", "lsm_boundary_conditions3d.h:236"),
+]
+
+SIGNATURES = {name: kinds.replace(" ", "") for name, kinds, _ in _TABLE}
+CITATIONS = {name: cite for name, _, cite in _TABLE}
+
+# The C macro name each F77 symbol is reached through in the reference headers.
+MACRO_OF = {
+    "lsm3dhjeno1_": "LSM3D_HJ_ENO1",
+    "lsm3dhjeno2_": "LSM3D_HJ_ENO2",
+    "lsm3dhjeno3_": "LSM3D_HJ_ENO3",
+    "lsm3dhjweno5_": "LSM3D_HJ_WENO5",
+    "lsm3dupwindhjeno1_": "LSM3D_UPWIND_HJ_ENO1",
+    "lsm3dupwindhjeno2_": "LSM3D_UPWIND_HJ_ENO2",
+    "lsm3dupwindhjeno3_": "LSM3D_UPWIND_HJ_ENO3",
+    "lsm3dupwindhjweno5_": "LSM3D_UPWIND_HJ_WENO5",
+    "lsm3dzerooutlevelseteqnrhs_": "LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS",
+    "lsm3daddadvectiontermtolserhs_": "LSM3D_ADD_ADVECTION_TERM_TO_LSE_RHS",
+    "lsm3daddnormalveltermtolserhs_": "LSM3D_ADD_NORMAL_VEL_TERM_TO_LSE_RHS",
+    "lsm3daddconstnormalveltermtolserhs_": "LSM3D_ADD_CONST_NORMAL_VEL_TERM_TO_LSE_RHS",
+    "lsm3daddexternalandnormalveltermtolserhs_":
+        "LSM3D_ADD_EXTERNAL_AND_NORMAL_VEL_TERM_TO_LSE_RHS",
+    "lsm3drk1step_": "LSM3D_RK1_STEP",
+    "lsm3dtvdrk2stage1_": "LSM3D_TVD_RK2_STAGE1",
+    "lsm3dtvdrk2stage2_": "LSM3D_TVD_RK2_STAGE2",
+    "lsm3dtvdrk3stage1_": "LSM3D_TVD_RK3_STAGE1",
+    "lsm3dtvdrk3stage2_": "LSM3D_TVD_RK3_STAGE2",
+    "lsm3dtvdrk3stage3_": "LSM3D_TVD_RK3_STAGE3",
+    "lsm3dcomputereinitializationeqnrhs_": "LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS",
+    "lsm3dcomputeorthogonalizationeqnrhs_": "LSM3D_COMPUTE_ORTHOGONALIZATION_EQN_RHS",
+    "lsm3dcomputedistanceforsubcellfix_": "LSM3D_COMPUTE_DISTANCE_FOR_SUBCELL_FIX",
+    "lsm3dcomputereinitializationeqnrhssubcellfixorder1_":
+        "LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS_SUBCELL_FIX_ORDER1",
+    "lsm3dmaxnormdiff_": "LSM3D_MAX_NORM_DIFF",
+    "lsm3dmaxnormdiffcontrolvolume_": "LSM3D_MAX_NORM_DIFF_CONTROL_VOLUME",
+    "lsm3dcomputestableadvectiondt_": "LSM3D_COMPUTE_STABLE_ADVECTION_DT",
+    "lsm3dcomputestableadvectiondtcontrolvolume_":
+        "LSM3D_COMPUTE_STABLE_ADVECTION_DT_CONTROL_VOLUME",
+    "lsm3dcomputestablenormalveldt_": "LSM3D_COMPUTE_STABLE_NORMAL_VEL_DT",
+    "lsm3dcomputestableconstnormalveldt_": "LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT",
+    "lsm3dvolumeintegralphilessthanzero_": "LSM3D_VOLUME_INTEGRAL_PHI_LESS_THAN_ZERO",
+    "lsm3dvolumeintegralphigreaterthanzero_": "LSM3D_VOLUME_INTEGRAL_PHI_GREATER_THAN_ZERO",
+    "lsm3dvolumeregionphilessthanzero_": "LSM3D_VOLUME_REGION_PHI_LESS_THAN_ZERO",
+    "lsm3dvolumeregionphigreaterthanzero_": "LSM3D_VOLUME_REGION_PHI_GREATER_THAN_ZERO",
+    "lsm3dlinearextrapolation_": "LSM3D_LINEAR_EXTRAPOLATION",
+    "lsm3dsignedlinearextrapolation_": "LSM3D_SIGNED_LINEAR_EXTRAPOLATION",
+    "lsm3dcopyextrapolation_": "LSM3D_COPY_EXTRAPOLATION",
+    "lsm3dhomogeneousneumanneno1_": "LSM3D_HOMOGENEOUS_NEUMANN_ENO1",
+    "lsm3dhomogeneousneumanneno2_": "LSM3D_HOMOGENEOUS_NEUMANN_ENO2",
+    "lsm3dhomogeneousneumanneno3_": "LSM3D_HOMOGENEOUS_NEUMANN_ENO3",
+    "lsm3dhomogeneousneumannweno5_": "LSM3D_HOMOGENEOUS_NEUMANN_WENO5",
+}
+
+assert set(MACRO_OF) == set(SIGNATURES)

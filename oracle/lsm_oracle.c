@@ -1,0 +1,1108 @@
+/*
+ * lsm_oracle.c -- CPU restatement of LSMLIB's 3D level-set update toolbox.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP library is
+ * compared against; it is never linked into, loaded by, or called from the
+ * lsmlib_amd product.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may use it.
+ *
+ * Every routine restates one reference Fortran-77 subroutine loop for loop,
+ * with the reference's operation order and constants, and is exported under
+ * the reference's by-reference F77 signature with an "oracle_" prefix (e.g.
+ * oracle_lsm3dhjweno5_ for lsm3dhjweno5_), so a test can call the product,
+ * this restatement and the flang build of the reference with one argument
+ * list.  File:line citations are relative to the LSMLIB source tree.
+ *
+ * Precision mirrors the reference's build switch (configure.ac:258-272):
+ *   -DORACLE_REAL_DOUBLE  real = double, zero_tol = 1e-11, tiny = 1e-99
+ *   (default)             real = float,  zero_tol = 1e-5,  tiny = 1e-35
+ * Fortran "d0" literals are written as C double literals and default-real
+ * literals as (real) casts, so the float build keeps the reference's mixed
+ * precision promotion (a float times 0.25d0 is evaluated in double).
+ *
+ * Compiled with -ffp-contract=off: the reference objects contain no fused
+ * multiply-adds.
+ */
+#include <math.h>
+#include <stddef.h>
+
+#ifdef ORACLE_REAL_DOUBLE
+typedef double real;
+#define ZERO_TOL ((real)1e-11)
+#define TINY ((real)1e-99)
+#define SIN sin
+#else
+typedef float real;
+#define ZERO_TOL ((real)1e-5f)
+#define TINY ((real)1e-35)
+#define SIN sinf
+#endif
+
+/* Fortran MAX/MIN of two reals: first argument wins ties and NaN handling
+ * follows "a > b ? a : b" as flang emits it for real operands. */
+static inline double fmax2(double a, double b) { return (a > b) ? a : b; }
+static inline double fmin2(double a, double b) { return (a < b) ? a : b; }
+
+/* ---------------------------------------------------------------------- */
+/* Boxes: a column-major array a(ilo:ihi, jlo:jhi, klo:khi), i fastest.    */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+  int ilo, ihi, jlo, jhi, klo, khi;
+  long nx, nxy;
+} box_t;
+
+#define BOXARGS(p)                                                        \
+  const int *ilo_##p, const int *ihi_##p, const int *jlo_##p,            \
+      const int *jhi_##p, const int *klo_##p, const int *khi_##p
+#define MKBOX(p) mkbox(*ilo_##p, *ihi_##p, *jlo_##p, *jhi_##p, *klo_##p, *khi_##p)
+
+static box_t mkbox(int ilo, int ihi, int jlo, int jhi, int klo, int khi) {
+  box_t b;
+  b.ilo = ilo; b.ihi = ihi; b.jlo = jlo; b.jhi = jhi; b.klo = klo; b.khi = khi;
+  b.nx = (long)(ihi - ilo + 1);
+  b.nxy = b.nx * (long)(jhi - jlo + 1);
+  return b;
+}
+
+static inline long at(const box_t *b, int i, int j, int k) {
+  return (long)(i - b->ilo) + b->nx * (long)(j - b->jlo) + b->nxy * (long)(k - b->klo);
+}
+#define A(arr, b, i, j, k) (arr)[at(&(b), (i), (j), (k))]
+
+/* ---------------------------------------------------------------------- */
+/* lsm3dComputeDn  lsm_spatial_derivatives3d.f.in:49-176                   */
+/* fb[6] = {ilo,ihi,jlo,jhi,klo,khi} of the (possibly widened) fill box.   */
+/* ---------------------------------------------------------------------- */
+static void compute_dn(real *Dn, box_t bd, const real *Dm, box_t bm,
+                       const int fb[6], int n, int dir) {
+  const real big = (real)1.e10;            /* :87 parameter (big=1.d10) */
+  int off[3] = {0, 0, 0}, sh[3] = {0, 0, 0};
+  real sgn;
+  if (n % 2 == 1) {                        /* :99-103 odd: face-centred */
+    off[dir] = 1; sgn = (real)1.0; sh[dir] = 1;
+  } else {                                 /* :104-108 even: centred */
+    off[dir] = -1; sgn = (real)-1.0; sh[dir] = 0;
+  }
+  for (int k = fb[4]; k <= fb[5] + sh[2]; k++)           /* :111-121 */
+    for (int j = fb[2]; j <= fb[3] + sh[1]; j++)
+      for (int i = fb[0]; i <= fb[1] + sh[0]; i++)
+        A(Dn, bd, i, j, k) =
+            sgn * (A(Dm, bm, i, j, k) - A(Dm, bm, i - off[0], j - off[1], k - off[2]));
+  /* :124-170 sentinel outside the computed range */
+  for (int k = bd.klo; k <= bd.khi; k++)
+    for (int j = bd.jlo; j <= bd.jhi; j++) {
+      for (int i = bd.ilo; i <= fb[0] - 1; i++) A(Dn, bd, i, j, k) = big;
+      for (int i = fb[1] + sh[0] + 1; i <= bd.ihi; i++) A(Dn, bd, i, j, k) = big;
+    }
+  for (int k = bd.klo; k <= bd.khi; k++) {
+    for (int j = bd.jlo; j <= fb[2] - 1; j++)
+      for (int i = bd.ilo; i <= bd.ihi; i++) A(Dn, bd, i, j, k) = big;
+    for (int j = fb[3] + sh[1] + 1; j <= bd.jhi; j++)
+      for (int i = bd.ilo; i <= bd.ihi; i++) A(Dn, bd, i, j, k) = big;
+  }
+  for (int k = bd.klo; k <= fb[4] - 1; k++)
+    for (int j = bd.jlo; j <= bd.jhi; j++)
+      for (int i = bd.ilo; i <= bd.ihi; i++) A(Dn, bd, i, j, k) = big;
+  for (int k = fb[5] + sh[2] + 1; k <= bd.khi; k++)
+    for (int j = bd.jlo; j <= bd.jhi; j++)
+      for (int i = bd.ilo; i <= bd.ihi; i++) A(Dn, bd, i, j, k) = big;
+}
+
+/* fill box widened by (lo, hi) cells along dir, as the callers of
+ * lsm3dComputeDn pass it (e.g. ilo_fb-2, ihi_fb+2 at :1277-1279). */
+static void widen(int out[6], const int fb[6], int dir, int lo, int hi) {
+  for (int d = 0; d < 6; d++) out[d] = fb[d];
+  out[2 * dir] -= lo;
+  out[2 * dir + 1] += hi;
+}
+
+/* unit offset along dir */
+#define DI(d) ((d) == 0)
+#define DJ(d) ((d) == 1)
+#define DK(d) ((d) == 2)
+
+/* ---------------------------------------------------------------------- */
+/* lsm3dHJENO1  lsm_spatial_derivatives3d.f.in:199-369                      */
+/* ---------------------------------------------------------------------- */
+void oracle_lsm3dhjeno1_(real *px_p, real *py_p, real *pz_p, BOXARGS(gp),
+                         real *px_m, real *py_m, real *pz_m, BOXARGS(gm),
+                         const real *phi, BOXARGS(phi), real *D1, BOXARGS(d1),
+                         BOXARGS(fb), const real *dx, const real *dy, const real *dz) {
+  box_t bp = MKBOX(gp), bm = MKBOX(gm), bphi = MKBOX(phi), bd1 = MKBOX(d1);
+  int fb[6] = {*ilo_fb, *ihi_fb, *jlo_fb, *jhi_fb, *klo_fb, *khi_fb};
+  real inv[3];
+  inv[0] = 1.0 / *dx; inv[1] = 1.0 / *dy; inv[2] = 1.0 / *dz;   /* :276-278 */
+  real *outp[3] = {px_p, py_p, pz_p}, *outm[3] = {px_m, py_m, pz_m};
+  for (int d = 0; d < 3; d++) {
+    compute_dn(D1, bd1, phi, bphi, fb, 1, d);                       /* :284-296 */
+    for (int k = fb[4]; k <= fb[5]; k++)
+      for (int j = fb[2]; j <= fb[3]; j++)
+        for (int i = fb[0]; i <= fb[1]; i++) {                      /* :299-308 */
+          A(outp[d], bp, i, j, k) = A(D1, bd1, i + DI(d), j + DJ(d), k + DK(d)) * inv[d];
+          A(outm[d], bm, i, j, k) = A(D1, bd1, i, j, k) * inv[d];
+        }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* lsm3dHJENO2  lsm_spatial_derivatives3d.f.in:393-664                      */
+/* ---------------------------------------------------------------------- */
+void oracle_lsm3dhjeno2_(real *px_p, real *py_p, real *pz_p, BOXARGS(gp),
+                         real *px_m, real *py_m, real *pz_m, BOXARGS(gm),
+                         const real *phi, BOXARGS(phi), real *D1, BOXARGS(d1),
+                         real *D2, BOXARGS(d2), BOXARGS(fb),
+                         const real *dx, const real *dy, const real *dz) {
+  box_t bp = MKBOX(gp), bm = MKBOX(gm), bphi = MKBOX(phi), bd1 = MKBOX(d1), bd2 = MKBOX(d2);
+  int fb[6] = {*ilo_fb, *ihi_fb, *jlo_fb, *jhi_fb, *klo_fb, *khi_fb}, w[6];
+  const real half = (real)0.5;                                       /* :471 */
+  real inv[3];
+  inv[0] = 1.0 / *dx; inv[1] = 1.0 / *dy; inv[2] = 1.0 / *dz;
+  real *outp[3] = {px_p, py_p, pz_p}, *outm[3] = {px_m, py_m, pz_m};
+  for (int d = 0; d < 3; d++) {
+    int a = DI(d), b = DJ(d), c = DK(d);
+    widen(w, fb, d, 1, 1);                                           /* :485-511 */
+    compute_dn(D1, bd1, phi, bphi, w, 1, d);
+    compute_dn(D2, bd2, D1, bd1, w, 2, d);
+    for (int k = fb[4]; k <= fb[5]; k++)
+      for (int j = fb[2]; j <= fb[3]; j++)
+        for (int i = fb[0]; i <= fb[1]; i++) {                      /* :518-536 */
+          real d2m = A(D2, bd2, i - a, j - b, k - c), d20 = A(D2, bd2, i, j, k);
+          real d2p = A(D2, bd2, i + a, j + b, k + c);
+          if (fabs(d20) < fabs(d2p))
+            A(outp[d], bp, i, j, k) = (A(D1, bd1, i + a, j + b, k + c) - half * d20) * inv[d];
+          else
+            A(outp[d], bp, i, j, k) = (A(D1, bd1, i + a, j + b, k + c) - half * d2p) * inv[d];
+          if (fabs(d2m) < fabs(d20))
+            A(outm[d], bm, i, j, k) = (A(D1, bd1, i, j, k) + half * d2m) * inv[d];
+          else
+            A(outm[d], bm, i, j, k) = (A(D1, bd1, i, j, k) + half * d20) * inv[d];
+        }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* lsm3dHJENO3  lsm_spatial_derivatives3d.f.in:689-1138                     */
+/* ---------------------------------------------------------------------- */
+void oracle_lsm3dhjeno3_(real *px_p, real *py_p, real *pz_p, BOXARGS(gp),
+                         real *px_m, real *py_m, real *pz_m, BOXARGS(gm),
+                         const real *phi, BOXARGS(phi), real *D1, BOXARGS(d1),
+                         real *D2, BOXARGS(d2), real *D3, BOXARGS(d3), BOXARGS(fb),
+                         const real *dx, const real *dy, const real *dz) {
+  box_t bp = MKBOX(gp), bm = MKBOX(gm), bphi = MKBOX(phi);
+  box_t bd1 = MKBOX(d1), bd2 = MKBOX(d2), bd3 = MKBOX(d3);
+  int fb[6] = {*ilo_fb, *ihi_fb, *jlo_fb, *jhi_fb, *klo_fb, *khi_fb}, w2[6], w1[6];
+  const real half = (real)0.5, third = (real)(1.0 / 3.0), sixth = (real)(1.0 / 6.0); /* :776-778 */
+  real inv[3];
+  inv[0] = 1.0 / *dx; inv[1] = 1.0 / *dy; inv[2] = 1.0 / *dz;
+  real *outp[3] = {px_p, py_p, pz_p}, *outm[3] = {px_m, py_m, pz_m};
+  for (int d = 0; d < 3; d++) {
+    int a = DI(d), b = DJ(d), c = DK(d);
+    widen(w2, fb, d, 2, 2);                                          /* :793-832 */
+    widen(w1, fb, d, 1, 1);
+    compute_dn(D1, bd1, phi, bphi, w2, 1, d);
+    compute_dn(D2, bd2, D1, bd1, w2, 2, d);
+    compute_dn(D3, bd3, D2, bd2, w1, 3, d);
+    for (int k = fb[4]; k <= fb[5]; k++)
+      for (int j = fb[2]; j <= fb[3]; j++)
+        for (int i = fb[0]; i <= fb[1]; i++) {
+#define D2AT(s) A(D2, bd2, i + (s) * a, j + (s) * b, k + (s) * c)
+#define D3AT(s) A(D3, bd3, i + (s) * a, j + (s) * b, k + (s) * c)
+          real v;                                                   /* :838-868 plus */
+          v = A(D1, bd1, i + a, j + b, k + c);
+          if (fabs(D2AT(0)) < fabs(D2AT(1))) {
+            v = v - half * D2AT(0);
+            if (fabs(D3AT(0)) < fabs(D3AT(1))) v = v - sixth * D3AT(0);
+            else v = v - sixth * D3AT(1);
+          } else {
+            v = v - half * D2AT(1);
+            if (fabs(D3AT(1)) < fabs(D3AT(2))) v = v + third * D3AT(1);
+            else v = v + third * D3AT(2);
+          }
+          A(outp[d], bp, i, j, k) = v * inv[d];
+          v = A(D1, bd1, i, j, k);                                  /* :872-896 minus */
+          if (fabs(D2AT(-1)) < fabs(D2AT(0))) {
+            v = v + half * D2AT(-1);
+            if (fabs(D3AT(-1)) < fabs(D3AT(0))) v = v + third * D3AT(-1);
+            else v = v + third * D3AT(0);
+          } else {
+            v = v + half * D2AT(0);
+            if (fabs(D3AT(0)) < fabs(D3AT(1))) v = v - sixth * D3AT(0);
+            else v = v - sixth * D3AT(1);
+          }
+          A(outm[d], bm, i, j, k) = v * inv[d];
+#undef D2AT
+#undef D3AT
+        }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* The HJ-WENO5 formula for one side of one axis,                           */
+/* lsm_spatial_derivatives3d.f.in:1292-1326 (identical in every variant).   */
+/* ---------------------------------------------------------------------- */
+static real weno5(real v1, real v2, real v3, real v4, real v5) {
+  const real one_third = (real)(1.0 / 3.0), seven_sixths = (real)(7.0 / 6.0);
+  const real eleven_sixths = (real)(11.0 / 6.0), one_sixth = (real)(1.0 / 6.0);
+  const real five_sixths = (real)(5.0 / 6.0), thirteen_twelfths = (real)(13.0 / 12.0);
+  const real one_fourth = (real)0.25;                                /* :1240-1247 */
+  real eps, p1, p2, p3, S1, S2, S3, a1, a2, a3, inv_sum_a, m;
+  /* :1296-1297  eps = 1e-6*max(v1*v1,...,v5*v5) + tiny  (MAX left to right) */
+  m = v1 * v1;
+  m = fmax2(m, v2 * v2);
+  m = fmax2(m, v3 * v3);
+  m = fmax2(m, v4 * v4);
+  m = fmax2(m, v5 * v5);
+  eps = (real)1e-6 * m + TINY;
+  /* :1300-1303 */
+  p1 = one_third * v1 - seven_sixths * v2 + eleven_sixths * v3;
+  p2 = -one_sixth * v2 + five_sixths * v3 + one_third * v4;
+  p3 = one_third * v3 + five_sixths * v4 - one_sixth * v5;
+  /* :1306-1311  (x)**2 evaluated as x*x, "2.d0*" terms promote to double */
+  {
+    double t1 = v1 - 2.0 * v2 + v3, t2 = v1 - 4.0 * v2 + 3.0 * v3;
+    S1 = thirteen_twelfths * (t1 * t1) + one_fourth * (t2 * t2);
+    double t3 = v2 - 2.0 * v3 + v4;
+    real t4 = v2 - v4;
+    S2 = thirteen_twelfths * (t3 * t3) + one_fourth * (t4 * t4);
+    double t5 = v3 - 2.0 * v4 + v5, t6 = 3.0 * v3 - 4.0 * v4 + v5;
+    S3 = thirteen_twelfths * (t5 * t5) + one_fourth * (t6 * t6);
+  }
+  /* :1314-1320 */
+  {
+    real s;
+    s = S1 + eps; a1 = 0.1 / (s * s);
+    s = S2 + eps; a2 = 0.6 / (s * s);
+    s = S3 + eps; a3 = 0.3 / (s * s);
+  }
+  inv_sum_a = 1.0 / (a1 + a2 + a3);
+  a1 = a1 * inv_sum_a;
+  a2 = a2 * inv_sum_a;
+  a3 = a3 * inv_sum_a;
+  return a1 * p1 + a2 * p2 + a3 * p3;                                /* :1323 */
+}
+
+/* ---------------------------------------------------------------------- */
+/* lsm3dHJWENO5  lsm_spatial_derivatives3d.f.in:1161-1612                   */
+/* ---------------------------------------------------------------------- */
+void oracle_lsm3dhjweno5_(real *px_p, real *py_p, real *pz_p, BOXARGS(gp),
+                          real *px_m, real *py_m, real *pz_m, BOXARGS(gm),
+                          const real *phi, BOXARGS(phi), real *D1, BOXARGS(d1),
+                          BOXARGS(fb), const real *dx, const real *dy, const real *dz) {
+  box_t bp = MKBOX(gp), bm = MKBOX(gm), bphi = MKBOX(phi), bd1 = MKBOX(d1);
+  int fb[6] = {*ilo_fb, *ihi_fb, *jlo_fb, *jhi_fb, *klo_fb, *khi_fb}, w[6];
+  real inv[3];
+  inv[0] = 1.0 / *dx; inv[1] = 1.0 / *dy; inv[2] = 1.0 / *dz;      /* :1256-1258 */
+  real *outp[3] = {px_p, py_p, pz_p}, *outm[3] = {px_m, py_m, pz_m};
+  for (int d = 0; d < 3; d++) {
+    int a = DI(d), b = DJ(d), c = DK(d);
+    widen(w, fb, d, 2, 2);                                           /* :1265-1277 */
+    compute_dn(D1, bd1, phi, bphi, w, 1, d);
+    for (int k = fb[4]; k <= fb[5]; k++)
+      for (int j = fb[2]; j <= fb[3]; j++)
+        for (int i = fb[0]; i <= fb[1]; i++) {
+#define D1AT(s) (A(D1, bd1, i + (s) * a, j + (s) * b, k + (s) * c) * inv[d])
+          real v1, v2, v3, v4, v5;
+          v1 = D1AT(3); v2 = D1AT(2); v3 = D1AT(1); v4 = D1AT(0); v5 = D1AT(-1); /* :1286-1290 */
+          A(outp[d], bp, i, j, k) = weno5(v1, v2, v3, v4, v5);
+          v1 = D1AT(-2); v2 = D1AT(-1); v3 = D1AT(0); v4 = D1AT(1); v5 = D1AT(2); /* :1330-1334 */
+          A(outm[d], bm, i, j, k) = weno5(v1, v2, v3, v4, v5);
+#undef D1AT
+        }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Upwind variants: one side per axis chosen by the velocity sign;          */
+/* |vel| < zero_tol gives 0  (lsm_spatial_derivatives3d.f.in:1735-1743,     */
+/* 1990-2010, 2330-2380, 2756-2810).                                        */
+/* ---------------------------------------------------------------------- */
+void oracle_lsm3dupwindhjeno1_(real *px, real *py, real *pz, BOXARGS(g),
+                               const real *phi, BOXARGS(phi), const real *vx,
+                               const real *vy, const real *vz, BOXARGS(vel), real *D1,
+                               BOXARGS(d1), BOXARGS(fb), const real *dx, const real *dy,
+                               const real *dz) {
+  /* lsm_spatial_derivatives3d.f.in:1630-1835 */
+  box_t bg = MKBOX(g), bphi = MKBOX(phi), bv = MKBOX(vel), bd1 = MKBOX(d1);
+  int fb[6] = {*ilo_fb, *ihi_fb, *jlo_fb, *jhi_fb, *klo_fb, *khi_fb};
+  real inv[3];
+  inv[0] = 1.0 / *dx; inv[1] = 1.0 / *dy; inv[2] = 1.0 / *dz;
+  real *out[3] = {px, py, pz};
+  const real *vel[3] = {vx, vy, vz};
+  for (int d = 0; d < 3; d++) {
+    int a = DI(d), b = DJ(d), c = DK(d);
+    compute_dn(D1, bd1, phi, bphi, fb, 1, d);
+    for (int k = fb[4]; k <= fb[5]; k++)
+      for (int j = fb[2]; j <= fb[3]; j++)
+        for (int i = fb[0]; i <= fb[1]; i++) {
+          real v = A(vel[d], bv, i, j, k);
+          if (fabs(v) < ZERO_TOL) A(out[d], bg, i, j, k) = (real)0.0;
+          else if (v > 0) A(out[d], bg, i, j, k) = A(D1, bd1, i, j, k) * inv[d];
+          else A(out[d], bg, i, j, k) = A(D1, bd1, i + a, j + b, k + c) * inv[d];
+        }
+  }
+}
+
+void oracle_lsm3dupwindhjeno2_(real *px, real *py, real *pz, BOXARGS(g),
+                               const real *phi, BOXARGS(phi), const real *vx,
+                               const real *vy, const real *vz, BOXARGS(vel), real *D1,
+                               BOXARGS(d1), real *D2, BOXARGS(d2), BOXARGS(fb),
+                               const real *dx, const real *dy, const real *dz) {
+  /* lsm_spatial_derivatives3d.f.in:1854-2153 */
+  box_t bg = MKBOX(g), bphi = MKBOX(phi), bv = MKBOX(vel), bd1 = MKBOX(d1), bd2 = MKBOX(d2);
+  int fb[6] = {*ilo_fb, *ihi_fb, *jlo_fb, *jhi_fb, *klo_fb, *khi_fb}, w[6];
+  const real half = (real)0.5;
+  real inv[3];
+  inv[0] = 1.0 / *dx; inv[1] = 1.0 / *dy; inv[2] = 1.0 / *dz;
+  real *out[3] = {px, py, pz};
+  const real *vel[3] = {vx, vy, vz};
+  for (int d = 0; d < 3; d++) {
+    int a = DI(d), b = DJ(d), c = DK(d);
+    widen(w, fb, d, 1, 1);
+    compute_dn(D1, bd1, phi, bphi, w, 1, d);
+    compute_dn(D2, bd2, D1, bd1, w, 2, d);
+    for (int k = fb[4]; k <= fb[5]; k++)
+      for (int j = fb[2]; j <= fb[3]; j++)
+        for (int i = fb[0]; i <= fb[1]; i++) {
+          real v = A(vel[d], bv, i, j, k);
+          real d2m = A(D2, bd2, i - a, j - b, k - c), d20 = A(D2, bd2, i, j, k);
+          real d2p = A(D2, bd2, i + a, j + b, k + c);
+          real r;
+          if (fabs(v) < ZERO_TOL) r = (real)0.0;
+          else if (v > 0) {
+            if (fabs(d2m) < fabs(d20)) r = (A(D1, bd1, i, j, k) + half * d2m) * inv[d];
+            else r = (A(D1, bd1, i, j, k) + half * d20) * inv[d];
+          } else {
+            if (fabs(d20) < fabs(d2p)) r = (A(D1, bd1, i + a, j + b, k + c) - half * d20) * inv[d];
+            else r = (A(D1, bd1, i + a, j + b, k + c) - half * d2p) * inv[d];
+          }
+          A(out[d], bg, i, j, k) = r;
+        }
+  }
+}
+
+void oracle_lsm3dupwindhjeno3_(real *px, real *py, real *pz, BOXARGS(g),
+                               const real *phi, BOXARGS(phi), const real *vx,
+                               const real *vy, const real *vz, BOXARGS(vel), real *D1,
+                               BOXARGS(d1), real *D2, BOXARGS(d2), real *D3, BOXARGS(d3),
+                               BOXARGS(fb), const real *dx, const real *dy, const real *dz) {
+  /* lsm_spatial_derivatives3d.f.in:2174-2609 */
+  box_t bg = MKBOX(g), bphi = MKBOX(phi), bv = MKBOX(vel);
+  box_t bd1 = MKBOX(d1), bd2 = MKBOX(d2), bd3 = MKBOX(d3);
+  int fb[6] = {*ilo_fb, *ihi_fb, *jlo_fb, *jhi_fb, *klo_fb, *khi_fb}, w2[6], w1[6];
+  const real half = (real)0.5, third = (real)(1.0 / 3.0), sixth = (real)(1.0 / 6.0);
+  real inv[3];
+  inv[0] = 1.0 / *dx; inv[1] = 1.0 / *dy; inv[2] = 1.0 / *dz;
+  real *out[3] = {px, py, pz};
+  const real *vel[3] = {vx, vy, vz};
+  for (int d = 0; d < 3; d++) {
+    int a = DI(d), b = DJ(d), c = DK(d);
+    widen(w2, fb, d, 2, 2);
+    widen(w1, fb, d, 1, 1);
+    compute_dn(D1, bd1, phi, bphi, w2, 1, d);
+    compute_dn(D2, bd2, D1, bd1, w2, 2, d);
+    compute_dn(D3, bd3, D2, bd2, w1, 3, d);
+    for (int k = fb[4]; k <= fb[5]; k++)
+      for (int j = fb[2]; j <= fb[3]; j++)
+        for (int i = fb[0]; i <= fb[1]; i++) {
+#define D2AT(s) A(D2, bd2, i + (s) * a, j + (s) * b, k + (s) * c)
+#define D3AT(s) A(D3, bd3, i + (s) * a, j + (s) * b, k + (s) * c)
+          real vv = A(vel[d], bv, i, j, k), r;
+          if (fabs(vv) < ZERO_TOL) {
+            r = (real)0.0;
+          } else if (vv > 0) {
+            r = A(D1, bd1, i, j, k);
+            if (fabs(D2AT(-1)) < fabs(D2AT(0))) {
+              r = r + half * D2AT(-1);
+              if (fabs(D3AT(-1)) < fabs(D3AT(0))) r = r + third * D3AT(-1);
+              else r = r + third * D3AT(0);
+            } else {
+              r = r + half * D2AT(0);
+              if (fabs(D3AT(0)) < fabs(D3AT(1))) r = r - sixth * D3AT(0);
+              else r = r - sixth * D3AT(1);
+            }
+            r = r * inv[d];
+          } else {
+            r = A(D1, bd1, i + a, j + b, k + c);
+            if (fabs(D2AT(0)) < fabs(D2AT(1))) {
+              r = r - half * D2AT(0);
+              if (fabs(D3AT(0)) < fabs(D3AT(1))) r = r - sixth * D3AT(0);
+              else r = r - sixth * D3AT(1);
+            } else {
+              r = r - half * D2AT(1);
+              if (fabs(D3AT(1)) < fabs(D3AT(2))) r = r + third * D3AT(1);
+              else r = r + third * D3AT(2);
+            }
+            r = r * inv[d];
+          }
+          A(out[d], bg, i, j, k) = r;
+#undef D2AT
+#undef D3AT
+        }
+  }
+}
+
+void oracle_lsm3dupwindhjweno5_(real *px, real *py, real *pz, BOXARGS(g),
+                                const real *phi, BOXARGS(phi), const real *vx,
+                                const real *vy, const real *vz, BOXARGS(vel), real *D1,
+                                BOXARGS(d1), BOXARGS(fb), const real *dx, const real *dy,
+                                const real *dz) {
+  /* lsm_spatial_derivatives3d.f.in:2628-2999 */
+  box_t bg = MKBOX(g), bphi = MKBOX(phi), bv = MKBOX(vel), bd1 = MKBOX(d1);
+  int fb[6] = {*ilo_fb, *ihi_fb, *jlo_fb, *jhi_fb, *klo_fb, *khi_fb}, w[6];
+  real inv[3];
+  inv[0] = 1.0 / *dx; inv[1] = 1.0 / *dy; inv[2] = 1.0 / *dz;
+  real *out[3] = {px, py, pz};
+  const real *vel[3] = {vx, vy, vz};
+  for (int d = 0; d < 3; d++) {
+    int a = DI(d), b = DJ(d), c = DK(d);
+    widen(w, fb, d, 2, 2);                                           /* :2735-2747 */
+    compute_dn(D1, bd1, phi, bphi, w, 1, d);
+    for (int k = fb[4]; k <= fb[5]; k++)
+      for (int j = fb[2]; j <= fb[3]; j++)
+        for (int i = fb[0]; i <= fb[1]; i++) {
+#define D1AT(s) (A(D1, bd1, i + (s) * a, j + (s) * b, k + (s) * c) * inv[d])
+          real vv = A(vel[d], bv, i, j, k);
+          if (fabs(vv) < ZERO_TOL) {                                /* :2756-2757 */
+            A(out[d], bg, i, j, k) = (real)0.0;
+          } else if (vv > 0) {                                      /* :2759-2766 */
+            A(out[d], bg, i, j, k) = weno5(D1AT(-2), D1AT(-1), D1AT(0), D1AT(1), D1AT(2));
+          } else {                                                  /* :2768-2775 */
+            A(out[d], bg, i, j, k) = weno5(D1AT(3), D1AT(2), D1AT(1), D1AT(0), D1AT(-1));
+          }
+#undef D1AT
+        }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Level set equation RHS  lsm_level_set_evolution3d.f.in                    */
+/* ---------------------------------------------------------------------- */
+void oracle_lsm3dzerooutlevelseteqnrhs_(real *rhs, BOXARGS(r)) {
+  /* :16-47 zero the whole ghost box */
+  box_t br = MKBOX(r);
+  for (int k = br.klo; k <= br.khi; k++)
+    for (int j = br.jlo; j <= br.jhi; j++)
+      for (int i = br.ilo; i <= br.ihi; i++) A(rhs, br, i, j, k) = (real)0.0;
+}
+
+void oracle_lsm3daddadvectiontermtolserhs_(real *rhs, BOXARGS(r), const real *px,
+                                           const real *py, const real *pz, BOXARGS(g),
+                                           const real *vx, const real *vy, const real *vz,
+                                           BOXARGS(vel), BOXARGS(fb)) {
+  /* :53-125; the update is at :113-116 */
+  box_t br = MKBOX(r), bg = MKBOX(g), bv = MKBOX(vel);
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++)
+        A(rhs, br, i, j, k) = A(rhs, br, i, j, k) -
+                              (A(vx, bv, i, j, k) * A(px, bg, i, j, k) +
+                               A(vy, bv, i, j, k) * A(py, bg, i, j, k) +
+                               A(vz, bv, i, j, k) * A(pz, bg, i, j, k));
+}
+
+/* Godunov |grad phi|^2 for a normal speed of the given sign,
+ * lsm_level_set_evolution3d.f.in:206-225 (and :308-327). */
+static real godunov_norm_sq(real vn, real xp, real yp, real zp, real xm, real ym, real zm) {
+  double r;
+  if (vn > 0.0) {
+    double a = fmax2(xm, 0.0), b = fmin2(xp, 0.0);
+    double c = fmax2(ym, 0.0), e = fmin2(yp, 0.0);
+    double f = fmax2(zm, 0.0), g = fmin2(zp, 0.0);
+    r = fmax2(a * a, b * b) + fmax2(c * c, e * e) + fmax2(f * f, g * g);
+  } else {
+    double a = fmin2(xm, 0.0), b = fmax2(xp, 0.0);
+    double c = fmin2(ym, 0.0), e = fmax2(yp, 0.0);
+    double f = fmin2(zm, 0.0), g = fmax2(zp, 0.0);
+    r = fmax2(a * a, b * b) + fmax2(c * c, e * e) + fmax2(f * f, g * g);
+  }
+  return (real)r;
+}
+
+void oracle_lsm3daddnormalveltermtolserhs_(real *rhs, BOXARGS(r), const real *xp,
+                                           const real *yp, const real *zp, BOXARGS(gp),
+                                           const real *xm, const real *ym, const real *zm,
+                                           BOXARGS(gm), const real *vn, BOXARGS(vel),
+                                           BOXARGS(fb)) {
+  /* :131-242 */
+  box_t br = MKBOX(r), bp = MKBOX(gp), bm = MKBOX(gm), bv = MKBOX(vel);
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        real v = A(vn, bv, i, j, k);
+        if (fabs(v) >= ZERO_TOL) {
+          real n2 = godunov_norm_sq(v, A(xp, bp, i, j, k), A(yp, bp, i, j, k), A(zp, bp, i, j, k),
+                                    A(xm, bm, i, j, k), A(ym, bm, i, j, k), A(zm, bm, i, j, k));
+          A(rhs, br, i, j, k) = A(rhs, br, i, j, k) - v * (real)sqrt(n2);   /* :229-230 */
+        }
+      }
+}
+
+void oracle_lsm3daddconstnormalveltermtolserhs_(real *rhs, BOXARGS(r), const real *xp,
+                                                const real *yp, const real *zp, BOXARGS(gp),
+                                                const real *xm, const real *ym, const real *zm,
+                                                BOXARGS(gm), const real *vn, BOXARGS(fb)) {
+  /* :248-350 */
+  box_t br = MKBOX(r), bp = MKBOX(gp), bm = MKBOX(gm);
+  real v = *vn;
+  if (fabs(v) >= ZERO_TOL)
+    for (int k = *klo_fb; k <= *khi_fb; k++)
+      for (int j = *jlo_fb; j <= *jhi_fb; j++)
+        for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+          real n2 = godunov_norm_sq(v, A(xp, bp, i, j, k), A(yp, bp, i, j, k), A(zp, bp, i, j, k),
+                                    A(xm, bm, i, j, k), A(ym, bm, i, j, k), A(zm, bm, i, j, k));
+          A(rhs, br, i, j, k) = A(rhs, br, i, j, k) - v * (real)sqrt(n2);
+        }
+}
+
+/* Godunov selection for the combined external + normal velocity
+ * Hamiltonian, lsm_level_set_evolution3d.f.in:630-650. */
+static real ext_normal_select(real vcomp, real vn, real dp, real dm) {
+  real Hp = vcomp + vn * dp, Hm = vcomp + vn * dm;
+  if (Hp > (real)0.0) {
+    if (Hm > (real)0.0) return dm;
+    return -(vcomp / vn);
+  }
+  if (Hm < (real)0.0) return dp;
+  if (fabs(Hp) > fabs(Hm)) return dp;
+  return dm;
+}
+
+void oracle_lsm3daddexternalandnormalveltermtolserhs_(
+    real *rhs, BOXARGS(r), const real *xp, const real *yp, const real *zp, BOXARGS(gp),
+    const real *xm, const real *ym, const real *zm, BOXARGS(gm), const real *vn,
+    const real *vx, const real *vy, const real *vz, BOXARGS(vel), BOXARGS(rfb)) {
+  /* :532-719.  The reference loops k = klo_rhs_fb..khi_rhs_gb,
+   * j = jlo_rhs_fb..jhi_rhs_gb, i = ilo_rhs_gb..ihi_rhs_gb (:620-622);
+   * restated as written. */
+  box_t br = MKBOX(r), bp = MKBOX(gp), bm = MKBOX(gm), bv = MKBOX(vel);
+  for (int k = *klo_rfb; k <= *khi_r; k++)
+    for (int j = *jlo_rfb; j <= *jhi_r; j++)
+      for (int i = *ilo_r; i <= *ihi_r; i++) {
+        real vnc = A(vn, bv, i, j, k), vxc = A(vx, bv, i, j, k);
+        real vyc = A(vy, bv, i, j, k), vzc = A(vz, bv, i, j, k);
+        real px = ext_normal_select(vxc, vnc, A(xp, bp, i, j, k), A(xm, bm, i, j, k));
+        real py = ext_normal_select(vyc, vnc, A(yp, bp, i, j, k), A(ym, bm, i, j, k));
+        real pz = ext_normal_select(vzc, vnc, A(zp, bp, i, j, k), A(zm, bm, i, j, k));
+        real n2 = px * px + py * py + pz * pz;                     /* :695-696 */
+        if (fabs(vnc) >= ZERO_TOL && n2 >= ZERO_TOL)                 /* :700-708 */
+          A(rhs, br, i, j, k) = A(rhs, br, i, j, k) - vxc * px - vyc * py - vzc * pz -
+                                vnc * (real)sqrt(n2);
+        else
+          A(rhs, br, i, j, k) = A(rhs, br, i, j, k) - vxc * px - vyc * py - vzc * pz;
+      }
+}
+
+/* ---------------------------------------------------------------------- */
+/* TVD Runge-Kutta stages  lsm_tvd_runge_kutta3d.f                          */
+/* ---------------------------------------------------------------------- */
+void oracle_lsm3drk1step_(real *un, BOXARGS(un), const real *uc, BOXARGS(uc),
+                          const real *rhs, BOXARGS(r), BOXARGS(fb), const real *dt) {
+  /* :32-87, update at :78 */
+  box_t bn = MKBOX(un), bc = MKBOX(uc), br = MKBOX(r);
+  real d = *dt;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++)
+        A(un, bn, i, j, k) = A(uc, bc, i, j, k) + d * A(rhs, br, i, j, k);
+}
+
+void oracle_lsm3dtvdrk2stage1_(real *u1, BOXARGS(u1), const real *uc, BOXARGS(uc),
+                               const real *rhs, BOXARGS(r), BOXARGS(fb), const real *dt) {
+  /* :107-155 forwards to lsm3dRK1Step */
+  oracle_lsm3drk1step_(u1, ilo_u1, ihi_u1, jlo_u1, jhi_u1, klo_u1, khi_u1, uc, ilo_uc, ihi_uc,
+                       jlo_uc, jhi_uc, klo_uc, khi_uc, rhs, ilo_r, ihi_r, jlo_r, jhi_r, klo_r,
+                       khi_r, ilo_fb, ihi_fb, jlo_fb, jhi_fb, klo_fb, khi_fb, dt);
+}
+
+void oracle_lsm3dtvdrk2stage2_(real *un, BOXARGS(un), const real *u1, BOXARGS(u1),
+                               const real *uc, BOXARGS(uc), const real *rhs, BOXARGS(r),
+                               BOXARGS(fb), const real *dt) {
+  /* :175-250, update at :240-241 */
+  box_t bn = MKBOX(un), b1 = MKBOX(u1), bc = MKBOX(uc), br = MKBOX(r);
+  real d = *dt;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++)
+        A(un, bn, i, j, k) =
+            0.5 * (A(uc, bc, i, j, k) + A(u1, b1, i, j, k) + d * A(rhs, br, i, j, k));
+}
+
+void oracle_lsm3dtvdrk3stage1_(real *u1, BOXARGS(u1), const real *uc, BOXARGS(uc),
+                               const real *rhs, BOXARGS(r), BOXARGS(fb), const real *dt) {
+  /* :269-329 forwards to lsm3dRK1Step */
+  oracle_lsm3drk1step_(u1, ilo_u1, ihi_u1, jlo_u1, jhi_u1, klo_u1, khi_u1, uc, ilo_uc, ihi_uc,
+                       jlo_uc, jhi_uc, klo_uc, khi_uc, rhs, ilo_r, ihi_r, jlo_r, jhi_r, klo_r,
+                       khi_r, ilo_fb, ihi_fb, jlo_fb, jhi_fb, klo_fb, khi_fb, dt);
+}
+
+void oracle_lsm3dtvdrk3stage2_(real *u2, BOXARGS(u2), const real *u1, BOXARGS(u1),
+                               const real *uc, BOXARGS(uc), const real *rhs, BOXARGS(r),
+                               BOXARGS(fb), const real *dt) {
+  /* :347-413, update at :403-404 */
+  box_t b2 = MKBOX(u2), b1 = MKBOX(u1), bc = MKBOX(uc), br = MKBOX(r);
+  real d = *dt;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++)
+        A(u2, b2, i, j, k) =
+            0.75 * A(uc, bc, i, j, k) + 0.25 * (A(u1, b1, i, j, k) + d * A(rhs, br, i, j, k));
+}
+
+void oracle_lsm3dtvdrk3stage3_(real *un, BOXARGS(un), const real *u2, BOXARGS(u2),
+                               const real *uc, BOXARGS(uc), const real *rhs, BOXARGS(r),
+                               BOXARGS(fb), const real *dt) {
+  /* :431-501, constants :477-478, update at :486-488 */
+  const real one_third = (real)(1.0 / 3.0), two_thirds = (real)(2.0 / 3.0);
+  box_t bn = MKBOX(un), b2 = MKBOX(u2), bc = MKBOX(uc), br = MKBOX(r);
+  real d = *dt;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++)
+        A(un, bn, i, j, k) = one_third * A(uc, bc, i, j, k) +
+                             two_thirds * (A(u2, b2, i, j, k) + d * A(rhs, br, i, j, k));
+}
+
+/* ---------------------------------------------------------------------- */
+/* Reinitialization / orthogonalization  lsm_reinitialization3d.f.in        */
+/* ---------------------------------------------------------------------- */
+/* Godunov selection of |grad phi|* for the sign of s (:168-185). */
+static void godunov_star(real s, const real gp[3], const real gm[3], real gs[3]) {
+  for (int d = 0; d < 3; d++) {
+    real p, m;
+    if (s > 0.0) {
+      p = (real)fmax2(-gp[d], 0.0);
+      m = (real)fmax2(gm[d], 0.0);
+    } else {
+      p = (real)fmax2(gp[d], 0.0);
+      m = (real)fmax2(-gm[d], 0.0);
+    }
+    gs[d] = (real)fmax2(p, m);
+  }
+}
+
+void oracle_lsm3dcomputereinitializationeqnrhs_(
+    real *rhs, BOXARGS(r), const real *phi, BOXARGS(phi), const real *phi0, BOXARGS(phi0),
+    const real *xp, const real *yp, const real *zp, BOXARGS(gp), const real *xm,
+    const real *ym, const real *zm, BOXARGS(gm), BOXARGS(fb), const real *dx, const real *dy,
+    const real *dz, const int *use_phi0_for_sgn) {
+  /* :48-268 */
+  box_t br = MKBOX(r), bphi = MKBOX(phi), bphi0 = MKBOX(phi0), bp = MKBOX(gp), bm = MKBOX(gm);
+  const real one = (real)1.0;
+  real dx_sq = (real)fmax2(fmax2(*dx, *dy), *dz);                   /* :140-141 */
+  dx_sq = dx_sq * dx_sq;
+  int use0 = (*use_phi0_for_sgn == 1);
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        real cur = use0 ? A(phi0, bphi0, i, j, k) : A(phi, bphi, i, j, k);   /* :157 / :215 */
+        real gpv[3] = {A(xp, bp, i, j, k), A(yp, bp, i, j, k), A(zp, bp, i, j, k)};
+        real gmv[3] = {A(xm, bm, i, j, k), A(ym, bm, i, j, k), A(zm, bm, i, j, k)};
+        real gs[3];
+        godunov_star(cur, gpv, gmv, gs);
+        if (fabs(cur) >= ZERO_TOL) {                               /* :189 / :247 */
+          real n2 = gs[0] * gs[0] + gs[1] * gs[1] + gs[2] * gs[2];
+          real sgn;
+          if (!use0) sgn = cur / (real)sqrt(cur * cur + n2 * dx_sq);      /* :193-194 */
+          else sgn = cur / (real)sqrt(cur * cur + dx_sq);                 /* :251 */
+          A(rhs, br, i, j, k) = sgn * (one - (real)sqrt(n2));
+        } else {
+          A(rhs, br, i, j, k) = (real)0.0;
+        }
+      }
+}
+
+void oracle_lsm3dcomputeorthogonalizationeqnrhs_(
+    real *rhs, BOXARGS(r), const real *xp, const real *yp, const real *zp, BOXARGS(gp),
+    const real *xm, const real *ym, const real *zm, BOXARGS(gm), const real *psi,
+    BOXARGS(psi), const real *sxp, const real *syp, const real *szp, BOXARGS(sp),
+    const real *sxm, const real *sym, const real *szm, BOXARGS(sm), BOXARGS(fb),
+    const real *dx, const real *dy, const real *dz) {
+  /* :296-471 */
+  box_t br = MKBOX(r), bp = MKBOX(gp), bm = MKBOX(gm), bpsi = MKBOX(psi);
+  box_t bsp = MKBOX(sp), bsm = MKBOX(sm);
+  const real half = (real)0.5;
+  real dx_sq = (real)fmax2(fmax2(*dx, *dy), *dz);                   /* :396-397 */
+  dx_sq = dx_sq * dx_sq;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        real g0 = half * (A(sxp, bsp, i, j, k) + A(sxm, bsm, i, j, k));   /* :411-416 */
+        real g1 = half * (A(syp, bsp, i, j, k) + A(sym, bsm, i, j, k));
+        real g2 = half * (A(szp, bsp, i, j, k) + A(szm, bsm, i, j, k));
+        real ng = g0 * g0 + g1 * g1 + g2 * g2;                     /* :419-422 */
+        ng = (real)sqrt(ng);
+        real ps = A(psi, bpsi, i, j, k);
+        if (fabs(ps) > ZERO_TOL && ng > ZERO_TOL) {                /* :425-426 */
+          real s = ps / (real)sqrt(ps * ps + dx_sq * (ng * ng));   /* :430-431 */
+          double acc = fmax2(s * g0, 0.0) * A(xm, bm, i, j, k);    /* :433-445 */
+          acc = acc + fmin2(s * g0, 0.0) * A(xp, bp, i, j, k);
+          acc = acc + fmax2(s * g1, 0.0) * A(ym, bm, i, j, k);
+          acc = acc + fmin2(s * g1, 0.0) * A(yp, bp, i, j, k);
+          acc = acc + fmax2(s * g2, 0.0) * A(zm, bm, i, j, k);
+          acc = acc + fmin2(s * g2, 0.0) * A(zp, bp, i, j, k);
+          A(rhs, br, i, j, k) = (real)(-1.0 / ng * acc);
+        } else {
+          A(rhs, br, i, j, k) = (real)0.0;
+        }
+      }
+}
+
+void oracle_lsm3dcomputedistanceforsubcellfix_(real *dist0, const real *phi0, BOXARGS(p0),
+                                               BOXARGS(fb), const real *dx, const real *dy,
+                                               const real *dz) {
+  /* :477-581 (Russo-Smereka "version 2", :548-562) */
+  box_t b = MKBOX(p0);
+  real max_dx = (real)fmax2(fmax2(*dx, *dy), *dz);
+  real flag = -1000.0 * max_dx;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        real c = A(phi0, b, i, j, k);
+        int near = 0;
+        const int nb[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+        for (int n = 0; n < 6; n++) {                              /* :518-540 */
+          real o = A(phi0, b, i + nb[n][0], j + nb[n][1], k + nb[n][2]);
+          if ((o > 0 && c <= 0) || (o <= 0 && c > 0)) near = 1;
+        }
+        if (!near) {
+          A(dist0, b, i, j, k) = flag;
+        } else {
+          real delta = ZERO_TOL;
+          for (int d = 0; d < 3; d++) {
+            int a = DI(d), bb = DJ(d), cc = DK(d);
+            real pp = A(phi0, b, i + a, j + bb, k + cc), pm = A(phi0, b, i - a, j - bb, k - cc);
+            real d1 = (real)(fabs(pp - pm) / 2.0);
+            real d2 = (real)fabs(pp - c);
+            real d3 = (real)fabs(c - pm);
+            /* MAX(d1,d2,d3,prev) evaluated left to right */
+            real m = (real)fmax2(d1, d2);
+            m = (real)fmax2(m, d3);
+            delta = (real)fmax2(m, delta);
+          }
+          A(dist0, b, i, j, k) = c / delta;
+        }
+      }
+}
+
+void oracle_lsm3dcomputereinitializationeqnrhssubcellfixorder1_(
+    real *rhs, BOXARGS(r), const real *phi, BOXARGS(phi), const real *phi0,
+    const real *dist0, BOXARGS(p0), const real *xp, const real *yp, const real *zp,
+    BOXARGS(gp), const real *xm, const real *ym, const real *zm, BOXARGS(gm), BOXARGS(fb),
+    const real *dx, const real *dy, const real *dz) {
+  /* :586-754 */
+  box_t br = MKBOX(r), bphi = MKBOX(phi), b0 = MKBOX(p0), bp = MKBOX(gp), bm = MKBOX(gm);
+  const real zero = (real)0.0;
+  real max_dx = (real)fmax2(fmax2(*dx, *dy), *dz);
+  real flag = -1000.0 * max_dx;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        real p0v = A(phi0, b0, i, j, k), s0;
+        if (fabs(p0v) > ZERO_TOL) s0 = (real)copysign(1.0, p0v);  /* :676-680 SIGN(one,phi0) */
+        else s0 = zero;
+        if (A(dist0, b0, i, j, k) == flag) {
+          real cur = A(phi, bphi, i, j, k);
+          real gpv[3] = {A(xp, bp, i, j, k), A(yp, bp, i, j, k), A(zp, bp, i, j, k)};
+          real gmv[3] = {A(xm, bm, i, j, k), A(ym, bm, i, j, k), A(zm, bm, i, j, k)};
+          real gs[3];
+          godunov_star(cur, gpv, gmv, gs);
+          if (fabs(cur) > ZERO_TOL) {
+            real n2 = gs[0] * gs[0] + gs[1] * gs[1] + gs[2] * gs[2];
+            A(rhs, br, i, j, k) = s0 - s0 * (real)sqrt(n2);        /* :722-723 */
+          } else {
+            A(rhs, br, i, j, k) = (real)0.0;
+          }
+        } else {
+          A(rhs, br, i, j, k) =
+              A(dist0, b0, i, j, k) - s0 * (real)fabs(A(phi, bphi, i, j, k)) / max_dx; /* :731-732 */
+        }
+      }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Reductions  lsm_utilities3d.f.in, geometry/lsm_geometry3d_fort.f.in      */
+/* ---------------------------------------------------------------------- */
+void oracle_lsm3dmaxnormdiff_(real *out, const real *f1, BOXARGS(f1), const real *f2,
+                              BOXARGS(f2), BOXARGS(ib)) {
+  /* lsm_utilities3d.f.in:16-76 */
+  box_t b1 = MKBOX(f1), b2 = MKBOX(f2);
+  real m = (real)fabs(A(f1, b1, *ilo_ib, *jlo_ib, *klo_ib) - A(f2, b2, *ilo_ib, *jlo_ib, *klo_ib));
+  for (int k = *klo_ib; k <= *khi_ib; k++)
+    for (int j = *jlo_ib; j <= *jhi_ib; j++)
+      for (int i = *ilo_ib; i <= *ihi_ib; i++) {
+        real d = (real)fabs(A(f1, b1, i, j, k) - A(f2, b2, i, j, k));
+        if (d > m) m = d;
+      }
+  *out = m;
+}
+
+void oracle_lsm3dmaxnormdiffcontrolvolume_(real *out, const real *f1, BOXARGS(f1),
+                                           const real *f2, BOXARGS(f2), const real *cv,
+                                           BOXARGS(cv), const int *cv_sgn, BOXARGS(ib)) {
+  /* lsm_utilities3d.f.in:852-955; initial value ignores the control volume */
+  box_t b1 = MKBOX(f1), b2 = MKBOX(f2), bc = MKBOX(cv);
+  real m = (real)fabs(A(f1, b1, *ilo_ib, *jlo_ib, *klo_ib) - A(f2, b2, *ilo_ib, *jlo_ib, *klo_ib));
+  for (int k = *klo_ib; k <= *khi_ib; k++)
+    for (int j = *jlo_ib; j <= *jhi_ib; j++)
+      for (int i = *ilo_ib; i <= *ihi_ib; i++) {
+        real c = A(cv, bc, i, j, k);
+        if ((*cv_sgn > 0) ? (c > 0.0) : (c < 0.0)) {
+          real d = (real)fabs(A(f1, b1, i, j, k) - A(f2, b2, i, j, k));
+          if (d > m) m = d;
+        }
+      }
+  *out = m;
+}
+
+static real adv_u_over_dx(real ux, real uy, real uz, real ix, real iy, real iz) {
+  /* lsm_utilities3d.f.in:306-308 */
+  return (real)fabs(ux) * ix + (real)fabs(uy) * iy + (real)fabs(uz) * iz;
+}
+
+void oracle_lsm3dcomputestableadvectiondt_(real *dt, const real *vx, const real *vy,
+                                           const real *vz, BOXARGS(vel), BOXARGS(ib),
+                                           const real *dx, const real *dy, const real *dz,
+                                           const real *cfl) {
+  /* lsm_utilities3d.f.in:253-327 */
+  box_t bv = MKBOX(vel);
+  real mx = (real)-1.0;                                             /* :297 */
+  real ix = 1.0 / *dx, iy = 1.0 / *dy, iz = 1.0 / *dz;
+  for (int k = *klo_ib; k <= *khi_ib; k++)
+    for (int j = *jlo_ib; j <= *jhi_ib; j++)
+      for (int i = *ilo_ib; i <= *ihi_ib; i++) {
+        real u = adv_u_over_dx(A(vx, bv, i, j, k), A(vy, bv, i, j, k), A(vz, bv, i, j, k), ix, iy, iz);
+        if (u > mx) mx = u;
+      }
+  *dt = *cfl / (mx + TINY);                                         /* :323 */
+}
+
+void oracle_lsm3dcomputestableadvectiondtcontrolvolume_(
+    real *dt, const real *vx, const real *vy, const real *vz, BOXARGS(vel), const real *cv,
+    BOXARGS(cv), const int *cv_sgn, BOXARGS(ib), const real *dx, const real *dy,
+    const real *dz, const real *cfl) {
+  /* lsm_utilities3d.f.in:960-1079 */
+  box_t bv = MKBOX(vel), bc = MKBOX(cv);
+  real mx = (real)-1.0;
+  real ix = 1.0 / *dx, iy = 1.0 / *dy, iz = 1.0 / *dz;
+  for (int k = *klo_ib; k <= *khi_ib; k++)
+    for (int j = *jlo_ib; j <= *jhi_ib; j++)
+      for (int i = *ilo_ib; i <= *ihi_ib; i++) {
+        real c = A(cv, bc, i, j, k);
+        if ((*cv_sgn > 0) ? (c > 0.0) : (c < 0.0)) {
+          real u = adv_u_over_dx(A(vx, bv, i, j, k), A(vy, bv, i, j, k), A(vz, bv, i, j, k), ix, iy, iz);
+          if (u > mx) mx = u;
+        }
+      }
+  *dt = *cfl / (mx + TINY);
+}
+
+/* H/dX for the normal-velocity CFL, lsm_utilities3d.f.in:404-429. */
+static real normal_h_over_dx(real avn, real xp, real yp, real zp, real xm, real ym, real zm,
+                             real max_dx_sq, real ix, real iy, real iz) {
+  real px = (real)fmax2(fabs(xp), fabs(xm));
+  real py = (real)fmax2(fabs(yp), fabs(ym));
+  real pz = (real)fmax2(fabs(zp), fabs(zm));
+  real ng = (real)sqrt(px * px + py * py + pz * pz + max_dx_sq);
+  return avn / ng * (px * ix + py * iy + pz * iz);
+}
+
+void oracle_lsm3dcomputestablenormalveldt_(real *dt, const real *vn, BOXARGS(vel),
+                                           const real *xp, const real *yp, const real *zp,
+                                           BOXARGS(gp), const real *xm, const real *ym,
+                                           const real *zm, BOXARGS(gm), BOXARGS(ib),
+                                           const real *dx, const real *dy, const real *dz,
+                                           const real *cfl) {
+  /* lsm_utilities3d.f.in:332-450 */
+  box_t bv = MKBOX(vel), bp = MKBOX(gp), bm = MKBOX(gm);
+  real mdx = (real)fmax2(fmax2(*dx, *dy), *dz);
+  real max_dx_sq = mdx * mdx;
+  real mx = (real)-1.0;
+  real ix = 1.0 / *dx, iy = 1.0 / *dy, iz = 1.0 / *dz;
+  for (int k = *klo_ib; k <= *khi_ib; k++)
+    for (int j = *jlo_ib; j <= *jhi_ib; j++)
+      for (int i = *ilo_ib; i <= *ihi_ib; i++) {
+        real h = normal_h_over_dx((real)fabs(A(vn, bv, i, j, k)), A(xp, bp, i, j, k),
+                                  A(yp, bp, i, j, k), A(zp, bp, i, j, k), A(xm, bm, i, j, k),
+                                  A(ym, bm, i, j, k), A(zm, bm, i, j, k), max_dx_sq, ix, iy, iz);
+        if (h > mx) mx = h;
+      }
+  *dt = *cfl / (mx + TINY);
+}
+
+void oracle_lsm3dcomputestableconstnormalveldt_(real *dt, const real *vn, const real *xp,
+                                                const real *yp, const real *zp, BOXARGS(gp),
+                                                const real *xm, const real *ym, const real *zm,
+                                                BOXARGS(gm), BOXARGS(ib), const real *dx,
+                                                const real *dy, const real *dz,
+                                                const real *cfl) {
+  /* lsm_utilities3d.f.in:455-568 */
+  box_t bp = MKBOX(gp), bm = MKBOX(gm);
+  real mdx = (real)fmax2(fmax2(*dx, *dy), *dz);
+  real max_dx_sq = mdx * mdx;
+  real mx = (real)-1;
+  real ix = 1.0 / *dx, iy = 1.0 / *dy, iz = 1.0 / *dz;
+  real avn = (real)fabs(*vn);
+  for (int k = *klo_ib; k <= *khi_ib; k++)
+    for (int j = *jlo_ib; j <= *jhi_ib; j++)
+      for (int i = *ilo_ib; i <= *ihi_ib; i++) {
+        real h = normal_h_over_dx(avn, A(xp, bp, i, j, k), A(yp, bp, i, j, k), A(zp, bp, i, j, k),
+                                  A(xm, bm, i, j, k), A(ym, bm, i, j, k), A(zm, bm, i, j, k),
+                                  max_dx_sq, ix, iy, iz);
+        if (h > mx) mx = h;
+      }
+  *dt = *cfl / (mx + TINY);
+}
+
+/* pi and the 14-digit 1/pi the reference uses for smoothed Heaviside
+ * integrals (lsm_utilities3d.f.in:619-621, lsm_geometry3d_fort.f.in:299-301) */
+#define PI_REF ((real)3.14159265358979323846)
+#define ONE_OVER_PI_REF ((real)0.31830988618379)
+
+void oracle_lsm3dvolumeintegralphilessthanzero_(real *out, const real *F, BOXARGS(F),
+                                                const real *phi, BOXARGS(phi), BOXARGS(ib),
+                                                const real *dx, const real *dy, const real *dz,
+                                                const real *eps) {
+  /* lsm_utilities3d.f.in:573-654 */
+  box_t bF = MKBOX(F), bphi = MKBOX(phi);
+  real dV = *dx * *dy * *dz, acc = (real)0.0, e = *eps;
+  for (int k = *klo_ib; k <= *khi_ib; k++)
+    for (int j = *jlo_ib; j <= *jhi_ib; j++)
+      for (int i = *ilo_ib; i <= *ihi_ib; i++) {
+        real p = A(phi, bphi, i, j, k), x = p / e;
+        if (p < -e) {
+          acc = acc + A(F, bF, i, j, k) * dV;
+        } else if (p < e) {
+          real omh = 0.5 * (1.0 - x - ONE_OVER_PI_REF * SIN(PI_REF * x));
+          acc = acc + omh * A(F, bF, i, j, k) * dV;
+        }
+      }
+  *out = acc;
+}
+
+void oracle_lsm3dvolumeintegralphigreaterthanzero_(real *out, const real *F, BOXARGS(F),
+                                                   const real *phi, BOXARGS(phi), BOXARGS(ib),
+                                                   const real *dx, const real *dy,
+                                                   const real *dz, const real *eps) {
+  /* lsm_utilities3d.f.in:659-740 */
+  box_t bF = MKBOX(F), bphi = MKBOX(phi);
+  real dV = *dx * *dy * *dz, acc = (real)0.0, e = *eps;
+  for (int k = *klo_ib; k <= *khi_ib; k++)
+    for (int j = *jlo_ib; j <= *jhi_ib; j++)
+      for (int i = *ilo_ib; i <= *ihi_ib; i++) {
+        real p = A(phi, bphi, i, j, k), x = p / e;
+        if (p > e) {
+          acc = acc + A(F, bF, i, j, k) * dV;
+        } else if (p > -e) {
+          real H = 0.5 * (1.0 + x + ONE_OVER_PI_REF * SIN(PI_REF * x));
+          acc = acc + H * A(F, bF, i, j, k) * dV;
+        }
+      }
+  *out = acc;
+}
+
+void oracle_lsm3dvolumeregionphilessthanzero_(real *out, const real *phi, BOXARGS(phi),
+                                              BOXARGS(ib), const real *dx, const real *dy,
+                                              const real *dz, const real *eps) {
+  /* lsm_geometry3d_fort.f.in:263-345 */
+  box_t bphi = MKBOX(phi);
+  real dV = *dx * *dy * *dz, vol = (real)0.0, e = *eps;
+  for (int k = *klo_ib; k <= *khi_ib; k++)
+    for (int j = *jlo_ib; j <= *jhi_ib; j++)
+      for (int i = *ilo_ib; i <= *ihi_ib; i++) {
+        real p = A(phi, bphi, i, j, k), x = p / e;
+        if (p < -e) {
+          vol = vol + dV;
+        } else if (p < e) {
+          real omh = 0.5 * (1 - x - ONE_OVER_PI_REF * SIN(PI_REF * x));   /* :321-323 */
+          vol = vol + omh * dV;
+        }
+      }
+  *out = vol;
+}
+
+void oracle_lsm3dvolumeregionphigreaterthanzero_(real *out, const real *phi, BOXARGS(phi),
+                                                 BOXARGS(ib), const real *dx, const real *dy,
+                                                 const real *dz, const real *eps) {
+  /* lsm_geometry3d_fort.f.in:352-422 */
+  box_t bphi = MKBOX(phi);
+  real dV = *dx * *dy * *dz, vol = (real)0.0, e = *eps;
+  for (int k = *klo_ib; k <= *khi_ib; k++)
+    for (int j = *jlo_ib; j <= *jhi_ib; j++)
+      for (int i = *ilo_ib; i <= *ihi_ib; i++) {
+        real p = A(phi, bphi, i, j, k), x = p / e;
+        if (p > e) {
+          vol = vol + dV;
+        } else if (p > -e) {
+          real H = (real)0.5 * (1 + x + ONE_OVER_PI_REF * SIN(PI_REF * x)); /* :409-410 */
+          vol = vol + H * dV;
+        }
+      }
+  *out = vol;
+}
+
+void oracle_lsmheaviside_(real *H, const real *x, const real *eps) {
+  /* lsm_calculus_toolbox.f:25-54 */
+  real e = *eps, xv = *x;
+  if (xv < -e) *H = (real)0.0;
+  else if (xv > e) *H = (real)1.0;
+  else {
+    real ioe = 1.0 / e;
+    *H = 0.5 * (1.0 + xv * ioe + ONE_OVER_PI_REF * SIN(PI_REF * xv * ioe));
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Boundary conditions  lsm_boundary_conditions3d.f                         */
+/* bdry_location_idx: 0 x_lo, 1 x_hi, 2 y_lo, 3 y_hi, 4 z_lo, 5 z_hi (:13-20) */
+/* Each face loops over the full ghost box in the two other directions.     */
+/* ---------------------------------------------------------------------- */
+/* mode 0 linear (:46-170), 1 signed linear (:193-330), 2 copy (:352-463) */
+static void extrapolate(real *phi, box_t g, const int fb[6], int loc, int mode) {
+  if (loc < 0 || loc > 5) return;
+  int d = loc / 2, hi = loc % 2;
+  int a = DI(d), b = DJ(d), c = DK(d);
+  int gl[3] = {g.ilo, g.jlo, g.klo}, gh[3] = {g.ihi, g.jhi, g.khi};
+  int bnd = hi ? fb[2 * d + 1] : fb[2 * d];   /* boundary fill-box index */
+  int in = hi ? -1 : 1;                        /* step into the interior */
+  int u = (d + 1) % 3, v = (d + 2) % 3;
+  for (int q = gl[v]; q <= gh[v]; q++)
+    for (int p = gl[u]; p <= gh[u]; p++) {
+      int idx[3];
+      idx[u] = p; idx[v] = q; idx[d] = bnd;
+      real pb = A(phi, g, idx[0], idx[1], idx[2]);
+      real pi1 = A(phi, g, idx[0] + in * a, idx[1] + in * b, idx[2] + in * c);
+      real slope = (real)0.0;
+      if (mode == 0) {
+        slope = pb - pi1;
+      } else if (mode == 1) {
+        real s = (real)copysign(1.0, pb);   /* SIGN(one, phi) */
+        real ad = (real)fabs(pb - pi1);
+        slope = s * ad;
+      }
+      int lo = hi ? bnd + 1 : gl[d], up = hi ? gh[d] : bnd - 1;
+      for (int t = lo; t <= up; t++) {
+        idx[d] = t;
+        if (mode == 2) {
+          A(phi, g, idx[0], idx[1], idx[2]) = pb;
+        } else {
+          real dist = (real)(hi ? (t - bnd) : (bnd - t));
+          A(phi, g, idx[0], idx[1], idx[2]) = pb + slope * dist;
+        }
+      }
+    }
+}
+
+#define BC_ENTRY(name, mode)                                                    \
+  void oracle_##name(real *phi, BOXARGS(g), BOXARGS(fb), const int *loc) {      \
+    int fb[6] = {*ilo_fb, *ihi_fb, *jlo_fb, *jhi_fb, *klo_fb, *khi_fb};         \
+    extrapolate(phi, MKBOX(g), fb, *loc, mode);                                  \
+  }
+BC_ENTRY(lsm3dlinearextrapolation_, 0)
+BC_ENTRY(lsm3dsignedlinearextrapolation_, 1)
+BC_ENTRY(lsm3dcopyextrapolation_, 2)
+/* Homogeneous Neumann for ENO1/2/3 and WENO5 reduces to copy extrapolation
+ * (lsm_boundary_conditions3d.f:486-640). */
+BC_ENTRY(lsm3dhomogeneousneumanneno1_, 2)
+BC_ENTRY(lsm3dhomogeneousneumanneno2_, 2)
+BC_ENTRY(lsm3dhomogeneousneumanneno3_, 2)
+BC_ENTRY(lsm3dhomogeneousneumannweno5_, 2)

@@ -1,0 +1,254 @@
+"""Seeded argument sets for every exported LSM3D_* routine.
+
+`make_case(name, real, seed)` returns a `Case` whose `args` follow the
+reference prototype (see lsmlib_amd/signatures.py): arrays on their own ghost
+boxes with non-zero, uneven lower bounds, the fill box, grid spacings and the
+routine's scalars.  Values are chosen so every branch of each routine is
+exercised: velocities include exact zeros and magnitudes under the zero
+tolerance, level-set values straddle the smoothing width, ENO differences are
+noisy enough to take both stencil choices.
+
+`scratch` lists the argument positions of the D1/D2/D3 scratch arrays, which
+the reference overwrites and the GPU library leaves untouched.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from lsmlib_amd.f77 import box_shape
+from lsmlib_amd.signatures import SIGNATURES
+
+ZERO_TOL = {"double": 1e-11, "float": 1e-5}
+
+
+@dataclass
+class Case:
+    name: str
+    real: str
+    args: list
+    scratch: list = field(default_factory=list)
+
+    def copy(self):
+        return Case(self.name, self.real,
+                    [a.copy() if isinstance(a, np.ndarray) else a for a in self.args],
+                    list(self.scratch))
+
+    def arrays(self):
+        return [(i, a) for i, a in enumerate(self.args) if isinstance(a, np.ndarray)]
+
+
+def grow(box, lo, hi=None):
+    hi = lo if hi is None else hi
+    lo = np.broadcast_to(lo, 3)
+    hi = np.broadcast_to(hi, 3)
+    return (box[0] - lo[0], box[1] + hi[0], box[2] - lo[1], box[3] + hi[1],
+            box[4] - lo[2], box[5] + hi[2])
+
+
+def _groups(kinds):
+    """Split a kinds string into items: ('arr', n) for n arrays + box,
+    ('box',), ('R',), ('I',), ('O',)."""
+    out, i = [], 0
+    while i < len(kinds):
+        if kinds[i] == "A":
+            n = 0
+            while kinds[i] == "A":
+                n += 1
+                i += 1
+            assert kinds[i] == "B"
+            out.append(("arr", n))
+            i += 1
+        elif kinds[i] == "B":
+            out.append(("box",))
+            i += 1
+        else:
+            out.append((kinds[i],))
+            i += 1
+    return out
+
+
+SPATIAL = {"lsm3dhjeno1_", "lsm3dhjeno2_", "lsm3dhjeno3_", "lsm3dhjweno5_",
+           "lsm3dupwindhjeno1_", "lsm3dupwindhjeno2_", "lsm3dupwindhjeno3_",
+           "lsm3dupwindhjweno5_"}
+BC = {n for n in SIGNATURES if "extrapolation" in n or "neumann" in n}
+
+
+def _field(rng, box, kind, real, eps=None):
+    shape = box_shape(box)
+    dt = np.float64 if real == "double" else np.float32
+    if kind == "phi":
+        # smooth level set plus noise so ENO stencil choices vary
+        k, j, i = np.meshgrid(*(np.arange(s, dtype=np.float64) for s in shape), indexing="ij")
+        a = 0.3 * np.sin(0.7 * i + 0.2) * np.cos(0.5 * j) + 0.05 * k - 0.4
+        a = a + 0.05 * rng.standard_normal(shape)
+    elif kind == "vel":
+        a = rng.standard_normal(shape)
+        m = rng.random(shape)
+        a[m < 0.08] = 0.0
+        tol = ZERO_TOL[real]
+        sel = (m >= 0.08) & (m < 0.14)
+        a[sel] = 0.3 * tol * rng.choice([-1.0, 1.0], size=int(sel.sum()))
+    elif kind == "near":
+        # values straddling +-eps for the smoothed Heaviside branches
+        a = eps * 3.0 * rng.uniform(-1, 1, shape)
+    else:
+        a = rng.standard_normal(shape)
+    return np.ascontiguousarray(a.astype(dt))
+
+
+def make_case(name, real="double", seed=0, fb=(-3, 8, 2, 12, 5, 14)):
+    rng = np.random.default_rng(seed + 1000 * sorted(SIGNATURES).index(name))
+    kinds = SIGNATURES[name]
+    dx, dy, dz = 0.1, 0.13, 0.07
+    eps = 1.5 * dx
+    groups = _groups(kinds)
+    args, scratch = [], []
+    n_arr_groups = sum(1 for g in groups if g[0] == "arr")
+    gi = 0
+    rscal = iter(_scalars(name, dx, dy, dz, eps))
+    for g in groups:
+        if g[0] == "arr":
+            n = g[1]
+            last_group = gi == n_arr_groups - 1
+            if name == "lsm3daddexternalandnormalveltermtolserhs_":
+                box = grow(fb, 2)          # the reference loops over the rhs ghost box
+            elif name in BC:
+                box = grow(fb, (3, 2, 3), (2, 3, 1))
+            elif name in SPATIAL:
+                # output grads: small ghost; phi: 3-4 ghosts; vel: 1; scratch: 4
+                if gi == 0 or (name.startswith("lsm3dhj") and gi == 1):
+                    box = grow(fb, (1, 0, 2), (0, 2, 1))
+                elif _is_phi_group(name, gi):
+                    box = grow(fb, (3, 4, 3), (4, 3, 3))
+                elif _is_scratch_group(name, gi):
+                    box = grow(fb, 4)
+                else:
+                    box = grow(fb, (1, 2, 0), (2, 1, 1))
+            else:
+                box = grow(fb, (gi % 3, (gi + 1) % 3, 1), (1, gi % 2, (gi + 2) % 3))
+                if name.startswith("lsm3dcomputedistance") or "subcellfix" in name:
+                    box = grow(fb, 1 + gi % 2)
+            for a_idx in range(n):
+                kind = _value_kind(name, gi, a_idx)
+                arr = _field(rng, box, kind, real, eps)
+                if _is_scratch_group(name, gi):
+                    scratch.append(len(args))
+                args.append(arr)
+            args.append(box)
+            gi += 1
+        elif g[0] == "box":
+            args.append(tuple(fb))
+        elif g[0] == "R":
+            args.append(next(rscal))
+        elif g[0] == "I":
+            args.append(_int_arg(name, rng))
+        elif g[0] == "O":
+            pass
+    case = Case(name, real, args, scratch)
+    if name == "lsm3dcomputereinitializationeqnrhssubcellfixorder1_":
+        _fix_distance0(case, real)
+    return case
+
+
+def _is_phi_group(name, gi):
+    if name.startswith("lsm3dhj"):
+        return gi == 2
+    return gi == 1
+
+
+def _is_scratch_group(name, gi):
+    if name not in SPATIAL:
+        return False
+    first = 3 if name.startswith("lsm3dhj") else 3
+    return gi >= first
+
+
+def _value_kind(name, gi, a_idx):
+    if name in SPATIAL:
+        if _is_phi_group(name, gi):
+            return "phi"
+        if name.startswith("lsm3dupwind") and gi == 2:
+            return "vel"
+        return "rand"
+    if name in ("lsm3dvolumeintegralphilessthanzero_", "lsm3dvolumeintegralphigreaterthanzero_"):
+        return "near" if gi == 1 else "rand"
+    if name.startswith("lsm3dvolumeregion"):
+        return "near"
+    if name == "lsm3daddnormalveltermtolserhs_" and gi == 3:
+        return "vel"
+    if name == "lsm3daddexternalandnormalveltermtolserhs_" and gi == 3:
+        return "vel"
+    if name == "lsm3dcomputereinitializationeqnrhs_" and gi in (1, 2):
+        return "vel"   # includes exact zeros and |phi| < zero_tol
+    if name == "lsm3dcomputeorthogonalizationeqnrhs_" and gi == 3:
+        return "vel"
+    if name == "lsm3dcomputestablenormalveldt_" and gi == 0:
+        return "vel"
+    if name in ("lsm3dcomputedistanceforsubcellfix_",):
+        return "rand"
+    return "rand"
+
+
+def _scalars(name, dx, dy, dz, eps):
+    """Scalar (R) arguments of each routine, in prototype order."""
+    if name == "lsm3daddconstnormalveltermtolserhs_":
+        return [-0.8]
+    if "rk" in name:
+        return [0.037]
+    if name == "lsm3dcomputestableconstnormalveldt_":
+        return [-0.8, dx, dy, dz, 0.5]
+    if "stable" in name:
+        return [dx, dy, dz, 0.5]
+    if "volume" in name:
+        return [dx, dy, dz, eps]
+    return [dx, dy, dz]
+
+
+def _int_arg(name, rng):
+    if name in BC:
+        return 0
+    if "controlvolume" in name:
+        return 1
+    if "reinitialization" in name:
+        return 0
+    return 0
+
+
+def _fix_distance0(case, real):
+    """distance0 for the subcell-fix RHS is the output of
+    LSM3D_COMPUTE_DISTANCE_FOR_SUBCELL_FIX on the same phi0."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+    from oracle.oracle import oracle_lib
+    lib = oracle_lib(real)
+    # args: rhs, rbox, phi, pbox, phi0, distance0, p0box, ...
+    phi0, dist0, p0box = case.args[4], case.args[5], case.args[6]
+    fb = case.args[-4]
+    dx, dy, dz = case.args[-3:]
+    # widen sign changes: make phi0 a smooth field with an interface
+    k, j, i = np.meshgrid(*(np.arange(s) for s in phi0.shape), indexing="ij")
+    phi0[...] = (0.2 * (i - phi0.shape[2] / 2.0) + 0.1 * j - 0.05 * k).astype(phi0.dtype)
+    lib.lsm3dcomputedistanceforsubcellfix_(dist0, phi0, p0box, fb, dx, dy, dz)
+
+
+def cases_for(name, real, seeds=(0, 1)):
+    out = []
+    for s in seeds:
+        c = make_case(name, real, s)
+        out.append(c)
+        if name in BC:
+            for loc in range(1, 6):
+                c2 = make_case(name, real, s)
+                c2.args[-1] = loc
+                out.append(c2)
+        if "controlvolume" in name:
+            c2 = make_case(name, real, s)
+            idx = [i for i, a in enumerate(c2.args) if isinstance(a, int) and not isinstance(a, bool)]
+            c2.args[idx[0]] = -1
+            out.append(c2)
+        if name == "lsm3dcomputereinitializationeqnrhs_":
+            c2 = make_case(name, real, s)
+            c2.args[-1] = 1
+            out.append(c2)
+    return out
