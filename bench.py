@@ -1,0 +1,229 @@
+"""Throughput of the WENO5 / TVD-RK3 level-set advection step on one MI355X.
+
+Workload (default): 512^3 cells on the periodic unit cube, phi0 = signed
+distance to the union of 64 random spheres (seed 5, radii U[0.02, 0.1]),
+stored Taylor-Green velocity u = sin2pix cos2piy cos2piz,
+v = -cos2pix sin2piy cos2piz, w = 0, CFL 0.5, 3-cell ghost layer.  One step
+is three fused Runge-Kutta stage kernels (upwind HJ-WENO5 + advection RHS +
+TVD-RK3 combination); periodic ghost images are written by the stages.
+
+The FP32 step is timed (value), then the FP64 step on the same initial data
+(fp64 block).  Kernel durations come from HIP events recorded around every
+stage launch on the library's stream.  Algorithmic bytes per cell: stage 1
+reads phi, u, v, w and writes phi1 (5 words); stages 2 and 3 also read phi^n
+(6 words each): 17 words per cell-step.
+
+cpu_baseline: the reference Fortran toolbox (flang build in oracle/_ref) or,
+where it is absent, the C restatement, on one host core, advancing the same
+workload at a smaller grid for a bounded time.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+METRIC = "cell-updates/s per WENO5-RK3 step (FP32 vs FP64) + % HBM roofline; adjoint s"
+HBM_PEAK_GBS = 8000.0
+WORDS = {1: 5, 32: 6, 33: 6}   # algorithmic words per cell for each stage form
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--tiling", type=str, default="auto",
+                    help="cfg,kz of the fused stage tiling, or 'auto' (tuned table)")
+    ap.add_argument("--fp32-only", action="store_true", help="skip the FP64 leg (profiling runs)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def run_gpu(real, n, steps, warmup, tiling, phi_init=None):
+    import torch
+    from lsmlib_amd import toolbox
+    from lsmlib_amd.pylsmlib import RK1, TVDRK3_STAGE2, TVDRK3_STAGE3, Stepper, stable_advection_dt
+    from lsmlib_amd.workloads import sphere_union_taylor_green
+    from lsmlib_amd.tuning import choose_tiling
+
+    dev = torch.device("cuda:0")
+    dtype = torch.float32 if real == "float" else torch.float64
+    grid, phi64, vel, bc = sphere_union_taylor_green(n, dtype=torch.float64, device=dev)
+    phi = phi64.to(dtype).contiguous()
+    del phi64
+    vel = tuple(v.to(dtype).contiguous() for v in vel)
+    lib = toolbox(real)
+    cfg, kz = choose_tiling(real, n) if tiling == "auto" else tuple(int(x) for x in tiling.split(","))
+    lib.lib.lsm3d_amd_set_stage_tiling(cfg, kz)
+    st = Stepper(grid, real, bc)
+    st.fill_ghosts(phi)
+    dt = stable_advection_dt(vel, grid, 0.5)
+    s1, s2 = st.empty(), st.empty()
+    stream = torch.cuda.current_stream(dev)
+    forms = (RK1, TVDRK3_STAGE2, TVDRK3_STAGE3)
+
+    def step(events=None):
+        for f in forms:
+            out, ps, pc = {RK1: (s1, phi, None), TVDRK3_STAGE2: (s2, s1, phi),
+                           TVDRK3_STAGE3: (phi, s2, phi)}[f]
+            if events is not None:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+            st.advect_stage(f, out, ps, pc, vel, dt)
+            if events is not None:
+                e1.record(stream)
+                events.append((f, e0, e1))
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    events = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(events)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = grid.interior_cells
+    word = 4 if real == "float" else 8
+    tot_bytes = sum(WORDS[f] * cells * word for f, _, _ in events)
+    tot_ms = sum(e0.elapsed_time(e1) for _, e0, e1 in events)
+    per_form = {}
+    for f, e0, e1 in events:
+        per_form.setdefault(f, []).append(e0.elapsed_time(e1))
+    res = {
+        "real": real, "grid": n, "dt": dt, "tiling": [cfg, kz],
+        "value": cells * steps / wall, "ms_per_step": 1e3 * wall / steps,
+        "kernel_ms_avg": tot_ms / len(events),
+        "kernel_ms_by_stage": {str(f): sum(v) / len(v) for f, v in per_form.items()},
+        "achieved_gbs": tot_bytes / (tot_ms * 1e-3) / 1e9,
+        "algorithmic_bytes_per_launch": tot_bytes / len(events),
+    }
+    return res, grid, phi
+
+
+def fp32_error(grid, phi32, phi64):
+    import torch
+    a = grid.interior(phi32).to(torch.float64)
+    b = grid.interior(phi64)
+    err = (a - b).abs()
+    h = min(grid.dx)
+    near = b.abs() < 3 * h
+    return {"max_abs_global": float(err.max()),
+            "max_abs_near_interface": float(err[near].max()) if bool(near.any()) else None,
+            "max_near_interface_over_dx": (float(err[near].max()) / h) if bool(near.any()) else None}
+
+
+def cpu_baseline(seconds):
+    """Reference toolbox chain on one host core, FP32 build, 48^3 sample."""
+    import numpy as np
+    import torch
+    from oracle.oracle import oracle_lib, ref_lib
+    from oracle.steps import OracleStepper, fill_ghosts
+    from lsmlib_amd.workloads import sphere_union_taylor_green
+
+    lib = ref_lib("float")
+    kind = "reference"
+    if lib is None:
+        lib, kind = oracle_lib("float"), "port"
+    n = 48
+    grid, phi, vel, bc = sphere_union_taylor_green(n, dtype=torch.float32, device="cpu")
+    ph = phi.numpy().copy()
+    vn = tuple(v.numpy().copy() for v in vel)
+    op = OracleStepper(lib, grid, bc)
+    fill_ghosts(lib, ph, grid, bc)
+    dt = lib.lsm3dcomputestableadvectiondt_(vn[0], vn[1], vn[2], grid.gb, grid.fb, *grid.dx, 0.5)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        op.advect_rk3_step(ph, vn, dt)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": grid.interior_cells * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+            "sample": f"{n}^3 periodic sphere-union Taylor-Green advection, FP32 build, WENO5/TVD-RK3 "
+                      f"toolbox chain, {steps} steps in {el:.1f} s"}
+
+
+def read_traffic():
+    p = REPO / "profiles" / "traffic_advect_stage_512_fp32.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
+              "(the multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a visible GPU", file=sys.stderr)
+        sys.exit(1)
+
+    r32, grid, phi32 = run_gpu("float", a.n, a.steps, a.warmup, a.tiling)
+    print(f"# fp32: {r32['value']:.4e} cell-updates/s, {r32['ms_per_step']:.3f} ms/step, "
+          f"kernel {r32['kernel_ms_avg']:.3f} ms, {r32['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+    r64 = None
+    err = None
+    if not a.fp32_only:
+        r64, _, phi64 = run_gpu("double", a.n, a.steps, a.warmup, a.tiling)
+        print(f"# fp64: {r64['value']:.4e} cell-updates/s, {r64['ms_per_step']:.3f} ms/step",
+              file=sys.stderr, flush=True)
+        err = fp32_error(grid, phi32, phi64)
+        del phi64
+    del phi32
+    torch.cuda.empty_cache()
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(a.cpu_seconds)
+        print(f"# cpu: {cpu['value']:.4e} cell-updates/s ({cpu['kind']})", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": r32["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"weno5_tvdrk3_advection_{a.n}cubed_periodic_64spheres_taylor_green",
+                   "grid": [a.n, a.n, a.n], "ghost": 3, "cfl": 0.5, "dt": r32["dt"],
+                   "tiling": r32["tiling"], "velocity": "stored Taylor-Green arrays"},
+        "fp64": None if r64 is None else {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
+                                          "kernel_ms_avg": r64["kernel_ms_avg"],
+                                          "achieved_gbs": r64["achieved_gbs"], "tiling": r64["tiling"]},
+        "fp32_over_fp64": None if r64 is None else r32["value"] / r64["value"],
+        "fp32_error_vs_fp64": err,
+        "roofline": {"bound": "hbm", "achieved": r32["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": r32["achieved_gbs"] / HBM_PEAK_GBS, "traffic": read_traffic(),
+                     "kernel": "k_stage (fused advection stage)",
+                     "kernel_ms_avg": r32["kernel_ms_avg"],
+                     "algorithmic_bytes_per_launch": r32["algorithmic_bytes_per_launch"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,403 @@
+/*
+ * lsm3d_amd.h -- C ABI of the gfx950 (MI355X) level-set update library.
+ *
+ * Two builds share these declarations, as LSMLIB's --enable-float switch
+ * does (configure.ac:258-263): liblsm3d_amd_double.so with LSMLIB_REAL =
+ * double and liblsm3d_amd_float.so with LSMLIB_REAL = float.
+ *
+ * Part 1 -- drop-in toolbox entry points.  Each keeps the symbol name and the
+ * by-reference Fortran-77 prototype of the LSMLIB routine it replaces (cited
+ * per function), so the serial package, the SAMRAI patch loops and the MATLAB
+ * MEX wrappers link against this library unchanged.  Arrays are column-major
+ * over their own ghost box (i fastest).  Arrays may be device memory (the
+ * kernels run in place) or host memory (staged through device buffers).  Calls
+ * complete before returning unless lsm_amd_set_sync(0) is in effect.
+ * Differences from the reference: the D1/D2/D3 scratch arguments are accepted
+ * and not written; out-of-range boxes are reported instead of being undefined
+ * behaviour; sums (volume integrals) accumulate in double precision.
+ *
+ * Part 2 -- runtime control (stream, synchronisation, error channel).
+ * Part 3 -- device-resident fused kernels (this library's own API, no
+ * counterpart in the reference): a whole TVD-RK stage of the advection or
+ * reinitialization equation in one pass, device ghost fills, and a batched
+ * patch-list advance.
+ */
+#ifndef LSM3D_AMD_H
+#define LSM3D_AMD_H
+
+#ifndef LSMLIB_REAL
+#ifdef LSM_REAL
+#define LSMLIB_REAL LSM_REAL
+#else
+#define LSMLIB_REAL double
+#endif
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* the library is built with hidden visibility; everything declared here is
+ * exported */
+#pragma GCC visibility push(default)
+
+/* six by-reference box bounds: ilo, ihi, jlo, jhi, klo, khi */
+#define LSM_AMD_BOX(p)                                                   \
+  const int *ilo_##p, const int *ihi_##p, const int *jlo_##p,           \
+      const int *jhi_##p, const int *klo_##p, const int *khi_##p
+
+/* ======================================================================
+ * Part 1: drop-in LSMLIB toolbox routines
+ * ====================================================================== */
+
+/* --- spatial derivatives: src/toolbox/spatial_derivatives/lsm_spatial_derivatives3d.h --- */
+/* LSM3D_HJ_ENO1, lsm_spatial_derivatives3d.h:83 */
+void lsm3dhjeno1_(LSMLIB_REAL *phi_x_plus, LSMLIB_REAL *phi_y_plus, LSMLIB_REAL *phi_z_plus,
+                  LSM_AMD_BOX(grad_phi_plus_gb), LSMLIB_REAL *phi_x_minus,
+                  LSMLIB_REAL *phi_y_minus, LSMLIB_REAL *phi_z_minus,
+                  LSM_AMD_BOX(grad_phi_minus_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                  LSMLIB_REAL *D1, LSM_AMD_BOX(D1_gb), LSM_AMD_BOX(fb), const LSMLIB_REAL *dx,
+                  const LSMLIB_REAL *dy, const LSMLIB_REAL *dz);
+/* LSM3D_HJ_ENO2, lsm_spatial_derivatives3d.h:159 */
+void lsm3dhjeno2_(LSMLIB_REAL *phi_x_plus, LSMLIB_REAL *phi_y_plus, LSMLIB_REAL *phi_z_plus,
+                  LSM_AMD_BOX(grad_phi_plus_gb), LSMLIB_REAL *phi_x_minus,
+                  LSMLIB_REAL *phi_y_minus, LSMLIB_REAL *phi_z_minus,
+                  LSM_AMD_BOX(grad_phi_minus_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                  LSMLIB_REAL *D1, LSM_AMD_BOX(D1_gb), LSMLIB_REAL *D2, LSM_AMD_BOX(D2_gb),
+                  LSM_AMD_BOX(fb), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                  const LSMLIB_REAL *dz);
+/* LSM3D_HJ_ENO3, lsm_spatial_derivatives3d.h:243 */
+void lsm3dhjeno3_(LSMLIB_REAL *phi_x_plus, LSMLIB_REAL *phi_y_plus, LSMLIB_REAL *phi_z_plus,
+                  LSM_AMD_BOX(grad_phi_plus_gb), LSMLIB_REAL *phi_x_minus,
+                  LSMLIB_REAL *phi_y_minus, LSMLIB_REAL *phi_z_minus,
+                  LSM_AMD_BOX(grad_phi_minus_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                  LSMLIB_REAL *D1, LSM_AMD_BOX(D1_gb), LSMLIB_REAL *D2, LSM_AMD_BOX(D2_gb),
+                  LSMLIB_REAL *D3, LSM_AMD_BOX(D3_gb), LSM_AMD_BOX(fb), const LSMLIB_REAL *dx,
+                  const LSMLIB_REAL *dy, const LSMLIB_REAL *dz);
+/* LSM3D_HJ_WENO5, lsm_spatial_derivatives3d.h:332 */
+void lsm3dhjweno5_(LSMLIB_REAL *phi_x_plus, LSMLIB_REAL *phi_y_plus, LSMLIB_REAL *phi_z_plus,
+                   LSM_AMD_BOX(grad_phi_plus_gb), LSMLIB_REAL *phi_x_minus,
+                   LSMLIB_REAL *phi_y_minus, LSMLIB_REAL *phi_z_minus,
+                   LSM_AMD_BOX(grad_phi_minus_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                   LSMLIB_REAL *D1, LSM_AMD_BOX(D1_gb), LSM_AMD_BOX(fb), const LSMLIB_REAL *dx,
+                   const LSMLIB_REAL *dy, const LSMLIB_REAL *dz);
+/* LSM3D_UPWIND_HJ_ENO1, lsm_spatial_derivatives3d.h:395 */
+void lsm3dupwindhjeno1_(LSMLIB_REAL *phi_x, LSMLIB_REAL *phi_y, LSMLIB_REAL *phi_z,
+                        LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                        const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y,
+                        const LSMLIB_REAL *vel_z, LSM_AMD_BOX(vel_gb), LSMLIB_REAL *D1,
+                        LSM_AMD_BOX(D1_gb), LSM_AMD_BOX(fb), const LSMLIB_REAL *dx,
+                        const LSMLIB_REAL *dy, const LSMLIB_REAL *dz);
+/* LSM3D_UPWIND_HJ_ENO2, lsm_spatial_derivatives3d.h:459 */
+void lsm3dupwindhjeno2_(LSMLIB_REAL *phi_x, LSMLIB_REAL *phi_y, LSMLIB_REAL *phi_z,
+                        LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                        const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y,
+                        const LSMLIB_REAL *vel_z, LSM_AMD_BOX(vel_gb), LSMLIB_REAL *D1,
+                        LSM_AMD_BOX(D1_gb), LSMLIB_REAL *D2, LSM_AMD_BOX(D2_gb), LSM_AMD_BOX(fb),
+                        const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz);
+/* LSM3D_UPWIND_HJ_ENO3, lsm_spatial_derivatives3d.h:531 */
+void lsm3dupwindhjeno3_(LSMLIB_REAL *phi_x, LSMLIB_REAL *phi_y, LSMLIB_REAL *phi_z,
+                        LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                        const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y,
+                        const LSMLIB_REAL *vel_z, LSM_AMD_BOX(vel_gb), LSMLIB_REAL *D1,
+                        LSM_AMD_BOX(D1_gb), LSMLIB_REAL *D2, LSM_AMD_BOX(D2_gb), LSMLIB_REAL *D3,
+                        LSM_AMD_BOX(D3_gb), LSM_AMD_BOX(fb), const LSMLIB_REAL *dx,
+                        const LSMLIB_REAL *dy, const LSMLIB_REAL *dz);
+/* LSM3D_UPWIND_HJ_WENO5, lsm_spatial_derivatives3d.h:608 */
+void lsm3dupwindhjweno5_(LSMLIB_REAL *phi_x, LSMLIB_REAL *phi_y, LSMLIB_REAL *phi_z,
+                         LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                         const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y,
+                         const LSMLIB_REAL *vel_z, LSM_AMD_BOX(vel_gb), LSMLIB_REAL *D1,
+                         LSM_AMD_BOX(D1_gb), LSM_AMD_BOX(fb), const LSMLIB_REAL *dx,
+                         const LSMLIB_REAL *dy, const LSMLIB_REAL *dz);
+
+/* --- level set equation RHS: src/toolbox/level_set_evolution/lsm_level_set_evolution3d.h --- */
+/* LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS, lsm_level_set_evolution3d.h:64 */
+void lsm3dzerooutlevelseteqnrhs_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb));
+/* LSM3D_ADD_ADVECTION_TERM_TO_LSE_RHS, lsm_level_set_evolution3d.h:94 */
+void lsm3daddadvectiontermtolserhs_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb),
+                                    const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
+                                    const LSMLIB_REAL *phi_z, LSM_AMD_BOX(grad_phi_gb),
+                                    const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y,
+                                    const LSMLIB_REAL *vel_z, LSM_AMD_BOX(vel_gb),
+                                    LSM_AMD_BOX(fb));
+/* LSM3D_ADD_NORMAL_VEL_TERM_TO_LSE_RHS, lsm_level_set_evolution3d.h:151 */
+void lsm3daddnormalveltermtolserhs_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb),
+                                    const LSMLIB_REAL *phi_x_plus, const LSMLIB_REAL *phi_y_plus,
+                                    const LSMLIB_REAL *phi_z_plus,
+                                    LSM_AMD_BOX(grad_phi_plus_gb),
+                                    const LSMLIB_REAL *phi_x_minus,
+                                    const LSMLIB_REAL *phi_y_minus,
+                                    const LSMLIB_REAL *phi_z_minus,
+                                    LSM_AMD_BOX(grad_phi_minus_gb), const LSMLIB_REAL *vel_n,
+                                    LSM_AMD_BOX(vel_gb), LSM_AMD_BOX(fb));
+/* LSM3D_ADD_CONST_NORMAL_VEL_TERM_TO_LSE_RHS, lsm_level_set_evolution3d.h:217 */
+void lsm3daddconstnormalveltermtolserhs_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb),
+                                         const LSMLIB_REAL *phi_x_plus,
+                                         const LSMLIB_REAL *phi_y_plus,
+                                         const LSMLIB_REAL *phi_z_plus,
+                                         LSM_AMD_BOX(grad_phi_plus_gb),
+                                         const LSMLIB_REAL *phi_x_minus,
+                                         const LSMLIB_REAL *phi_y_minus,
+                                         const LSMLIB_REAL *phi_z_minus,
+                                         LSM_AMD_BOX(grad_phi_minus_gb),
+                                         const LSMLIB_REAL *vel_n, LSM_AMD_BOX(fb));
+/* LSM3D_ADD_EXTERNAL_AND_NORMAL_VEL_TERM_TO_LSE_RHS, lsm_level_set_evolution3d.h:398.
+ * Computed over the reference's own loop box (its k and j loops run from the
+ * fill box to the rhs ghost box, its i loop over the rhs ghost box,
+ * lsm_level_set_evolution3d.f.in:620-622); every array must cover that box. */
+void lsm3daddexternalandnormalveltermtolserhs_(
+    LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(rhs_gb), const LSMLIB_REAL *phi_x_plus,
+    const LSMLIB_REAL *phi_y_plus, const LSMLIB_REAL *phi_z_plus, LSM_AMD_BOX(grad_phi_plus_gb),
+    const LSMLIB_REAL *phi_x_minus, const LSMLIB_REAL *phi_y_minus,
+    const LSMLIB_REAL *phi_z_minus, LSM_AMD_BOX(grad_phi_minus_gb), const LSMLIB_REAL *vel_n,
+    const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y, const LSMLIB_REAL *vel_z,
+    LSM_AMD_BOX(vel_gb), LSM_AMD_BOX(rhs_fb));
+
+/* --- TVD Runge-Kutta: src/toolbox/time_integration/lsm_tvd_runge_kutta3d.h --- */
+/* LSM3D_RK1_STEP, lsm_tvd_runge_kutta3d.h:59 */
+void lsm3drk1step_(LSMLIB_REAL *u_next, LSM_AMD_BOX(u_next_gb), const LSMLIB_REAL *u_cur,
+                   LSM_AMD_BOX(u_cur_gb), const LSMLIB_REAL *rhs, LSM_AMD_BOX(rhs_gb),
+                   LSM_AMD_BOX(fb), const LSMLIB_REAL *dt);
+/* LSM3D_TVD_RK2_STAGE1, lsm_tvd_runge_kutta3d.h:108 */
+void lsm3dtvdrk2stage1_(LSMLIB_REAL *u_stage1, LSM_AMD_BOX(u_stage1_gb),
+                        const LSMLIB_REAL *u_cur, LSM_AMD_BOX(u_cur_gb), const LSMLIB_REAL *rhs,
+                        LSM_AMD_BOX(rhs_gb), LSM_AMD_BOX(fb), const LSMLIB_REAL *dt);
+/* LSM3D_TVD_RK2_STAGE2, lsm_tvd_runge_kutta3d.h:154 */
+void lsm3dtvdrk2stage2_(LSMLIB_REAL *u_next, LSM_AMD_BOX(u_next_gb),
+                        const LSMLIB_REAL *u_stage1, LSM_AMD_BOX(u_stage1_gb),
+                        const LSMLIB_REAL *u_cur, LSM_AMD_BOX(u_cur_gb), const LSMLIB_REAL *rhs,
+                        LSM_AMD_BOX(rhs_gb), LSM_AMD_BOX(fb), const LSMLIB_REAL *dt);
+/* LSM3D_TVD_RK3_STAGE1, lsm_tvd_runge_kutta3d.h:210 */
+void lsm3dtvdrk3stage1_(LSMLIB_REAL *u_stage1, LSM_AMD_BOX(u_stage1_gb),
+                        const LSMLIB_REAL *u_cur, LSM_AMD_BOX(u_cur_gb), const LSMLIB_REAL *rhs,
+                        LSM_AMD_BOX(rhs_gb), LSM_AMD_BOX(fb), const LSMLIB_REAL *dt);
+/* LSM3D_TVD_RK3_STAGE2, lsm_tvd_runge_kutta3d.h:256 */
+void lsm3dtvdrk3stage2_(LSMLIB_REAL *u_stage2, LSM_AMD_BOX(u_stage2_gb),
+                        const LSMLIB_REAL *u_stage1, LSM_AMD_BOX(u_stage1_gb),
+                        const LSMLIB_REAL *u_cur, LSM_AMD_BOX(u_cur_gb), const LSMLIB_REAL *rhs,
+                        LSM_AMD_BOX(rhs_gb), LSM_AMD_BOX(fb), const LSMLIB_REAL *dt);
+/* LSM3D_TVD_RK3_STAGE3, lsm_tvd_runge_kutta3d.h:309 */
+void lsm3dtvdrk3stage3_(LSMLIB_REAL *u_next, LSM_AMD_BOX(u_next_gb),
+                        const LSMLIB_REAL *u_stage2, LSM_AMD_BOX(u_stage2_gb),
+                        const LSMLIB_REAL *u_cur, LSM_AMD_BOX(u_cur_gb), const LSMLIB_REAL *rhs,
+                        LSM_AMD_BOX(rhs_gb), LSM_AMD_BOX(fb), const LSMLIB_REAL *dt);
+
+/* --- reinitialization: src/toolbox/reinitialization/lsm_reinitialization3d.h --- */
+/* LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS, lsm_reinitialization3d.h:77 */
+void lsm3dcomputereinitializationeqnrhs_(
+    LSMLIB_REAL *reinit_rhs, LSM_AMD_BOX(rhs_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+    const LSMLIB_REAL *phi0, LSM_AMD_BOX(phi0_gb), const LSMLIB_REAL *phi_x_plus,
+    const LSMLIB_REAL *phi_y_plus, const LSMLIB_REAL *phi_z_plus, LSM_AMD_BOX(grad_phi_plus_gb),
+    const LSMLIB_REAL *phi_x_minus, const LSMLIB_REAL *phi_y_minus,
+    const LSMLIB_REAL *phi_z_minus, LSM_AMD_BOX(grad_phi_minus_gb), LSM_AMD_BOX(fb),
+    const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+    const int *use_phi0_for_sgn);
+/* LSM3D_COMPUTE_ORTHOGONALIZATION_EQN_RHS, lsm_reinitialization3d.h:163 */
+void lsm3dcomputeorthogonalizationeqnrhs_(
+    LSMLIB_REAL *ortho_rhs, LSM_AMD_BOX(rhs_gb), const LSMLIB_REAL *phi_x_plus,
+    const LSMLIB_REAL *phi_y_plus, const LSMLIB_REAL *phi_z_plus, LSM_AMD_BOX(grad_phi_plus_gb),
+    const LSMLIB_REAL *phi_x_minus, const LSMLIB_REAL *phi_y_minus,
+    const LSMLIB_REAL *phi_z_minus, LSM_AMD_BOX(grad_phi_minus_gb), const LSMLIB_REAL *psi,
+    LSM_AMD_BOX(psi_gb), const LSMLIB_REAL *psi_x_plus, const LSMLIB_REAL *psi_y_plus,
+    const LSMLIB_REAL *psi_z_plus, LSM_AMD_BOX(grad_psi_plus_gb),
+    const LSMLIB_REAL *psi_x_minus, const LSMLIB_REAL *psi_y_minus,
+    const LSMLIB_REAL *psi_z_minus, LSM_AMD_BOX(grad_psi_minus_gb), LSM_AMD_BOX(fb),
+    const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz);
+/* LSM3D_COMPUTE_DISTANCE_FOR_SUBCELL_FIX, lsm_reinitialization3d.h:247 */
+void lsm3dcomputedistanceforsubcellfix_(LSMLIB_REAL *distance0, const LSMLIB_REAL *phi0,
+                                        LSM_AMD_BOX(phi0_gb), LSM_AMD_BOX(fb),
+                                        const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                        const LSMLIB_REAL *dz);
+/* LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS_SUBCELL_FIX_ORDER1, lsm_reinitialization3d.h:295 */
+void lsm3dcomputereinitializationeqnrhssubcellfixorder1_(
+    LSMLIB_REAL *reinit_rhs, LSM_AMD_BOX(rhs_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+    const LSMLIB_REAL *phi0, const LSMLIB_REAL *distance0, LSM_AMD_BOX(phi0_gb),
+    const LSMLIB_REAL *phi_x_plus, const LSMLIB_REAL *phi_y_plus, const LSMLIB_REAL *phi_z_plus,
+    LSM_AMD_BOX(grad_phi_plus_gb), const LSMLIB_REAL *phi_x_minus,
+    const LSMLIB_REAL *phi_y_minus, const LSMLIB_REAL *phi_z_minus,
+    LSM_AMD_BOX(grad_phi_minus_gb), LSM_AMD_BOX(fb), const LSMLIB_REAL *dx,
+    const LSMLIB_REAL *dy, const LSMLIB_REAL *dz);
+
+/* --- reductions: src/toolbox/utilities/lsm_utilities3d.h, geometry/lsm_geometry3d.h --- */
+/* LSM3D_MAX_NORM_DIFF, lsm_utilities3d.h:87 */
+void lsm3dmaxnormdiff_(LSMLIB_REAL *max_norm_diff, const LSMLIB_REAL *field1,
+                       LSM_AMD_BOX(field1_gb), const LSMLIB_REAL *field2, LSM_AMD_BOX(field2_gb),
+                       LSM_AMD_BOX(ib));
+/* LSM3D_MAX_NORM_DIFF_CONTROL_VOLUME, lsm_utilities3d.h:533 */
+void lsm3dmaxnormdiffcontrolvolume_(LSMLIB_REAL *max_norm_diff, const LSMLIB_REAL *field1,
+                                    LSM_AMD_BOX(field1_gb), const LSMLIB_REAL *field2,
+                                    LSM_AMD_BOX(field2_gb), const LSMLIB_REAL *control_vol,
+                                    LSM_AMD_BOX(control_vol_gb), const int *control_vol_sgn,
+                                    LSM_AMD_BOX(ib));
+/* LSM3D_COMPUTE_STABLE_ADVECTION_DT, lsm_utilities3d.h:221 */
+void lsm3dcomputestableadvectiondt_(LSMLIB_REAL *dt, const LSMLIB_REAL *vel_x,
+                                    const LSMLIB_REAL *vel_y, const LSMLIB_REAL *vel_z,
+                                    LSM_AMD_BOX(vel_gb), LSM_AMD_BOX(ib), const LSMLIB_REAL *dx,
+                                    const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+                                    const LSMLIB_REAL *cfl_number);
+/* LSM3D_COMPUTE_STABLE_ADVECTION_DT_CONTROL_VOLUME, lsm_utilities3d.h:585 */
+void lsm3dcomputestableadvectiondtcontrolvolume_(
+    LSMLIB_REAL *dt, const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y,
+    const LSMLIB_REAL *vel_z, LSM_AMD_BOX(vel_gb), const LSMLIB_REAL *control_vol,
+    LSM_AMD_BOX(control_vol_gb), const int *control_vol_sgn, LSM_AMD_BOX(ib),
+    const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+    const LSMLIB_REAL *cfl_number);
+/* LSM3D_COMPUTE_STABLE_NORMAL_VEL_DT, lsm_utilities3d.h:268 */
+void lsm3dcomputestablenormalveldt_(LSMLIB_REAL *dt, const LSMLIB_REAL *vel_n,
+                                    LSM_AMD_BOX(vel_gb), const LSMLIB_REAL *phi_x_plus,
+                                    const LSMLIB_REAL *phi_y_plus, const LSMLIB_REAL *phi_z_plus,
+                                    LSM_AMD_BOX(grad_phi_plus_gb),
+                                    const LSMLIB_REAL *phi_x_minus,
+                                    const LSMLIB_REAL *phi_y_minus,
+                                    const LSMLIB_REAL *phi_z_minus,
+                                    LSM_AMD_BOX(grad_phi_minus_gb), LSM_AMD_BOX(ib),
+                                    const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                    const LSMLIB_REAL *dz, const LSMLIB_REAL *cfl_number);
+/* LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT, lsm_utilities3d.h:331 */
+void lsm3dcomputestableconstnormalveldt_(LSMLIB_REAL *dt, const LSMLIB_REAL *vel_n,
+                                         const LSMLIB_REAL *phi_x_plus,
+                                         const LSMLIB_REAL *phi_y_plus,
+                                         const LSMLIB_REAL *phi_z_plus,
+                                         LSM_AMD_BOX(grad_phi_plus_gb),
+                                         const LSMLIB_REAL *phi_x_minus,
+                                         const LSMLIB_REAL *phi_y_minus,
+                                         const LSMLIB_REAL *phi_z_minus,
+                                         LSM_AMD_BOX(grad_phi_minus_gb), LSM_AMD_BOX(ib),
+                                         const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                         const LSMLIB_REAL *dz, const LSMLIB_REAL *cfl_number);
+/* LSM3D_VOLUME_INTEGRAL_PHI_LESS_THAN_ZERO, lsm_utilities3d.h:383 */
+void lsm3dvolumeintegralphilessthanzero_(LSMLIB_REAL *int_F, const LSMLIB_REAL *F,
+                                         LSM_AMD_BOX(F_gb), const LSMLIB_REAL *phi,
+                                         LSM_AMD_BOX(phi_gb), LSM_AMD_BOX(ib),
+                                         const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                         const LSMLIB_REAL *dz, const LSMLIB_REAL *epsilon);
+/* LSM3D_VOLUME_INTEGRAL_PHI_GREATER_THAN_ZERO, lsm_utilities3d.h:430 */
+void lsm3dvolumeintegralphigreaterthanzero_(LSMLIB_REAL *int_F, const LSMLIB_REAL *F,
+                                            LSM_AMD_BOX(F_gb), const LSMLIB_REAL *phi,
+                                            LSM_AMD_BOX(phi_gb), LSM_AMD_BOX(ib),
+                                            const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                            const LSMLIB_REAL *dz, const LSMLIB_REAL *epsilon);
+/* LSM3D_VOLUME_REGION_PHI_LESS_THAN_ZERO (interface mass), lsm_geometry3d.h:180 */
+void lsm3dvolumeregionphilessthanzero_(LSMLIB_REAL *volume, const LSMLIB_REAL *phi,
+                                       LSM_AMD_BOX(phi_gb), LSM_AMD_BOX(ib),
+                                       const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                       const LSMLIB_REAL *dz, const LSMLIB_REAL *epsilon);
+/* LSM3D_VOLUME_REGION_PHI_GREATER_THAN_ZERO, lsm_geometry3d.h:217 */
+void lsm3dvolumeregionphigreaterthanzero_(LSMLIB_REAL *volume, const LSMLIB_REAL *phi,
+                                          LSM_AMD_BOX(phi_gb), LSM_AMD_BOX(ib),
+                                          const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                          const LSMLIB_REAL *dz, const LSMLIB_REAL *epsilon);
+
+/* --- ghost fill: src/toolbox/boundary_conditions/lsm_boundary_conditions3d.h --- */
+/* bdry_location_idx: 0 x_lo, 1 x_hi, 2 y_lo, 3 y_hi, 4 z_lo, 5 z_hi */
+/* LSM3D_LINEAR_EXTRAPOLATION, lsm_boundary_conditions3d.h:71 */
+void lsm3dlinearextrapolation_(LSMLIB_REAL *phi, LSM_AMD_BOX(gb), LSM_AMD_BOX(fb),
+                               const int *bdry_location_idx);
+/* LSM3D_SIGNED_LINEAR_EXTRAPOLATION, lsm_boundary_conditions3d.h:111 */
+void lsm3dsignedlinearextrapolation_(LSMLIB_REAL *phi, LSM_AMD_BOX(gb), LSM_AMD_BOX(fb),
+                                     const int *bdry_location_idx);
+/* LSM3D_COPY_EXTRAPOLATION, lsm_boundary_conditions3d.h:135 */
+void lsm3dcopyextrapolation_(LSMLIB_REAL *phi, LSM_AMD_BOX(gb), LSM_AMD_BOX(fb),
+                             const int *bdry_location_idx);
+/* LSM3D_HOMOGENEOUS_NEUMANN_ENO1, lsm_boundary_conditions3d.h:161 */
+void lsm3dhomogeneousneumanneno1_(LSMLIB_REAL *phi, LSM_AMD_BOX(gb), LSM_AMD_BOX(fb),
+                                  const int *bdry_location_idx);
+/* LSM3D_HOMOGENEOUS_NEUMANN_ENO2, lsm_boundary_conditions3d.h:186 */
+void lsm3dhomogeneousneumanneno2_(LSMLIB_REAL *phi, LSM_AMD_BOX(gb), LSM_AMD_BOX(fb),
+                                  const int *bdry_location_idx);
+/* LSM3D_HOMOGENEOUS_NEUMANN_ENO3, lsm_boundary_conditions3d.h:211 */
+void lsm3dhomogeneousneumanneno3_(LSMLIB_REAL *phi, LSM_AMD_BOX(gb), LSM_AMD_BOX(fb),
+                                  const int *bdry_location_idx);
+/* LSM3D_HOMOGENEOUS_NEUMANN_WENO5, lsm_boundary_conditions3d.h:236 */
+void lsm3dhomogeneousneumannweno5_(LSMLIB_REAL *phi, LSM_AMD_BOX(gb), LSM_AMD_BOX(fb),
+                                   const int *bdry_location_idx);
+
+/* ======================================================================
+ * Part 2: runtime control (no counterpart in the reference: the Fortran
+ * library has no stream, no error channel and no device)
+ * ====================================================================== */
+/* Stream (a hipStream_t) that this host thread's calls queue work on; NULL =
+ * the default stream. */
+void lsm_amd_set_stream(void *stream);
+void *lsm_amd_get_stream(void);
+/* 1 (default): Part 1 calls complete before returning, as the Fortran does.
+ * 0: they only order work on the stream (host-memory and reduction calls
+ * still wait, since they must hand back host data). */
+void lsm_amd_set_sync(int sync);
+/* 0 (default): invalid arguments or HIP errors print a message and abort.
+ * 1: the message is recorded for lsm_amd_last_error() and the call returns. */
+void lsm_amd_set_error_mode(int mode);
+int lsm_amd_error_pending(void);
+const char *lsm_amd_last_error(void);
+void lsm_amd_clear_error(void);
+/* sizeof(LSMLIB_REAL) of this build */
+int lsm_amd_real_size(void);
+
+/* ======================================================================
+ * Part 3: fused device-resident stages (this library's own API; no
+ * counterpart in the reference).  All arrays must be device memory; work is
+ * queued on the library stream; calls return at once under
+ * lsm_amd_set_sync(0).  Boxes are int[6] = {ilo, ihi, jlo, jhi, klo, khi}.
+ * ====================================================================== */
+/* Runge-Kutta forms: phi_out = ...
+ *   1  phi_cur + dt*rhs                       (RK1, TVD-RK2/3 stage 1)
+ *   22 0.5*(phi_cur + phi_stage + dt*rhs)     (TVD-RK2 stage 2)
+ *   32 0.75*phi_cur + 0.25*(phi_stage + dt*rhs)   (TVD-RK3 stage 2)
+ *   33 phi_cur/3 + 2/3*(phi_stage + dt*rhs)   (TVD-RK3 stage 3)
+ * phi_cur may be NULL for form 1 (then phi_cur = phi_stage).  phi_out may
+ * alias phi_cur (it is read only at the cell being written) but not
+ * phi_stage.  periodic_mask bit d (x=1, y=2, z=4): the stage also writes
+ * phi_out's periodic ghost images along axis d (3-cell ghost layer), so the
+ * next stage needs no ghost fill. */
+#define LSM_AMD_RK1 1
+#define LSM_AMD_TVDRK2_STAGE2 22
+#define LSM_AMD_TVDRK3_STAGE2 32
+#define LSM_AMD_TVDRK3_STAGE3 33
+
+/* One stage of phi_t + V.grad(phi) = 0: rhs = -V.grad(phi_stage) with upwind
+ * HJ-WENO5, fused with the Runge-Kutta combination.  Equivalent to
+ * LSM3D_UPWIND_HJ_WENO5 + LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS +
+ * LSM3D_ADD_ADVECTION_TERM_TO_LSE_RHS + LSM3D_TVD_RK*_STAGE* in one pass. */
+void lsm3d_amd_advect_stage(int rk_form, LSMLIB_REAL *phi_out, const int out_gb[6],
+                            const LSMLIB_REAL *phi_stage, const int stage_gb[6],
+                            const LSMLIB_REAL *phi_cur, const int cur_gb[6],
+                            const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y,
+                            const LSMLIB_REAL *vel_z, const int vel_gb[6], const int fb[6],
+                            LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt,
+                            int periodic_mask);
+/* As lsm3d_amd_advect_stage with a constant velocity (per-patch velocity). */
+void lsm3d_amd_advect_const_stage(int rk_form, LSMLIB_REAL *phi_out, const int out_gb[6],
+                                  const LSMLIB_REAL *phi_stage, const int stage_gb[6],
+                                  const LSMLIB_REAL *phi_cur, const int cur_gb[6],
+                                  LSMLIB_REAL vel_x, LSMLIB_REAL vel_y, LSMLIB_REAL vel_z,
+                                  const int fb[6], LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz,
+                                  LSMLIB_REAL dt, int periodic_mask);
+/* One stage of the reinitialization equation phi_t = S(phi)(1 - |grad phi|):
+ * HJ-WENO5 plus/minus derivatives of phi_stage, the Godunov right-hand side of
+ * LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS (smoothed sign from phi0 when
+ * use_phi0_for_sgn == 1, else from phi_stage) and the Runge-Kutta form. */
+void lsm3d_amd_reinit_stage(int rk_form, LSMLIB_REAL *phi_out, const int out_gb[6],
+                            const LSMLIB_REAL *phi_stage, const int stage_gb[6],
+                            const LSMLIB_REAL *phi_cur, const int cur_gb[6],
+                            const LSMLIB_REAL *phi0, const int phi0_gb[6], int use_phi0_for_sgn,
+                            const int fb[6], LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz,
+                            LSMLIB_REAL dt, int periodic_mask);
+/* Fills the ghost cells of phi outside fb, face by face in the order x_lo,
+ * x_hi, y_lo, y_hi, z_lo, z_hi (each face over the full ghost box in the
+ * other directions, as the LSM3D_*_EXTRAPOLATION routines do).  bc[face]:
+ * 0 none, 1 homogeneous Neumann (copy), 2 linear, 3 signed linear
+ * extrapolation (codes of BoundaryConditionModule.h:125-130), 5 periodic. */
+void lsm3d_amd_fill_ghosts(LSMLIB_REAL *phi, const int gb[6], const int fb[6], const int bc[6]);
+/* Workgroup tiling of the fused stages: configuration cfg (0 ..
+ * lsm3d_amd_stage_tiling_count()-1, shape from lsm3d_amd_stage_tiling_shape)
+ * and kz planes per workgroup, for this host thread. */
+void lsm3d_amd_set_stage_tiling(int cfg, int kz);
+int lsm3d_amd_stage_tiling_count(void);
+void lsm3d_amd_stage_tiling_shape(int cfg, int *tx, int *ty);
+
+#pragma GCC visibility pop
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSM3D_AMD_H */

@@ -1,0 +1,103 @@
+"""Loads the in-tree gfx950 libraries.
+
+There is no CPU fallback: if the shared library is missing, or no GPU is
+visible when a compute entry point is called, the call raises.
+"""
+import ctypes
+from pathlib import Path
+
+try:  # load torch's HIP runtime first so torch and this library share one
+    import torch  # noqa: F401  (same SONAME libamdhip64.so.7: first loaded wins)
+except ImportError:  # pragma: no cover - torch is optional for the C ABI
+    torch = None
+
+from .f77 import F77Library
+
+PKG = Path(__file__).resolve().parent
+LIBDIR = PKG / "lib"
+
+_LIBS = {}
+
+
+class LsmError(RuntimeError):
+    """An entry point reported invalid arguments or a HIP failure."""
+
+
+def lib_path(real):
+    return LIBDIR / f"liblsm3d_amd_{real}.so"
+
+
+class ToolboxLibrary(F77Library):
+    """The product library: LSM3D_* entry points plus runtime control.
+
+    Calls run in the library's "record" error mode and raise LsmError with
+    the library's message instead of aborting the process.
+    """
+
+    def __init__(self, real="double"):
+        p = lib_path(real)
+        if not p.exists():
+            raise LsmError(f"{p} is not built; run `python -m lsmlib_amd.build` "
+                           "(there is no CPU fallback)")
+        super().__init__(p, real=real, prefix="")
+        L = self.lib
+        L.lsm_amd_set_error_mode.argtypes = [ctypes.c_int]
+        L.lsm_amd_set_stream.argtypes = [ctypes.c_void_p]
+        L.lsm_amd_get_stream.restype = ctypes.c_void_p
+        L.lsm_amd_set_sync.argtypes = [ctypes.c_int]
+        L.lsm_amd_last_error.restype = ctypes.c_char_p
+        L.lsm_amd_error_pending.restype = ctypes.c_int
+        L.lsm_amd_real_size.restype = ctypes.c_int
+        self._bind_fused()
+        want = 8 if real == "double" else 4
+        if L.lsm_amd_real_size() != want:
+            raise LsmError(f"{p} was built for a different LSMLIB_REAL")
+
+    def _bind_fused(self):
+        """argtypes of the fused device API (plain values, int[6] boxes)."""
+        L, R = self.lib, self._ct
+        P, I, B = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)
+        L.lsm3d_amd_advect_stage.argtypes = [I, P, B, P, B, P, B, P, P, P, B, B, R, R, R, R, I]
+        L.lsm3d_amd_advect_const_stage.argtypes = [I, P, B, P, B, P, B, R, R, R, B, R, R, R, R, I]
+        L.lsm3d_amd_reinit_stage.argtypes = [I, P, B, P, B, P, B, P, B, I, B, R, R, R, R, I]
+        L.lsm3d_amd_fill_ghosts.argtypes = [P, B, B, B]
+        L.lsm3d_amd_set_stage_tiling.argtypes = [I, I]
+        L.lsm3d_amd_stage_tiling_count.restype = I
+        L.lsm3d_amd_stage_tiling_shape.argtypes = [I, B, B]
+        for f in ("lsm3d_amd_advect_stage", "lsm3d_amd_advect_const_stage", "lsm3d_amd_reinit_stage",
+                  "lsm3d_amd_fill_ghosts", "lsm3d_amd_set_stage_tiling"):
+            getattr(L, f).restype = None
+
+    def fused(self, fname, *args):
+        """Calls a fused-API function (Part 3 of include/lsm3d_amd.h)."""
+        self._prepare()
+        getattr(self.lib, fname)(*args)
+        self._check(fname)
+
+    def _prepare(self):
+        self.lib.lsm_amd_set_error_mode(1)
+
+    def _check(self, name):
+        if self.lib.lsm_amd_error_pending():
+            msg = self.lib.lsm_amd_last_error().decode(errors="replace")
+            self.lib.lsm_amd_clear_error()
+            raise LsmError(msg)
+
+    def call(self, name, *args):
+        self._prepare()
+        out = super().call(name, *args)
+        self._check(name)
+        return out
+
+    def set_stream(self, stream_handle):
+        self.lib.lsm_amd_set_stream(ctypes.c_void_p(int(stream_handle or 0)))
+
+    def set_sync(self, sync):
+        self.lib.lsm_amd_set_sync(int(bool(sync)))
+
+
+def toolbox(real="double"):
+    """The LSM3D_* library for `real` ('double' or 'float'), loaded once."""
+    if real not in _LIBS:
+        _LIBS[real] = ToolboxLibrary(real)
+    return _LIBS[real]

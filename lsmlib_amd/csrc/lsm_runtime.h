@@ -1,0 +1,86 @@
+// Host-side runtime shared by every entry point: the stream work is queued
+// on, the error channel, and staging of host (malloc'ed) arrays for callers
+// that pass host memory as the reference's callers do.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lsm_common.h"
+
+namespace lsm {
+
+// --- configuration (per host thread) ---------------------------------------
+hipStream_t current_stream();
+bool sync_on_return();
+
+// --- errors -------------------------------------------------------------------
+// Reports an error.  In abort mode (the default, matching a Fortran library
+// with no error channel) the message goes to stderr and the process aborts; in
+// record mode the message is kept for lsm_amd_last_error() and the entry point
+// returns without touching its outputs.
+void raise_error(const std::string& msg);
+bool error_pending();
+void check_hip(hipError_t e, const char* what);
+
+// --- memory ---------------------------------------------------------------------
+// True when p is device (or managed) memory that kernels can address.
+bool is_device_pointer(const void* p);
+
+// Device scratch of at least `bytes`, owned by the library per device and
+// reused across calls (reductions keep per-block partials here).
+void* device_scratch(size_t bytes);
+
+// Collects the arrays of one entry-point call.  Device pointers pass through;
+// host arrays are copied to a device buffer (whole ghost box) before the
+// launch and, for outputs, copied back by finish().  finish() also
+// synchronizes when the call must complete before returning.
+class Call {
+ public:
+  explicit Call(const char* name);
+  ~Call();
+  template <typename T>
+  Field<T> out(T* p, const Box& gb) {
+    return make_field<T>(reinterpret_cast<T*>(stage(p, gb, sizeof(T), true)), gb);
+  }
+  template <typename T>
+  Field<const T> in(const T* p, const Box& gb) {
+    return make_field<const T>(reinterpret_cast<const T*>(stage(const_cast<T*>(p), gb, sizeof(T), false)),
+                               gb);
+  }
+  hipStream_t stream() const { return stream_; }
+  const char* name() const { return name_; }
+  void launched(const char* kernel);  // checks the launch
+  void finish();
+  bool ok() const { return !error_pending(); }
+
+ private:
+  void* stage(void* p, const Box& gb, size_t elem, bool is_out);
+  struct Staged {
+    void* host;
+    void* dev;
+    size_t bytes;
+    bool is_out;
+  };
+  const char* name_;
+  hipStream_t stream_;
+  std::vector<Staged> staged_;
+  bool finished_ = false;
+};
+
+// --- argument checks --------------------------------------------------------------
+// `inner` grown by `halo` cells (per axis) must lie inside `gb`.
+bool box_contains(const Box& gb, const Box& inner, int hx, int hy, int hz);
+void require_inside(const char* fn, const char* arr, const Box& gb, const Box& fb, int hx, int hy,
+                    int hz);
+
+inline Box box_of(const int* ilo, const int* ihi, const int* jlo, const int* jhi, const int* klo,
+                  const int* khi) {
+  return Box{*ilo, *ihi, *jlo, *jhi, *klo, *khi};
+}
+
+}  // namespace lsm

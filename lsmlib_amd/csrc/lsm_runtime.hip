@@ -1,0 +1,172 @@
+// Runtime helpers: stream selection, error channel, host-array staging and
+// the library-owned reduction scratch.
+#include "lsm_runtime.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+
+#include "../../include/lsm3d_amd.h"
+
+namespace lsm {
+
+namespace {
+thread_local hipStream_t tl_stream = nullptr;
+thread_local int tl_sync = 1;
+thread_local int tl_error_mode = 0;  // 0 abort, 1 record
+thread_local bool tl_error = false;
+thread_local std::string tl_message;
+
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch[kMaxDevices] = {nullptr};
+size_t g_scratch_bytes[kMaxDevices] = {0};
+}  // namespace
+
+hipStream_t current_stream() { return tl_stream; }
+bool sync_on_return() { return tl_sync != 0; }
+
+void raise_error(const std::string& msg) {
+  if (tl_error_mode == 0) {
+    std::fprintf(stderr, "lsmlib_amd: %s\n", msg.c_str());
+    std::fflush(stderr);
+    std::abort();
+  }
+  if (!tl_error) {  // keep the first error of a call
+    tl_error = true;
+    tl_message = msg;
+  }
+}
+
+bool error_pending() { return tl_error; }
+
+void check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) raise_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+bool is_device_pointer(const void* p) {
+  if (p == nullptr) return false;
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // unregistered host memory reports an error; clear it
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+         attr.type == hipMemoryTypeUnified;
+}
+
+void* device_scratch(size_t bytes) {
+  int dev = 0;
+  check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) {
+    raise_error("device index out of range");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_bytes[dev] < bytes) {
+    if (g_scratch[dev]) {
+      // another thread's queued work may still read the old buffer
+      check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      check_hip(hipFree(g_scratch[dev]), "hipFree(scratch)");
+    }
+    size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
+    g_scratch[dev] = nullptr;
+    g_scratch_bytes[dev] = 0;
+    check_hip(hipMalloc(&g_scratch[dev], want), "hipMalloc(scratch)");
+    if (g_scratch[dev]) g_scratch_bytes[dev] = want;
+  }
+  return g_scratch[dev];
+}
+
+Call::Call(const char* name) : name_(name), stream_(tl_stream) {
+  tl_error = false;
+  tl_message.clear();
+}
+
+Call::~Call() {
+  if (!finished_) finish();
+}
+
+void* Call::stage(void* p, const Box& gb, size_t elem, bool is_out) {
+  if (error_pending()) return nullptr;
+  if (p == nullptr) {
+    raise_error(std::string(name_) + ": null array argument");
+    return nullptr;
+  }
+  if (gb.cells() <= 0) {
+    raise_error(std::string(name_) + ": empty ghost box for an array argument");
+    return nullptr;
+  }
+  if (is_device_pointer(p)) return p;
+  for (auto& s : staged_)  // the same host array passed twice shares one buffer
+    if (s.host == p) {
+      s.is_out = s.is_out || is_out;
+      return s.dev;
+    }
+  Staged s{p, nullptr, (size_t)gb.cells() * elem, is_out};
+  check_hip(hipMalloc(&s.dev, s.bytes), "hipMalloc(staging)");
+  if (error_pending()) return nullptr;
+  check_hip(hipMemcpyAsync(s.dev, p, s.bytes, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync H2D");
+  staged_.push_back(s);
+  return s.dev;
+}
+
+void Call::launched(const char* kernel) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) raise_error(std::string(name_) + ": launch of " + kernel + " failed: " +
+                                   hipGetErrorString(e));
+}
+
+void Call::finish() {
+  finished_ = true;
+  bool any_host = !staged_.empty();
+  if (!error_pending()) {
+    for (auto& s : staged_)
+      if (s.is_out)
+        check_hip(hipMemcpyAsync(s.host, s.dev, s.bytes, hipMemcpyDeviceToHost, stream_),
+                  "hipMemcpyAsync D2H");
+  }
+  if (any_host || sync_on_return()) check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  for (auto& s : staged_) (void)hipFree(s.dev);
+  staged_.clear();
+}
+
+bool box_contains(const Box& gb, const Box& b, int hx, int hy, int hz) {
+  return b.ilo - hx >= gb.ilo && b.ihi + hx <= gb.ihi && b.jlo - hy >= gb.jlo &&
+         b.jhi + hy <= gb.jhi && b.klo - hz >= gb.klo && b.khi + hz <= gb.khi;
+}
+
+void require_inside(const char* fn, const char* arr, const Box& gb, const Box& fb, int hx, int hy,
+                    int hz) {
+  if (error_pending()) return;
+  if (fb.cells() == 0) return;  // nothing is touched
+  if (!box_contains(gb, fb, hx, hy, hz)) {
+    char buf[512];
+    std::snprintf(buf, sizeof buf,
+                  "%s: box [%d:%d,%d:%d,%d:%d] grown by (%d,%d,%d) is not inside the ghost box "
+                  "[%d:%d,%d:%d,%d:%d] of %s",
+                  fn, fb.ilo, fb.ihi, fb.jlo, fb.jhi, fb.klo, fb.khi, hx, hy, hz, gb.ilo, gb.ihi,
+                  gb.jlo, gb.jhi, gb.klo, gb.khi, arr);
+    raise_error(buf);
+  }
+}
+
+}  // namespace lsm
+
+// --- configuration and error entry points ---------------------------------------
+extern "C" {
+
+void lsm_amd_set_stream(void* stream) { lsm::tl_stream = reinterpret_cast<hipStream_t>(stream); }
+void* lsm_amd_get_stream(void) { return reinterpret_cast<void*>(lsm::tl_stream); }
+void lsm_amd_set_sync(int sync) { lsm::tl_sync = sync; }
+void lsm_amd_set_error_mode(int mode) { lsm::tl_error_mode = mode; }
+int lsm_amd_error_pending(void) { return lsm::tl_error ? 1 : 0; }
+const char* lsm_amd_last_error(void) { return lsm::tl_message.c_str(); }
+void lsm_amd_clear_error(void) {
+  lsm::tl_error = false;
+  lsm::tl_message.clear();
+}
+int lsm_amd_real_size(void) { return (int)sizeof(real_t); }
+
+}  // extern "C"

@@ -1,0 +1,174 @@
+// One-sided Hamilton-Jacobi ENO1/2/3 and WENO5 derivatives along one axis.
+//
+// Every routine takes the seven samples p[0..6] = phi(c-3 .. c+3) along the
+// axis through cell c (only p[3-R .. 3+R] are read, R = scheme radius) and
+// the reciprocal spacing inv = 1/dx computed on the host as the reference
+// does (inv_dx = 1.0d0/dx, lsm_spatial_derivatives3d.f.in:276).
+//
+// The undivided differences are formed exactly as lsm3dComputeDn builds them
+// (lsm_spatial_derivatives3d.f.in:111-121): with d1[q] = D1(c+q-2),
+//   D1(i) = phi(i) - phi(i-1)           (face i-1/2 stored at i)
+//   D2(i) = D1(i+1) - D1(i)             (cell centred)
+//   D3(i) = D2(i) - D2(i-1)             (face centred)
+// Those are pure subtractions, so the ENO stencil choices (strict '<' on
+// |D2|, |D3|) are bit-identical to the reference's whatever the compiler
+// fuses elsewhere.
+#pragma once
+
+#include "lsm_common.h"
+
+namespace lsm {
+
+enum Scheme { ENO1 = 1, ENO2 = 2, ENO3 = 3, WENO5 = 5 };
+
+template <int S> struct SchemeRadius { static constexpr int R = (S == ENO1) ? 1 : (S == ENO2) ? 2 : 3; };
+
+// WENO5 combination of five divided differences, in the reference's order
+// (lsm_spatial_derivatives3d.f.in:1292-1326).
+template <typename T>
+__device__ __forceinline__ T weno5_ref(T v1, T v2, T v3, T v4, T v5) {
+  const T one_third = T(1.0 / 3.0), seven_sixths = T(7.0 / 6.0), eleven_sixths = T(11.0 / 6.0);
+  const T one_sixth = T(1.0 / 6.0), five_sixths = T(5.0 / 6.0);
+  const T thirteen_twelfths = T(13.0 / 12.0), one_fourth = T(0.25);
+  T m = v1 * v1;
+  m = fmax(m, v2 * v2);
+  m = fmax(m, v3 * v3);
+  m = fmax(m, v4 * v4);
+  m = fmax(m, v5 * v5);
+  const T eps = T(1e-6) * m + Tol<T>::tiny;
+  const T p1 = one_third * v1 - seven_sixths * v2 + eleven_sixths * v3;
+  const T p2 = -one_sixth * v2 + five_sixths * v3 + one_third * v4;
+  const T p3 = one_third * v3 + five_sixths * v4 - one_sixth * v5;
+  const T t1 = v1 - T(2) * v2 + v3, t2 = v1 - T(4) * v2 + T(3) * v3;
+  const T t3 = v2 - T(2) * v3 + v4, t4 = v2 - v4;
+  const T t5 = v3 - T(2) * v4 + v5, t6 = T(3) * v3 - T(4) * v4 + v5;
+  const T S1 = thirteen_twelfths * (t1 * t1) + one_fourth * (t2 * t2);
+  const T S2 = thirteen_twelfths * (t3 * t3) + one_fourth * (t4 * t4);
+  const T S3 = thirteen_twelfths * (t5 * t5) + one_fourth * (t6 * t6);
+  const T s1 = S1 + eps, s2 = S2 + eps, s3 = S3 + eps;
+  T a1 = T(0.1) / (s1 * s1);
+  T a2 = T(0.6) / (s2 * s2);
+  T a3 = T(0.3) / (s3 * s3);
+  const T inv_sum_a = T(1) / (a1 + a2 + a3);
+  a1 = a1 * inv_sum_a;
+  a2 = a2 * inv_sum_a;
+  a3 = a3 * inv_sum_a;
+  return a1 * p1 + a2 * p2 + a3 * p3;
+}
+
+// d1[q] = D1(c+q-2), q = 0..5 (only those the scheme needs are formed).
+template <typename T, int S>
+__device__ __forceinline__ void first_differences(const T* p, T* d1) {
+  if (S == ENO1) {
+    d1[2] = p[3] - p[2];
+    d1[3] = p[4] - p[3];
+  } else if (S == ENO2) {
+#pragma unroll
+    for (int q = 1; q <= 4; q++) d1[q] = p[q + 1] - p[q];
+  } else {
+#pragma unroll
+    for (int q = 0; q <= 5; q++) d1[q] = p[q + 1] - p[q];
+  }
+}
+
+// Plus (forward) derivative at c, lsm_spatial_derivatives3d.f.in:299-308 (ENO1),
+// :518-527 (ENO2), :838-868 (ENO3), :1286-1326 (WENO5).
+template <typename T, int S>
+__device__ __forceinline__ T hj_plus(const T* p, T inv) {
+  T d1[6];
+  first_differences<T, S>(p, d1);
+  if (S == ENO1) {
+    return d1[3] * inv;
+  } else if (S == ENO2) {
+    const T d2c = d1[3] - d1[2], d2p = d1[4] - d1[3];
+    return (fabs(d2c) < fabs(d2p)) ? (d1[3] - T(0.5) * d2c) * inv : (d1[3] - T(0.5) * d2p) * inv;
+  } else if (S == ENO3) {
+    const T d2m = d1[2] - d1[1], d2c = d1[3] - d1[2], d2p = d1[4] - d1[3], d2pp = d1[5] - d1[4];
+    const T d3c = d2c - d2m, d3p = d2p - d2c, d3pp = d2pp - d2p;
+    T v = d1[3];
+    if (fabs(d2c) < fabs(d2p)) {
+      v = v - T(0.5) * d2c;
+      v = (fabs(d3c) < fabs(d3p)) ? v - T(1.0 / 6.0) * d3c : v - T(1.0 / 6.0) * d3p;
+    } else {
+      v = v - T(0.5) * d2p;
+      v = (fabs(d3p) < fabs(d3pp)) ? v + T(1.0 / 3.0) * d3p : v + T(1.0 / 3.0) * d3pp;
+    }
+    return v * inv;
+  } else {
+    return weno5_ref<T>(d1[5] * inv, d1[4] * inv, d1[3] * inv, d1[2] * inv, d1[1] * inv);
+  }
+}
+
+// Minus (backward) derivative at c, lsm_spatial_derivatives3d.f.in:299-308 (ENO1),
+// :529-535 (ENO2), :872-896 (ENO3), :1330-1370 (WENO5).
+template <typename T, int S>
+__device__ __forceinline__ T hj_minus(const T* p, T inv) {
+  T d1[6];
+  first_differences<T, S>(p, d1);
+  if (S == ENO1) {
+    return d1[2] * inv;
+  } else if (S == ENO2) {
+    const T d2m = d1[2] - d1[1], d2c = d1[3] - d1[2];
+    return (fabs(d2m) < fabs(d2c)) ? (d1[2] + T(0.5) * d2m) * inv : (d1[2] + T(0.5) * d2c) * inv;
+  } else if (S == ENO3) {
+    const T d2mm = d1[1] - d1[0], d2m = d1[2] - d1[1], d2c = d1[3] - d1[2], d2p = d1[4] - d1[3];
+    const T d3m = d2m - d2mm, d3c = d2c - d2m, d3p = d2p - d2c;
+    T v = d1[2];
+    if (fabs(d2m) < fabs(d2c)) {
+      v = v + T(0.5) * d2m;
+      v = (fabs(d3m) < fabs(d3c)) ? v + T(1.0 / 3.0) * d3m : v + T(1.0 / 3.0) * d3c;
+    } else {
+      v = v + T(0.5) * d2c;
+      v = (fabs(d3c) < fabs(d3p)) ? v - T(1.0 / 6.0) * d3c : v - T(1.0 / 6.0) * d3p;
+    }
+    return v * inv;
+  } else {
+    return weno5_ref<T>(d1[0] * inv, d1[1] * inv, d1[2] * inv, d1[3] * inv, d1[4] * inv);
+  }
+}
+
+// Upwind selection (lsm_spatial_derivatives3d.f.in:1735-1743, 2756-2775):
+// |vel| < zero_tol -> 0; vel > 0 -> backward; otherwise forward.
+template <typename T, int S>
+__device__ __forceinline__ T hj_upwind(const T* p, T vel, T inv) {
+  if (fabs(vel) < Tol<T>::zero) return T(0);
+  return (vel > T(0)) ? hj_minus<T, S>(p, inv) : hj_plus<T, S>(p, inv);
+}
+
+// Godunov |grad phi|* of the reinitialization equation for the sign of s
+// (lsm_reinitialization3d.f.in:168-185); returns |grad phi|*^2
+template <typename T>
+__device__ __forceinline__ T godunov_star_sq(T s, T xp, T yp, T zp, T xm, T ym, T zm) {
+  const T z = T(0);
+  T gx, gy, gz;
+  if (s > z) {
+    gx = fmax(fmax(-xp, z), fmax(xm, z));
+    gy = fmax(fmax(-yp, z), fmax(ym, z));
+    gz = fmax(fmax(-zp, z), fmax(zm, z));
+  } else {
+    gx = fmax(fmax(xp, z), fmax(-xm, z));
+    gy = fmax(fmax(yp, z), fmax(-ym, z));
+    gz = fmax(fmax(zp, z), fmax(-zm, z));
+  }
+  return gx * gx + gy * gy + gz * gz;
+}
+
+// TVD Runge-Kutta stage combinations (lsm_tvd_runge_kutta3d.f), given the
+// current solution uc, the stage input us and the stage right-hand side r.
+enum RKForm { RK_EULER = 1, RK2_STAGE2 = 22, RK3_STAGE2 = 32, RK3_STAGE3 = 33 };
+
+template <typename T>
+__device__ __forceinline__ T rk_combine(int form, T uc, T us, T r, T dt) {
+  switch (form) {
+    case RK2_STAGE2:  // :240-241
+      return T(0.5) * (uc + us + dt * r);
+    case RK3_STAGE2:  // :403-404
+      return T(0.75) * uc + T(0.25) * (us + dt * r);
+    case RK3_STAGE3:  // :486-488
+      return T(1.0 / 3.0) * uc + T(2.0 / 3.0) * (us + dt * r);
+    default:  // u_next = u_cur + dt*rhs, :78
+      return uc + dt * r;
+  }
+}
+
+}  // namespace lsm

@@ -1,0 +1,1027 @@
+// gfx950 kernels behind LSMLIB's LSM3D_* toolbox entry points, and the
+// extern "C" Fortran-77-convention entry points themselves.
+//
+// Each entry point keeps the reference prototype (every argument by
+// reference, one ghost box per array, a fill/include box, spacings), checks
+// that every stencil read stays inside its array's ghost box, accepts device
+// or host memory (host arrays are staged through device buffers), queues one
+// kernel on the library stream and, like the Fortran, completes before it
+// returns (unless lsm_amd_set_sync(0) asked for stream ordering only).
+//
+// The D1/D2/D3 scratch arrays of the derivative routines are accepted and
+// not written: no caller reads them after a call, and the kernels form the
+// undivided differences in registers instead of re-reading them from HBM.
+#include <cmath>
+#include <string>
+
+#include "../../include/lsm3d_amd.h"
+#include "lsm_common.h"
+#include "lsm_runtime.h"
+#include "lsm_stencil.h"
+
+using lsm::Box;
+using lsm::Call;
+using lsm::Field;
+
+namespace lsm {
+
+// ============================================================================
+// Spatial derivatives
+// ============================================================================
+template <typename T, int S, int DIR>
+__device__ __forceinline__ void gather(const Field<const T>& phi, int i, int j, int k, T* p) {
+  constexpr int R = SchemeRadius<S>::R;
+#pragma unroll
+  for (int q = -R; q <= R; q++)
+    p[3 + q] = phi.at(i + (DIR == 0 ? q : 0), j + (DIR == 1 ? q : 0), k + (DIR == 2 ? q : 0));
+}
+
+template <typename T, int S>
+__global__ void __launch_bounds__(256)
+    k_hj_pm(Field<T> xp, Field<T> yp, Field<T> zp, Field<T> xm, Field<T> ym, Field<T> zm,
+            Field<const T> phi, Box fb, T ix, T iy, T iz) {
+  LSM_CELL_LOOP_BEGIN(fb)
+  T p[7];
+  gather<T, S, 0>(phi, i, j, k, p);
+  xp.at(i, j, k) = hj_plus<T, S>(p, ix);
+  xm.at(i, j, k) = hj_minus<T, S>(p, ix);
+  gather<T, S, 1>(phi, i, j, k, p);
+  yp.at(i, j, k) = hj_plus<T, S>(p, iy);
+  ym.at(i, j, k) = hj_minus<T, S>(p, iy);
+  gather<T, S, 2>(phi, i, j, k, p);
+  zp.at(i, j, k) = hj_plus<T, S>(p, iz);
+  zm.at(i, j, k) = hj_minus<T, S>(p, iz);
+  LSM_CELL_LOOP_END
+}
+
+template <typename T, int S>
+__global__ void __launch_bounds__(256)
+    k_hj_upwind(Field<T> gx, Field<T> gy, Field<T> gz, Field<const T> phi, Field<const T> vx,
+                Field<const T> vy, Field<const T> vz, Box fb, T ix, T iy, T iz) {
+  LSM_CELL_LOOP_BEGIN(fb)
+  T p[7];
+  gather<T, S, 0>(phi, i, j, k, p);
+  gx.at(i, j, k) = hj_upwind<T, S>(p, vx.at(i, j, k), ix);
+  gather<T, S, 1>(phi, i, j, k, p);
+  gy.at(i, j, k) = hj_upwind<T, S>(p, vy.at(i, j, k), iy);
+  gather<T, S, 2>(phi, i, j, k, p);
+  gz.at(i, j, k) = hj_upwind<T, S>(p, vz.at(i, j, k), iz);
+  LSM_CELL_LOOP_END
+}
+
+// ============================================================================
+// Level set equation RHS  (lsm_level_set_evolution3d.f.in)
+// ============================================================================
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill(Field<T> a, Box b, T v) {
+  LSM_CELL_LOOP_BEGIN(b)
+  a.at(i, j, k) = v;
+  LSM_CELL_LOOP_END
+}
+
+// :113-116
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_add_advection(Field<T> rhs, Field<const T> px, Field<const T> py, Field<const T> pz,
+                    Field<const T> vx, Field<const T> vy, Field<const T> vz, Box fb) {
+  LSM_CELL_LOOP_BEGIN(fb)
+  rhs.at(i, j, k) = rhs.at(i, j, k) - (vx.at(i, j, k) * px.at(i, j, k) + vy.at(i, j, k) * py.at(i, j, k) +
+                                       vz.at(i, j, k) * pz.at(i, j, k));
+  LSM_CELL_LOOP_END
+}
+
+// Godunov |grad phi|^2 for normal speed vn, :206-225
+template <typename T>
+__device__ __forceinline__ T godunov_norm_sq(T vn, T xp, T yp, T zp, T xm, T ym, T zm) {
+  const T z = T(0);
+  if (vn > z) {
+    const T a = fmax(xm, z), b = fmin(xp, z), c = fmax(ym, z), d = fmin(yp, z), e = fmax(zm, z),
+            f = fmin(zp, z);
+    return fmax(a * a, b * b) + fmax(c * c, d * d) + fmax(e * e, f * f);
+  }
+  const T a = fmin(xm, z), b = fmax(xp, z), c = fmin(ym, z), d = fmax(yp, z), e = fmin(zm, z),
+          f = fmax(zp, z);
+  return fmax(a * a, b * b) + fmax(c * c, d * d) + fmax(e * e, f * f);
+}
+
+// :131-242 (field vn) and :248-350 (constant vn: vn_field.p == nullptr)
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_add_normal_vel(Field<T> rhs, Field<const T> xp, Field<const T> yp, Field<const T> zp,
+                     Field<const T> xm, Field<const T> ym, Field<const T> zm, Field<const T> vnf,
+                     T vn_const, Box fb) {
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T vn = vnf.p ? vnf.at(i, j, k) : vn_const;
+  if (fabs(vn) >= Tol<T>::zero) {
+    const T n2 = godunov_norm_sq<T>(vn, xp.at(i, j, k), yp.at(i, j, k), zp.at(i, j, k), xm.at(i, j, k),
+                                    ym.at(i, j, k), zm.at(i, j, k));
+    rhs.at(i, j, k) = rhs.at(i, j, k) - vn * sqrt(n2);
+  }
+  LSM_CELL_LOOP_END
+}
+
+// Godunov selection for the combined external + normal velocity, :630-650
+template <typename T>
+__device__ __forceinline__ T ext_normal_select(T vc, T vn, T dp, T dm) {
+  const T hp = vc + vn * dp, hm = vc + vn * dm;
+  if (hp > T(0)) return (hm > T(0)) ? dm : -(vc / vn);
+  if (hm < T(0)) return dp;
+  return (fabs(hp) > fabs(hm)) ? dp : dm;
+}
+
+// :532-719 over the reference's loop box (see the entry point)
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_add_ext_normal_vel(Field<T> rhs, Field<const T> xp, Field<const T> yp, Field<const T> zp,
+                         Field<const T> xm, Field<const T> ym, Field<const T> zm, Field<const T> vn,
+                         Field<const T> vx, Field<const T> vy, Field<const T> vz, Box lb) {
+  LSM_CELL_LOOP_BEGIN(lb)
+  const T vnc = vn.at(i, j, k), vxc = vx.at(i, j, k), vyc = vy.at(i, j, k), vzc = vz.at(i, j, k);
+  const T px = ext_normal_select<T>(vxc, vnc, xp.at(i, j, k), xm.at(i, j, k));
+  const T py = ext_normal_select<T>(vyc, vnc, yp.at(i, j, k), ym.at(i, j, k));
+  const T pz = ext_normal_select<T>(vzc, vnc, zp.at(i, j, k), zm.at(i, j, k));
+  const T n2 = px * px + py * py + pz * pz;
+  T r = rhs.at(i, j, k) - vxc * px - vyc * py - vzc * pz;
+  if (fabs(vnc) >= Tol<T>::zero && n2 >= Tol<T>::zero) r = r - vnc * sqrt(n2);
+  rhs.at(i, j, k) = r;
+  LSM_CELL_LOOP_END
+}
+
+// ============================================================================
+// TVD Runge-Kutta stages  (lsm_tvd_runge_kutta3d.f)
+// ============================================================================
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_rk(int form, Field<T> out, Field<const T> us, Field<const T> uc, Field<const T> rhs, Box fb, T dt) {
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T ucv = uc.at(i, j, k);
+  const T usv = us.p ? us.at(i, j, k) : ucv;
+  out.at(i, j, k) = rk_combine<T>(form, ucv, usv, rhs.at(i, j, k), dt);
+  LSM_CELL_LOOP_END
+}
+
+// ============================================================================
+// Reinitialization / orthogonalization  (lsm_reinitialization3d.f.in)
+// ============================================================================
+// :48-268
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_reinit_rhs(Field<T> rhs, Field<const T> phi, Field<const T> phi0, Field<const T> xp,
+                 Field<const T> yp, Field<const T> zp, Field<const T> xm, Field<const T> ym,
+                 Field<const T> zm, Box fb, T dx_sq, int use_phi0) {
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T cur = use_phi0 ? phi0.at(i, j, k) : phi.at(i, j, k);
+  T r = T(0);
+  if (fabs(cur) >= Tol<T>::zero) {
+    const T n2 = godunov_star_sq<T>(cur, xp.at(i, j, k), yp.at(i, j, k), zp.at(i, j, k),
+                                    xm.at(i, j, k), ym.at(i, j, k), zm.at(i, j, k));
+    const T sgn = use_phi0 ? cur / sqrt(cur * cur + dx_sq) : cur / sqrt(cur * cur + n2 * dx_sq);
+    r = sgn * (T(1) - sqrt(n2));
+  }
+  rhs.at(i, j, k) = r;
+  LSM_CELL_LOOP_END
+}
+
+// :296-471
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_ortho_rhs(Field<T> rhs, Field<const T> xp, Field<const T> yp, Field<const T> zp,
+                Field<const T> xm, Field<const T> ym, Field<const T> zm, Field<const T> psi,
+                Field<const T> sxp, Field<const T> syp, Field<const T> szp, Field<const T> sxm,
+                Field<const T> sym, Field<const T> szm, Box fb, T dx_sq) {
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T g0 = T(0.5) * (sxp.at(i, j, k) + sxm.at(i, j, k));
+  const T g1 = T(0.5) * (syp.at(i, j, k) + sym.at(i, j, k));
+  const T g2 = T(0.5) * (szp.at(i, j, k) + szm.at(i, j, k));
+  const T ng = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+  const T ps = psi.at(i, j, k);
+  T r = T(0);
+  if (fabs(ps) > Tol<T>::zero && ng > Tol<T>::zero) {
+    const T s = ps / sqrt(ps * ps + dx_sq * (ng * ng));
+    const T z = T(0);
+    T acc = fmax(s * g0, z) * xm.at(i, j, k);
+    acc = acc + fmin(s * g0, z) * xp.at(i, j, k);
+    acc = acc + fmax(s * g1, z) * ym.at(i, j, k);
+    acc = acc + fmin(s * g1, z) * yp.at(i, j, k);
+    acc = acc + fmax(s * g2, z) * zm.at(i, j, k);
+    acc = acc + fmin(s * g2, z) * zp.at(i, j, k);
+    r = T(-1) / ng * acc;
+  }
+  rhs.at(i, j, k) = r;
+  LSM_CELL_LOOP_END
+}
+
+// :477-581 (Russo-Smereka distance estimate near the interface)
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_subcell_distance(Field<T> dist0, Field<const T> phi0, Box fb, T flag) {
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T c = phi0.at(i, j, k);
+  const T z = T(0);
+  const T nb[6] = {phi0.at(i + 1, j, k), phi0.at(i - 1, j, k), phi0.at(i, j + 1, k),
+                   phi0.at(i, j - 1, k), phi0.at(i, j, k + 1), phi0.at(i, j, k - 1)};
+  bool near = false;
+#pragma unroll
+  for (int n = 0; n < 6; n++) near = near || (nb[n] > z && c <= z) || (nb[n] <= z && c > z);
+  if (!near) {
+    dist0.at(i, j, k) = flag;
+  } else {
+    T delta = Tol<T>::zero;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const T pp = nb[2 * d], pm = nb[2 * d + 1];
+      const T d1 = fabs(pp - pm) / T(2), d2 = fabs(pp - c), d3 = fabs(c - pm);
+      delta = fmax(fmax(fmax(d1, d2), d3), delta);
+    }
+    dist0.at(i, j, k) = c / delta;
+  }
+  LSM_CELL_LOOP_END
+}
+
+// :586-754
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_subcell_rhs(Field<T> rhs, Field<const T> phi, Field<const T> phi0, Field<const T> dist0,
+                  Field<const T> xp, Field<const T> yp, Field<const T> zp, Field<const T> xm,
+                  Field<const T> ym, Field<const T> zm, Box fb, T max_dx, T flag) {
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T p0 = phi0.at(i, j, k);
+  const T s0 = (fabs(p0) > Tol<T>::zero) ? copysign(T(1), p0) : T(0);
+  const T d0 = dist0.at(i, j, k);
+  const T cur = phi.at(i, j, k);
+  T r;
+  if (d0 == flag) {
+    if (fabs(cur) > Tol<T>::zero) {
+      const T n2 = godunov_star_sq<T>(cur, xp.at(i, j, k), yp.at(i, j, k), zp.at(i, j, k),
+                                      xm.at(i, j, k), ym.at(i, j, k), zm.at(i, j, k));
+      r = s0 - s0 * sqrt(n2);
+    } else {
+      r = T(0);
+    }
+  } else {
+    r = d0 - s0 * fabs(cur) / max_dx;
+  }
+  rhs.at(i, j, k) = r;
+  LSM_CELL_LOOP_END
+}
+
+// ============================================================================
+// Boundary conditions  (lsm_boundary_conditions3d.f)
+// ============================================================================
+// One face: the threads cover the full ghost box in the two other directions
+// (the reference loops there, :66-67), each writes the ghost cells of its line.
+// mode 0 linear (:46-170), 1 signed linear (:193-330), 2 copy (:352-463)
+template <typename T>
+__global__ void __launch_bounds__(256) k_extrapolate(Field<T> phi, Box gb, Box fb, int loc, int mode) {
+  const int d = loc / 2, hi = loc % 2;
+  const int u = (d + 1) % 3, v = (d + 2) % 3;
+  const int glo[3] = {gb.ilo, gb.jlo, gb.klo}, ghi[3] = {gb.ihi, gb.jhi, gb.khi};
+  const int flo[3] = {fb.ilo, fb.jlo, fb.klo}, fhi[3] = {fb.ihi, fb.jhi, fb.khi};
+  const int pu = glo[u] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int pv = glo[v] + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (pu > ghi[u] || pv > ghi[v]) return;
+  const int bnd = hi ? fhi[d] : flo[d];
+  const int in = hi ? -1 : 1;
+  int idx[3];
+  idx[u] = pu;
+  idx[v] = pv;
+  idx[d] = bnd;
+  const T pb = phi.at(idx[0], idx[1], idx[2]);
+  idx[d] = bnd + in;
+  const T p1 = phi.at(idx[0], idx[1], idx[2]);
+  T slope = T(0);
+  if (mode == 0) slope = pb - p1;
+  if (mode == 1) slope = copysign(T(1), pb) * fabs(pb - p1);
+  const int lo = hi ? bnd + 1 : glo[d], up = hi ? ghi[d] : bnd - 1;
+  for (int t = lo; t <= up; t++) {
+    idx[d] = t;
+    if (mode == 2) {
+      phi.at(idx[0], idx[1], idx[2]) = pb;
+    } else {
+      const T dist = T(hi ? (t - bnd) : (bnd - t));
+      phi.at(idx[0], idx[1], idx[2]) = pb + slope * dist;
+    }
+  }
+}
+
+// ============================================================================
+// Reductions (FP64 accumulation; one partial per block, then one block)
+// ============================================================================
+enum RedOp { RED_MAX = 0, RED_SUM = 1 };
+
+__device__ __forceinline__ double red2(int op, double a, double b) {
+  return op == RED_MAX ? (a > b ? a : b) : a + b;
+}
+
+// Block-wide reduction of one double per thread (256 threads = 4 waves of 64).
+__device__ __forceinline__ double block_reduce(int op, double v, double ident) {
+  __shared__ double s_w[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = red2(op, v, __shfl_xor(v, o, 64));
+  const int t = threadIdx.x + blockDim.x * threadIdx.y;
+  if ((t & 63) == 0) s_w[t >> 6] = v;
+  __syncthreads();
+  double r = ident;
+  if (t == 0) {
+    const int nw = (blockDim.x * blockDim.y + 63) / 64;
+    for (int w = 0; w < nw; w++) r = red2(op, r, s_w[w]);
+  }
+  return r;  // valid in thread 0
+}
+
+// Per-cell terms.  Each returns false when the cell is excluded.
+struct RedArgs {
+  int kind;
+  real_t ix, iy, iz, eps, dV, max_dx_sq, vn_const;
+  int cv_sgn;
+  int ci, cj, ck;  // always-included cell (max-norm initial value)
+};
+enum RedKind {
+  RK_ADV_DT = 1,
+  RK_NORMAL_DT = 2,
+  RK_CONST_NORMAL_DT = 3,
+  RK_MAX_DIFF = 4,
+  RK_VOL_INT_LT = 5,
+  RK_VOL_INT_GT = 6,
+  RK_VOL_REG_LT = 7,
+  RK_VOL_REG_GT = 8
+};
+
+template <typename T>
+__device__ __forceinline__ T h_over_dx(T avn, T xp, T yp, T zp, T xm, T ym, T zm, T mdx2, T ix, T iy,
+                                       T iz) {
+  // lsm_utilities3d.f.in:404-429
+  const T px = fmax(fabs(xp), fabs(xm)), py = fmax(fabs(yp), fabs(ym)), pz = fmax(fabs(zp), fabs(zm));
+  const T ng = sqrt(px * px + py * py + pz * pz + mdx2);
+  return avn / ng * (px * ix + py * iy + pz * iz);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_reduce(RedArgs a, Box ib, Field<const T> f0, Field<const T> f1, Field<const T> f2,
+             Field<const T> f3, Field<const T> f4, Field<const T> f5, Field<const T> f6,
+             Field<const T> cv, double* partials) {
+  const int op = (a.kind >= RK_VOL_INT_LT) ? RED_SUM : RED_MAX;
+  const double ident = (op == RED_MAX) ? -1.0e308 : 0.0;
+  double acc = ident;
+  const int i = ib.ilo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = ib.jlo + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i <= ib.ihi && j <= ib.jhi) {
+    for (int k = ib.klo + (int)blockIdx.z; k <= ib.khi; k += (int)gridDim.z) {
+      bool inc = true;
+      if (cv.p) {
+        const T c = cv.at(i, j, k);
+        inc = (a.cv_sgn > 0) ? (c > T(0)) : (c < T(0));
+        if (a.kind == RK_MAX_DIFF && i == a.ci && j == a.cj && k == a.ck) inc = true;
+      }
+      if (!inc) continue;
+      T v = T(0);
+      switch (a.kind) {
+        case RK_ADV_DT:  // lsm_utilities3d.f.in:306-308
+          v = fabs(f0.at(i, j, k)) * T(a.ix) + fabs(f1.at(i, j, k)) * T(a.iy) + fabs(f2.at(i, j, k)) * T(a.iz);
+          break;
+        case RK_NORMAL_DT:
+          v = h_over_dx<T>(fabs(f6.at(i, j, k)), f0.at(i, j, k), f1.at(i, j, k), f2.at(i, j, k),
+                           f3.at(i, j, k), f4.at(i, j, k), f5.at(i, j, k), T(a.max_dx_sq), T(a.ix),
+                           T(a.iy), T(a.iz));
+          break;
+        case RK_CONST_NORMAL_DT:
+          v = h_over_dx<T>(fabs(T(a.vn_const)), f0.at(i, j, k), f1.at(i, j, k), f2.at(i, j, k),
+                           f3.at(i, j, k), f4.at(i, j, k), f5.at(i, j, k), T(a.max_dx_sq), T(a.ix),
+                           T(a.iy), T(a.iz));
+          break;
+        case RK_MAX_DIFF:  // lsm_utilities3d.f.in:56-64
+          v = fabs(f0.at(i, j, k) - f1.at(i, j, k));
+          break;
+        case RK_VOL_INT_LT: {  // lsm_utilities3d.f.in:628-645
+          const T p = f1.at(i, j, k), e = T(a.eps), x = p / e;
+          if (p < -e) v = f0.at(i, j, k) * T(a.dV);
+          else if (p < e)
+            v = T(0.5) * (T(1) - x - T(0.31830988618379) * sin(T(3.14159265358979323846) * x)) *
+                f0.at(i, j, k) * T(a.dV);
+          else continue;
+          break;
+        }
+        case RK_VOL_INT_GT: {  // lsm_utilities3d.f.in:713-731
+          const T p = f1.at(i, j, k), e = T(a.eps), x = p / e;
+          if (p > e) v = f0.at(i, j, k) * T(a.dV);
+          else if (p > -e)
+            v = T(0.5) * (T(1) + x + T(0.31830988618379) * sin(T(3.14159265358979323846) * x)) *
+                f0.at(i, j, k) * T(a.dV);
+          else continue;
+          break;
+        }
+        case RK_VOL_REG_LT: {  // lsm_geometry3d_fort.f.in:315-325
+          const T p = f0.at(i, j, k), e = T(a.eps), x = p / e;
+          if (p < -e) v = T(a.dV);
+          else if (p < e)
+            v = T(0.5) * (T(1) - x - T(0.31830988618379) * sin(T(3.14159265358979323846) * x)) * T(a.dV);
+          else continue;
+          break;
+        }
+        case RK_VOL_REG_GT: {  // lsm_geometry3d_fort.f.in:405-412
+          const T p = f0.at(i, j, k), e = T(a.eps), x = p / e;
+          if (p > e) v = T(a.dV);
+          else if (p > -e)
+            v = T(0.5) * (T(1) + x + T(0.31830988618379) * sin(T(3.14159265358979323846) * x)) * T(a.dV);
+          else continue;
+          break;
+        }
+      }
+      acc = red2(op, acc, (double)v);
+    }
+  }
+  const double r = block_reduce(op, acc, ident);
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = r;
+}
+
+__global__ void __launch_bounds__(256) k_reduce_final(const double* partials, int64_t n, int op, double* out) {
+  const double ident = (op == RED_MAX) ? -1.0e308 : 0.0;
+  double acc = ident;
+  for (int64_t q = threadIdx.x; q < n; q += blockDim.x) acc = red2(op, acc, partials[q]);
+  const double r = block_reduce(op, acc, ident);
+  if (threadIdx.x == 0) out[0] = r;
+}
+
+}  // namespace lsm
+
+// ============================================================================
+// Host side: launch helpers
+// ============================================================================
+namespace {
+
+using namespace lsm;
+typedef real_t R;
+
+template <typename K, typename... Args>
+void launch_cells(Call& c, const char* kname, const Box& b, K kernel, Args... args) {
+  if (!c.ok() || b.cells() == 0) return;
+  hipLaunchKernelGGL(kernel, cell_grid(b), dim3(kBX, kBY, 1), 0, c.stream(), args...);
+  c.launched(kname);
+}
+
+Field<const R> no_field() {
+  Field<const R> f;
+  f.p = nullptr;
+  f.ilo = f.jlo = f.klo = 0;
+  f.sy = f.sz = 0;
+  return f;
+}
+
+// Runs one reduction; returns the combined value (FP64).
+double run_reduce(Call& c, RedArgs a, const Box& ib, Field<const R> f0, Field<const R> f1,
+                  Field<const R> f2, Field<const R> f3, Field<const R> f4, Field<const R> f5,
+                  Field<const R> f6, Field<const R> cv) {
+  const int op = (a.kind >= RK_VOL_INT_LT) ? RED_SUM : RED_MAX;
+  double ident = (op == RED_MAX) ? -1.0e308 : 0.0;
+  if (!c.ok() || ib.cells() == 0) return ident;
+  dim3 g = cell_grid(ib);
+  if (g.z > 64) g.z = 64;
+  const int64_t nb = (int64_t)g.x * g.y * g.z;
+  double* scratch = reinterpret_cast<double*>(device_scratch((size_t)(nb + 1) * sizeof(double)));
+  if (!c.ok()) return ident;
+  hipLaunchKernelGGL(k_reduce<R>, g, dim3(kBX, kBY, 1), 0, c.stream(), a, ib, f0, f1, f2, f3, f4, f5, f6,
+                     cv, scratch + 1);
+  c.launched("k_reduce");
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream(), scratch + 1, nb, op, scratch);
+  c.launched("k_reduce_final");
+  double out = ident;
+  check_hip(hipMemcpyAsync(&out, scratch, sizeof(double), hipMemcpyDeviceToHost, c.stream()),
+            "hipMemcpyAsync(reduction result)");
+  check_hip(hipStreamSynchronize(c.stream()), "hipStreamSynchronize");
+  return out;
+}
+
+}  // namespace
+
+// ============================================================================
+// Entry points (extern "C", reference prototypes; see include/lsm3d_amd.h)
+// ============================================================================
+#define BOXP(p)                                                                                \
+  const int *ilo_##p, const int *ihi_##p, const int *jlo_##p, const int *jhi_##p, const int *klo_##p, \
+      const int *khi_##p
+#define BOX(p) lsm::box_of(ilo_##p, ihi_##p, jlo_##p, jhi_##p, klo_##p, khi_##p)
+
+namespace {
+
+template <int S>
+void hj_pm_entry(const char* fn, R* xp, R* yp, R* zp, Box gp, R* xm, R* ym, R* zm, Box gm, const R* phi,
+                 Box gphi, Box fb, R dx, R dy, R dz) {
+  Call c(fn);
+  constexpr int Rr = SchemeRadius<S>::R;
+  require_inside(fn, "grad_phi_plus", gp, fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_minus", gm, fb, 0, 0, 0);
+  require_inside(fn, "phi", gphi, fb, Rr, Rr, Rr);
+  auto fxp = c.out(xp, gp), fyp = c.out(yp, gp), fzp = c.out(zp, gp);
+  auto fxm = c.out(xm, gm), fym = c.out(ym, gm), fzm = c.out(zm, gm);
+  auto fphi = c.in(phi, gphi);
+  launch_cells(c, "k_hj_pm", fb, k_hj_pm<R, S>, fxp, fyp, fzp, fxm, fym, fzm, fphi, fb, R(1) / dx,
+               R(1) / dy, R(1) / dz);
+  c.finish();
+}
+
+template <int S>
+void hj_upwind_entry(const char* fn, R* gx, R* gy, R* gz, Box gg, const R* phi, Box gphi, const R* vx,
+                     const R* vy, const R* vz, Box gv, Box fb, R dx, R dy, R dz) {
+  Call c(fn);
+  constexpr int Rr = SchemeRadius<S>::R;
+  require_inside(fn, "grad_phi", gg, fb, 0, 0, 0);
+  require_inside(fn, "phi", gphi, fb, Rr, Rr, Rr);
+  require_inside(fn, "vel", gv, fb, 0, 0, 0);
+  auto fx = c.out(gx, gg), fy = c.out(gy, gg), fz = c.out(gz, gg);
+  auto fphi = c.in(phi, gphi);
+  auto fvx = c.in(vx, gv), fvy = c.in(vy, gv), fvz = c.in(vz, gv);
+  launch_cells(c, "k_hj_upwind", fb, k_hj_upwind<R, S>, fx, fy, fz, fphi, fvx, fvy, fvz, fb, R(1) / dx,
+               R(1) / dy, R(1) / dz);
+  c.finish();
+}
+
+void rk_entry(const char* fn, int form, R* out, Box go, const R* us, Box gs, const R* uc, Box gc,
+              const R* rhs, Box gr, Box fb, R dt) {
+  Call c(fn);
+  require_inside(fn, "u_out", go, fb, 0, 0, 0);
+  if (us) require_inside(fn, "u_stage", gs, fb, 0, 0, 0);
+  require_inside(fn, "u_cur", gc, fb, 0, 0, 0);
+  require_inside(fn, "rhs", gr, fb, 0, 0, 0);
+  auto fo = c.out(out, go);
+  Field<const R> fs = us ? c.in(us, gs) : no_field();
+  auto fc = c.in(uc, gc);
+  auto fr = c.in(rhs, gr);
+  launch_cells(c, "k_rk", fb, k_rk<R>, form, fo, fs, fc, fr, fb, dt);
+  c.finish();
+}
+
+void bc_entry(const char* fn, R* phi, Box gb, Box fb, int loc, int mode) {
+  Call c(fn);
+  if (loc < 0 || loc > 5) {  // the reference sets no ghost cells (:34-36)
+    c.finish();
+    return;
+  }
+  const int d = loc / 2;
+  Box need = fb;  // the two cells next to the face must be interior cells
+  if (d == 0 && fb.nx() < 2) raise_error(std::string(fn) + ": fill box needs 2 cells in x");
+  if (d == 1 && fb.ny() < 2) raise_error(std::string(fn) + ": fill box needs 2 cells in y");
+  if (d == 2 && fb.nz() < 2) raise_error(std::string(fn) + ": fill box needs 2 cells in z");
+  require_inside(fn, "phi", gb, need, 0, 0, 0);
+  auto f = c.out(phi, gb);
+  if (c.ok()) {
+    const int u = (d + 1) % 3, v = (d + 2) % 3;
+    const int n[3] = {gb.nx(), gb.ny(), gb.nz()};
+    dim3 g((unsigned)((n[u] + 63) / 64), (unsigned)((n[v] + 3) / 4), 1);
+    hipLaunchKernelGGL(k_extrapolate<R>, g, dim3(64, 4, 1), 0, c.stream(), f, gb, fb, loc, mode);
+    c.launched("k_extrapolate");
+  }
+  c.finish();
+}
+
+R max_of3(R a, R b, R c) {
+  R m = a;
+  if (b > m) m = b;
+  if (c > m) m = c;
+  return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- spatial derivatives (lsm_spatial_derivatives3d.h) -----------------------
+void lsm3dhjeno1_(R* xp, R* yp, R* zp, BOXP(gp), R* xm, R* ym, R* zm, BOXP(gm), const R* phi, BOXP(phi),
+                  R* D1, BOXP(d1), BOXP(fb), const R* dx, const R* dy, const R* dz) {
+  (void)D1;
+  hj_pm_entry<ENO1>("LSM3D_HJ_ENO1", xp, yp, zp, BOX(gp), xm, ym, zm, BOX(gm), phi, BOX(phi), BOX(fb),
+                    *dx, *dy, *dz);
+}
+void lsm3dhjeno2_(R* xp, R* yp, R* zp, BOXP(gp), R* xm, R* ym, R* zm, BOXP(gm), const R* phi, BOXP(phi),
+                  R* D1, BOXP(d1), R* D2, BOXP(d2), BOXP(fb), const R* dx, const R* dy, const R* dz) {
+  (void)D1;
+  (void)D2;
+  hj_pm_entry<ENO2>("LSM3D_HJ_ENO2", xp, yp, zp, BOX(gp), xm, ym, zm, BOX(gm), phi, BOX(phi), BOX(fb),
+                    *dx, *dy, *dz);
+}
+void lsm3dhjeno3_(R* xp, R* yp, R* zp, BOXP(gp), R* xm, R* ym, R* zm, BOXP(gm), const R* phi, BOXP(phi),
+                  R* D1, BOXP(d1), R* D2, BOXP(d2), R* D3, BOXP(d3), BOXP(fb), const R* dx, const R* dy,
+                  const R* dz) {
+  (void)D1;
+  (void)D2;
+  (void)D3;
+  hj_pm_entry<ENO3>("LSM3D_HJ_ENO3", xp, yp, zp, BOX(gp), xm, ym, zm, BOX(gm), phi, BOX(phi), BOX(fb),
+                    *dx, *dy, *dz);
+}
+void lsm3dhjweno5_(R* xp, R* yp, R* zp, BOXP(gp), R* xm, R* ym, R* zm, BOXP(gm), const R* phi, BOXP(phi),
+                   R* D1, BOXP(d1), BOXP(fb), const R* dx, const R* dy, const R* dz) {
+  (void)D1;
+  hj_pm_entry<WENO5>("LSM3D_HJ_WENO5", xp, yp, zp, BOX(gp), xm, ym, zm, BOX(gm), phi, BOX(phi), BOX(fb),
+                     *dx, *dy, *dz);
+}
+void lsm3dupwindhjeno1_(R* gx, R* gy, R* gz, BOXP(g), const R* phi, BOXP(phi), const R* vx, const R* vy,
+                        const R* vz, BOXP(v), R* D1, BOXP(d1), BOXP(fb), const R* dx, const R* dy,
+                        const R* dz) {
+  (void)D1;
+  hj_upwind_entry<ENO1>("LSM3D_UPWIND_HJ_ENO1", gx, gy, gz, BOX(g), phi, BOX(phi), vx, vy, vz, BOX(v),
+                        BOX(fb), *dx, *dy, *dz);
+}
+void lsm3dupwindhjeno2_(R* gx, R* gy, R* gz, BOXP(g), const R* phi, BOXP(phi), const R* vx, const R* vy,
+                        const R* vz, BOXP(v), R* D1, BOXP(d1), R* D2, BOXP(d2), BOXP(fb), const R* dx,
+                        const R* dy, const R* dz) {
+  (void)D1;
+  (void)D2;
+  hj_upwind_entry<ENO2>("LSM3D_UPWIND_HJ_ENO2", gx, gy, gz, BOX(g), phi, BOX(phi), vx, vy, vz, BOX(v),
+                        BOX(fb), *dx, *dy, *dz);
+}
+void lsm3dupwindhjeno3_(R* gx, R* gy, R* gz, BOXP(g), const R* phi, BOXP(phi), const R* vx, const R* vy,
+                        const R* vz, BOXP(v), R* D1, BOXP(d1), R* D2, BOXP(d2), R* D3, BOXP(d3),
+                        BOXP(fb), const R* dx, const R* dy, const R* dz) {
+  (void)D1;
+  (void)D2;
+  (void)D3;
+  hj_upwind_entry<ENO3>("LSM3D_UPWIND_HJ_ENO3", gx, gy, gz, BOX(g), phi, BOX(phi), vx, vy, vz, BOX(v),
+                        BOX(fb), *dx, *dy, *dz);
+}
+void lsm3dupwindhjweno5_(R* gx, R* gy, R* gz, BOXP(g), const R* phi, BOXP(phi), const R* vx,
+                         const R* vy, const R* vz, BOXP(v), R* D1, BOXP(d1), BOXP(fb), const R* dx,
+                         const R* dy, const R* dz) {
+  (void)D1;
+  hj_upwind_entry<WENO5>("LSM3D_UPWIND_HJ_WENO5", gx, gy, gz, BOX(g), phi, BOX(phi), vx, vy, vz, BOX(v),
+                         BOX(fb), *dx, *dy, *dz);
+}
+
+// ---- level set equation RHS (lsm_level_set_evolution3d.h) --------------------
+void lsm3dzerooutlevelseteqnrhs_(R* rhs, BOXP(r)) {
+  const char* fn = "LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS";
+  Call c(fn);
+  Box gr = BOX(r);
+  auto f = c.out(rhs, gr);
+  launch_cells(c, "k_fill", gr, k_fill<R>, f, gr, R(0));
+  c.finish();
+}
+
+void lsm3daddadvectiontermtolserhs_(R* rhs, BOXP(r), const R* px, const R* py, const R* pz, BOXP(g),
+                                    const R* vx, const R* vy, const R* vz, BOXP(v), BOXP(fb)) {
+  const char* fn = "LSM3D_ADD_ADVECTION_TERM_TO_LSE_RHS";
+  Call c(fn);
+  Box fb = BOX(fb);
+  require_inside(fn, "lse_rhs", BOX(r), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi", BOX(g), fb, 0, 0, 0);
+  require_inside(fn, "vel", BOX(v), fb, 0, 0, 0);
+  auto fr = c.out(rhs, BOX(r));
+  auto fx = c.in(px, BOX(g)), fy = c.in(py, BOX(g)), fz = c.in(pz, BOX(g));
+  auto ux = c.in(vx, BOX(v)), uy = c.in(vy, BOX(v)), uz = c.in(vz, BOX(v));
+  launch_cells(c, "k_add_advection", fb, k_add_advection<R>, fr, fx, fy, fz, ux, uy, uz, fb);
+  c.finish();
+}
+
+void lsm3daddnormalveltermtolserhs_(R* rhs, BOXP(r), const R* xp, const R* yp, const R* zp, BOXP(gp),
+                                    const R* xm, const R* ym, const R* zm, BOXP(gm), const R* vn,
+                                    BOXP(v), BOXP(fb)) {
+  const char* fn = "LSM3D_ADD_NORMAL_VEL_TERM_TO_LSE_RHS";
+  Call c(fn);
+  Box fb = BOX(fb);
+  require_inside(fn, "lse_rhs", BOX(r), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_plus", BOX(gp), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_minus", BOX(gm), fb, 0, 0, 0);
+  require_inside(fn, "vel_n", BOX(v), fb, 0, 0, 0);
+  auto fr = c.out(rhs, BOX(r));
+  auto a = c.in(xp, BOX(gp)), b = c.in(yp, BOX(gp)), d = c.in(zp, BOX(gp));
+  auto e = c.in(xm, BOX(gm)), f = c.in(ym, BOX(gm)), g = c.in(zm, BOX(gm));
+  auto n = c.in(vn, BOX(v));
+  launch_cells(c, "k_add_normal_vel", fb, k_add_normal_vel<R>, fr, a, b, d, e, f, g, n, R(0), fb);
+  c.finish();
+}
+
+void lsm3daddconstnormalveltermtolserhs_(R* rhs, BOXP(r), const R* xp, const R* yp, const R* zp,
+                                         BOXP(gp), const R* xm, const R* ym, const R* zm, BOXP(gm),
+                                         const R* vn, BOXP(fb)) {
+  const char* fn = "LSM3D_ADD_CONST_NORMAL_VEL_TERM_TO_LSE_RHS";
+  Call c(fn);
+  Box fb = BOX(fb);
+  require_inside(fn, "lse_rhs", BOX(r), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_plus", BOX(gp), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_minus", BOX(gm), fb, 0, 0, 0);
+  if (std::fabs(*vn) >= Tol<R>::zero) {  // :300 the whole loop is skipped otherwise
+    auto fr = c.out(rhs, BOX(r));
+    auto a = c.in(xp, BOX(gp)), b = c.in(yp, BOX(gp)), d = c.in(zp, BOX(gp));
+    auto e = c.in(xm, BOX(gm)), f = c.in(ym, BOX(gm)), g = c.in(zm, BOX(gm));
+    launch_cells(c, "k_add_normal_vel", fb, k_add_normal_vel<R>, fr, a, b, d, e, f, g, no_field(), *vn, fb);
+  }
+  c.finish();
+}
+
+void lsm3daddexternalandnormalveltermtolserhs_(R* rhs, BOXP(r), const R* xp, const R* yp, const R* zp,
+                                               BOXP(gp), const R* xm, const R* ym, const R* zm,
+                                               BOXP(gm), const R* vn, const R* vx, const R* vy,
+                                               const R* vz, BOXP(v), BOXP(fb)) {
+  // The reference's loops run k = klo_rhs_fb..khi_rhs_gb, j = jlo_rhs_fb..jhi_rhs_gb,
+  // i = ilo_rhs_gb..ihi_rhs_gb (lsm_level_set_evolution3d.f.in:620-622); the same box
+  // is used here so results are identical, and every array must cover it.
+  const char* fn = "LSM3D_ADD_EXTERNAL_AND_NORMAL_VEL_TERM_TO_LSE_RHS";
+  Call c(fn);
+  Box gr = BOX(r), fb = BOX(fb);
+  Box lb{gr.ilo, gr.ihi, fb.jlo, gr.jhi, fb.klo, gr.khi};
+  require_inside(fn, "lse_rhs", gr, lb, 0, 0, 0);
+  require_inside(fn, "grad_phi_plus", BOX(gp), lb, 0, 0, 0);
+  require_inside(fn, "grad_phi_minus", BOX(gm), lb, 0, 0, 0);
+  require_inside(fn, "vel", BOX(v), lb, 0, 0, 0);
+  auto fr = c.out(rhs, gr);
+  auto a = c.in(xp, BOX(gp)), b = c.in(yp, BOX(gp)), d = c.in(zp, BOX(gp));
+  auto e = c.in(xm, BOX(gm)), f = c.in(ym, BOX(gm)), g = c.in(zm, BOX(gm));
+  auto n = c.in(vn, BOX(v)), ux = c.in(vx, BOX(v)), uy = c.in(vy, BOX(v)), uz = c.in(vz, BOX(v));
+  launch_cells(c, "k_add_ext_normal_vel", lb, k_add_ext_normal_vel<R>, fr, a, b, d, e, f, g, n, ux, uy, uz,
+               lb);
+  c.finish();
+}
+
+// ---- TVD Runge-Kutta (lsm_tvd_runge_kutta3d.h) --------------------------------
+void lsm3drk1step_(R* un, BOXP(un), const R* uc, BOXP(uc), const R* rhs, BOXP(r), BOXP(fb), const R* dt) {
+  rk_entry("LSM3D_RK1_STEP", RK_EULER, un, BOX(un), nullptr, Box{}, uc, BOX(uc), rhs, BOX(r), BOX(fb), *dt);
+}
+void lsm3dtvdrk2stage1_(R* u1, BOXP(u1), const R* uc, BOXP(uc), const R* rhs, BOXP(r), BOXP(fb),
+                        const R* dt) {
+  rk_entry("LSM3D_TVD_RK2_STAGE1", RK_EULER, u1, BOX(u1), nullptr, Box{}, uc, BOX(uc), rhs, BOX(r), BOX(fb),
+           *dt);
+}
+void lsm3dtvdrk2stage2_(R* un, BOXP(un), const R* u1, BOXP(u1), const R* uc, BOXP(uc), const R* rhs,
+                        BOXP(r), BOXP(fb), const R* dt) {
+  rk_entry("LSM3D_TVD_RK2_STAGE2", RK2_STAGE2, un, BOX(un), u1, BOX(u1), uc, BOX(uc), rhs, BOX(r), BOX(fb),
+           *dt);
+}
+void lsm3dtvdrk3stage1_(R* u1, BOXP(u1), const R* uc, BOXP(uc), const R* rhs, BOXP(r), BOXP(fb),
+                        const R* dt) {
+  rk_entry("LSM3D_TVD_RK3_STAGE1", RK_EULER, u1, BOX(u1), nullptr, Box{}, uc, BOX(uc), rhs, BOX(r), BOX(fb),
+           *dt);
+}
+void lsm3dtvdrk3stage2_(R* u2, BOXP(u2), const R* u1, BOXP(u1), const R* uc, BOXP(uc), const R* rhs,
+                        BOXP(r), BOXP(fb), const R* dt) {
+  rk_entry("LSM3D_TVD_RK3_STAGE2", RK3_STAGE2, u2, BOX(u2), u1, BOX(u1), uc, BOX(uc), rhs, BOX(r), BOX(fb),
+           *dt);
+}
+void lsm3dtvdrk3stage3_(R* un, BOXP(un), const R* u2, BOXP(u2), const R* uc, BOXP(uc), const R* rhs,
+                        BOXP(r), BOXP(fb), const R* dt) {
+  rk_entry("LSM3D_TVD_RK3_STAGE3", RK3_STAGE3, un, BOX(un), u2, BOX(u2), uc, BOX(uc), rhs, BOX(r), BOX(fb),
+           *dt);
+}
+
+// ---- reinitialization (lsm_reinitialization3d.h) ----------------------------------
+void lsm3dcomputereinitializationeqnrhs_(R* rhs, BOXP(r), const R* phi, BOXP(phi), const R* phi0,
+                                         BOXP(phi0), const R* xp, const R* yp, const R* zp, BOXP(gp),
+                                         const R* xm, const R* ym, const R* zm, BOXP(gm), BOXP(fb),
+                                         const R* dx, const R* dy, const R* dz, const int* use_phi0) {
+  const char* fn = "LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS";
+  Call c(fn);
+  Box fb = BOX(fb);
+  const int u0 = (*use_phi0 == 1);
+  require_inside(fn, "reinit_rhs", BOX(r), fb, 0, 0, 0);
+  if (u0) require_inside(fn, "phi0", BOX(phi0), fb, 0, 0, 0);
+  else require_inside(fn, "phi", BOX(phi), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_plus", BOX(gp), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_minus", BOX(gm), fb, 0, 0, 0);
+  auto fr = c.out(rhs, BOX(r));
+  Field<const R> fphi = u0 ? no_field() : c.in(phi, BOX(phi));
+  Field<const R> fphi0 = u0 ? c.in(phi0, BOX(phi0)) : no_field();
+  auto a = c.in(xp, BOX(gp)), b = c.in(yp, BOX(gp)), d = c.in(zp, BOX(gp));
+  auto e = c.in(xm, BOX(gm)), f = c.in(ym, BOX(gm)), g = c.in(zm, BOX(gm));
+  R dx_sq = max_of3(*dx, *dy, *dz);  // :140-141
+  dx_sq = dx_sq * dx_sq;
+  launch_cells(c, "k_reinit_rhs", fb, k_reinit_rhs<R>, fr, fphi, fphi0, a, b, d, e, f, g, fb, dx_sq, u0);
+  c.finish();
+}
+
+void lsm3dcomputeorthogonalizationeqnrhs_(R* rhs, BOXP(r), const R* xp, const R* yp, const R* zp, BOXP(gp),
+                                          const R* xm, const R* ym, const R* zm, BOXP(gm), const R* psi,
+                                          BOXP(psi), const R* sxp, const R* syp, const R* szp, BOXP(sp),
+                                          const R* sxm, const R* sym, const R* szm, BOXP(sm), BOXP(fb),
+                                          const R* dx, const R* dy, const R* dz) {
+  const char* fn = "LSM3D_COMPUTE_ORTHOGONALIZATION_EQN_RHS";
+  Call c(fn);
+  Box fb = BOX(fb);
+  require_inside(fn, "ortho_rhs", BOX(r), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_plus", BOX(gp), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_minus", BOX(gm), fb, 0, 0, 0);
+  require_inside(fn, "psi", BOX(psi), fb, 0, 0, 0);
+  require_inside(fn, "grad_psi_plus", BOX(sp), fb, 0, 0, 0);
+  require_inside(fn, "grad_psi_minus", BOX(sm), fb, 0, 0, 0);
+  auto fr = c.out(rhs, BOX(r));
+  auto a = c.in(xp, BOX(gp)), b = c.in(yp, BOX(gp)), d = c.in(zp, BOX(gp));
+  auto e = c.in(xm, BOX(gm)), f = c.in(ym, BOX(gm)), g = c.in(zm, BOX(gm));
+  auto fpsi = c.in(psi, BOX(psi));
+  auto s1 = c.in(sxp, BOX(sp)), s2 = c.in(syp, BOX(sp)), s3 = c.in(szp, BOX(sp));
+  auto s4 = c.in(sxm, BOX(sm)), s5 = c.in(sym, BOX(sm)), s6 = c.in(szm, BOX(sm));
+  R dx_sq = max_of3(*dx, *dy, *dz);
+  dx_sq = dx_sq * dx_sq;
+  launch_cells(c, "k_ortho_rhs", fb, k_ortho_rhs<R>, fr, a, b, d, e, f, g, fpsi, s1, s2, s3, s4, s5, s6, fb,
+               dx_sq);
+  c.finish();
+}
+
+void lsm3dcomputedistanceforsubcellfix_(R* dist0, const R* phi0, BOXP(p0), BOXP(fb), const R* dx,
+                                        const R* dy, const R* dz) {
+  const char* fn = "LSM3D_COMPUTE_DISTANCE_FOR_SUBCELL_FIX";
+  Call c(fn);
+  Box fb = BOX(fb), g0 = BOX(p0);
+  require_inside(fn, "phi0", g0, fb, 1, 1, 1);
+  auto fd = c.out(dist0, g0);
+  auto fp = c.in(phi0, g0);
+  const R max_dx = max_of3(*dx, *dy, *dz);
+  const R flag = R(-1000.0 * max_dx);
+  launch_cells(c, "k_subcell_distance", fb, k_subcell_distance<R>, fd, fp, fb, flag);
+  c.finish();
+}
+
+void lsm3dcomputereinitializationeqnrhssubcellfixorder1_(R* rhs, BOXP(r), const R* phi, BOXP(phi),
+                                                         const R* phi0, const R* dist0, BOXP(p0),
+                                                         const R* xp, const R* yp, const R* zp, BOXP(gp),
+                                                         const R* xm, const R* ym, const R* zm, BOXP(gm),
+                                                         BOXP(fb), const R* dx, const R* dy, const R* dz) {
+  const char* fn = "LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS_SUBCELL_FIX_ORDER1";
+  Call c(fn);
+  Box fb = BOX(fb);
+  require_inside(fn, "reinit_rhs", BOX(r), fb, 0, 0, 0);
+  require_inside(fn, "phi", BOX(phi), fb, 0, 0, 0);
+  require_inside(fn, "phi0", BOX(p0), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_plus", BOX(gp), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_minus", BOX(gm), fb, 0, 0, 0);
+  auto fr = c.out(rhs, BOX(r));
+  auto fphi = c.in(phi, BOX(phi));
+  auto fp0 = c.in(phi0, BOX(p0)), fd0 = c.in(dist0, BOX(p0));
+  auto a = c.in(xp, BOX(gp)), b = c.in(yp, BOX(gp)), d = c.in(zp, BOX(gp));
+  auto e = c.in(xm, BOX(gm)), f = c.in(ym, BOX(gm)), g = c.in(zm, BOX(gm));
+  const R max_dx = max_of3(*dx, *dy, *dz);
+  const R flag = R(-1000.0 * max_dx);
+  launch_cells(c, "k_subcell_rhs", fb, k_subcell_rhs<R>, fr, fphi, fp0, fd0, a, b, d, e, f, g, fb, max_dx, flag);
+  c.finish();
+}
+
+// ---- reductions (lsm_utilities3d.h, lsm_geometry3d.h) ----------------------------
+void lsm3dmaxnormdiff_(R* out, const R* f1, BOXP(f1), const R* f2, BOXP(f2), BOXP(ib)) {
+  const char* fn = "LSM3D_MAX_NORM_DIFF";
+  Call c(fn);
+  Box ib = BOX(ib);
+  require_inside(fn, "field1", BOX(f1), ib, 0, 0, 0);
+  require_inside(fn, "field2", BOX(f2), ib, 0, 0, 0);
+  auto a = c.in(f1, BOX(f1)), b = c.in(f2, BOX(f2));
+  RedArgs ra{};
+  ra.kind = RK_MAX_DIFF;
+  double m = run_reduce(c, ra, ib, a, b, no_field(), no_field(), no_field(), no_field(), no_field(), no_field());
+  c.finish();
+  if (!lsm::error_pending()) *out = (ib.cells() > 0) ? R(m) : R(0);
+}
+
+void lsm3dmaxnormdiffcontrolvolume_(R* out, const R* f1, BOXP(f1), const R* f2, BOXP(f2), const R* cv,
+                                    BOXP(cv), const int* cv_sgn, BOXP(ib)) {
+  const char* fn = "LSM3D_MAX_NORM_DIFF_CONTROL_VOLUME";
+  Call c(fn);
+  Box ib = BOX(ib);
+  require_inside(fn, "field1", BOX(f1), ib, 0, 0, 0);
+  require_inside(fn, "field2", BOX(f2), ib, 0, 0, 0);
+  require_inside(fn, "control_vol", BOX(cv), ib, 0, 0, 0);
+  auto a = c.in(f1, BOX(f1)), b = c.in(f2, BOX(f2)), fc = c.in(cv, BOX(cv));
+  RedArgs ra{};
+  ra.kind = RK_MAX_DIFF;
+  ra.cv_sgn = *cv_sgn;
+  ra.ci = ib.ilo;
+  ra.cj = ib.jlo;
+  ra.ck = ib.klo;
+  double m = run_reduce(c, ra, ib, a, b, no_field(), no_field(), no_field(), no_field(), no_field(), fc);
+  c.finish();
+  if (!lsm::error_pending()) *out = (ib.cells() > 0) ? R(m) : R(0);
+}
+
+static void adv_dt_entry(const char* fn, R* dt, const R* vx, const R* vy, const R* vz, Box gv, const R* cv,
+                         Box gcv, int cv_sgn, Box ib, R dx, R dy, R dz, R cfl) {
+  Call c(fn);
+  require_inside(fn, "vel", gv, ib, 0, 0, 0);
+  if (cv) require_inside(fn, "control_vol", gcv, ib, 0, 0, 0);
+  auto a = c.in(vx, gv), b = c.in(vy, gv), d = c.in(vz, gv);
+  Field<const R> fc = cv ? c.in(cv, gcv) : no_field();
+  RedArgs ra{};
+  ra.kind = RK_ADV_DT;
+  ra.ix = R(1) / dx;
+  ra.iy = R(1) / dy;
+  ra.iz = R(1) / dz;
+  ra.cv_sgn = cv_sgn;
+  ra.ci = ra.cj = ra.ck = INT32_MIN;
+  double m = run_reduce(c, ra, ib, a, b, d, no_field(), no_field(), no_field(), no_field(), fc);
+  c.finish();
+  R mx = R(-1.0);  // lsm_utilities3d.f.in:297
+  if (m > (double)mx) mx = R(m);
+  if (!lsm::error_pending()) *dt = cfl / (mx + Tol<R>::tiny);  // :323
+}
+
+void lsm3dcomputestableadvectiondt_(R* dt, const R* vx, const R* vy, const R* vz, BOXP(v), BOXP(ib),
+                                    const R* dx, const R* dy, const R* dz, const R* cfl) {
+  adv_dt_entry("LSM3D_COMPUTE_STABLE_ADVECTION_DT", dt, vx, vy, vz, BOX(v), nullptr, Box{}, 0, BOX(ib), *dx,
+               *dy, *dz, *cfl);
+}
+
+void lsm3dcomputestableadvectiondtcontrolvolume_(R* dt, const R* vx, const R* vy, const R* vz, BOXP(v),
+                                                 const R* cv, BOXP(cv), const int* cv_sgn, BOXP(ib),
+                                                 const R* dx, const R* dy, const R* dz, const R* cfl) {
+  adv_dt_entry("LSM3D_COMPUTE_STABLE_ADVECTION_DT_CONTROL_VOLUME", dt, vx, vy, vz, BOX(v), cv, BOX(cv), *cv_sgn,
+               BOX(ib), *dx, *dy, *dz, *cfl);
+}
+
+static void normal_dt_entry(const char* fn, R* dt, const R* vn, Box gv, R vn_const, const R* xp, const R* yp,
+                            const R* zp, Box gp, const R* xm, const R* ym, const R* zm, Box gm, Box ib, R dx,
+                            R dy, R dz, R cfl) {
+  Call c(fn);
+  if (vn) require_inside(fn, "vel_n", gv, ib, 0, 0, 0);
+  require_inside(fn, "grad_phi_plus", gp, ib, 0, 0, 0);
+  require_inside(fn, "grad_phi_minus", gm, ib, 0, 0, 0);
+  Field<const R> fvn = vn ? c.in(vn, gv) : no_field();
+  auto a = c.in(xp, gp), b = c.in(yp, gp), d = c.in(zp, gp);
+  auto e = c.in(xm, gm), f = c.in(ym, gm), g = c.in(zm, gm);
+  RedArgs ra{};
+  ra.kind = vn ? RK_NORMAL_DT : RK_CONST_NORMAL_DT;
+  ra.ix = R(1) / dx;
+  ra.iy = R(1) / dy;
+  ra.iz = R(1) / dz;
+  R mdx = max_of3(dx, dy, dz);
+  ra.max_dx_sq = mdx * mdx;
+  ra.vn_const = vn_const;
+  ra.ci = ra.cj = ra.ck = INT32_MIN;
+  double m = run_reduce(c, ra, ib, a, b, d, e, f, g, fvn, no_field());
+  c.finish();
+  R mx = R(-1.0);
+  if (m > (double)mx) mx = R(m);
+  if (!lsm::error_pending()) *dt = cfl / (mx + Tol<R>::tiny);
+}
+
+void lsm3dcomputestablenormalveldt_(R* dt, const R* vn, BOXP(v), const R* xp, const R* yp, const R* zp,
+                                    BOXP(gp), const R* xm, const R* ym, const R* zm, BOXP(gm), BOXP(ib),
+                                    const R* dx, const R* dy, const R* dz, const R* cfl) {
+  normal_dt_entry("LSM3D_COMPUTE_STABLE_NORMAL_VEL_DT", dt, vn, BOX(v), R(0), xp, yp, zp, BOX(gp), xm, ym, zm,
+                  BOX(gm), BOX(ib), *dx, *dy, *dz, *cfl);
+}
+
+void lsm3dcomputestableconstnormalveldt_(R* dt, const R* vn, const R* xp, const R* yp, const R* zp, BOXP(gp),
+                                         const R* xm, const R* ym, const R* zm, BOXP(gm), BOXP(ib),
+                                         const R* dx, const R* dy, const R* dz, const R* cfl) {
+  normal_dt_entry("LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT", dt, nullptr, Box{}, *vn, xp, yp, zp, BOX(gp), xm,
+                  ym, zm, BOX(gm), BOX(ib), *dx, *dy, *dz, *cfl);
+}
+
+static void volume_entry(const char* fn, int kind, R* out, const R* F, Box gF, const R* phi, Box gphi,
+                         Box ib, R dx, R dy, R dz, R eps) {
+  Call c(fn);
+  if (F) require_inside(fn, "F", gF, ib, 0, 0, 0);
+  require_inside(fn, "phi", gphi, ib, 0, 0, 0);
+  Field<const R> fF = F ? c.in(F, gF) : no_field();
+  auto fp = c.in(phi, gphi);
+  RedArgs ra{};
+  ra.kind = kind;
+  ra.eps = eps;
+  ra.dV = dx * dy * dz;  // dV = dx * dy * dz in LSMLIB_REAL
+  ra.ci = ra.cj = ra.ck = INT32_MIN;
+  double s;
+  if (F) s = run_reduce(c, ra, ib, fF, fp, no_field(), no_field(), no_field(), no_field(), no_field(), no_field());
+  else s = run_reduce(c, ra, ib, fp, no_field(), no_field(), no_field(), no_field(), no_field(), no_field(), no_field());
+  c.finish();
+  if (!lsm::error_pending()) *out = R(s);
+}
+
+void lsm3dvolumeintegralphilessthanzero_(R* out, const R* F, BOXP(F), const R* phi, BOXP(phi), BOXP(ib),
+                                         const R* dx, const R* dy, const R* dz, const R* eps) {
+  volume_entry("LSM3D_VOLUME_INTEGRAL_PHI_LESS_THAN_ZERO", RK_VOL_INT_LT, out, F, BOX(F), phi, BOX(phi), BOX(ib),
+               *dx, *dy, *dz, *eps);
+}
+void lsm3dvolumeintegralphigreaterthanzero_(R* out, const R* F, BOXP(F), const R* phi, BOXP(phi), BOXP(ib),
+                                            const R* dx, const R* dy, const R* dz, const R* eps) {
+  volume_entry("LSM3D_VOLUME_INTEGRAL_PHI_GREATER_THAN_ZERO", RK_VOL_INT_GT, out, F, BOX(F), phi, BOX(phi),
+               BOX(ib), *dx, *dy, *dz, *eps);
+}
+void lsm3dvolumeregionphilessthanzero_(R* out, const R* phi, BOXP(phi), BOXP(ib), const R* dx, const R* dy,
+                                       const R* dz, const R* eps) {
+  volume_entry("LSM3D_VOLUME_REGION_PHI_LESS_THAN_ZERO", RK_VOL_REG_LT, out, nullptr, Box{}, phi, BOX(phi),
+               BOX(ib), *dx, *dy, *dz, *eps);
+}
+void lsm3dvolumeregionphigreaterthanzero_(R* out, const R* phi, BOXP(phi), BOXP(ib), const R* dx, const R* dy,
+                                          const R* dz, const R* eps) {
+  volume_entry("LSM3D_VOLUME_REGION_PHI_GREATER_THAN_ZERO", RK_VOL_REG_GT, out, nullptr, Box{}, phi, BOX(phi),
+               BOX(ib), *dx, *dy, *dz, *eps);
+}
+
+// ---- boundary conditions (lsm_boundary_conditions3d.h) ---------------------------
+void lsm3dlinearextrapolation_(R* phi, BOXP(g), BOXP(fb), const int* loc) {
+  bc_entry("LSM3D_LINEAR_EXTRAPOLATION", phi, BOX(g), BOX(fb), *loc, 0);
+}
+void lsm3dsignedlinearextrapolation_(R* phi, BOXP(g), BOXP(fb), const int* loc) {
+  bc_entry("LSM3D_SIGNED_LINEAR_EXTRAPOLATION", phi, BOX(g), BOX(fb), *loc, 1);
+}
+void lsm3dcopyextrapolation_(R* phi, BOXP(g), BOXP(fb), const int* loc) {
+  bc_entry("LSM3D_COPY_EXTRAPOLATION", phi, BOX(g), BOX(fb), *loc, 2);
+}
+void lsm3dhomogeneousneumanneno1_(R* phi, BOXP(g), BOXP(fb), const int* loc) {
+  bc_entry("LSM3D_HOMOGENEOUS_NEUMANN_ENO1", phi, BOX(g), BOX(fb), *loc, 2);
+}
+void lsm3dhomogeneousneumanneno2_(R* phi, BOXP(g), BOXP(fb), const int* loc) {
+  bc_entry("LSM3D_HOMOGENEOUS_NEUMANN_ENO2", phi, BOX(g), BOX(fb), *loc, 2);
+}
+void lsm3dhomogeneousneumanneno3_(R* phi, BOXP(g), BOXP(fb), const int* loc) {
+  bc_entry("LSM3D_HOMOGENEOUS_NEUMANN_ENO3", phi, BOX(g), BOX(fb), *loc, 2);
+}
+void lsm3dhomogeneousneumannweno5_(R* phi, BOXP(g), BOXP(fb), const int* loc) {
+  bc_entry("LSM3D_HOMOGENEOUS_NEUMANN_WENO5", phi, BOX(g), BOX(fb), *loc, 2);
+}
+
+}  // extern "C"

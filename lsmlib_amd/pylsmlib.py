@@ -1,0 +1,191 @@
+"""Torch-facing solvers over the gfx950 library (device-resident fields).
+
+Fields are torch tensors of shape (nk, nj, ni) over a ghost box (i fastest,
+the reference's column-major layout), on the GPU.  The drivers here sequence
+the fused Part-3 entry points of include/lsm3d_amd.h the way LSMLIB's
+LevelSetFunctionIntegrator and ReinitializationAlgorithm sequence the toolbox
+routines (LevelSetFunctionIntegrator.cc:1319-1510,
+ReinitializationAlgorithm.cc:177-362): ghost fill, one fused stage, repeat.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import torch
+
+from ._native import toolbox
+
+RK1, TVDRK2_STAGE2, TVDRK3_STAGE2, TVDRK3_STAGE3 = 1, 22, 32, 33
+
+# boundary-condition codes (BoundaryConditionModule.h:125-130) plus periodic
+BC_NONE, BC_NEUMANN, BC_LINEAR, BC_SIGNED_LINEAR, BC_PERIODIC = 0, 1, 2, 3, 5
+
+_DTYPES = {"double": torch.float64, "float": torch.float32}
+
+
+def real_of(t):
+    if t.dtype == torch.float64:
+        return "double"
+    if t.dtype == torch.float32:
+        return "float"
+    raise TypeError(f"unsupported dtype {t.dtype}")
+
+
+def box_c(b):
+    return (ctypes.c_int * 6)(*[int(v) for v in b])
+
+
+def grow(box, g):
+    return (box[0] - g, box[1] + g, box[2] - g, box[3] + g, box[4] - g, box[5] + g)
+
+
+@dataclass
+class Grid:
+    """Uniform grid: n interior cells per axis on [lower, upper], `ghost`
+    ghost layers, fill box indices 0..n-1 (the serial package's convention,
+    lsm_grid.c:788-811)."""
+    n: tuple
+    lower: tuple
+    upper: tuple
+    ghost: int = 3
+
+    @property
+    def fb(self):
+        return (0, self.n[0] - 1, 0, self.n[1] - 1, 0, self.n[2] - 1)
+
+    @property
+    def gb(self):
+        return grow(self.fb, self.ghost)
+
+    @property
+    def shape(self):
+        g = self.ghost
+        return (self.n[2] + 2 * g, self.n[1] + 2 * g, self.n[0] + 2 * g)
+
+    @property
+    def dx(self):
+        return tuple((self.upper[d] - self.lower[d]) / self.n[d] for d in range(3))
+
+    @property
+    def interior_cells(self):
+        return self.n[0] * self.n[1] * self.n[2]
+
+    def centers(self, dtype=torch.float64, device="cpu"):
+        """Cell-centre coordinates X, Y, Z over the ghost box, each (nk, nj, ni)."""
+        g = self.ghost
+        ax = [self.lower[d] + (torch.arange(-g, self.n[d] + g, dtype=torch.float64) + 0.5) * self.dx[d]
+              for d in range(3)]
+        Z, Y, X = torch.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        return X.to(dtype=dtype, device=device), Y.to(dtype=dtype, device=device), Z.to(dtype=dtype, device=device)
+
+    def interior(self, a):
+        g = self.ghost
+        return a[g:a.shape[0] - g, g:a.shape[1] - g, g:a.shape[2] - g]
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("fused stages need device tensors")
+    if not t.is_contiguous():
+        raise ValueError("tensors must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Stepper:
+    """Device-resident TVD Runge-Kutta drivers for one grid and precision.
+
+    bc: 6 boundary-condition codes (x_lo, x_hi, y_lo, y_hi, z_lo, z_hi).  When
+    every face is periodic the stages write their own periodic ghost images
+    and no separate fill kernel runs.
+    """
+
+    def __init__(self, grid, real="double", bc=(BC_PERIODIC,) * 6, device="cuda"):
+        self.grid = grid
+        self.real = real
+        self.dtype = _DTYPES[real]
+        self.lib = toolbox(real)
+        self.bc = tuple(int(b) for b in bc)
+        self.device = torch.device(device)
+        self._gb = box_c(grid.gb)
+        self._fb = box_c(grid.fb)
+        self._bc = box_c(self.bc)
+        self.all_periodic = all(b == BC_PERIODIC for b in self.bc)
+        self.periodic_mask = 7 if self.all_periodic else 0
+        self.sync(False)
+
+    def sync(self, flag):
+        self.lib.set_sync(flag)
+        self.lib.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def empty(self):
+        return torch.empty(self.grid.shape, dtype=self.dtype, device=self.device)
+
+    def fill_ghosts(self, phi):
+        self.lib.fused("lsm3d_amd_fill_ghosts", _ptr(phi), self._gb, self._fb, self._bc)
+
+    def _prefill(self, phi):
+        if not self.all_periodic:
+            self.fill_ghosts(phi)
+
+    # -- advection ---------------------------------------------------------
+    def advect_stage(self, form, out, ps, pc, vel, dt):
+        dx, dy, dz = self.grid.dx
+        if isinstance(vel, (tuple, list)) and len(vel) == 3 and torch.is_tensor(vel[0]):
+            self.lib.fused("lsm3d_amd_advect_stage", form, _ptr(out), self._gb, _ptr(ps), self._gb,
+                           _ptr(pc), self._gb, _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), self._gb,
+                           self._fb, dx, dy, dz, dt, self.periodic_mask)
+        else:
+            u, v, w = (float(c) for c in vel)
+            self.lib.fused("lsm3d_amd_advect_const_stage", form, _ptr(out), self._gb, _ptr(ps), self._gb,
+                           _ptr(pc), self._gb, u, v, w, self._fb, dx, dy, dz, dt, self.periodic_mask)
+
+    def advect_rk3_step(self, phi, s1, s2, vel, dt):
+        """phi <- one WENO5 / TVD-RK3 step of phi_t + V.grad(phi) = 0 (in place).
+        phi's ghost cells must be valid on entry; they are valid on exit."""
+        self.advect_stage(RK1, s1, phi, None, vel, dt)
+        self._prefill(s1)
+        self.advect_stage(TVDRK3_STAGE2, s2, s1, phi, vel, dt)
+        self._prefill(s2)
+        self.advect_stage(TVDRK3_STAGE3, phi, s2, phi, vel, dt)
+        self._prefill(phi)
+
+    def advect_rk2_step(self, phi, s1, vel, dt):
+        self.advect_stage(RK1, s1, phi, None, vel, dt)
+        self._prefill(s1)
+        self.advect_stage(TVDRK2_STAGE2, phi, s1, phi, vel, dt)
+        self._prefill(phi)
+
+    # -- reinitialization --------------------------------------------------
+    def reinit_stage(self, form, out, ps, pc, phi0, use_phi0, dt):
+        dx, dy, dz = self.grid.dx
+        self.lib.fused("lsm3d_amd_reinit_stage", form, _ptr(out), self._gb, _ptr(ps), self._gb, _ptr(pc),
+                       self._gb, _ptr(phi0) if phi0 is not None else None, self._gb, int(use_phi0),
+                       self._fb, dx, dy, dz, dt, self.periodic_mask)
+
+    def reinit_rk3_iteration(self, phi, s1, s2, dt, phi0=None, use_phi0=False):
+        """One TVD-RK3 pseudo-time iteration of the reinitialization equation."""
+        if use_phi0 and phi0 is None:
+            raise ValueError("use_phi0 needs phi0")
+        self.reinit_stage(RK1, s1, phi, None, phi0, use_phi0, dt)
+        self._prefill(s1)
+        self.reinit_stage(TVDRK3_STAGE2, s2, s1, phi, phi0, use_phi0, dt)
+        self._prefill(s2)
+        self.reinit_stage(TVDRK3_STAGE3, phi, s2, phi, phi0, use_phi0, dt)
+        self._prefill(phi)
+
+
+def stable_advection_dt(vel, grid, cfl, real=None):
+    """LSM3D_COMPUTE_STABLE_ADVECTION_DT over the fill box (device tensors)."""
+    real = real or real_of(vel[0])
+    lib = toolbox(real)
+    dx, dy, dz = grid.dx
+    return lib.lsm3dcomputestableadvectiondt_(vel[0], vel[1], vel[2], grid.gb, grid.fb, dx, dy, dz, cfl)
+
+
+def volume_phi_less_than_zero(phi, grid, eps, real=None):
+    """LSM3D_VOLUME_REGION_PHI_LESS_THAN_ZERO (the interface mass)."""
+    real = real or real_of(phi)
+    lib = toolbox(real)
+    dx, dy, dz = grid.dx
+    return lib.lsm3dvolumeregionphilessthanzero_(phi, grid.gb, grid.fb, dx, dy, dz, eps)

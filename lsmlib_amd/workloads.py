@@ -1,0 +1,71 @@
+"""Synthetic level-set workloads (seeded), built with torch on any device.
+
+  rotating_sphere(n)       sphere r=0.35 centred (0.35,0,0) on [-1,1]^3, rigid
+                           rotation u=(-y,x,0), signed-linear ghost fill
+                           (advection parity case, CFL 0.5)
+  wavy_sphere_reinit(n)    phi0 = (r-0.5)(1+0.4 sin3x sin3y sin3z) on [-1,1]^3
+                           (reinitialization case)
+  sphere_union_taylor_green(n, n_spheres=64, seed=5)
+                           periodic [0,1]^3, phi0 = min of sphere signed
+                           distances (radii U[0.02,0.1]), Taylor-Green velocity
+                           u=sin2pix cos2piy cos2piz, v=-cos2pix sin2piy cos2piz,
+                           w=0 (large advection case)
+"""
+import math
+
+import numpy as np
+import torch
+
+from .pylsmlib import BC_PERIODIC, BC_SIGNED_LINEAR, Grid
+
+
+def rotating_sphere(n=128, dtype=torch.float64, device="cpu"):
+    grid = Grid((n, n, n), (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), 3)
+    X, Y, Z = grid.centers(torch.float64, device)
+    phi = torch.sqrt((X - 0.35) ** 2 + Y ** 2 + Z ** 2) - 0.35
+    vel = (-Y, X, torch.zeros_like(X))
+    return (grid, phi.to(dtype).contiguous(), tuple(v.to(dtype).contiguous() for v in vel),
+            (BC_SIGNED_LINEAR,) * 6)
+
+
+def wavy_sphere_reinit(n=256, dtype=torch.float64, device="cpu"):
+    grid = Grid((n, n, n), (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), 3)
+    X, Y, Z = grid.centers(torch.float64, device)
+    r = torch.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    phi = (r - 0.5) * (1 + 0.4 * torch.sin(3 * X) * torch.sin(3 * Y) * torch.sin(3 * Z))
+    return grid, phi.to(dtype).contiguous(), (BC_SIGNED_LINEAR,) * 6
+
+
+def taylor_green(grid, dtype=torch.float64, device="cpu"):
+    X, Y, Z = grid.centers(torch.float64, device)
+    tp = 2 * math.pi
+    u = torch.sin(tp * X) * torch.cos(tp * Y) * torch.cos(tp * Z)
+    v = -torch.cos(tp * X) * torch.sin(tp * Y) * torch.cos(tp * Z)
+    w = torch.zeros_like(u)
+    return tuple(a.to(dtype).contiguous() for a in (u, v, w))
+
+
+def sphere_union(grid, n_spheres=64, seed=5, rmin=0.02, rmax=0.1, dtype=torch.float64, device="cpu",
+                 periodic=True):
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0.0, 1.0, size=(n_spheres, 3))
+    radii = rng.uniform(rmin, rmax, size=n_spheres)
+    X, Y, Z = grid.centers(torch.float64, device)
+    phi = torch.full_like(X, float("inf"))
+    for c, r in zip(centres, radii):
+        d = []
+        for A, cc in ((X, c[0]), (Y, c[1]), (Z, c[2])):
+            t = A - cc
+            if periodic:
+                t = t - torch.round(t)  # minimum image on the unit torus
+            d.append(t)
+        phi = torch.minimum(phi, torch.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2) - r)
+        del d
+    return phi.to(dtype).contiguous()
+
+
+def sphere_union_taylor_green(n=512, n_spheres=64, seed=5, dtype=torch.float64, device="cpu"):
+    grid = Grid((n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 3)
+    phi = sphere_union(grid, n_spheres, seed, dtype=dtype, device=device)
+    vel = taylor_green(grid, dtype, device)
+    return grid, phi, vel, (BC_PERIODIC,) * 6

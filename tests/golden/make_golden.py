@@ -1,0 +1,84 @@
+"""Writes the golden vectors of every LSM3D_* routine from the flang build of
+the reference toolbox (oracle/_ref, built from /root/reference by
+oracle/Makefile).
+
+Inputs are regenerated deterministically by tests/lsm_cases.py (seed 0, fill
+box 6x5x4 with uneven ghost boxes); each fixture stores a SHA-256 of the
+inputs, the output values that changed (flat index + value per array
+argument) and the returned scalars.  Output: tests/golden/toolbox_<real>.npz.
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parents[1]
+sys.path.insert(0, str(REPO))
+sys.path.insert(0, str(REPO / "tests"))
+
+from lsm_cases import make_case  # noqa: E402
+from lsmlib_amd.signatures import SIGNATURES  # noqa: E402
+from oracle.oracle import ref_lib  # noqa: E402
+
+GOLDEN_FB = (-2, 3, 1, 5, 3, 6)
+
+
+def golden_cases(name, real):
+    out = [make_case(name, real, 0, fb=GOLDEN_FB)]
+    if "extrapolation" in name or "neumann" in name:
+        for loc in range(1, 6):
+            c = make_case(name, real, 0, fb=GOLDEN_FB)
+            c.args[-1] = loc
+            out.append(c)
+    if "controlvolume" in name:
+        c = make_case(name, real, 0, fb=GOLDEN_FB)
+        idx = [i for i, a in enumerate(c.args) if isinstance(a, int) and not isinstance(a, bool)]
+        c.args[idx[0]] = -1
+        out.append(c)
+    if name == "lsm3dcomputereinitializationeqnrhs_":
+        c = make_case(name, real, 0, fb=GOLDEN_FB)
+        c.args[-1] = 1
+        out.append(c)
+    return out
+
+
+def input_digest(case):
+    h = hashlib.sha256()
+    for a in case.args:
+        if isinstance(a, np.ndarray):
+            h.update(a.tobytes())
+        else:
+            h.update(repr(a).encode())
+    return h.hexdigest()
+
+
+def main():
+    for real in ("double", "float"):
+        lib = ref_lib(real)
+        if lib is None:
+            raise SystemExit("oracle/_ref is not built: make -C oracle ref")
+        store = {}
+        for name in sorted(SIGNATURES):
+            for ci, case in enumerate(golden_cases(name, real)):
+                key = f"{name}#{ci}"
+                before = case.copy()
+                ret = lib.call(name, *case.args)
+                store[f"{key}/digest"] = np.frombuffer(bytes.fromhex(input_digest(before)), dtype=np.uint8)
+                for (i, b), (_, a) in zip(before.arrays(), case.arrays()):
+                    changed = np.flatnonzero(a.ravel() != b.ravel()).astype(np.int32)
+                    if changed.size:
+                        store[f"{key}/arg{i}/idx"] = changed
+                        store[f"{key}/arg{i}/val"] = a.ravel()[changed]
+                if ret is not None:
+                    store[f"{key}/ret"] = np.atleast_1d(np.asarray(ret, dtype=np.float64))
+        path = HERE / f"toolbox_{real}.npz"
+        np.savez_compressed(path, **store)
+        print(path, path.stat().st_size, "bytes")
+
+
+if __name__ == "__main__":
+    main()

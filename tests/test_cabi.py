@@ -1,0 +1,36 @@
+"""The C-ABI libraries load and export every symbol include/*.h declares
+(no compute calls: these run without a GPU)."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from lsmlib_amd import SIGNATURES, lib_path
+
+REPO = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    names = set()
+    for h in (REPO / "include").glob("*.h"):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:void|int|const char\s*\*|void\s*\*)\s*\*?\s*(\w+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_every_toolbox_symbol():
+    decl = declared_symbols()
+    missing = set(SIGNATURES) - decl
+    assert not missing, f"not declared in include/: {sorted(missing)}"
+
+
+@pytest.mark.parametrize("real", ["double", "float"])
+def test_library_exports_declared_symbols(real):
+    p = lib_path(real)
+    assert p.exists(), f"{p} not built (python -m lsmlib_amd.build)"
+    lib = ctypes.CDLL(str(p))
+    missing = [n for n in sorted(declared_symbols()) if not hasattr(lib, n)]
+    assert not missing, f"{p.name} lacks {missing}"
