@@ -43,12 +43,17 @@ def parse():
     ap.add_argument("--tiling", type=str, default="auto",
                     help="cfg,kz of the fused stage tiling, or 'auto' (tuned table)")
     ap.add_argument("--fp32-only", action="store_true", help="skip the FP64 leg (profiling runs)")
+    ap.add_argument("--ghosts", choices=["synth", "images"], default="images",
+                    help="synth: stages synthesise periodic ghost values; images: stages write and read "
+                         "stored periodic ghost images")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-patch and reinitialization measurements")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
 
 
-def run_gpu(real, n, steps, warmup, tiling, phi_init=None):
+def run_gpu(real, n, steps, warmup, tiling, ghosts="synth"):
     import torch
     from lsmlib_amd import toolbox
     from lsmlib_amd.pylsmlib import RK1, TVDRK3_STAGE2, TVDRK3_STAGE3, Stepper, stable_advection_dt
@@ -58,13 +63,13 @@ def run_gpu(real, n, steps, warmup, tiling, phi_init=None):
     dev = torch.device("cuda:0")
     dtype = torch.float32 if real == "float" else torch.float64
     grid, phi64, vel, bc = sphere_union_taylor_green(n, dtype=torch.float64, device=dev)
-    phi = phi64.to(dtype).contiguous()
+    phi = phi64.to(dtype).clone()
     del phi64
     vel = tuple(v.to(dtype).contiguous() for v in vel)
     lib = toolbox(real)
     cfg, kz = choose_tiling(real, n) if tiling == "auto" else tuple(int(x) for x in tiling.split(","))
     lib.lib.lsm3d_amd_set_stage_tiling(cfg, kz)
-    st = Stepper(grid, real, bc)
+    st = Stepper(grid, real, bc, ghosts=ghosts)
     st.fill_ghosts(phi)
     dt = stable_advection_dt(vel, grid, 0.5)
     s1, s2 = st.empty(), st.empty()
@@ -109,6 +114,70 @@ def run_gpu(real, n, steps, warmup, tiling, phi_init=None):
         "algorithmic_bytes_per_launch": tot_bytes / len(events),
     }
     return res, grid, phi
+
+
+def run_batched(real="float", npatch=512, steps=20, reps=2):
+    """512 uneven patches (extents U{16..96}, seed 3, dx 1/128, 3 ghosts,
+    per-patch constant velocity), `steps` WENO5/TVD-RK3 steps captured into
+    one hipGraph; replays are timed."""
+    import torch
+    from lsmlib_amd.patches import PatchLevel, random_patch_level
+    desc = random_patch_level(npatch=npatch, lo=16, hi=96, seed=3, dx=1.0 / 128)
+    lvl = PatchLevel(desc, real, tiling=(0, 32))
+    phi = lvl.initial_phi()
+    s1, s2 = lvl.empty_pool(), lvl.empty_pool()
+    dt = lvl.stable_dt(0.5)
+    lvl.capture_rk3(steps, phi, s1, s2, dt)
+    lvl.replay()  # warm-up
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        lvl.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    word = 4 if real == "float" else 8
+    return {"real": real, "patches": npatch, "cells": lvl.cells, "steps_per_graph": steps,
+            "graph_ms": ms, "value": lvl.cells * steps / (ms * 1e-3), "unit": "cell-updates/s",
+            "stage_launches_per_graph": 3 * steps, "tiles": int(lvl.tiles), "workgroups": int(lvl.workgroups),
+            "achieved_gbs": 8 * word * lvl.cells * steps / (ms * 1e-3) / 1e9,
+            "words_per_cell_step": 8}
+
+
+def run_reinit(n=256, iters=50):
+    """Reinitialization, 256^3 on [-1,1]^3, phi0 = (r-0.5)(1+0.4 sin3x sin3y sin3z),
+    smoothed sign from the frozen phi0 (use_phi0_for_sgn = 1), HJ-WENO5 +
+    TVD-RK3, pseudo-dt 0.5 dx, 50 iterations; FP64 and FP32, FP32 error vs FP64."""
+    import torch
+    from lsmlib_amd.pylsmlib import Stepper
+    from lsmlib_amd.workloads import wavy_sphere_reinit
+    dev = torch.device("cuda:0")
+    grid, phi64, bc = wavy_sphere_reinit(n, torch.float64, dev)
+    out = {}
+    res = {}
+    for real, dtype in (("double", torch.float64), ("float", torch.float32)):
+        st = Stepper(grid, real, bc)
+        phi = phi64.to(dtype).clone()  # a copy even when dtype is already float64
+        phi0 = phi.clone()
+        s1, s2 = st.empty(), st.empty()
+        dt = 0.5 * grid.dx[0]
+        st.reinit_rk3_iteration(phi, s1, s2, dt, phi0=phi0, use_phi0=True)
+        phi.copy_(phi0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            st.reinit_rk3_iteration(phi, s1, s2, dt, phi0=phi0, use_phi0=True)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res[real] = phi
+        out[real] = {"ms_per_iteration": 1e3 * el / iters,
+                     "value": grid.interior_cells * iters / el, "unit": "cell-updates/s"}
+    err = fp32_error(grid, res["float"], res["double"])
+    out["fp32_error_vs_fp64"] = err
+    out["config"] = f"{n}^3 wavy sphere, use_phi0_for_sgn=1, {iters} TVD-RK3 iterations, dt=0.5dx"
+    return out
 
 
 def fp32_error(grid, phi32, phi64):
@@ -176,13 +245,13 @@ def main():
         print("bench.py needs a visible GPU", file=sys.stderr)
         sys.exit(1)
 
-    r32, grid, phi32 = run_gpu("float", a.n, a.steps, a.warmup, a.tiling)
+    r32, grid, phi32 = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts)
     print(f"# fp32: {r32['value']:.4e} cell-updates/s, {r32['ms_per_step']:.3f} ms/step, "
           f"kernel {r32['kernel_ms_avg']:.3f} ms, {r32['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     r64 = None
     err = None
     if not a.fp32_only:
-        r64, _, phi64 = run_gpu("double", a.n, a.steps, a.warmup, a.tiling)
+        r64, _, phi64 = run_gpu("double", a.n, a.steps, a.warmup, a.tiling, a.ghosts)
         print(f"# fp64: {r64['value']:.4e} cell-updates/s, {r64['ms_per_step']:.3f} ms/step",
               file=sys.stderr, flush=True)
         err = fp32_error(grid, phi32, phi64)
@@ -193,6 +262,17 @@ def main():
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(a.cpu_seconds)
         print(f"# cpu: {cpu['value']:.4e} cell-updates/s ({cpu['kind']})", file=sys.stderr, flush=True)
+
+    extras = {}
+    if not a.no_extras:
+        extras["batched_patches_fp32"] = run_batched("float")
+        print(f"# batched: {extras['batched_patches_fp32']['value']:.4e} cell-updates/s", file=sys.stderr,
+              flush=True)
+        torch.cuda.empty_cache()
+        extras["reinit_256"] = run_reinit()
+        print(f"# reinit: fp64 {extras['reinit_256']['double']['ms_per_iteration']:.2f} ms/it, fp32 "
+              f"{extras['reinit_256']['float']['ms_per_iteration']:.2f} ms/it", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
 
     line = {
         "metric": METRIC,
@@ -209,7 +289,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"weno5_tvdrk3_advection_{a.n}cubed_periodic_64spheres_taylor_green",
                    "grid": [a.n, a.n, a.n], "ghost": 3, "cfl": 0.5, "dt": r32["dt"],
-                   "tiling": r32["tiling"], "velocity": "stored Taylor-Green arrays"},
+                   "tiling": r32["tiling"], "velocity": "stored Taylor-Green arrays",
+                   "ghosts": a.ghosts},
         "fp64": None if r64 is None else {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
                                           "kernel_ms_avg": r64["kernel_ms_avg"],
                                           "achieved_gbs": r64["achieved_gbs"], "tiling": r64["tiling"]},
@@ -221,6 +302,7 @@ def main():
                      "kernel_ms_avg": r32["kernel_ms_avg"],
                      "algorithmic_bytes_per_launch": r32["algorithmic_bytes_per_launch"]},
         "cpu_baseline": cpu,
+        "extras": extras or None,
     }
     print(json.dumps(line), flush=True)
 

@@ -346,9 +346,16 @@ int lsm_amd_real_size(void);
  *   33 phi_cur/3 + 2/3*(phi_stage + dt*rhs)   (TVD-RK3 stage 3)
  * phi_cur may be NULL for form 1 (then phi_cur = phi_stage).  phi_out may
  * alias phi_cur (it is read only at the cell being written) but not
- * phi_stage.  periodic_mask bit d (x=1, y=2, z=4): the stage also writes
- * phi_out's periodic ghost images along axis d (3-cell ghost layer), so the
- * next stage needs no ghost fill. */
+ * phi_stage.
+ * bc[6] (x_lo, x_hi, y_lo, y_hi, z_lo, z_hi), or NULL: how the stage obtains
+ * the ghost values of phi_stage it reads.  0 = read the stored ghost cells;
+ * 1 copy (homogeneous Neumann), 2 linear, 3 signed linear extrapolation, 5
+ * periodic = synthesised from the fill box in the kernel, with exactly the
+ * values LSM3D_*_EXTRAPOLATION (or a periodic fill) would have stored, so no
+ * ghost fill is needed between stages.
+ * periodic_mask bit d (x=1, y=2, z=4): the stage also writes phi_out's
+ * periodic ghost images along axis d (3-cell ghost layer), for callers that
+ * read phi_out's ghost cells afterwards. */
 #define LSM_AMD_RK1 1
 #define LSM_AMD_TVDRK2_STAGE2 22
 #define LSM_AMD_TVDRK3_STAGE2 32
@@ -364,14 +371,14 @@ void lsm3d_amd_advect_stage(int rk_form, LSMLIB_REAL *phi_out, const int out_gb[
                             const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y,
                             const LSMLIB_REAL *vel_z, const int vel_gb[6], const int fb[6],
                             LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt,
-                            int periodic_mask);
+                            const int bc[6], int periodic_mask);
 /* As lsm3d_amd_advect_stage with a constant velocity (per-patch velocity). */
 void lsm3d_amd_advect_const_stage(int rk_form, LSMLIB_REAL *phi_out, const int out_gb[6],
                                   const LSMLIB_REAL *phi_stage, const int stage_gb[6],
                                   const LSMLIB_REAL *phi_cur, const int cur_gb[6],
                                   LSMLIB_REAL vel_x, LSMLIB_REAL vel_y, LSMLIB_REAL vel_z,
                                   const int fb[6], LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz,
-                                  LSMLIB_REAL dt, int periodic_mask);
+                                  LSMLIB_REAL dt, const int bc[6], int periodic_mask);
 /* One stage of the reinitialization equation phi_t = S(phi)(1 - |grad phi|):
  * HJ-WENO5 plus/minus derivatives of phi_stage, the Godunov right-hand side of
  * LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS (smoothed sign from phi0 when
@@ -381,7 +388,7 @@ void lsm3d_amd_reinit_stage(int rk_form, LSMLIB_REAL *phi_out, const int out_gb[
                             const LSMLIB_REAL *phi_cur, const int cur_gb[6],
                             const LSMLIB_REAL *phi0, const int phi0_gb[6], int use_phi0_for_sgn,
                             const int fb[6], LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz,
-                            LSMLIB_REAL dt, int periodic_mask);
+                            LSMLIB_REAL dt, const int bc[6], int periodic_mask);
 /* Fills the ghost cells of phi outside fb, face by face in the order x_lo,
  * x_hi, y_lo, y_hi, z_lo, z_hi (each face over the full ghost box in the
  * other directions, as the LSM3D_*_EXTRAPOLATION routines do).  bc[face]:
@@ -394,6 +401,37 @@ void lsm3d_amd_fill_ghosts(LSMLIB_REAL *phi, const int gb[6], const int fb[6], c
 void lsm3d_amd_set_stage_tiling(int cfg, int kz);
 int lsm3d_amd_stage_tiling_count(void);
 void lsm3d_amd_stage_tiling_shape(int cfg, int *tx, int *ty);
+
+/* --- patch lists: a level of uneven patches advanced together ---------
+ * Patch q has fill box fbs[6q..6q+5] and ghost box gbs[6q..6q+5] (at least 3
+ * ghost cells per side), stored at element offset offsets[q] of every field
+ * pool (pool_elems elements each), and the constant velocity
+ * vel[3q..3q+2].  The plan keeps device copies of the descriptors; one
+ * persistent launch per Runge-Kutta stage covers all patches (the reference
+ * loops over patches, LevelSetMethodToolbox.cc:2446-2592).  Ghost values
+ * follow bc[6] as for the fused stages (extrapolation or stored; not
+ * periodic). */
+typedef struct lsm3d_amd_patch_plan lsm3d_amd_patch_plan;
+lsm3d_amd_patch_plan *lsm3d_amd_patch_plan_create(int npatch, const int *fbs, const int *gbs,
+                                                  const long long *offsets, long long pool_elems,
+                                                  const LSMLIB_REAL *vel, int tiling_cfg, int kz);
+void lsm3d_amd_patch_plan_destroy(lsm3d_amd_patch_plan *plan);
+long long lsm3d_amd_patch_plan_cells(const lsm3d_amd_patch_plan *plan);
+long long lsm3d_amd_patch_plan_tiles(const lsm3d_amd_patch_plan *plan);
+int lsm3d_amd_patch_plan_workgroups(const lsm3d_amd_patch_plan *plan);
+/* One Runge-Kutta stage of phi_t + V.grad(phi) = 0 on every patch. */
+void lsm3d_amd_patch_plan_stage(lsm3d_amd_patch_plan *plan, int rk_form, LSMLIB_REAL *out_pool,
+                                const LSMLIB_REAL *stage_pool, const LSMLIB_REAL *cur_pool,
+                                LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt,
+                                const int bc[6]);
+/* Captures nsteps TVD-RK3 steps (phi advanced in place through s1, s2)
+ * into a hipGraph held by the plan; replay launches it on the library
+ * stream. */
+void lsm3d_amd_patch_plan_capture_rk3(lsm3d_amd_patch_plan *plan, int nsteps, LSMLIB_REAL *phi_pool,
+                                      LSMLIB_REAL *s1_pool, LSMLIB_REAL *s2_pool, LSMLIB_REAL dx,
+                                      LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt,
+                                      const int bc[6]);
+void lsm3d_amd_patch_plan_replay(lsm3d_amd_patch_plan *plan);
 
 #pragma GCC visibility pop
 #ifdef __cplusplus

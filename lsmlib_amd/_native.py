@@ -57,22 +57,37 @@ class ToolboxLibrary(F77Library):
         """argtypes of the fused device API (plain values, int[6] boxes)."""
         L, R = self.lib, self._ct
         P, I, B = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)
-        L.lsm3d_amd_advect_stage.argtypes = [I, P, B, P, B, P, B, P, P, P, B, B, R, R, R, R, I]
-        L.lsm3d_amd_advect_const_stage.argtypes = [I, P, B, P, B, P, B, R, R, R, B, R, R, R, R, I]
-        L.lsm3d_amd_reinit_stage.argtypes = [I, P, B, P, B, P, B, P, B, I, B, R, R, R, R, I]
+        L.lsm3d_amd_advect_stage.argtypes = [I, P, B, P, B, P, B, P, P, P, B, B, R, R, R, R, B, I]
+        L.lsm3d_amd_advect_const_stage.argtypes = [I, P, B, P, B, P, B, R, R, R, B, R, R, R, R, B, I]
+        L.lsm3d_amd_reinit_stage.argtypes = [I, P, B, P, B, P, B, P, B, I, B, R, R, R, R, B, I]
         L.lsm3d_amd_fill_ghosts.argtypes = [P, B, B, B]
         L.lsm3d_amd_set_stage_tiling.argtypes = [I, I]
         L.lsm3d_amd_stage_tiling_count.restype = I
         L.lsm3d_amd_stage_tiling_shape.argtypes = [I, B, B]
+        LL = ctypes.POINTER(ctypes.c_longlong)
+        L.lsm3d_amd_patch_plan_create.argtypes = [I, B, B, LL, ctypes.c_longlong, ctypes.POINTER(R), I, I]
+        L.lsm3d_amd_patch_plan_create.restype = P
+        L.lsm3d_amd_patch_plan_destroy.argtypes = [P]
+        for f in ("cells", "tiles"):
+            getattr(L, f"lsm3d_amd_patch_plan_{f}").argtypes = [P]
+            getattr(L, f"lsm3d_amd_patch_plan_{f}").restype = ctypes.c_longlong
+        L.lsm3d_amd_patch_plan_workgroups.argtypes = [P]
+        L.lsm3d_amd_patch_plan_workgroups.restype = I
+        L.lsm3d_amd_patch_plan_stage.argtypes = [P, I, P, P, P, R, R, R, R, B]
+        L.lsm3d_amd_patch_plan_capture_rk3.argtypes = [P, I, P, P, P, R, R, R, R, B]
+        L.lsm3d_amd_patch_plan_replay.argtypes = [P]
         for f in ("lsm3d_amd_advect_stage", "lsm3d_amd_advect_const_stage", "lsm3d_amd_reinit_stage",
-                  "lsm3d_amd_fill_ghosts", "lsm3d_amd_set_stage_tiling"):
+                  "lsm3d_amd_fill_ghosts", "lsm3d_amd_set_stage_tiling", "lsm3d_amd_patch_plan_destroy",
+                  "lsm3d_amd_patch_plan_stage", "lsm3d_amd_patch_plan_capture_rk3",
+                  "lsm3d_amd_patch_plan_replay"):
             getattr(L, f).restype = None
 
     def fused(self, fname, *args):
         """Calls a fused-API function (Part 3 of include/lsm3d_amd.h)."""
         self._prepare()
-        getattr(self.lib, fname)(*args)
+        out = getattr(self.lib, fname)(*args)
         self._check(fname)
+        return out
 
     def _prepare(self):
         self.lib.lsm_amd_set_error_mode(1)

@@ -1,0 +1,310 @@
+// Batched advance of a list of uneven patches (SAMRAI-style patch level).
+//
+// The reference advances a patch level patch by patch, one toolbox call per
+// patch and routine (LevelSetMethodToolbox.cc:2446-2592 for the RK stage,
+// :106-760 for the derivatives).  Here a plan holds every patch's boxes,
+// pool offset and constant velocity on the device; one persistent launch per
+// Runge-Kutta stage walks a global list of tiles of all patches, and a
+// whole multi-step RK3 advance can be captured once into a hipGraph and
+// replayed.
+//
+// Work distribution: tiles are numbered patch by patch; the 8 XCDs each own
+// a contiguous range of the tile list (so a patch's halo re-reads stay in
+// one L2) and that XCD's workgroups stride through it.  A tile finds its
+// patch by binary search over the per-patch tile prefix.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lsm3d_amd.h"
+#include "lsm_common.h"
+#include "lsm_march.h"
+#include "lsm_runtime.h"
+#include "lsm_stencil.h"
+
+namespace lsm {
+
+struct PatchDesc {
+  Box fb, gb;
+  int64_t offset;  // element offset of the patch's ghost box in every pool
+  real_t vel[3];
+  TileGrid tg;
+};
+
+template <typename T, int TX, int TY, bool SYNTH>
+__global__ void __launch_bounds__(TX* TY)
+    k_patch_stage(const PatchDesc* __restrict__ desc, const int64_t* __restrict__ prefix, int npatch,
+                  int64_t ntiles, T* out_pool, const T* ps_pool, const T* pc_pool, GhostRule ghost, T ix, T iy,
+                  T iz, T dt, int form) {
+  __shared__ T tile[TY + 6][TX + 7];
+  const int G = (int)gridDim.x;  // a multiple of the XCD count
+  const int b = (int)blockIdx.x;
+  const int x = b % kNumXcd, local = b / kNumXcd, nloc = G / kNumXcd;
+  const int64_t lo = ntiles * x / kNumXcd, hi = ntiles * (x + 1) / kNumXcd;
+  for (int64_t t = lo + local; t < hi; t += nloc) {
+    int l = 0, r = npatch - 1;
+    while (l < r) {
+      const int m = (l + r + 1) >> 1;
+      if (prefix[m] <= t) l = m;
+      else r = m - 1;
+    }
+    const PatchDesc d = desc[l];
+    StageArgs<T> a;
+    a.out = make_field<T>(out_pool + d.offset, d.gb);
+    a.ps = make_field<const T>(ps_pool + d.offset, d.gb);
+    if (pc_pool) a.pc = make_field<const T>(pc_pool + d.offset, d.gb);
+    else a.pc.p = nullptr;
+    a.vx.p = a.vy.p = a.vz.p = a.phi0.p = nullptr;
+    a.cvx = T(d.vel[0]);
+    a.cvy = T(d.vel[1]);
+    a.cvz = T(d.vel[2]);
+    a.fb = d.fb;
+    a.tg = d.tg;
+    a.ghost = ghost;
+    a.ix = ix;
+    a.iy = iy;
+    a.iz = iz;
+    a.dt = dt;
+    a.dx_sq = T(0);
+    a.form = form;
+    a.periodic = 0;
+    a.use_phi0 = 0;
+    int bx, by, bz;
+    tile_coords(d.tg, (int)(t - prefix[l]), bx, by, bz);
+    if (SYNTH && !tile_is_interior<TX, TY>(d.fb, d.tg, bx, by, bz))
+      march<T, TX, TY, STAGE_ADVECT_CONST, true, (sizeof(T) == 4)>(a, bx, by, bz, tile);
+    else
+      march<T, TX, TY, STAGE_ADVECT_CONST, false, (sizeof(T) == 4)>(a, bx, by, bz, tile);
+  }
+}
+
+}  // namespace lsm
+
+// ===========================================================================
+// Host side: the plan
+// ===========================================================================
+struct lsm3d_amd_patch_plan {
+  int npatch = 0;
+  int cfg = 0, kz = 32, tx = 64, ty = 4;
+  int64_t ntiles = 0, cells = 0, pool_elems = 0;
+  int grid = 0;  // persistent workgroups, a multiple of 8
+  int device = 0;
+  std::vector<lsm::PatchDesc> host_desc;
+  lsm::PatchDesc* desc = nullptr;
+  int64_t* prefix = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_steps = 0;
+};
+
+namespace {
+using namespace lsm;
+typedef real_t R;
+
+template <int TX, int TY>
+void launch_patch_t(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R* ps, const R* pc,
+                    const GhostRule& g, R ix, R iy, R iz, R dt, int form) {
+  if (g.any)
+    hipLaunchKernelGGL((k_patch_stage<R, TX, TY, true>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
+                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, form);
+  else
+    hipLaunchKernelGGL((k_patch_stage<R, TX, TY, false>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
+                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, form);
+}
+
+void launch_patch(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R* ps, const R* pc,
+                  const GhostRule& g, R dx, R dy, R dz, R dt, int form) {
+  const R ix = R(1) / dx, iy = R(1) / dy, iz = R(1) / dz;
+  switch (p->cfg) {
+    case 0: launch_patch_t<64, 4>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    case 1: launch_patch_t<64, 8>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    case 2: launch_patch_t<32, 8>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    case 3: launch_patch_t<128, 2>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    case 4: launch_patch_t<128, 4>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    default: launch_patch_t<32, 16>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+  }
+}
+
+int occupancy_blocks(int cfg) {
+  int n = 0;
+  hipError_t e = hipSuccess;
+  switch (cfg) {
+    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 64, 4, true>, 256, 0); break;
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 64, 8, true>, 512, 0); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 8, true>, 256, 0); break;
+    case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 2, true>, 256, 0); break;
+    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 4, true>, 512, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 16, true>, 512, 0); break;
+  }
+  if (e != hipSuccess || n < 1) {
+    (void)hipGetLastError();
+    n = 2;
+  }
+  return n;
+}
+
+bool check_plan(const char* fn, const lsm3d_amd_patch_plan* p) {
+  if (!p) {
+    raise_error(std::string(fn) + ": null plan");
+    return false;
+  }
+  return true;
+}
+
+void free_graph(lsm3d_amd_patch_plan* p) {
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->exec = nullptr;
+  p->graph = nullptr;
+  p->graph_steps = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+lsm3d_amd_patch_plan* lsm3d_amd_patch_plan_create(int npatch, const int* fbs, const int* gbs,
+                                                  const long long* offsets, long long pool_elems,
+                                                  const LSMLIB_REAL* vel, int tiling_cfg, int kz) {
+  const char* fn = "lsm3d_amd_patch_plan_create";
+  lsm::Call c(fn);
+  if (npatch < 1 || !fbs || !gbs || !offsets || !vel) raise_error(std::string(fn) + ": empty or null patch list");
+  if (tiling_cfg < 0 || tiling_cfg >= tiling_count() || kz < 1) raise_error(std::string(fn) + ": invalid tiling");
+  if (!c.ok()) return nullptr;
+  auto* p = new lsm3d_amd_patch_plan();
+  p->npatch = npatch;
+  p->cfg = tiling_cfg;
+  p->kz = kz;
+  const TileShape ts = tile_shape(tiling_cfg);
+  p->tx = ts.tx;
+  p->ty = ts.ty;
+  p->pool_elems = pool_elems;
+  std::vector<int64_t> prefix(npatch + 1, 0);
+  for (int q = 0; q < npatch && c.ok(); q++) {
+    PatchDesc d{};
+    d.fb = Box{fbs[6 * q], fbs[6 * q + 1], fbs[6 * q + 2], fbs[6 * q + 3], fbs[6 * q + 4], fbs[6 * q + 5]};
+    d.gb = Box{gbs[6 * q], gbs[6 * q + 1], gbs[6 * q + 2], gbs[6 * q + 3], gbs[6 * q + 4], gbs[6 * q + 5]};
+    d.offset = offsets[q];
+    d.vel[0] = vel[3 * q];
+    d.vel[1] = vel[3 * q + 1];
+    d.vel[2] = vel[3 * q + 2];
+    if (d.fb.cells() <= 0) raise_error(std::string(fn) + ": empty patch fill box");
+    require_inside(fn, "patch ghost box", d.gb, d.fb, 3, 3, 3);
+    if (d.offset < 0 || d.offset + d.gb.cells() > pool_elems)
+      raise_error(std::string(fn) + ": patch storage outside the pool");
+    if (d.fb.nx() < 2 || d.fb.ny() < 2 || d.fb.nz() < 2)
+      raise_error(std::string(fn) + ": patches need 2 cells per axis");
+    d.tg = make_tile_grid(d.fb, p->tx, p->ty, kz);
+    prefix[q + 1] = prefix[q] + d.tg.count();
+    p->cells += d.fb.cells();
+    p->host_desc.push_back(d);
+  }
+  p->ntiles = prefix[npatch];
+  if (c.ok()) {
+    check_hip(hipGetDevice(&p->device), "hipGetDevice");
+    int ncu = 0;
+    check_hip(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device), "hipDeviceGetAttribute");
+    long long want = (long long)ncu * occupancy_blocks(tiling_cfg);
+    if (want > p->ntiles) want = p->ntiles;
+    want = ((want + kNumXcd - 1) / kNumXcd) * kNumXcd;
+    if (want < kNumXcd) want = kNumXcd;
+    p->grid = (int)want;
+    check_hip(hipMalloc(&p->desc, sizeof(PatchDesc) * npatch), "hipMalloc(patch descriptors)");
+    check_hip(hipMalloc(&p->prefix, sizeof(int64_t) * (npatch + 1)), "hipMalloc(tile prefix)");
+    if (c.ok()) {
+      check_hip(hipMemcpy(p->desc, p->host_desc.data(), sizeof(PatchDesc) * npatch, hipMemcpyHostToDevice),
+                "hipMemcpy(patch descriptors)");
+      check_hip(hipMemcpy(p->prefix, prefix.data(), sizeof(int64_t) * (npatch + 1), hipMemcpyHostToDevice),
+                "hipMemcpy(tile prefix)");
+    }
+  }
+  c.finish();
+  if (!c.ok()) {
+    if (p->desc) (void)hipFree(p->desc);
+    if (p->prefix) (void)hipFree(p->prefix);
+    delete p;
+    return nullptr;
+  }
+  return p;
+}
+
+void lsm3d_amd_patch_plan_destroy(lsm3d_amd_patch_plan* p) {
+  if (!p) return;
+  free_graph(p);
+  if (p->desc) (void)hipFree(p->desc);
+  if (p->prefix) (void)hipFree(p->prefix);
+  delete p;
+}
+
+long long lsm3d_amd_patch_plan_cells(const lsm3d_amd_patch_plan* p) { return p ? p->cells : 0; }
+long long lsm3d_amd_patch_plan_tiles(const lsm3d_amd_patch_plan* p) { return p ? p->ntiles : 0; }
+int lsm3d_amd_patch_plan_workgroups(const lsm3d_amd_patch_plan* p) { return p ? p->grid : 0; }
+
+void lsm3d_amd_patch_plan_stage(lsm3d_amd_patch_plan* p, int rk_form, LSMLIB_REAL* out_pool,
+                                const LSMLIB_REAL* stage_pool, const LSMLIB_REAL* cur_pool, LSMLIB_REAL dx,
+                                LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt, const int bc[6]) {
+  const char* fn = "lsm3d_amd_patch_plan_stage";
+  lsm::Call c(fn);
+  if (!check_plan(fn, p)) return;
+  if (!valid_form(rk_form)) raise_error(std::string(fn) + ": unknown Runge-Kutta form");
+  if (!out_pool || !stage_pool) raise_error(std::string(fn) + ": null pool");
+  if (out_pool == stage_pool) raise_error(std::string(fn) + ": phi_out may not alias phi_stage");
+  device_only(fn, out_pool, "out pool");
+  device_only(fn, stage_pool, "stage pool");
+  device_only(fn, cur_pool, "cur pool");
+  GhostRule g = make_ghost_rule(fn, bc, p->host_desc[0].fb);
+  for (int f = 0; f < 6 && bc; f++)
+    if (bc[f] == 5) raise_error(std::string(fn) + ": patches take extrapolated or stored ghosts, not periodic");
+  if (c.ok()) {
+    launch_patch(c.stream(), p, out_pool, stage_pool, cur_pool, g, dx, dy, dz, dt, rk_form);
+    c.launched("k_patch_stage");
+  }
+  c.finish();
+}
+
+// Captures nsteps TVD-RK3 steps (3 stage launches each; phi advanced in
+// place through s1 and s2) into a hipGraph owned by the plan.
+void lsm3d_amd_patch_plan_capture_rk3(lsm3d_amd_patch_plan* p, int nsteps, LSMLIB_REAL* phi_pool,
+                                      LSMLIB_REAL* s1_pool, LSMLIB_REAL* s2_pool, LSMLIB_REAL dx, LSMLIB_REAL dy,
+                                      LSMLIB_REAL dz, LSMLIB_REAL dt, const int bc[6]) {
+  const char* fn = "lsm3d_amd_patch_plan_capture_rk3";
+  lsm::Call c(fn);
+  if (!check_plan(fn, p)) return;
+  if (nsteps < 1) raise_error(std::string(fn) + ": nsteps < 1");
+  device_only(fn, phi_pool, "phi pool");
+  device_only(fn, s1_pool, "s1 pool");
+  device_only(fn, s2_pool, "s2 pool");
+  GhostRule g = make_ghost_rule(fn, bc, p->host_desc[0].fb);
+  if (!c.ok()) {
+    c.finish();
+    return;
+  }
+  free_graph(p);
+  hipStream_t cs = nullptr;
+  check_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
+  check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  if (c.ok()) {
+    for (int s = 0; s < nsteps; s++) {
+      launch_patch(cs, p, s1_pool, phi_pool, nullptr, g, dx, dy, dz, dt, RK_EULER);
+      launch_patch(cs, p, s2_pool, s1_pool, phi_pool, g, dx, dy, dz, dt, RK3_STAGE2);
+      launch_patch(cs, p, phi_pool, s2_pool, phi_pool, g, dx, dy, dz, dt, RK3_STAGE3);
+    }
+    check_hip(hipStreamEndCapture(cs, &p->graph), "hipStreamEndCapture");
+  }
+  if (c.ok()) check_hip(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  if (c.ok()) p->graph_steps = nsteps;
+  if (cs) (void)hipStreamDestroy(cs);
+  c.finish();
+}
+
+void lsm3d_amd_patch_plan_replay(lsm3d_amd_patch_plan* p) {
+  const char* fn = "lsm3d_amd_patch_plan_replay";
+  lsm::Call c(fn);
+  if (!check_plan(fn, p)) return;
+  if (!p->exec) raise_error(std::string(fn) + ": no captured graph");
+  if (c.ok()) check_hip(hipGraphLaunch(p->exec, c.stream()), "hipGraphLaunch");
+  c.finish();
+}
+
+}  // extern "C"

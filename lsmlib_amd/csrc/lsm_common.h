@@ -42,7 +42,7 @@ struct Field {
 };
 
 template <typename T>
-inline Field<T> make_field(T* p, const Box& gb) {
+__host__ __device__ inline Field<T> make_field(T* p, const Box& gb) {
   Field<T> f;
   f.p = p;
   f.ilo = gb.ilo;

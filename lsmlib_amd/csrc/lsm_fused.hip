@@ -10,16 +10,12 @@
 //               with HJ-WENO5 plus/minus derivatives of phi_s
 //               (LSM3D_HJ_WENO5 -> LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS -> RK,
 //                ReinitializationAlgorithm.cc:698-1020)
-// so each stage reads phi_s, the velocity (or phi0) and phi_cur once from HBM
-// and writes phi_out once; the undivided differences, one-sided derivatives
-// and right-hand side never leave registers/LDS.
-//
-// Kernel shape: a workgroup owns a TX x TY column of cells and marches up a
-// chunk of kz planes.  The current plane of phi_s sits in LDS with a 3-cell
-// halo in x and y; the seven z-neighbours of each column sit in a register
-// queue, so each phi_s value is fetched from HBM once per stage (plus the
-// halo re-reads, which hit L2).  Workgroup ids are remapped so each XCD owns
-// a contiguous slab of tiles.
+// Each stage reads phi_s, the velocity (or phi0) and phi_cur once from HBM
+// and writes phi_out once; undivided differences, one-sided derivatives and
+// the right-hand side stay in registers and LDS.  Ghost cells of phi_s are
+// either read as stored or synthesised from the fill box per face
+// (extrapolation or periodic image), which removes the ghost-fill pass
+// between stages.
 #include <cstdio>
 #include <string>
 
@@ -31,115 +27,18 @@
 
 namespace lsm {
 
-enum StageKind { STAGE_ADVECT_FIELD = 0, STAGE_ADVECT_CONST = 1, STAGE_REINIT = 2 };
-
-template <typename T>
-struct StageArgs {
-  Field<T> out;
-  Field<const T> ps;   // stage input (ghosts filled)
-  Field<const T> pc;   // phi at the start of the step (p == nullptr: use ps)
-  Field<const T> vx, vy, vz;  // velocity (advection, field form)
-  Field<const T> phi0;        // reinit: frozen phi0 for the smoothed sign
-  T cvx, cvy, cvz;            // constant velocity
-  Box fb;
-  TileGrid tg;
-  T ix, iy, iz, dt, dx_sq;
-  int form;      // RKForm
-  int periodic;  // bit mask of periodic axes: write ghost images of phi_out
-  int use_phi0;  // reinit sign from phi0 (1) or phi_s (0)
-};
-
-template <typename T, int TX, int TY, int KIND>
+// SYNTH: ghost values are synthesised per face; otherwise stored ghost cells
+// are read.  FP32 uses the rearranged WENO5 form, FP64 the reference order.
+template <typename T, int TX, int TY, int KIND, bool SYNTH>
 __global__ void __launch_bounds__(TX* TY) k_stage(StageArgs<T> a) {
-  __shared__ T tile[TY + 6][TX + 6 + 1];
-  const int t = threadIdx.x;
-  const int tx = t % TX, ty = t / TX;
+  __shared__ T tile[TY + 6][TX + 7];
   const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-  const int bx = b % a.tg.ntx;
-  const int rem = b / a.tg.ntx;
-  const int by = rem % a.tg.nty;
-  const int bz = rem / a.tg.nty;
-  const Box fb = a.fb;
-  const int i0 = fb.ilo + bx * TX, j0 = fb.jlo + by * TY;
-  const int i = i0 + tx, j = j0 + ty;
-  const int k0 = fb.klo + bz * a.tg.kz;
-  const int k1 = min(k0 + a.tg.kz - 1, fb.khi);
-  const bool active = (i <= fb.ihi) && (j <= fb.jhi);
-  const bool loadable = (i <= fb.ihi + 3) && (j <= fb.jhi + 3);
-
-  T zq[7];
-#pragma unroll
-  for (int q = 0; q < 6; q++) zq[q] = loadable ? a.ps.at(i, j, k0 - 3 + q) : T(0);
-
-  for (int k = k0; k <= k1; k++) {
-    zq[6] = loadable ? a.ps.at(i, j, k + 3) : T(0);
-    T ux = T(0), uy = T(0), uz = T(0), pcv = T(0), p0v = T(0);
-    if (active) {
-      if (KIND == STAGE_ADVECT_FIELD) {
-        ux = a.vx.at(i, j, k);
-        uy = a.vy.at(i, j, k);
-        uz = a.vz.at(i, j, k);
-      } else if (KIND == STAGE_ADVECT_CONST) {
-        ux = a.cvx;
-        uy = a.cvy;
-        uz = a.cvz;
-      } else if (a.use_phi0) {
-        p0v = a.phi0.at(i, j, k);
-      }
-      if (a.pc.p) pcv = a.pc.at(i, j, k);
-    }
-    __syncthreads();  // previous plane's LDS reads are done
-    tile[ty + 3][tx + 3] = zq[3];
-    if (tx < 3) {
-      const int gi = i0 - 3 + tx;
-      tile[ty + 3][tx] = (j <= fb.jhi + 3) ? a.ps.at(gi, j, k) : T(0);
-    } else if (tx < 6) {
-      const int gi = i0 + TX + (tx - 3);
-      tile[ty + 3][TX + tx] = (gi <= fb.ihi + 3 && j <= fb.jhi + 3) ? a.ps.at(gi, j, k) : T(0);
-    }
-#pragma unroll
-    for (int r = ty; r < 6; r += TY) {
-      const int gj = (r < 3) ? j0 - 3 + r : j0 + TY + (r - 3);
-      const int row = (r < 3) ? r : TY + r;
-      tile[row][tx + 3] = (i <= fb.ihi + 3 && gj <= fb.jhi + 3) ? a.ps.at(i, gj, k) : T(0);
-    }
-    __syncthreads();
-
-    if (active) {
-      T px[7], py[7];
-#pragma unroll
-      for (int q = 0; q < 7; q++) {
-        px[q] = tile[ty + 3][tx + q];
-        py[q] = tile[ty + q][tx + 3];
-      }
-      const T us = zq[3];
-      const T uc = a.pc.p ? pcv : us;
-      T rhs;
-      if (KIND == STAGE_REINIT) {
-        const T xp = hj_plus<T, WENO5>(px, a.ix), xm = hj_minus<T, WENO5>(px, a.ix);
-        const T yp = hj_plus<T, WENO5>(py, a.iy), ym = hj_minus<T, WENO5>(py, a.iy);
-        const T zp = hj_plus<T, WENO5>(zq, a.iz), zm = hj_minus<T, WENO5>(zq, a.iz);
-        // lsm_reinitialization3d.f.in:150-268
-        const T cur = a.use_phi0 ? p0v : us;
-        rhs = T(0);
-        if (fabs(cur) >= Tol<T>::zero) {
-          const T n2 = godunov_star_sq<T>(cur, xp, yp, zp, xm, ym, zm);
-          const T sgn = a.use_phi0 ? cur / sqrt(cur * cur + a.dx_sq) : cur / sqrt(cur * cur + n2 * a.dx_sq);
-          rhs = sgn * (T(1) - sqrt(n2));
-        }
-      } else {
-        const T gx = upwind_weno5_select<T>(px, ux, a.ix);
-        const T gy = upwind_weno5_select<T>(py, uy, a.iy);
-        const T gz = upwind_weno5_select<T>(zq, uz, a.iz);
-        // lsm_level_set_evolution3d.f.in:113-116 on a zeroed rhs
-        rhs = -(ux * gx + uy * gy + uz * gz);
-      }
-      const T v = rk_combine<T>(a.form, uc, us, rhs, a.dt);
-      store_with_periodic_images<T>(a.out, fb, i, j, k, v, a.periodic, 3);
-    }
-#pragma unroll
-    for (int q = 0; q < 6; q++) zq[q] = zq[q + 1];
-  }
+  int bx, by, bz;
+  tile_coords(a.tg, b, bx, by, bz);
+  if (SYNTH && !tile_is_interior<TX, TY>(a.fb, a.tg, bx, by, bz))
+    march<T, TX, TY, KIND, true, (sizeof(T) == 4)>(a, bx, by, bz, tile);
+  else
+    march<T, TX, TY, KIND, false, (sizeof(T) == 4)>(a, bx, by, bz, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -167,8 +66,7 @@ __global__ void __launch_bounds__(256)
   if (bc == 5) {
     for (int t = lo; t <= up; t++) {
       idx[d] = hi ? t - n : t + n;
-      const int src0 = idx[0], src1 = idx[1], src2 = idx[2];
-      const T val = phi.at(src0, src1, src2);
+      const T val = phi.at(idx[0], idx[1], idx[2]);
       idx[d] = t;
       phi.at(idx[0], idx[1], idx[2]) = val;
     }
@@ -179,17 +77,9 @@ __global__ void __launch_bounds__(256)
   const T pb = phi.at(idx[0], idx[1], idx[2]);
   idx[d] = bnd + (hi ? -1 : 1);
   const T p1 = phi.at(idx[0], idx[1], idx[2]);
-  T slope = T(0);
-  if (bc == 2) slope = pb - p1;
-  if (bc == 3) slope = copysign(T(1), pb) * fabs(pb - p1);
   for (int t = lo; t <= up; t++) {
     idx[d] = t;
-    if (bc == 1) {
-      phi.at(idx[0], idx[1], idx[2]) = pb;
-    } else {
-      const T dist = T(hi ? (t - bnd) : (bnd - t));
-      phi.at(idx[0], idx[1], idx[2]) = pb + slope * dist;
-    }
+    phi.at(idx[0], idx[1], idx[2]) = extrapolate<T>(bc, pb, p1, hi ? (t - bnd) : (bnd - t));
   }
 }
 
@@ -198,26 +88,72 @@ __global__ void __launch_bounds__(256)
 // ===========================================================================
 // Host side
 // ===========================================================================
-namespace {
-using namespace lsm;
-typedef real_t R;
+namespace lsm {
 
 // Tile configurations compiled in (TX x TY threads).  The autotuner picks
 // one per grid-size class through lsm3d_amd_set_stage_tiling().
-struct Tiling {
-  int cfg;  // index into kTiles
-  int kz;   // planes per workgroup
-};
-struct TileShape {
-  int tx, ty;
-};
 constexpr TileShape kTiles[] = {{64, 4}, {64, 8}, {32, 8}, {128, 2}, {128, 4}, {32, 16}};
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+
 thread_local Tiling tl_tiling = {0, 32};
+
+Tiling current_tiling() { return tl_tiling; }
+TileShape tile_shape(int cfg) { return kTiles[cfg]; }
+int tiling_count() { return kNumTiles; }
+
+bool valid_form(int form) {
+  return form == RK_EULER || form == RK2_STAGE2 || form == RK3_STAGE2 || form == RK3_STAGE3;
+}
+
+bool device_only(const char* fn, const void* p, const char* what) {
+  if (p && !is_device_pointer(p)) {
+    raise_error(std::string(fn) + ": " + what + " must be device memory");
+    return false;
+  }
+  return true;
+}
+
+// Validates the 6 ghost codes for a fill box; returns the rule.
+GhostRule make_ghost_rule(const char* fn, const int* bc, const Box& fb) {
+  GhostRule g{};
+  if (!bc) return g;
+  const int n[3] = {fb.nx(), fb.ny(), fb.nz()};
+  for (int f = 0; f < 6; f++) {
+    const int c = bc[f];
+    if (c != 0 && c != 1 && c != 2 && c != 3 && c != 5) raise_error(std::string(fn) + ": unknown ghost code");
+    if ((c == 2 || c == 3) && n[f / 2] < 2) raise_error(std::string(fn) + ": extrapolation needs 2 cells");
+    if (c == 5 && n[f / 2] < 3) raise_error(std::string(fn) + ": a periodic axis needs 3 cells");
+    g.code[f] = c;
+    g.any |= c;
+  }
+  for (int d = 0; d < 3; d++)
+    if ((bc[2 * d] == 5) != (bc[2 * d + 1] == 5))
+      raise_error(std::string(fn) + ": periodic faces come in pairs");
+  return g;
+}
+
+Field<const real_t> cfield(const real_t* p, const int* gb) {
+  Field<const real_t> f;
+  if (!p) {
+    f.p = nullptr;
+    f.ilo = f.jlo = f.klo = 0;
+    f.sy = f.sz = 0;
+    return f;
+  }
+  return make_field<const real_t>(p, Box{gb[0], gb[1], gb[2], gb[3], gb[4], gb[5]});
+}
+
+namespace {
+typedef real_t R;
+
+Box box6(const int* b) { return Box{b[0], b[1], b[2], b[3], b[4], b[5]}; }
 
 template <int TX, int TY, int KIND>
 void launch_stage_t(hipStream_t s, const StageArgs<R>& a, int nblk) {
-  hipLaunchKernelGGL((k_stage<R, TX, TY, KIND>), dim3((unsigned)nblk), dim3(TX * TY), 0, s, a);
+  if (a.ghost.any)
+    hipLaunchKernelGGL((k_stage<R, TX, TY, KIND, true>), dim3((unsigned)nblk), dim3(TX * TY), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_stage<R, TX, TY, KIND, false>), dim3((unsigned)nblk), dim3(TX * TY), 0, s, a);
 }
 
 template <int KIND>
@@ -232,47 +168,11 @@ void launch_stage_kind(hipStream_t s, const StageArgs<R>& a, int cfg, int nblk) 
   }
 }
 
-Box box6(const int* b) { return Box{b[0], b[1], b[2], b[3], b[4], b[5]}; }
-
-Field<const R> cfield(const R* p, const int* gb) {
-  if (!p) {
-    Field<const R> f;
-    f.p = nullptr;
-    f.ilo = f.jlo = f.klo = 0;
-    f.sy = f.sz = 0;
-    return f;
-  }
-  return make_field<const R>(p, box6(gb));
-}
-
-bool device_only(const char* fn, const void* p, const char* what) {
-  if (p && !is_device_pointer(p)) {
-    raise_error(std::string(fn) + ": " + what + " must be device memory");
-    return false;
-  }
-  return true;
-}
-
-bool valid_form(int form) {
-  return form == RK_EULER || form == RK2_STAGE2 || form == RK3_STAGE2 || form == RK3_STAGE3;
-}
-
-int check_periodic(const char* fn, const Box& fb, int periodic) {
-  const int n[3] = {fb.nx(), fb.ny(), fb.nz()};
-  for (int d = 0; d < 3; d++)
-    if ((periodic >> d) & 1)
-      if (n[d] < 6) {
-        raise_error(std::string(fn) + ": a periodic axis needs at least 6 cells");
-        return 0;
-      }
-  return periodic & 7;
-}
-
 // Shared by the advection and reinitialization stage entry points.
 void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, const R* ps,
                const int* ps_gb, const R* pc, const int* pc_gb, const R* vx, const R* vy, const R* vz,
                const int* vel_gb, const R* phi0, const int* phi0_gb, R cvx, R cvy, R cvz,
-               const int* fbp, R dx, R dy, R dz, R dt, int use_phi0, int periodic) {
+               const int* fbp, R dx, R dy, R dz, R dt, int use_phi0, const int* bc, int periodic) {
   Call c(fn);
   const Box fb = box6(fbp);
   if (!valid_form(form)) raise_error(std::string(fn) + ": unknown Runge-Kutta form");
@@ -281,9 +181,13 @@ void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, co
   if (pc) require_inside(fn, "phi_cur", box6(pc_gb), fb, 0, 0, 0);
   if (kind == STAGE_ADVECT_FIELD) require_inside(fn, "vel", box6(vel_gb), fb, 0, 0, 0);
   if (kind == STAGE_REINIT && use_phi0) require_inside(fn, "phi0", box6(phi0_gb), fb, 0, 0, 0);
+  const GhostRule ghost = make_ghost_rule(fn, bc, fb);
+  periodic &= 7;
   if (periodic) {
     require_inside(fn, "phi_out", box6(out_gb), fb, 3, 3, 3);
-    periodic = check_periodic(fn, fb, periodic);
+    const int n[3] = {fb.nx(), fb.ny(), fb.nz()};
+    for (int d = 0; d < 3; d++)
+      if (((periodic >> d) & 1) && n[d] < 6) raise_error(std::string(fn) + ": periodic images need 6 cells");
   }
   if (!(device_only(fn, out, "phi_out") && device_only(fn, ps, "phi_stage") &&
         device_only(fn, pc, "phi_cur") && device_only(fn, vx, "vel_x") && device_only(fn, vy, "vel_y") &&
@@ -308,6 +212,7 @@ void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, co
   a.cvz = cvz;
   a.fb = fb;
   a.tg = make_tile_grid(fb, ts.tx, ts.ty, tl.kz);
+  a.ghost = ghost;
   a.ix = R(1) / dx;
   a.iy = R(1) / dy;
   a.iz = R(1) / dz;
@@ -333,6 +238,9 @@ void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, co
 }
 
 }  // namespace
+}  // namespace lsm
+
+using namespace lsm;
 
 extern "C" {
 
@@ -360,9 +268,9 @@ void lsm3d_amd_advect_stage(int rk_form, LSMLIB_REAL* phi_out, const int out_gb[
                             const LSMLIB_REAL* phi_stage, const int stage_gb[6], const LSMLIB_REAL* phi_cur,
                             const int cur_gb[6], const LSMLIB_REAL* vel_x, const LSMLIB_REAL* vel_y,
                             const LSMLIB_REAL* vel_z, const int vel_gb[6], const int fb[6], LSMLIB_REAL dx,
-                            LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt, int periodic_mask) {
+                            LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt, const int bc[6], int periodic_mask) {
   run_stage("lsm3d_amd_advect_stage", STAGE_ADVECT_FIELD, rk_form, phi_out, out_gb, phi_stage, stage_gb,
-            phi_cur, cur_gb, vel_x, vel_y, vel_z, vel_gb, nullptr, nullptr, 0, 0, 0, fb, dx, dy, dz, dt, 0,
+            phi_cur, cur_gb, vel_x, vel_y, vel_z, vel_gb, nullptr, nullptr, 0, 0, 0, fb, dx, dy, dz, dt, 0, bc,
             periodic_mask);
 }
 
@@ -370,26 +278,28 @@ void lsm3d_amd_advect_const_stage(int rk_form, LSMLIB_REAL* phi_out, const int o
                                   const LSMLIB_REAL* phi_stage, const int stage_gb[6],
                                   const LSMLIB_REAL* phi_cur, const int cur_gb[6], LSMLIB_REAL vel_x,
                                   LSMLIB_REAL vel_y, LSMLIB_REAL vel_z, const int fb[6], LSMLIB_REAL dx,
-                                  LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt, int periodic_mask) {
+                                  LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt, const int bc[6],
+                                  int periodic_mask) {
   run_stage("lsm3d_amd_advect_const_stage", STAGE_ADVECT_CONST, rk_form, phi_out, out_gb, phi_stage,
             stage_gb, phi_cur, cur_gb, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, vel_x, vel_y,
-            vel_z, fb, dx, dy, dz, dt, 0, periodic_mask);
+            vel_z, fb, dx, dy, dz, dt, 0, bc, periodic_mask);
 }
 
 void lsm3d_amd_reinit_stage(int rk_form, LSMLIB_REAL* phi_out, const int out_gb[6],
                             const LSMLIB_REAL* phi_stage, const int stage_gb[6], const LSMLIB_REAL* phi_cur,
                             const int cur_gb[6], const LSMLIB_REAL* phi0, const int phi0_gb[6],
                             int use_phi0_for_sgn, const int fb[6], LSMLIB_REAL dx, LSMLIB_REAL dy,
-                            LSMLIB_REAL dz, LSMLIB_REAL dt, int periodic_mask) {
+                            LSMLIB_REAL dz, LSMLIB_REAL dt, const int bc[6], int periodic_mask) {
   run_stage("lsm3d_amd_reinit_stage", STAGE_REINIT, rk_form, phi_out, out_gb, phi_stage, stage_gb, phi_cur,
             cur_gb, nullptr, nullptr, nullptr, nullptr, phi0, phi0_gb, 0, 0, 0, fb, dx, dy, dz, dt,
-            use_phi0_for_sgn == 1, periodic_mask);
+            use_phi0_for_sgn == 1, bc, periodic_mask);
 }
 
 void lsm3d_amd_fill_ghosts(LSMLIB_REAL* phi, const int gb[6], const int fb[6], const int bc[6]) {
   const char* fn = "lsm3d_amd_fill_ghosts";
   Call c(fn);
-  const Box g = box6(gb), f = box6(fb);
+  const Box g = Box{gb[0], gb[1], gb[2], gb[3], gb[4], gb[5]};
+  const Box f = Box{fb[0], fb[1], fb[2], fb[3], fb[4], fb[5]};
   require_inside(fn, "phi", g, f, 0, 0, 0);
   const int n[3] = {f.nx(), f.ny(), f.nz()};
   const int gw[3][2] = {{f.ilo - g.ilo, g.ihi - f.ihi}, {f.jlo - g.jlo, g.jhi - f.jhi}, {f.klo - g.klo, g.khi - f.khi}};
@@ -403,13 +313,13 @@ void lsm3d_amd_fill_ghosts(LSMLIB_REAL* phi, const int gb[6], const int fb[6], c
     }
   device_only(fn, phi, "phi");
   if (c.ok()) {
-    Field<R> fp = make_field<R>(phi, g);
+    Field<real_t> fp = make_field<real_t>(phi, g);
     const int ext[3] = {g.nx(), g.ny(), g.nz()};
     for (int d = 0; d < 3; d++) {  // x faces, then y, then z, as faces 0..5 are applied in order
       if (bc[2 * d] == 0 && bc[2 * d + 1] == 0) continue;
       const int u = (d + 1) % 3, v = (d + 2) % 3;
       dim3 grid((unsigned)((ext[u] + 63) / 64), (unsigned)((ext[v] + 3) / 4), 2);
-      hipLaunchKernelGGL(k_fill_axis<R>, grid, dim3(64, 4, 1), 0, c.stream(), fp, g, f, d, bc[2 * d],
+      hipLaunchKernelGGL(k_fill_axis<real_t>, grid, dim3(64, 4, 1), 0, c.stream(), fp, g, f, d, bc[2 * d],
                          bc[2 * d + 1]);
       c.launched("k_fill_axis");
     }

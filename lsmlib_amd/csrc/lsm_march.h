@@ -1,6 +1,7 @@
-// Device building blocks of the fused stage kernels: XCD-aware tile mapping
-// and the 2.5D z-march that keeps a (TY+6) x (TX+6) plane of the stage field
-// in LDS and the seven z-neighbours of each column in registers.
+// Device building blocks of the fused stage kernels: XCD-aware tile mapping,
+// the boundary-aware loader and the 2.5D z-march that keeps a (TY+6) x (TX+6)
+// plane of the stage field in LDS and the seven z-neighbours of each column
+// in registers.
 #pragma once
 
 #include "lsm_common.h"
@@ -20,11 +21,11 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
   return (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
 }
 
-// Tile geometry of one launch: fill box split into TX x TY columns and
-// chunks of kz planes.
+// Tile geometry: fill box split into TX x TY columns and chunks of kz planes.
 struct TileGrid {
   int ntx, nty, nkz;  // tiles along x, y and number of z chunks
   int kz;             // planes per chunk
+  __host__ __device__ int count() const { return ntx * nty * nkz; }
 };
 
 inline TileGrid make_tile_grid(const Box& fb, int TX, int TY, int kz) {
@@ -36,28 +37,29 @@ inline TileGrid make_tile_grid(const Box& fb, int TX, int TY, int kz) {
   return g;
 }
 
-// Selects the five divided differences WENO5 needs on the upwind side and
-// evaluates it once (no divergence across a velocity sign change).
-// p[0..6] = phi(c-3..c+3).  vel > 0 takes the backward stencil
-// (lsm_spatial_derivatives3d.f.in:2759-2766), otherwise the forward one
-// (:2768-2775); |vel| < zero_tol gives 0 (:2756-2757).
+// Ghost-cell codes per face (x_lo, x_hi, y_lo, y_hi, z_lo, z_hi):
+//   0 read the stored ghost cell, 1 copy (homogeneous Neumann), 2 linear,
+//   3 signed linear extrapolation, 5 periodic.
+// A cross-shaped stencil centred in the fill box leaves it through one face
+// at a time, so every ghost value it reads is the single-face extrapolation
+// of two interior cells (lsm_boundary_conditions3d.f:66-77, 213-225) or a
+// periodic image; synthesising it here gives the same value the reference's
+// fill-then-differentiate sequence reads, without a fill pass.
+struct GhostRule {
+  int code[6];
+  int any;  // nonzero if some face is synthesised
+};
+
 template <typename T>
-__device__ __forceinline__ T upwind_weno5_select(const T* p, T vel, T inv) {
-  T d[6];
-#pragma unroll
-  for (int q = 0; q < 6; q++) d[q] = (p[q + 1] - p[q]) * inv;
-  const bool bwd = vel > T(0);
-  const T v1 = bwd ? d[0] : d[5];
-  const T v2 = bwd ? d[1] : d[4];
-  const T v3 = bwd ? d[2] : d[3];
-  const T v4 = bwd ? d[3] : d[2];
-  const T v5 = bwd ? d[4] : d[1];
-  const T r = weno5_ref<T>(v1, v2, v3, v4, v5);
-  return (fabs(vel) < Tol<T>::zero) ? T(0) : r;
+__device__ __forceinline__ T extrapolate(int code, T pb, T p1, int dist) {
+  if (code == 1) return pb;
+  T slope = (code == 2) ? pb - p1 : copysign(T(1), pb) * fabs(pb - p1);
+  return pb + slope * T(dist);
 }
 
-// Writes `val` at (i,j,k) and, for each periodic axis whose image of the
-// cell lies in the ghost layer, at the periodic images (up to 7 more cells).
+// Writes `val` at (i,j,k) and, for each axis in `periodic` whose image of
+// the cell lies in the g-wide ghost layer, at the periodic images (up to 7
+// more cells).
 template <typename T>
 __device__ __forceinline__ void store_with_periodic_images(const Field<T>& out, const Box& fb, int i,
                                                            int j, int k, T val, int periodic, int g) {
@@ -81,6 +83,303 @@ __device__ __forceinline__ void store_with_periodic_images(const Field<T>& out, 
     for (int b = 0; b < nys; b++)
       for (int a = 0; a < nxs; a++)
         if (a | b | c) out.at(xi[a], yj[b], zk[c]) = val;
+}
+
+// ---------------------------------------------------------------------------
+// One fused Runge-Kutta stage over one tile.
+// ---------------------------------------------------------------------------
+enum StageKind { STAGE_ADVECT_FIELD = 0, STAGE_ADVECT_CONST = 1, STAGE_REINIT = 2 };
+
+template <typename T>
+struct StageArgs {
+  Field<T> out;
+  Field<const T> ps;          // stage input
+  Field<const T> pc;          // phi at the start of the step (p == nullptr: use ps)
+  Field<const T> vx, vy, vz;  // velocity (advection, field form)
+  Field<const T> phi0;        // reinit: frozen phi0 for the smoothed sign
+  T cvx, cvy, cvz;            // constant velocity
+  Box fb;
+  TileGrid tg;
+  GhostRule ghost;
+  T ix, iy, iz, dt, dx_sq;
+  int form;      // RKForm
+  int periodic;  // bit mask: also write phi_out's periodic ghost images
+  int use_phi0;  // reinit sign from phi0 (1) or phi_s (0)
+};
+
+// ---------------------------------------------------------------------------
+// Fast march: stored ghost cells only, clamped (always in-bounds) loads, one
+// 64-bit plane pointer per field and 32-bit in-plane offsets.  FASTW selects
+// the rearranged WENO5 (weno5_fast) instead of the reference-order form.
+// ---------------------------------------------------------------------------
+template <typename T, bool FASTW>
+__device__ __forceinline__ T upwind_weno5(const T* p, T vel, T inv, T tiny_dx2) {
+  T d[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) d[q] = p[q + 1] - p[q];
+  const bool bwd = vel > T(0);
+  const T D1 = bwd ? d[0] : d[5], D2 = bwd ? d[1] : d[4], D3 = bwd ? d[2] : d[3];
+  const T D4 = bwd ? d[3] : d[2], D5 = bwd ? d[4] : d[1];
+  T r;
+  if (FASTW) r = weno5_fast<T>(D1, D2, D3, D4, D5, tiny_dx2) * inv;
+  else r = weno5_ref<T>(D1 * inv, D2 * inv, D3 * inv, D4 * inv, D5 * inv);
+  return (fabs(vel) < Tol<T>::zero) ? T(0) : r;
+}
+
+template <typename T, bool FASTW>
+__device__ __forceinline__ void hj_weno5_pm(const T* p, T inv, T tiny_dx2, T& plus, T& minus) {
+  if (FASTW) {
+    T d[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) d[q] = p[q + 1] - p[q];
+    plus = weno5_fast<T>(d[5], d[4], d[3], d[2], d[1], tiny_dx2) * inv;
+    minus = weno5_fast<T>(d[0], d[1], d[2], d[3], d[4], tiny_dx2) * inv;
+  } else {
+    plus = hj_plus<T, WENO5>(p, inv);
+    minus = hj_minus<T, WENO5>(p, inv);
+  }
+}
+
+template <typename T>
+struct Plane {  // a field addressed as plane pointer + in-plane offset
+  const T* base;
+  int ilo, jlo, klo;
+  int sy;
+  int64_t sz;
+  __device__ __forceinline__ int off(int i, int j) const { return (i - ilo) + sy * (j - jlo); }
+  __device__ __forceinline__ const T* plane(int k) const { return base + sz * (int64_t)(k - klo); }
+};
+
+template <typename T>
+__device__ __forceinline__ Plane<T> plane_of(const Field<const T>& f) {
+  Plane<T> p;
+  p.base = f.p;
+  p.ilo = f.ilo;
+  p.jlo = f.jlo;
+  p.klo = f.klo;
+  p.sy = (int)f.sy;
+  p.sz = f.sz;
+  return p;
+}
+
+// How one thread obtains the stage field along one (i, j) column: stored
+// value at offset o (code 0), or the single-face extrapolation of the two
+// fill-box cells at ob, o1 at distance dist (codes 1-3).  Periodic faces
+// are resolved to a wrapped offset when the rule is built.
+struct ColumnRule {
+  int o, code, ob, o1, dist;
+};
+
+template <typename T>
+__device__ __forceinline__ ColumnRule column_rule(const Plane<T>& ps, const Box& fb, const GhostRule& g,
+                                                  int ci, int cj) {
+  ColumnRule r{ps.off(ci, cj), 0, 0, 0, 0};
+  if (g.code[0] == 5 && ci < fb.ilo) ci += fb.nx();
+  if (g.code[1] == 5 && ci > fb.ihi) ci -= fb.nx();
+  if (g.code[2] == 5 && cj < fb.jlo) cj += fb.ny();
+  if (g.code[3] == 5 && cj > fb.jhi) cj -= fb.ny();
+  r.o = ps.off(ci, cj);
+  if (ci < fb.ilo && g.code[0]) {
+    r.code = g.code[0];
+    r.ob = ps.off(fb.ilo, cj);
+    r.o1 = ps.off(fb.ilo + 1, cj);
+    r.dist = fb.ilo - ci;
+  } else if (ci > fb.ihi && g.code[1]) {
+    r.code = g.code[1];
+    r.ob = ps.off(fb.ihi, cj);
+    r.o1 = ps.off(fb.ihi - 1, cj);
+    r.dist = ci - fb.ihi;
+  } else if (cj < fb.jlo && g.code[2]) {
+    r.code = g.code[2];
+    r.ob = ps.off(ci, fb.jlo);
+    r.o1 = ps.off(ci, fb.jlo + 1);
+    r.dist = fb.jlo - cj;
+  } else if (cj > fb.jhi && g.code[3]) {
+    r.code = g.code[3];
+    r.ob = ps.off(ci, fb.jhi);
+    r.o1 = ps.off(ci, fb.jhi - 1);
+    r.dist = cj - fb.jhi;
+  }
+  return r;
+}
+
+template <typename T, bool GHOST>
+__device__ __forceinline__ T column_load(const ColumnRule& r, const T* pl) {
+  if (GHOST && r.code) return extrapolate<T>(r.code, pl[r.ob], pl[r.o1], r.dist);
+  return pl[r.o];
+}
+
+// Column value at plane kk, applying the z-face rule when kk leaves the
+// fill box (kk depends only on the loop counter: uniform per workgroup).
+template <typename T, bool GHOST>
+__device__ __forceinline__ T column_at(const ColumnRule& r, const Plane<T>& ps, const Box& fb,
+                                       const GhostRule& g, int kk) {
+  if (GHOST) {
+    if (kk < fb.klo && g.code[4]) {
+      if (g.code[4] == 5) return column_load<T, GHOST>(r, ps.plane(kk + fb.nz()));
+      return extrapolate<T>(g.code[4], column_load<T, GHOST>(r, ps.plane(fb.klo)),
+                            column_load<T, GHOST>(r, ps.plane(fb.klo + 1)), fb.klo - kk);
+    }
+    if (kk > fb.khi && g.code[5]) {
+      if (g.code[5] == 5) return column_load<T, GHOST>(r, ps.plane(kk - fb.nz()));
+      return extrapolate<T>(g.code[5], column_load<T, GHOST>(r, ps.plane(fb.khi)),
+                            column_load<T, GHOST>(r, ps.plane(fb.khi - 1)), kk - fb.khi);
+    }
+  }
+  return column_load<T, GHOST>(r, ps.plane(kk));
+}
+
+// The workgroup (TX*TY threads, 1-D) owns tile (bx, by) x z-chunk bz:
+// column (i, j) per thread, planes k0..k1 marched bottom to top.  GHOST:
+// ghost values are synthesised per the rule a.ghost; otherwise stored ghost
+// cells are read.  FASTW selects the rearranged WENO5 form.
+template <typename T, int TX, int TY, int KIND, bool GHOST, bool FASTW>
+__device__ __forceinline__ void march(const StageArgs<T>& a, int bx, int by, int bz, T (*tile)[TX + 7]) {
+  const int t = threadIdx.x;
+  const int tx = t % TX, ty = t / TX;
+  const Box& fb = a.fb;
+  const int i0 = fb.ilo + bx * TX, j0 = fb.jlo + by * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const int k0 = fb.klo + bz * a.tg.kz;
+  const int k1 = min(k0 + a.tg.kz - 1, fb.khi);
+  const bool active = (i <= fb.ihi) && (j <= fb.jhi);
+  // clamped load positions (values at clamped positions are never used)
+  const int ic = min(i, fb.ihi + 3), jc = min(j, fb.jhi + 3);
+  const Plane<T> ps = plane_of<T>(a.ps);
+  const GhostRule& g = a.ghost;
+  const ColumnRule rc = GHOST ? column_rule<T>(ps, fb, g, ic, jc) : ColumnRule{ps.off(ic, jc), 0, 0, 0, 0};
+  const bool do_x = tx < 6;
+  const int xi = (tx < 3) ? i0 - 3 + tx : min(i0 + TX + tx - 3, fb.ihi + 3);
+  const ColumnRule rx = GHOST ? column_rule<T>(ps, fb, g, xi, jc) : ColumnRule{ps.off(xi, jc), 0, 0, 0, 0};
+  // y halo: 6 rows (3 below, 3 above the tile) spread over the TY thread rows
+  constexpr int NR = (6 + TY - 1) / TY;
+  ColumnRule ry[NR];
+  int tyr[NR];
+  bool doy[NR];
+#pragma unroll
+  for (int n = 0; n < NR; n++) {
+    const int r = ty + n * TY;
+    doy[n] = r < 6;
+    const int yj = !doy[n] ? jc : (r < 3) ? j0 - 3 + r : min(j0 + TY + (r - 3), fb.jhi + 3);
+    tyr[n] = !doy[n] ? 0 : (r < 3) ? r : TY + r;
+    ry[n] = GHOST ? column_rule<T>(ps, fb, g, ic, yj) : ColumnRule{ps.off(ic, yj), 0, 0, 0, 0};
+  }
+  const int txl = (tx < 3) ? tx : TX + tx;
+
+  const Plane<T> vx = plane_of<T>(a.vx), vy = plane_of<T>(a.vy), vz = plane_of<T>(a.vz);
+  const Plane<T> pc = plane_of<T>(a.pc), p0 = plane_of<T>(a.phi0);
+  const int ovel = (KIND == STAGE_ADVECT_FIELD) ? vx.off(ic, jc) : 0;
+  const int opc = a.pc.p ? pc.off(ic, jc) : 0;
+  const int op0 = (KIND == STAGE_REINIT && a.use_phi0) ? p0.off(ic, jc) : 0;
+  T* const outp = a.out.p;
+  const int oout = (i - a.out.ilo) + (int)a.out.sy * (j - a.out.jlo);
+  const T tiny_dx2x = T(12) * Tol<T>::tiny / (a.ix * a.ix);
+  const T tiny_dx2y = T(12) * Tol<T>::tiny / (a.iy * a.iy);
+  const T tiny_dx2z = T(12) * Tol<T>::tiny / (a.iz * a.iz);
+
+  T zq[7];
+#pragma unroll
+  for (int q = 0; q < 6; q++) zq[q] = column_at<T, GHOST>(rc, ps, fb, g, k0 - 3 + q);
+
+  for (int k = k0; k <= k1; k++) {
+    const T* pl = ps.plane(k);
+    zq[6] = column_at<T, GHOST>(rc, ps, fb, g, k + 3);
+    const T hx = do_x ? column_load<T, GHOST>(rx, pl) : T(0);
+    T hy[NR];
+#pragma unroll
+    for (int n = 0; n < NR; n++) hy[n] = doy[n] ? column_load<T, GHOST>(ry[n], pl) : T(0);
+    T ux = T(0), uy = T(0), uz = T(0), pcv = T(0), p0v = T(0);
+    if (KIND == STAGE_ADVECT_FIELD) {
+      ux = vx.plane(k)[ovel];
+      uy = vy.plane(k)[ovel];
+      uz = vz.plane(k)[ovel];
+    } else if (KIND == STAGE_ADVECT_CONST) {
+      ux = a.cvx;
+      uy = a.cvy;
+      uz = a.cvz;
+    } else if (a.use_phi0) {
+      p0v = p0.plane(k)[op0];
+    }
+    if (a.pc.p) pcv = pc.plane(k)[opc];
+    __syncthreads();  // the previous plane's LDS reads are done
+    tile[ty + 3][tx + 3] = zq[3];
+    if (do_x) tile[ty + 3][txl] = hx;
+#pragma unroll
+    for (int n = 0; n < NR; n++)
+      if (doy[n]) tile[tyr[n]][tx + 3] = hy[n];
+    __syncthreads();
+
+    if (active) {
+      T px[7], py[7];
+#pragma unroll
+      for (int q = 0; q < 7; q++) {
+        px[q] = tile[ty + 3][tx + q];
+        py[q] = tile[ty + q][tx + 3];
+      }
+      const T us = zq[3];
+      const T uc = a.pc.p ? pcv : us;
+      T rhs;
+      if (KIND == STAGE_REINIT) {
+        T xp, xm, yp, ym, zp, zm;
+        hj_weno5_pm<T, FASTW>(px, a.ix, tiny_dx2x, xp, xm);
+        hj_weno5_pm<T, FASTW>(py, a.iy, tiny_dx2y, yp, ym);
+        hj_weno5_pm<T, FASTW>(zq, a.iz, tiny_dx2z, zp, zm);
+        // lsm_reinitialization3d.f.in:150-268
+        const T cur = a.use_phi0 ? p0v : us;
+        rhs = T(0);
+        if (fabs(cur) >= Tol<T>::zero) {
+          const T n2 = godunov_star_sq<T>(cur, xp, yp, zp, xm, ym, zm);
+          const T sgn = a.use_phi0 ? cur / sqrt(cur * cur + a.dx_sq) : cur / sqrt(cur * cur + n2 * a.dx_sq);
+          rhs = sgn * (T(1) - sqrt(n2));
+        }
+      } else {
+        const T gx = upwind_weno5<T, FASTW>(px, ux, a.ix, tiny_dx2x);
+        const T gy = upwind_weno5<T, FASTW>(py, uy, a.iy, tiny_dx2y);
+        const T gz = upwind_weno5<T, FASTW>(zq, uz, a.iz, tiny_dx2z);
+        // lsm_level_set_evolution3d.f.in:113-116 on a zeroed rhs
+        rhs = -(ux * gx + uy * gy + uz * gz);
+      }
+      const T v = rk_combine<T>(a.form, uc, us, rhs, a.dt);
+      if (a.periodic) store_with_periodic_images<T>(a.out, fb, i, j, k, v, a.periodic, 3);
+      else (outp + a.out.sz * (int64_t)(k - a.out.klo))[oout] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) zq[q] = zq[q + 1];
+  }
+}
+
+// Host-side pieces shared by the fused-stage and patch-list entry points
+// (defined in lsm_fused.hip).
+struct TileShape {
+  int tx, ty;
+};
+struct Tiling {
+  int cfg;  // tile shape index
+  int kz;   // planes per workgroup
+};
+Tiling current_tiling();
+TileShape tile_shape(int cfg);
+int tiling_count();
+bool valid_form(int form);
+bool device_only(const char* fn, const void* p, const char* what);
+GhostRule make_ghost_rule(const char* fn, const int* bc, const Box& fb);
+
+// True when no stencil read of tile (bx, by, bz) leaves the fill box, so the
+// march needs no ghost values at all (uniform per workgroup).
+template <int TX, int TY>
+__device__ __forceinline__ bool tile_is_interior(const Box& fb, const TileGrid& tg, int bx, int by, int bz) {
+  const int i0 = fb.ilo + bx * TX, j0 = fb.jlo + by * TY, k0 = fb.klo + bz * tg.kz;
+  const int k1 = min(k0 + tg.kz - 1, fb.khi);
+  return i0 - 3 >= fb.ilo && i0 + TX + 2 <= fb.ihi && j0 - 3 >= fb.jlo && j0 + TY + 2 <= fb.jhi &&
+         k0 - 3 >= fb.klo && k1 + 3 <= fb.khi;
+}
+
+// Decomposes a logical tile index of one fill box into (bx, by, bz).
+__device__ __forceinline__ void tile_coords(const TileGrid& tg, int b, int& bx, int& by, int& bz) {
+  bx = b % tg.ntx;
+  const int rem = b / tg.ntx;
+  by = rem % tg.nty;
+  bz = rem / tg.nty;
 }
 
 }  // namespace lsm

@@ -1,0 +1,150 @@
+"""A level of uneven patches advanced with one launch per Runge-Kutta stage.
+
+`random_patch_level(...)` builds the batched workload: patches with extents
+drawn uniformly per axis (seeded), 3 ghost layers, spacing dx, origins
+U[0, 1 - n dx], phi = minimum of 1-4 sphere signed distances (centres
+U[0,1]^3, radii U[0.05, 0.3]) and a constant velocity U[-1,1]^3 per patch.
+
+`PatchLevel` stores every field of all patches in one pool tensor (each
+patch's ghost box contiguous, column-major) and drives the plan API of
+include/lsm3d_amd.h: per-stage launches, or a captured hipGraph of several
+RK3 steps.  Ghost values come from per-face extrapolation synthesised inside
+the kernel (default signed linear, as LSM3D_SIGNED_LINEAR_EXTRAPOLATION fills
+them between the reference's per-patch stages).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from ._native import toolbox
+from .pylsmlib import BC_SIGNED_LINEAR, RK1, TVDRK3_STAGE2, TVDRK3_STAGE3, box_c
+
+_DT = {"double": torch.float64, "float": torch.float32}
+
+
+def random_patch_level(npatch=512, lo=16, hi=96, seed=3, dx=1.0 / 128, ghost=3):
+    """Patch boxes, velocities and sphere sets (pure description, no arrays)."""
+    rng = np.random.default_rng(seed)
+    ext = rng.integers(lo, hi + 1, size=(npatch, 3))
+    origins = rng.uniform(0.0, 1.0 - ext * dx)
+    vel = rng.uniform(-1.0, 1.0, size=(npatch, 3))
+    spheres = []
+    for _ in range(npatch):
+        k = int(rng.integers(1, 5))
+        spheres.append((rng.uniform(0, 1, size=(k, 3)), rng.uniform(0.05, 0.3, size=k)))
+    return {"ext": ext, "origins": origins, "vel": vel, "spheres": spheres, "dx": dx, "ghost": ghost}
+
+
+class PatchLevel:
+    def __init__(self, desc, real="double", device="cuda", bc=(BC_SIGNED_LINEAR,) * 6, tiling=(0, 32)):
+        self.real = real
+        self.dtype = _DT[real]
+        self.device = torch.device(device)
+        self.lib = toolbox(real)
+        self.desc = desc
+        g = desc["ghost"]
+        ext = desc["ext"]
+        self.npatch = len(ext)
+        # index boxes: patch q covers cells 0..n-1 per axis (its own index space)
+        self.fbs = np.zeros((self.npatch, 6), dtype=np.int32)
+        self.gbs = np.zeros((self.npatch, 6), dtype=np.int32)
+        for q, (nx, ny, nz) in enumerate(ext):
+            self.fbs[q] = (0, nx - 1, 0, ny - 1, 0, nz - 1)
+            self.gbs[q] = (-g, nx - 1 + g, -g, ny - 1 + g, -g, nz - 1 + g)
+        sizes = np.prod(ext + 2 * g, axis=1).astype(np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        self.pool_elems = int(sizes.sum())
+        self.sizes = sizes
+        self.cells = int(np.prod(ext, axis=1).sum())
+        self.bc = box_c(bc)
+        self.dx = desc["dx"]
+        ct = ctypes.c_double if real == "double" else ctypes.c_float
+        vel = np.ascontiguousarray(desc["vel"], dtype=np.float64 if real == "double" else np.float32)
+        self._vel = vel
+        cfg, kz = tiling
+        self.plan = self.lib.fused(
+            "lsm3d_amd_patch_plan_create", self.npatch,
+            self.fbs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            self.gbs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            self.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), self.pool_elems,
+            vel.ctypes.data_as(ctypes.POINTER(ct)), int(cfg), int(kz))
+        if not self.plan:
+            raise RuntimeError("patch plan creation failed")
+        self.lib.set_sync(False)
+        self.lib.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def __del__(self):
+        try:
+            if getattr(self, "plan", None):
+                self.lib.lib.lsm3d_amd_patch_plan_destroy(self.plan)
+        except Exception:
+            pass
+
+    # -- fields ------------------------------------------------------------
+    def empty_pool(self):
+        return torch.empty(self.pool_elems, dtype=self.dtype, device=self.device)
+
+    def patch_view(self, pool, q):
+        nx, ny, nz = self.desc["ext"][q]
+        g = self.desc["ghost"]
+        o, n = int(self.offsets[q]), int(self.sizes[q])
+        return pool[o:o + n].view(nz + 2 * g, ny + 2 * g, nx + 2 * g)
+
+    def initial_phi(self):
+        """phi = min over the patch's spheres of the signed distance, at cell
+        centres x = origin + (i + 1/2) dx, over the ghost box."""
+        pool = torch.empty(self.pool_elems, dtype=torch.float64, device=self.device)
+        g, h = self.desc["ghost"], self.dx
+        for q in range(self.npatch):
+            nx, ny, nz = self.desc["ext"][q]
+            o = self.desc["origins"][q]
+            ax = [o[d] + (torch.arange(-g, n + g, dtype=torch.float64, device=self.device) + 0.5) * h
+                  for d, n in enumerate((nx, ny, nz))]
+            Z, Y, X = torch.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+            phi = torch.full_like(X, float("inf"))
+            for c, r in zip(*self.desc["spheres"][q]):
+                phi = torch.minimum(phi, torch.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) - r)
+            self.patch_view(pool, q).copy_(phi)
+        return pool.to(self.dtype)
+
+    def stable_dt(self, cfl=0.5):
+        """min over patches of LSM3D_COMPUTE_STABLE_ADVECTION_DT with the
+        patch's constant velocity (lsm_utilities3d.f.in:297-323), the global
+        dt the reference takes by a min-reduction over patches
+        (LevelSetFunctionIntegrator.cc:567)."""
+        np_t = np.float64 if self.real == "double" else np.float32
+        inv = np_t(1) / np_t(self.dx)
+        v = np.abs(self._vel.astype(np_t))
+        m = ((v[:, 0] * inv + v[:, 1] * inv) + v[:, 2] * inv).max()
+        tiny = 1e-99 if self.real == "double" else np.float32(1e-35)
+        return float(np_t(cfl) / (np_t(max(m, -1.0)) + np_t(tiny)))
+
+    # -- stepping ------------------------------------------------------------
+    def stage(self, form, out, ps, pc, dt):
+        h = self.dx
+        self.lib.fused("lsm3d_amd_patch_plan_stage", self.plan, form, ctypes.c_void_p(out.data_ptr()),
+                       ctypes.c_void_p(ps.data_ptr()), ctypes.c_void_p(pc.data_ptr()) if pc is not None else None,
+                       h, h, h, dt, self.bc)
+
+    def rk3_step(self, phi, s1, s2, dt):
+        self.stage(RK1, s1, phi, None, dt)
+        self.stage(TVDRK3_STAGE2, s2, s1, phi, dt)
+        self.stage(TVDRK3_STAGE3, phi, s2, phi, dt)
+
+    def capture_rk3(self, nsteps, phi, s1, s2, dt):
+        h = self.dx
+        self.lib.fused("lsm3d_amd_patch_plan_capture_rk3", self.plan, int(nsteps),
+                       ctypes.c_void_p(phi.data_ptr()), ctypes.c_void_p(s1.data_ptr()),
+                       ctypes.c_void_p(s2.data_ptr()), h, h, h, dt, self.bc)
+
+    def replay(self):
+        self.lib.fused("lsm3d_amd_patch_plan_replay", self.plan)
+
+    @property
+    def tiles(self):
+        return self.lib.lib.lsm3d_amd_patch_plan_tiles(self.plan)
+
+    @property
+    def workgroups(self):
+        return self.lib.lib.lsm3d_amd_patch_plan_workgroups(self.plan)

@@ -95,23 +95,37 @@ def _ptr(t):
 class Stepper:
     """Device-resident TVD Runge-Kutta drivers for one grid and precision.
 
-    bc: 6 boundary-condition codes (x_lo, x_hi, y_lo, y_hi, z_lo, z_hi).  When
-    every face is periodic the stages write their own periodic ghost images
-    and no separate fill kernel runs.
+    bc: 6 boundary-condition codes (x_lo, x_hi, y_lo, y_hi, z_lo, z_hi).
+    ghosts="auto" (default) picks "images" for all-periodic boundaries and
+    "synth" otherwise.  ghosts="synth": each stage synthesises the ghost values it
+    reads from the fill box, so a step is exactly three launches and ghost
+    cells of the outputs are not maintained.  ghosts="fill": a device ghost
+    fill runs after every stage and the stages read stored ghost cells (the
+    reference's fill-then-differentiate order).  ghosts="images" (all faces
+    periodic only): the stages write the periodic ghost images of their
+    output and read stored ghosts.  Results are identical in every mode.
     """
 
-    def __init__(self, grid, real="double", bc=(BC_PERIODIC,) * 6, device="cuda"):
+    def __init__(self, grid, real="double", bc=(BC_PERIODIC,) * 6, device="cuda", ghosts="auto"):
+        if ghosts == "auto":  # the fastest mode for the boundary conditions
+            ghosts = "images" if all(int(b) == BC_PERIODIC for b in bc) else "synth"
+        if ghosts not in ("synth", "fill", "images"):
+            raise ValueError(ghosts)
         self.grid = grid
         self.real = real
         self.dtype = _DTYPES[real]
         self.lib = toolbox(real)
         self.bc = tuple(int(b) for b in bc)
         self.device = torch.device(device)
+        self.ghosts = ghosts
         self._gb = box_c(grid.gb)
         self._fb = box_c(grid.fb)
         self._bc = box_c(self.bc)
         self.all_periodic = all(b == BC_PERIODIC for b in self.bc)
-        self.periodic_mask = 7 if self.all_periodic else 0
+        if ghosts == "images" and not self.all_periodic:
+            raise ValueError("ghosts='images' needs periodic faces")
+        self._stage_bc = self._bc if ghosts == "synth" else None
+        self.periodic_mask = 7 if ghosts == "images" else 0
         self.sync(False)
 
     def sync(self, flag):
@@ -125,7 +139,7 @@ class Stepper:
         self.lib.fused("lsm3d_amd_fill_ghosts", _ptr(phi), self._gb, self._fb, self._bc)
 
     def _prefill(self, phi):
-        if not self.all_periodic:
+        if self.ghosts == "fill":
             self.fill_ghosts(phi)
 
     # -- advection ---------------------------------------------------------
@@ -134,11 +148,12 @@ class Stepper:
         if isinstance(vel, (tuple, list)) and len(vel) == 3 and torch.is_tensor(vel[0]):
             self.lib.fused("lsm3d_amd_advect_stage", form, _ptr(out), self._gb, _ptr(ps), self._gb,
                            _ptr(pc), self._gb, _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), self._gb,
-                           self._fb, dx, dy, dz, dt, self.periodic_mask)
+                           self._fb, dx, dy, dz, dt, self._stage_bc, self.periodic_mask)
         else:
             u, v, w = (float(c) for c in vel)
             self.lib.fused("lsm3d_amd_advect_const_stage", form, _ptr(out), self._gb, _ptr(ps), self._gb,
-                           _ptr(pc), self._gb, u, v, w, self._fb, dx, dy, dz, dt, self.periodic_mask)
+                           _ptr(pc), self._gb, u, v, w, self._fb, dx, dy, dz, dt, self._stage_bc,
+                           self.periodic_mask)
 
     def advect_rk3_step(self, phi, s1, s2, vel, dt):
         """phi <- one WENO5 / TVD-RK3 step of phi_t + V.grad(phi) = 0 (in place).
@@ -161,7 +176,7 @@ class Stepper:
         dx, dy, dz = self.grid.dx
         self.lib.fused("lsm3d_amd_reinit_stage", form, _ptr(out), self._gb, _ptr(ps), self._gb, _ptr(pc),
                        self._gb, _ptr(phi0) if phi0 is not None else None, self._gb, int(use_phi0),
-                       self._fb, dx, dy, dz, dt, self.periodic_mask)
+                       self._fb, dx, dy, dz, dt, self._stage_bc, self.periodic_mask)
 
     def reinit_rk3_iteration(self, phi, s1, s2, dt, phi0=None, use_phi0=False):
         """One TVD-RK3 pseudo-time iteration of the reinitialization equation."""

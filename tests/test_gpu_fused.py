@@ -40,9 +40,10 @@ def small_periodic(n=(24, 20, 18), dtype=torch.float64):
     return grid, phi, vel
 
 
-def run_both(grid, phi0, vel, bc, steps, real, dt, mode="advect", use_phi0=False, vel_const=None):
+def run_both(grid, phi0, vel, bc, steps, real, dt, mode="advect", use_phi0=False, vel_const=None,
+             ghosts="synth"):
     dev = torch.device("cuda:0")
-    st = Stepper(grid, real, bc)
+    st = Stepper(grid, real, bc, ghosts=ghosts)
     phi = phi0.to(dev).clone()
     s1, s2 = st.empty(), st.empty()
     v = vel_const if vel_const is not None else (tuple(a.to(dev) for a in vel) if vel is not None else None)
@@ -78,18 +79,33 @@ def run_both(grid, phi0, vel, bc, steps, real, dt, mode="advect", use_phi0=False
     return grid.interior(_np(phi)), grid.interior(ph)
 
 
-def test_advect_rk3_periodic_fp64():
+@pytest.mark.parametrize("ghosts", ["synth", "fill", "images"])
+def test_advect_rk3_periodic_fp64(ghosts):
     grid, phi, vel = small_periodic()
     dt = 0.5 * min(grid.dx)
-    got, want = run_both(grid, phi, vel, (BC_PERIODIC,) * 6, 4, "double", dt)
+    got, want = run_both(grid, phi, vel, (BC_PERIODIC,) * 6, 4, "double", dt, ghosts=ghosts)
     assert _rel(got, want) <= FP64_TOL
 
 
-def test_advect_rk3_signed_linear_fp64():
+@pytest.mark.parametrize("ghosts", ["synth", "fill"])
+def test_advect_rk3_signed_linear_fp64(ghosts):
     grid, phi, vel, bc = rotating_sphere(20)
     dt = 0.5 * grid.dx[0] / 1.8
-    got, want = run_both(grid, phi, vel, bc, 4, "double", dt)
+    got, want = run_both(grid, phi, vel, bc, 4, "double", dt, ghosts=ghosts)
     assert _rel(got, want) <= FP64_TOL
+
+
+@pytest.mark.parametrize("real", ["double", "float"])
+def test_synthesised_ghosts_equal_filled_ghosts(real):
+    """Stages that synthesise ghost values give bit-identical steps to stages
+    that read ghosts filled by the device extrapolation kernel."""
+    grid, phi, vel, bc = rotating_sphere(19)
+    dt = 0.01
+    dt_ = torch.float64 if real == "double" else torch.float32
+    outs = []
+    for ghosts in ("synth", "fill"):
+        outs.append(run_both(grid, phi.to(dt_), tuple(v.to(dt_) for v in vel), bc, 2, real, dt, ghosts=ghosts)[0])
+    assert np.array_equal(outs[0], outs[1])
 
 
 @pytest.mark.parametrize("bc", [BC_LINEAR, BC_NEUMANN])
@@ -121,10 +137,12 @@ def test_reinit_rk3_fp64(use_phi0):
     assert _rel(got, want) <= FP64_TOL
 
 
-def test_advect_fp32_vs_fp64_oracle():
+@pytest.mark.parametrize("ghosts", ["synth", "images"])
+def test_advect_fp32_vs_fp64_oracle(ghosts):
     grid, phi, vel = small_periodic()
     dt = 0.5 * min(grid.dx)
-    got, want = run_both(grid, phi.float(), tuple(v.float() for v in vel), (BC_PERIODIC,) * 6, 4, "float", dt)
+    got, want = run_both(grid, phi.float(), tuple(v.float() for v in vel), (BC_PERIODIC,) * 6, 4, "float", dt,
+                         ghosts=ghosts)
     err = np.abs(got.astype(np.float64) - want)
     near = np.abs(want) < 3 * min(grid.dx)
     assert err.max() <= 1e-5 * np.abs(want).max()
@@ -145,23 +163,26 @@ def test_every_tiling_gives_identical_results(real):
     dev = torch.device("cuda:0")
     grid, phi, vel = small_periodic((37, 29, 23), torch.float64 if real == "double" else torch.float32)
     lib = toolbox(real)
-    st = Stepper(grid, real)
+    st = Stepper(grid, real, ghosts="synth")
     vel = tuple(v.to(dev) for v in vel)
     ref = None
-    for cfg in range(lib.lib.lsm3d_amd_stage_tiling_count()):
-        for kz in (1, 7, 64):
-            lib.lib.lsm3d_amd_set_stage_tiling(cfg, kz)
-            p = phi.to(dev).clone()
-            s1, s2 = st.empty(), st.empty()
-            st.advect_rk3_step(p, s1, s2, vel, 0.01)
-            st.reinit_rk3_iteration(p, s1, s2, 0.005)
-            torch.cuda.synchronize()
-            out = _np(p)
-            if ref is None:
-                ref = out
-            else:
-                assert np.array_equal(out, ref), f"tiling {cfg}/{kz} differs"
-    lib.lib.lsm3d_amd_set_stage_tiling(0, 32)
+    try:
+        for cfg in range(lib.lib.lsm3d_amd_stage_tiling_count()):
+            for kz in (1, 7, 64):
+                lib.lib.lsm3d_amd_set_stage_tiling(cfg, kz)
+                p = phi.to(dev).clone()
+                s1, s2 = st.empty(), st.empty()
+                st.advect_rk3_step(p, s1, s2, vel, 0.01)
+                st.reinit_rk3_iteration(p, s1, s2, 0.005)
+                torch.cuda.synchronize()
+                out = _np(p)
+                assert np.isfinite(out).all(), f"tiling {cfg}/{kz} left non-finite values"
+                if ref is None:
+                    ref = out
+                else:
+                    assert np.array_equal(out, ref), f"tiling {cfg}/{kz} differs"
+    finally:
+        lib.lib.lsm3d_amd_set_stage_tiling(0, 32)
 
 
 def test_dropin_chain_on_device_matches_fused():
