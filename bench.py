@@ -47,6 +47,8 @@ def parse():
                     help="synth: stages synthesise periodic ghost values; images: stages write and read "
                          "stored periodic ghost images")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 2 GiB device-to-device copy (scale for HBM byte counters)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-patch and reinitialization measurements")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -244,6 +246,16 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a visible GPU", file=sys.stderr)
         sys.exit(1)
+    if a.copy_calibration:
+        n = (2 << 30) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n,
+                          "launches": 3}), flush=True)
+        return
 
     r32, grid, phi32 = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts)
     print(f"# fp32: {r32['value']:.4e} cell-updates/s, {r32['ms_per_step']:.3f} ms/step, "
