@@ -51,6 +51,9 @@ def parse():
                     help="only run a plain 2 GiB device-to-device copy (scale for HBM byte counters)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-patch and reinitialization measurements")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (adjoint) measurement")
+    ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the gradient")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the gradient measurement")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
 
@@ -182,6 +185,110 @@ def run_reinit(n=256, iters=50):
     return out
 
 
+def run_adjoint(n=128, nsteps=40, every=8, reps=3, fd=True):
+    """Gradient workload: 128^3 periodic [-1,1]^3, phi0 = |x| - 0.3, velocity
+    sum_m P(k_m) c_m sin(pi k_m.x + theta_m) with 64 integer wavevectors
+    (1 <= max|k_i| <= 4), c ~ N(0, 0.1^2) (seed 4); dt frozen at the CFL-0.5
+    LSM3D_COMPUTE_STABLE_ADVECTION_DT of the initial velocity; 40 WENO5 /
+    TVD-RK3 steps; loss sum (H_eps(phi_T) - H_eps(phi_tgt))^2, eps = 1.5 dx,
+    phi_tgt advected with c + delta, delta ~ N(0, 0.02^2) (seed 9);
+    gradients with respect to c (64 x 3) and phi0, checkpoint every 8 steps.
+
+    adjoint_s: wall time of one forward + backward in FP32 (velocity from c,
+    40 steps, loss, reverse sweep, chain to c), best of `reps`.  The FP64
+    gradient is the reference for the FP32 one; central differences of the
+    FP64 loss check the FP64 gradient along 8 random unit directions in c,
+    the 8 coefficients with the largest |dL/dc|, and 2 smooth phi0 bumps."""
+    import numpy as np
+    import torch
+    from lsmlib_amd import toolbox
+    from lsmlib_amd.adjoint import AdvectionRK3, advect_weno5_rk3, heaviside_mismatch
+    from lsmlib_amd.workloads import adjoint_case
+    dev = torch.device("cuda:0")
+    grid, phi0, fv, _ = adjoint_case(n, torch.float64, dev)
+    c0 = fv.c0.to(dev)
+    v0 = fv.velocity(c0)
+    dt = float(toolbox("double").lsm3dcomputestableadvectiondt_(v0[0].contiguous(), v0[1].contiguous(),
+                                                               v0[2].contiguous(), grid.gb, grid.fb,
+                                                               *grid.dx, 0.5))
+    del v0
+    rng = np.random.default_rng(9)
+    delta = torch.tensor(rng.normal(0.0, 0.02, size=tuple(c0.shape)), dtype=torch.float64, device=dev)
+    eps = 1.5 * grid.dx[0]
+    out = {"config": f"{n}^3 periodic [-1,1]^3, sphere r=0.3, 64x3 Fourier modes |k|_inf 1..4, "
+                     f"{nsteps} WENO5/TVD-RK3 steps, dt={dt:.6g} frozen, checkpoint every {every}, "
+                     f"loss sum (H(phi_T)-H(phi_tgt))^2 eps=1.5dx"}
+    grads, losses = {}, {}
+    for real, dtype in (("double", torch.float64), ("float", torch.float32)):
+        runner = AdvectionRK3(grid, real, dev, checkpoint_every=every)
+        with torch.no_grad():
+            vt = tuple(v.to(dtype).contiguous() for v in fv.velocity(c0 + delta))
+            target, _ = runner.run(phi0.to(dtype), vt, nsteps, dt, keep_checkpoints=False)
+            del vt
+
+        def loss_of(p0, c):
+            vel = tuple(v.to(dtype).contiguous() for v in fv.velocity(c))
+            phi = advect_weno5_rk3(p0, vel, runner, nsteps, dt)
+            return heaviside_mismatch(phi, target, grid, eps)
+
+        def fwd_bwd():
+            c = c0.clone().requires_grad_(True)
+            p0 = phi0.to(dtype).clone().requires_grad_(True)
+            L = loss_of(p0, c)
+            L.backward()
+            return L, c.grad, p0.grad
+
+        fwd_bwd()
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            L, gc, gp = fwd_bwd()
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            best = el if best is None else min(best, el)
+        grads[real] = (gc.double(), gp.double())
+        losses[real] = float(L.detach())
+        out[real] = {"forward_backward_s": best, "loss": float(L.detach())}
+        if real == "double" and fd:
+            checks = []
+            with torch.no_grad():
+                def fd_dir(dc=None, dp=None, h=1e-4):
+                    cp = c0 + h * dc if dc is not None else c0
+                    cm = c0 - h * dc if dc is not None else c0
+                    pp = phi0 + h * dp if dp is not None else phi0
+                    pm = phi0 - h * dp if dp is not None else phi0
+                    return (float(loss_of(pp, cp)) - float(loss_of(pm, cm))) / (2 * h)
+                for _ in range(8):
+                    e = torch.tensor(rng.normal(size=tuple(c0.shape)), dtype=torch.float64, device=dev)
+                    e /= e.norm()
+                    ad = float((gc * e).sum())
+                    checks.append(("c_random", abs(fd_dir(dc=e) - ad) / abs(ad)))
+                top = torch.argsort(gc.abs().flatten(), descending=True)[:8]
+                for idx in top.tolist():
+                    e = torch.zeros_like(c0).flatten()
+                    e[idx] = 1.0
+                    e = e.reshape(c0.shape)
+                    ad = float(gc.flatten()[idx])
+                    checks.append((f"c[{idx // 3},{idx % 3}]", abs(fd_dir(dc=e) - ad) / abs(ad)))
+                X, Y, Z = grid.centers(torch.float64, dev)
+                for w in ((1.0, 2.0, 0.5), (3.0, 1.0, 2.0)):
+                    e = torch.sin(np.pi * (w[0] * X + w[1] * Y) + w[2] * Z) * torch.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 0.2)
+                    ad = float((gp * e).sum())
+                    checks.append(("phi0_bump", abs(fd_dir(dp=e) - ad) / abs(ad)))
+                del X, Y, Z
+            out["fd_check_fp64"] = {"max_rel_err": max(v for _, v in checks), "n_directions": len(checks),
+                                    "h": 1e-4, "per_direction": {f"{k}#{q}": v for q, (k, v) in enumerate(checks)}}
+        del runner, target
+        torch.cuda.empty_cache()
+    out["fp32_grad_rel_err_vs_fp64"] = {
+        "c": float((grads["float"][0] - grads["double"][0]).norm() / grads["double"][0].norm()),
+        "phi0": float((grads["float"][1] - grads["double"][1]).norm() / grads["double"][1].norm())}
+    out["adjoint_s"] = out["float"]["forward_backward_s"]
+    out["dt"] = dt
+    return out
+
+
 def fp32_error(grid, phi32, phi64):
     import torch
     a = grid.interior(phi32).to(torch.float64)
@@ -257,6 +364,9 @@ def main():
                           "launches": 3}), flush=True)
         return
 
+    if a.adjoint_only:
+        print(json.dumps({"adjoint": run_adjoint(fd=not a.no_fd)}), flush=True)
+        return
     r32, grid, phi32 = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts)
     print(f"# fp32: {r32['value']:.4e} cell-updates/s, {r32['ms_per_step']:.3f} ms/step, "
           f"kernel {r32['kernel_ms_avg']:.3f} ms, {r32['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
@@ -284,6 +394,13 @@ def main():
         extras["reinit_256"] = run_reinit()
         print(f"# reinit: fp64 {extras['reinit_256']['double']['ms_per_iteration']:.2f} ms/it, fp32 "
               f"{extras['reinit_256']['float']['ms_per_iteration']:.2f} ms/it", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+    adj = None
+    if not a.no_adjoint:
+        adj = run_adjoint(fd=not a.no_fd)
+        print(f"# adjoint: fp32 {adj['adjoint_s']:.3f} s fwd+bwd, fp64 {adj['double']['forward_backward_s']:.3f} s, "
+              f"fp32 grad err {adj['fp32_grad_rel_err_vs_fp64']}, fd {adj.get('fd_check_fp64', {}).get('max_rel_err')}",
+              file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
 
     line = {
@@ -314,6 +431,8 @@ def main():
                      "kernel_ms_avg": r32["kernel_ms_avg"],
                      "algorithmic_bytes_per_launch": r32["algorithmic_bytes_per_launch"]},
         "cpu_baseline": cpu,
+        "adjoint_s": None if adj is None else adj["adjoint_s"],
+        "adjoint": adj,
         "extras": extras or None,
     }
     print(json.dumps(line), flush=True)

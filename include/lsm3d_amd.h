@@ -433,6 +433,44 @@ void lsm3d_amd_patch_plan_capture_rk3(lsm3d_amd_patch_plan *plan, int nsteps, LS
                                       const int bc[6]);
 void lsm3d_amd_patch_plan_replay(lsm3d_amd_patch_plan *plan);
 
+/* ======================================================================
+ * Part 4 - reverse mode (no counterpart in the reference, which has no
+ * adjoint; these serve gradient-based callers of the WENO5 / TVD-RK
+ * advection path).  Device memory only.
+ * ====================================================================== */
+
+/* Elements of the work array the reverse stage needs for fill box fb
+ * (18 per cell). */
+long long lsm3d_amd_advect_adjoint_work_elems(const int fb[6]);
+
+/* Reverse of lsm3d_amd_advect_stage(rk_form, ...) on a fill box periodic on
+ * every axis (periodic_mask must be 7; stored ghost cells are not read).
+ * With out = a_c phi_cur + a_s (phi_stage + dt r(phi_stage, V)) and adj_out
+ * the adjoint of out:
+ *   adj_stage (=|+=) a_s (adj_out + dt (dr/dphi_stage)^T adj_out)
+ *   adj_cur   (=|+=) a_c adj_out                    (may be NULL for RK1)
+ *   vel_adj_{x,y,z} += -a_s dt adj_out D_{x,y,z}    (may all be NULL)
+ * accumulate bit 0 selects += for adj_stage, bit 1 for adj_cur.  The upwind
+ * stencil choice is taken from the sign of V as in the forward stage. */
+void lsm3d_amd_advect_stage_adjoint(int rk_form, const LSMLIB_REAL *adj_out, const int adj_out_gb[6],
+                                    LSMLIB_REAL *adj_stage, const int adj_stage_gb[6],
+                                    LSMLIB_REAL *adj_cur, const int adj_cur_gb[6], int accumulate,
+                                    const LSMLIB_REAL *phi_stage, const int stage_gb[6],
+                                    const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y,
+                                    const LSMLIB_REAL *vel_z, const int vel_gb[6],
+                                    LSMLIB_REAL *vel_adj_x, LSMLIB_REAL *vel_adj_y,
+                                    LSMLIB_REAL *vel_adj_z, const int vel_adj_gb[6], LSMLIB_REAL *work,
+                                    const int fb[6], LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz,
+                                    LSMLIB_REAL dt, int periodic_mask);
+
+/* loss[0] = sum over fb of (H(phi) - H(target))^2 (FP64 accumulation) with
+ * the smoothed Heaviside of lsmHeaviside (lsm_calculus_toolbox.f:25-54) of
+ * half-width epsilon; grad (optional) receives d loss / d phi. */
+void lsm3d_amd_heaviside_mismatch(const LSMLIB_REAL *phi, const int phi_gb[6],
+                                  const LSMLIB_REAL *target, const int target_gb[6], const int fb[6],
+                                  LSMLIB_REAL epsilon, double *loss, LSMLIB_REAL *grad,
+                                  const int grad_gb[6]);
+
 #pragma GCC visibility pop
 #ifdef __cplusplus
 }

@@ -76,10 +76,16 @@ class ToolboxLibrary(F77Library):
         L.lsm3d_amd_patch_plan_stage.argtypes = [P, I, P, P, P, R, R, R, R, B]
         L.lsm3d_amd_patch_plan_capture_rk3.argtypes = [P, I, P, P, P, R, R, R, R, B]
         L.lsm3d_amd_patch_plan_replay.argtypes = [P]
+        L.lsm3d_amd_advect_adjoint_work_elems.argtypes = [B]
+        L.lsm3d_amd_advect_adjoint_work_elems.restype = ctypes.c_longlong
+        L.lsm3d_amd_advect_stage_adjoint.argtypes = [I, P, B, P, B, P, B, I, P, B, P, P, P, B, P, P, P, B, P, B,
+                                                     R, R, R, R, I]
+        L.lsm3d_amd_heaviside_mismatch.argtypes = [P, B, P, B, B, R, P, P, B]
         for f in ("lsm3d_amd_advect_stage", "lsm3d_amd_advect_const_stage", "lsm3d_amd_reinit_stage",
                   "lsm3d_amd_fill_ghosts", "lsm3d_amd_set_stage_tiling", "lsm3d_amd_patch_plan_destroy",
                   "lsm3d_amd_patch_plan_stage", "lsm3d_amd_patch_plan_capture_rk3",
-                  "lsm3d_amd_patch_plan_replay"):
+                  "lsm3d_amd_patch_plan_replay", "lsm3d_amd_advect_stage_adjoint",
+                  "lsm3d_amd_heaviside_mismatch"):
             getattr(L, f).restype = None
 
     def fused(self, fname, *args):

@@ -10,6 +10,12 @@
                            distances (radii U[0.02,0.1]), Taylor-Green velocity
                            u=sin2pix cos2piy cos2piz, v=-cos2pix sin2piy cos2piz,
                            w=0 (large advection case)
+  FourierVelocity(grid, ...) / adjoint_case(n)
+                           divergence-free velocity sum_m P(k_m) c_m
+                           sin(pi k_m.x + theta_m), 64 integer wavevectors with
+                           1 <= max|k_i| <= 4, c ~ N(0, 0.1^2), on periodic
+                           [-1,1]^3; phi0 = sphere r=0.3 at the origin
+                           (gradient case)
 """
 import math
 
@@ -69,3 +75,48 @@ def sphere_union_taylor_green(n=512, n_spheres=64, seed=5, dtype=torch.float64, 
     phi = sphere_union(grid, n_spheres, seed, dtype=dtype, device=device)
     vel = taylor_green(grid, dtype, device)
     return grid, phi, vel, (BC_PERIODIC,) * 6
+
+
+class FourierVelocity:
+    """V(x) = sum_m P(k_m) c_m sin(pi k_m . x + theta_m), P(k) = I - k k^T/|k|^2
+    (divergence-free), as (vx, vy, vz) over the grid's ghost box.  The modes
+    (k_m integer, 1 <= max_i |k_mi| <= kmax), phases theta_m ~ U[0, 2 pi)
+    and coefficients c ~ N(0, sigma^2) (n_modes x 3) are drawn from `seed`.
+    velocity(c) is differentiable in c (a product with the stored basis)."""
+
+    def __init__(self, grid, n_modes=64, kmax=4, sigma=0.1, seed=4, dtype=torch.float64, device="cpu"):
+        rng = np.random.default_rng(seed)
+        ks = []
+        while len(ks) < n_modes:
+            k = rng.integers(-kmax, kmax + 1, size=3)
+            if np.abs(k).max() >= 1:
+                ks.append(k)
+        self.k = np.array(ks, dtype=np.int64)
+        self.theta = rng.uniform(0.0, 2 * math.pi, size=n_modes)
+        self.c0 = torch.tensor(rng.normal(0.0, sigma, size=(n_modes, 3)), dtype=torch.float64)
+        self.grid, self.dtype, self.device = grid, dtype, torch.device(device)
+        kf = torch.tensor(self.k, dtype=torch.float64)
+        proj = torch.eye(3, dtype=torch.float64) - kf[:, :, None] * kf[:, None, :] / (kf ** 2).sum(1)[:, None, None]
+        self.proj = proj.to(dtype=dtype, device=self.device)  # (m, 3, 3), symmetric
+        X, Y, Z = grid.centers(torch.float64, device)
+        basis = torch.empty((n_modes,) + tuple(X.shape), dtype=dtype, device=self.device)
+        for m in range(n_modes):
+            kx, ky, kz = (float(v) for v in self.k[m])
+            basis[m] = torch.sin(math.pi * (kx * X + ky * Y + kz * Z) + float(self.theta[m])).to(dtype)
+        self.basis = basis.reshape(n_modes, -1)
+        self.shape = tuple(X.shape)
+
+    def velocity(self, c):
+        """(vx, vy, vz), each contiguous over the ghost box."""
+        q = torch.einsum("mab,mb->ma", self.proj, c.to(self.dtype))  # P(k_m) c_m
+        v = torch.matmul(q.t().contiguous(), self.basis)             # (3, cells)
+        return tuple(v[a].reshape(self.shape) for a in range(3))
+
+
+def adjoint_case(n=128, dtype=torch.float64, device="cpu", seed=4):
+    """Grid [-1,1]^3 periodic, phi0 = |x| - 0.3 and the Fourier velocity."""
+    grid = Grid((n, n, n), (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), 3)
+    X, Y, Z = grid.centers(torch.float64, device)
+    phi0 = (torch.sqrt(X ** 2 + Y ** 2 + Z ** 2) - 0.3).to(dtype).contiguous()
+    fv = FourierVelocity(grid, seed=seed, dtype=dtype, device=device)
+    return grid, phi0, fv, (BC_PERIODIC,) * 6
