@@ -21,6 +21,9 @@
  * counterpart in the reference): a whole TVD-RK stage of the advection or
  * reinitialization equation in one pass, device ghost fills, and a batched
  * patch-list advance.
+ * Part 4 -- reverse mode of the fused advection stage (no counterpart in the
+ * reference) and a smoothed-Heaviside mismatch loss, for gradient-based
+ * callers.
  */
 #ifndef LSM3D_AMD_H
 #define LSM3D_AMD_H
@@ -400,6 +403,14 @@ void lsm3d_amd_fill_ghosts(LSMLIB_REAL *phi, const int gb[6], const int fb[6], c
  * and kz planes per workgroup, for this host thread. */
 void lsm3d_amd_set_stage_tiling(int cfg, int kz);
 int lsm3d_amd_stage_tiling_count(void);
+/* Threshold of the upwind switch (|V| < tol: no advection term) and of the
+ * reinitialization cut-off (|phi| < tol: zero right-hand side) used by the
+ * fused and patch-list stages of the calling host thread.  The default (and
+ * tol < 0) is the build's LSMLIB_ZERO_TOL (1e-11 double, 1e-5 float,
+ * configure.ac:264-272), as in the toolbox routines; the float library
+ * with 1e-11 follows the double build's switch in FP32 arithmetic. */
+void lsm3d_amd_set_zero_tol(double tol);
+double lsm3d_amd_zero_tol(void);
 void lsm3d_amd_stage_tiling_shape(int cfg, int *tx, int *ty);
 
 /* --- patch lists: a level of uneven patches advanced together ---------

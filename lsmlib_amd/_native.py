@@ -76,6 +76,10 @@ class ToolboxLibrary(F77Library):
         L.lsm3d_amd_patch_plan_stage.argtypes = [P, I, P, P, P, R, R, R, R, B]
         L.lsm3d_amd_patch_plan_capture_rk3.argtypes = [P, I, P, P, P, R, R, R, R, B]
         L.lsm3d_amd_patch_plan_replay.argtypes = [P]
+        L.lsm3d_amd_set_zero_tol.argtypes = [ctypes.c_double]
+        L.lsm3d_amd_set_zero_tol.restype = None
+        L.lsm3d_amd_zero_tol.argtypes = []
+        L.lsm3d_amd_zero_tol.restype = ctypes.c_double
         L.lsm3d_amd_advect_adjoint_work_elems.argtypes = [B]
         L.lsm3d_amd_advect_adjoint_work_elems.restype = ctypes.c_longlong
         L.lsm3d_amd_advect_stage_adjoint.argtypes = [I, P, B, P, B, P, B, I, P, B, P, P, P, B, P, P, P, B, P, B,
@@ -115,6 +119,14 @@ class ToolboxLibrary(F77Library):
 
     def set_sync(self, sync):
         self.lib.lsm_amd_set_sync(int(bool(sync)))
+
+    def set_zero_tol(self, tol=None):
+        """Upwind-switch / reinit cut-off of the fused stages on this host
+        thread; None restores the build's LSMLIB_ZERO_TOL."""
+        self.lib.lsm3d_amd_set_zero_tol(-1.0 if tol is None else float(tol))
+
+    def zero_tol(self):
+        return self.lib.lsm3d_amd_zero_tol()
 
 
 def toolbox(real="double"):

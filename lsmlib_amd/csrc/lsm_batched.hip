@@ -36,7 +36,7 @@ template <typename T, int TX, int TY, bool SYNTH>
 __global__ void __launch_bounds__(TX* TY)
     k_patch_stage(const PatchDesc* __restrict__ desc, const int64_t* __restrict__ prefix, int npatch,
                   int64_t ntiles, T* out_pool, const T* ps_pool, const T* pc_pool, GhostRule ghost, T ix, T iy,
-                  T iz, T dt, int form) {
+                  T iz, T dt, T zt, int form) {
   __shared__ T tile[TY + 6][TX + 7];
   const int G = (int)gridDim.x;  // a multiple of the XCD count
   const int b = (int)blockIdx.x;
@@ -67,6 +67,7 @@ __global__ void __launch_bounds__(TX* TY)
     a.iz = iz;
     a.dt = dt;
     a.dx_sq = T(0);
+    a.zero_tol = zt;
     a.form = form;
     a.periodic = 0;
     a.use_phi0 = 0;
@@ -75,7 +76,7 @@ __global__ void __launch_bounds__(TX* TY)
     if (SYNTH && !tile_is_interior<TX, TY>(d.fb, d.tg, bx, by, bz))
       march<T, TX, TY, STAGE_ADVECT_CONST, true, (sizeof(T) == 4)>(a, bx, by, bz, tile);
     else
-      march<T, TX, TY, STAGE_ADVECT_CONST, false, (sizeof(T) == 4)>(a, bx, by, bz, tile);
+      march_stored<T, TX, TY, STAGE_ADVECT_CONST, (sizeof(T) == 4)>(a, bx, by, bz, tile);
   }
 }
 
@@ -105,12 +106,13 @@ typedef real_t R;
 template <int TX, int TY>
 void launch_patch_t(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R* ps, const R* pc,
                     const GhostRule& g, R ix, R iy, R iz, R dt, int form) {
+  const R zt = (R)current_zero_tol();
   if (g.any)
     hipLaunchKernelGGL((k_patch_stage<R, TX, TY, true>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
-                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, form);
+                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, zt, form);
   else
     hipLaunchKernelGGL((k_patch_stage<R, TX, TY, false>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
-                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, form);
+                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, zt, form);
 }
 
 void launch_patch(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R* ps, const R* pc,

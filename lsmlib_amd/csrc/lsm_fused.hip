@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(StageArgs<T> a) {
   if (SYNTH && !tile_is_interior<TX, TY>(a.fb, a.tg, bx, by, bz))
     march<T, TX, TY, KIND, true, (sizeof(T) == 4)>(a, bx, by, bz, tile);
   else
-    march<T, TX, TY, KIND, false, (sizeof(T) == 4)>(a, bx, by, bz, tile);
+    march_stored<T, TX, TY, KIND, (sizeof(T) == 4)>(a, bx, by, bz, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -96,8 +96,13 @@ constexpr TileShape kTiles[] = {{64, 4}, {64, 8}, {32, 8}, {128, 2}, {128, 4}, {
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 thread_local Tiling tl_tiling = {0, 32};
+// Threshold of the upwind switch (|V| < tol -> no advection term) and of the
+// reinitialization cut-off (|phi| < tol -> zero rhs).  The default is the
+// build's LSMLIB_ZERO_TOL (configure.ac:264-272), as in the toolbox routines.
+thread_local double tl_zero_tol = Tol<real_t>::zero;
 
 Tiling current_tiling() { return tl_tiling; }
+double current_zero_tol() { return tl_zero_tol; }
 TileShape tile_shape(int cfg) { return kTiles[cfg]; }
 int tiling_count() { return kNumTiles; }
 
@@ -221,6 +226,7 @@ void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, co
   if (dy > mdx) mdx = dy;
   if (dz > mdx) mdx = dz;
   a.dx_sq = mdx * mdx;  // lsm_reinitialization3d.f.in:140-141
+  a.zero_tol = (R)tl_zero_tol;
   a.form = form;
   a.periodic = periodic;
   a.use_phi0 = use_phi0 ? 1 : 0;
@@ -254,6 +260,10 @@ void lsm3d_amd_set_stage_tiling(int cfg, int kz) {
 }
 
 int lsm3d_amd_stage_tiling_count(void) { return kNumTiles; }
+
+void lsm3d_amd_set_zero_tol(double tol) { tl_zero_tol = tol < 0 ? (double)Tol<real_t>::zero : tol; }
+
+double lsm3d_amd_zero_tol(void) { return tl_zero_tol; }
 
 void lsm3d_amd_stage_tiling_shape(int cfg, int* tx, int* ty) {
   if (cfg < 0 || cfg >= kNumTiles) {

@@ -102,6 +102,7 @@ struct StageArgs {
   TileGrid tg;
   GhostRule ghost;
   T ix, iy, iz, dt, dx_sq;
+  T zero_tol;    // upwind switch / reinit cut-off (LSMLIB_ZERO_TOL of the build by default)
   int form;      // RKForm
   int periodic;  // bit mask: also write phi_out's periodic ghost images
   int use_phi0;  // reinit sign from phi0 (1) or phi_s (0)
@@ -113,7 +114,7 @@ struct StageArgs {
 // the rearranged WENO5 (weno5_fast) instead of the reference-order form.
 // ---------------------------------------------------------------------------
 template <typename T, bool FASTW>
-__device__ __forceinline__ T upwind_weno5(const T* p, T vel, T inv, T tiny_dx2) {
+__device__ __forceinline__ T upwind_weno5(const T* p, T vel, T inv, T tiny_dx2, T zero_tol) {
   T d[6];
 #pragma unroll
   for (int q = 0; q < 6; q++) d[q] = p[q + 1] - p[q];
@@ -123,7 +124,7 @@ __device__ __forceinline__ T upwind_weno5(const T* p, T vel, T inv, T tiny_dx2) 
   T r;
   if (FASTW) r = weno5_fast<T>(D1, D2, D3, D4, D5, tiny_dx2) * inv;
   else r = weno5_ref<T>(D1 * inv, D2 * inv, D3 * inv, D4 * inv, D5 * inv);
-  return (fabs(vel) < Tol<T>::zero) ? T(0) : r;
+  return (fabs(vel) < zero_tol) ? T(0) : r;
 }
 
 template <typename T, bool FASTW>
@@ -327,15 +328,15 @@ __device__ __forceinline__ void march(const StageArgs<T>& a, int bx, int by, int
         // lsm_reinitialization3d.f.in:150-268
         const T cur = a.use_phi0 ? p0v : us;
         rhs = T(0);
-        if (fabs(cur) >= Tol<T>::zero) {
+        if (fabs(cur) >= a.zero_tol) {
           const T n2 = godunov_star_sq<T>(cur, xp, yp, zp, xm, ym, zm);
           const T sgn = a.use_phi0 ? cur / sqrt(cur * cur + a.dx_sq) : cur / sqrt(cur * cur + n2 * a.dx_sq);
           rhs = sgn * (T(1) - sqrt(n2));
         }
       } else {
-        const T gx = upwind_weno5<T, FASTW>(px, ux, a.ix, tiny_dx2x);
-        const T gy = upwind_weno5<T, FASTW>(py, uy, a.iy, tiny_dx2y);
-        const T gz = upwind_weno5<T, FASTW>(zq, uz, a.iz, tiny_dx2z);
+        const T gx = upwind_weno5<T, FASTW>(px, ux, a.ix, tiny_dx2x, a.zero_tol);
+        const T gy = upwind_weno5<T, FASTW>(py, uy, a.iy, tiny_dx2y, a.zero_tol);
+        const T gz = upwind_weno5<T, FASTW>(zq, uz, a.iz, tiny_dx2z, a.zero_tol);
         // lsm_level_set_evolution3d.f.in:113-116 on a zeroed rhs
         rhs = -(ux * gx + uy * gy + uz * gz);
       }
@@ -345,6 +346,185 @@ __device__ __forceinline__ void march(const StageArgs<T>& a, int bx, int by, int
     }
 #pragma unroll
     for (int q = 0; q < 6; q++) zq[q] = zq[q + 1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The same stage for stored ghost cells (no synthesis), with the addressing
+// taken off the vector ALU: every field plane is a buffer resource built
+// from a workgroup-uniform plane pointer (advanced by one plane per step in
+// scalar registers) and read with a 32-bit per-lane byte offset fixed for
+// the whole march.  Halo loads are unconditional (lanes without a halo
+// cell read their own column) so no lane masking is needed, and periodic
+// ghost images are written only by tiles and planes within 3 cells of a
+// periodic face.  Results are bit-identical to march<..., GHOST=false, ...>.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* p) {
+  // raw buffer, no stride, no range limit, 32-bit data format (gfx9 word 3)
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, -1, 0x00020000);
+}
+
+typedef unsigned int lsm_u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(lsm_u32x2, v), r, (int)off, 0, 0);
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t plane_byte_offset(const Field<const T>& f, int i, int j) {
+  return (uint32_t)((i - f.ilo) + (int)f.sy * (j - f.jlo)) * (uint32_t)sizeof(T);
+}
+
+// First byte of plane k of a field, and the byte distance between planes.
+template <typename T>
+__device__ __forceinline__ const char* plane_ptr(const Field<const T>& f, int k) {
+  return reinterpret_cast<const char*>(f.p) + (int64_t)(k - f.klo) * f.sz * (int64_t)sizeof(T);
+}
+
+template <typename T, int TX, int TY, int KIND, bool FASTW>
+__device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int by, int bz, T (*tile)[TX + 7]) {
+  const int t = threadIdx.x;
+  const int tx = t % TX, ty = t / TX;
+  const Box& fb = a.fb;
+  const int i0 = fb.ilo + bx * TX, j0 = fb.jlo + by * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const int k0 = fb.klo + bz * a.tg.kz;
+  const int k1 = min(k0 + a.tg.kz - 1, fb.khi);
+  const bool active = (i <= fb.ihi) && (j <= fb.jhi);
+  const int ic = min(i, fb.ihi + 3), jc = min(j, fb.jhi + 3);  // clamped (unused values)
+  const uint32_t oc = plane_byte_offset<T>(a.ps, ic, jc);
+  const bool do_x = tx < 6;
+  const int xi = (tx < 3) ? i0 - 3 + tx : min(i0 + TX + tx - 3, fb.ihi + 3);
+  const uint32_t ox = do_x ? plane_byte_offset<T>(a.ps, xi, jc) : oc;
+  constexpr int NR = (6 + TY - 1) / TY;
+  uint32_t oy[NR];
+  int tyr[NR];
+  bool doy[NR];
+#pragma unroll
+  for (int n = 0; n < NR; n++) {
+    const int r = ty + n * TY;
+    doy[n] = r < 6;
+    const int yj = !doy[n] ? jc : (r < 3) ? j0 - 3 + r : min(j0 + TY + (r - 3), fb.jhi + 3);
+    tyr[n] = !doy[n] ? 0 : (r < 3) ? r : TY + r;
+    oy[n] = plane_byte_offset<T>(a.ps, ic, yj);
+  }
+  const int txl = (tx < 3) ? tx : TX + tx;
+  const uint32_t ov = (KIND == STAGE_ADVECT_FIELD) ? plane_byte_offset<T>(a.vx, ic, jc) : 0u;
+  const uint32_t opc = a.pc.p ? plane_byte_offset<T>(a.pc, ic, jc) : 0u;
+  const uint32_t op0 = (KIND == STAGE_REINIT && a.use_phi0) ? plane_byte_offset<T>(a.phi0, ic, jc) : 0u;
+  const uint32_t oo = (uint32_t)((ic - a.out.ilo) + (int)a.out.sy * (jc - a.out.jlo)) * (uint32_t)sizeof(T);
+  const T tiny_dx2x = T(12) * Tol<T>::tiny / (a.ix * a.ix);
+  const T tiny_dx2y = T(12) * Tol<T>::tiny / (a.iy * a.iy);
+  const T tiny_dx2z = T(12) * Tol<T>::tiny / (a.iz * a.iz);
+  // periodic images: only tiles within 3 cells of a periodic x or y face
+  const bool img_xy = ((a.periodic & 1) && (i0 - fb.ilo < 3 || fb.ihi - (i0 + TX - 1) < 3)) ||
+                      ((a.periodic & 2) && (j0 - fb.jlo < 3 || fb.jhi - (j0 + TY - 1) < 3));
+
+  // plane pointers, advanced by one plane per step
+  const int64_t s_ps = a.ps.sz * (int64_t)sizeof(T), s_out = a.out.sz * (int64_t)sizeof(T);
+  const int64_t s_vel = a.vx.sz * (int64_t)sizeof(T), s_pc = a.pc.sz * (int64_t)sizeof(T);
+  const int64_t s_p0 = a.phi0.sz * (int64_t)sizeof(T);
+  const char* p_ps = plane_ptr<T>(a.ps, k0);
+  const char* p_top = plane_ptr<T>(a.ps, k0 + 3);
+  const char* p_vx = (KIND == STAGE_ADVECT_FIELD) ? plane_ptr<T>(a.vx, k0) : nullptr;
+  const char* p_vy = (KIND == STAGE_ADVECT_FIELD) ? plane_ptr<T>(a.vy, k0) : nullptr;
+  const char* p_vz = (KIND == STAGE_ADVECT_FIELD) ? plane_ptr<T>(a.vz, k0) : nullptr;
+  const char* p_pc = a.pc.p ? plane_ptr<T>(a.pc, k0) : nullptr;
+  const char* p_p0 = (KIND == STAGE_REINIT && a.use_phi0) ? plane_ptr<T>(a.phi0, k0) : nullptr;
+  char* p_out = reinterpret_cast<char*>(a.out.p) + (int64_t)(k0 - a.out.klo) * s_out;
+
+  T zq[7];
+#pragma unroll
+  for (int q = 0; q < 6; q++) zq[q] = buf_load<T>(plane_rsrc(p_ps + (int64_t)(q - 3) * s_ps), oc);
+
+  for (int k = k0; k <= k1; k++) {
+    const __amdgpu_buffer_rsrc_t rps = plane_rsrc(p_ps);
+    zq[6] = buf_load<T>(plane_rsrc(p_top), oc);
+    const T hx = buf_load<T>(rps, ox);
+    T hy[NR];
+#pragma unroll
+    for (int n = 0; n < NR; n++) hy[n] = buf_load<T>(rps, oy[n]);
+    T ux = T(0), uy = T(0), uz = T(0), pcv = T(0), p0v = T(0);
+    if (KIND == STAGE_ADVECT_FIELD) {
+      ux = buf_load<T>(plane_rsrc(p_vx), ov);
+      uy = buf_load<T>(plane_rsrc(p_vy), ov);
+      uz = buf_load<T>(plane_rsrc(p_vz), ov);
+    } else if (KIND == STAGE_ADVECT_CONST) {
+      ux = a.cvx;
+      uy = a.cvy;
+      uz = a.cvz;
+    } else if (a.use_phi0) {
+      p0v = buf_load<T>(plane_rsrc(p_p0), op0);
+    }
+    if (a.pc.p) pcv = buf_load<T>(plane_rsrc(p_pc), opc);
+    __syncthreads();  // the previous plane's LDS reads are done
+    tile[ty + 3][tx + 3] = zq[3];
+    if (do_x) tile[ty + 3][txl] = hx;
+#pragma unroll
+    for (int n = 0; n < NR; n++)
+      if (doy[n]) tile[tyr[n]][tx + 3] = hy[n];
+    __syncthreads();
+
+    if (active) {
+      T px[7], py[7];
+#pragma unroll
+      for (int q = 0; q < 7; q++) {
+        px[q] = tile[ty + 3][tx + q];
+        py[q] = tile[ty + q][tx + 3];
+      }
+      const T us = zq[3];
+      const T uc = a.pc.p ? pcv : us;
+      T rhs;
+      if (KIND == STAGE_REINIT) {
+        T xp, xm, yp, ym, zp, zm;
+        hj_weno5_pm<T, FASTW>(px, a.ix, tiny_dx2x, xp, xm);
+        hj_weno5_pm<T, FASTW>(py, a.iy, tiny_dx2y, yp, ym);
+        hj_weno5_pm<T, FASTW>(zq, a.iz, tiny_dx2z, zp, zm);
+        // lsm_reinitialization3d.f.in:150-268
+        const T cur = a.use_phi0 ? p0v : us;
+        rhs = T(0);
+        if (fabs(cur) >= a.zero_tol) {
+          const T n2 = godunov_star_sq<T>(cur, xp, yp, zp, xm, ym, zm);
+          const T sgn = a.use_phi0 ? cur / sqrt(cur * cur + a.dx_sq) : cur / sqrt(cur * cur + n2 * a.dx_sq);
+          rhs = sgn * (T(1) - sqrt(n2));
+        }
+      } else {
+        const T gx = upwind_weno5<T, FASTW>(px, ux, a.ix, tiny_dx2x, a.zero_tol);
+        const T gy = upwind_weno5<T, FASTW>(py, uy, a.iy, tiny_dx2y, a.zero_tol);
+        const T gz = upwind_weno5<T, FASTW>(zq, uz, a.iz, tiny_dx2z, a.zero_tol);
+        // lsm_level_set_evolution3d.f.in:113-116 on a zeroed rhs
+        rhs = -(ux * gx + uy * gy + uz * gz);
+      }
+      const T v = rk_combine<T>(a.form, uc, us, rhs, a.dt);
+      const bool img_z = (a.periodic & 4) && (k - fb.klo < 3 || fb.khi - k < 3);
+      if (img_xy || img_z) store_with_periodic_images<T>(a.out, fb, i, j, k, v, a.periodic, 3);
+      else buf_store(v, plane_rsrc(p_out), oo);
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) zq[q] = zq[q + 1];
+    p_ps += s_ps;
+    p_top += s_ps;
+    p_out += s_out;
+    if (KIND == STAGE_ADVECT_FIELD) {
+      p_vx += s_vel;
+      p_vy += s_vel;
+      p_vz += s_vel;
+    }
+    if (p_pc) p_pc += s_pc;
+    if (p_p0) p_p0 += s_p0;
   }
 }
 
@@ -358,6 +538,7 @@ struct Tiling {
   int kz;   // planes per workgroup
 };
 Tiling current_tiling();
+double current_zero_tol();  // the fused stages' upwind switch threshold (per host thread)
 TileShape tile_shape(int cfg);
 int tiling_count();
 bool valid_form(int form);

@@ -62,37 +62,37 @@ __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcp
 __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
 
 // The same WENO5 combination as weno5_ref, rearranged to about a third of
-// the operations, on UNDIVIDED differences D1..D5 (= v1..v5 times dx); the
-// result is the undivided derivative (multiply by 1/dx).
-//   a = D1-D2, b = D2-D3, c = D3-D4, d = D4-D5 (second differences)
-//   12 S1 = 13(a-b)^2 + 3(a-3b)^2, 12 S2 = 13(b-c)^2 + 3(b+c)^2,
-//   12 S3 = 13(c-d)^2 + 3(3c-d)^2          (Jiang-Shu indicators x 12)
+// the operations and two reciprocals, on UNDIVIDED differences D1..D5
+// (= v1..v5 times dx); the result is the undivided derivative (multiply by
+// 1/dx).  With the second differences a = D1-D2, b = D2-D3, c = D3-D4,
+// d = D4-D5, the Jiang-Shu indicators times 12 are quadratic forms
+//   12 S1 = 13(a-b)^2 + 3(a-3b)^2 = 16 (a (a - 2.75 b) + 2.5 b^2)
+//   12 S2 = 13(b-c)^2 + 3(b+c)^2  = 16 (b^2 - 1.25 bc + c^2)
+//   12 S3 = 13(c-d)^2 + 3(3c-d)^2 = 16 (d (d - 2.75 c) + 2.5 c^2)
+// and eps = 1e-6 max(v_k^2) + tiny becomes E = 12e-6 max|D|^2 + tiny_dx2
+// (tiny_dx2 = 12 * tiny * dx^2 from the caller).  The weights
+// alpha_k = d_k/(S_k + eps)^2, d = (0.1, 0.6, 0.3), are formed on
+// sigma_k = (12 S_k + E)/E >= 1, so that with P_kl = sigma_k^2 sigma_l^2
+//   w1 = P23 / Den,  w3 = 3 P12 / Den,  Den = P23 + 6 P13 + 3 P12
+// (sigma_k <= ~4e7 keeps Den finite in FP32).  The result is
 //   phi_x = p2 + w1 (p1 - p2) + w3 (p3 - p2),
-//   p2 = (-D2 + 5 D3 + 2 D4)/6, p1 - p2 = (a - 2b + c)/3, p3 - p2 = (b - 2c + d)/6
-// with w_k = alpha_k / sum alpha, alpha_k = d_k/(S_k + eps)^2,
-// d = (0.1, 0.6, 0.3), and eps = 1e-6 max(v_k^2) + tiny scaled by 12 dx^2
-// (tiny_dx2 = 12 * tiny * dx^2 from the caller).
+//   p2 = (-D2 + 5 D3 + 2 D4)/6 = D3 - (b + 2c)/6,
+//   p1 - p2 = (a - 2b + c)/3,  p3 - p2 = (b - 2c + d)/6.
 template <typename T>
 __device__ __forceinline__ T weno5_fast(T D1, T D2, T D3, T D4, T D5, T tiny_dx2) {
   const T a = D1 - D2, b = D2 - D3, c = D3 - D4, d = D4 - D5;
-  T m = fmax(fmax(fabs(D1), fabs(D2)), fmax(fabs(D3), fabs(D4)));
-  m = fmax(m, fabs(D5));
-  const T E = T(12e-6) * (m * m) + tiny_dx2;
-  T t = a - b, u = a - T(3) * b;
-  const T s1 = T(13) * (t * t) + T(3) * (u * u) + E;
-  t = b - c;
-  u = b + c;
-  const T s2 = T(13) * (t * t) + T(3) * (u * u) + E;
-  t = c - d;
-  u = T(3) * c - d;
-  const T s3 = T(13) * (t * t) + T(3) * (u * u) + E;
-  const T b1 = T(0.1 / 3.0) * fast_rcp(s1 * s1);
-  const T b2 = T(0.6) * fast_rcp(s2 * s2);
-  const T b3 = T(0.3 / 6.0) * fast_rcp(s3 * s3);
-  const T r = fast_rcp(T(3) * b1 + b2 + T(6) * b3);
-  const T p2 = (T(5) * D3 + T(2) * D4 - D2) * T(1.0 / 6.0);
-  const T q1 = a - T(2) * b + c, q2 = b - T(2) * c + d;
-  return p2 + (b1 * r) * q1 + (b3 * r) * q2;
+  const T m = fmax(fmax(fmax(fabs(D1), fabs(D2)), fabs(D3)), fmax(fabs(D4), fabs(D5)));
+  const T ie = fast_rcp(T(0.75e-6) * (m * m) + tiny_dx2 * T(1.0 / 16.0));  // 16 / E
+  const T bb = b * b, cc = c * c;
+  const T q1 = a * (a - T(2.75) * b) + T(2.5) * bb;
+  const T q2 = (bb - T(1.25) * (b * c)) + cc;
+  const T q3 = d * (d - T(2.75) * c) + T(2.5) * cc;
+  const T s1 = q1 * ie + T(1), s2 = q2 * ie + T(1), s3 = q3 * ie + T(1);
+  const T t1 = s1 * s1, t2 = s2 * s2, t3 = s3 * s3;
+  const T p23 = t2 * t3, p13 = t1 * t3, p12 = t1 * t2;
+  const T rden = fast_rcp((p23 + T(6) * p13) + T(3) * p12);
+  const T x = (p23 * ((a - T(2) * b) + c)) * T(1.0 / 3.0) + (p12 * ((b - T(2) * c) + d)) * T(0.5);
+  return (D3 - (b + T(2) * c) * T(1.0 / 6.0)) + x * rden;
 }
 
 // d1[q] = D1(c+q-2), q = 0..5 (only those the scheme needs are formed).

@@ -37,8 +37,10 @@ def random_patch_level(npatch=512, lo=16, hi=96, seed=3, dx=1.0 / 128, ghost=3):
 
 
 class PatchLevel:
-    def __init__(self, desc, real="double", device="cuda", bc=(BC_SIGNED_LINEAR,) * 6, tiling=(0, 32)):
+    def __init__(self, desc, real="double", device="cuda", bc=(BC_SIGNED_LINEAR,) * 6, tiling=(0, 32),
+                 zero_tol=None):
         self.real = real
+        self.zero_tol = zero_tol
         self.dtype = _DT[real]
         self.device = torch.device(device)
         self.lib = toolbox(real)
@@ -123,6 +125,7 @@ class PatchLevel:
     # -- stepping ------------------------------------------------------------
     def stage(self, form, out, ps, pc, dt):
         h = self.dx
+        self.lib.set_zero_tol(self.zero_tol)
         self.lib.fused("lsm3d_amd_patch_plan_stage", self.plan, form, ctypes.c_void_p(out.data_ptr()),
                        ctypes.c_void_p(ps.data_ptr()), ctypes.c_void_p(pc.data_ptr()) if pc is not None else None,
                        h, h, h, dt, self.bc)
@@ -134,6 +137,7 @@ class PatchLevel:
 
     def capture_rk3(self, nsteps, phi, s1, s2, dt):
         h = self.dx
+        self.lib.set_zero_tol(self.zero_tol)
         self.lib.fused("lsm3d_amd_patch_plan_capture_rk3", self.plan, int(nsteps),
                        ctypes.c_void_p(phi.data_ptr()), ctypes.c_void_p(s1.data_ptr()),
                        ctypes.c_void_p(s2.data_ptr()), h, h, h, dt, self.bc)

@@ -104,9 +104,14 @@ class Stepper:
     reference's fill-then-differentiate order).  ghosts="images" (all faces
     periodic only): the stages write the periodic ghost images of their
     output and read stored ghosts.  Results are identical in every mode.
+
+    zero_tol: threshold of the upwind switch (|V| < zero_tol: no advection
+    term) and of the reinitialization cut-off; None keeps the build's
+    LSMLIB_ZERO_TOL (1e-11 double, 1e-5 float).  1e-11 in the float library
+    runs the double build's switch in FP32 arithmetic.
     """
 
-    def __init__(self, grid, real="double", bc=(BC_PERIODIC,) * 6, device="cuda", ghosts="auto"):
+    def __init__(self, grid, real="double", bc=(BC_PERIODIC,) * 6, device="cuda", ghosts="auto", zero_tol=None):
         if ghosts == "auto":  # the fastest mode for the boundary conditions
             ghosts = "images" if all(int(b) == BC_PERIODIC for b in bc) else "synth"
         if ghosts not in ("synth", "fill", "images"):
@@ -126,6 +131,7 @@ class Stepper:
             raise ValueError("ghosts='images' needs periodic faces")
         self._stage_bc = self._bc if ghosts == "synth" else None
         self.periodic_mask = 7 if ghosts == "images" else 0
+        self.zero_tol = zero_tol
         self.sync(False)
 
     def sync(self, flag):
@@ -145,6 +151,7 @@ class Stepper:
     # -- advection ---------------------------------------------------------
     def advect_stage(self, form, out, ps, pc, vel, dt):
         dx, dy, dz = self.grid.dx
+        self.lib.set_zero_tol(self.zero_tol)
         if isinstance(vel, (tuple, list)) and len(vel) == 3 and torch.is_tensor(vel[0]):
             self.lib.fused("lsm3d_amd_advect_stage", form, _ptr(out), self._gb, _ptr(ps), self._gb,
                            _ptr(pc), self._gb, _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), self._gb,
@@ -174,6 +181,7 @@ class Stepper:
     # -- reinitialization --------------------------------------------------
     def reinit_stage(self, form, out, ps, pc, phi0, use_phi0, dt):
         dx, dy, dz = self.grid.dx
+        self.lib.set_zero_tol(self.zero_tol)
         self.lib.fused("lsm3d_amd_reinit_stage", form, _ptr(out), self._gb, _ptr(ps), self._gb, _ptr(pc),
                        self._gb, _ptr(phi0) if phi0 is not None else None, self._gb, int(use_phi0),
                        self._fb, dx, dy, dz, dt, self._stage_bc, self.periodic_mask)

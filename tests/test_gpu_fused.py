@@ -41,9 +41,9 @@ def small_periodic(n=(24, 20, 18), dtype=torch.float64):
 
 
 def run_both(grid, phi0, vel, bc, steps, real, dt, mode="advect", use_phi0=False, vel_const=None,
-             ghosts="synth"):
+             ghosts="synth", zero_tol=None):
     dev = torch.device("cuda:0")
-    st = Stepper(grid, real, bc, ghosts=ghosts)
+    st = Stepper(grid, real, bc, ghosts=ghosts, zero_tol=zero_tol)
     phi = phi0.to(dev).clone()
     s1, s2 = st.empty(), st.empty()
     v = vel_const if vel_const is not None else (tuple(a.to(dev) for a in vel) if vel is not None else None)
@@ -147,6 +147,30 @@ def test_advect_fp32_vs_fp64_oracle(ghosts):
     near = np.abs(want) < 3 * min(grid.dx)
     assert err.max() <= 1e-5 * np.abs(want).max()
     assert err[near].max() <= 1e-5 * min(grid.dx)
+
+
+def test_fp32_upwind_switch_threshold():
+    """Velocities below the float build's LSMLIB_ZERO_TOL (1e-5) but above the
+    double build's (1e-11): by default the FP32 stages add no advection term
+    there (the float build's behaviour); with zero_tol=1e-11 they follow the
+    FP64 oracle to FP32 rounding."""
+    grid, phi, vel = small_periodic()
+    slow = tuple((9e-6 * v).float() for v in vel)  # |V| < 1e-5 everywhere
+    dt, steps = 5.0, 3
+    phi32 = phi.float()
+    got_default, want = run_both(grid, phi32, slow, (BC_PERIODIC,) * 6, steps, "float", dt)
+    start = grid.interior(phi32.numpy()).astype(np.float64)
+    moved = np.abs(want - start).max()
+    scale = np.abs(want).max()
+    assert moved > 1e-4
+    assert np.abs(got_default.astype(np.float64) - start).max() <= 1e-6 * scale
+    got, want2 = run_both(grid, phi32, slow, (BC_PERIODIC,) * 6, steps, "float", dt, zero_tol=1e-11)
+    assert np.array_equal(want, want2)
+    assert np.abs(got.astype(np.float64) - want).max() <= 1e-6 * scale
+    lib = Stepper(grid, "float", (BC_PERIODIC,) * 6).lib
+    assert lib.zero_tol() == pytest.approx(1e-11)  # the setting is per host thread and persists
+    lib.set_zero_tol(None)
+    assert lib.zero_tol() == pytest.approx(1e-5)
 
 
 def test_reinit_fp32_vs_fp64_oracle():
