@@ -66,12 +66,12 @@ def run_gpu(real, n, steps, warmup, tiling, ghosts="synth", zero_tol=None):
     import torch
     from lsmlib_amd import toolbox
     from lsmlib_amd.pylsmlib import RK1, TVDRK3_STAGE2, TVDRK3_STAGE3, Stepper, stable_advection_dt
-    from lsmlib_amd.workloads import sphere_union_taylor_green
+    from lsmlib_amd.workloads import line_elems, sphere_union_taylor_green
     from lsmlib_amd.tuning import choose_tiling
 
     dev = torch.device("cuda:0")
     dtype = torch.float32 if real == "float" else torch.float64
-    grid, phi64, vel, bc = sphere_union_taylor_green(n, dtype=torch.float64, device=dev)
+    grid, phi64, vel, bc = sphere_union_taylor_green(n, dtype=torch.float64, device=dev, align=line_elems(dtype))
     phi = phi64.to(dtype).clone()
     del phi64
     dt = stable_advection_dt(vel, grid, 0.5, "double")  # one dt for both precisions
@@ -161,12 +161,12 @@ def run_reinit(n=256, iters=50):
     TVD-RK3, pseudo-dt 0.5 dx, 50 iterations; FP64 and FP32, FP32 error vs FP64."""
     import torch
     from lsmlib_amd.pylsmlib import Stepper
-    from lsmlib_amd.workloads import wavy_sphere_reinit
+    from lsmlib_amd.workloads import line_elems, wavy_sphere_reinit
     dev = torch.device("cuda:0")
-    grid, phi64, bc = wavy_sphere_reinit(n, torch.float64, dev)
     out = {}
     res = {}
     for real, dtype in (("double", torch.float64), ("float", torch.float32)):
+        grid, phi64, bc = wavy_sphere_reinit(n, torch.float64, dev, align=line_elems(dtype))
         st = Stepper(grid, real, bc)
         phi = phi64.to(dtype).clone()  # a copy even when dtype is already float64
         phi0 = phi.clone()
@@ -180,10 +180,10 @@ def run_reinit(n=256, iters=50):
             st.reinit_rk3_iteration(phi, s1, s2, dt, phi0=phi0, use_phi0=True)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        res[real] = phi
+        res[real] = (grid, phi)
         out[real] = {"ms_per_iteration": 1e3 * el / iters,
                      "value": grid.interior_cells * iters / el, "unit": "cell-updates/s"}
-    err = fp32_error(grid, res["float"], res["double"])
+    err = fp32_error(*res["float"], *res["double"])
     out["fp32_error_vs_fp64"] = err
     out["config"] = f"{n}^3 wavy sphere, use_phi0_for_sgn=1, {iters} TVD-RK3 iterations, dt=0.5dx"
     return out
@@ -293,12 +293,13 @@ def run_adjoint(n=128, nsteps=40, every=8, reps=3, fd=True):
     return out
 
 
-def fp32_error(grid, phi32, phi64):
+def fp32_error(grid32, phi32, grid64, phi64):
+    """Max |phi32 - phi64| over the fill box and near the interface (|phi64| < 3 dx)."""
     import torch
-    a = grid.interior(phi32).to(torch.float64)
-    b = grid.interior(phi64)
+    a = grid32.interior(phi32).to(torch.float64)
+    b = grid64.interior(phi64)
     err = (a - b).abs()
-    h = min(grid.dx)
+    h = min(grid64.dx)
     near = b.abs() < 3 * h
     return {"max_abs_global": float(err.max()),
             "max_abs_near_interface": float(err[near].max()) if bool(near.any()) else None,
@@ -371,20 +372,20 @@ def main():
     if a.adjoint_only:
         print(json.dumps({"adjoint": run_adjoint(fd=not a.no_fd)}), flush=True)
         return
-    r32, grid, phi32 = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts, zero_tol=SWITCH_TOL)
+    r32, grid32, phi32 = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts, zero_tol=SWITCH_TOL)
     print(f"# fp32: {r32['value']:.4e} cell-updates/s, {r32['ms_per_step']:.3f} ms/step, "
           f"kernel {r32['kernel_ms_avg']:.3f} ms, {r32['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     r64 = None
     err = err_build_tol = None
     if not a.fp32_only:
-        r64, _, phi64 = run_gpu("double", a.n, a.steps, a.warmup, a.tiling, a.ghosts)
+        r64, grid64, phi64 = run_gpu("double", a.n, a.steps, a.warmup, a.tiling, a.ghosts)
         print(f"# fp64: {r64['value']:.4e} cell-updates/s, {r64['ms_per_step']:.3f} ms/step",
               file=sys.stderr, flush=True)
-        err = fp32_error(grid, phi32, phi64)
+        err = fp32_error(grid32, phi32, grid64, phi64)
         del phi32
         # the same FP32 run with the float build's LSMLIB_ZERO_TOL (1e-5) in the upwind switch
         _, _, phi32b = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts, zero_tol=None)
-        err_build_tol = fp32_error(grid, phi32b, phi64)
+        err_build_tol = fp32_error(grid32, phi32b, grid64, phi64)
         print(f"# fp32 error vs fp64 near interface: {err['max_near_interface_over_dx']:.3e} dx "
               f"(switch 1e-11), {err_build_tol['max_near_interface_over_dx']:.3e} dx (switch 1e-5)",
               file=sys.stderr, flush=True)

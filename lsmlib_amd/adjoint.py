@@ -41,15 +41,23 @@ _ACC_STAGE, _ACC_CUR = 1, 2
 
 
 def fold_periodic_ghosts(a, g):
-    """Adds the values on the ghost layers of `a` (width g, every axis) to
-    the fill-box cells they are periodic images of, and zeroes the ghosts
-    (the transpose of a periodic ghost fill).  In place; returns a."""
+    """Adds the values on the ghost layers of `a` to the fill-box cells they
+    are periodic images of, and zeroes the ghosts (the transpose of a
+    periodic ghost fill).  g: ghost width on every side, or a Grid (per-axis
+    widths, x possibly padded).  In place; returns a."""
+    if isinstance(g, int):
+        widths = {2: (g, g), 1: (g, g), 0: (g, g)}
+    else:
+        widths = {2: (g.xlo, g.xhi), 1: (g.ghost, g.ghost), 0: (g.ghost, g.ghost)}
     for axis in (2, 1, 0):
-        n = a.shape[axis] - 2 * g
-        lo_ghost = a.narrow(axis, 0, g)
-        hi_ghost = a.narrow(axis, n + g, g)
-        a.narrow(axis, n, g).add_(lo_ghost)      # ghost t < 0 is cell t + n
-        a.narrow(axis, g, g).add_(hi_ghost)      # ghost t >= n is cell t - n
+        lo, hi = widths[axis]
+        n = a.shape[axis] - lo - hi
+        if lo > n or hi > n:
+            raise ValueError("ghost layers wider than the periodic period")
+        lo_ghost = a.narrow(axis, 0, lo)
+        hi_ghost = a.narrow(axis, lo + n, hi)
+        a.narrow(axis, n, lo).add_(lo_ghost)      # ghost t < 0 is cell t + n
+        a.narrow(axis, lo, hi).add_(hi_ghost)     # ghost t >= n is cell t - n
         lo_ghost.zero_()
         hi_ghost.zero_()
     return a
@@ -117,7 +125,7 @@ class AdvectionRK3:
         st = self.stepper
         st.sync(False)
         zeros = lambda: torch.zeros(self.grid.shape, dtype=self.dtype, device=self.device)
-        pbar = fold_periodic_ghosts(phibar_T.detach().to(self.dtype).clone().contiguous(), self.grid.ghost)
+        pbar = fold_periodic_ghosts(phibar_T.detach().to(self.dtype).clone().contiguous(), self.grid)
         acc, s1b, s2b = zeros(), zeros(), zeros()
         vbar = [zeros(), zeros(), zeros()] if want_vel else None
         nseg = (nsteps + self.every - 1) // self.every

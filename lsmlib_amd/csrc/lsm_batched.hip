@@ -36,7 +36,7 @@ template <typename T, int TX, int TY, bool SYNTH>
 __global__ void __launch_bounds__(TX* TY)
     k_patch_stage(const PatchDesc* __restrict__ desc, const int64_t* __restrict__ prefix, int npatch,
                   int64_t ntiles, T* out_pool, const T* ps_pool, const T* pc_pool, GhostRule ghost, T ix, T iy,
-                  T iz, T dt, T zt, int form) {
+                  T iz, T dt, T zt, T tdx2x, T tdx2y, T tdx2z, int form) {
   __shared__ T tile[TY + 6][TX + 7];
   const int G = (int)gridDim.x;  // a multiple of the XCD count
   const int b = (int)blockIdx.x;
@@ -68,6 +68,9 @@ __global__ void __launch_bounds__(TX* TY)
     a.dt = dt;
     a.dx_sq = T(0);
     a.zero_tol = zt;
+    a.tdx2[0] = tdx2x;
+    a.tdx2[1] = tdx2y;
+    a.tdx2[2] = tdx2z;
     a.form = form;
     a.periodic = 0;
     a.use_phi0 = 0;
@@ -107,12 +110,15 @@ template <int TX, int TY>
 void launch_patch_t(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R* ps, const R* pc,
                     const GhostRule& g, R ix, R iy, R iz, R dt, int form) {
   const R zt = (R)current_zero_tol();
+  // 12 * tiny * dx^2 per axis (the WENO5 epsilon floor in undivided units)
+  const R tx2 = R(12) * Tol<R>::tiny / (ix * ix), ty2 = R(12) * Tol<R>::tiny / (iy * iy);
+  const R tz2 = R(12) * Tol<R>::tiny / (iz * iz);
   if (g.any)
     hipLaunchKernelGGL((k_patch_stage<R, TX, TY, true>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
-                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, zt, form);
+                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, zt, tx2, ty2, tz2, form);
   else
     hipLaunchKernelGGL((k_patch_stage<R, TX, TY, false>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
-                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, zt, form);
+                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, zt, tx2, ty2, tz2, form);
 }
 
 void launch_patch(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R* ps, const R* pc,

@@ -221,6 +221,9 @@ void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, co
   a.ix = R(1) / dx;
   a.iy = R(1) / dy;
   a.iz = R(1) / dz;
+  a.tdx2[0] = R(12) * Tol<R>::tiny / (a.ix * a.ix);
+  a.tdx2[1] = R(12) * Tol<R>::tiny / (a.iy * a.iy);
+  a.tdx2[2] = R(12) * Tol<R>::tiny / (a.iz * a.iz);
   a.dt = dt;
   R mdx = dx;
   if (dy > mdx) mdx = dy;

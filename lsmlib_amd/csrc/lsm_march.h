@@ -103,6 +103,7 @@ struct StageArgs {
   GhostRule ghost;
   T ix, iy, iz, dt, dx_sq;
   T zero_tol;    // upwind switch / reinit cut-off (LSMLIB_ZERO_TOL of the build by default)
+  T tdx2[3];     // 12 * tiny * dx^2 per axis (the WENO5 epsilon floor in undivided units)
   int form;      // RKForm
   int periodic;  // bit mask: also write phi_out's periodic ghost images
   int use_phi0;  // reinit sign from phi0 (1) or phi_s (0)
@@ -113,11 +114,10 @@ struct StageArgs {
 // 64-bit plane pointer per field and 32-bit in-plane offsets.  FASTW selects
 // the rearranged WENO5 (weno5_fast) instead of the reference-order form.
 // ---------------------------------------------------------------------------
+// Upwind WENO5 derivative from the six first differences d[q] = p[q+1] - p[q]
+// of the samples p[0..6] around the cell.
 template <typename T, bool FASTW>
-__device__ __forceinline__ T upwind_weno5(const T* p, T vel, T inv, T tiny_dx2, T zero_tol) {
-  T d[6];
-#pragma unroll
-  for (int q = 0; q < 6; q++) d[q] = p[q + 1] - p[q];
+__device__ __forceinline__ T upwind_weno5_d(const T* d, T vel, T inv, T tiny_dx2, T zero_tol) {
   const bool bwd = vel > T(0);
   const T D1 = bwd ? d[0] : d[5], D2 = bwd ? d[1] : d[4], D3 = bwd ? d[2] : d[3];
   const T D4 = bwd ? d[3] : d[2], D5 = bwd ? d[4] : d[1];
@@ -128,17 +128,32 @@ __device__ __forceinline__ T upwind_weno5(const T* p, T vel, T inv, T tiny_dx2, 
 }
 
 template <typename T, bool FASTW>
-__device__ __forceinline__ void hj_weno5_pm(const T* p, T inv, T tiny_dx2, T& plus, T& minus) {
-  if (FASTW) {
-    T d[6];
+__device__ __forceinline__ T upwind_weno5(const T* p, T vel, T inv, T tiny_dx2, T zero_tol) {
+  T d[6];
 #pragma unroll
-    for (int q = 0; q < 6; q++) d[q] = p[q + 1] - p[q];
+  for (int q = 0; q < 6; q++) d[q] = p[q + 1] - p[q];
+  return upwind_weno5_d<T, FASTW>(d, vel, inv, tiny_dx2, zero_tol);
+}
+
+// Plus and minus WENO5 derivatives from the six first differences
+// (hj_plus / hj_minus for WENO5: lsm_spatial_derivatives3d.f.in:1286-1370).
+template <typename T, bool FASTW>
+__device__ __forceinline__ void hj_weno5_pm_d(const T* d, T inv, T tiny_dx2, T& plus, T& minus) {
+  if (FASTW) {
     plus = weno5_fast<T>(d[5], d[4], d[3], d[2], d[1], tiny_dx2) * inv;
     minus = weno5_fast<T>(d[0], d[1], d[2], d[3], d[4], tiny_dx2) * inv;
   } else {
-    plus = hj_plus<T, WENO5>(p, inv);
-    minus = hj_minus<T, WENO5>(p, inv);
+    plus = weno5_ref<T>(d[5] * inv, d[4] * inv, d[3] * inv, d[2] * inv, d[1] * inv);
+    minus = weno5_ref<T>(d[0] * inv, d[1] * inv, d[2] * inv, d[3] * inv, d[4] * inv);
   }
+}
+
+template <typename T, bool FASTW>
+__device__ __forceinline__ void hj_weno5_pm(const T* p, T inv, T tiny_dx2, T& plus, T& minus) {
+  T d[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) d[q] = p[q + 1] - p[q];
+  hj_weno5_pm_d<T, FASTW>(d, inv, tiny_dx2, plus, minus);
 }
 
 template <typename T>
@@ -274,9 +289,7 @@ __device__ __forceinline__ void march(const StageArgs<T>& a, int bx, int by, int
   const int op0 = (KIND == STAGE_REINIT && a.use_phi0) ? p0.off(ic, jc) : 0;
   T* const outp = a.out.p;
   const int oout = (i - a.out.ilo) + (int)a.out.sy * (j - a.out.jlo);
-  const T tiny_dx2x = T(12) * Tol<T>::tiny / (a.ix * a.ix);
-  const T tiny_dx2y = T(12) * Tol<T>::tiny / (a.iy * a.iy);
-  const T tiny_dx2z = T(12) * Tol<T>::tiny / (a.iz * a.iz);
+  const T tiny_dx2x = a.tdx2[0], tiny_dx2y = a.tdx2[1], tiny_dx2z = a.tdx2[2];
 
   T zq[7];
 #pragma unroll
@@ -426,9 +439,7 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
   const uint32_t opc = a.pc.p ? plane_byte_offset<T>(a.pc, ic, jc) : 0u;
   const uint32_t op0 = (KIND == STAGE_REINIT && a.use_phi0) ? plane_byte_offset<T>(a.phi0, ic, jc) : 0u;
   const uint32_t oo = (uint32_t)((ic - a.out.ilo) + (int)a.out.sy * (jc - a.out.jlo)) * (uint32_t)sizeof(T);
-  const T tiny_dx2x = T(12) * Tol<T>::tiny / (a.ix * a.ix);
-  const T tiny_dx2y = T(12) * Tol<T>::tiny / (a.iy * a.iy);
-  const T tiny_dx2z = T(12) * Tol<T>::tiny / (a.iz * a.iz);
+  const T tiny_dx2x = a.tdx2[0], tiny_dx2y = a.tdx2[1], tiny_dx2z = a.tdx2[2];
   // periodic images: only tiles within 3 cells of a periodic x or y face
   const bool img_xy = ((a.periodic & 1) && (i0 - fb.ilo < 3 || fb.ihi - (i0 + TX - 1) < 3)) ||
                       ((a.periodic & 2) && (j0 - fb.jlo < 3 || fb.jhi - (j0 + TY - 1) < 3));
@@ -446,85 +457,114 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
   const char* p_p0 = (KIND == STAGE_REINIT && a.use_phi0) ? plane_ptr<T>(a.phi0, k0) : nullptr;
   char* p_out = reinterpret_cast<char*>(a.out.p) + (int64_t)(k0 - a.out.klo) * s_out;
 
-  T zq[7];
+  // z column, unrolled by U planes: for the plane k + u of a group,
+  // zv[u + q] = phi(k + u + q), q = 0..3, and zd[u + q] = phi(k + u + q - 2) -
+  // phi(k + u + q - 3), q = 0..5; one new value and difference per plane,
+  // the queues shift once per group.
+  constexpr int U = 4;
+  T zv[U + 3], zd[U + 5];
+  {
+    T z0[6];
 #pragma unroll
-  for (int q = 0; q < 6; q++) zq[q] = buf_load<T>(plane_rsrc(p_ps + (int64_t)(q - 3) * s_ps), oc);
+    for (int q = 0; q < 6; q++) z0[q] = buf_load<T>(plane_rsrc(p_ps + (int64_t)(q - 3) * s_ps), oc);
+#pragma unroll
+    for (int q = 0; q < 5; q++) zd[q] = z0[q + 1] - z0[q];
+#pragma unroll
+    for (int q = 0; q < 3; q++) zv[q] = z0[q + 3];
+  }
 
-  for (int k = k0; k <= k1; k++) {
-    const __amdgpu_buffer_rsrc_t rps = plane_rsrc(p_ps);
-    zq[6] = buf_load<T>(plane_rsrc(p_top), oc);
-    const T hx = buf_load<T>(rps, ox);
-    T hy[NR];
+  for (int kg = k0; kg <= k1; kg += U) {
 #pragma unroll
-    for (int n = 0; n < NR; n++) hy[n] = buf_load<T>(rps, oy[n]);
-    T ux = T(0), uy = T(0), uz = T(0), pcv = T(0), p0v = T(0);
-    if (KIND == STAGE_ADVECT_FIELD) {
-      ux = buf_load<T>(plane_rsrc(p_vx), ov);
-      uy = buf_load<T>(plane_rsrc(p_vy), ov);
-      uz = buf_load<T>(plane_rsrc(p_vz), ov);
-    } else if (KIND == STAGE_ADVECT_CONST) {
-      ux = a.cvx;
-      uy = a.cvy;
-      uz = a.cvz;
-    } else if (a.use_phi0) {
-      p0v = buf_load<T>(plane_rsrc(p_p0), op0);
-    }
-    if (a.pc.p) pcv = buf_load<T>(plane_rsrc(p_pc), opc);
-    __syncthreads();  // the previous plane's LDS reads are done
-    tile[ty + 3][tx + 3] = zq[3];
-    if (do_x) tile[ty + 3][txl] = hx;
+    for (int u = 0; u < U; u++) {
+      const int k = kg + u;
+      if (k > k1) break;  // uniform
+      const __amdgpu_buffer_rsrc_t rps = plane_rsrc(p_ps);
+      zv[u + 3] = buf_load<T>(plane_rsrc(p_top), oc);
+      const T hx = buf_load<T>(rps, ox);
+      T hy[NR];
 #pragma unroll
-    for (int n = 0; n < NR; n++)
-      if (doy[n]) tile[tyr[n]][tx + 3] = hy[n];
-    __syncthreads();
-
-    if (active) {
-      T px[7], py[7];
-#pragma unroll
-      for (int q = 0; q < 7; q++) {
-        px[q] = tile[ty + 3][tx + q];
-        py[q] = tile[ty + q][tx + 3];
+      for (int n = 0; n < NR; n++) hy[n] = buf_load<T>(rps, oy[n]);
+      T ux = T(0), uy = T(0), uz = T(0), pcv = T(0), p0v = T(0);
+      if (KIND == STAGE_ADVECT_FIELD) {
+        ux = buf_load<T>(plane_rsrc(p_vx), ov);
+        uy = buf_load<T>(plane_rsrc(p_vy), ov);
+        uz = buf_load<T>(plane_rsrc(p_vz), ov);
+      } else if (KIND == STAGE_ADVECT_CONST) {
+        ux = a.cvx;
+        uy = a.cvy;
+        uz = a.cvz;
+      } else if (a.use_phi0) {
+        p0v = buf_load<T>(plane_rsrc(p_p0), op0);
       }
-      const T us = zq[3];
-      const T uc = a.pc.p ? pcv : us;
-      T rhs;
-      if (KIND == STAGE_REINIT) {
-        T xp, xm, yp, ym, zp, zm;
-        hj_weno5_pm<T, FASTW>(px, a.ix, tiny_dx2x, xp, xm);
-        hj_weno5_pm<T, FASTW>(py, a.iy, tiny_dx2y, yp, ym);
-        hj_weno5_pm<T, FASTW>(zq, a.iz, tiny_dx2z, zp, zm);
-        // lsm_reinitialization3d.f.in:150-268
-        const T cur = a.use_phi0 ? p0v : us;
-        rhs = T(0);
-        if (fabs(cur) >= a.zero_tol) {
-          const T n2 = godunov_star_sq<T>(cur, xp, yp, zp, xm, ym, zm);
-          const T sgn = a.use_phi0 ? cur / sqrt(cur * cur + a.dx_sq) : cur / sqrt(cur * cur + n2 * a.dx_sq);
-          rhs = sgn * (T(1) - sqrt(n2));
+      if (a.pc.p) pcv = buf_load<T>(plane_rsrc(p_pc), opc);
+      __syncthreads();  // the previous plane's LDS reads are done
+      tile[ty + 3][tx + 3] = zv[u];
+      if (do_x) tile[ty + 3][txl] = hx;
+#pragma unroll
+      for (int n = 0; n < NR; n++)
+        if (doy[n]) tile[tyr[n]][tx + 3] = hy[n];
+      zd[u + 5] = zv[u + 3] - zv[u + 2];
+      __syncthreads();
+
+      if (active) {
+        T dx_[6], dy_[6];
+        {
+          T px[7], py[7];
+#pragma unroll
+          for (int q = 0; q < 7; q++) {
+            px[q] = tile[ty + 3][tx + q];
+            py[q] = tile[ty + q][tx + 3];
+          }
+#pragma unroll
+          for (int q = 0; q < 6; q++) {
+            dx_[q] = px[q + 1] - px[q];
+            dy_[q] = py[q + 1] - py[q];
+          }
         }
-      } else {
-        const T gx = upwind_weno5<T, FASTW>(px, ux, a.ix, tiny_dx2x, a.zero_tol);
-        const T gy = upwind_weno5<T, FASTW>(py, uy, a.iy, tiny_dx2y, a.zero_tol);
-        const T gz = upwind_weno5<T, FASTW>(zq, uz, a.iz, tiny_dx2z, a.zero_tol);
-        // lsm_level_set_evolution3d.f.in:113-116 on a zeroed rhs
-        rhs = -(ux * gx + uy * gy + uz * gz);
+        const T* dz_ = zd + u;
+        const T us = zv[u];
+        const T uc = a.pc.p ? pcv : us;
+        T rhs;
+        if (KIND == STAGE_REINIT) {
+          T xp, xm, yp, ym, zp, zm;
+          hj_weno5_pm_d<T, FASTW>(dx_, a.ix, tiny_dx2x, xp, xm);
+          hj_weno5_pm_d<T, FASTW>(dy_, a.iy, tiny_dx2y, yp, ym);
+          hj_weno5_pm_d<T, FASTW>(dz_, a.iz, tiny_dx2z, zp, zm);
+          // lsm_reinitialization3d.f.in:150-268
+          const T cur = a.use_phi0 ? p0v : us;
+          rhs = T(0);
+          if (fabs(cur) >= a.zero_tol) {
+            const T n2 = godunov_star_sq<T>(cur, xp, yp, zp, xm, ym, zm);
+            const T sgn = a.use_phi0 ? cur / sqrt(cur * cur + a.dx_sq) : cur / sqrt(cur * cur + n2 * a.dx_sq);
+            rhs = sgn * (T(1) - sqrt(n2));
+          }
+        } else {
+          const T gx = upwind_weno5_d<T, FASTW>(dx_, ux, a.ix, tiny_dx2x, a.zero_tol);
+          const T gy = upwind_weno5_d<T, FASTW>(dy_, uy, a.iy, tiny_dx2y, a.zero_tol);
+          const T gz = upwind_weno5_d<T, FASTW>(dz_, uz, a.iz, tiny_dx2z, a.zero_tol);
+          // lsm_level_set_evolution3d.f.in:113-116 on a zeroed rhs
+          rhs = -(ux * gx + uy * gy + uz * gz);
+        }
+        const T v = rk_combine<T>(a.form, uc, us, rhs, a.dt);
+        const bool img_z = (a.periodic & 4) && (k - fb.klo < 3 || fb.khi - k < 3);
+        if (img_xy || img_z) store_with_periodic_images<T>(a.out, fb, i, j, k, v, a.periodic, 3);
+        else buf_store(v, plane_rsrc(p_out), oo);
       }
-      const T v = rk_combine<T>(a.form, uc, us, rhs, a.dt);
-      const bool img_z = (a.periodic & 4) && (k - fb.klo < 3 || fb.khi - k < 3);
-      if (img_xy || img_z) store_with_periodic_images<T>(a.out, fb, i, j, k, v, a.periodic, 3);
-      else buf_store(v, plane_rsrc(p_out), oo);
+      p_ps += s_ps;
+      p_top += s_ps;
+      p_out += s_out;
+      if (KIND == STAGE_ADVECT_FIELD) {
+        p_vx += s_vel;
+        p_vy += s_vel;
+        p_vz += s_vel;
+      }
+      if (p_pc) p_pc += s_pc;
+      if (p_p0) p_p0 += s_p0;
     }
 #pragma unroll
-    for (int q = 0; q < 6; q++) zq[q] = zq[q + 1];
-    p_ps += s_ps;
-    p_top += s_ps;
-    p_out += s_out;
-    if (KIND == STAGE_ADVECT_FIELD) {
-      p_vx += s_vel;
-      p_vy += s_vel;
-      p_vz += s_vel;
-    }
-    if (p_pc) p_pc += s_pc;
-    if (p_p0) p_p0 += s_p0;
+    for (int q = 0; q < 3; q++) zv[q] = zv[q + U];
+#pragma unroll
+    for (int q = 0; q < 5; q++) zd[q] = zd[q + U];
   }
 }
 

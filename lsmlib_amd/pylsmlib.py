@@ -42,11 +42,29 @@ def grow(box, g):
 class Grid:
     """Uniform grid: n interior cells per axis on [lower, upper], `ghost`
     ghost layers, fill box indices 0..n-1 (the serial package's convention,
-    lsm_grid.c:788-811)."""
+    lsm_grid.c:788-811).
+
+    align (elements, 0 = off): widen the x ghost region so that every row of
+    the ghost box, and the first fill-box cell of every row, start on an
+    `align`-element boundary (32 for FP32 and 16 for FP64 put them on
+    128-byte cache-line boundaries of a 256-byte aligned allocation).  The
+    kernels still read only `ghost` cells beyond the fill box; the extra
+    cells are padding."""
     n: tuple
     lower: tuple
     upper: tuple
     ghost: int = 3
+    align: int = 0
+
+    @property
+    def xlo(self):  # ghost cells below the fill box along x
+        return max(self.ghost, self.align) if self.align else self.ghost
+
+    @property
+    def xhi(self):  # ghost cells above the fill box along x
+        if not self.align:
+            return self.ghost
+        return self.ghost + (-(self.n[0] + self.xlo + self.ghost)) % self.align
 
     @property
     def fb(self):
@@ -54,12 +72,13 @@ class Grid:
 
     @property
     def gb(self):
-        return grow(self.fb, self.ghost)
+        g = self.ghost
+        return (-self.xlo, self.n[0] - 1 + self.xhi, -g, self.n[1] - 1 + g, -g, self.n[2] - 1 + g)
 
     @property
     def shape(self):
         g = self.ghost
-        return (self.n[2] + 2 * g, self.n[1] + 2 * g, self.n[0] + 2 * g)
+        return (self.n[2] + 2 * g, self.n[1] + 2 * g, self.n[0] + self.xlo + self.xhi)
 
     @property
     def dx(self):
@@ -71,15 +90,15 @@ class Grid:
 
     def centers(self, dtype=torch.float64, device="cpu"):
         """Cell-centre coordinates X, Y, Z over the ghost box, each (nk, nj, ni)."""
-        g = self.ghost
-        ax = [self.lower[d] + (torch.arange(-g, self.n[d] + g, dtype=torch.float64) + 0.5) * self.dx[d]
+        gb = self.gb
+        ax = [self.lower[d] + (torch.arange(gb[2 * d], gb[2 * d + 1] + 1, dtype=torch.float64) + 0.5) * self.dx[d]
               for d in range(3)]
         Z, Y, X = torch.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
         return X.to(dtype=dtype, device=device), Y.to(dtype=dtype, device=device), Z.to(dtype=dtype, device=device)
 
     def interior(self, a):
         g = self.ghost
-        return a[g:a.shape[0] - g, g:a.shape[1] - g, g:a.shape[2] - g]
+        return a[g:a.shape[0] - g, g:a.shape[1] - g, self.xlo:self.xlo + self.n[0]]
 
 
 def _ptr(t):

@@ -6,7 +6,6 @@ default.  `autotune(...)` times every compiled tile shape and several z-chunk
 depths on the GPU for given grid sizes and records the fastest.
 """
 import json
-import time
 from pathlib import Path
 
 TABLE = Path(__file__).resolve().parent / "tuning_table.json"
@@ -35,21 +34,24 @@ def choose_tiling(real, n, kind="advect"):
     return tuple(t) if t else DEFAULT
 
 
-def autotune(reals=("float", "double"), sizes=(128, 256, 512), kind="advect", reps=3, verbose=True):
-    """Times one stage of `kind` for every (tile shape, kz) and stores the best."""
+def autotune(reals=("float", "double"), sizes=(128, 256, 512), kind="advect", reps=5, verbose=True, path=None):
+    """Times one stage of `kind` for every (tile shape, kz) with HIP events
+    (median of `reps`) and stores the best in the table at `path`."""
     import torch
     from . import toolbox
     from .pylsmlib import RK1, Stepper
-    from .workloads import sphere_union_taylor_green
+    from .workloads import line_elems, sphere_union_taylor_green
 
-    table = load_table()
+    path = Path(path) if path else TABLE
+    table = json.loads(path.read_text()) if path.exists() else {}
     dev = torch.device("cuda:0")
     for real in reals:
         lib = toolbox(real)
         dtype = torch.float32 if real == "float" else torch.float64
         ncfg = lib.lib.lsm3d_amd_stage_tiling_count()
         for n in sizes:
-            grid, phi, vel, bc = sphere_union_taylor_green(n, n_spheres=8, dtype=dtype, device=dev)
+            grid, phi, vel, bc = sphere_union_taylor_green(n, n_spheres=8, dtype=dtype, device=dev,
+                                                           align=line_elems(dtype))
             st = Stepper(grid, real, bc)
             out = st.empty()
             best = None
@@ -61,12 +63,18 @@ def autotune(reals=("float", "double"), sizes=(128, 256, 512), kind="advect", re
                     else:
                         run = lambda: st.reinit_stage(RK1, out, phi, None, None, False, 1e-3)  # noqa: E731
                     run()
-                    torch.cuda.synchronize()
-                    t0 = time.perf_counter()
+                    times = []
                     for _ in range(reps):
+                        e0 = torch.cuda.Event(enable_timing=True)
+                        e1 = torch.cuda.Event(enable_timing=True)
+                        e0.record()
                         run()
-                    torch.cuda.synchronize()
-                    ms = (time.perf_counter() - t0) * 1e3 / reps
+                        e1.record()
+                        e1.synchronize()
+                        times.append(e0.elapsed_time(e1))
+                    ms = sorted(times)[len(times) // 2]
+                    if verbose:
+                        print(f"  {kind} {real} n={n} cfg={cfg} kz={kz}: {ms:.3f} ms", flush=True)
                     if best is None or ms < best[0]:
                         best = (ms, cfg, kz)
             table[f"{kind}/{real}/{_size_class(n)}"] = [best[1], best[2]]
@@ -75,10 +83,16 @@ def autotune(reals=("float", "double"), sizes=(128, 256, 512), kind="advect", re
                       flush=True)
             del phi, vel, out
             torch.cuda.empty_cache()
-    TABLE.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
+    path.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
     return table
 
 
 if __name__ == "__main__":
-    autotune()
-    autotune(kind="reinit", sizes=(256,))
+    import argparse
+    ap = argparse.ArgumentParser(description="time every stage tiling and record the fastest")
+    ap.add_argument("--out", default=str(TABLE))
+    ap.add_argument("--sizes", default="128,256,512")
+    a = ap.parse_args()
+    sizes = tuple(int(x) for x in a.sizes.split(","))
+    autotune(sizes=sizes, path=a.out)
+    autotune(kind="reinit", sizes=(256,), path=a.out)

@@ -10,6 +10,8 @@
                            distances (radii U[0.02,0.1]), Taylor-Green velocity
                            u=sin2pix cos2piy cos2piz, v=-cos2pix sin2piy cos2piz,
                            w=0 (large advection case)
+  align=line_elems(dtype) pads the x ghost region so rows start on 128-byte
+  cache-line boundaries (see pylsmlib.Grid)
   FourierVelocity(grid, ...) / adjoint_case(n)
                            divergence-free velocity sum_m P(k_m) c_m
                            sin(pi k_m.x + theta_m), 64 integer wavevectors with
@@ -25,8 +27,13 @@ import torch
 from .pylsmlib import BC_PERIODIC, BC_SIGNED_LINEAR, Grid
 
 
-def rotating_sphere(n=128, dtype=torch.float64, device="cpu"):
-    grid = Grid((n, n, n), (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), 3)
+def line_elems(dtype):
+    """Elements per 128-byte cache line (the Grid alignment for `dtype`)."""
+    return 128 // torch.empty((), dtype=dtype).element_size()
+
+
+def rotating_sphere(n=128, dtype=torch.float64, device="cpu", align=0):
+    grid = Grid((n, n, n), (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), 3, align)
     X, Y, Z = grid.centers(torch.float64, device)
     phi = torch.sqrt((X - 0.35) ** 2 + Y ** 2 + Z ** 2) - 0.35
     vel = (-Y, X, torch.zeros_like(X))
@@ -34,8 +41,8 @@ def rotating_sphere(n=128, dtype=torch.float64, device="cpu"):
             (BC_SIGNED_LINEAR,) * 6)
 
 
-def wavy_sphere_reinit(n=256, dtype=torch.float64, device="cpu"):
-    grid = Grid((n, n, n), (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), 3)
+def wavy_sphere_reinit(n=256, dtype=torch.float64, device="cpu", align=0):
+    grid = Grid((n, n, n), (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), 3, align)
     X, Y, Z = grid.centers(torch.float64, device)
     r = torch.sqrt(X ** 2 + Y ** 2 + Z ** 2)
     phi = (r - 0.5) * (1 + 0.4 * torch.sin(3 * X) * torch.sin(3 * Y) * torch.sin(3 * Z))
@@ -70,8 +77,8 @@ def sphere_union(grid, n_spheres=64, seed=5, rmin=0.02, rmax=0.1, dtype=torch.fl
     return phi.to(dtype).contiguous()
 
 
-def sphere_union_taylor_green(n=512, n_spheres=64, seed=5, dtype=torch.float64, device="cpu"):
-    grid = Grid((n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 3)
+def sphere_union_taylor_green(n=512, n_spheres=64, seed=5, dtype=torch.float64, device="cpu", align=0):
+    grid = Grid((n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 3, align)
     phi = sphere_union(grid, n_spheres, seed, dtype=dtype, device=device)
     vel = taylor_green(grid, dtype, device)
     return grid, phi, vel, (BC_PERIODIC,) * 6

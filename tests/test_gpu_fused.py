@@ -149,6 +149,27 @@ def test_advect_fp32_vs_fp64_oracle(ghosts):
     assert err[near].max() <= 1e-5 * min(grid.dx)
 
 
+@pytest.mark.parametrize("real", ["double", "float"])
+@pytest.mark.parametrize("ghosts", ["images", "synth"])
+def test_padded_rows_step_identically(real, ghosts):
+    """A grid whose x ghost region is padded so rows start on 128-byte lines
+    (Grid(align=...)) steps bit-identically to the plain 3-cell ghost layout."""
+    dt_ = torch.float64 if real == "double" else torch.float32
+    outs = []
+    for align in (0, 32):
+        grid = Grid((40, 36, 24), (0.0, 0.0, 0.0), (1.0, 0.9, 0.6), 3, align)
+        phi = sphere_union(grid, n_spheres=5, seed=11, rmin=0.15, rmax=0.3, dtype=dt_).cuda()
+        vel = tuple(v.cuda() for v in taylor_green(grid, dt_))
+        st = Stepper(grid, real, (BC_PERIODIC,) * 6, ghosts=ghosts)
+        st.fill_ghosts(phi)
+        s1, s2 = st.empty(), st.empty()
+        for _ in range(3):
+            st.advect_rk3_step(phi, s1, s2, vel, 0.4 * min(grid.dx))
+        torch.cuda.synchronize()
+        outs.append(grid.interior(phi).cpu())
+    assert torch.equal(outs[0], outs[1])
+
+
 def test_fp32_upwind_switch_threshold():
     """Velocities below the float build's LSMLIB_ZERO_TOL (1e-5) but above the
     double build's (1e-11): by default the FP32 stages add no advection term
