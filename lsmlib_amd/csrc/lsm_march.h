@@ -379,21 +379,23 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* p) {
 
 typedef unsigned int lsm_u32x2 __attribute__((ext_vector_type(2)));
 
-template <typename T>
-__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
-template <>
-__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
-}
-template <>
-__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+// Cache policy: kCached for the stage field (its halo is re-read by the
+// neighbouring tiles), kStream (the non-temporal bit) for data touched once:
+// velocity, phi_cur, phi0 and the output.
+constexpr int kCached = 0, kStream = 2;
+
+template <typename T, int POLICY = kCached>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, POLICY));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, POLICY));
 }
 __device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)off, 0, kStream);
 }
 __device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(lsm_u32x2, v), r, (int)off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(lsm_u32x2, v), r, (int)off, 0, kStream);
 }
 
 template <typename T>
@@ -460,9 +462,10 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
   // z column, unrolled by U planes: for the plane k + u of a group,
   // zv[u + q] = phi(k + u + q), q = 0..3, and zd[u + q] = phi(k + u + q - 2) -
   // phi(k + u + q - 3), q = 0..5; one new value and difference per plane,
-  // the queues shift once per group.
+  // the queues shift once per group.  The loads of plane k + 1 are issued
+  // before the arithmetic of plane k (one plane of software pipelining).
   constexpr int U = 4;
-  T zv[U + 3], zd[U + 5];
+  T zv[U + 4], zd[U + 5];
   {
     T z0[6];
 #pragma unroll
@@ -472,31 +475,45 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
 #pragma unroll
     for (int q = 0; q < 3; q++) zv[q] = z0[q + 3];
   }
+  // plane values of the next plane to process
+  T hx, hy[NR], ux = T(0), uy = T(0), uz = T(0), pcv = T(0), p0v = T(0);
+  auto fetch = [&](T& top) {
+    const __amdgpu_buffer_rsrc_t rps = plane_rsrc(p_ps);
+    top = buf_load<T>(plane_rsrc(p_top), oc);
+    hx = buf_load<T>(rps, ox);
+#pragma unroll
+    for (int n = 0; n < NR; n++) hy[n] = buf_load<T>(rps, oy[n]);
+    if (KIND == STAGE_ADVECT_FIELD) {
+      ux = buf_load<T, kStream>(plane_rsrc(p_vx), ov);
+      uy = buf_load<T, kStream>(plane_rsrc(p_vy), ov);
+      uz = buf_load<T, kStream>(plane_rsrc(p_vz), ov);
+    } else if (KIND == STAGE_ADVECT_CONST) {
+      ux = a.cvx;
+      uy = a.cvy;
+      uz = a.cvz;
+    } else if (a.use_phi0) {
+      p0v = buf_load<T, kStream>(plane_rsrc(p_p0), op0);
+    }
+    if (a.pc.p) pcv = buf_load<T, kStream>(plane_rsrc(p_pc), opc);
+  };
+  auto advance = [&]() {
+    p_ps += s_ps;
+    p_top += s_ps;
+    if (KIND == STAGE_ADVECT_FIELD) {
+      p_vx += s_vel;
+      p_vy += s_vel;
+      p_vz += s_vel;
+    }
+    if (p_pc) p_pc += s_pc;
+    if (p_p0) p_p0 += s_p0;
+  };
+  if (k0 <= k1) fetch(zv[3]);
 
   for (int kg = k0; kg <= k1; kg += U) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int k = kg + u;
       if (k > k1) break;  // uniform
-      const __amdgpu_buffer_rsrc_t rps = plane_rsrc(p_ps);
-      zv[u + 3] = buf_load<T>(plane_rsrc(p_top), oc);
-      const T hx = buf_load<T>(rps, ox);
-      T hy[NR];
-#pragma unroll
-      for (int n = 0; n < NR; n++) hy[n] = buf_load<T>(rps, oy[n]);
-      T ux = T(0), uy = T(0), uz = T(0), pcv = T(0), p0v = T(0);
-      if (KIND == STAGE_ADVECT_FIELD) {
-        ux = buf_load<T>(plane_rsrc(p_vx), ov);
-        uy = buf_load<T>(plane_rsrc(p_vy), ov);
-        uz = buf_load<T>(plane_rsrc(p_vz), ov);
-      } else if (KIND == STAGE_ADVECT_CONST) {
-        ux = a.cvx;
-        uy = a.cvy;
-        uz = a.cvz;
-      } else if (a.use_phi0) {
-        p0v = buf_load<T>(plane_rsrc(p_p0), op0);
-      }
-      if (a.pc.p) pcv = buf_load<T>(plane_rsrc(p_pc), opc);
       __syncthreads();  // the previous plane's LDS reads are done
       tile[ty + 3][tx + 3] = zv[u];
       if (do_x) tile[ty + 3][txl] = hx;
@@ -504,7 +521,10 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
       for (int n = 0; n < NR; n++)
         if (doy[n]) tile[tyr[n]][tx + 3] = hy[n];
       zd[u + 5] = zv[u + 3] - zv[u + 2];
+      const T cux = ux, cuy = uy, cuz = uz, cpc = pcv, cp0 = p0v;
       __syncthreads();
+      advance();
+      if (k < k1) fetch(zv[u + 4]);  // uniform
 
       if (active) {
         T dx_[6], dy_[6];
@@ -523,7 +543,7 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
         }
         const T* dz_ = zd + u;
         const T us = zv[u];
-        const T uc = a.pc.p ? pcv : us;
+        const T uc = a.pc.p ? cpc : us;
         T rhs;
         if (KIND == STAGE_REINIT) {
           T xp, xm, yp, ym, zp, zm;
@@ -531,7 +551,7 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
           hj_weno5_pm_d<T, FASTW>(dy_, a.iy, tiny_dx2y, yp, ym);
           hj_weno5_pm_d<T, FASTW>(dz_, a.iz, tiny_dx2z, zp, zm);
           // lsm_reinitialization3d.f.in:150-268
-          const T cur = a.use_phi0 ? p0v : us;
+          const T cur = a.use_phi0 ? cp0 : us;
           rhs = T(0);
           if (fabs(cur) >= a.zero_tol) {
             const T n2 = godunov_star_sq<T>(cur, xp, yp, zp, xm, ym, zm);
@@ -539,30 +559,21 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
             rhs = sgn * (T(1) - sqrt(n2));
           }
         } else {
-          const T gx = upwind_weno5_d<T, FASTW>(dx_, ux, a.ix, tiny_dx2x, a.zero_tol);
-          const T gy = upwind_weno5_d<T, FASTW>(dy_, uy, a.iy, tiny_dx2y, a.zero_tol);
-          const T gz = upwind_weno5_d<T, FASTW>(dz_, uz, a.iz, tiny_dx2z, a.zero_tol);
+          const T gx = upwind_weno5_d<T, FASTW>(dx_, cux, a.ix, tiny_dx2x, a.zero_tol);
+          const T gy = upwind_weno5_d<T, FASTW>(dy_, cuy, a.iy, tiny_dx2y, a.zero_tol);
+          const T gz = upwind_weno5_d<T, FASTW>(dz_, cuz, a.iz, tiny_dx2z, a.zero_tol);
           // lsm_level_set_evolution3d.f.in:113-116 on a zeroed rhs
-          rhs = -(ux * gx + uy * gy + uz * gz);
+          rhs = -(cux * gx + cuy * gy + cuz * gz);
         }
         const T v = rk_combine<T>(a.form, uc, us, rhs, a.dt);
         const bool img_z = (a.periodic & 4) && (k - fb.klo < 3 || fb.khi - k < 3);
         if (img_xy || img_z) store_with_periodic_images<T>(a.out, fb, i, j, k, v, a.periodic, 3);
         else buf_store(v, plane_rsrc(p_out), oo);
       }
-      p_ps += s_ps;
-      p_top += s_ps;
       p_out += s_out;
-      if (KIND == STAGE_ADVECT_FIELD) {
-        p_vx += s_vel;
-        p_vy += s_vel;
-        p_vz += s_vel;
-      }
-      if (p_pc) p_pc += s_pc;
-      if (p_p0) p_p0 += s_p0;
     }
 #pragma unroll
-    for (int q = 0; q < 3; q++) zv[q] = zv[q + U];
+    for (int q = 0; q < 4; q++) zv[q] = zv[q + U];
 #pragma unroll
     for (int q = 0; q < 5; q++) zd[q] = zd[q + U];
   }
