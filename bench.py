@@ -76,10 +76,8 @@ def run_gpu(real, n, steps, warmup, tiling, ghosts="synth", zero_tol=None):
     del phi64
     dt = stable_advection_dt(vel, grid, 0.5, "double")  # one dt for both precisions
     vel = tuple(v.to(dtype).contiguous() for v in vel)
-    lib = toolbox(real)
     cfg, kz = choose_tiling(real, n) if tiling == "auto" else tuple(int(x) for x in tiling.split(","))
-    lib.lib.lsm3d_amd_set_stage_tiling(cfg, kz)
-    st = Stepper(grid, real, bc, ghosts=ghosts, zero_tol=zero_tol)
+    st = Stepper(grid, real, bc, ghosts=ghosts, zero_tol=zero_tol, tiling=(cfg, kz))
     st.fill_ghosts(phi)
     s1, s2 = st.empty(), st.empty()
     stream = torch.cuda.current_stream(dev)
@@ -131,8 +129,10 @@ def run_batched(real="float", npatch=512, steps=20, reps=2):
     one hipGraph; replays are timed."""
     import torch
     from lsmlib_amd.patches import PatchLevel, random_patch_level
+    from lsmlib_amd.workloads import line_elems
     desc = random_patch_level(npatch=npatch, lo=16, hi=96, seed=3, dx=1.0 / 128)
-    lvl = PatchLevel(desc, real, tiling=(0, 32))
+    dtype = torch.float32 if real == "float" else torch.float64
+    lvl = PatchLevel(desc, real, align=line_elems(dtype))
     phi = lvl.initial_phi()
     s1, s2 = lvl.empty_pool(), lvl.empty_pool()
     dt = lvl.stable_dt(0.5)
@@ -151,6 +151,7 @@ def run_batched(real="float", npatch=512, steps=20, reps=2):
     return {"real": real, "patches": npatch, "cells": lvl.cells, "steps_per_graph": steps,
             "graph_ms": ms, "value": lvl.cells * steps / (ms * 1e-3), "unit": "cell-updates/s",
             "stage_launches_per_graph": 3 * steps, "tiles": int(lvl.tiles), "workgroups": int(lvl.workgroups),
+            "tiling": list(lvl.tiling), "ghosts": "stored, face layers refilled after each stage",
             "achieved_gbs": 8 * word * lvl.cells * steps / (ms * 1e-3) / 1e9,
             "words_per_cell_step": 8}
 

@@ -76,6 +76,7 @@ class ToolboxLibrary(F77Library):
         L.lsm3d_amd_patch_plan_stage.argtypes = [P, I, P, P, P, R, R, R, R, B]
         L.lsm3d_amd_patch_plan_capture_rk3.argtypes = [P, I, P, P, P, R, R, R, R, B]
         L.lsm3d_amd_patch_plan_replay.argtypes = [P]
+        L.lsm3d_amd_patch_plan_fill_ghosts.argtypes = [P, P, B]
         L.lsm3d_amd_set_zero_tol.argtypes = [ctypes.c_double]
         L.lsm3d_amd_set_zero_tol.restype = None
         L.lsm3d_amd_zero_tol.argtypes = []
@@ -88,7 +89,7 @@ class ToolboxLibrary(F77Library):
         for f in ("lsm3d_amd_advect_stage", "lsm3d_amd_advect_const_stage", "lsm3d_amd_reinit_stage",
                   "lsm3d_amd_fill_ghosts", "lsm3d_amd_set_stage_tiling", "lsm3d_amd_patch_plan_destroy",
                   "lsm3d_amd_patch_plan_stage", "lsm3d_amd_patch_plan_capture_rk3",
-                  "lsm3d_amd_patch_plan_replay", "lsm3d_amd_advect_stage_adjoint",
+                  "lsm3d_amd_patch_plan_replay", "lsm3d_amd_patch_plan_fill_ghosts", "lsm3d_amd_advect_stage_adjoint",
                   "lsm3d_amd_heaviside_mismatch"):
             getattr(L, f).restype = None
 

@@ -12,6 +12,16 @@
 // a contiguous range of the tile list (so a patch's halo re-reads stay in
 // one L2) and that XCD's workgroups stride through it.  A tile finds its
 // patch by binary search over the per-patch tile prefix.
+//
+// Ghost cells follow the reference's order, fill then differentiate
+// (LevelSetFunctionIntegrator.cc:1319-1510 fills the boundary ghost cells
+// before each stage): the stages read the stored ghost cells of their input
+// and, after each stage, one batched launch extrapolates the face ghost
+// layers of every patch's output.  Only face ghosts (the three layers over
+// the fill-box range of the two other axes) are written: the cross-shaped
+// stencils never read edge or corner ghost cells, and a face ghost so
+// filled equals what the reference's face-by-face fill over the whole
+// ghost box leaves there (lsm_boundary_conditions3d.f:46-640).
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -32,10 +42,47 @@ struct PatchDesc {
   TileGrid tg;
 };
 
-template <typename T, int TX, int TY, bool SYNTH>
+// Face ghost fill of every patch: workgroup (patch, face) = blockIdx.x
+// covers that face's columns with a 64 x 4 thread layout (no index
+// division); each column's 3 ghost cells get the single-face extrapolation
+// of the two fill-box cells next to the face.  Codes: 0 leave, 1 copy,
+// 2 linear, 3 signed linear.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_patch_fill_faces(const PatchDesc* __restrict__ desc, T* pool, GhostRule g) {
+  const int q = (int)blockIdx.x / 6, face = (int)blockIdx.x % 6;
+  const int code = g.code[face];
+  if (code == 0) return;
+  const PatchDesc d = desc[q];
+  const Box& fb = d.fb;
+  const Field<T> f = make_field<T>(pool + d.offset, d.gb);
+  const int axis = face / 2, hi = face % 2;
+  const int ua = axis == 0 ? 1 : 0, va = axis == 2 ? 1 : 2;  // the two other axes
+  const int flo[3] = {fb.ilo, fb.jlo, fb.klo}, fhi[3] = {fb.ihi, fb.jhi, fb.khi};
+  const int bnd = hi ? fhi[axis] : flo[axis], step = hi ? -1 : 1;
+  const int tx = (int)threadIdx.x % 64, ty = (int)threadIdx.x / 64;
+  for (int v = flo[va] + ty; v <= fhi[va]; v += 4) {
+    for (int u = flo[ua] + tx; u <= fhi[ua]; u += 64) {
+      int idx[3];
+      idx[ua] = u;
+      idx[va] = v;
+      idx[axis] = bnd;
+      const T pb = f.at(idx[0], idx[1], idx[2]);
+      idx[axis] = bnd + step;
+      const T p1 = f.at(idx[0], idx[1], idx[2]);
+#pragma unroll
+      for (int dist = 1; dist <= 3; dist++) {
+        idx[axis] = bnd - step * dist;
+        f.at(idx[0], idx[1], idx[2]) = extrapolate<T>(code, pb, p1, dist);
+      }
+    }
+  }
+}
+
+template <typename T, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY)
     k_patch_stage(const PatchDesc* __restrict__ desc, const int64_t* __restrict__ prefix, int npatch,
-                  int64_t ntiles, T* out_pool, const T* ps_pool, const T* pc_pool, GhostRule ghost, T ix, T iy,
+                  int64_t ntiles, T* out_pool, const T* ps_pool, const T* pc_pool, T ix, T iy,
                   T iz, T dt, T zt, T tdx2x, T tdx2y, T tdx2z, int form) {
   __shared__ T tile[TY + 6][TX + 7];
   const int G = (int)gridDim.x;  // a multiple of the XCD count
@@ -61,7 +108,7 @@ __global__ void __launch_bounds__(TX* TY)
     a.cvz = T(d.vel[2]);
     a.fb = d.fb;
     a.tg = d.tg;
-    a.ghost = ghost;
+    a.ghost = GhostRule{};
     a.ix = ix;
     a.iy = iy;
     a.iz = iz;
@@ -76,10 +123,7 @@ __global__ void __launch_bounds__(TX* TY)
     a.use_phi0 = 0;
     int bx, by, bz;
     tile_coords(d.tg, (int)(t - prefix[l]), bx, by, bz);
-    if (SYNTH && !tile_is_interior<TX, TY>(d.fb, d.tg, bx, by, bz))
-      march<T, TX, TY, STAGE_ADVECT_CONST, true, (sizeof(T) == 4)>(a, bx, by, bz, tile);
-    else
-      march_stored<T, TX, TY, STAGE_ADVECT_CONST, (sizeof(T) == 4)>(a, bx, by, bz, tile);
+    march_stored<T, TX, TY, STAGE_ADVECT_CONST, (sizeof(T) == 4)>(a, bx, by, bz, tile);
   }
 }
 
@@ -96,7 +140,7 @@ struct lsm3d_amd_patch_plan {
   int device = 0;
   std::vector<lsm::PatchDesc> host_desc;
   lsm::PatchDesc* desc = nullptr;
-  int64_t* prefix = nullptr;
+  int64_t* prefix = nullptr;  // tiles before each patch
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
@@ -106,44 +150,48 @@ namespace {
 using namespace lsm;
 typedef real_t R;
 
+void launch_fill(hipStream_t s, const lsm3d_amd_patch_plan* p, R* pool, const GhostRule& g) {
+  if (!g.any) return;
+  hipLaunchKernelGGL(k_patch_fill_faces<R>, dim3((unsigned)(6 * p->npatch)), dim3(256), 0, s, p->desc, pool, g);
+}
+
 template <int TX, int TY>
-void launch_patch_t(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R* ps, const R* pc,
-                    const GhostRule& g, R ix, R iy, R iz, R dt, int form) {
+void launch_patch_t(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R* ps, const R* pc, R ix, R iy,
+                    R iz, R dt, int form) {
   const R zt = (R)current_zero_tol();
   // 12 * tiny * dx^2 per axis (the WENO5 epsilon floor in undivided units)
   const R tx2 = R(12) * Tol<R>::tiny / (ix * ix), ty2 = R(12) * Tol<R>::tiny / (iy * iy);
   const R tz2 = R(12) * Tol<R>::tiny / (iz * iz);
-  if (g.any)
-    hipLaunchKernelGGL((k_patch_stage<R, TX, TY, true>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
-                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, zt, tx2, ty2, tz2, form);
-  else
-    hipLaunchKernelGGL((k_patch_stage<R, TX, TY, false>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
-                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, zt, tx2, ty2, tz2, form);
+  hipLaunchKernelGGL((k_patch_stage<R, TX, TY>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc, p->prefix,
+                     p->npatch, p->ntiles, out, ps, pc, ix, iy, iz, dt, zt, tx2, ty2, tz2, form);
 }
 
+// One stage over every patch (reading stored ghosts of ps), then the face
+// ghost fill of the output.
 void launch_patch(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R* ps, const R* pc,
                   const GhostRule& g, R dx, R dy, R dz, R dt, int form) {
   const R ix = R(1) / dx, iy = R(1) / dy, iz = R(1) / dz;
   switch (p->cfg) {
-    case 0: launch_patch_t<64, 4>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
-    case 1: launch_patch_t<64, 8>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
-    case 2: launch_patch_t<32, 8>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
-    case 3: launch_patch_t<128, 2>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
-    case 4: launch_patch_t<128, 4>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
-    default: launch_patch_t<32, 16>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    case 0: launch_patch_t<64, 4>(s, p, out, ps, pc, ix, iy, iz, dt, form); break;
+    case 1: launch_patch_t<64, 8>(s, p, out, ps, pc, ix, iy, iz, dt, form); break;
+    case 2: launch_patch_t<32, 8>(s, p, out, ps, pc, ix, iy, iz, dt, form); break;
+    case 3: launch_patch_t<128, 2>(s, p, out, ps, pc, ix, iy, iz, dt, form); break;
+    case 4: launch_patch_t<128, 4>(s, p, out, ps, pc, ix, iy, iz, dt, form); break;
+    default: launch_patch_t<32, 16>(s, p, out, ps, pc, ix, iy, iz, dt, form); break;
   }
+  launch_fill(s, p, out, g);
 }
 
 int occupancy_blocks(int cfg) {
   int n = 0;
   hipError_t e = hipSuccess;
   switch (cfg) {
-    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 64, 4, true>, 256, 0); break;
-    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 64, 8, true>, 512, 0); break;
-    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 8, true>, 256, 0); break;
-    case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 2, true>, 256, 0); break;
-    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 4, true>, 512, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 16, true>, 512, 0); break;
+    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 64, 4>, 256, 0); break;
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 64, 8>, 512, 0); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 8>, 256, 0); break;
+    case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 2>, 256, 0); break;
+    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 4>, 512, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 16>, 512, 0); break;
   }
   if (e != hipSuccess || n < 1) {
     (void)hipGetLastError();
@@ -284,6 +332,8 @@ void lsm3d_amd_patch_plan_capture_rk3(lsm3d_amd_patch_plan* p, int nsteps, LSMLI
   device_only(fn, s1_pool, "s1 pool");
   device_only(fn, s2_pool, "s2 pool");
   GhostRule g = make_ghost_rule(fn, bc, p->host_desc[0].fb);
+  for (int f = 0; f < 6 && bc; f++)
+    if (bc[f] == 5) raise_error(std::string(fn) + ": patches take extrapolated or stored ghosts, not periodic");
   if (!c.ok()) {
     c.finish();
     return;
@@ -293,6 +343,7 @@ void lsm3d_amd_patch_plan_capture_rk3(lsm3d_amd_patch_plan* p, int nsteps, LSMLI
   check_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
   check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   if (c.ok()) {
+    launch_fill(cs, p, phi_pool, g);  // the replayed steps start from filled ghosts
     for (int s = 0; s < nsteps; s++) {
       launch_patch(cs, p, s1_pool, phi_pool, nullptr, g, dx, dy, dz, dt, RK_EULER);
       launch_patch(cs, p, s2_pool, s1_pool, phi_pool, g, dx, dy, dz, dt, RK3_STAGE2);
@@ -303,6 +354,22 @@ void lsm3d_amd_patch_plan_capture_rk3(lsm3d_amd_patch_plan* p, int nsteps, LSMLI
   if (c.ok()) check_hip(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
   if (c.ok()) p->graph_steps = nsteps;
   if (cs) (void)hipStreamDestroy(cs);
+  c.finish();
+}
+
+void lsm3d_amd_patch_plan_fill_ghosts(lsm3d_amd_patch_plan* p, LSMLIB_REAL* pool, const int bc[6]) {
+  const char* fn = "lsm3d_amd_patch_plan_fill_ghosts";
+  lsm::Call c(fn);
+  if (!check_plan(fn, p)) return;
+  if (!pool) raise_error(std::string(fn) + ": null pool");
+  device_only(fn, pool, "pool");
+  GhostRule g = make_ghost_rule(fn, bc, p->host_desc[0].fb);
+  for (int f = 0; f < 6 && bc; f++)
+    if (bc[f] == 5) raise_error(std::string(fn) + ": patches take extrapolated or stored ghosts, not periodic");
+  if (c.ok()) {
+    launch_fill(c.stream(), p, pool, g);
+    c.launched("k_patch_fill_faces");
+  }
   c.finish();
 }
 

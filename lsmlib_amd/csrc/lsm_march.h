@@ -50,8 +50,11 @@ struct GhostRule {
   int any;  // nonzero if some face is synthesised
 };
 
+// The product and sum are rounded separately, as in the reference (no fused
+// multiply-add), so extrapolated ghost values are bit-identical to its fill.
 template <typename T>
 __device__ __forceinline__ T extrapolate(int code, T pb, T p1, int dist) {
+#pragma clang fp contract(off)
   if (code == 1) return pb;
   T slope = (code == 2) ? pb - p1 : copysign(T(1), pb) * fabs(pb - p1);
   return pb + slope * T(dist);

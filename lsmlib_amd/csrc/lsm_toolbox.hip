@@ -273,6 +273,7 @@ __global__ void __launch_bounds__(256)
 // mode 0 linear (:46-170), 1 signed linear (:193-330), 2 copy (:352-463)
 template <typename T>
 __global__ void __launch_bounds__(256) k_extrapolate(Field<T> phi, Box gb, Box fb, int loc, int mode) {
+#pragma clang fp contract(off)  // pb + slope * dist rounded twice, as the reference does
   const int d = loc / 2, hi = loc % 2;
   const int u = (d + 1) % 3, v = (d + 2) % 3;
   const int glo[3] = {gb.ilo, gb.jlo, gb.klo}, ghi[3] = {gb.ihi, gb.jhi, gb.khi};

@@ -8,9 +8,10 @@ U[0,1]^3, radii U[0.05, 0.3]) and a constant velocity U[-1,1]^3 per patch.
 `PatchLevel` stores every field of all patches in one pool tensor (each
 patch's ghost box contiguous, column-major) and drives the plan API of
 include/lsm3d_amd.h: per-stage launches, or a captured hipGraph of several
-RK3 steps.  Ghost values come from per-face extrapolation synthesised inside
-the kernel (default signed linear, as LSM3D_SIGNED_LINEAR_EXTRAPOLATION fills
-them between the reference's per-patch stages).
+RK3 steps.  Ghost cells are stored: every stage reads its input's face
+ghosts and refills its output's by per-face extrapolation (default signed
+linear, as LSM3D_SIGNED_LINEAR_EXTRAPOLATION fills them between the
+reference's per-patch stages).
 """
 import ctypes
 
@@ -37,10 +38,15 @@ def random_patch_level(npatch=512, lo=16, hi=96, seed=3, dx=1.0 / 128, ghost=3):
 
 
 class PatchLevel:
-    def __init__(self, desc, real="double", device="cuda", bc=(BC_SIGNED_LINEAR,) * 6, tiling=(0, 32),
-                 zero_tol=None):
+    def __init__(self, desc, real="double", device="cuda", bc=(BC_SIGNED_LINEAR,) * 6, tiling="auto",
+                 zero_tol=None, align=0):
+        """align (elements, 0 = off): pad each patch's x ghost region and
+        round its pool offset so that rows and the first fill-box cell of
+        every row start on `align`-element boundaries (32 FP32 / 16 FP64
+        = 128-byte cache lines), as pylsmlib.Grid does."""
         self.real = real
         self.zero_tol = zero_tol
+        self.align = int(align)
         self.dtype = _DT[real]
         self.device = torch.device(device)
         self.lib = toolbox(real)
@@ -51,12 +57,24 @@ class PatchLevel:
         # index boxes: patch q covers cells 0..n-1 per axis (its own index space)
         self.fbs = np.zeros((self.npatch, 6), dtype=np.int32)
         self.gbs = np.zeros((self.npatch, 6), dtype=np.int32)
+        self.xlo = np.zeros(self.npatch, dtype=np.int64)
+        self.rowlen = np.zeros(self.npatch, dtype=np.int64)
+        al = self.align
         for q, (nx, ny, nz) in enumerate(ext):
+            xlo = max(g, al) if al else g
+            xhi = g + ((-(nx + xlo + g)) % al if al else 0)
             self.fbs[q] = (0, nx - 1, 0, ny - 1, 0, nz - 1)
-            self.gbs[q] = (-g, nx - 1 + g, -g, ny - 1 + g, -g, nz - 1 + g)
-        sizes = np.prod(ext + 2 * g, axis=1).astype(np.int64)
-        self.offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
-        self.pool_elems = int(sizes.sum())
+            self.gbs[q] = (-xlo, nx - 1 + xhi, -g, ny - 1 + g, -g, nz - 1 + g)
+            self.xlo[q], self.rowlen[q] = xlo, nx + xlo + xhi
+        sizes = (self.rowlen * (ext[:, 1] + 2 * g) * (ext[:, 2] + 2 * g)).astype(np.int64)
+        offsets, o = [], 0
+        for n in sizes:
+            offsets.append(o)
+            o += int(n)
+            if al:
+                o = -(-o // al) * al
+        self.offsets = np.array(offsets, dtype=np.int64)
+        self.pool_elems = int(o)
         self.sizes = sizes
         self.cells = int(np.prod(ext, axis=1).sum())
         self.bc = box_c(bc)
@@ -64,7 +82,11 @@ class PatchLevel:
         ct = ctypes.c_double if real == "double" else ctypes.c_float
         vel = np.ascontiguousarray(desc["vel"], dtype=np.float64 if real == "double" else np.float32)
         self._vel = vel
+        if tiling == "auto":  # the autotuned tile shape for patch levels
+            from .tuning import choose_tiling
+            tiling = choose_tiling(real, 0, "patches")
         cfg, kz = tiling
+        self.tiling = (int(cfg), int(kz))
         self.plan = self.lib.fused(
             "lsm3d_amd_patch_plan_create", self.npatch,
             self.fbs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
@@ -88,21 +110,29 @@ class PatchLevel:
         return torch.empty(self.pool_elems, dtype=self.dtype, device=self.device)
 
     def patch_view(self, pool, q):
+        """Patch q's ghost box of `pool` as (nk, nj, ni)."""
         nx, ny, nz = self.desc["ext"][q]
         g = self.desc["ghost"]
         o, n = int(self.offsets[q]), int(self.sizes[q])
-        return pool[o:o + n].view(nz + 2 * g, ny + 2 * g, nx + 2 * g)
+        return pool[o:o + n].view(nz + 2 * g, ny + 2 * g, int(self.rowlen[q]))
+
+    def interior_view(self, pool, q):
+        """Patch q's fill box of `pool` as (nk, nj, ni)."""
+        nx = int(self.desc["ext"][q][0])
+        g, xlo = self.desc["ghost"], int(self.xlo[q])
+        v = self.patch_view(pool, q)
+        return v[g:v.shape[0] - g, g:v.shape[1] - g, xlo:xlo + nx]
 
     def initial_phi(self):
         """phi = min over the patch's spheres of the signed distance, at cell
         centres x = origin + (i + 1/2) dx, over the ghost box."""
-        pool = torch.empty(self.pool_elems, dtype=torch.float64, device=self.device)
-        g, h = self.desc["ghost"], self.dx
+        pool = torch.zeros(self.pool_elems, dtype=torch.float64, device=self.device)
+        h = self.dx
         for q in range(self.npatch):
-            nx, ny, nz = self.desc["ext"][q]
             o = self.desc["origins"][q]
-            ax = [o[d] + (torch.arange(-g, n + g, dtype=torch.float64, device=self.device) + 0.5) * h
-                  for d, n in enumerate((nx, ny, nz))]
+            gb = self.gbs[q]
+            ax = [o[d] + (torch.arange(int(gb[2 * d]), int(gb[2 * d + 1]) + 1, dtype=torch.float64,
+                                       device=self.device) + 0.5) * h for d in range(3)]
             Z, Y, X = torch.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
             phi = torch.full_like(X, float("inf"))
             for c, r in zip(*self.desc["spheres"][q]):
@@ -130,7 +160,13 @@ class PatchLevel:
                        ctypes.c_void_p(ps.data_ptr()), ctypes.c_void_p(pc.data_ptr()) if pc is not None else None,
                        h, h, h, dt, self.bc)
 
+    def fill_ghosts(self, pool):
+        self.lib.fused("lsm3d_amd_patch_plan_fill_ghosts", self.plan, ctypes.c_void_p(pool.data_ptr()), self.bc)
+
     def rk3_step(self, phi, s1, s2, dt):
+        """One TVD-RK3 step of every patch; phi's face ghosts are refilled
+        first (each stage refills its own output's)."""
+        self.fill_ghosts(phi)
         self.stage(RK1, s1, phi, None, dt)
         self.stage(TVDRK3_STAGE2, s2, s1, phi, dt)
         self.stage(TVDRK3_STAGE3, phi, s2, phi, dt)

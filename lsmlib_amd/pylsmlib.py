@@ -128,9 +128,14 @@ class Stepper:
     term) and of the reinitialization cut-off; None keeps the build's
     LSMLIB_ZERO_TOL (1e-11 double, 1e-5 float).  1e-11 in the float library
     runs the double build's switch in FP32 arithmetic.
+
+    tiling: "auto" applies the autotuned (tile shape, z-chunk) for the stage
+    kind, precision and grid size (lsmlib_amd.tuning); a (cfg, kz) pair fixes
+    it; None leaves the library's current setting.
     """
 
-    def __init__(self, grid, real="double", bc=(BC_PERIODIC,) * 6, device="cuda", ghosts="auto", zero_tol=None):
+    def __init__(self, grid, real="double", bc=(BC_PERIODIC,) * 6, device="cuda", ghosts="auto", zero_tol=None,
+                 tiling="auto"):
         if ghosts == "auto":  # the fastest mode for the boundary conditions
             ghosts = "images" if all(int(b) == BC_PERIODIC for b in bc) else "synth"
         if ghosts not in ("synth", "fill", "images"):
@@ -151,6 +156,15 @@ class Stepper:
         self._stage_bc = self._bc if ghosts == "synth" else None
         self.periodic_mask = 7 if ghosts == "images" else 0
         self.zero_tol = zero_tol
+        if tiling == "auto":
+            from .tuning import choose_tiling
+            n = max(grid.n)
+            self._tiling = {k: choose_tiling(real, n, k) for k in ("advect", "reinit")}
+        elif tiling is None:
+            self._tiling = None
+        else:
+            cfg, kz = (int(v) for v in tiling)
+            self._tiling = {"advect": (cfg, kz), "reinit": (cfg, kz)}
         self.sync(False)
 
     def sync(self, flag):
@@ -168,9 +182,14 @@ class Stepper:
             self.fill_ghosts(phi)
 
     # -- advection ---------------------------------------------------------
+    def _configure(self, kind):
+        self.lib.set_zero_tol(self.zero_tol)
+        if self._tiling is not None:
+            self.lib.lib.lsm3d_amd_set_stage_tiling(*self._tiling[kind])
+
     def advect_stage(self, form, out, ps, pc, vel, dt):
         dx, dy, dz = self.grid.dx
-        self.lib.set_zero_tol(self.zero_tol)
+        self._configure("advect")
         if isinstance(vel, (tuple, list)) and len(vel) == 3 and torch.is_tensor(vel[0]):
             self.lib.fused("lsm3d_amd_advect_stage", form, _ptr(out), self._gb, _ptr(ps), self._gb,
                            _ptr(pc), self._gb, _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), self._gb,
@@ -200,7 +219,7 @@ class Stepper:
     # -- reinitialization --------------------------------------------------
     def reinit_stage(self, form, out, ps, pc, phi0, use_phi0, dt):
         dx, dy, dz = self.grid.dx
-        self.lib.set_zero_tol(self.zero_tol)
+        self._configure("reinit")
         self.lib.fused("lsm3d_amd_reinit_stage", form, _ptr(out), self._gb, _ptr(ps), self._gb, _ptr(pc),
                        self._gb, _ptr(phi0) if phi0 is not None else None, self._gb, int(use_phi0),
                        self._fb, dx, dy, dz, dt, self._stage_bc, self.periodic_mask)

@@ -29,8 +29,14 @@ def load_table():
     return {}
 
 
+def _key(real, n, kind):
+    return f"patches/{real}" if kind == "patches" else f"{kind}/{real}/{_size_class(n)}"
+
+
 def choose_tiling(real, n, kind="advect"):
-    t = load_table().get(f"{kind}/{real}/{_size_class(n)}")
+    """(cfg, kz) for a stage kind ("advect", "reinit", or "patches" for the
+    batched patch level, where n is ignored)."""
+    t = load_table().get(_key(real, n, kind))
     return tuple(t) if t else DEFAULT
 
 
@@ -52,7 +58,7 @@ def autotune(reals=("float", "double"), sizes=(128, 256, 512), kind="advect", re
         for n in sizes:
             grid, phi, vel, bc = sphere_union_taylor_green(n, n_spheres=8, dtype=dtype, device=dev,
                                                            align=line_elems(dtype))
-            st = Stepper(grid, real, bc)
+            st = Stepper(grid, real, bc, tiling=None)  # the sweep sets the tiling
             out = st.empty()
             best = None
             for cfg in range(ncfg):
@@ -87,12 +93,62 @@ def autotune(reals=("float", "double"), sizes=(128, 256, 512), kind="advect", re
     return table
 
 
+def autotune_patches(reals=("float", "double"), reps=3, verbose=True, path=None, desc=None):
+    """Times one captured TVD-RK3 step of the batched patch level (512 uneven
+    patches by default) for every (tile shape, kz) and stores the best."""
+    import torch
+    from . import toolbox
+    from .patches import PatchLevel, random_patch_level
+    from .workloads import line_elems
+
+    path = Path(path) if path else TABLE
+    table = json.loads(path.read_text()) if path.exists() else {}
+    desc = desc or random_patch_level()
+    for real in reals:
+        dtype = torch.float32 if real == "float" else torch.float64
+        ncfg = toolbox(real).lib.lsm3d_amd_stage_tiling_count()
+        best = None
+        for cfg in range(ncfg):
+            for kz in KZ_CHOICES:
+                lvl = PatchLevel(desc, real, tiling=(cfg, kz), align=line_elems(dtype))
+                phi = lvl.initial_phi()
+                s1, s2 = lvl.empty_pool(), lvl.empty_pool()
+                lvl.capture_rk3(1, phi, s1, s2, lvl.stable_dt(0.5))
+                lvl.replay()
+                times = []
+                for _ in range(reps):
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    lvl.replay()
+                    e1.record()
+                    e1.synchronize()
+                    times.append(e0.elapsed_time(e1))
+                ms = sorted(times)[len(times) // 2]
+                if verbose:
+                    print(f"  patches {real} cfg={cfg} kz={kz}: {ms:.3f} ms/step", flush=True)
+                if best is None or ms < best[0]:
+                    best = (ms, cfg, kz)
+                del lvl, phi, s1, s2
+        table[_key(real, 0, "patches")] = [best[1], best[2]]
+        if verbose:
+            print(f"autotune patches {real}: cfg={best[1]} kz={best[2]} {best[0]:.3f} ms/step", flush=True)
+        torch.cuda.empty_cache()
+    path.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
+    return table
+
+
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser(description="time every stage tiling and record the fastest")
     ap.add_argument("--out", default=str(TABLE))
     ap.add_argument("--sizes", default="128,256,512")
+    ap.add_argument("--only", default=None, help="advect, reinit or patches")
     a = ap.parse_args()
     sizes = tuple(int(x) for x in a.sizes.split(","))
-    autotune(sizes=sizes, path=a.out)
-    autotune(kind="reinit", sizes=(256,), path=a.out)
+    if a.only in (None, "advect"):
+        autotune(sizes=sizes, path=a.out)
+    if a.only in (None, "reinit"):
+        autotune(kind="reinit", sizes=(256,), path=a.out)
+    if a.only in (None, "patches"):
+        autotune_patches(path=a.out)

@@ -19,9 +19,9 @@ from oracle.steps import OracleStepper, fill_ghosts
 pytestmark = pytest.mark.gpu
 
 
-def small_level(real="double", npatch=7):
+def small_level(real="double", npatch=7, align=0):
     desc = random_patch_level(npatch=npatch, lo=6, hi=14, seed=17, dx=1.0 / 32)
-    return PatchLevel(desc, real, tiling=(0, 8))
+    return PatchLevel(desc, real, tiling=(0, 8), align=align)
 
 
 def oracle_advance(level, phi_pool, dt, steps):
@@ -31,7 +31,8 @@ def oracle_advance(level, phi_pool, dt, steps):
     for q in range(level.npatch):
         nx, ny, nz = (int(v) for v in level.desc["ext"][q])
         grid = Grid((nx, ny, nz), (0.0, 0.0, 0.0), (nx * h, ny * h, nz * h), 3)
-        ph = level.patch_view(phi_pool, q).to(torch.float64).cpu().numpy().copy()
+        xlo = int(level.xlo[q])
+        ph = level.patch_view(phi_pool, q)[:, :, xlo - 3:xlo + nx + 3].to(torch.float64).cpu().numpy().copy()
         vel = tuple(np.full(grid.shape, float(level._vel[q, d])) for d in range(3))
         bc = (BC_SIGNED_LINEAR,) * 6
         st = OracleStepper(ora, grid, bc)
@@ -43,13 +44,12 @@ def oracle_advance(level, phi_pool, dt, steps):
 
 
 def gpu_interiors(level, pool):
-    g = level.desc["ghost"]
-    return [level.patch_view(pool, q)[g:-g, g:-g, g:-g].to(torch.float64).cpu().numpy()
-            for q in range(level.npatch)]
+    return [level.interior_view(pool, q).to(torch.float64).cpu().numpy() for q in range(level.npatch)]
 
 
-def test_patch_level_matches_oracle_fp64():
-    lvl = small_level("double")
+@pytest.mark.parametrize("align", [0, 16])
+def test_patch_level_matches_oracle_fp64(align):
+    lvl = small_level("double", align=align)
     phi0 = lvl.initial_phi()
     dt = lvl.stable_dt(0.5)
     phi, s1, s2 = phi0.clone(), lvl.empty_pool(), lvl.empty_pool()
@@ -108,3 +108,31 @@ def test_tiling_does_not_change_results():
                 ref = out
             else:
                 assert torch.equal(out, ref), f"tiling {cfg}/{kz}"
+
+
+@pytest.mark.parametrize("real", ["double", "float"])
+def test_stage_writes_extrapolated_face_ghosts(real):
+    """After a stage, every patch's face ghost layers of the output hold the
+    signed-linear extrapolation the oracle's face-by-face fill produces from
+    the output's fill box (edges and corners are not read and not compared)."""
+    lvl = small_level(real, npatch=6, align=32 if real == "float" else 16)
+    phi = lvl.initial_phi()
+    s1 = lvl.empty_pool()
+    s1.zero_()
+    lvl.fill_ghosts(phi)
+    lvl.stage(1, s1, phi, None, lvl.stable_dt(0.4))
+    torch.cuda.synchronize()
+    ora = oracle_lib(real)
+    for q in range(lvl.npatch):
+        nx, ny, nz = (int(v) for v in lvl.desc["ext"][q])
+        grid = Grid((nx, ny, nz), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 3)
+        xlo = int(lvl.xlo[q])
+        got = lvl.patch_view(s1, q)[:, :, xlo - 3:xlo + nx + 3].cpu().numpy()
+        want = got.copy()
+        fill_ghosts(ora, want, grid, (BC_SIGNED_LINEAR,) * 6)
+        g = 3
+        faces = [(slice(g, -g), slice(g, -g), slice(0, g)), (slice(g, -g), slice(g, -g), slice(-g, None)),
+                 (slice(g, -g), slice(0, g), slice(g, -g)), (slice(g, -g), slice(-g, None), slice(g, -g)),
+                 (slice(0, g), slice(g, -g), slice(g, -g)), (slice(-g, None), slice(g, -g), slice(g, -g))]
+        for f, sl in enumerate(faces):
+            assert np.array_equal(got[sl], want[sl]), f"patch {q} face {f}"
