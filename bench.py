@@ -57,7 +57,8 @@ def parse():
                     help="skip the batched-patch and reinitialization measurements")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (adjoint) measurement")
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the gradient")
-    ap.add_argument("--adjoint-only", action="store_true", help="only the gradient measurement")
+    ap.add_argument("--only", choices=["batched", "reinit", "adjoint"], default=None,
+                    help="run one secondary measurement alone and print its JSON (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
 
@@ -151,7 +152,7 @@ def run_batched(real="float", npatch=512, steps=20, reps=2):
     return {"real": real, "patches": npatch, "cells": lvl.cells, "steps_per_graph": steps,
             "graph_ms": ms, "value": lvl.cells * steps / (ms * 1e-3), "unit": "cell-updates/s",
             "stage_launches_per_graph": 3 * steps, "tiles": int(lvl.tiles), "workgroups": int(lvl.workgroups),
-            "tiling": list(lvl.tiling), "ghosts": "stored, face layers refilled after each stage",
+            "tiling": list(lvl.tiling), "ghosts": "signed-linear face ghosts synthesised in the stage kernel",
             "achieved_gbs": 8 * word * lvl.cells * steps / (ms * 1e-3) / 1e9,
             "words_per_cell_step": 8}
 
@@ -370,8 +371,14 @@ def main():
                           "launches": 3}), flush=True)
         return
 
-    if a.adjoint_only:
+    if a.only == "adjoint":
         print(json.dumps({"adjoint": run_adjoint(fd=not a.no_fd)}), flush=True)
+        return
+    if a.only == "batched":
+        print(json.dumps({"batched_patches_fp32": run_batched("float")}), flush=True)
+        return
+    if a.only == "reinit":
+        print(json.dumps({"reinit_256": run_reinit()}), flush=True)
         return
     r32, grid32, phi32 = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts, zero_tol=SWITCH_TOL)
     print(f"# fp32: {r32['value']:.4e} cell-updates/s, {r32['ms_per_step']:.3f} ms/step, "

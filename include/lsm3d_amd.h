@@ -419,12 +419,9 @@ void lsm3d_amd_stage_tiling_shape(int cfg, int *tx, int *ty);
  * pool (pool_elems elements each), and the constant velocity
  * vel[3q..3q+2].  The plan keeps device copies of the descriptors; one
  * persistent launch per Runge-Kutta stage covers all patches (the reference
- * loops over patches, LevelSetMethodToolbox.cc:2446-2592).  Ghost cells are
- * stored, in the reference's fill-then-differentiate order: a stage reads
- * the stage pool's ghost cells and then extrapolates the face ghost layers
- * of every patch of its output by bc[6] (codes 0 leave, 1 copy, 2 linear, 3
- * signed linear; not periodic).  Fill the initial field once with
- * lsm3d_amd_patch_plan_fill_ghosts. */
+ * loops over patches, LevelSetMethodToolbox.cc:2446-2592).  Ghost values
+ * follow bc[6] as for the fused stages: synthesised per face (codes 1 copy,
+ * 2 linear, 3 signed linear) or read as stored (code 0); not periodic. */
 typedef struct lsm3d_amd_patch_plan lsm3d_amd_patch_plan;
 lsm3d_amd_patch_plan *lsm3d_amd_patch_plan_create(int npatch, const int *fbs, const int *gbs,
                                                   const long long *offsets, long long pool_elems,
@@ -433,17 +430,17 @@ void lsm3d_amd_patch_plan_destroy(lsm3d_amd_patch_plan *plan);
 long long lsm3d_amd_patch_plan_cells(const lsm3d_amd_patch_plan *plan);
 long long lsm3d_amd_patch_plan_tiles(const lsm3d_amd_patch_plan *plan);
 int lsm3d_amd_patch_plan_workgroups(const lsm3d_amd_patch_plan *plan);
-/* Face ghost layers of every patch of pool by bc[6] (codes 0-3). */
+/* Writes the face ghost layers of every patch of pool by bc[6] (codes 1-3;
+ * 0 leaves a face), for callers that want the extrapolated ghosts stored. */
 void lsm3d_amd_patch_plan_fill_ghosts(lsm3d_amd_patch_plan *plan, LSMLIB_REAL *pool, const int bc[6]);
-/* One Runge-Kutta stage of phi_t + V.grad(phi) = 0 on every patch, followed
- * by the face ghost fill of out_pool. */
+/* One Runge-Kutta stage of phi_t + V.grad(phi) = 0 on every patch. */
 void lsm3d_amd_patch_plan_stage(lsm3d_amd_patch_plan *plan, int rk_form, LSMLIB_REAL *out_pool,
                                 const LSMLIB_REAL *stage_pool, const LSMLIB_REAL *cur_pool,
                                 LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt,
                                 const int bc[6]);
-/* Captures a ghost fill of phi and nsteps TVD-RK3 steps (phi advanced in
- * place through s1, s2) into a hipGraph held by the plan; replay launches it
- * on the library stream. */
+/* Captures nsteps TVD-RK3 steps (phi advanced in place through s1, s2)
+ * into a hipGraph held by the plan; replay launches it on the library
+ * stream. */
 void lsm3d_amd_patch_plan_capture_rk3(lsm3d_amd_patch_plan *plan, int nsteps, LSMLIB_REAL *phi_pool,
                                       LSMLIB_REAL *s1_pool, LSMLIB_REAL *s2_pool, LSMLIB_REAL dx,
                                       LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt,

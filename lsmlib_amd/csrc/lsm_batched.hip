@@ -13,15 +13,15 @@
 // one L2) and that XCD's workgroups stride through it.  A tile finds its
 // patch by binary search over the per-patch tile prefix.
 //
-// Ghost cells follow the reference's order, fill then differentiate
-// (LevelSetFunctionIntegrator.cc:1319-1510 fills the boundary ghost cells
-// before each stage): the stages read the stored ghost cells of their input
-// and, after each stage, one batched launch extrapolates the face ghost
-// layers of every patch's output.  Only face ghosts (the three layers over
-// the fill-box range of the two other axes) are written: the cross-shaped
-// stencils never read edge or corner ghost cells, and a face ghost so
-// filled equals what the reference's face-by-face fill over the whole
-// ghost box leaves there (lsm_boundary_conditions3d.f:46-640).
+// Ghost values: the reference fills every patch's boundary ghost cells
+// before each stage (LevelSetFunctionIntegrator.cc:1319-1510).  Here the
+// stage synthesises, on the tiles that touch a face, the face ghost values
+// its stencils read (single-face extrapolation of the fill-box cells, equal
+// to what the reference's face-by-face fill leaves there,
+// lsm_boundary_conditions3d.f:46-640), so no fill pass and no ghost writes
+// run between stages; faces with code 0 read the stored ghost cells.
+// lsm3d_amd_patch_plan_fill_ghosts writes the face ghost layers of a pool
+// for callers that want them stored.
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -79,10 +79,10 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-template <typename T, int TX, int TY>
+template <typename T, int TX, int TY, bool SYNTH>
 __global__ void __launch_bounds__(TX* TY)
     k_patch_stage(const PatchDesc* __restrict__ desc, const int64_t* __restrict__ prefix, int npatch,
-                  int64_t ntiles, T* out_pool, const T* ps_pool, const T* pc_pool, T ix, T iy,
+                  int64_t ntiles, T* out_pool, const T* ps_pool, const T* pc_pool, GhostRule ghost, T ix, T iy,
                   T iz, T dt, T zt, T tdx2x, T tdx2y, T tdx2z, int form) {
   __shared__ T tile[TY + 6][TX + 7];
   const int G = (int)gridDim.x;  // a multiple of the XCD count
@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(TX* TY)
     a.cvz = T(d.vel[2]);
     a.fb = d.fb;
     a.tg = d.tg;
-    a.ghost = GhostRule{};
+    a.ghost = ghost;
     a.ix = ix;
     a.iy = iy;
     a.iz = iz;
@@ -123,7 +123,10 @@ __global__ void __launch_bounds__(TX* TY)
     a.use_phi0 = 0;
     int bx, by, bz;
     tile_coords(d.tg, (int)(t - prefix[l]), bx, by, bz);
-    march_stored<T, TX, TY, STAGE_ADVECT_CONST, (sizeof(T) == 4)>(a, bx, by, bz, tile);
+    if (SYNTH && !tile_is_interior<TX, TY>(d.fb, d.tg, bx, by, bz))
+      march_tile<T, TX, TY, STAGE_ADVECT_CONST, (sizeof(T) == 4), true>(a, bx, by, bz, tile);
+    else
+      march_tile<T, TX, TY, STAGE_ADVECT_CONST, (sizeof(T) == 4), false>(a, bx, by, bz, tile);
   }
 }
 
@@ -156,42 +159,44 @@ void launch_fill(hipStream_t s, const lsm3d_amd_patch_plan* p, R* pool, const Gh
 }
 
 template <int TX, int TY>
-void launch_patch_t(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R* ps, const R* pc, R ix, R iy,
-                    R iz, R dt, int form) {
+void launch_patch_t(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R* ps, const R* pc,
+                    const GhostRule& g, R ix, R iy, R iz, R dt, int form) {
   const R zt = (R)current_zero_tol();
   // 12 * tiny * dx^2 per axis (the WENO5 epsilon floor in undivided units)
   const R tx2 = R(12) * Tol<R>::tiny / (ix * ix), ty2 = R(12) * Tol<R>::tiny / (iy * iy);
   const R tz2 = R(12) * Tol<R>::tiny / (iz * iz);
-  hipLaunchKernelGGL((k_patch_stage<R, TX, TY>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc, p->prefix,
-                     p->npatch, p->ntiles, out, ps, pc, ix, iy, iz, dt, zt, tx2, ty2, tz2, form);
+  if (g.any)
+    hipLaunchKernelGGL((k_patch_stage<R, TX, TY, true>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
+                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, zt, tx2, ty2, tz2, form);
+  else
+    hipLaunchKernelGGL((k_patch_stage<R, TX, TY, false>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
+                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, zt, tx2, ty2, tz2, form);
 }
 
-// One stage over every patch (reading stored ghosts of ps), then the face
-// ghost fill of the output.
+// One stage over every patch.
 void launch_patch(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R* ps, const R* pc,
                   const GhostRule& g, R dx, R dy, R dz, R dt, int form) {
   const R ix = R(1) / dx, iy = R(1) / dy, iz = R(1) / dz;
   switch (p->cfg) {
-    case 0: launch_patch_t<64, 4>(s, p, out, ps, pc, ix, iy, iz, dt, form); break;
-    case 1: launch_patch_t<64, 8>(s, p, out, ps, pc, ix, iy, iz, dt, form); break;
-    case 2: launch_patch_t<32, 8>(s, p, out, ps, pc, ix, iy, iz, dt, form); break;
-    case 3: launch_patch_t<128, 2>(s, p, out, ps, pc, ix, iy, iz, dt, form); break;
-    case 4: launch_patch_t<128, 4>(s, p, out, ps, pc, ix, iy, iz, dt, form); break;
-    default: launch_patch_t<32, 16>(s, p, out, ps, pc, ix, iy, iz, dt, form); break;
+    case 0: launch_patch_t<64, 4>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    case 1: launch_patch_t<64, 8>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    case 2: launch_patch_t<32, 8>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    case 3: launch_patch_t<128, 2>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    case 4: launch_patch_t<128, 4>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    default: launch_patch_t<32, 16>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
   }
-  launch_fill(s, p, out, g);
 }
 
 int occupancy_blocks(int cfg) {
   int n = 0;
   hipError_t e = hipSuccess;
   switch (cfg) {
-    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 64, 4>, 256, 0); break;
-    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 64, 8>, 512, 0); break;
-    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 8>, 256, 0); break;
-    case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 2>, 256, 0); break;
-    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 4>, 512, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 16>, 512, 0); break;
+    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 64, 4, true>, 256, 0); break;
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 64, 8, true>, 512, 0); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 8, true>, 256, 0); break;
+    case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 2, true>, 256, 0); break;
+    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 4, true>, 512, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 16, true>, 512, 0); break;
   }
   if (e != hipSuccess || n < 1) {
     (void)hipGetLastError();
@@ -343,7 +348,6 @@ void lsm3d_amd_patch_plan_capture_rk3(lsm3d_amd_patch_plan* p, int nsteps, LSMLI
   check_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
   check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   if (c.ok()) {
-    launch_fill(cs, p, phi_pool, g);  // the replayed steps start from filled ghosts
     for (int s = 0; s < nsteps; s++) {
       launch_patch(cs, p, s1_pool, phi_pool, nullptr, g, dx, dy, dz, dt, RK_EULER);
       launch_patch(cs, p, s2_pool, s1_pool, phi_pool, g, dx, dy, dz, dt, RK3_STAGE2);

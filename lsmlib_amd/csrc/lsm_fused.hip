@@ -36,9 +36,9 @@ __global__ void __launch_bounds__(TX* TY) k_stage(StageArgs<T> a) {
   int bx, by, bz;
   tile_coords(a.tg, b, bx, by, bz);
   if (SYNTH && !tile_is_interior<TX, TY>(a.fb, a.tg, bx, by, bz))
-    march<T, TX, TY, KIND, true, (sizeof(T) == 4)>(a, bx, by, bz, tile);
+    march_tile<T, TX, TY, KIND, (sizeof(T) == 4), true>(a, bx, by, bz, tile);
   else
-    march_stored<T, TX, TY, KIND, (sizeof(T) == 4)>(a, bx, by, bz, tile);
+    march_tile<T, TX, TY, KIND, (sizeof(T) == 4), false>(a, bx, by, bz, tile);
 }
 
 // ---------------------------------------------------------------------------

@@ -113,9 +113,8 @@ struct StageArgs {
 };
 
 // ---------------------------------------------------------------------------
-// Fast march: stored ghost cells only, clamped (always in-bounds) loads, one
-// 64-bit plane pointer per field and 32-bit in-plane offsets.  FASTW selects
-// the rearranged WENO5 (weno5_fast) instead of the reference-order form.
+// Per-cell derivatives.  FASTW selects the rearranged WENO5 (weno5_fast)
+// instead of the reference-order form.
 // ---------------------------------------------------------------------------
 // Upwind WENO5 derivative from the six first differences d[q] = p[q+1] - p[q]
 // of the samples p[0..6] around the cell.
@@ -159,221 +158,25 @@ __device__ __forceinline__ void hj_weno5_pm(const T* p, T inv, T tiny_dx2, T& pl
   hj_weno5_pm_d<T, FASTW>(d, inv, tiny_dx2, plus, minus);
 }
 
-template <typename T>
-struct Plane {  // a field addressed as plane pointer + in-plane offset
-  const T* base;
-  int ilo, jlo, klo;
-  int sy;
-  int64_t sz;
-  __device__ __forceinline__ int off(int i, int j) const { return (i - ilo) + sy * (j - jlo); }
-  __device__ __forceinline__ const T* plane(int k) const { return base + sz * (int64_t)(k - klo); }
-};
-
-template <typename T>
-__device__ __forceinline__ Plane<T> plane_of(const Field<const T>& f) {
-  Plane<T> p;
-  p.base = f.p;
-  p.ilo = f.ilo;
-  p.jlo = f.jlo;
-  p.klo = f.klo;
-  p.sy = (int)f.sy;
-  p.sz = f.sz;
-  return p;
-}
-
-// How one thread obtains the stage field along one (i, j) column: stored
-// value at offset o (code 0), or the single-face extrapolation of the two
-// fill-box cells at ob, o1 at distance dist (codes 1-3).  Periodic faces
-// are resolved to a wrapped offset when the rule is built.
-struct ColumnRule {
-  int o, code, ob, o1, dist;
-};
-
-template <typename T>
-__device__ __forceinline__ ColumnRule column_rule(const Plane<T>& ps, const Box& fb, const GhostRule& g,
-                                                  int ci, int cj) {
-  ColumnRule r{ps.off(ci, cj), 0, 0, 0, 0};
-  if (g.code[0] == 5 && ci < fb.ilo) ci += fb.nx();
-  if (g.code[1] == 5 && ci > fb.ihi) ci -= fb.nx();
-  if (g.code[2] == 5 && cj < fb.jlo) cj += fb.ny();
-  if (g.code[3] == 5 && cj > fb.jhi) cj -= fb.ny();
-  r.o = ps.off(ci, cj);
-  if (ci < fb.ilo && g.code[0]) {
-    r.code = g.code[0];
-    r.ob = ps.off(fb.ilo, cj);
-    r.o1 = ps.off(fb.ilo + 1, cj);
-    r.dist = fb.ilo - ci;
-  } else if (ci > fb.ihi && g.code[1]) {
-    r.code = g.code[1];
-    r.ob = ps.off(fb.ihi, cj);
-    r.o1 = ps.off(fb.ihi - 1, cj);
-    r.dist = ci - fb.ihi;
-  } else if (cj < fb.jlo && g.code[2]) {
-    r.code = g.code[2];
-    r.ob = ps.off(ci, fb.jlo);
-    r.o1 = ps.off(ci, fb.jlo + 1);
-    r.dist = fb.jlo - cj;
-  } else if (cj > fb.jhi && g.code[3]) {
-    r.code = g.code[3];
-    r.ob = ps.off(ci, fb.jhi);
-    r.o1 = ps.off(ci, fb.jhi - 1);
-    r.dist = cj - fb.jhi;
-  }
-  return r;
-}
-
-template <typename T, bool GHOST>
-__device__ __forceinline__ T column_load(const ColumnRule& r, const T* pl) {
-  if (GHOST && r.code) return extrapolate<T>(r.code, pl[r.ob], pl[r.o1], r.dist);
-  return pl[r.o];
-}
-
-// Column value at plane kk, applying the z-face rule when kk leaves the
-// fill box (kk depends only on the loop counter: uniform per workgroup).
-template <typename T, bool GHOST>
-__device__ __forceinline__ T column_at(const ColumnRule& r, const Plane<T>& ps, const Box& fb,
-                                       const GhostRule& g, int kk) {
-  if (GHOST) {
-    if (kk < fb.klo && g.code[4]) {
-      if (g.code[4] == 5) return column_load<T, GHOST>(r, ps.plane(kk + fb.nz()));
-      return extrapolate<T>(g.code[4], column_load<T, GHOST>(r, ps.plane(fb.klo)),
-                            column_load<T, GHOST>(r, ps.plane(fb.klo + 1)), fb.klo - kk);
-    }
-    if (kk > fb.khi && g.code[5]) {
-      if (g.code[5] == 5) return column_load<T, GHOST>(r, ps.plane(kk - fb.nz()));
-      return extrapolate<T>(g.code[5], column_load<T, GHOST>(r, ps.plane(fb.khi)),
-                            column_load<T, GHOST>(r, ps.plane(fb.khi - 1)), kk - fb.khi);
-    }
-  }
-  return column_load<T, GHOST>(r, ps.plane(kk));
-}
-
-// The workgroup (TX*TY threads, 1-D) owns tile (bx, by) x z-chunk bz:
-// column (i, j) per thread, planes k0..k1 marched bottom to top.  GHOST:
-// ghost values are synthesised per the rule a.ghost; otherwise stored ghost
-// cells are read.  FASTW selects the rearranged WENO5 form.
-template <typename T, int TX, int TY, int KIND, bool GHOST, bool FASTW>
-__device__ __forceinline__ void march(const StageArgs<T>& a, int bx, int by, int bz, T (*tile)[TX + 7]) {
-  const int t = threadIdx.x;
-  const int tx = t % TX, ty = t / TX;
-  const Box& fb = a.fb;
-  const int i0 = fb.ilo + bx * TX, j0 = fb.jlo + by * TY;
-  const int i = i0 + tx, j = j0 + ty;
-  const int k0 = fb.klo + bz * a.tg.kz;
-  const int k1 = min(k0 + a.tg.kz - 1, fb.khi);
-  const bool active = (i <= fb.ihi) && (j <= fb.jhi);
-  // clamped load positions (values at clamped positions are never used)
-  const int ic = min(i, fb.ihi + 3), jc = min(j, fb.jhi + 3);
-  const Plane<T> ps = plane_of<T>(a.ps);
-  const GhostRule& g = a.ghost;
-  const ColumnRule rc = GHOST ? column_rule<T>(ps, fb, g, ic, jc) : ColumnRule{ps.off(ic, jc), 0, 0, 0, 0};
-  const bool do_x = tx < 6;
-  const int xi = (tx < 3) ? i0 - 3 + tx : min(i0 + TX + tx - 3, fb.ihi + 3);
-  const ColumnRule rx = GHOST ? column_rule<T>(ps, fb, g, xi, jc) : ColumnRule{ps.off(xi, jc), 0, 0, 0, 0};
-  // y halo: 6 rows (3 below, 3 above the tile) spread over the TY thread rows
-  constexpr int NR = (6 + TY - 1) / TY;
-  ColumnRule ry[NR];
-  int tyr[NR];
-  bool doy[NR];
-#pragma unroll
-  for (int n = 0; n < NR; n++) {
-    const int r = ty + n * TY;
-    doy[n] = r < 6;
-    const int yj = !doy[n] ? jc : (r < 3) ? j0 - 3 + r : min(j0 + TY + (r - 3), fb.jhi + 3);
-    tyr[n] = !doy[n] ? 0 : (r < 3) ? r : TY + r;
-    ry[n] = GHOST ? column_rule<T>(ps, fb, g, ic, yj) : ColumnRule{ps.off(ic, yj), 0, 0, 0, 0};
-  }
-  const int txl = (tx < 3) ? tx : TX + tx;
-
-  const Plane<T> vx = plane_of<T>(a.vx), vy = plane_of<T>(a.vy), vz = plane_of<T>(a.vz);
-  const Plane<T> pc = plane_of<T>(a.pc), p0 = plane_of<T>(a.phi0);
-  const int ovel = (KIND == STAGE_ADVECT_FIELD) ? vx.off(ic, jc) : 0;
-  const int opc = a.pc.p ? pc.off(ic, jc) : 0;
-  const int op0 = (KIND == STAGE_REINIT && a.use_phi0) ? p0.off(ic, jc) : 0;
-  T* const outp = a.out.p;
-  const int oout = (i - a.out.ilo) + (int)a.out.sy * (j - a.out.jlo);
-  const T tiny_dx2x = a.tdx2[0], tiny_dx2y = a.tdx2[1], tiny_dx2z = a.tdx2[2];
-
-  T zq[7];
-#pragma unroll
-  for (int q = 0; q < 6; q++) zq[q] = column_at<T, GHOST>(rc, ps, fb, g, k0 - 3 + q);
-
-  for (int k = k0; k <= k1; k++) {
-    const T* pl = ps.plane(k);
-    zq[6] = column_at<T, GHOST>(rc, ps, fb, g, k + 3);
-    const T hx = do_x ? column_load<T, GHOST>(rx, pl) : T(0);
-    T hy[NR];
-#pragma unroll
-    for (int n = 0; n < NR; n++) hy[n] = doy[n] ? column_load<T, GHOST>(ry[n], pl) : T(0);
-    T ux = T(0), uy = T(0), uz = T(0), pcv = T(0), p0v = T(0);
-    if (KIND == STAGE_ADVECT_FIELD) {
-      ux = vx.plane(k)[ovel];
-      uy = vy.plane(k)[ovel];
-      uz = vz.plane(k)[ovel];
-    } else if (KIND == STAGE_ADVECT_CONST) {
-      ux = a.cvx;
-      uy = a.cvy;
-      uz = a.cvz;
-    } else if (a.use_phi0) {
-      p0v = p0.plane(k)[op0];
-    }
-    if (a.pc.p) pcv = pc.plane(k)[opc];
-    __syncthreads();  // the previous plane's LDS reads are done
-    tile[ty + 3][tx + 3] = zq[3];
-    if (do_x) tile[ty + 3][txl] = hx;
-#pragma unroll
-    for (int n = 0; n < NR; n++)
-      if (doy[n]) tile[tyr[n]][tx + 3] = hy[n];
-    __syncthreads();
-
-    if (active) {
-      T px[7], py[7];
-#pragma unroll
-      for (int q = 0; q < 7; q++) {
-        px[q] = tile[ty + 3][tx + q];
-        py[q] = tile[ty + q][tx + 3];
-      }
-      const T us = zq[3];
-      const T uc = a.pc.p ? pcv : us;
-      T rhs;
-      if (KIND == STAGE_REINIT) {
-        T xp, xm, yp, ym, zp, zm;
-        hj_weno5_pm<T, FASTW>(px, a.ix, tiny_dx2x, xp, xm);
-        hj_weno5_pm<T, FASTW>(py, a.iy, tiny_dx2y, yp, ym);
-        hj_weno5_pm<T, FASTW>(zq, a.iz, tiny_dx2z, zp, zm);
-        // lsm_reinitialization3d.f.in:150-268
-        const T cur = a.use_phi0 ? p0v : us;
-        rhs = T(0);
-        if (fabs(cur) >= a.zero_tol) {
-          const T n2 = godunov_star_sq<T>(cur, xp, yp, zp, xm, ym, zm);
-          const T sgn = a.use_phi0 ? cur / sqrt(cur * cur + a.dx_sq) : cur / sqrt(cur * cur + n2 * a.dx_sq);
-          rhs = sgn * (T(1) - sqrt(n2));
-        }
-      } else {
-        const T gx = upwind_weno5<T, FASTW>(px, ux, a.ix, tiny_dx2x, a.zero_tol);
-        const T gy = upwind_weno5<T, FASTW>(py, uy, a.iy, tiny_dx2y, a.zero_tol);
-        const T gz = upwind_weno5<T, FASTW>(zq, uz, a.iz, tiny_dx2z, a.zero_tol);
-        // lsm_level_set_evolution3d.f.in:113-116 on a zeroed rhs
-        rhs = -(ux * gx + uy * gy + uz * gz);
-      }
-      const T v = rk_combine<T>(a.form, uc, us, rhs, a.dt);
-      if (a.periodic) store_with_periodic_images<T>(a.out, fb, i, j, k, v, a.periodic, 3);
-      else (outp + a.out.sz * (int64_t)(k - a.out.klo))[oout] = v;
-    }
-#pragma unroll
-    for (int q = 0; q < 6; q++) zq[q] = zq[q + 1];
-  }
-}
-
 // ---------------------------------------------------------------------------
-// The same stage for stored ghost cells (no synthesis), with the addressing
-// taken off the vector ALU: every field plane is a buffer resource built
-// from a workgroup-uniform plane pointer (advanced by one plane per step in
-// scalar registers) and read with a 32-bit per-lane byte offset fixed for
-// the whole march.  Halo loads are unconditional (lanes without a halo
-// cell read their own column) so no lane masking is needed, and periodic
-// ghost images are written only by tiles and planes within 3 cells of a
-// periodic face.  Results are bit-identical to march<..., GHOST=false, ...>.
+// The z-march of one tile.  The workgroup (TX*TY threads, 1-D) owns tile
+// (bx, by) x z-chunk bz: column (i, j) per thread, planes k0..k1 bottom to
+// top; a (TY+6) x (TX+7) plane of the stage field sits in LDS and the z
+// neighbours of each column in registers.
+//
+// Addressing is kept off the vector ALU: every field plane is a buffer
+// resource built from a workgroup-uniform plane pointer (advanced by one
+// plane per step in scalar registers) and read with a 32-bit per-lane byte
+// offset fixed for the whole march.  Halo loads are unconditional (lanes
+// without a halo cell read their own column), and periodic ghost images are
+// written only by tiles and planes within 3 cells of a periodic face.
+//
+// SYNTH: halo values outside the fill box follow the ghost rule a.ghost per
+// face (extrapolation or periodic image) instead of stored ghost cells.  A
+// cross-shaped stencil leaves the fill box through one face at a time, so
+// only three kinds of loads can need it: the x-halo column and y-halo rows
+// of a lane (a per-lane LoadRule), and the z neighbours of the column near
+// the z faces (uniform per plane).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* p) {
   // raw buffer, no stride, no range limit, 32-bit data format (gfx9 word 3)
@@ -412,8 +215,61 @@ __device__ __forceinline__ const char* plane_ptr(const Field<const T>& f, int k)
   return reinterpret_cast<const char*>(f.p) + (int64_t)(k - f.klo) * f.sz * (int64_t)sizeof(T);
 }
 
-template <typename T, int TX, int TY, int KIND, bool FASTW>
-__device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int by, int bz, T (*tile)[TX + 7]) {
+// How a lane obtains one halo value of the stage field in every plane: the
+// stored cell at byte offset a (code 0), or the single-face extrapolation
+// (codes 1-3) of the fill-box cells at a (on the face) and b (next to it) at
+// distance dist.  Periodic faces are resolved to a wrapped offset here.
+struct LoadRule {
+  uint32_t a, b;
+  int code, dist;
+};
+
+template <typename T, bool SYNTH>
+__device__ __forceinline__ LoadRule load_rule(const Field<const T>& f, const Box& fb, const GhostRule& g, int ci,
+                                              int cj) {
+  if (SYNTH) {
+    if (g.code[0] == 5 && ci < fb.ilo) ci += fb.nx();
+    if (g.code[1] == 5 && ci > fb.ihi) ci -= fb.nx();
+    if (g.code[2] == 5 && cj < fb.jlo) cj += fb.ny();
+    if (g.code[3] == 5 && cj > fb.jhi) cj -= fb.ny();
+    if (ci < fb.ilo && g.code[0])
+      return {plane_byte_offset<T>(f, fb.ilo, cj), plane_byte_offset<T>(f, fb.ilo + 1, cj), g.code[0], fb.ilo - ci};
+    if (ci > fb.ihi && g.code[1])
+      return {plane_byte_offset<T>(f, fb.ihi, cj), plane_byte_offset<T>(f, fb.ihi - 1, cj), g.code[1], ci - fb.ihi};
+    if (cj < fb.jlo && g.code[2])
+      return {plane_byte_offset<T>(f, ci, fb.jlo), plane_byte_offset<T>(f, ci, fb.jlo + 1), g.code[2], fb.jlo - cj};
+    if (cj > fb.jhi && g.code[3])
+      return {plane_byte_offset<T>(f, ci, fb.jhi), plane_byte_offset<T>(f, ci, fb.jhi - 1), g.code[3], cj - fb.jhi};
+  }
+  const uint32_t o = plane_byte_offset<T>(f, ci, cj);
+  return {o, o, 0, 0};
+}
+
+template <typename T, bool SYNTH>
+__device__ __forceinline__ T rule_load(__amdgpu_buffer_rsrc_t r, const LoadRule& lr) {
+  T v = buf_load<T>(r, lr.a);
+  if (SYNTH && lr.code) v = extrapolate<T>(lr.code, v, buf_load<T>(r, lr.b), lr.dist);
+  return v;
+}
+
+// A halo value loaded one plane ahead: both cells of the rule are fetched
+// (b = a when no extrapolation applies) and combined only when consumed, so
+// the loads stay in flight across the previous plane's arithmetic.
+template <typename T, bool SYNTH>
+struct Pending {
+  T va, vb;
+  __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t r, const LoadRule& lr) {
+    va = buf_load<T>(r, lr.a);
+    if (SYNTH) vb = buf_load<T>(r, lr.b);
+  }
+  __device__ __forceinline__ T value(const LoadRule& lr) const {
+    if (SYNTH && lr.code) return extrapolate<T>(lr.code, va, vb, lr.dist);
+    return va;
+  }
+};
+
+template <typename T, int TX, int TY, int KIND, bool FASTW, bool SYNTH>
+__device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by, int bz, T (*tile)[TX + 7]) {
   const int t = threadIdx.x;
   const int tx = t % TX, ty = t / TX;
   const Box& fb = a.fb;
@@ -423,12 +279,17 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
   const int k1 = min(k0 + a.tg.kz - 1, fb.khi);
   const bool active = (i <= fb.ihi) && (j <= fb.jhi);
   const int ic = min(i, fb.ihi + 3), jc = min(j, fb.jhi + 3);  // clamped (unused values)
+  const GhostRule& g = a.ghost;
   const uint32_t oc = plane_byte_offset<T>(a.ps, ic, jc);
+  // The column's own value in planes inside the fill box.  Lanes past the
+  // last column or row of a partial tile hold the x- or y-halo of the active
+  // cells in LDS, so under SYNTH their values follow the ghost rule too.
+  const LoadRule rc = load_rule<T, SYNTH>(a.ps, fb, g, ic, jc);
   const bool do_x = tx < 6;
   const int xi = (tx < 3) ? i0 - 3 + tx : min(i0 + TX + tx - 3, fb.ihi + 3);
-  const uint32_t ox = do_x ? plane_byte_offset<T>(a.ps, xi, jc) : oc;
+  const LoadRule rx = do_x ? load_rule<T, SYNTH>(a.ps, fb, g, xi, jc) : LoadRule{oc, oc, 0, 0};
   constexpr int NR = (6 + TY - 1) / TY;
-  uint32_t oy[NR];
+  LoadRule ry[NR];
   int tyr[NR];
   bool doy[NR];
 #pragma unroll
@@ -437,7 +298,7 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
     doy[n] = r < 6;
     const int yj = !doy[n] ? jc : (r < 3) ? j0 - 3 + r : min(j0 + TY + (r - 3), fb.jhi + 3);
     tyr[n] = !doy[n] ? 0 : (r < 3) ? r : TY + r;
-    oy[n] = plane_byte_offset<T>(a.ps, ic, yj);
+    ry[n] = doy[n] ? load_rule<T, SYNTH>(a.ps, fb, g, ic, yj) : LoadRule{oc, oc, 0, 0};
   }
   const int txl = (tx < 3) ? tx : TX + tx;
   const uint32_t ov = (KIND == STAGE_ADVECT_FIELD) ? plane_byte_offset<T>(a.vx, ic, jc) : 0u;
@@ -462,30 +323,52 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
   const char* p_p0 = (KIND == STAGE_REINIT && a.use_phi0) ? plane_ptr<T>(a.phi0, k0) : nullptr;
   char* p_out = reinterpret_cast<char*>(a.out.p) + (int64_t)(k0 - a.out.klo) * s_out;
 
-  // z column, unrolled by U planes: for the plane k + u of a group,
-  // zv[u + q] = phi(k + u + q), q = 0..3, and zd[u + q] = phi(k + u + q - 2) -
-  // phi(k + u + q - 3), q = 0..5; one new value and difference per plane,
-  // the queues shift once per group.  The loads of plane k + 1 are issued
-  // before the arithmetic of plane k (one plane of software pipelining).
-  constexpr int U = 4;
-  T zv[U + 4], zd[U + 5];
+  // z column: zv[q] = phi(k + q), q = 0..3, and zd[q] = phi(k + q - 2) -
+  // phi(k + q - 3), q = 0..5 (one new value and difference per plane).  The
+  // loads of plane k + 1 are issued before the arithmetic of plane k (one
+  // plane of software pipelining).  The plane loop is not unrolled: every
+  // plane then runs the same compiled arithmetic, so results do not depend
+  // on the z-chunk (unrolled copies may contract multiply-adds differently).
+  constexpr int U = 1;
+  T zv[U + 3], zd[U + 5];
+  // the column's value in plane kk, applying the z-face rule outside the fill
+  // box (kk is uniform; there the value only feeds the column's own z
+  // derivative, so lanes outside the fill box may read any stored cell)
+  auto zval = [&](int kk) -> T {
+    if (SYNTH && kk < fb.klo && g.code[4]) {
+      if (g.code[4] == 5) return buf_load<T>(plane_rsrc(plane_ptr<T>(a.ps, kk + fb.nz())), oc);
+      return extrapolate<T>(g.code[4], buf_load<T>(plane_rsrc(plane_ptr<T>(a.ps, fb.klo)), oc),
+                            buf_load<T>(plane_rsrc(plane_ptr<T>(a.ps, fb.klo + 1)), oc), fb.klo - kk);
+    }
+    if (SYNTH && kk > fb.khi && g.code[5]) {
+      if (g.code[5] == 5) return buf_load<T>(plane_rsrc(plane_ptr<T>(a.ps, kk - fb.nz())), oc);
+      return extrapolate<T>(g.code[5], buf_load<T>(plane_rsrc(plane_ptr<T>(a.ps, fb.khi)), oc),
+                            buf_load<T>(plane_rsrc(plane_ptr<T>(a.ps, fb.khi - 1)), oc), kk - fb.khi);
+    }
+    return rule_load<T, SYNTH>(plane_rsrc(plane_ptr<T>(a.ps, kk)), rc);
+  };
   {
     T z0[6];
 #pragma unroll
-    for (int q = 0; q < 6; q++) z0[q] = buf_load<T>(plane_rsrc(p_ps + (int64_t)(q - 3) * s_ps), oc);
+    for (int q = 0; q < 6; q++) z0[q] = zval(k0 - 3 + q);
 #pragma unroll
     for (int q = 0; q < 5; q++) zd[q] = z0[q + 1] - z0[q];
 #pragma unroll
     for (int q = 0; q < 3; q++) zv[q] = z0[q + 3];
   }
-  // plane values of the next plane to process
-  T hx, hy[NR], ux = T(0), uy = T(0), uz = T(0), pcv = T(0), p0v = T(0);
-  auto fetch = [&](T& top) {
+  // plane values of the next plane to process (halo and top values pending)
+  Pending<T, SYNTH> phx, phy[NR], ptop;
+  bool top_z = false;  // the pending top value came from the z-face rule
+  T ux = T(0), uy = T(0), uz = T(0), pcv = T(0), p0v = T(0);
+  // loads plane kn's values (and the column value 3 planes up)
+  auto fetch = [&](int kn) {
     const __amdgpu_buffer_rsrc_t rps = plane_rsrc(p_ps);
-    top = buf_load<T>(plane_rsrc(p_top), oc);
-    hx = buf_load<T>(rps, ox);
+    top_z = SYNTH && kn + 3 > fb.khi;  // uniform
+    if (top_z) ptop.va = zval(kn + 3);
+    else ptop.load(plane_rsrc(p_top), rc);
+    phx.load(rps, rx);
 #pragma unroll
-    for (int n = 0; n < NR; n++) hy[n] = buf_load<T>(rps, oy[n]);
+    for (int n = 0; n < NR; n++) phy[n].load(rps, ry[n]);
     if (KIND == STAGE_ADVECT_FIELD) {
       ux = buf_load<T, kStream>(plane_rsrc(p_vx), ov);
       uy = buf_load<T, kStream>(plane_rsrc(p_vy), ov);
@@ -510,13 +393,18 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
     if (p_pc) p_pc += s_pc;
     if (p_p0) p_p0 += s_p0;
   };
-  if (k0 <= k1) fetch(zv[3]);
+  if (k0 <= k1) fetch(k0);
 
   for (int kg = k0; kg <= k1; kg += U) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int k = kg + u;
       if (k > k1) break;  // uniform
+      zv[u + 3] = top_z ? ptop.va : ptop.value(rc);
+      const T hx = phx.value(rx);
+      T hy[NR];
+#pragma unroll
+      for (int n = 0; n < NR; n++) hy[n] = phy[n].value(ry[n]);
       __syncthreads();  // the previous plane's LDS reads are done
       tile[ty + 3][tx + 3] = zv[u];
       if (do_x) tile[ty + 3][txl] = hx;
@@ -527,7 +415,7 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
       const T cux = ux, cuy = uy, cuz = uz, cpc = pcv, cp0 = p0v;
       __syncthreads();
       advance();
-      if (k < k1) fetch(zv[u + 4]);  // uniform
+      if (k < k1) fetch(k + 1);  // uniform
 
       if (active) {
         T dx_[6], dy_[6];
@@ -576,7 +464,7 @@ __device__ __forceinline__ void march_stored(const StageArgs<T>& a, int bx, int 
       p_out += s_out;
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++) zv[q] = zv[q + U];
+    for (int q = 0; q < 3; q++) zv[q] = zv[q + U];
 #pragma unroll
     for (int q = 0; q < 5; q++) zd[q] = zd[q + U];
   }

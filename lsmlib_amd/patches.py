@@ -8,10 +8,10 @@ U[0,1]^3, radii U[0.05, 0.3]) and a constant velocity U[-1,1]^3 per patch.
 `PatchLevel` stores every field of all patches in one pool tensor (each
 patch's ghost box contiguous, column-major) and drives the plan API of
 include/lsm3d_amd.h: per-stage launches, or a captured hipGraph of several
-RK3 steps.  Ghost cells are stored: every stage reads its input's face
-ghosts and refills its output's by per-face extrapolation (default signed
-linear, as LSM3D_SIGNED_LINEAR_EXTRAPOLATION fills them between the
-reference's per-patch stages).
+RK3 steps.  Ghost values come from per-face extrapolation synthesised inside
+the stage kernel (default signed linear, as LSM3D_SIGNED_LINEAR_EXTRAPOLATION
+fills them between the reference's per-patch stages); `fill_ghosts` writes
+them into a pool when they are wanted stored.
 """
 import ctypes
 
@@ -164,9 +164,6 @@ class PatchLevel:
         self.lib.fused("lsm3d_amd_patch_plan_fill_ghosts", self.plan, ctypes.c_void_p(pool.data_ptr()), self.bc)
 
     def rk3_step(self, phi, s1, s2, dt):
-        """One TVD-RK3 step of every patch; phi's face ghosts are refilled
-        first (each stage refills its own output's)."""
-        self.fill_ghosts(phi)
         self.stage(RK1, s1, phi, None, dt)
         self.stage(TVDRK3_STAGE2, s2, s1, phi, dt)
         self.stage(TVDRK3_STAGE3, phi, s2, phi, dt)

@@ -111,23 +111,20 @@ def test_tiling_does_not_change_results():
 
 
 @pytest.mark.parametrize("real", ["double", "float"])
-def test_stage_writes_extrapolated_face_ghosts(real):
-    """After a stage, every patch's face ghost layers of the output hold the
-    signed-linear extrapolation the oracle's face-by-face fill produces from
-    the output's fill box (edges and corners are not read and not compared)."""
+def test_fill_ghosts_writes_extrapolated_faces(real):
+    """lsm3d_amd_patch_plan_fill_ghosts: every patch's face ghost layers hold
+    the signed-linear extrapolation the oracle's face-by-face fill produces
+    from the fill box (edges and corners are not written and not compared)."""
     lvl = small_level(real, npatch=6, align=32 if real == "float" else 16)
     phi = lvl.initial_phi()
-    s1 = lvl.empty_pool()
-    s1.zero_()
     lvl.fill_ghosts(phi)
-    lvl.stage(1, s1, phi, None, lvl.stable_dt(0.4))
     torch.cuda.synchronize()
     ora = oracle_lib(real)
     for q in range(lvl.npatch):
         nx, ny, nz = (int(v) for v in lvl.desc["ext"][q])
         grid = Grid((nx, ny, nz), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 3)
         xlo = int(lvl.xlo[q])
-        got = lvl.patch_view(s1, q)[:, :, xlo - 3:xlo + nx + 3].cpu().numpy()
+        got = lvl.patch_view(phi, q)[:, :, xlo - 3:xlo + nx + 3].cpu().numpy()
         want = got.copy()
         fill_ghosts(ora, want, grid, (BC_SIGNED_LINEAR,) * 6)
         g = 3
@@ -136,3 +133,29 @@ def test_stage_writes_extrapolated_face_ghosts(real):
                  (slice(0, g), slice(g, -g), slice(g, -g)), (slice(-g, None), slice(g, -g), slice(g, -g))]
         for f, sl in enumerate(faces):
             assert np.array_equal(got[sl], want[sl]), f"patch {q} face {f}"
+
+
+def test_stored_face_ghosts_equal_synthesised():
+    """Stages with code-0 faces reading ghosts stored by fill_ghosts give the
+    same step as stages synthesising the signed-linear ghosts."""
+    desc = random_patch_level(npatch=6, lo=6, hi=14, seed=23, dx=1.0 / 32)
+    outs = []
+    for bc in ((BC_SIGNED_LINEAR,) * 6, (0,) * 6):
+        lvl = PatchLevel(desc, "double", tiling=(2, 8), bc=bc)
+        phi = lvl.initial_phi()
+        s1, s2 = lvl.empty_pool(), lvl.empty_pool()
+        fill = PatchLevel(desc, "double", tiling=(2, 8))
+        dt = lvl.stable_dt(0.4)
+        for _ in range(2):
+            if bc[0] == 0:
+                fill.fill_ghosts(phi)
+            lvl.stage(1, s1, phi, None, dt)
+            if bc[0] == 0:
+                fill.fill_ghosts(s1)
+            lvl.stage(32, s2, s1, phi, dt)
+            if bc[0] == 0:
+                fill.fill_ghosts(s2)
+            lvl.stage(33, phi, s2, phi, dt)
+        torch.cuda.synchronize()
+        outs.append(torch.cat([torch.from_numpy(a.ravel()) for a in gpu_interiors(lvl, phi)]))
+    assert torch.equal(outs[0], outs[1])
