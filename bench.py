@@ -57,7 +57,8 @@ def parse():
                     help="skip the batched-patch and reinitialization measurements")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (adjoint) measurement")
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the gradient")
-    ap.add_argument("--only", choices=["batched", "reinit", "adjoint"], default=None,
+    ap.add_argument("--only", choices=["batched", "reinit", "adjoint", "config1", "config5", "throughput"],
+                    default=None,
                     help="run one secondary measurement alone and print its JSON (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
@@ -295,6 +296,96 @@ def run_adjoint(n=128, nsteps=40, every=8, reps=3, fd=True):
     return out
 
 
+def run_config1(n=128, steps=100):
+    """Plain FP64 advection, 128^3 on [-1,1]^3, sphere r=0.35 at (0.35,0,0), rigid
+    rotation u=(-y,x,0), signed-linear ghosts, HJ-WENO5 + TVD-RK3, CFL 0.5,
+    100 steps: the GPU FP64 fused path against the reference Fortran FP64
+    toolbox (flang build) run on this host over the same steps."""
+    import numpy as np
+    import torch
+    from lsmlib_amd.pylsmlib import Stepper, stable_advection_dt
+    from lsmlib_amd.workloads import rotating_sphere
+    from oracle.oracle import oracle_lib, ref_lib
+    from oracle.steps import OracleStepper, fill_ghosts
+    dev = torch.device("cuda:0")
+    grid, phi, vel, bc = rotating_sphere(n, torch.float64, dev)
+    st = Stepper(grid, "double", bc)
+    dt = stable_advection_dt(vel, grid, 0.5)
+    p = phi.clone()
+    s1, s2 = st.empty(), st.empty()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.advect_rk3_step(p, s1, s2, vel, dt)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    got = grid.interior(p).cpu().numpy()
+    lib = ref_lib("double")
+    kind = "reference"
+    if lib is None:
+        lib, kind = oracle_lib("double"), "port"
+    ph = phi.cpu().numpy().copy()
+    vn = tuple(v.cpu().numpy().copy() for v in vel)
+    op = OracleStepper(lib, grid, bc)
+    fill_ghosts(lib, ph, grid, bc)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        op.advect_rk3_step(ph, vn, dt)
+    cpu_s = time.perf_counter() - t0
+    want = grid.interior(ph)
+    cells = grid.interior_cells * steps
+    return {"config": f"{n}^3 rotating sphere, signed-linear ghosts, CFL 0.5, {steps} steps, FP64",
+            "gpu_s": gpu_s, "gpu_value": cells / gpu_s, "cpu_s": cpu_s, "cpu_value": cells / cpu_s,
+            "cpu_kind": kind, "cpu_cores": 1, "unit": "cell-updates/s",
+            "max_rel_diff_vs_reference": float(np.abs(got - want).max() / np.abs(want).max())}
+
+
+def run_config5(n=512, steps=200, reinit_every=10, reinit_iters=5):
+    """Config 5 in full: 512^3 periodic sphere union (64 spheres, seed 5),
+    Taylor-Green velocity, WENO5/TVD-RK3 advection with 5 reinitialization
+    iterations (sign from the current phi, pseudo-dt 0.5 dx) every 10 steps,
+    200 steps, FP32 (upwind switch 1e-11) and FP64; error of FP32 vs FP64."""
+    import torch
+    from lsmlib_amd.pylsmlib import Stepper, stable_advection_dt
+    from lsmlib_amd.workloads import line_elems, sphere_union_taylor_green
+    dev = torch.device("cuda:0")
+    out, res = {}, {}
+    for real, dtype in (("float", torch.float32), ("double", torch.float64)):
+        grid, phi64, vel, bc = sphere_union_taylor_green(n, dtype=torch.float64, device=dev,
+                                                         align=line_elems(dtype))
+        dt = stable_advection_dt(vel, grid, 0.5, "double")
+        phi = phi64.to(dtype).clone()
+        del phi64
+        vel = tuple(v.to(dtype).contiguous() for v in vel)
+        st = Stepper(grid, real, bc, zero_tol=SWITCH_TOL if real == "float" else None)
+        st.fill_ghosts(phi)
+        s1, s2 = st.empty(), st.empty()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for step in range(steps):
+            st.advect_rk3_step(phi, s1, s2, vel, dt)
+            if (step + 1) % reinit_every == 0:
+                for _ in range(reinit_iters):
+                    st.reinit_rk3_iteration(phi, s1, s2, 0.5 * grid.dx[0])
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[real] = {"seconds": el, "value": grid.interior_cells * steps / el, "unit": "cell-updates/s"}
+        res[real] = (grid, phi)
+        del s1, s2, vel
+        torch.cuda.empty_cache()
+    out["fp32_error_vs_fp64"] = fp32_error(*res["float"], *res["double"])
+    out["config"] = (f"{n}^3 periodic sphere union + Taylor-Green, {steps} WENO5/TVD-RK3 steps with "
+                     f"{reinit_iters} reinitialization iterations every {reinit_every} steps")
+    return out
+
+
+def run_throughput(n=1024, steps=5, warmup=1):
+    """FP32 advection throughput at 1024^3 (same workload as the headline)."""
+    r, _, _ = run_gpu("float", n, steps, warmup, "auto", "images", zero_tol=SWITCH_TOL)
+    return {"grid": n, "value": r["value"], "unit": "cell-updates/s", "ms_per_step": r["ms_per_step"],
+            "kernel_ms_avg": r["kernel_ms_avg"], "achieved_gbs": r["achieved_gbs"], "tiling": r["tiling"]}
+
+
 def fp32_error(grid32, phi32, grid64, phi64):
     """Max |phi32 - phi64| over the fill box and near the interface (|phi64| < 3 dx)."""
     import torch
@@ -380,6 +471,15 @@ def main():
     if a.only == "reinit":
         print(json.dumps({"reinit_256": run_reinit()}), flush=True)
         return
+    if a.only == "config1":
+        print(json.dumps({"config1_fp64_128": run_config1()}), flush=True)
+        return
+    if a.only == "config5":
+        print(json.dumps({"config5_512_full": run_config5()}), flush=True)
+        return
+    if a.only == "throughput":
+        print(json.dumps({"throughput_1024_fp32": run_throughput()}), flush=True)
+        return
     r32, grid32, phi32 = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts, zero_tol=SWITCH_TOL)
     print(f"# fp32: {r32['value']:.4e} cell-updates/s, {r32['ms_per_step']:.3f} ms/step, "
           f"kernel {r32['kernel_ms_avg']:.3f} ms, {r32['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
@@ -415,6 +515,19 @@ def main():
         print(f"# reinit: fp64 {extras['reinit_256']['double']['ms_per_iteration']:.2f} ms/it, fp32 "
               f"{extras['reinit_256']['float']['ms_per_iteration']:.2f} ms/it", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
+        extras["config5_512_full"] = run_config5()
+        c5 = extras["config5_512_full"]
+        print(f"# config5: fp32 {c5['float']['seconds']:.2f} s, fp64 {c5['double']['seconds']:.2f} s, near-interface "
+              f"error {c5['fp32_error_vs_fp64']['max_near_interface_over_dx']:.3e} dx", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        extras["throughput_1024_fp32"] = run_throughput()
+        print(f"# 1024^3: {extras['throughput_1024_fp32']['value']:.4e} cell-updates/s", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        if not a.no_cpu_baseline:
+            extras["config1_fp64_128"] = run_config1()
+            c1 = extras["config1_fp64_128"]
+            print(f"# config1: gpu {c1['gpu_s']:.2f} s, cpu {c1['cpu_s']:.1f} s, max rel diff "
+                  f"{c1['max_rel_diff_vs_reference']:.2e}", file=sys.stderr, flush=True)
     adj = None
     if not a.no_adjoint:
         adj = run_adjoint(fd=not a.no_fd)

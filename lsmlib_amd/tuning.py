@@ -35,8 +35,16 @@ def _key(real, n, kind):
 
 def choose_tiling(real, n, kind="advect"):
     """(cfg, kz) for a stage kind ("advect", "reinit", or "patches" for the
-    batched patch level, where n is ignored)."""
-    t = load_table().get(_key(real, n, kind))
+    batched patch level, where n is ignored).  A size class without an entry
+    takes the nearest tuned class below it, else the one above, else the
+    default."""
+    table = load_table()
+    t = table.get(_key(real, n, kind))
+    if t is None and kind != "patches":
+        classes = sorted(int(k.rsplit("/", 1)[1]) for k in table if k.startswith(f"{kind}/{real}/"))
+        below = [c for c in classes if c <= _size_class(n)]
+        pick = below[-1] if below else (classes[0] if classes else None)
+        t = table.get(f"{kind}/{real}/{pick}") if pick is not None else None
     return tuple(t) if t else DEFAULT
 
 
