@@ -98,8 +98,11 @@ def run_gpu(real, n, steps, warmup, tiling, ghosts="synth", zero_tol=None):
                 e1.record(stream)
                 events.append((f, e0, e1))
 
-    for _ in range(warmup):
+    one_step = None
+    for w in range(warmup):
         step()
+        if w == 0:
+            one_step = grid.interior(phi).clone()  # the field after exactly one step
     torch.cuda.synchronize()
     events = []
     t0 = time.perf_counter()
@@ -121,7 +124,9 @@ def run_gpu(real, n, steps, warmup, tiling, ghosts="synth", zero_tol=None):
         "kernel_ms_by_stage": {str(f): sum(v) / len(v) for f, v in per_form.items()},
         "achieved_gbs": tot_bytes / (tot_ms * 1e-3) / 1e9,
         "algorithmic_bytes_per_launch": tot_bytes / len(events),
+        "steps_total": warmup + steps,
     }
+    res["_one_step"] = one_step
     return res, grid, phi
 
 
@@ -388,9 +393,13 @@ def run_throughput(n=1024, steps=5, warmup=1):
 
 def fp32_error(grid32, phi32, grid64, phi64):
     """Max |phi32 - phi64| over the fill box and near the interface (|phi64| < 3 dx)."""
+    return interior_error(grid32.interior(phi32), grid64.interior(phi64), grid64)
+
+
+def interior_error(a32, b64, grid64):
     import torch
-    a = grid32.interior(phi32).to(torch.float64)
-    b = grid64.interior(phi64)
+    a = a32.to(torch.float64)
+    b = b64
     err = (a - b).abs()
     h = min(grid64.dx)
     near = b.abs() < 3 * h
@@ -484,16 +493,19 @@ def main():
     print(f"# fp32: {r32['value']:.4e} cell-updates/s, {r32['ms_per_step']:.3f} ms/step, "
           f"kernel {r32['kernel_ms_avg']:.3f} ms, {r32['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     r64 = None
-    err = err_build_tol = None
+    err = err_build_tol = err_one = None
     if not a.fp32_only:
         r64, grid64, phi64 = run_gpu("double", a.n, a.steps, a.warmup, a.tiling, a.ghosts)
         print(f"# fp64: {r64['value']:.4e} cell-updates/s, {r64['ms_per_step']:.3f} ms/step",
               file=sys.stderr, flush=True)
         err = fp32_error(grid32, phi32, grid64, phi64)
+        err["after_steps"] = r32["steps_total"]
+        err_one = interior_error(r32.pop("_one_step"), r64.pop("_one_step"), grid64)
         del phi32
         # the same FP32 run with the float build's LSMLIB_ZERO_TOL (1e-5) in the upwind switch
         _, _, phi32b = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts, zero_tol=None)
         err_build_tol = fp32_error(grid32, phi32b, grid64, phi64)
+        err_build_tol["after_steps"] = r32["steps_total"]
         print(f"# fp32 error vs fp64 near interface: {err['max_near_interface_over_dx']:.3e} dx "
               f"(switch 1e-11), {err_build_tol['max_near_interface_over_dx']:.3e} dx (switch 1e-5)",
               file=sys.stderr, flush=True)
@@ -557,6 +569,7 @@ def main():
                                           "kernel_ms_avg": r64["kernel_ms_avg"],
                                           "achieved_gbs": r64["achieved_gbs"], "tiling": r64["tiling"]},
         "fp32_over_fp64": None if r64 is None else r32["value"] / r64["value"],
+        "fp32_error_vs_fp64_one_step": err_one,
         "fp32_error_vs_fp64": err,
         "fp32_error_vs_fp64_switch_1e-5": err_build_tol,
         "roofline": {"bound": "hbm", "achieved": r32["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -456,7 +456,7 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
           // lsm_level_set_evolution3d.f.in:113-116 on a zeroed rhs
           rhs = -(cux * gx + cuy * gy + cuz * gz);
         }
-        const T v = rk_combine<T>(a.form, uc, us, rhs, a.dt);
+        const T v = rk_combine<T, FASTW>(a.form, uc, us, rhs, a.dt);
         const bool img_z = (a.periodic & 4) && (k - fb.klo < 3 || fb.khi - k < 3);
         if (img_xy || img_z) store_with_periodic_images<T>(a.out, fb, i, j, k, v, a.periodic, 3);
         else buf_store(v, plane_rsrc(p_out), oo);

@@ -194,9 +194,15 @@ __device__ __forceinline__ T godunov_star_sq(T s, T xp, T yp, T zp, T xm, T ym, 
 
 // TVD Runge-Kutta stage combinations (lsm_tvd_runge_kutta3d.f), given the
 // current solution uc, the stage input us and the stage right-hand side r.
+// UNBIASED (the FP32 fused stages) writes TVD-RK3 stage 3 as
+// uc + 2/3 (us + dt r - uc): in FP32 the rounded 1/3 and 2/3 sum to
+// 1 + 3e-8, so the reference's form scales phi by that factor every step
+// (the reference float build does, lsm_tvd_runge_kutta3d.f:481-488); here the
+// rounded 2/3 multiplies only the step's change.  The other stage weights are
+// exact in binary.
 enum RKForm { RK_EULER = 1, RK2_STAGE2 = 22, RK3_STAGE2 = 32, RK3_STAGE3 = 33 };
 
-template <typename T>
+template <typename T, bool UNBIASED = false>
 __device__ __forceinline__ T rk_combine(int form, T uc, T us, T r, T dt) {
   switch (form) {
     case RK2_STAGE2:  // :240-241
@@ -204,6 +210,7 @@ __device__ __forceinline__ T rk_combine(int form, T uc, T us, T r, T dt) {
     case RK3_STAGE2:  // :403-404
       return T(0.75) * uc + T(0.25) * (us + dt * r);
     case RK3_STAGE3:  // :486-488
+      if (UNBIASED) return uc + T(2.0 / 3.0) * ((us + dt * r) - uc);
       return T(1.0 / 3.0) * uc + T(2.0 / 3.0) * (us + dt * r);
     default:  // u_next = u_cur + dt*rhs, :78
       return uc + dt * r;
