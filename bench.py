@@ -57,7 +57,8 @@ def parse():
                     help="skip the batched-patch and reinitialization measurements")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (adjoint) measurement")
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the gradient")
-    ap.add_argument("--only", choices=["batched", "reinit", "adjoint", "config1", "config5", "throughput"],
+    ap.add_argument("--only", choices=["batched", "reinit", "adjoint", "config1", "config5", "throughput",
+                                       "curvature"],
                     default=None,
                     help="run one secondary measurement alone and print its JSON (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -391,6 +392,122 @@ def run_throughput(n=1024, steps=5, warmup=1):
             "kernel_ms_avg": r["kernel_ms_avg"], "achieved_gbs": r["achieved_gbs"], "tiling": r["tiling"]}
 
 
+CURV_PUBLISHED_S = {0.04: 105.0, 0.02: 1276.0}   # curvature_example/README:141-144 (regular method)
+
+
+def run_curvature(dxs=(0.04, 0.02), big_dx=1.5 / 512, big_steps=10, cpu=True):
+    """The curvature example (phi_t + a|grad phi| = b kappa|grad phi|, a = 0.4,
+    b = 0.05, mask = three unit spheres; curvature_model3d.c:41-626) on its
+    default problem.  (1) Whole runs to the example's own stopping rule at
+    dx 0.04 and 0.02, FP64 and FP32, timed end to end (host loop included)
+    beside the published timings; at dx 0.04 the reference Fortran toolbox
+    chain (oracle/_ref) runs the same loop on one host core and the FP64
+    fields are compared.  (2) The model step on the same geometry at
+    dx = 1.5/512 (512 x 512 x 341 cells): per-kernel HIP-event times of the
+    dt reduction, the two fused stages and the ghost fills."""
+    import numpy as np
+    import torch
+    from lsmlib_amd.curvature import CurvatureFlow
+    from lsmlib_amd.pylsmlib import RK1, TVDRK2_STAGE2
+    from lsmlib_amd.workloads import pore_throat
+
+    dev = torch.device("cuda:0")
+    out = {"model": "a=0.4, b=0.05, cfl 0.5, ENO2 / TVD-RK2, mask, reinit every 0.1 (11 iterations)",
+           "runs": {}}
+    finals = {}
+    for real, dt_ in (("double", torch.float64), ("float", torch.float32)):
+        for dx in dxs:
+            g, phi, mask = pore_throat(dx, dt_, dev)
+            m = CurvatureFlow(g, real, mask, device=dev)
+            p = phi.clone()
+            m.run(p, max_outer=1)          # warm-up (kernel loads, allocations)
+            p = phi.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = m.run(p)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            out["runs"][f"{real}_dx{dx}"] = {
+                "grid": list(g.n), "seconds": el, "steps": st["steps"], "intervals": st["outer"],
+                "reinit_iterations": st["reinit_iters"], "t_final": st["t"], "vol_phi": st["vol_phi"],
+                "max_change_last_interval": st["max_abs_err"],
+                "published_reference_s": CURV_PUBLISHED_S.get(dx),
+                "speedup_vs_published": CURV_PUBLISHED_S[dx] / el if dx in CURV_PUBLISHED_S else None}
+            finals[(real, dx)] = (g, p)
+            print(f"# curvature {real} dx={dx} {g.n}: {el:.3f} s, {st['steps']} steps", file=sys.stderr, flush=True)
+    # FP32 against FP64 after the whole run, near the interface
+    for dx in dxs:
+        g, p64 = finals[("double", dx)]
+        _, p32 = finals[("float", dx)]
+        e = interior_error(p32, p64, g)
+        out["runs"][f"float_dx{dx}"]["error_vs_fp64"] = e
+    if cpu:
+        from oracle.oracle import oracle_lib, ref_lib
+        from oracle.steps import CurvatureFlowOracle
+        lib = ref_lib("double")
+        kind = "reference"
+        if lib is None:
+            lib, kind = oracle_lib("double"), "port"
+        g, phi, mask = pore_throat(0.04)
+        pc = phi.numpy().copy()
+        t0 = time.perf_counter()
+        stc = CurvatureFlowOracle(lib, g, mask.numpy().copy()).run(pc)
+        el = time.perf_counter() - t0
+        gpu_p = finals[("double", 0.04)][1].cpu().numpy()
+        out["cpu_dx0.04"] = {"kind": kind, "cores": 1, "seconds": el, "steps": stc["steps"],
+                             "identical_final_field_fp64": bool(np.array_equal(gpu_p, pc)),
+                             "max_abs_diff_fp64": float(np.abs(gpu_p - pc).max())}
+        print(f"# curvature cpu ({kind}) dx=0.04: {el:.2f} s, same field: "
+              f"{out['cpu_dx0.04']['identical_final_field_fp64']}", file=sys.stderr, flush=True)
+    # throughput of the model step at 512 x 512 x 341
+    big = {}
+    for real, dt_ in (("float", torch.float32), ("double", torch.float64)):
+        g, phi, mask = pore_throat(big_dx, dt_, dev)
+        m = CurvatureFlow(g, real, mask, device=dev)
+        p = phi.clone()
+        m.fill_ghosts(p, masked=False)
+        esz = 4 if real == "float" else 8
+        cells = g.n[0] * g.n[1] * g.n[2]
+        ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+        names = ("dt", "stage1", "ghosts1", "stage2", "ghosts2")
+        acc = {k: 0.0 for k in names}
+        m.step(p, None, 0.0, 1.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(big_steps):
+            e = [ev() for _ in range(6)]
+            e[0].record()
+            dt = m._choose_dt(m.normal_dt(p), 0.0, 1.0)
+            e[1].record()
+            m.curvature_stage(RK1, m.s1, p, None, dt)
+            e[2].record()
+            m.fill_ghosts(m.s1)
+            e[3].record()
+            m.curvature_stage(TVDRK2_STAGE2, p, m.s1, p, dt)
+            e[4].record()
+            m.fill_ghosts(p)
+            e[5].record()
+            e[5].synchronize()
+            for q, k in enumerate(names):
+                acc[k] += e[q].elapsed_time(e[q + 1])
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        ms = {k: v / big_steps for k, v in acc.items()}
+        bytes_cell = {"stage1": 3 * esz, "stage2": 4 * esz, "dt": esz}
+        big[real] = {"grid": list(g.n), "cells": cells, "steps": big_steps, "ms_per_step": 1e3 * el / big_steps,
+                     "cell_steps_per_s": cells * big_steps / el, "kernel_ms": ms,
+                     "stage2_gbs": bytes_cell["stage2"] * cells / ms["stage2"] / 1e6,
+                     "stage1_gbs": bytes_cell["stage1"] * cells / ms["stage1"] / 1e6,
+                     "dt_gbs": bytes_cell["dt"] * cells / ms["dt"] / 1e6,
+                     "algorithmic_bytes_per_cell": bytes_cell}
+        print(f"# curvature step {real} {g.n}: {big[real]['ms_per_step']:.3f} ms/step, stage2 "
+              f"{ms['stage2']:.3f} ms ({big[real]['stage2_gbs']:.0f} GB/s)", file=sys.stderr, flush=True)
+        del m, p, phi, mask
+        torch.cuda.empty_cache()
+    out["step_512x512x341"] = big
+    return out
+
+
 def fp32_error(grid32, phi32, grid64, phi64):
     """Max |phi32 - phi64| over the fill box and near the interface (|phi64| < 3 dx)."""
     return interior_error(grid32.interior(phi32), grid64.interior(phi64), grid64)
@@ -489,6 +606,9 @@ def main():
     if a.only == "throughput":
         print(json.dumps({"throughput_1024_fp32": run_throughput()}), flush=True)
         return
+    if a.only == "curvature":
+        print(json.dumps({"curvature_example": run_curvature(cpu=not a.no_cpu_baseline)}), flush=True)
+        return
     r32, grid32, phi32 = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts, zero_tol=SWITCH_TOL)
     print(f"# fp32: {r32['value']:.4e} cell-updates/s, {r32['ms_per_step']:.3f} ms/step, "
           f"kernel {r32['kernel_ms_avg']:.3f} ms, {r32['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
@@ -534,6 +654,8 @@ def main():
         torch.cuda.empty_cache()
         extras["throughput_1024_fp32"] = run_throughput()
         print(f"# 1024^3: {extras['throughput_1024_fp32']['value']:.4e} cell-updates/s", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        extras["curvature_example"] = run_curvature(cpu=not a.no_cpu_baseline)
         torch.cuda.empty_cache()
         if not a.no_cpu_baseline:
             extras["config1_fp64_128"] = run_config1()

@@ -113,6 +113,25 @@ void lsm3dupwindhjweno5_(LSMLIB_REAL *phi_x, LSMLIB_REAL *phi_y, LSMLIB_REAL *ph
                          LSM_AMD_BOX(D1_gb), LSM_AMD_BOX(fb), const LSMLIB_REAL *dx,
                          const LSMLIB_REAL *dy, const LSMLIB_REAL *dz);
 
+/* LSM3D_CENTRAL_GRAD_ORDER2, lsm_spatial_derivatives3d.h:673 (phi read at fb +- 1) */
+void lsm3dcentralgradorder2_(LSMLIB_REAL *phi_x, LSMLIB_REAL *phi_y, LSMLIB_REAL *phi_z,
+                             LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                             LSM_AMD_BOX(fb), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                             const LSMLIB_REAL *dz);
+/* LSM3D_CENTRAL_GRAD_ORDER4, lsm_spatial_derivatives3d.h:723 (phi read at fb +- 2) */
+void lsm3dcentralgradorder4_(LSMLIB_REAL *phi_x, LSMLIB_REAL *phi_y, LSMLIB_REAL *phi_z,
+                             LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                             LSM_AMD_BOX(fb), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                             const LSMLIB_REAL *dz);
+/* LSM3D_LAPLACIAN_ORDER2, lsm_spatial_derivatives3d.h:777 */
+void lsm3dlaplacianorder2_(LSMLIB_REAL *laplacian_phi, LSM_AMD_BOX(laplacian_phi_gb),
+                           const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb), LSM_AMD_BOX(fb),
+                           const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz);
+/* LSM3D_GRADIENT_MAGNITUDE, lsm_spatial_derivatives3d.h:939 */
+void lsm3dgradientmagnitude_(const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
+                             const LSMLIB_REAL *phi_z, LSMLIB_REAL *grad_phi_mag,
+                             LSM_AMD_BOX(grad_phi_gb), LSM_AMD_BOX(fb));
+
 /* --- level set equation RHS: src/toolbox/level_set_evolution/lsm_level_set_evolution3d.h --- */
 /* LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS, lsm_level_set_evolution3d.h:64 */
 void lsm3dzerooutlevelseteqnrhs_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb));
@@ -144,6 +163,17 @@ void lsm3daddconstnormalveltermtolserhs_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_r
                                          const LSMLIB_REAL *phi_z_minus,
                                          LSM_AMD_BOX(grad_phi_minus_gb),
                                          const LSMLIB_REAL *vel_n, LSM_AMD_BOX(fb));
+/* LSM3D_ADD_CONST_CURV_TERM_TO_LSE_RHS, lsm_level_set_evolution3d.h:277:
+ * lse_rhs += b * kappa |grad phi| from central first derivatives (grad_phi_gb)
+ * and second derivatives (grad2_phi_gb) */
+void lsm3daddconstcurvtermtolserhs_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb),
+                                    const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
+                                    const LSMLIB_REAL *phi_z, LSM_AMD_BOX(grad_phi_gb),
+                                    const LSMLIB_REAL *phi_xx, const LSMLIB_REAL *phi_xy,
+                                    const LSMLIB_REAL *phi_xz, const LSMLIB_REAL *phi_yy,
+                                    const LSMLIB_REAL *phi_yz, const LSMLIB_REAL *phi_zz,
+                                    LSM_AMD_BOX(grad2_phi_gb), const LSMLIB_REAL *b,
+                                    LSM_AMD_BOX(fb));
 /* LSM3D_ADD_EXTERNAL_AND_NORMAL_VEL_TERM_TO_LSE_RHS, lsm_level_set_evolution3d.h:398.
  * Computed over the reference's own loop box (its k and j loops run from the
  * fill box to the rhs ghost box, its i loop over the rhs ghost box,
@@ -446,6 +476,58 @@ void lsm3d_amd_patch_plan_capture_rk3(lsm3d_amd_patch_plan *plan, int nsteps, LS
                                       LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt,
                                       const int bc[6]);
 void lsm3d_amd_patch_plan_replay(lsm3d_amd_patch_plan *plan);
+
+/* --- curvature model: phi_t + vel_n |grad phi| = b kappa |grad phi|, phi <= mask ---
+ * The "medium accuracy" path of LSMLIB's curvature example
+ * (examples/serial/curvature_example/curvature_model3d.c:41-626): HJ-ENO2
+ * upwinding, TVD-RK2, second-order central curvature, signed linear ghost
+ * fill on all six faces after every stage, then the mask.
+ * rk_form is 1 (stage 1) or 22 (stage 2; phi_out may alias phi_cur).
+ * phi_stage needs valid ghost cells 2 cells out.  With a mask (not NULL),
+ * phi_out = max(mask, RK result) on the fill box (IMPOSE_MASK) and phi_raw
+ * receives the unmasked result on the two outermost fill-box layers, which
+ * lsm3d_amd_extrapolate_ghosts_masked then extrapolates (the reference fills
+ * the ghost cells before masking).  zero_tol: lsm3d_amd_set_zero_tol.
+ *
+ * One stage of the curvature model: LSM3D_HJ_ENO2 +
+ * LSM3D_ADD_CONST_NORMAL_VEL_TERM_TO_LSE_RHS (vel_n; none when |vel_n| <
+ * zero_tol) + 4 x LSM3D_CENTRAL_GRAD_ORDER2 + LSM3D_ADD_CONST_CURV_TERM_TO_LSE_RHS
+ * (b; none when 0) + LSM3D_TVD_RK2_STAGE* in one pass (curvature_model3d.c:146-414). */
+void lsm3d_amd_curvature_stage(int rk_form, LSMLIB_REAL *phi_out, const int out_gb[6],
+                               LSMLIB_REAL *phi_raw, const int raw_gb[6], const LSMLIB_REAL *phi_stage,
+                               const int stage_gb[6], const LSMLIB_REAL *phi_cur, const int cur_gb[6],
+                               const LSMLIB_REAL *mask, const int mask_gb[6], const int fb[6],
+                               LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL vel_n,
+                               LSMLIB_REAL b, LSMLIB_REAL dt);
+/* One stage of the reinitialization equation with HJ-ENO2 (LSM3D_HJ_ENO2 +
+ * LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS + LSM3D_TVD_RK2_STAGE*,
+ * curvature_model3d.c:484-625), with the optional mask as above. */
+void lsm3d_amd_reinit_eno2_stage(int rk_form, LSMLIB_REAL *phi_out, const int out_gb[6],
+                                 LSMLIB_REAL *phi_raw, const int raw_gb[6], const LSMLIB_REAL *phi_stage,
+                                 const int stage_gb[6], const LSMLIB_REAL *phi_cur, const int cur_gb[6],
+                                 const LSMLIB_REAL *phi0, const int phi0_gb[6], int use_phi0_for_sgn,
+                                 const LSMLIB_REAL *mask, const int mask_gb[6], const int fb[6],
+                                 LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt);
+/* The ghost cells of phi (gb outside fb) as the serial package's six-face
+ * fill leaves them (signedLinearExtrapolationBC(phi, grid, 9),
+ * src/serial/lsm_boundary_conditions.c:306-575, for bc = {3,3,3,3,3,3};
+ * codes 1 copy and 2 linear per face as in lsm3d_amd_fill_ghosts), with the
+ * fill-box values read from src (NULL: phi itself), then max(mask, value)
+ * when mask is not NULL.  Writes ghost cells only. */
+void lsm3d_amd_extrapolate_ghosts_masked(LSMLIB_REAL *phi, const int gb[6], const LSMLIB_REAL *src,
+                                         const int src_gb[6], const LSMLIB_REAL *mask,
+                                         const int mask_gb[6], const int fb[6], const int bc[6]);
+/* *dt = LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT of the HJ-ENO2 derivatives
+ * of phi (lsm_utilities3d.f.in:455-565) without storing them; phi needs
+ * ghost cells 2 cells out.  Synchronizes. */
+void lsm3d_amd_const_normal_vel_dt_eno2(LSMLIB_REAL *dt, const LSMLIB_REAL *phi, const int phi_gb[6],
+                                        const int fb[6], LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz,
+                                        LSMLIB_REAL vel_n, LSMLIB_REAL cfl);
+/* IMPOSE_MASK(phi_masked, mask, phi, grid) of the serial package
+ * (src/serial/lsm_macros.h:84-91): phi_masked[q] = max(mask[q], phi[q]) for
+ * q < num_gridpts (phi_masked may alias phi). */
+void lsm3d_amd_impose_mask(LSMLIB_REAL *phi_masked, const LSMLIB_REAL *mask, const LSMLIB_REAL *phi,
+                           long long num_gridpts);
 
 /* ======================================================================
  * Part 4 - reverse mode (no counterpart in the reference, which has no

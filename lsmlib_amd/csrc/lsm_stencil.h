@@ -114,6 +114,7 @@ __device__ __forceinline__ void first_differences(const T* p, T* d1) {
 // :518-527 (ENO2), :838-868 (ENO3), :1286-1326 (WENO5).
 template <typename T, int S>
 __device__ __forceinline__ T hj_plus(const T* p, T inv) {
+#pragma clang fp contract(off)  // products and sums rounded separately, as the reference's
   T d1[6];
   first_differences<T, S>(p, d1);
   if (S == ENO1) {
@@ -142,6 +143,7 @@ __device__ __forceinline__ T hj_plus(const T* p, T inv) {
 // :529-535 (ENO2), :872-896 (ENO3), :1330-1370 (WENO5).
 template <typename T, int S>
 __device__ __forceinline__ T hj_minus(const T* p, T inv) {
+#pragma clang fp contract(off)  // products and sums rounded separately, as the reference's
   T d1[6];
   first_differences<T, S>(p, d1);
   if (S == ENO1) {
@@ -178,6 +180,7 @@ __device__ __forceinline__ T hj_upwind(const T* p, T vel, T inv) {
 // (lsm_reinitialization3d.f.in:168-185); returns |grad phi|*^2
 template <typename T>
 __device__ __forceinline__ T godunov_star_sq(T s, T xp, T yp, T zp, T xm, T ym, T zm) {
+#pragma clang fp contract(off)  // products and sums rounded separately, as the reference's
   const T z = T(0);
   T gx, gy, gz;
   if (s > z) {
@@ -192,6 +195,50 @@ __device__ __forceinline__ T godunov_star_sq(T s, T xp, T yp, T zp, T xm, T ym, 
   return gx * gx + gy * gy + gz * gz;
 }
 
+// Godunov |grad phi|^2 for normal speed vn, lsm_level_set_evolution3d.f.in:206-225
+template <typename T>
+__device__ __forceinline__ T godunov_norm_sq(T vn, T xp, T yp, T zp, T xm, T ym, T zm) {
+#pragma clang fp contract(off)  // products and sums rounded separately, as the reference's
+  const T z = T(0);
+  if (vn > z) {
+    const T a = fmax(xm, z), b = fmin(xp, z), c = fmax(ym, z), d = fmin(yp, z), e = fmax(zm, z),
+            f = fmin(zp, z);
+    return fmax(a * a, b * b) + fmax(c * c, d * d) + fmax(e * e, f * f);
+  }
+  const T a = fmin(xm, z), b = fmax(xp, z), c = fmin(ym, z), d = fmax(yp, z), e = fmin(zm, z),
+          f = fmax(zp, z);
+  return fmax(a * a, b * b) + fmax(c * c, d * d) + fmax(e * e, f * f);
+}
+
+// kappa |grad phi| from central first and second derivatives,
+// lsm_level_set_evolution3d.f.in:433-447 (0 where |grad phi|^2 < zero_tol),
+// every product and sum rounded as the reference rounds it.
+template <typename T>
+__device__ __forceinline__ T curvature_term(T x, T y, T z, T xx, T xy, T xz, T yy, T yz, T zz, T zero_tol) {
+#pragma clang fp contract(off)
+  const T g2 = (x * x + y * y) + z * z;
+  if (g2 < zero_tol) return T(0);
+  T n = (xx * y) * y + (yy * x) * x;
+  n = n - ((T(2) * xy) * x) * y;
+  n = n + (xx * z) * z;
+  n = n + (zz * x) * x;
+  n = n - ((T(2) * xz) * x) * z;
+  n = n + (yy * z) * z;
+  n = n + (zz * y) * y;
+  n = n - ((T(2) * yz) * y) * z;
+  return n / g2;
+}
+
+// CFL speed of a normal velocity, lsm_utilities3d.f.in:404-429
+template <typename T>
+__device__ __forceinline__ T h_over_dx(T avn, T xp, T yp, T zp, T xm, T ym, T zm, T mdx2, T ix, T iy,
+                                       T iz) {
+#pragma clang fp contract(off)  // products and sums rounded separately, as the reference's
+  const T px = fmax(fabs(xp), fabs(xm)), py = fmax(fabs(yp), fabs(ym)), pz = fmax(fabs(zp), fabs(zm));
+  const T ng = sqrt(px * px + py * py + pz * pz + mdx2);
+  return avn / ng * (px * ix + py * iy + pz * iz);
+}
+
 // TVD Runge-Kutta stage combinations (lsm_tvd_runge_kutta3d.f), given the
 // current solution uc, the stage input us and the stage right-hand side r.
 // UNBIASED (the FP32 fused stages) writes TVD-RK3 stage 3 as
@@ -204,6 +251,7 @@ enum RKForm { RK_EULER = 1, RK2_STAGE2 = 22, RK3_STAGE2 = 32, RK3_STAGE3 = 33 };
 
 template <typename T, bool UNBIASED = false>
 __device__ __forceinline__ T rk_combine(int form, T uc, T us, T r, T dt) {
+#pragma clang fp contract(off)  // products and sums rounded separately, as the reference's
   switch (form) {
     case RK2_STAGE2:  // :240-241
       return T(0.5) * (uc + us + dt * r);

@@ -70,6 +70,68 @@ __global__ void __launch_bounds__(256)
 }
 
 // ============================================================================
+// Central differences  (lsm_spatial_derivatives3d.f.in:3016-3264, 3586-3650)
+// Products and sums are rounded one by one, as the reference's are.
+// ============================================================================
+// :3016-3088
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_central_grad2(Field<T> gx, Field<T> gy, Field<T> gz, Field<const T> phi, Box fb, T fx, T fy, T fz) {
+#pragma clang fp contract(off)
+  LSM_CELL_LOOP_BEGIN(fb)
+  gx.at(i, j, k) = (phi.at(i + 1, j, k) - phi.at(i - 1, j, k)) * fx;
+  gy.at(i, j, k) = (phi.at(i, j + 1, k) - phi.at(i, j - 1, k)) * fy;
+  gz.at(i, j, k) = (phi.at(i, j, k + 1) - phi.at(i, j, k - 1)) * fz;
+  LSM_CELL_LOOP_END
+}
+
+// :3094-3174
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_central_grad4(Field<T> gx, Field<T> gy, Field<T> gz, Field<const T> phi, Box fb, T fx, T fy, T fz) {
+#pragma clang fp contract(off)
+  const T e = T(8);
+  LSM_CELL_LOOP_BEGIN(fb)
+  gx.at(i, j, k) = (((-phi.at(i + 2, j, k) + e * phi.at(i + 1, j, k)) + phi.at(i - 2, j, k)) -
+                    e * phi.at(i - 1, j, k)) * fx;
+  gy.at(i, j, k) = (((-phi.at(i, j + 2, k) + e * phi.at(i, j + 1, k)) + phi.at(i, j - 2, k)) -
+                    e * phi.at(i, j - 1, k)) * fy;
+  gz.at(i, j, k) = (((-phi.at(i, j, k + 2) + e * phi.at(i, j, k + 1)) + phi.at(i, j, k - 2)) -
+                    e * phi.at(i, j, k - 1)) * fz;
+  LSM_CELL_LOOP_END
+}
+
+// :3180-3264
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_laplacian2(Field<T> lap, Field<const T> phi, Box fb, T ix2, T iy2, T iz2) {
+#pragma clang fp contract(off)
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T c = T(2) * phi.at(i, j, k);
+  lap.at(i, j, k) = (ix2 * ((phi.at(i + 1, j, k) - c) + phi.at(i - 1, j, k)) +
+                     iy2 * ((phi.at(i, j + 1, k) - c) + phi.at(i, j - 1, k))) +
+                    iz2 * ((phi.at(i, j, k + 1) - c) + phi.at(i, j, k - 1));
+  LSM_CELL_LOOP_END
+}
+
+// :3586-3650 (tol = 1e-8 on the squares and on the magnitude)
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_grad_mag(Field<T> mag, Field<const T> gx, Field<const T> gy, Field<const T> gz, Box fb) {
+#pragma clang fp contract(off)
+  const T tol = T(1e-8), z = T(0);
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T a = gx.at(i, j, k), b = gy.at(i, j, k), c = gz.at(i, j, k);
+  T x2 = a * a, y2 = b * b, z2 = c * c;
+  if (x2 < tol) x2 = z;
+  if (y2 < tol) y2 = z;
+  if (z2 < tol) z2 = z;
+  const T t = sqrt((x2 + y2) + z2);
+  mag.at(i, j, k) = (t < tol) ? z : t;
+  LSM_CELL_LOOP_END
+}
+
+// ============================================================================
 // Level set equation RHS  (lsm_level_set_evolution3d.f.in)
 // ============================================================================
 template <typename T>
@@ -90,20 +152,6 @@ __global__ void __launch_bounds__(256)
   LSM_CELL_LOOP_END
 }
 
-// Godunov |grad phi|^2 for normal speed vn, :206-225
-template <typename T>
-__device__ __forceinline__ T godunov_norm_sq(T vn, T xp, T yp, T zp, T xm, T ym, T zm) {
-  const T z = T(0);
-  if (vn > z) {
-    const T a = fmax(xm, z), b = fmin(xp, z), c = fmax(ym, z), d = fmin(yp, z), e = fmax(zm, z),
-            f = fmin(zp, z);
-    return fmax(a * a, b * b) + fmax(c * c, d * d) + fmax(e * e, f * f);
-  }
-  const T a = fmin(xm, z), b = fmax(xp, z), c = fmin(ym, z), d = fmax(yp, z), e = fmin(zm, z),
-          f = fmax(zp, z);
-  return fmax(a * a, b * b) + fmax(c * c, d * d) + fmax(e * e, f * f);
-}
-
 // :131-242 (field vn) and :248-350 (constant vn: vn_field.p == nullptr)
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -117,6 +165,21 @@ __global__ void __launch_bounds__(256)
                                     ym.at(i, j, k), zm.at(i, j, k));
     rhs.at(i, j, k) = rhs.at(i, j, k) - vn * sqrt(n2);
   }
+  LSM_CELL_LOOP_END
+}
+
+// :356-460
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_add_const_curv(Field<T> rhs, Field<const T> px, Field<const T> py, Field<const T> pz, Field<const T> pxx,
+                     Field<const T> pxy, Field<const T> pxz, Field<const T> pyy, Field<const T> pyz,
+                     Field<const T> pzz, Box fb, T b) {
+#pragma clang fp contract(off)
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T curv = curvature_term<T>(px.at(i, j, k), py.at(i, j, k), pz.at(i, j, k), pxx.at(i, j, k),
+                                   pxy.at(i, j, k), pxz.at(i, j, k), pyy.at(i, j, k), pyz.at(i, j, k),
+                                   pzz.at(i, j, k), Tol<T>::zero);
+  rhs.at(i, j, k) = rhs.at(i, j, k) + b * curv;
   LSM_CELL_LOOP_END
 }
 
@@ -347,15 +410,6 @@ enum RedKind {
   RK_VOL_REG_LT = 7,
   RK_VOL_REG_GT = 8
 };
-
-template <typename T>
-__device__ __forceinline__ T h_over_dx(T avn, T xp, T yp, T zp, T xm, T ym, T zm, T mdx2, T ix, T iy,
-                                       T iz) {
-  // lsm_utilities3d.f.in:404-429
-  const T px = fmax(fabs(xp), fabs(xm)), py = fmax(fabs(yp), fabs(ym)), pz = fmax(fabs(zp), fabs(zm));
-  const T ng = sqrt(px * px + py * py + pz * pz + mdx2);
-  return avn / ng * (px * ix + py * iy + pz * iz);
-}
 
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -648,6 +702,62 @@ void lsm3dupwindhjweno5_(R* gx, R* gy, R* gz, BOXP(g), const R* phi, BOXP(phi), 
                          BOX(fb), *dx, *dy, *dz);
 }
 
+// ---- central differences (lsm_spatial_derivatives3d.h:673-960) ------------------
+// The factors are formed in double and rounded to LSMLIB_REAL, as the
+// reference's d0 literals make them (:3058-3060, :3138-3140, :3220-3222).
+static void central_grad_entry(const char* fn, int order, R* gx, R* gy, R* gz, Box gg, const R* phi, Box gphi,
+                               Box fb, R dx, R dy, R dz) {
+  Call c(fn);
+  const int r = order / 2;
+  require_inside(fn, "grad_phi", gg, fb, 0, 0, 0);
+  require_inside(fn, "phi", gphi, fb, r, r, r);
+  auto fx = c.out(gx, gg), fy = c.out(gy, gg), fz = c.out(gz, gg);
+  auto fp = c.in(phi, gphi);
+  if (order == 2)
+    launch_cells(c, "k_central_grad2", fb, k_central_grad2<R>, fx, fy, fz, fp, fb, R(0.5 / (double)dx),
+                 R(0.5 / (double)dy), R(0.5 / (double)dz));
+  else
+    launch_cells(c, "k_central_grad4", fb, k_central_grad4<R>, fx, fy, fz, fp, fb,
+                 R(0.0833333333333333333333 / (double)dx), R(0.0833333333333333333333 / (double)dy),
+                 R(0.0833333333333333333333 / (double)dz));
+  c.finish();
+}
+
+void lsm3dcentralgradorder2_(R* gx, R* gy, R* gz, BOXP(g), const R* phi, BOXP(phi), BOXP(fb), const R* dx,
+                             const R* dy, const R* dz) {
+  central_grad_entry("LSM3D_CENTRAL_GRAD_ORDER2", 2, gx, gy, gz, BOX(g), phi, BOX(phi), BOX(fb), *dx, *dy, *dz);
+}
+
+void lsm3dcentralgradorder4_(R* gx, R* gy, R* gz, BOXP(g), const R* phi, BOXP(phi), BOXP(fb), const R* dx,
+                             const R* dy, const R* dz) {
+  central_grad_entry("LSM3D_CENTRAL_GRAD_ORDER4", 4, gx, gy, gz, BOX(g), phi, BOX(phi), BOX(fb), *dx, *dy, *dz);
+}
+
+void lsm3dlaplacianorder2_(R* lap, BOXP(l), const R* phi, BOXP(phi), BOXP(fb), const R* dx, const R* dy,
+                           const R* dz) {
+  const char* fn = "LSM3D_LAPLACIAN_ORDER2";
+  Call c(fn);
+  Box fb = BOX(fb);
+  require_inside(fn, "laplacian_phi", BOX(l), fb, 0, 0, 0);
+  require_inside(fn, "phi", BOX(phi), fb, 1, 1, 1);
+  auto fl = c.out(lap, BOX(l));
+  auto fp = c.in(phi, BOX(phi));
+  const double x = *dx, y = *dy, z = *dz;
+  launch_cells(c, "k_laplacian2", fb, k_laplacian2<R>, fl, fp, fb, R(1.0 / x / x), R(1.0 / y / y), R(1.0 / z / z));
+  c.finish();
+}
+
+void lsm3dgradientmagnitude_(const R* gx, const R* gy, const R* gz, R* mag, BOXP(g), BOXP(fb)) {
+  const char* fn = "LSM3D_GRADIENT_MAGNITUDE";
+  Call c(fn);
+  Box fb = BOX(fb);
+  require_inside(fn, "grad_phi", BOX(g), fb, 0, 0, 0);
+  auto fm = c.out(mag, BOX(g));
+  auto fx = c.in(gx, BOX(g)), fy = c.in(gy, BOX(g)), fz = c.in(gz, BOX(g));
+  launch_cells(c, "k_grad_mag", fb, k_grad_mag<R>, fm, fx, fy, fz, fb);
+  c.finish();
+}
+
 // ---- level set equation RHS (lsm_level_set_evolution3d.h) --------------------
 void lsm3dzerooutlevelseteqnrhs_(R* rhs, BOXP(r)) {
   const char* fn = "LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS";
@@ -706,6 +816,23 @@ void lsm3daddconstnormalveltermtolserhs_(R* rhs, BOXP(r), const R* xp, const R* 
     auto e = c.in(xm, BOX(gm)), f = c.in(ym, BOX(gm)), g = c.in(zm, BOX(gm));
     launch_cells(c, "k_add_normal_vel", fb, k_add_normal_vel<R>, fr, a, b, d, e, f, g, no_field(), *vn, fb);
   }
+  c.finish();
+}
+
+void lsm3daddconstcurvtermtolserhs_(R* rhs, BOXP(r), const R* px, const R* py, const R* pz, BOXP(g1),
+                                    const R* pxx, const R* pxy, const R* pxz, const R* pyy, const R* pyz,
+                                    const R* pzz, BOXP(g2), const R* b, BOXP(fb)) {
+  const char* fn = "LSM3D_ADD_CONST_CURV_TERM_TO_LSE_RHS";
+  Call c(fn);
+  Box fb = BOX(fb);
+  require_inside(fn, "lse_rhs", BOX(r), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi", BOX(g1), fb, 0, 0, 0);
+  require_inside(fn, "grad2_phi", BOX(g2), fb, 0, 0, 0);
+  auto fr = c.out(rhs, BOX(r));
+  auto x = c.in(px, BOX(g1)), y = c.in(py, BOX(g1)), z = c.in(pz, BOX(g1));
+  auto xx = c.in(pxx, BOX(g2)), xy = c.in(pxy, BOX(g2)), xz = c.in(pxz, BOX(g2));
+  auto yy = c.in(pyy, BOX(g2)), yz = c.in(pyz, BOX(g2)), zz = c.in(pzz, BOX(g2));
+  launch_cells(c, "k_add_const_curv", fb, k_add_const_curv<R>, fr, x, y, z, xx, xy, xz, yy, yz, zz, fb, *b);
   c.finish();
 }
 

@@ -49,12 +49,17 @@ class Grid:
     `align`-element boundary (32 for FP32 and 16 for FP64 put them on
     128-byte cache-line boundaries of a 256-byte aligned allocation).  The
     kernels still read only `ghost` cells beyond the fill box; the extra
-    cells are padding."""
+    cells are padding.
+
+    spacing: the grid spacing per axis when given (the serial package's
+    createGridSetDx takes dx itself, lsm_grid.c:50-114); otherwise
+    (upper - lower) / n."""
     n: tuple
     lower: tuple
     upper: tuple
     ghost: int = 3
     align: int = 0
+    spacing: tuple = None
 
     @property
     def xlo(self):  # ghost cells below the fill box along x
@@ -82,6 +87,8 @@ class Grid:
 
     @property
     def dx(self):
+        if self.spacing is not None:
+            return tuple(float(h) for h in self.spacing)
         return tuple((self.upper[d] - self.lower[d]) / self.n[d] for d in range(3))
 
     @property

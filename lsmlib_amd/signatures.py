@@ -25,6 +25,10 @@ _TABLE = [
     ("lsm3dupwindhjeno2_", "AAAB AB AAAB AB AB B RRR", "lsm_spatial_derivatives3d.h:459"),
     ("lsm3dupwindhjeno3_", "AAAB AB AAAB AB AB AB B RRR", "lsm_spatial_derivatives3d.h:531"),
     ("lsm3dupwindhjweno5_", "AAAB AB AAAB AB B RRR", "lsm_spatial_derivatives3d.h:608"),
+    ("lsm3dcentralgradorder2_", "AAAB AB B RRR", "lsm_spatial_derivatives3d.h:673"),
+    ("lsm3dcentralgradorder4_", "AAAB AB B RRR", "lsm_spatial_derivatives3d.h:723"),
+    ("lsm3dlaplacianorder2_", "AB AB B RRR", "lsm_spatial_derivatives3d.h:777"),
+    ("lsm3dgradientmagnitude_", "AAAAB B", "lsm_spatial_derivatives3d.h:939"),
     # level set equation RHS -- lsm_level_set_evolution3d.h
     ("lsm3dzerooutlevelseteqnrhs_", "AB", "lsm_level_set_evolution3d.h:64"),
     ("lsm3daddadvectiontermtolserhs_", "AB AAAB AAAB B", "lsm_level_set_evolution3d.h:94"),
@@ -32,6 +36,7 @@ _TABLE = [
     ("lsm3daddconstnormalveltermtolserhs_", "AB AAAB AAAB R B", "lsm_level_set_evolution3d.h:217"),
     ("lsm3daddexternalandnormalveltermtolserhs_", "AB AAAB AAAB AAAAB B",
      "lsm_level_set_evolution3d.h:398"),
+    ("lsm3daddconstcurvtermtolserhs_", "AB AAAB AAAAAAB R B", "lsm_level_set_evolution3d.h:277"),
     # TVD Runge-Kutta -- lsm_tvd_runge_kutta3d.h
     ("lsm3drk1step_", "AB AB AB B R", "lsm_tvd_runge_kutta3d.h:59"),
     ("lsm3dtvdrk2stage1_", "AB AB AB B R", "lsm_tvd_runge_kutta3d.h:108"),
@@ -82,12 +87,17 @@ MACRO_OF = {
     "lsm3dupwindhjeno2_": "LSM3D_UPWIND_HJ_ENO2",
     "lsm3dupwindhjeno3_": "LSM3D_UPWIND_HJ_ENO3",
     "lsm3dupwindhjweno5_": "LSM3D_UPWIND_HJ_WENO5",
+    "lsm3dcentralgradorder2_": "LSM3D_CENTRAL_GRAD_ORDER2",
+    "lsm3dcentralgradorder4_": "LSM3D_CENTRAL_GRAD_ORDER4",
+    "lsm3dlaplacianorder2_": "LSM3D_LAPLACIAN_ORDER2",
+    "lsm3dgradientmagnitude_": "LSM3D_GRADIENT_MAGNITUDE",
     "lsm3dzerooutlevelseteqnrhs_": "LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS",
     "lsm3daddadvectiontermtolserhs_": "LSM3D_ADD_ADVECTION_TERM_TO_LSE_RHS",
     "lsm3daddnormalveltermtolserhs_": "LSM3D_ADD_NORMAL_VEL_TERM_TO_LSE_RHS",
     "lsm3daddconstnormalveltermtolserhs_": "LSM3D_ADD_CONST_NORMAL_VEL_TERM_TO_LSE_RHS",
     "lsm3daddexternalandnormalveltermtolserhs_":
         "LSM3D_ADD_EXTERNAL_AND_NORMAL_VEL_TERM_TO_LSE_RHS",
+    "lsm3daddconstcurvtermtolserhs_": "LSM3D_ADD_CONST_CURV_TERM_TO_LSE_RHS",
     "lsm3drk1step_": "LSM3D_RK1_STEP",
     "lsm3dtvdrk2stage1_": "LSM3D_TVD_RK2_STAGE1",
     "lsm3dtvdrk2stage2_": "LSM3D_TVD_RK2_STAGE2",

@@ -18,6 +18,11 @@
                            1 <= max|k_i| <= 4, c ~ N(0, 0.1^2), on periodic
                            [-1,1]^3; phi0 = sphere r=0.3 at the origin
                            (gradient case)
+  pore_throat(dx)          the curvature example's default problem: a planar
+                           front x = x_lo + 20 dx in the pore throat between
+                           three unit spheres, box [-0.75,0.75]^2 x [0,1],
+                           mask = the spheres (positive inside), node
+                           coordinates of the serial package
 """
 import math
 
@@ -127,3 +132,37 @@ def adjoint_case(n=128, dtype=torch.float64, device="cpu", seed=4):
     phi0 = (torch.sqrt(X ** 2 + Y ** 2 + Z ** 2) - 0.3).to(dtype).contiguous()
     fv = FourierVelocity(grid, seed=seed, dtype=dtype, device=device)
     return grid, phi0, fv, (BC_PERIODIC,) * 6
+
+
+def pore_throat(dx=0.04, dtype=torch.float64, device="cpu", align=0):
+    """(grid, phi, mask) of the curvature example's default run: the grid of
+    createMaskThroatFromSpheres3d (curvature_model_top.c:247-296; ghost width 3
+    of the medium-accuracy grid, lsm_grid.c:46,50-114), the mask of
+    createIntersectionOfSpheres with inside_flag 1 (lsm_initialization3d.c:115-166),
+    the plane of createPlane through (x_lo + 20 dx, 0, 0) with normal (1, 0, 0)
+    (lsm_initialization3d.c:18-82) and IMPOSE_MASK of the plane
+    (curvature_model_top.c:58-79).  Grid points sit at x_lo_gb + dx * q for
+    ghost-box index q (no half-cell offset), as in those routines; values are
+    formed in double precision in the routines' operation order."""
+    r = 1.0
+    x_lo = (-3 * r / 4, -3 * r / 4, 0.0)
+    x_hi = (3 * r / 4, 3 * r / 4, r)
+    n = tuple(int(np.rint((x_hi[d] - x_lo[d]) / dx)) for d in range(3))
+    g = 3
+    grid = Grid(n, x_lo, tuple(x_lo[d] + n[d] * dx for d in range(3)), g, align, spacing=(dx, dx, dx))
+    gb = grid.gb
+    ax = []
+    for d in range(3):
+        q = np.arange(gb[2 * d], gb[2 * d + 1] + 1, dtype=np.float64) + g  # ghost-box index
+        ax.append((x_lo[d] - g * dx) + dx * q)
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    centers = ((0.0, -r, 0.0), (0.0, r, 0.0), (0.0, 0.0, r * math.sqrt(3.0)))
+    mask = np.full(X.shape, -np.finfo(np.float32).max, dtype=np.float64)
+    for cx, cy, cz in centers:
+        sd = -(np.sqrt((X - cx) * (X - cx) + (Y - cy) * (Y - cy) + (Z - cz) * (Z - cz)) - r)
+        mask = np.where(sd > mask, sd, mask)
+    px = x_lo[0] + 20 * dx
+    phi = ((X - px) * 1.0 + (Y - 0.0) * 0.0 + (Z - 0.0) * 0.0) / math.sqrt(1.0)
+    phi = np.where(mask > phi, mask, phi)
+    to = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dtype=dtype, device=device)  # noqa: E731
+    return grid, to(phi), to(mask)

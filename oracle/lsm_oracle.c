@@ -554,6 +554,35 @@ void oracle_lsm3daddconstnormalveltermtolserhs_(real *rhs, BOXARGS(r), const rea
         }
 }
 
+/* lsm3dAddConstCurvTermToLSERHS  lsm_level_set_evolution3d.f.in:356-460:
+ * rhs += b * kappa |grad phi| with kappa |grad phi| formed from the central
+ * first and second derivatives as  N / |grad phi|^2,  N the sum at :439-447
+ * (left to right, products left-associated; 0 where |grad phi|^2 < zero_tol). */
+void oracle_lsm3daddconstcurvtermtolserhs_(real *rhs, BOXARGS(r), const real *px,
+                                           const real *py, const real *pz, BOXARGS(g1),
+                                           const real *pxx, const real *pxy, const real *pxz,
+                                           const real *pyy, const real *pyz, const real *pzz,
+                                           BOXARGS(g2), const real *b, BOXARGS(fb)) {
+  box_t br = MKBOX(r), b1 = MKBOX(g1), b2 = MKBOX(g2);
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        real x = A(px, b1, i, j, k), y = A(py, b1, i, j, k), z = A(pz, b1, i, j, k);
+        real xx = A(pxx, b2, i, j, k), xy = A(pxy, b2, i, j, k), xz = A(pxz, b2, i, j, k);
+        real yy = A(pyy, b2, i, j, k), yz = A(pyz, b2, i, j, k), zz = A(pzz, b2, i, j, k);
+        real g2 = x * x + y * y + z * z;                     /* :433-435 */
+        real curv;
+        if (g2 < ZERO_TOL) {
+          curv = (real)0.0;
+        } else {
+          curv = xx * y * y + yy * x * x - (real)2 * xy * x * y + xx * z * z + zz * x * x -
+                 (real)2 * xz * x * z + yy * z * z + zz * y * y - (real)2 * yz * y * z;
+          curv = curv / g2;
+        }
+        A(rhs, br, i, j, k) = A(rhs, br, i, j, k) + *b * curv;   /* :451 */
+      }
+}
+
 /* Godunov selection for the combined external + normal velocity
  * Hamiltonian, lsm_level_set_evolution3d.f.in:630-650. */
 static real ext_normal_select(real vcomp, real vn, real dp, real dm) {
@@ -1049,6 +1078,81 @@ void oracle_lsmheaviside_(real *H, const real *x, const real *eps) {
     real ioe = 1.0 / e;
     *H = 0.5 * (1.0 + xv * ioe + ONE_OVER_PI_REF * SIN(PI_REF * xv * ioe));
   }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Central differences  lsm_spatial_derivatives3d.f.in:3016-3264, 3586-3650 */
+/* ---------------------------------------------------------------------- */
+void oracle_lsm3dcentralgradorder2_(real *gx, real *gy, real *gz, BOXARGS(g), const real *phi,
+                                    BOXARGS(phi), BOXARGS(fb), const real *dx, const real *dy,
+                                    const real *dz) {
+  /* :3016-3088; the factors are 0.5d0/dx rounded to real (:3058-3060) */
+  box_t bg = MKBOX(g), bp = MKBOX(phi);
+  real fx = 0.5 / *dx, fy = 0.5 / *dy, fz = 0.5 / *dz;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        A(gx, bg, i, j, k) = (A(phi, bp, i + 1, j, k) - A(phi, bp, i - 1, j, k)) * fx;
+        A(gy, bg, i, j, k) = (A(phi, bp, i, j + 1, k) - A(phi, bp, i, j - 1, k)) * fy;
+        A(gz, bg, i, j, k) = (A(phi, bp, i, j, k + 1) - A(phi, bp, i, j, k - 1)) * fz;
+      }
+}
+
+void oracle_lsm3dcentralgradorder4_(real *gx, real *gy, real *gz, BOXARGS(g), const real *phi,
+                                    BOXARGS(phi), BOXARGS(fb), const real *dx, const real *dy,
+                                    const real *dz) {
+  /* :3094-3174; eight is a real parameter, the factor 1/12 a d0 literal */
+  box_t bg = MKBOX(g), bp = MKBOX(phi);
+  const real eight = (real)8.0;
+  real fx = 0.0833333333333333333333 / *dx, fy = 0.0833333333333333333333 / *dy,
+       fz = 0.0833333333333333333333 / *dz;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        A(gx, bg, i, j, k) = (-A(phi, bp, i + 2, j, k) + eight * A(phi, bp, i + 1, j, k) +
+                              A(phi, bp, i - 2, j, k) - eight * A(phi, bp, i - 1, j, k)) * fx;
+        A(gy, bg, i, j, k) = (-A(phi, bp, i, j + 2, k) + eight * A(phi, bp, i, j + 1, k) +
+                              A(phi, bp, i, j - 2, k) - eight * A(phi, bp, i, j - 1, k)) * fy;
+        A(gz, bg, i, j, k) = (-A(phi, bp, i, j, k + 2) + eight * A(phi, bp, i, j, k + 1) +
+                              A(phi, bp, i, j, k - 2) - eight * A(phi, bp, i, j, k - 1)) * fz;
+      }
+}
+
+void oracle_lsm3dlaplacianorder2_(real *lap, BOXARGS(l), const real *phi, BOXARGS(phi),
+                                  BOXARGS(fb), const real *dx, const real *dy, const real *dz) {
+  /* :3180-3264.  1.0d0/dx/dx is rounded to real; 2.0d0*phi promotes each
+   * bracket, the products and the sum to double before the store. */
+  box_t bl = MKBOX(l), bp = MKBOX(phi);
+  real ix2 = 1.0 / *dx / *dx, iy2 = 1.0 / *dy / *dy, iz2 = 1.0 / *dz / *dz;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        double c = 2.0 * A(phi, bp, i, j, k);
+        A(lap, bl, i, j, k) =
+            (real)(ix2 * (A(phi, bp, i + 1, j, k) - c + A(phi, bp, i - 1, j, k)) +
+                   iy2 * (A(phi, bp, i, j + 1, k) - c + A(phi, bp, i, j - 1, k)) +
+                   iz2 * (A(phi, bp, i, j, k + 1) - c + A(phi, bp, i, j, k - 1)));
+      }
+}
+
+void oracle_lsm3dgradientmagnitude_(const real *gx, const real *gy, const real *gz, real *mag,
+                                    BOXARGS(g), BOXARGS(fb)) {
+  /* :3586-3650: squares below tol = 1e-8 (a real parameter) count as 0,
+   * and a magnitude below tol is stored as 0 */
+  box_t bg = MKBOX(g);
+  const real tol = (real)1.e-8;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        real x2 = A(gx, bg, i, j, k) * A(gx, bg, i, j, k);
+        real y2 = A(gy, bg, i, j, k) * A(gy, bg, i, j, k);
+        real z2 = A(gz, bg, i, j, k) * A(gz, bg, i, j, k);
+        if (x2 < tol) x2 = (real)0.0;
+        if (y2 < tol) y2 = (real)0.0;
+        if (z2 < tol) z2 = (real)0.0;
+        real t = (real)sqrt(x2 + y2 + z2);
+        A(mag, bg, i, j, k) = (t < tol) ? (real)0.0 : t;
+      }
 }
 
 /* ---------------------------------------------------------------------- */

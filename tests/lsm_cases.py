@@ -11,6 +11,7 @@ noisy enough to take both stencil choices.
 `scratch` lists the argument positions of the D1/D2/D3 scratch arrays, which
 the reference overwrites and the GPU library leaves untouched.
 """
+import zlib
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -71,6 +72,7 @@ SPATIAL = {"lsm3dhjeno1_", "lsm3dhjeno2_", "lsm3dhjeno3_", "lsm3dhjweno5_",
            "lsm3dupwindhjeno1_", "lsm3dupwindhjeno2_", "lsm3dupwindhjeno3_",
            "lsm3dupwindhjweno5_"}
 BC = {n for n in SIGNATURES if "extrapolation" in n or "neumann" in n}
+CENTRAL = {"lsm3dcentralgradorder2_", "lsm3dcentralgradorder4_", "lsm3dlaplacianorder2_"}
 
 
 def _field(rng, box, kind, real, eps=None):
@@ -97,7 +99,7 @@ def _field(rng, box, kind, real, eps=None):
 
 
 def make_case(name, real="double", seed=0, fb=(-3, 8, 2, 12, 5, 14)):
-    rng = np.random.default_rng(seed + 1000 * sorted(SIGNATURES).index(name))
+    rng = np.random.default_rng(seed + 1000 * (zlib.crc32(name.encode()) % 100003))
     kinds = SIGNATURES[name]
     dx, dy, dz = 0.1, 0.13, 0.07
     eps = 1.5 * dx
@@ -114,6 +116,9 @@ def make_case(name, real="double", seed=0, fb=(-3, 8, 2, 12, 5, 14)):
                 box = grow(fb, 2)          # the reference loops over the rhs ghost box
             elif name in BC:
                 box = grow(fb, (3, 2, 3), (2, 3, 1))
+            elif name in CENTRAL:
+                # outputs: small uneven ghosts; phi: >= 2 ghosts (order 4 reads +-2)
+                box = grow(fb, (1, 0, 2), (0, 2, 1)) if gi == 0 else grow(fb, (2, 3, 2), (3, 2, 2))
             elif name in SPATIAL:
                 # output grads: small ghost; phi: 3-4 ghosts; vel: 1; scratch: 4
                 if gi == 0 or (name.startswith("lsm3dhj") and gi == 1):
@@ -147,6 +152,10 @@ def make_case(name, real="double", seed=0, fb=(-3, 8, 2, 12, 5, 14)):
     case = Case(name, real, args, scratch)
     if name == "lsm3dcomputereinitializationeqnrhssubcellfixorder1_":
         _fix_distance0(case, real)
+    if name == "lsm3dgradientmagnitude_":
+        _small_gradients(case, rng, 1e-5)
+    if name == "lsm3daddconstcurvtermtolserhs_":
+        _small_gradients(case, rng, 0.0)
     return case
 
 
@@ -191,6 +200,8 @@ def _value_kind(name, gi, a_idx):
 
 def _scalars(name, dx, dy, dz, eps):
     """Scalar (R) arguments of each routine, in prototype order."""
+    if name == "lsm3daddconstcurvtermtolserhs_":
+        return [-0.35]
     if name == "lsm3daddconstnormalveltermtolserhs_":
         return [-0.8]
     if "rk" in name:
@@ -212,6 +223,17 @@ def _int_arg(name, rng):
     if "reinitialization" in name:
         return 0
     return 0
+
+
+def _small_gradients(case, rng, v):
+    """Sets the three gradient components (the first three arrays after the
+    output, or the first three) to v or 0 on a random tenth of the cells so
+    the routines' small-gradient branches run."""
+    arrs = [a for a in case.args if isinstance(a, np.ndarray)]
+    g = arrs[:3] if case.name == "lsm3dgradientmagnitude_" else arrs[1:4]
+    sel = rng.random(g[0].shape) < 0.1
+    for d, a in enumerate(g):
+        a[sel] = v if d != 1 else 0.0
 
 
 def _fix_distance0(case, real):
