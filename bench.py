@@ -58,7 +58,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (adjoint) measurement")
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the gradient")
     ap.add_argument("--only", choices=["batched", "reinit", "adjoint", "config1", "config5", "throughput",
-                                       "curvature"],
+                                       "curvature", "curvature_step"],
                     default=None,
                     help="run one secondary measurement alone and print its JSON (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -395,7 +395,7 @@ def run_throughput(n=1024, steps=5, warmup=1):
 CURV_PUBLISHED_S = {0.04: 105.0, 0.02: 1276.0}   # curvature_example/README:141-144 (regular method)
 
 
-def run_curvature(dxs=(0.04, 0.02), big_dx=1.5 / 512, big_steps=10, cpu=True):
+def run_curvature(dxs=(0.04, 0.02), big_dx=1.5 / 512, big_steps=10, cpu=True, runs=True):
     """The curvature example (phi_t + a|grad phi| = b kappa|grad phi|, a = 0.4,
     b = 0.05, mask = three unit spheres; curvature_model3d.c:41-626) on its
     default problem.  (1) Whole runs to the example's own stopping rule at
@@ -415,6 +415,8 @@ def run_curvature(dxs=(0.04, 0.02), big_dx=1.5 / 512, big_steps=10, cpu=True):
     out = {"model": "a=0.4, b=0.05, cfl 0.5, ENO2 / TVD-RK2, mask, reinit every 0.1 (11 iterations)",
            "runs": {}}
     finals = {}
+    if not runs:
+        dxs, cpu = (), False
     for real, dt_ in (("double", torch.float64), ("float", torch.float32)):
         for dx in dxs:
             g, phi, mask = pore_throat(dx, dt_, dev)
@@ -608,6 +610,9 @@ def main():
         return
     if a.only == "curvature":
         print(json.dumps({"curvature_example": run_curvature(cpu=not a.no_cpu_baseline)}), flush=True)
+        return
+    if a.only == "curvature_step":  # the 512 x 512 x 341 model step alone (profiling runs)
+        print(json.dumps({"curvature_example": run_curvature(runs=False, big_steps=a.steps)}), flush=True)
         return
     r32, grid32, phi32 = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts, zero_tol=SWITCH_TOL)
     print(f"# fp32: {r32['value']:.4e} cell-updates/s, {r32['ms_per_step']:.3f} ms/step, "

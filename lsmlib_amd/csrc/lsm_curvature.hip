@@ -73,6 +73,8 @@ __global__ void __launch_bounds__(TX* TY) k_medium_stage(MediumArgs<T> a) {
   // This thread's cells of a plane's (W x H) window: LDS position and the
   // element offset inside a plane of phi_stage (the same for every plane).
   // Lanes past the fill box of a partial tile load clamped (unused) cells.
+  constexpr int SLOT = H * (W + 1);  // elements per ring slot
+  T* lds = &ring[0][0][0];
   int lpos[PER], goff[PER];
 #pragma unroll
   for (int r = 0; r < PER; r++) {
@@ -82,41 +84,53 @@ __global__ void __launch_bounds__(TX* TY) k_medium_stage(MediumArgs<T> a) {
     const int gi = min(i0 - 2 + qx, fb.ihi + 2), gj = min(j0 - 2 + qy, fb.jhi + 2);
     goff[r] = (gi - a.ps.ilo) + (int)a.ps.sy * (gj - a.ps.jlo);
   }
+  // In-plane offsets of the cell in the other fields (plane bases advance with k).
+  const int cpos = (ty + 2) * (W + 1) + (tx + 2);
+  const int o_out = (i - a.out.ilo) + (int)a.out.sy * (j - a.out.jlo);
+  const int o_raw = (i - a.raw.ilo) + (int)a.raw.sy * (j - a.raw.jlo);
+  const int o_mask = (i - a.mask.ilo) + (int)a.mask.sy * (j - a.mask.jlo);
+  const int o_pc = (i - a.pc.ilo) + (int)a.pc.sy * (j - a.pc.jlo);
+  const int o_phi0 = (i - a.phi0.ilo) + (int)a.phi0.sy * (j - a.phi0.jlo);
   T nxt[PER];
   auto fetch = [&](int kk) {  // plane kk into registers
     const T* base = a.ps.p + a.ps.sz * (int64_t)(kk - a.ps.klo);
 #pragma unroll
     for (int r = 0; r < PER; r++) nxt[r] = base[goff[r]];
   };
-  auto commit = [&](int kk) {  // registers into ring slot (kk - k0 + 2) % 6
-    T* dst = &ring[(kk - k0 + 2) % 6][0][0];
+  auto commit = [&](int slot) {  // registers into ring slot `slot`
+    T* dst = lds + slot * SLOT;
 #pragma unroll
     for (int r = 0; r < PER; r++)
       if (r < PER - 1 || t + r * NT < W * H) dst[lpos[r]] = nxt[r];
   };
+  auto wrap6 = [](int v) { return v >= 6 ? v - 6 : (v < 0 ? v + 6 : v); };
 
   double acc = -1.0e308;  // MED_DT: running max of the CFL speed
-  for (int kk = k0 - 2; kk <= k0 + 1; kk++) {
+  for (int kk = k0 - 2; kk <= k0 + 1; kk++) {  // plane kk -> slot kk - k0 + 2
     fetch(kk);
-    commit(kk);
+    commit(kk - k0 + 2);
   }
   fetch(k0 + 2);
-  for (int k = k0; k <= k1; k++) {
+  int sc = 2;  // ring slot of plane k (uniform)
+  for (int k = k0; k <= k1; k++, sc = wrap6(sc + 1)) {
+    sc = __builtin_amdgcn_readfirstlane(sc);
     // Plane k+2 enters the ring (slot k-4 mod 6, read last at step k-2) and
     // plane k+3 is fetched; its loads stay in flight over this step's
     // arithmetic.  The one barrier orders the ring writes before the reads.
-    commit(k + 2);
+    commit(wrap6(sc + 2));
     if (k + 3 <= k1 + 2) fetch(k + 3);
     T m = T(0), uc = T(0);
     if (active && KIND != MED_DT) {
-      if (a.mask.p) m = __builtin_nontemporal_load(&a.mask.at(i, j, k));
-      if (a.form == RK2_STAGE2) uc = __builtin_nontemporal_load(&a.pc.at(i, j, k));
+      if (a.mask.p) m = __builtin_nontemporal_load(a.mask.p + a.mask.sz * (int64_t)(k - a.mask.klo) + o_mask);
+      if (a.form == RK2_STAGE2) uc = __builtin_nontemporal_load(a.pc.p + a.pc.sz * (int64_t)(k - a.pc.klo) + o_pc);
     }
     __syncthreads();
     if (active) {
-      const int s = k - k0 + 2;
+      // slot bases of planes k-2 .. k+2
+      const int sb[5] = {wrap6(sc - 2) * SLOT + cpos, wrap6(sc - 1) * SLOT + cpos, sc * SLOT + cpos,
+                         wrap6(sc + 1) * SLOT + cpos, wrap6(sc + 2) * SLOT + cpos};
       // P(dz, dy, dx): u_s at (i+dx, j+dy, k+dz)
-      auto P = [&](int dz, int dy, int dx) -> T { return ring[(s + dz) % 6][ty + 2 + dy][tx + 2 + dx]; };
+      auto P = [&](int dz, int dy, int dx) -> T { return lds[sb[dz + 2] + dy * (W + 1) + dx]; };
       T p[7];
 #pragma unroll
       for (int q = -2; q <= 2; q++) p[3 + q] = P(0, 0, q);
@@ -156,7 +170,7 @@ __global__ void __launch_bounds__(TX* TY) k_medium_stage(MediumArgs<T> a) {
             r = r + a.b * curvature_term<T>(gx, gy, gz, xx, xy, xz, yy, yz, zz, a.zero_tol);
           }
         } else {  // lsm_reinitialization3d.f.in:160-200
-          const T cur = a.use_phi0 ? a.phi0.at(i, j, k) : us;
+          const T cur = a.use_phi0 ? a.phi0.p[a.phi0.sz * (int64_t)(k - a.phi0.klo) + o_phi0] : us;
           if (fabs(cur) >= a.zero_tol) {
             const T n2 = godunov_star_sq<T>(cur, xp, yp, zp, xm, ym, zm);
             const T sgn = a.use_phi0 ? cur / sqrt(cur * cur + a.dx_sq) : cur / sqrt(cur * cur + n2 * a.dx_sq);
@@ -169,9 +183,9 @@ __global__ void __launch_bounds__(TX* TY) k_medium_stage(MediumArgs<T> a) {
           o = (m > y) ? m : y;
           const bool shell = (i - fb.ilo < 2) || (fb.ihi - i < 2) || (j - fb.jlo < 2) || (fb.jhi - j < 2) ||
                              (k - fb.klo < 2) || (fb.khi - k < 2);
-          if (shell) a.raw.at(i, j, k) = y;
+          if (shell) a.raw.p[a.raw.sz * (int64_t)(k - a.raw.klo) + o_raw] = y;
         }
-        __builtin_nontemporal_store(o, &a.out.at(i, j, k));
+        __builtin_nontemporal_store(o, a.out.p + a.out.sz * (int64_t)(k - a.out.klo) + o_out);
       }
     }
   }
@@ -403,6 +417,12 @@ void medium_stage(const char* fn, int kind, int form, R* out, const int* out_gb,
     else require_inside(fn, "phi_raw", box6(raw_gb), fb, 0, 0, 0);
   }
   if (kind == MED_REINIT && use_phi0) require_inside(fn, "phi0", box6(phi0_gb), fb, 0, 0, 0);
+  // the kernel keeps in-plane offsets in 32 bits
+  const int* gbs[5] = {out_gb, raw ? raw_gb : nullptr, pc ? pc_gb : nullptr, mask ? mask_gb : nullptr,
+                       (kind == MED_REINIT && use_phi0) ? phi0_gb : nullptr};
+  for (const int* g : gbs)
+    if (g && (long long)box6(g).nx() * box6(g).ny() > 0x7fffffffLL)
+      raise_error(std::string(fn) + ": array planes above 2^31 elements");
   if (!(device_only(fn, out, "phi_out") && device_only(fn, raw, "phi_raw") && device_only(fn, ps, "phi_stage") &&
         device_only(fn, pc, "phi_cur") && device_only(fn, mask, "mask") && device_only(fn, phi0, "phi0")))
     return;

@@ -35,14 +35,16 @@ class CurvatureFlow:
 
     mask: the masking level set (negative in the admissible region), a
     device tensor over grid.gb, or None (no mask).  a, b: normal speed and
-    curvature coefficient (a >= 0, b >= 0 as in the example).  zero_tol and
-    tiling as for pylsmlib.Stepper (None: the build's LSMLIB_ZERO_TOL; the
-    library's current tiling)."""
+    curvature coefficient (a >= 0, b >= 0 as in the example).  zero_tol as
+    for pylsmlib.Stepper (None: the build's LSMLIB_ZERO_TOL).  tiling: "auto"
+    applies the autotuned (tile shape, z-chunk) of the curvature stages for
+    the precision and grid size (lsmlib_amd.tuning), a (cfg, kz) pair fixes
+    it, None leaves the library's current setting."""
 
     DT_MIN_TO_CORRECT, DT_MIN = 1e-5, 0.001  # curvature_model3d.c:36-37
 
     def __init__(self, grid, real="double", mask=None, a=0.4, b=0.05, cfl=0.5, device="cuda", zero_tol=None,
-                 tiling=None):
+                 tiling="auto"):
         if a < 0 or b < 0:
             raise ValueError("the example's model takes a >= 0 and b >= 0")
         self.grid, self.real = grid, real
@@ -59,6 +61,9 @@ class CurvatureFlow:
         self.s1 = self.empty()
         self.raw = self.empty() if mask is not None else None
         self.zero_tol = zero_tol
+        if tiling == "auto":
+            from .tuning import choose_tiling
+            tiling = choose_tiling(real, max(grid.n), "curvature")
         self.tiling = tiling
         dx = grid.dx
         self.dt_corr = (1.0 / (dx[0] * dx[0]) + 1.0 / (dx[1] * dx[1]) + 1.0 / (dx[2] * dx[2])) * (2.0 * self.b)
@@ -108,6 +113,7 @@ class CurvatureFlow:
     def normal_dt(self, phi):
         """LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT of HJ-ENO2 derivatives of
         phi (one fused reduction; synchronizes)."""
+        self._configure()
         out = self.lib._ct(0)
         dx, dy, dz = self.grid.dx
         self.lib.fused("lsm3d_amd_const_normal_vel_dt_eno2", ctypes.byref(out), _ptr(phi), self._gb, self._fb,

@@ -11,6 +11,7 @@ from pathlib import Path
 TABLE = Path(__file__).resolve().parent / "tuning_table.json"
 DEFAULT = (0, 32)
 KZ_CHOICES = (8, 16, 32, 64, 128)
+CURVATURE_KZ_CHOICES = (2, 4, 8, 16, 32, 64)
 
 
 def _size_class(n):
@@ -34,8 +35,9 @@ def _key(real, n, kind):
 
 
 def choose_tiling(real, n, kind="advect"):
-    """(cfg, kz) for a stage kind ("advect", "reinit", or "patches" for the
-    batched patch level, where n is ignored).  A size class without an entry
+    """(cfg, kz) for a stage kind ("advect", "reinit", "curvature" (the
+    ENO2 / TVD-RK2 curvature-model stages), or "patches" for the batched
+    patch level, where n is ignored).  A size class without an entry
     takes the nearest tuned class below it, else the one above, else the
     default."""
     table = load_table()
@@ -59,6 +61,8 @@ def autotune(reals=("float", "double"), sizes=(128, 256, 512), kind="advect", re
     path = Path(path) if path else TABLE
     table = json.loads(path.read_text()) if path.exists() else {}
     dev = torch.device("cuda:0")
+    if kind == "curvature":
+        return _autotune_curvature(reals, sizes, reps, verbose, path, table)
     for real in reals:
         lib = toolbox(real)
         dtype = torch.float32 if real == "float" else torch.float64
@@ -96,6 +100,57 @@ def autotune(reals=("float", "double"), sizes=(128, 256, 512), kind="advect", re
                 print(f"autotune {kind} {real} n={n}: cfg={best[1]} kz={best[2]} {best[0]:.3f} ms/stage",
                       flush=True)
             del phi, vel, out
+            torch.cuda.empty_cache()
+    path.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
+    return table
+
+
+def _autotune_curvature(reals, sizes, reps, verbose, path, table):
+    """The curvature-model stage (lsmlib_amd.curvature) on the curvature
+    example's pore throat at dx = 1.5 / n (n x n x 2n/3 cells)."""
+    import torch
+    from . import toolbox
+    from .curvature import CurvatureFlow
+    from .pylsmlib import TVDRK2_STAGE2
+    from .workloads import pore_throat
+
+    dev = torch.device("cuda:0")
+    for real in reals:
+        lib = toolbox(real)
+        dtype = torch.float32 if real == "float" else torch.float64
+        ncfg = lib.lib.lsm3d_amd_stage_tiling_count()
+        for n in sizes:
+            grid, phi, mask = pore_throat(1.5 / n, dtype, dev)
+            m = CurvatureFlow(grid, real, mask, device=dev, tiling=None)
+            m.fill_ghosts(phi, masked=False)
+            m.curvature_stage(1, m.s1, phi, None, 1e-4)
+            m.fill_ghosts(m.s1)
+            out = m.empty()
+            best = None
+            for cfg in range(ncfg):
+                for kz in CURVATURE_KZ_CHOICES:
+                    lib.lib.lsm3d_amd_set_stage_tiling(cfg, kz)
+                    run = lambda: m.curvature_stage(TVDRK2_STAGE2, out, m.s1, phi, 1e-4)  # noqa: E731
+                    run()
+                    times = []
+                    for _ in range(reps):
+                        e0 = torch.cuda.Event(enable_timing=True)
+                        e1 = torch.cuda.Event(enable_timing=True)
+                        e0.record()
+                        run()
+                        e1.record()
+                        e1.synchronize()
+                        times.append(e0.elapsed_time(e1))
+                    ms = sorted(times)[len(times) // 2]
+                    if verbose:
+                        print(f"  curvature {real} n={max(grid.n)} cfg={cfg} kz={kz}: {ms:.4f} ms", flush=True)
+                    if best is None or ms < best[0]:
+                        best = (ms, cfg, kz)
+            table[f"curvature/{real}/{_size_class(max(grid.n))}"] = [best[1], best[2]]
+            if verbose:
+                print(f"autotune curvature {real} n={max(grid.n)}: cfg={best[1]} kz={best[2]} {best[0]:.4f} ms/stage",
+                      flush=True)
+            del phi, mask, out, m
             torch.cuda.empty_cache()
     path.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
     return table
@@ -151,12 +206,14 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser(description="time every stage tiling and record the fastest")
     ap.add_argument("--out", default=str(TABLE))
     ap.add_argument("--sizes", default="128,256,512")
-    ap.add_argument("--only", default=None, help="advect, reinit or patches")
+    ap.add_argument("--only", default=None, help="advect, reinit, curvature or patches")
     a = ap.parse_args()
     sizes = tuple(int(x) for x in a.sizes.split(","))
     if a.only in (None, "advect"):
         autotune(sizes=sizes, path=a.out)
     if a.only in (None, "reinit"):
         autotune(kind="reinit", sizes=(256,), path=a.out)
+    if a.only in (None, "curvature"):
+        autotune(kind="curvature", sizes=(38, 75, 256, 512), path=a.out)
     if a.only in (None, "patches"):
         autotune_patches(path=a.out)
