@@ -58,7 +58,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (adjoint) measurement")
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the gradient")
     ap.add_argument("--only", choices=["batched", "reinit", "adjoint", "config1", "config5", "throughput",
-                                       "curvature", "curvature_step"],
+                                       "curvature", "curvature_step", "field_extension"],
                     default=None,
                     help="run one secondary measurement alone and print its JSON (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -510,6 +510,48 @@ def run_curvature(dxs=(0.04, 0.02), big_dx=1.5 / 512, big_steps=10, cpu=True, ru
     return out
 
 
+def run_field_extension(n=512, iters=10):
+    """Field extension (FieldExtensionAlgorithm.cc:195-440) on n^3: phi =
+    distance to the union of 64 spheres, S = sin(2 pi x) y + z; the PHI_UPWIND
+    signed normal and the extension velocity once, then TVD-RK3 iterations
+    (fused WENO5 advection stages with that velocity).  Times the normal
+    computation and the iterations with HIP events, FP32 and FP64."""
+    import torch
+    from lsmlib_amd.field_extension import FieldExtension
+    from lsmlib_amd.pylsmlib import BC_LINEAR
+    from lsmlib_amd.workloads import line_elems, sphere_union_taylor_green
+
+    out = {}
+    for real, dt_ in (("float", torch.float32), ("double", torch.float64)):
+        grid, phi, vel, _ = sphere_union_taylor_green(n, dtype=dt_, device="cuda", align=line_elems(dt_))
+        del vel
+        X, Y, Z = grid.centers(dt_, "cuda")
+        S = (torch.sin(2 * torch.pi * X) * Y + Z).contiguous()
+        del X, Y, Z
+        fe = FieldExtension(grid, real, (BC_LINEAR,) * 6, device="cuda")
+        fe.stepper.fill_ghosts(phi)
+        fe.normals(phi)
+        fe.step(S)
+        torch.cuda.synchronize()
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        fe.normals(phi)
+        e[1].record()
+        for _ in range(iters):
+            fe.step(S)
+        e[2].record()
+        e[2].synchronize()
+        ms_it = e[1].elapsed_time(e[2]) / iters
+        cells = grid.interior_cells
+        out[real] = {"grid": [n, n, n], "normals_ms": e[0].elapsed_time(e[1]), "ms_per_iteration": ms_it,
+                     "cell_iterations_per_s": cells / (ms_it * 1e-3), "iterations": iters}
+        print(f"# field extension {real} {n}^3: normals {out[real]['normals_ms']:.2f} ms, "
+              f"{ms_it:.3f} ms/iteration", file=sys.stderr, flush=True)
+        del fe, phi, S
+        torch.cuda.empty_cache()
+    return out
+
+
 def fp32_error(grid32, phi32, grid64, phi64):
     """Max |phi32 - phi64| over the fill box and near the interface (|phi64| < 3 dx)."""
     return interior_error(grid32.interior(phi32), grid64.interior(phi64), grid64)
@@ -611,6 +653,9 @@ def main():
     if a.only == "curvature":
         print(json.dumps({"curvature_example": run_curvature(cpu=not a.no_cpu_baseline)}), flush=True)
         return
+    if a.only == "field_extension":
+        print(json.dumps({"field_extension_512": run_field_extension()}), flush=True)
+        return
     if a.only == "curvature_step":  # the 512 x 512 x 341 model step alone (profiling runs)
         print(json.dumps({"curvature_example": run_curvature(runs=False, big_steps=a.steps)}), flush=True)
         return
@@ -661,6 +706,8 @@ def main():
         print(f"# 1024^3: {extras['throughput_1024_fp32']['value']:.4e} cell-updates/s", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
         extras["curvature_example"] = run_curvature(cpu=not a.no_cpu_baseline)
+        torch.cuda.empty_cache()
+        extras["field_extension_512"] = run_field_extension()
         torch.cuda.empty_cache()
         if not a.no_cpu_baseline:
             extras["config1_fp64_128"] = run_config1()

@@ -132,6 +132,22 @@ void lsm3dgradientmagnitude_(const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
                              const LSMLIB_REAL *phi_z, LSMLIB_REAL *grad_phi_mag,
                              LSM_AMD_BOX(grad_phi_gb), LSM_AMD_BOX(fb));
 
+/* LSM3D_PHI_UPWIND_GRAD_F, lsm_spatial_derivatives3d.h:831 (phi read at fb +- 1;
+ * where phi == 0 the outputs are left unchanged, as in the reference) */
+void lsm3dphiupwindgradf_(LSMLIB_REAL *F_x, LSMLIB_REAL *F_y, LSMLIB_REAL *F_z, LSM_AMD_BOX(grad_F_gb),
+                          const LSMLIB_REAL *F_x_plus, const LSMLIB_REAL *F_y_plus,
+                          const LSMLIB_REAL *F_z_plus, LSM_AMD_BOX(grad_F_plus_gb),
+                          const LSMLIB_REAL *F_x_minus, const LSMLIB_REAL *F_y_minus,
+                          const LSMLIB_REAL *F_z_minus, LSM_AMD_BOX(grad_F_minus_gb),
+                          const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb), LSM_AMD_BOX(fb));
+/* LSM3D_AVERAGE_GRAD_PHI, lsm_spatial_derivatives3d.h:893 */
+void lsm3daveragegradphi_(LSMLIB_REAL *phi_x, LSMLIB_REAL *phi_y, LSMLIB_REAL *phi_z,
+                          LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *phi_x_plus,
+                          const LSMLIB_REAL *phi_y_plus, const LSMLIB_REAL *phi_z_plus,
+                          LSM_AMD_BOX(grad_phi_plus_gb), const LSMLIB_REAL *phi_x_minus,
+                          const LSMLIB_REAL *phi_y_minus, const LSMLIB_REAL *phi_z_minus,
+                          LSM_AMD_BOX(grad_phi_minus_gb), LSM_AMD_BOX(fb));
+
 /* --- level set equation RHS: src/toolbox/level_set_evolution/lsm_level_set_evolution3d.h --- */
 /* LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS, lsm_level_set_evolution3d.h:64 */
 void lsm3dzerooutlevelseteqnrhs_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb));
@@ -185,6 +201,31 @@ void lsm3daddexternalandnormalveltermtolserhs_(
     const LSMLIB_REAL *phi_z_minus, LSM_AMD_BOX(grad_phi_minus_gb), const LSMLIB_REAL *vel_n,
     const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y, const LSMLIB_REAL *vel_z,
     LSM_AMD_BOX(vel_gb), LSM_AMD_BOX(rhs_fb));
+
+/* --- normals: src/toolbox/geometry/lsm_geometry3d.h --- */
+/* LSM3D_COMPUTE_UNIT_NORMAL, lsm_geometry3d.h:74 */
+void lsm3dcomputeunitnormal_(LSMLIB_REAL *normal_x, LSMLIB_REAL *normal_y, LSMLIB_REAL *normal_z,
+                             LSM_AMD_BOX(normal_gb), const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
+                             const LSMLIB_REAL *phi_z, LSM_AMD_BOX(grad_phi_gb), LSM_AMD_BOX(fb));
+/* LSM3D_COMPUTE_SIGNED_UNIT_NORMAL, lsm_geometry3d.h:127 */
+void lsm3dcomputesignedunitnormal_(LSMLIB_REAL *normal_x, LSMLIB_REAL *normal_y, LSMLIB_REAL *normal_z,
+                                   LSM_AMD_BOX(normal_gb), const LSMLIB_REAL *phi_x,
+                                   const LSMLIB_REAL *phi_y, const LSMLIB_REAL *phi_z,
+                                   LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                                   LSM_AMD_BOX(fb), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                   const LSMLIB_REAL *dz);
+
+/* --- field extension: src/toolbox/field_extension/lsm_field_extension3d.h --- */
+/* LSM3D_COMPUTE_FIELD_EXTENSION_EQN_RHS, lsm_field_extension3d.h:65 (phi read at
+ * fb +- 1; S is accepted and not read, as in the reference) */
+void lsm3dcomputefieldextensioneqnrhs_(LSMLIB_REAL *rhs, LSM_AMD_BOX(rhs_gb), const LSMLIB_REAL *S,
+                                       LSM_AMD_BOX(S_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                                       const LSMLIB_REAL *S_x_upwind, const LSMLIB_REAL *S_y_upwind,
+                                       const LSMLIB_REAL *S_z_upwind, LSM_AMD_BOX(grad_S_upwind_gb),
+                                       const LSMLIB_REAL *signed_normal_x, const LSMLIB_REAL *signed_normal_y,
+                                       const LSMLIB_REAL *signed_normal_z, LSM_AMD_BOX(signed_normal_gb),
+                                       LSM_AMD_BOX(fb), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                       const LSMLIB_REAL *dz);
 
 /* --- TVD Runge-Kutta: src/toolbox/time_integration/lsm_tvd_runge_kutta3d.h --- */
 /* LSM3D_RK1_STEP, lsm_tvd_runge_kutta3d.h:59 */
@@ -476,6 +517,18 @@ void lsm3d_amd_patch_plan_capture_rk3(lsm3d_amd_patch_plan *plan, int nsteps, LS
                                       LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt,
                                       const int bc[6]);
 void lsm3d_amd_patch_plan_replay(lsm3d_amd_patch_plan *plan);
+
+/* --- field extension: S_t + sgn(phi) n . grad S = 0 ---
+ * V = signed normal where LSM3D_COMPUTE_FIELD_EXTENSION_EQN_RHS is active
+ * (|phi| > 3 max(dx), or phi of one sign on the six neighbours), else 0, over
+ * fb (phi read at fb +- 1).  The fused advection stages with this V advance
+ * the extension equation: -V.grad(S), upwind by V, is that routine's
+ * right-hand side cell by cell (FieldExtensionAlgorithm.cc:333-440). */
+void lsm3d_amd_field_extension_velocity(LSMLIB_REAL *vel_x, LSMLIB_REAL *vel_y, LSMLIB_REAL *vel_z,
+                                        const int vel_gb[6], const LSMLIB_REAL *normal_x,
+                                        const LSMLIB_REAL *normal_y, const LSMLIB_REAL *normal_z,
+                                        const int normal_gb[6], const LSMLIB_REAL *phi, const int phi_gb[6],
+                                        const int fb[6], LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz);
 
 /* --- curvature model: phi_t + vel_n |grad phi| = b kappa |grad phi|, phi <= mask ---
  * The "medium accuracy" path of LSMLIB's curvature example

@@ -91,13 +91,14 @@ class ToolboxLibrary(F77Library):
         L.lsm3d_amd_extrapolate_ghosts_masked.argtypes = [P, B, P, B, P, B, B, B]
         L.lsm3d_amd_const_normal_vel_dt_eno2.argtypes = [ctypes.POINTER(R), P, B, B, R, R, R, R, R]
         L.lsm3d_amd_impose_mask.argtypes = [P, P, P, ctypes.c_longlong]
+        L.lsm3d_amd_field_extension_velocity.argtypes = [P, P, P, B, P, P, P, B, P, B, B, R, R, R]
         for f in ("lsm3d_amd_advect_stage", "lsm3d_amd_advect_const_stage", "lsm3d_amd_reinit_stage",
                   "lsm3d_amd_fill_ghosts", "lsm3d_amd_set_stage_tiling", "lsm3d_amd_patch_plan_destroy",
                   "lsm3d_amd_patch_plan_stage", "lsm3d_amd_patch_plan_capture_rk3",
                   "lsm3d_amd_patch_plan_replay", "lsm3d_amd_patch_plan_fill_ghosts", "lsm3d_amd_advect_stage_adjoint",
                   "lsm3d_amd_heaviside_mismatch", "lsm3d_amd_curvature_stage", "lsm3d_amd_reinit_eno2_stage",
                   "lsm3d_amd_extrapolate_ghosts_masked", "lsm3d_amd_const_normal_vel_dt_eno2",
-                  "lsm3d_amd_impose_mask"):
+                  "lsm3d_amd_impose_mask", "lsm3d_amd_field_extension_velocity"):
             getattr(L, f).restype = None
 
     def fused(self, fname, *args):

@@ -132,6 +132,138 @@ __global__ void __launch_bounds__(256)
 }
 
 // ============================================================================
+// Normals and the field-extension right-hand side
+// ============================================================================
+// phi-upwind choice of one component, lsm_spatial_derivatives3d.f.in:3270-3480:
+// false (F left unchanged) when phi == 0.
+template <typename T>
+__device__ __forceinline__ bool phi_upwind_pick(T pc, T pm, T pp, T fp, T fm, T& out) {
+  if (pc > T(0)) {
+    if (pm <= pc) out = (pp < pm) ? fp : fm;
+    else if (pp <= pc) out = fp;
+    else out = T(0);
+    return true;
+  }
+  if (pc < T(0)) {
+    if (pm >= pc) out = (pp > pm) ? fp : fm;
+    else if (pp >= pc) out = fp;
+    else out = T(0);
+    return true;
+  }
+  return false;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_phi_upwind_grad(Field<T> fx, Field<T> fy, Field<T> fz, Field<const T> fxp, Field<const T> fyp,
+                      Field<const T> fzp, Field<const T> fxm, Field<const T> fym, Field<const T> fzm,
+                      Field<const T> phi, Box fb) {
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T pc = phi.at(i, j, k);
+  T v;
+  if (phi_upwind_pick<T>(pc, phi.at(i - 1, j, k), phi.at(i + 1, j, k), fxp.at(i, j, k), fxm.at(i, j, k), v))
+    fx.at(i, j, k) = v;
+  if (phi_upwind_pick<T>(pc, phi.at(i, j - 1, k), phi.at(i, j + 1, k), fyp.at(i, j, k), fym.at(i, j, k), v))
+    fy.at(i, j, k) = v;
+  if (phi_upwind_pick<T>(pc, phi.at(i, j, k - 1), phi.at(i, j, k + 1), fzp.at(i, j, k), fzm.at(i, j, k), v))
+    fz.at(i, j, k) = v;
+  LSM_CELL_LOOP_END
+}
+
+// lsm_spatial_derivatives3d.f.in:3486-3580
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_average_grad(Field<T> gx, Field<T> gy, Field<T> gz, Field<const T> xp, Field<const T> yp, Field<const T> zp,
+                   Field<const T> xm, Field<const T> ym, Field<const T> zm, Box fb) {
+#pragma clang fp contract(off)
+  LSM_CELL_LOOP_BEGIN(fb)
+  gx.at(i, j, k) = T(0.5) * (xp.at(i, j, k) + xm.at(i, j, k));
+  gy.at(i, j, k) = T(0.5) * (yp.at(i, j, k) + ym.at(i, j, k));
+  gz.at(i, j, k) = T(0.5) * (zp.at(i, j, k) + zm.at(i, j, k));
+  LSM_CELL_LOOP_END
+}
+
+// lsm_geometry3d_fort.f.in:27-110 (unsigned) and :129-240 (signed: sgn(phi)
+// smoothed with max(dx)^2, zero where |phi| <= zero_tol)
+template <typename T, bool SIGNED>
+__global__ void __launch_bounds__(256)
+    k_unit_normal(Field<T> nx, Field<T> ny, Field<T> nz, Field<const T> px, Field<const T> py, Field<const T> pz,
+                  Field<const T> phi, Box fb, T dx_sq) {
+#pragma clang fp contract(off)
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T x = px.at(i, j, k), y = py.at(i, j, k), z = pz.at(i, j, k);
+  T a = T(1), b = T(0), c = T(0);
+  if (SIGNED) {
+    const T pc = phi.at(i, j, k);
+    if (fabs(pc) > Tol<T>::zero) {
+      const T n2 = (x * x + y * y) + z * z;
+      if (n2 >= Tol<T>::zero) {
+        const T sgn = pc / sqrt(pc * pc + n2 * dx_sq);
+        const T inv = T(1) / sqrt(n2);
+        a = (sgn * x) * inv;
+        b = (sgn * y) * inv;
+        c = (sgn * z) * inv;
+      }
+    } else {
+      a = T(0);
+    }
+  } else {
+    const T norm = sqrt((x * x + y * y) + z * z);
+    if (norm >= Tol<T>::zero) {
+      const T inv = T(1) / norm;
+      a = x * inv;
+      b = y * inv;
+      c = z * inv;
+    }
+  }
+  nx.at(i, j, k) = a;
+  ny.at(i, j, k) = b;
+  nz.at(i, j, k) = c;
+  LSM_CELL_LOOP_END
+}
+
+// lsm_field_extension3d.f:31-158: -(n . grad S) unless |phi| <= cutoff and
+// phi changes sign towards one of the six neighbours (then 0)
+template <typename T>
+__device__ __forceinline__ bool field_ext_active(const Field<const T>& phi, int i, int j, int k, T cutoff) {
+#pragma clang fp contract(off)
+  const T pc = phi.at(i, j, k), z = T(0);
+  if (fabs(pc) > cutoff) return true;
+  return pc * phi.at(i - 1, j, k) > z && pc * phi.at(i + 1, j, k) > z && pc * phi.at(i, j - 1, k) > z &&
+         pc * phi.at(i, j + 1, k) > z && pc * phi.at(i, j, k - 1) > z && pc * phi.at(i, j, k + 1) > z;
+}
+
+// The field-extension velocity: the signed normal where the extension
+// equation is active, 0 where LSM3D_COMPUTE_FIELD_EXTENSION_EQN_RHS sets the
+// right-hand side to zero.  With it, -V.grad(S) (upwind by V) equals that
+// routine's right-hand side cell by cell, so the fused advection stages
+// advance the extension equation.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_field_ext_velocity(Field<T> vx, Field<T> vy, Field<T> vz, Field<const T> nx, Field<const T> ny,
+                         Field<const T> nz, Field<const T> phi, Box fb, T cutoff) {
+  LSM_CELL_LOOP_BEGIN(fb)
+  const bool on = field_ext_active<T>(phi, i, j, k, cutoff);
+  vx.at(i, j, k) = on ? nx.at(i, j, k) : T(0);
+  vy.at(i, j, k) = on ? ny.at(i, j, k) : T(0);
+  vz.at(i, j, k) = on ? nz.at(i, j, k) : T(0);
+  LSM_CELL_LOOP_END
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_field_ext_rhs(Field<T> rhs, Field<const T> phi, Field<const T> sx, Field<const T> sy, Field<const T> sz,
+                    Field<const T> nx, Field<const T> ny, Field<const T> nz, Box fb, T cutoff) {
+#pragma clang fp contract(off)
+  LSM_CELL_LOOP_BEGIN(fb)
+  T r = T(0);
+  if (field_ext_active<T>(phi, i, j, k, cutoff))
+    r = -((nx.at(i, j, k) * sx.at(i, j, k) + ny.at(i, j, k) * sy.at(i, j, k)) + nz.at(i, j, k) * sz.at(i, j, k));
+  rhs.at(i, j, k) = r;
+  LSM_CELL_LOOP_END
+}
+
+// ============================================================================
 // Level set equation RHS  (lsm_level_set_evolution3d.f.in)
 // ============================================================================
 template <typename T>
@@ -755,6 +887,111 @@ void lsm3dgradientmagnitude_(const R* gx, const R* gy, const R* gz, R* mag, BOXP
   auto fm = c.out(mag, BOX(g));
   auto fx = c.in(gx, BOX(g)), fy = c.in(gy, BOX(g)), fz = c.in(gz, BOX(g));
   launch_cells(c, "k_grad_mag", fb, k_grad_mag<R>, fm, fx, fy, fz, fb);
+  c.finish();
+}
+
+void lsm3dphiupwindgradf_(R* fx, R* fy, R* fz, BOXP(g), const R* fxp, const R* fyp, const R* fzp, BOXP(gp),
+                          const R* fxm, const R* fym, const R* fzm, BOXP(gm), const R* phi, BOXP(phi), BOXP(fb)) {
+  const char* fn = "LSM3D_PHI_UPWIND_GRAD_F";
+  Call c(fn);
+  Box fb = BOX(fb);
+  require_inside(fn, "grad_F", BOX(g), fb, 0, 0, 0);
+  require_inside(fn, "grad_F_plus", BOX(gp), fb, 0, 0, 0);
+  require_inside(fn, "grad_F_minus", BOX(gm), fb, 0, 0, 0);
+  require_inside(fn, "phi", BOX(phi), fb, 1, 1, 1);
+  auto ox = c.out(fx, BOX(g)), oy = c.out(fy, BOX(g)), oz = c.out(fz, BOX(g));
+  auto a = c.in(fxp, BOX(gp)), b = c.in(fyp, BOX(gp)), d = c.in(fzp, BOX(gp));
+  auto e = c.in(fxm, BOX(gm)), f = c.in(fym, BOX(gm)), g = c.in(fzm, BOX(gm));
+  auto p = c.in(phi, BOX(phi));
+  launch_cells(c, "k_phi_upwind_grad", fb, k_phi_upwind_grad<R>, ox, oy, oz, a, b, d, e, f, g, p, fb);
+  c.finish();
+}
+
+void lsm3daveragegradphi_(R* gx, R* gy, R* gz, BOXP(g), const R* xp, const R* yp, const R* zp, BOXP(gp),
+                          const R* xm, const R* ym, const R* zm, BOXP(gm), BOXP(fb)) {
+  const char* fn = "LSM3D_AVERAGE_GRAD_PHI";
+  Call c(fn);
+  Box fb = BOX(fb);
+  require_inside(fn, "grad_phi", BOX(g), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_plus", BOX(gp), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_minus", BOX(gm), fb, 0, 0, 0);
+  auto ox = c.out(gx, BOX(g)), oy = c.out(gy, BOX(g)), oz = c.out(gz, BOX(g));
+  auto a = c.in(xp, BOX(gp)), b = c.in(yp, BOX(gp)), d = c.in(zp, BOX(gp));
+  auto e = c.in(xm, BOX(gm)), f = c.in(ym, BOX(gm)), g = c.in(zm, BOX(gm));
+  launch_cells(c, "k_average_grad", fb, k_average_grad<R>, ox, oy, oz, a, b, d, e, f, g, fb);
+  c.finish();
+}
+
+// ---- normals (lsm_geometry3d.h) and field extension (lsm_field_extension3d.h) ---
+void lsm3dcomputeunitnormal_(R* nx, R* ny, R* nz, BOXP(n), const R* px, const R* py, const R* pz, BOXP(g),
+                             BOXP(fb)) {
+  const char* fn = "LSM3D_COMPUTE_UNIT_NORMAL";
+  Call c(fn);
+  Box fb = BOX(fb);
+  require_inside(fn, "normal", BOX(n), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi", BOX(g), fb, 0, 0, 0);
+  auto ox = c.out(nx, BOX(n)), oy = c.out(ny, BOX(n)), oz = c.out(nz, BOX(n));
+  auto a = c.in(px, BOX(g)), b = c.in(py, BOX(g)), d = c.in(pz, BOX(g));
+  launch_cells(c, "k_unit_normal", fb, k_unit_normal<R, false>, ox, oy, oz, a, b, d, no_field(), fb, R(0));
+  c.finish();
+}
+
+void lsm3dcomputesignedunitnormal_(R* nx, R* ny, R* nz, BOXP(n), const R* px, const R* py, const R* pz, BOXP(g),
+                                   const R* phi, BOXP(phi), BOXP(fb), const R* dx, const R* dy, const R* dz) {
+  const char* fn = "LSM3D_COMPUTE_SIGNED_UNIT_NORMAL";
+  Call c(fn);
+  Box fb = BOX(fb);
+  require_inside(fn, "normal", BOX(n), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi", BOX(g), fb, 0, 0, 0);
+  require_inside(fn, "phi", BOX(phi), fb, 0, 0, 0);
+  auto ox = c.out(nx, BOX(n)), oy = c.out(ny, BOX(n)), oz = c.out(nz, BOX(n));
+  auto a = c.in(px, BOX(g)), b = c.in(py, BOX(g)), d = c.in(pz, BOX(g));
+  auto p = c.in(phi, BOX(phi));
+  const R m = max_of3(*dx, *dy, *dz);
+  launch_cells(c, "k_unit_normal", fb, k_unit_normal<R, true>, ox, oy, oz, a, b, d, p, fb, m * m);
+  c.finish();
+}
+
+void lsm3dcomputefieldextensioneqnrhs_(R* rhs, BOXP(r), const R* S, BOXP(S), const R* phi, BOXP(phi),
+                                       const R* sx, const R* sy, const R* sz, BOXP(gs), const R* nx, const R* ny,
+                                       const R* nz, BOXP(n), BOXP(fb), const R* dx, const R* dy, const R* dz) {
+  const char* fn = "LSM3D_COMPUTE_FIELD_EXTENSION_EQN_RHS";
+  (void)S;
+  Call c(fn);
+  Box fb = BOX(fb);
+  require_inside(fn, "rhs", BOX(r), fb, 0, 0, 0);
+  require_inside(fn, "phi", BOX(phi), fb, 1, 1, 1);
+  require_inside(fn, "grad_S_upwind", BOX(gs), fb, 0, 0, 0);
+  require_inside(fn, "signed_normal", BOX(n), fb, 0, 0, 0);
+  auto fr = c.out(rhs, BOX(r));
+  auto p = c.in(phi, BOX(phi));
+  auto a = c.in(sx, BOX(gs)), b = c.in(sy, BOX(gs)), d = c.in(sz, BOX(gs));
+  auto e = c.in(nx, BOX(n)), f = c.in(ny, BOX(n)), g = c.in(nz, BOX(n));
+  // zero_level_set_cutoff = 3.0d0*max(dx,dy,dz) rounded to LSMLIB_REAL (:115)
+  const R cutoff = R(3.0 * (double)max_of3(*dx, *dy, *dz));
+  launch_cells(c, "k_field_ext_rhs", fb, k_field_ext_rhs<R>, fr, p, a, b, d, e, f, g, fb, cutoff);
+  c.finish();
+}
+
+// Part 3: the velocity that turns the fused advection stage into the
+// field-extension stage (see k_field_ext_velocity).
+void lsm3d_amd_field_extension_velocity(R* vx, R* vy, R* vz, const int vel_gb[6], const R* nx, const R* ny,
+                                        const R* nz, const int normal_gb[6], const R* phi, const int phi_gb[6],
+                                        const int fbp[6], R dx, R dy, R dz) {
+  const char* fn = "lsm3d_amd_field_extension_velocity";
+  Call c(fn);
+  const Box fb{fbp[0], fbp[1], fbp[2], fbp[3], fbp[4], fbp[5]};
+  const Box gv{vel_gb[0], vel_gb[1], vel_gb[2], vel_gb[3], vel_gb[4], vel_gb[5]};
+  const Box gn{normal_gb[0], normal_gb[1], normal_gb[2], normal_gb[3], normal_gb[4], normal_gb[5]};
+  const Box gp{phi_gb[0], phi_gb[1], phi_gb[2], phi_gb[3], phi_gb[4], phi_gb[5]};
+  require_inside(fn, "vel", gv, fb, 0, 0, 0);
+  require_inside(fn, "signed_normal", gn, fb, 0, 0, 0);
+  require_inside(fn, "phi", gp, fb, 1, 1, 1);
+  auto ox = c.out(vx, gv), oy = c.out(vy, gv), oz = c.out(vz, gv);
+  auto a = c.in(nx, gn), b = c.in(ny, gn), d = c.in(nz, gn);
+  auto p = c.in(phi, gp);
+  const R cutoff = R(3.0 * (double)max_of3(dx, dy, dz));
+  launch_cells(c, "k_field_ext_velocity", fb, k_field_ext_velocity<R>, ox, oy, oz, a, b, d, p, fb, cutoff);
   c.finish();
 }
 

@@ -29,6 +29,8 @@ _TABLE = [
     ("lsm3dcentralgradorder4_", "AAAB AB B RRR", "lsm_spatial_derivatives3d.h:723"),
     ("lsm3dlaplacianorder2_", "AB AB B RRR", "lsm_spatial_derivatives3d.h:777"),
     ("lsm3dgradientmagnitude_", "AAAAB B", "lsm_spatial_derivatives3d.h:939"),
+    ("lsm3dphiupwindgradf_", "AAAB AAAB AAAB AB B", "lsm_spatial_derivatives3d.h:831"),
+    ("lsm3daveragegradphi_", "AAAB AAAB AAAB B", "lsm_spatial_derivatives3d.h:893"),
     # level set equation RHS -- lsm_level_set_evolution3d.h
     ("lsm3dzerooutlevelseteqnrhs_", "AB", "lsm_level_set_evolution3d.h:64"),
     ("lsm3daddadvectiontermtolserhs_", "AB AAAB AAAB B", "lsm_level_set_evolution3d.h:94"),
@@ -52,6 +54,10 @@ _TABLE = [
     ("lsm3dcomputedistanceforsubcellfix_", "AAB B RRR", "lsm_reinitialization3d.h:247"),
     ("lsm3dcomputereinitializationeqnrhssubcellfixorder1_", "AB AB AAB AAAB AAAB B RRR",
      "lsm_reinitialization3d.h:295"),
+    # normals -- lsm_geometry3d.h; field extension -- lsm_field_extension3d.h
+    ("lsm3dcomputeunitnormal_", "AAAB AAAB B", "lsm_geometry3d.h:74"),
+    ("lsm3dcomputesignedunitnormal_", "AAAB AAAB AB B RRR", "lsm_geometry3d.h:127"),
+    ("lsm3dcomputefieldextensioneqnrhs_", "AB AB AB AAAB AAAB B RRR", "lsm_field_extension3d.h:65"),
     # reductions -- lsm_utilities3d.h, lsm_geometry3d.h
     ("lsm3dmaxnormdiff_", "O AB AB B", "lsm_utilities3d.h:87"),
     ("lsm3dmaxnormdiffcontrolvolume_", "O AB AB AB I B", "lsm_utilities3d.h:533"),
@@ -91,6 +97,11 @@ MACRO_OF = {
     "lsm3dcentralgradorder4_": "LSM3D_CENTRAL_GRAD_ORDER4",
     "lsm3dlaplacianorder2_": "LSM3D_LAPLACIAN_ORDER2",
     "lsm3dgradientmagnitude_": "LSM3D_GRADIENT_MAGNITUDE",
+    "lsm3dphiupwindgradf_": "LSM3D_PHI_UPWIND_GRAD_F",
+    "lsm3daveragegradphi_": "LSM3D_AVERAGE_GRAD_PHI",
+    "lsm3dcomputeunitnormal_": "LSM3D_COMPUTE_UNIT_NORMAL",
+    "lsm3dcomputesignedunitnormal_": "LSM3D_COMPUTE_SIGNED_UNIT_NORMAL",
+    "lsm3dcomputefieldextensioneqnrhs_": "LSM3D_COMPUTE_FIELD_EXTENSION_EQN_RHS",
     "lsm3dzerooutlevelseteqnrhs_": "LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS",
     "lsm3daddadvectiontermtolserhs_": "LSM3D_ADD_ADVECTION_TERM_TO_LSE_RHS",
     "lsm3daddnormalveltermtolserhs_": "LSM3D_ADD_NORMAL_VEL_TERM_TO_LSE_RHS",

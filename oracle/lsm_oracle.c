@@ -1156,6 +1156,155 @@ void oracle_lsm3dgradientmagnitude_(const real *gx, const real *gy, const real *
 }
 
 /* ---------------------------------------------------------------------- */
+/* Normals and the field-extension right-hand side                         */
+/* ---------------------------------------------------------------------- */
+/* phi-upwind selection of one component (lsm_spatial_derivatives3d.f.in:3270-3480):
+ * returns 1 and sets *out, or 0 when phi == 0 (the reference leaves F alone). */
+static int phi_upwind_pick(real pc, real pm, real pp, real fp, real fm, real *out) {
+  if (pc > 0) {
+    if (pm <= pc) *out = (pp < pm) ? fp : fm;
+    else if (pp <= pc) *out = fp;
+    else *out = (real)0.0;
+    return 1;
+  }
+  if (pc < 0) {
+    if (pm >= pc) *out = (pp > pm) ? fp : fm;
+    else if (pp >= pc) *out = fp;
+    else *out = (real)0.0;
+    return 1;
+  }
+  return 0;
+}
+
+void oracle_lsm3dphiupwindgradf_(real *fx, real *fy, real *fz, BOXARGS(g), const real *fxp,
+                                 const real *fyp, const real *fzp, BOXARGS(gp), const real *fxm,
+                                 const real *fym, const real *fzm, BOXARGS(gm), const real *phi,
+                                 BOXARGS(phi), BOXARGS(fb)) {
+  box_t bg = MKBOX(g), bp = MKBOX(gp), bm = MKBOX(gm), bphi = MKBOX(phi);
+  real *out[3] = {fx, fy, fz};
+  const real *plus[3] = {fxp, fyp, fzp}, *minus[3] = {fxm, fym, fzm};
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        real pc = A(phi, bphi, i, j, k);
+        for (int d = 0; d < 3; d++) {
+          int a = DI(d), b = DJ(d), c = DK(d);
+          real v;
+          if (phi_upwind_pick(pc, A(phi, bphi, i - a, j - b, k - c), A(phi, bphi, i + a, j + b, k + c),
+                              A(plus[d], bp, i, j, k), A(minus[d], bm, i, j, k), &v))
+            A(out[d], bg, i, j, k) = v;
+        }
+      }
+}
+
+void oracle_lsm3daveragegradphi_(real *gx, real *gy, real *gz, BOXARGS(g), const real *xp,
+                                 const real *yp, const real *zp, BOXARGS(gp), const real *xm,
+                                 const real *ym, const real *zm, BOXARGS(gm), BOXARGS(fb)) {
+  /* lsm_spatial_derivatives3d.f.in:3486-3580; half is a real parameter */
+  box_t bg = MKBOX(g), bp = MKBOX(gp), bm = MKBOX(gm);
+  const real half = (real)0.5;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        A(gx, bg, i, j, k) = half * (A(xp, bp, i, j, k) + A(xm, bm, i, j, k));
+        A(gy, bg, i, j, k) = half * (A(yp, bp, i, j, k) + A(ym, bm, i, j, k));
+        A(gz, bg, i, j, k) = half * (A(zp, bp, i, j, k) + A(zm, bm, i, j, k));
+      }
+}
+
+void oracle_lsm3dcomputeunitnormal_(real *nx, real *ny, real *nz, BOXARGS(n), const real *px,
+                                    const real *py, const real *pz, BOXARGS(g), BOXARGS(fb)) {
+  /* lsm_geometry3d_fort.f.in:27-110; 1.0d0/norm rounded to real */
+  box_t bn = MKBOX(n), bg = MKBOX(g);
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        real x = A(px, bg, i, j, k), y = A(py, bg, i, j, k), z = A(pz, bg, i, j, k);
+        real norm = (real)sqrt(x * x + y * y + z * z);
+        if (norm >= ZERO_TOL) {
+          real inv = 1.0 / norm;
+          A(nx, bn, i, j, k) = x * inv;
+          A(ny, bn, i, j, k) = y * inv;
+          A(nz, bn, i, j, k) = z * inv;
+        } else {
+          A(nx, bn, i, j, k) = (real)1.0;
+          A(ny, bn, i, j, k) = (real)0.0;
+          A(nz, bn, i, j, k) = (real)0.0;
+        }
+      }
+}
+
+void oracle_lsm3dcomputesignedunitnormal_(real *nx, real *ny, real *nz, BOXARGS(n), const real *px,
+                                          const real *py, const real *pz, BOXARGS(g), const real *phi,
+                                          BOXARGS(phi), BOXARGS(fb), const real *dx, const real *dy,
+                                          const real *dz) {
+  /* lsm_geometry3d_fort.f.in:129-240 */
+  box_t bn = MKBOX(n), bg = MKBOX(g), bphi = MKBOX(phi);
+  real m = *dx;
+  if (*dy > m) m = *dy;
+  if (*dz > m) m = *dz;
+  real dx_sq = m * m;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        real pc = A(phi, bphi, i, j, k);
+        if (fabs(pc) > ZERO_TOL) {
+          real x = A(px, bg, i, j, k), y = A(py, bg, i, j, k), z = A(pz, bg, i, j, k);
+          real n2 = x * x + y * y + z * z;
+          if (n2 >= ZERO_TOL) {
+            real sgn = pc / (real)sqrt(pc * pc + n2 * dx_sq);     /* :215-216 */
+            real inv = 1.0 / (real)sqrt(n2);                      /* :218 */
+            A(nx, bn, i, j, k) = sgn * x * inv;
+            A(ny, bn, i, j, k) = sgn * y * inv;
+            A(nz, bn, i, j, k) = sgn * z * inv;
+          } else {
+            A(nx, bn, i, j, k) = (real)1.0;
+            A(ny, bn, i, j, k) = (real)0.0;
+            A(nz, bn, i, j, k) = (real)0.0;
+          }
+        } else {
+          A(nx, bn, i, j, k) = (real)0.0;
+          A(ny, bn, i, j, k) = (real)0.0;
+          A(nz, bn, i, j, k) = (real)0.0;
+        }
+      }
+}
+
+void oracle_lsm3dcomputefieldextensioneqnrhs_(real *rhs, BOXARGS(r), const real *S, BOXARGS(S),
+                                              const real *phi, BOXARGS(phi), const real *sx,
+                                              const real *sy, const real *sz, BOXARGS(gs),
+                                              const real *nx, const real *ny, const real *nz,
+                                              BOXARGS(n), BOXARGS(fb), const real *dx,
+                                              const real *dy, const real *dz) {
+  /* lsm_field_extension3d.f:31-158: -(n . grad S) away from the zero level set
+   * (|phi| > 3 max(dx) or phi of one sign on the 6 neighbours), else 0 */
+  box_t br = MKBOX(r), bphi = MKBOX(phi), bs = MKBOX(gs), bn = MKBOX(n);
+  (void)S; (void)ilo_S; (void)ihi_S; (void)jlo_S; (void)jhi_S; (void)klo_S; (void)khi_S;
+  real m = *dx;
+  if (*dy > m) m = *dy;
+  if (*dz > m) m = *dz;
+  real cutoff = 3.0 * m;                                           /* :115 */
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        real pc = A(phi, bphi, i, j, k);
+        int active = fabs(pc) > cutoff;
+        if (!active) {
+          real z = (real)0.0;
+          active = pc * A(phi, bphi, i - 1, j, k) > z && pc * A(phi, bphi, i + 1, j, k) > z &&
+                   pc * A(phi, bphi, i, j - 1, k) > z && pc * A(phi, bphi, i, j + 1, k) > z &&
+                   pc * A(phi, bphi, i, j, k - 1) > z && pc * A(phi, bphi, i, j, k + 1) > z;
+        }
+        if (active)
+          A(rhs, br, i, j, k) = -(A(nx, bn, i, j, k) * A(sx, bs, i, j, k) +
+                                  A(ny, bn, i, j, k) * A(sy, bs, i, j, k) +
+                                  A(nz, bn, i, j, k) * A(sz, bs, i, j, k));
+        else
+          A(rhs, br, i, j, k) = (real)0.0;
+      }
+}
+
+/* ---------------------------------------------------------------------- */
 /* Boundary conditions  lsm_boundary_conditions3d.f                         */
 /* bdry_location_idx: 0 x_lo, 1 x_hi, 2 y_lo, 3 y_hi, 4 z_lo, 5 z_hi (:13-20) */
 /* Each face loops over the full ghost box in the two other directions.     */

@@ -284,3 +284,63 @@ class CurvatureFlowOracle:
         stats["max_abs_err"] = float(max_abs_err)
         stats["vol_phi"] = float(vol_phi)
         return stats
+
+
+class FieldExtensionOracle:
+    """FieldExtensionAlgorithm::computeExtensionField on one box, composed from
+    the toolbox routines of `lib` (FieldExtensionAlgorithm.cc:195-440,
+    1005-1170): the PHI_UPWIND signed unit normal of phi (LSM3D_HJ_WENO5 ->
+    LSM3D_PHI_UPWIND_GRAD_F -> LSM3D_COMPUTE_SIGNED_UNIT_NORMAL), then TVD-RK3
+    steps of S_t = rhs with rhs = LSM3D_UPWIND_HJ_WENO5(S, velocity = normal)
+    -> LSM3D_COMPUTE_FIELD_EXTENSION_EQN_RHS, dt = cfl * min(dx), ghost fill
+    of S (bc) after every stage."""
+
+    def __init__(self, lib, grid, bc, cfl=0.5):
+        self.o, self.grid, self.bc, self.cfl = lib, grid, tuple(bc), float(cfl)
+        z = np.zeros(grid.shape, dtype=lib.dtype)
+        self.pm = [z.copy() for _ in range(6)]
+        self.grad = [z.copy() for _ in range(3)]
+        self.normal = [z.copy() for _ in range(3)]
+        self.sg = [z.copy() for _ in range(3)]
+        self.rhs, self.D1 = z.copy(), z.copy()
+        self.phi = None
+        self.dt = self.cfl * min(grid.dx)
+
+    def normals(self, phi):
+        o, g = self.o, self.grid
+        xp, yp, zp, xm, ym, zm = self.pm
+        o.lsm3dhjweno5_(xp, yp, zp, g.gb, xm, ym, zm, g.gb, phi, g.gb, self.D1, g.gb, g.fb, *g.dx)
+        o.lsm3dphiupwindgradf_(*self.grad, g.gb, xp, yp, zp, g.gb, xm, ym, zm, g.gb, phi, g.gb, g.fb)
+        o.lsm3dcomputesignedunitnormal_(*self.normal, g.gb, *self.grad, g.gb, phi, g.gb, g.fb, *g.dx)
+        self.phi = phi
+        return self.normal
+
+    def _rhs(self, S):
+        o, g = self.o, self.grid
+        o.lsm3dupwindhjweno5_(*self.sg, g.gb, S, g.gb, *self.normal, g.gb, self.D1, g.gb, g.fb, *g.dx)
+        o.lsm3dcomputefieldextensioneqnrhs_(self.rhs, g.gb, S, g.gb, self.phi, g.gb, *self.sg, g.gb,
+                                            *self.normal, g.gb, g.fb, *g.dx)
+        return self.rhs
+
+    def step(self, S):
+        o, g = self.o, self.grid
+        s1, s2, new = S.copy(), S.copy(), S.copy()
+        o.lsm3dtvdrk3stage1_(s1, g.gb, S, g.gb, self._rhs(S), g.gb, g.fb, self.dt)
+        fill_ghosts(o, s1, g, self.bc)
+        o.lsm3dtvdrk3stage2_(s2, g.gb, s1, g.gb, S, g.gb, self._rhs(s1), g.gb, g.fb, self.dt)
+        fill_ghosts(o, s2, g, self.bc)
+        o.lsm3dtvdrk3stage3_(new, g.gb, s2, g.gb, S, g.gb, self._rhs(s2), g.gb, g.fb, self.dt)
+        fill_ghosts(o, new, g, self.bc)
+        S[...] = new
+
+    def extend(self, S, phi, max_iterations=100, stop_tol=None):
+        self.normals(phi)
+        fill_ghosts(self.o, S, self.grid, self.bc)
+        count, delta = 0, None
+        while count < max_iterations and (stop_tol is None or delta is None or delta > stop_tol):
+            prev = S.copy()
+            self.step(S)
+            count += 1
+            if stop_tol is not None:
+                delta = self.o.lsm3dmaxnormdiff_(S, self.grid.gb, prev, self.grid.gb, self.grid.fb)
+        return count, delta

@@ -73,6 +73,8 @@ SPATIAL = {"lsm3dhjeno1_", "lsm3dhjeno2_", "lsm3dhjeno3_", "lsm3dhjweno5_",
            "lsm3dupwindhjweno5_"}
 BC = {n for n in SIGNATURES if "extrapolation" in n or "neumann" in n}
 CENTRAL = {"lsm3dcentralgradorder2_", "lsm3dcentralgradorder4_", "lsm3dlaplacianorder2_"}
+# routines that read phi at the six neighbours: (routine, phi group index)
+PHI_NEIGHBOURS = {"lsm3dphiupwindgradf_": 3, "lsm3dcomputefieldextensioneqnrhs_": 2}
 
 
 def _field(rng, box, kind, real, eps=None):
@@ -116,6 +118,8 @@ def make_case(name, real="double", seed=0, fb=(-3, 8, 2, 12, 5, 14)):
                 box = grow(fb, 2)          # the reference loops over the rhs ghost box
             elif name in BC:
                 box = grow(fb, (3, 2, 3), (2, 3, 1))
+            elif name in PHI_NEIGHBOURS and gi == PHI_NEIGHBOURS[name]:
+                box = grow(fb, (1, 2, 1), (2, 1, 1))
             elif name in CENTRAL:
                 # outputs: small uneven ghosts; phi: >= 2 ghosts (order 4 reads +-2)
                 box = grow(fb, (1, 0, 2), (0, 2, 1)) if gi == 0 else grow(fb, (2, 3, 2), (3, 2, 2))
@@ -153,9 +157,14 @@ def make_case(name, real="double", seed=0, fb=(-3, 8, 2, 12, 5, 14)):
     if name == "lsm3dcomputereinitializationeqnrhssubcellfixorder1_":
         _fix_distance0(case, real)
     if name == "lsm3dgradientmagnitude_":
-        _small_gradients(case, rng, 1e-5)
+        _small_gradients(case, rng, 1e-5, 0)
     if name == "lsm3daddconstcurvtermtolserhs_":
-        _small_gradients(case, rng, 0.0)
+        _small_gradients(case, rng, 0.0, 1)
+    if name in ("lsm3dcomputeunitnormal_", "lsm3dcomputesignedunitnormal_"):
+        _small_gradients(case, rng, 0.0, 3)
+    if name in ("lsm3dcomputesignedunitnormal_", "lsm3dphiupwindgradf_"):
+        phi = [a for a in case.args if isinstance(a, np.ndarray)][-1]
+        phi[rng.random(phi.shape) < 0.05] = 0.0   # phi == 0 branches
     return case
 
 
@@ -183,6 +192,8 @@ def _value_kind(name, gi, a_idx):
         return "near" if gi == 1 else "rand"
     if name.startswith("lsm3dvolumeregion"):
         return "near"
+    if name == "lsm3dcomputefieldextensioneqnrhs_" and gi == 2:
+        return "near"   # |phi| around the 3 dx cut-off, sign changes between neighbours
     if name == "lsm3daddnormalveltermtolserhs_" and gi == 3:
         return "vel"
     if name == "lsm3daddexternalandnormalveltermtolserhs_" and gi == 3:
@@ -225,12 +236,12 @@ def _int_arg(name, rng):
     return 0
 
 
-def _small_gradients(case, rng, v):
-    """Sets the three gradient components (the first three arrays after the
-    output, or the first three) to v or 0 on a random tenth of the cells so
-    the routines' small-gradient branches run."""
+def _small_gradients(case, rng, v, first):
+    """Sets the three gradient components (arrays first..first+2 of the
+    argument list) to v (the y component to 0) on a random tenth of the cells
+    so the routines' small-gradient branches run."""
     arrs = [a for a in case.args if isinstance(a, np.ndarray)]
-    g = arrs[:3] if case.name == "lsm3dgradientmagnitude_" else arrs[1:4]
+    g = arrs[first:first + 3]
     sel = rng.random(g[0].shape) < 0.1
     for d, a in enumerate(g):
         a[sel] = v if d != 1 else 0.0
