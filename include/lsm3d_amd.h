@@ -386,6 +386,390 @@ void lsm3dhomogeneousneumanneno3_(LSMLIB_REAL *phi, LSM_AMD_BOX(gb), LSM_AMD_BOX
 void lsm3dhomogeneousneumannweno5_(LSMLIB_REAL *phi, LSM_AMD_BOX(gb), LSM_AMD_BOX(fb),
                                    const int *bdry_location_idx);
 
+/* --- narrow band (Peng et al.'s local level set method) ---------------------
+ * Index lists index_x/y/z(nlo:nhi): entry l at element l - nlo of the array
+ * passed.  The lists, index_outer and narrow_band may be device or host
+ * memory; n_lo / n_hi, change_sign and the other int outputs are host
+ * memory.  Marks compare as signed bytes (the Fortran's integer*1).  List
+ * entries outside the arrays they index are skipped and reported as an error
+ * (undefined behaviour in the reference).  HJ_ENO1/2_LOCAL write D1/D2 as
+ * the reference does.  COMPUTE_STABLE_CONST_NORMAL_VEL_DT_LOCAL uses the box
+ * routine's |vel_n| and smoothed |grad phi| (the reference reads two
+ * uninitialised variables there, lsm_utilities3d_local.f.in:728).  Inputs
+ * the reference headers declare non-const (and the outputs of COPY_DATA_LOCAL
+ * and GRADIENT_MAGNITUDE_LOCAL, which they declare const) are declared here
+ * by how the routines use them; the symbols and argument order are theirs. */
+/* --- narrow band: src/toolbox/localization/lsm_localization3d.h --- */
+/* LSM3D_DETERMINE_NARROW_BAND, lsm_localization3d.h:89 */
+void lsm3ddeterminenarrowband_(const LSMLIB_REAL *phi, LSM_AMD_BOX(gb), unsigned char *narrow_band,
+                               LSM_AMD_BOX(nb_gb), int *index_x, int *index_y, int *index_z,
+                               const int *nlo_index, const int *nhi_index, int *n_lo, int *n_hi,
+                               int *index_outer, const int *nlo_index_outer,
+                               const int *nhi_index_outer, int *nlo_index_outer_plus,
+                               int *nhi_index_outer_plus, int *nlo_index_outer_minus,
+                               int *nhi_index_outer_minus, const LSMLIB_REAL *width,
+                               const LSMLIB_REAL *width_inner, const int *level);
+/* LSM3D_MARK_NARROW_BAND_BOUNDARY_LAYER, lsm_localization3d.h:138 */
+void lsm3dmarknarrowbandboundarylayer_(unsigned char *narrow_band, LSM_AMD_BOX(gb),
+                                       LSM_AMD_BOX(fb), const unsigned char *mark_boundary_layer);
+/* LSM3D_DETERMINE_NARROW_BAND_FROM_MASK, lsm_localization3d.h:187 */
+void lsm3ddeterminenarrowbandfrommask_(const LSMLIB_REAL *mask, LSM_AMD_BOX(mask_gb),
+                                       unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                                       int *index_x, int *index_y, int *index_z, int *nlo_index,
+                                       int *nhi_index, int *n_lo, int *n_hi, const int *level,
+                                       const int *use_mask_sign);
+/* LSM3D_MULTIPLY_CUT_OFF_LSE_RHS_LOCAL, lsm_localization3d.h:230 */
+void lsm3dmultiplycutofflserhslocal_(const LSMLIB_REAL *phi, LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(gb),
+                                     const int *index_x, const int *index_y, const int *index_z,
+                                     const int *nlo_index, const int *nhi_index,
+                                     const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                                     const unsigned char *mark_fb, const LSMLIB_REAL *beta,
+                                     const LSMLIB_REAL *gamma);
+/* LSM3D_CHECK_OUTER_NARROW_BAND_LAYER, lsm_localization3d.h:270 */
+void lsm3dcheckouternarrowbandlayer_(int *change_sign, const LSMLIB_REAL *phi, LSM_AMD_BOX(gb),
+                                     const int *index_x, const int *index_y, const int *index_z,
+                                     const int *nlo_index, const int *nhi_index, const int *index_outer,
+                                     const int *nlo_index_outer, const int *nhi_index_outer,
+                                     const int *nlo_index_outer_plus,
+                                     const int *nhi_index_outer_plus,
+                                     const int *nlo_index_outer_minus,
+                                     const int *nhi_index_outer_minus);
+/* LSM3D_IMPOSE_MASK_LOCAL, lsm_localization3d.h:307 */
+void lsm3dimposemasklocal_(LSMLIB_REAL *dest, LSM_AMD_BOX(dest_gb), const LSMLIB_REAL *src,
+                           LSM_AMD_BOX(src_gb), const LSMLIB_REAL *mask, LSM_AMD_BOX(mask_gb),
+                           const int *index_x, const int *index_y, const int *index_z,
+                           const int *nlo_index, const int *nhi_index);
+/* LSM3D_COPY_DATA_LOCAL, lsm_localization3d.h:347 */
+void lsm3dcopydatalocal_(LSMLIB_REAL *dest, LSM_AMD_BOX(dest_gb), const LSMLIB_REAL *src,
+                         LSM_AMD_BOX(src_gb), const int *index_x, const int *index_y,
+                         const int *index_z, const int *nlo_index, const int *nhi_index);
+
+/* --- narrow-band kernels: src/toolbox/spatial_derivatives/lsm_spatial_derivatives3d_local.h --- */
+/* LSM3D_HJ_ENO1_LOCAL, lsm_spatial_derivatives3d_local.h:78 */
+void lsm3dhjeno1local_(LSMLIB_REAL *phi_x_plus, LSMLIB_REAL *phi_y_plus, LSMLIB_REAL *phi_z_plus,
+                       LSM_AMD_BOX(grad_phi_plus_gb), LSMLIB_REAL *phi_x_minus,
+                       LSMLIB_REAL *phi_y_minus, LSMLIB_REAL *phi_z_minus,
+                       LSM_AMD_BOX(grad_phi_minus_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                       LSMLIB_REAL *D1, LSM_AMD_BOX(D1_gb), const LSMLIB_REAL *dx,
+                       const LSMLIB_REAL *dy, const LSMLIB_REAL *dz, const int *index_x,
+                       const int *index_y, const int *index_z, const int *nlo_index0,
+                       const int *nhi_index0, const int *nlo_index1, const int *nhi_index1,
+                       const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                       const unsigned char *mark_fb, const unsigned char *mark_D1);
+/* LSM3D_HJ_ENO2_LOCAL, lsm_spatial_derivatives3d_local.h:159 */
+void lsm3dhjeno2local_(LSMLIB_REAL *phi_x_plus, LSMLIB_REAL *phi_y_plus, LSMLIB_REAL *phi_z_plus,
+                       LSM_AMD_BOX(grad_phi_plus_gb), LSMLIB_REAL *phi_x_minus,
+                       LSMLIB_REAL *phi_y_minus, LSMLIB_REAL *phi_z_minus,
+                       LSM_AMD_BOX(grad_phi_minus_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                       LSMLIB_REAL *D1, LSM_AMD_BOX(D1_gb), LSMLIB_REAL *D2, LSM_AMD_BOX(D2_gb),
+                       const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+                       const int *index_x, const int *index_y, const int *index_z,
+                       const int *nlo_index0, const int *nhi_index0, const int *nlo_index1,
+                       const int *nhi_index1, const int *nlo_index2, const int *nhi_index2,
+                       const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                       const unsigned char *mark_fb, const unsigned char *mark_D1,
+                       const unsigned char *mark_D2);
+/* LSM3D_CENTRAL_GRAD_ORDER2_LOCAL, lsm_spatial_derivatives3d_local.h:240 */
+void lsm3dcentralgradorder2local_(LSMLIB_REAL *phi_x, LSMLIB_REAL *phi_y, LSMLIB_REAL *phi_z,
+                                  LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *phi,
+                                  LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *dx,
+                                  const LSMLIB_REAL *dy, const LSMLIB_REAL *dz, const int *index_x,
+                                  const int *index_y, const int *index_z, const int *nlo_index,
+                                  const int *nhi_index, const unsigned char *narrow_band,
+                                  LSM_AMD_BOX(nb_gb), const unsigned char *mark_fb);
+/* LSM3D_CENTRAL_GRAD_ORDER4_LOCAL, lsm_spatial_derivatives3d_local.h:293 */
+void lsm3dcentralgradorder4local_(LSMLIB_REAL *phi_x, LSMLIB_REAL *phi_y, LSMLIB_REAL *phi_z,
+                                  LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *phi,
+                                  LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *dx,
+                                  const LSMLIB_REAL *dy, const LSMLIB_REAL *dz, const int *index_x,
+                                  const int *index_y, const int *index_z, const int *nlo_index,
+                                  const int *nhi_index, const unsigned char *narrow_band,
+                                  LSM_AMD_BOX(nb_gb), const unsigned char *mark_fb);
+/* LSM3D_LAPLACIAN_ORDER2_LOCAL, lsm_spatial_derivatives3d_local.h:345 */
+void lsm3dlaplacianorder2local_(LSMLIB_REAL *laplacian_phi, LSM_AMD_BOX(laplacian_phi_gb),
+                                const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *dx,
+                                const LSMLIB_REAL *dy, const LSMLIB_REAL *dz, const int *index_x,
+                                const int *index_y, const int *index_z, const int *nlo_index,
+                                const int *nhi_index, const unsigned char *narrow_band,
+                                LSM_AMD_BOX(nb_gb), const unsigned char *mark_fb);
+/* LSM3D_COMPUTE_AVE_GRAD_PHI_LOCAL, lsm_spatial_derivatives3d_local.h:397 */
+void lsm3dcomputeavegradphilocal_(LSMLIB_REAL *grad_phi_ave, const LSMLIB_REAL *phi,
+                                  LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *dx,
+                                  const LSMLIB_REAL *dy, const LSMLIB_REAL *dz, const int *index_x,
+                                  const int *index_y, const int *index_z, const int *nlo_index,
+                                  const int *nhi_index, const unsigned char *narrow_band,
+                                  LSM_AMD_BOX(nb_gb), const unsigned char *mark_fb);
+/* LSM3D_GRADIENT_MAGNITUDE_LOCAL, lsm_spatial_derivatives3d_local.h:438 */
+void lsm3dgradientmagnitudelocal_(const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
+                                  const LSMLIB_REAL *phi_z, LSMLIB_REAL *grad_phi_mag,
+                                  LSM_AMD_BOX(grad_phi_gb), const int *index_x, const int *index_y,
+                                  const int *index_z, const int *nlo_index, const int *nhi_index,
+                                  const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                                  const unsigned char *mark_fb);
+
+/* --- narrow-band kernels: src/toolbox/time_integration/lsm_tvd_runge_kutta3d_local.h --- */
+/* LSM3D_RK1_STEP_LOCAL, lsm_tvd_runge_kutta3d_local.h:35 */
+void lsm3drk1steplocal_(LSMLIB_REAL *u_next, LSM_AMD_BOX(u_next_gb), const LSMLIB_REAL *u_cur,
+                        LSM_AMD_BOX(u_cur_gb), const LSMLIB_REAL *rhs, LSM_AMD_BOX(rhs_gb),
+                        const LSMLIB_REAL *dt, const int *index_x, const int *index_y,
+                        const int *index_z, const int *nlo_index, const int *nhi_index,
+                        const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                        const unsigned char *mark_fb);
+/* LSM3D_TVD_RK2_STAGE1_LOCAL, lsm_tvd_runge_kutta3d_local.h:73 */
+void lsm3dtvdrk2stage1local_(LSMLIB_REAL *u_stage1, LSM_AMD_BOX(u_stage1_gb),
+                             const LSMLIB_REAL *u_cur, LSM_AMD_BOX(u_cur_gb),
+                             const LSMLIB_REAL *rhs, LSM_AMD_BOX(rhs_gb), const LSMLIB_REAL *dt,
+                             const int *index_x, const int *index_y, const int *index_z,
+                             const int *nlo_index, const int *nhi_index,
+                             const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                             const unsigned char *mark_fb);
+/* LSM3D_TVD_RK2_STAGE2_LOCAL, lsm_tvd_runge_kutta3d_local.h:111 */
+void lsm3dtvdrk2stage2local_(LSMLIB_REAL *u_next, LSM_AMD_BOX(u_next_gb),
+                             const LSMLIB_REAL *u_stage1, LSM_AMD_BOX(u_stage1_gb),
+                             const LSMLIB_REAL *u_cur, LSM_AMD_BOX(u_cur_gb),
+                             const LSMLIB_REAL *rhs, LSM_AMD_BOX(rhs_gb), const LSMLIB_REAL *dt,
+                             const int *index_x, const int *index_y, const int *index_z,
+                             const int *nlo_index, const int *nhi_index,
+                             const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                             const unsigned char *mark_fb);
+/* LSM3D_TVD_RK3_STAGE1_LOCAL, lsm_tvd_runge_kutta3d_local.h:155 */
+void lsm3dtvdrk3stage1local_(LSMLIB_REAL *u_stage1, LSM_AMD_BOX(u_stage1_gb),
+                             const LSMLIB_REAL *u_cur, LSM_AMD_BOX(u_cur_gb),
+                             const LSMLIB_REAL *rhs, LSM_AMD_BOX(rhs_gb), const LSMLIB_REAL *dt,
+                             const int *index_x, const int *index_y, const int *index_z,
+                             const int *nlo_index, const int *nhi_index,
+                             const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                             const unsigned char *mark_fb);
+/* LSM3D_TVD_RK3_STAGE2_LOCAL, lsm_tvd_runge_kutta3d_local.h:192 */
+void lsm3dtvdrk3stage2local_(LSMLIB_REAL *u_stage2, LSM_AMD_BOX(u_stage2_gb),
+                             const LSMLIB_REAL *u_stage1, LSM_AMD_BOX(u_stage1_gb),
+                             const LSMLIB_REAL *u_cur, LSM_AMD_BOX(u_cur_gb),
+                             const LSMLIB_REAL *rhs, LSM_AMD_BOX(rhs_gb), const LSMLIB_REAL *dt,
+                             const int *index_x, const int *index_y, const int *index_z,
+                             const int *nlo_index, const int *nhi_index,
+                             const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                             const unsigned char *mark_fb);
+/* LSM3D_TVD_RK3_STAGE3_LOCAL, lsm_tvd_runge_kutta3d_local.h:237 */
+void lsm3dtvdrk3stage3local_(LSMLIB_REAL *u_next, LSM_AMD_BOX(u_next_gb),
+                             const LSMLIB_REAL *u_stage2, LSM_AMD_BOX(u_stage2_gb),
+                             const LSMLIB_REAL *u_cur, LSM_AMD_BOX(u_cur_gb),
+                             const LSMLIB_REAL *rhs, LSM_AMD_BOX(rhs_gb), const LSMLIB_REAL *dt,
+                             const int *index_x, const int *index_y, const int *index_z,
+                             const int *nlo_index, const int *nhi_index,
+                             const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                             const unsigned char *mark_fb);
+
+/* --- narrow-band kernels: src/toolbox/level_set_evolution/lsm_level_set_evolution3d_local.h --- */
+/* LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS_LOCAL, lsm_level_set_evolution3d_local.h:67 */
+void lsm3dzerooutlevelseteqnrhslocal_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb),
+                                      const int *index_x, const int *index_y, const int *index_z,
+                                      const int *nlo_index, const int *nhi_index);
+/* LSM3D_ADD_ADVECTION_TERM_TO_LSE_RHS_LOCAL, lsm_level_set_evolution3d_local.h:102 */
+void lsm3daddadvectiontermtolserhslocal_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb),
+                                         const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
+                                         const LSMLIB_REAL *phi_z, LSM_AMD_BOX(grad_phi_gb),
+                                         const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y,
+                                         const LSMLIB_REAL *vel_z, LSM_AMD_BOX(vel_gb),
+                                         const int *index_x, const int *index_y,
+                                         const int *index_z, const int *nlo_index,
+                                         const int *nhi_index, const unsigned char *narrow_band,
+                                         LSM_AMD_BOX(nb_gb), const unsigned char *mark_fb);
+/* LSM3D_ADD_NORMAL_VEL_TERM_TO_LSE_RHS_LOCAL, lsm_level_set_evolution3d_local.h:166 */
+void lsm3daddnormalveltermtolserhslocal_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb),
+                                         const LSMLIB_REAL *phi_x_plus,
+                                         const LSMLIB_REAL *phi_y_plus,
+                                         const LSMLIB_REAL *phi_z_plus,
+                                         LSM_AMD_BOX(grad_phi_plus_gb),
+                                         const LSMLIB_REAL *phi_x_minus,
+                                         const LSMLIB_REAL *phi_y_minus,
+                                         const LSMLIB_REAL *phi_z_minus,
+                                         LSM_AMD_BOX(grad_phi_minus_gb), const LSMLIB_REAL *vel_n,
+                                         LSM_AMD_BOX(vel_gb), const int *index_x,
+                                         const int *index_y, const int *index_z,
+                                         const int *nlo_index, const int *nhi_index,
+                                         const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                                         const unsigned char *mark_fb);
+/* LSM3D_ADD_CONST_NORMAL_VEL_TERM_TO_LSE_RHS_LOCAL, lsm_level_set_evolution3d_local.h:237 */
+void lsm3daddconstnormalveltermtolserhslocal_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb),
+                                              const LSMLIB_REAL *phi_x_plus,
+                                              const LSMLIB_REAL *phi_y_plus,
+                                              const LSMLIB_REAL *phi_z_plus,
+                                              LSM_AMD_BOX(grad_phi_plus_gb),
+                                              const LSMLIB_REAL *phi_x_minus,
+                                              const LSMLIB_REAL *phi_y_minus,
+                                              const LSMLIB_REAL *phi_z_minus,
+                                              LSM_AMD_BOX(grad_phi_minus_gb),
+                                              const LSMLIB_REAL *vel_n, const int *index_x,
+                                              const int *index_y, const int *index_z,
+                                              const int *nlo_index, const int *nhi_index,
+                                              const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                                              const unsigned char *mark_fb);
+/* LSM3D_ADD_CONST_CURV_TERM_TO_LSE_RHS_LOCAL, lsm_level_set_evolution3d_local.h:300 */
+void lsm3daddconstcurvtermtolserhslocal_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb),
+                                         const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
+                                         const LSMLIB_REAL *phi_z, LSM_AMD_BOX(grad_phi_gb),
+                                         const LSMLIB_REAL *phi_xx, const LSMLIB_REAL *phi_xy,
+                                         const LSMLIB_REAL *phi_xz, const LSMLIB_REAL *phi_yy,
+                                         const LSMLIB_REAL *phi_yz, const LSMLIB_REAL *phi_zz,
+                                         LSM_AMD_BOX(grad2_phi_gb), const LSMLIB_REAL *b,
+                                         const int *index_x, const int *index_y,
+                                         const int *index_z, const int *nlo_index,
+                                         const int *nhi_index, const unsigned char *narrow_band,
+                                         LSM_AMD_BOX(nb_gb), const unsigned char *mark_fb);
+/* LSM3D_ADD_CONST_PRECOMPUTED_CURV_TERM_TO_LSE_RHS_LOCAL, lsm_level_set_evolution3d_local.h:374 */
+void lsm3daddconstprecomputedcurvtermtolserhslocal_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(rhs_gb),
+                                                    const LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb),
+                                                    const LSMLIB_REAL *grad_mag_phi,
+                                                    LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *b,
+                                                    const int *index_x, const int *index_y,
+                                                    const int *index_z, const int *nlo_index,
+                                                    const int *nhi_index,
+                                                    const unsigned char *narrow_band,
+                                                    LSM_AMD_BOX(nb_gb),
+                                                    const unsigned char *mark_fb);
+/* LSM3D_ADD_EXTERNAL_AND_NORMAL_VEL_TERM_TO_LSE_RHS_LOCAL, lsm_level_set_evolution3d_local.h:449 */
+void lsm3daddexternalandnormalveltermtolserhslocal_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(rhs_gb),
+                                                    const LSMLIB_REAL *phi_x_plus,
+                                                    const LSMLIB_REAL *phi_y_plus,
+                                                    const LSMLIB_REAL *phi_z_plus,
+                                                    LSM_AMD_BOX(grad_phi_plus_gb),
+                                                    const LSMLIB_REAL *phi_x_minus,
+                                                    const LSMLIB_REAL *phi_y_minus,
+                                                    const LSMLIB_REAL *phi_z_minus,
+                                                    LSM_AMD_BOX(grad_phi_minus_gb),
+                                                    const LSMLIB_REAL *vel_n,
+                                                    const LSMLIB_REAL *vel_x,
+                                                    const LSMLIB_REAL *vel_y,
+                                                    const LSMLIB_REAL *vel_z, LSM_AMD_BOX(vel_gb),
+                                                    const int *index_x, const int *index_y,
+                                                    const int *index_z, const int *nlo_index,
+                                                    const int *nhi_index,
+                                                    const unsigned char *narrow_band,
+                                                    LSM_AMD_BOX(nb_gb),
+                                                    const unsigned char *mark_fb);
+
+/* --- narrow-band kernels: src/toolbox/reinitialization/lsm_reinitialization3d_local.h --- */
+/* LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS_LOCAL, lsm_reinitialization3d_local.h:41 */
+void lsm3dcomputereinitializationeqnrhslocal_(LSMLIB_REAL *reinit_rhs, LSM_AMD_BOX(rhs_gb),
+                                              const LSMLIB_REAL * phi, LSM_AMD_BOX(phi_gb),
+                                              const LSMLIB_REAL * phi0, LSM_AMD_BOX(phi0_gb),
+                                              const LSMLIB_REAL *phi_x_plus,
+                                              const LSMLIB_REAL *phi_y_plus,
+                                              const LSMLIB_REAL *phi_z_plus,
+                                              LSM_AMD_BOX(grad_phi_plus_gb),
+                                              const LSMLIB_REAL *phi_x_minus,
+                                              const LSMLIB_REAL *phi_y_minus,
+                                              const LSMLIB_REAL *phi_z_minus,
+                                              LSM_AMD_BOX(grad_phi_minus_gb),
+                                              const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                              const LSMLIB_REAL *dz, const int *use_phi0_for_sgn,
+                                              const int *index_x, const int *index_y,
+                                              const int *index_z, const int *nlo_index,
+                                              const int *nhi_index,
+                                              const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                                              const unsigned char *mark_fb);
+/* LSM3D_COMPUTE_ORTHOGONALIZATION_EQN_RHS_LOCAL, lsm_reinitialization3d_local.h:99 */
+void lsm3dcomputeorthogonalizationeqnrhslocal_(LSMLIB_REAL *ortho_rhs, LSM_AMD_BOX(rhs_gb),
+                                               const LSMLIB_REAL *phi_x_plus,
+                                               const LSMLIB_REAL *phi_y_plus,
+                                               const LSMLIB_REAL *phi_z_plus,
+                                               LSM_AMD_BOX(grad_phi_plus_gb),
+                                               const LSMLIB_REAL *phi_x_minus,
+                                               const LSMLIB_REAL *phi_y_minus,
+                                               const LSMLIB_REAL *phi_z_minus,
+                                               LSM_AMD_BOX(grad_phi_minus_gb),
+                                               const LSMLIB_REAL * psi, LSM_AMD_BOX(psi_gb),
+                                               const LSMLIB_REAL *psi_x_plus,
+                                               const LSMLIB_REAL *psi_y_plus,
+                                               const LSMLIB_REAL *psi_z_plus,
+                                               LSM_AMD_BOX(grad_psi_plus_gb),
+                                               const LSMLIB_REAL *psi_x_minus,
+                                               const LSMLIB_REAL *psi_y_minus,
+                                               const LSMLIB_REAL *psi_z_minus,
+                                               LSM_AMD_BOX(grad_psi_minus_gb),
+                                               const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                               const LSMLIB_REAL *dz, const int *index_x,
+                                               const int *index_y, const int *index_z,
+                                               const int *nlo_index, const int *nhi_index,
+                                               const unsigned char *narrow_band,
+                                               LSM_AMD_BOX(nb_gb), const unsigned char *mark_fb);
+
+/* --- narrow-band kernels: src/toolbox/utilities/lsm_utilities3d_local.h --- */
+/* LSM3D_MAX_NORM_DIFF_LOCAL, lsm_utilities3d_local.h:65 */
+void lsm3dmaxnormdifflocal_(LSMLIB_REAL *max_norm_diff, const LSMLIB_REAL *field1,
+                            LSM_AMD_BOX(field1_gb), const LSMLIB_REAL *field2,
+                            LSM_AMD_BOX(field2_gb), const int *index_x, const int *index_y,
+                            const int *index_z, const int *nlo_index, const int *nhi_index,
+                            const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                            const unsigned char *mark_fb);
+/* LSM3D_COMPUTE_STABLE_ADVECTION_DT_LOCAL, lsm_utilities3d_local.h:112 */
+void lsm3dcomputestableadvectiondtlocal_(LSMLIB_REAL *dt, const LSMLIB_REAL *vel_x,
+                                         const LSMLIB_REAL *vel_y, const LSMLIB_REAL *vel_z,
+                                         LSM_AMD_BOX(vel_gb), const LSMLIB_REAL *dx,
+                                         const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+                                         const LSMLIB_REAL *cfl_number, const int *index_x,
+                                         const int *index_y, const int *index_z,
+                                         const int *nlo_index, const int *nhi_index,
+                                         const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                                         const unsigned char *mark_fb);
+/* LSM3D_COMPUTE_STABLE_NORMAL_VEL_DT_LOCAL, lsm_utilities3d_local.h:169 */
+void lsm3dcomputestablenormalveldtlocal_(LSMLIB_REAL *dt, const LSMLIB_REAL *vel_n,
+                                         LSM_AMD_BOX(vel_gb), const LSMLIB_REAL *phi_x_plus,
+                                         const LSMLIB_REAL *phi_y_plus,
+                                         const LSMLIB_REAL *phi_z_plus,
+                                         LSM_AMD_BOX(grad_phi_plus_gb),
+                                         const LSMLIB_REAL *phi_x_minus,
+                                         const LSMLIB_REAL *phi_y_minus,
+                                         const LSMLIB_REAL *phi_z_minus,
+                                         LSM_AMD_BOX(grad_phi_minus_gb), const LSMLIB_REAL *dx,
+                                         const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+                                         const LSMLIB_REAL *cfl_number, const int *index_x,
+                                         const int *index_y, const int *index_z,
+                                         const int *nlo_index, const int *nhi_index,
+                                         const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                                         const unsigned char *mark_fb);
+/* LSM3D_COMPUTE_STABLE_NORMAL_VEL_DT_CONTROL_VOLUME_LOCAL, lsm_utilities3d_local.h:241 */
+void lsm3dcomputestablenormalveldtcontrolvolumelocal_(LSMLIB_REAL *dt, const LSMLIB_REAL *vel_n,
+                                                      LSM_AMD_BOX(vel_gb),
+                                                      const LSMLIB_REAL *phi_x_plus,
+                                                      const LSMLIB_REAL *phi_y_plus,
+                                                      const LSMLIB_REAL *phi_z_plus,
+                                                      LSM_AMD_BOX(grad_phi_plus_gb),
+                                                      const LSMLIB_REAL *phi_x_minus,
+                                                      const LSMLIB_REAL *phi_y_minus,
+                                                      const LSMLIB_REAL *phi_z_minus,
+                                                      LSM_AMD_BOX(grad_phi_minus_gb),
+                                                      const LSMLIB_REAL *control_vol,
+                                                      LSM_AMD_BOX(control_vol_gb),
+                                                      const int *control_vol_sgn,
+                                                      const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                                      const LSMLIB_REAL *dz,
+                                                      const LSMLIB_REAL *cfl_number,
+                                                      const int *index_x, const int *index_y,
+                                                      const int *index_z, const int *nlo_index,
+                                                      const int *nhi_index,
+                                                      const unsigned char *narrow_band,
+                                                      LSM_AMD_BOX(nb_gb),
+                                                      const unsigned char *mark_fb);
+/* LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT_LOCAL, lsm_utilities3d_local.h:322 */
+void lsm3dcomputestableconstnormalveldtlocal_(LSMLIB_REAL *dt, const LSMLIB_REAL *vel_n,
+                                              const LSMLIB_REAL *phi_x_plus,
+                                              const LSMLIB_REAL *phi_y_plus,
+                                              const LSMLIB_REAL *phi_z_plus,
+                                              LSM_AMD_BOX(grad_phi_plus_gb),
+                                              const LSMLIB_REAL *phi_x_minus,
+                                              const LSMLIB_REAL *phi_y_minus,
+                                              const LSMLIB_REAL *phi_z_minus,
+                                              LSM_AMD_BOX(grad_phi_minus_gb),
+                                              const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                              const LSMLIB_REAL *dz, const LSMLIB_REAL *cfl_number,
+                                              const int *index_x, const int *index_y,
+                                              const int *index_z, const int *nlo_index,
+                                              const int *nhi_index,
+                                              const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
+                                              const unsigned char *mark_fb);
+
+
 /* ======================================================================
  * Part 2: runtime control (no counterpart in the reference: the Fortran
  * library has no stream, no error channel and no device)

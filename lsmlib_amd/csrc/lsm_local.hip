@@ -1,0 +1,1715 @@
+// gfx950 kernels behind LSMLIB's narrow-band routines: the band builders of
+// lsm_localization3d.f and the *LOCAL kernels, which visit the points named
+// by index lists instead of a box (Peng et al.'s local level set method).
+//
+// Lists.  index_x/y/z(nlo:nhi) hold the (i, j, k) of each band point.  A
+// routine given (nlo, nhi) visits entries nlo..nhi; entry l sits at element
+// l - nlo of the arrays passed (the Fortran declares them
+// index_x(nlo_index:nhi_index)).  Marks are the Fortran's integer*1, so they
+// compare as signed bytes; a point takes part when narrow_band <= mark.
+//
+// Kernels.  One thread per list entry.  Each entry is checked against the
+// boxes of every array it touches, with the stencil's reach; an entry outside
+// is skipped and counted, and a call that completes synchronously reports
+// skipped entries as an error (the reference reads and writes outside its
+// arrays there).  Sums (the average gradient) accumulate in double precision.
+//
+// Band building.  The reference scans phi's box in Fortran order appending
+// every point with |phi| < width, then, level by level, appends the unmarked
+// neighbours of the previous level's points, in the order
+// (i+1, i-1, j+1, j-1, k+1, k-1) of first discovery
+// (lsm_localization3d.f:48-298).  Both orders are reproduced exactly:
+//   level 0: ordered stream compaction -- per-block counts, one scan of the
+//            block counts, then each block writes its points at its offset
+//            (wave ballots give the order inside a block);
+//   level l: every (parent, direction) pair that reaches an unmarked cell
+//            bids its pair number into a per-cell key with atomicMin; the
+//            pair holding the minimum is the first discovery; the winning
+//            pairs are compacted in pair order as in level 0.
+// The host reads the level sizes back (n_lo / n_hi are host outputs).
+#include <climits>
+#include <cmath>
+#include <string>
+
+#include "../../include/lsm3d_amd.h"
+#include "lsm_common.h"
+#include "lsm_runtime.h"
+#include "lsm_stencil.h"
+
+using lsm::Box;
+using lsm::Call;
+using lsm::Field;
+
+namespace lsm {
+
+typedef int8_t nb_t;
+
+__device__ __forceinline__ bool inside(const Box& b, int i, int j, int k) {
+  return i >= b.ilo && i <= b.ihi && j >= b.jlo && j <= b.jhi && k >= b.klo && k <= b.khi;
+}
+
+// One launch's list entries.
+struct Entries {
+  const int* ix;
+  const int* iy;
+  const int* iz;
+  int n;
+  Field<const nb_t> nb;  // nb.p == nullptr: no mark test
+  int mark;
+  Box any;    // every entry: the arrays touched whatever the mark (narrow_band, a Dn output)
+  Box valid;  // marked entries: the array boxes less the stencil's reach
+  int* skipped;
+};
+
+__device__ __forceinline__ bool entry_ok(const Entries& e, int i, int j, int k, bool& marked) {
+  if (!inside(e.any, i, j, k)) return false;
+  marked = (e.nb.p == nullptr) || ((int)e.nb.at(i, j, k) <= e.mark);
+  return !marked || inside(e.valid, i, j, k);
+}
+
+template <typename Op>
+__global__ void __launch_bounds__(256) k_entries(Entries e, Op op) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= e.n) return;
+  const int i = e.ix[t], j = e.iy[t], k = e.iz[t];
+  bool marked = false;
+  if (!entry_ok(e, i, j, k, marked)) {
+    atomicAdd(e.skipped, 1);
+    return;
+  }
+  op(i, j, k, marked);
+}
+
+// ---------------------------------------------------------------------------
+// Per-entry operations.  `m`: the point passes the mark test.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct OpCopy {  // lsm_localization3d.f:724-776
+  Field<T> dst;
+  Field<const T> src;
+  __device__ void operator()(int i, int j, int k, bool) const { dst.at(i, j, k) = src.at(i, j, k); }
+};
+
+template <typename T>
+struct OpMaskMax {  // lsm_localization3d.f:660-718: max(mask, src)
+  Field<T> dst;
+  Field<const T> src, mask;
+  __device__ void operator()(int i, int j, int k, bool) const {
+    const T a = mask.at(i, j, k), b = src.at(i, j, k);
+    dst.at(i, j, k) = (a > b) ? a : b;
+  }
+};
+
+template <typename T>
+struct OpZero {  // lsm_level_set_evolution3d_local.f.in:16-55
+  Field<T> rhs;
+  __device__ void operator()(int i, int j, int k, bool) const { rhs.at(i, j, k) = T(0); }
+};
+
+template <typename T>
+struct OpCutoff {  // lsm_localization3d.f:482-562
+  Field<T> rhs;
+  Field<const T> phi;
+  T beta, gamma, c1, c2;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    if (!m) return;
+    const T a = fabs(phi.at(i, j, k));
+    T cut;
+    if (a <= beta) {
+      cut = T(1);
+    } else if (a <= gamma) {
+      const T t = a - gamma;
+      cut = ((t * t) * (T(2) * a + c1)) / c2;
+    } else {
+      cut = T(0);
+    }
+    rhs.at(i, j, k) = cut * rhs.at(i, j, k);
+  }
+};
+
+template <typename T>
+struct OpRk {  // lsm_tvd_runge_kutta3d_local.f
+  int form;
+  Field<T> out;
+  Field<const T> us, uc, rhs;
+  T dt;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+    if (!m) return;
+    const T c = uc.at(i, j, k);
+    out.at(i, j, k) = rk_combine<T>(form, c, us.p ? us.at(i, j, k) : c, rhs.at(i, j, k), dt);
+  }
+};
+
+// lsm3dComputeDnLOCAL (lsm_spatial_derivatives3d_local.f.in:43-164):
+// Dn = sgn (D(n-1)(c) - D(n-1)(c - off)) on marked points, 1e10 elsewhere
+template <typename T>
+struct OpDn {
+  Field<T> dn;
+  Field<const T> src;
+  int oi, oj, ok;
+  T sgn;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    dn.at(i, j, k) = m ? sgn * (src.at(i, j, k) - src.at(i - oi, j - oj, k - ok)) : T(1.e10);
+  }
+};
+
+template <typename T>
+struct OpEno1 {  // lsm_spatial_derivatives3d_local.f.in:290-300
+  Field<T> plus, minus;
+  Field<const T> d1;
+  int di, dj, dk;
+  T inv;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+    if (!m) return;
+    plus.at(i, j, k) = d1.at(i + di, j + dj, k + dk) * inv;
+    minus.at(i, j, k) = d1.at(i, j, k) * inv;
+  }
+};
+
+template <typename T>
+struct OpEno2 {  // lsm_spatial_derivatives3d_local.f.in:571-601
+  Field<T> plus, minus;
+  Field<const T> d1, d2;
+  int di, dj, dk;
+  T inv;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    if (!m) return;
+    const T h = T(0.5);
+    const T d2m = d2.at(i - di, j - dj, k - dk), d20 = d2.at(i, j, k), d2p = d2.at(i + di, j + dj, k + dk);
+    const T d1p = d1.at(i + di, j + dj, k + dk), d10 = d1.at(i, j, k);
+    plus.at(i, j, k) = (fabs(d20) < fabs(d2p)) ? (d1p - h * d20) * inv : (d1p - h * d2p) * inv;
+    minus.at(i, j, k) = (fabs(d2m) < fabs(d20)) ? (d10 + h * d2m) * inv : (d10 + h * d20) * inv;
+  }
+};
+
+template <typename T>
+struct OpCentral2 {  // lsm_spatial_derivatives3d_local.f.in:826-840
+  Field<T> gx, gy, gz;
+  Field<const T> phi;
+  T fx, fy, fz;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    if (!m) return;
+    gx.at(i, j, k) = (phi.at(i + 1, j, k) - phi.at(i - 1, j, k)) * fx;
+    gy.at(i, j, k) = (phi.at(i, j + 1, k) - phi.at(i, j - 1, k)) * fy;
+    gz.at(i, j, k) = (phi.at(i, j, k + 1) - phi.at(i, j, k - 1)) * fz;
+  }
+};
+
+template <typename T>
+struct OpCentral4 {  // lsm_spatial_derivatives3d_local.f.in:936-955
+  Field<T> gx, gy, gz;
+  Field<const T> phi;
+  T fx, fy, fz;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    if (!m) return;
+    const T e = T(8);
+    gx.at(i, j, k) = (((-phi.at(i + 2, j, k) + e * phi.at(i + 1, j, k)) + phi.at(i - 2, j, k)) -
+                      e * phi.at(i - 1, j, k)) * fx;
+    gy.at(i, j, k) = (((-phi.at(i, j + 2, k) + e * phi.at(i, j + 1, k)) + phi.at(i, j - 2, k)) -
+                      e * phi.at(i, j - 1, k)) * fy;
+    gz.at(i, j, k) = (((-phi.at(i, j, k + 2) + e * phi.at(i, j, k + 1)) + phi.at(i, j, k - 2)) -
+                      e * phi.at(i, j, k - 1)) * fz;
+  }
+};
+
+template <typename T>
+struct OpLaplacian {  // lsm_spatial_derivatives3d_local.f.in:1040-1055
+  Field<T> lap;
+  Field<const T> phi;
+  T ix2, iy2, iz2;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    if (!m) return;
+    const T c = T(2) * phi.at(i, j, k);
+    lap.at(i, j, k) = (ix2 * ((phi.at(i + 1, j, k) - c) + phi.at(i - 1, j, k)) +
+                       iy2 * ((phi.at(i, j + 1, k) - c) + phi.at(i, j - 1, k))) +
+                      iz2 * ((phi.at(i, j, k + 1) - c) + phi.at(i, j, k - 1));
+  }
+};
+
+template <typename T>
+struct OpGradMag {  // lsm_spatial_derivatives3d_local.f.in:1232-1258 (tolerance LSMLIB_ZERO_TOL)
+  Field<T> mag;
+  Field<const T> gx, gy, gz;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    if (!m) return;
+    const T tol = Tol<T>::zero, z = T(0);
+    const T a = gx.at(i, j, k), b = gy.at(i, j, k), c = gz.at(i, j, k);
+    T x2 = a * a, y2 = b * b, z2 = c * c;
+    if (x2 < tol) x2 = z;
+    if (y2 < tol) y2 = z;
+    if (z2 < tol) z2 = z;
+    const T t = sqrt((x2 + y2) + z2);
+    mag.at(i, j, k) = (t < tol) ? z : t;
+  }
+};
+
+template <typename T>
+struct OpAdvection {  // lsm_level_set_evolution3d_local.f.in:130-142
+  Field<T> rhs;
+  Field<const T> px, py, pz, vx, vy, vz;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    if (!m) return;
+    rhs.at(i, j, k) = rhs.at(i, j, k) - ((vx.at(i, j, k) * px.at(i, j, k) + vy.at(i, j, k) * py.at(i, j, k)) +
+                                         vz.at(i, j, k) * pz.at(i, j, k));
+  }
+};
+
+template <typename T>
+struct OpNormalVel {  // :236-272 (field vn) and :357-392 (constant vn: vnf.p == nullptr)
+  Field<T> rhs;
+  Field<const T> xp, yp, zp, xm, ym, zm, vnf;
+  T vn_const;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    if (!m) return;
+    const T vn = vnf.p ? vnf.at(i, j, k) : vn_const;
+    if (fabs(vn) >= Tol<T>::zero) {
+      const T n2 = godunov_norm_sq<T>(vn, xp.at(i, j, k), yp.at(i, j, k), zp.at(i, j, k), xm.at(i, j, k),
+                                      ym.at(i, j, k), zm.at(i, j, k));
+      rhs.at(i, j, k) = rhs.at(i, j, k) - vn * sqrt(n2);
+    }
+  }
+};
+
+template <typename T>
+struct OpConstCurv {  // :490-518
+  Field<T> rhs;
+  Field<const T> px, py, pz, pxx, pxy, pxz, pyy, pyz, pzz;
+  T b;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    if (!m) return;
+    const T curv = curvature_term<T>(px.at(i, j, k), py.at(i, j, k), pz.at(i, j, k), pxx.at(i, j, k),
+                                     pxy.at(i, j, k), pxz.at(i, j, k), pyy.at(i, j, k), pyz.at(i, j, k),
+                                     pzz.at(i, j, k), Tol<T>::zero);
+    rhs.at(i, j, k) = rhs.at(i, j, k) + b * curv;
+  }
+};
+
+template <typename T>
+struct OpPrecomputedCurv {  // :593-601
+  Field<T> rhs;
+  Field<const T> kappa, gmag;
+  T b;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    if (!m) return;
+    rhs.at(i, j, k) = rhs.at(i, j, k) + (b * kappa.at(i, j, k)) * gmag.at(i, j, k);
+  }
+};
+
+template <typename T>
+struct OpExtNormal {  // :712-800
+  Field<T> rhs;
+  Field<const T> xp, yp, zp, xm, ym, zm, vn, vx, vy, vz;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    if (!m) return;
+    const T vnc = vn.at(i, j, k), vxc = vx.at(i, j, k), vyc = vy.at(i, j, k), vzc = vz.at(i, j, k);
+    const T px = ext_normal_select<T>(vxc, vnc, xp.at(i, j, k), xm.at(i, j, k));
+    const T py = ext_normal_select<T>(vyc, vnc, yp.at(i, j, k), ym.at(i, j, k));
+    const T pz = ext_normal_select<T>(vzc, vnc, zp.at(i, j, k), zm.at(i, j, k));
+    const T n2 = (px * px + py * py) + pz * pz;
+    T r = ((rhs.at(i, j, k) - vxc * px) - vyc * py) - vzc * pz;
+    if (fabs(vnc) >= Tol<T>::zero && n2 >= Tol<T>::zero) r = r - vnc * sqrt(n2);
+    rhs.at(i, j, k) = r;
+  }
+};
+
+template <typename T>
+struct OpReinit {  // lsm_reinitialization3d_local.f.in:50-320
+  Field<T> rhs;
+  Field<const T> phi, phi0, xp, yp, zp, xm, ym, zm;
+  T dx_sq;
+  int use_phi0;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    if (!m) return;
+    const T cur = use_phi0 ? phi0.at(i, j, k) : phi.at(i, j, k);
+    T r = T(0);
+    if (fabs(cur) >= Tol<T>::zero) {
+      const T n2 = godunov_star_sq<T>(cur, xp.at(i, j, k), yp.at(i, j, k), zp.at(i, j, k), xm.at(i, j, k),
+                                      ym.at(i, j, k), zm.at(i, j, k));
+      const T sgn = use_phi0 ? cur / sqrt(cur * cur + dx_sq) : cur / sqrt(cur * cur + n2 * dx_sq);
+      r = sgn * (T(1) - sqrt(n2));
+    }
+    rhs.at(i, j, k) = r;
+  }
+};
+
+template <typename T>
+struct OpOrtho {  // lsm_reinitialization3d_local.f.in:326-470
+  Field<T> rhs;
+  Field<const T> xp, yp, zp, xm, ym, zm, psi, sxp, syp, szp, sxm, sym, szm;
+  T dx_sq;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+#pragma clang fp contract(off)
+    if (!m) return;
+    const T g0 = T(0.5) * (sxp.at(i, j, k) + sxm.at(i, j, k));
+    const T g1 = T(0.5) * (syp.at(i, j, k) + sym.at(i, j, k));
+    const T g2 = T(0.5) * (szp.at(i, j, k) + szm.at(i, j, k));
+    const T ng = sqrt((g0 * g0 + g1 * g1) + g2 * g2);
+    const T ps = psi.at(i, j, k);
+    T r = T(0);
+    if (fabs(ps) > Tol<T>::zero && ng > Tol<T>::zero) {
+      const T s = ps / sqrt(ps * ps + dx_sq * (ng * ng));
+      const T z = T(0);
+      T acc = fmax(s * g0, z) * xm.at(i, j, k);
+      acc = acc + fmin(s * g0, z) * xp.at(i, j, k);
+      acc = acc + fmax(s * g1, z) * ym.at(i, j, k);
+      acc = acc + fmin(s * g1, z) * yp.at(i, j, k);
+      acc = acc + fmax(s * g2, z) * zm.at(i, j, k);
+      acc = acc + fmin(s * g2, z) * zp.at(i, j, k);
+      r = T(-1) / ng * acc;
+    }
+    rhs.at(i, j, k) = r;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Reductions over list entries: a max or a sum, and the count of included
+// entries; FP64 partials per block, then one block combines them.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RedMaxDiff {  // lsm_utilities3d_local.f.in:16-92
+  Field<const T> f1, f2;
+  __device__ bool operator()(int i, int j, int k, bool m, double& v) const {
+    if (!m) return false;
+    v = (double)fabs(f1.at(i, j, k) - f2.at(i, j, k));
+    return true;
+  }
+};
+
+template <typename T>
+struct RedAveGrad {  // lsm_spatial_derivatives3d_local.f.in:1140-1170
+  Field<const T> phi;
+  T fx, fy, fz;
+  __device__ bool operator()(int i, int j, int k, bool m, double& v) const {
+#pragma clang fp contract(off)
+    if (!m) return false;
+    const T x = (phi.at(i + 1, j, k) - phi.at(i - 1, j, k)) * fx;
+    const T y = (phi.at(i, j + 1, k) - phi.at(i, j - 1, k)) * fy;
+    const T z = (phi.at(i, j, k + 1) - phi.at(i, j, k - 1)) * fz;
+    v = (double)sqrt((x * x + y * y) + z * z);
+    return true;
+  }
+};
+
+template <typename T>
+struct RedAdvDt {  // lsm_utilities3d_local.f.in:262-275
+  Field<const T> vx, vy, vz;
+  T ix, iy, iz;
+  __device__ bool operator()(int i, int j, int k, bool m, double& v) const {
+#pragma clang fp contract(off)
+    if (!m) return false;
+    v = (double)((fabs(vx.at(i, j, k)) * ix + fabs(vy.at(i, j, k)) * iy) + fabs(vz.at(i, j, k)) * iz);
+    return true;
+  }
+};
+
+// Normal-velocity CFL.  Field vn (cv optional): lsm_utilities3d_local.f.in:
+// 390-420 and :540-615, taken where the smoothed |grad phi| exceeds
+// LSMLIB_ZERO_TOL.  Constant vn (vnf.p == nullptr): the box routine's H/dX,
+// the formula the reference's :628-752 intends (it reads two variables it
+// never assigns).
+template <typename T>
+struct RedNormalDt {
+  Field<const T> xp, yp, zp, xm, ym, zm, vnf, cv;
+  T avn_const, mdx2, ix, iy, iz;
+  int cv_sgn;
+  __device__ bool operator()(int i, int j, int k, bool m, double& v) const {
+#pragma clang fp contract(off)
+    if (!m) return false;
+    if (cv.p) {
+      const T c = cv.at(i, j, k);
+      if (!((cv_sgn > 0) ? (c > T(0)) : (c < T(0)))) return false;
+    }
+    const T a = xp.at(i, j, k), b = yp.at(i, j, k), c = zp.at(i, j, k);
+    const T d = xm.at(i, j, k), e = ym.at(i, j, k), f = zm.at(i, j, k);
+    if (vnf.p) {
+      const T px = fmax(fabs(a), fabs(d)), py = fmax(fabs(b), fabs(e)), pz = fmax(fabs(c), fabs(f));
+      const T ng = sqrt(((px * px + py * py) + pz * pz) + mdx2);
+      if (!(ng > Tol<T>::zero)) return false;
+      v = (double)(fabs(vnf.at(i, j, k)) / ng * ((px * ix + py * iy) + pz * iz));
+    } else {
+      v = (double)h_over_dx<T>(avn_const, a, b, c, d, e, f, mdx2, ix, iy, iz);
+    }
+    return true;
+  }
+};
+
+enum { RED_LIST_MAX = 0, RED_LIST_SUM = 1 };
+
+__device__ __forceinline__ void block_reduce2(int op, double& v, double& c) {
+  __shared__ double s_v[4], s_c[4];
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_xor(v, o, 64);
+    v = (op == RED_LIST_MAX) ? fmax(v, w) : v + w;
+    c += __shfl_xor(c, o, 64);
+  }
+  const int t = threadIdx.x;
+  if ((t & 63) == 0) {
+    s_v[t >> 6] = v;
+    s_c[t >> 6] = c;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int w = 1; w < (int)(blockDim.x / 64); w++) {
+      v = (op == RED_LIST_MAX) ? fmax(v, s_v[w]) : v + s_v[w];
+      c += s_c[w];
+    }
+  }
+}
+
+template <typename Op>
+__global__ void __launch_bounds__(256) k_entries_reduce(Entries e, Op op, int red, double* partials) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  double v = (red == RED_LIST_MAX) ? -1.0e308 : 0.0, c = 0.0;
+  if (t < e.n) {
+    const int i = e.ix[t], j = e.iy[t], k = e.iz[t];
+    bool marked = false;
+    if (!entry_ok(e, i, j, k, marked)) {
+      atomicAdd(e.skipped, 1);
+    } else {
+      double x;
+      if (op(i, j, k, marked, x)) {
+        v = x;
+        c = 1.0;
+      }
+    }
+  }
+  block_reduce2(red, v, c);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = v;
+    partials[2 * blockIdx.x + 1] = c;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_entries_reduce_final(const double* partials, int nblocks, int red,
+                                                              double* out) {
+  double v = (red == RED_LIST_MAX) ? -1.0e308 : 0.0, c = 0.0;
+  for (int q = threadIdx.x; q < nblocks; q += blockDim.x) {
+    const double w = partials[2 * q];
+    v = (red == RED_LIST_MAX) ? fmax(v, w) : v + w;
+    c += partials[2 * q + 1];
+  }
+  block_reduce2(red, v, c);
+  if (threadIdx.x == 0) {
+    out[0] = v;
+    out[1] = c;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Band building
+// ---------------------------------------------------------------------------
+constexpr int kChunkRounds = 8;
+constexpr int kChunk = 256 * kChunkRounds;  // cells (or pairs) per block
+
+enum { BAND_PHI = 0, BAND_MASK_NONNEG = 1, BAND_MASK_NONPOS = 2 };
+
+// lsm_localization3d.f:232-255 and :428-455
+template <typename T>
+__device__ __forceinline__ void band_flags(int mode, T v, T width, T width_inner, bool& in, bool& om, bool& op) {
+  om = op = false;
+  if (mode == BAND_PHI) {
+    const T a = fabs(v);
+    in = a < width;
+    const bool outer = in && a >= width_inner;
+    om = outer && (v <= T(0));
+    op = outer && !(v <= T(0));
+  } else if (mode == BAND_MASK_NONNEG) {
+    in = v >= T(0);
+  } else {
+    in = v <= T(0);
+  }
+}
+
+struct CellMap {  // linear cell index of a box, Fortran order, and back
+  Box b;
+  int nx, nxy;
+  __device__ __forceinline__ void ijk(int c, int& i, int& j, int& k) const {
+    const int kk = c / nxy;
+    const int r = c - kk * nxy;
+    const int jj = r / nx;
+    i = b.ilo + (r - jj * nx);
+    j = b.jlo + jj;
+    k = b.klo + kk;
+  }
+  __device__ __forceinline__ int lin(int i, int j, int k) const {
+    return (i - b.ilo) + nx * (j - b.jlo) + nxy * (k - b.klo);
+  }
+};
+
+// Block-wide counts of up to three flags; the wave sums go through LDS.
+__device__ __forceinline__ void block_sum3(int& a, int& b, int& c) {
+  __shared__ int s[3][4];
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+    c += __shfl_xor(c, o, 64);
+  }
+  const int t = threadIdx.x;
+  if ((t & 63) == 0) {
+    s[0][t >> 6] = a;
+    s[1][t >> 6] = b;
+    s[2][t >> 6] = c;
+  }
+  __syncthreads();
+  a = s[0][0] + s[0][1] + s[0][2] + s[0][3];
+  b = s[1][0] + s[1][1] + s[1][2] + s[1][3];
+  c = s[2][0] + s[2][1] + s[2][2] + s[2][3];
+}
+
+// Exclusive in-block ranks of three flags for one round of 256 items, in
+// item order; `tot` returns the round's totals.  Ends with a barrier, so the
+// LDS it uses can be reused by the next round.
+__device__ __forceinline__ void block_rank3(const bool f[3], int rank[3], int tot[3]) {
+  __shared__ int s[3][4];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  unsigned long long bal[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    bal[q] = __ballot(f[q]);
+    if (lane == 0) s[q][w] = __popcll(bal[q]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    int before = 0;
+    for (int x = 0; x < w; x++) before += s[q][x];
+    rank[q] = before + __popcll(bal[q] & lt);
+    tot[q] = s[q][0] + s[q][1] + s[q][2] + s[q][3];
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_band_count(Field<const T> f, CellMap cm, int ncell, int mode, T width, T width_inner, int* counts) {
+  const int base = (int)blockIdx.x * kChunk;
+  int a = 0, b = 0, c = 0;
+  for (int r = 0; r < kChunkRounds; r++) {
+    const int x = base + r * 256 + (int)threadIdx.x;
+    if (x < ncell) {
+      int i, j, k;
+      cm.ijk(x, i, j, k);
+      bool in, om, op;
+      band_flags<T>(mode, f.at(i, j, k), width, width_inner, in, om, op);
+      a += in;
+      b += om;
+      c += op;
+    }
+  }
+  block_sum3(a, b, c);
+  if (threadIdx.x == 0) {
+    counts[3 * blockIdx.x] = a;
+    counts[3 * blockIdx.x + 1] = b;
+    counts[3 * blockIdx.x + 2] = c;
+  }
+}
+
+// Exclusive scan, in place, of `nb` rows of three counts (one block of 1024
+// threads); totals[0..2] receive the sums.
+__global__ void __launch_bounds__(1024) k_scan3(int* counts, int nb, int* totals) {
+  __shared__ int s[3][1024];
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int lo = t * per, hi = min(nb, lo + per);
+  int sum[3] = {0, 0, 0};
+  for (int r = lo; r < hi; r++)
+    for (int q = 0; q < 3; q++) sum[q] += counts[3 * r + q];
+  for (int q = 0; q < 3; q++) s[q][t] = sum[q];
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan of the thread sums
+    int v[3];
+    for (int q = 0; q < 3; q++) v[q] = (t >= off) ? s[q][t - off] : 0;
+    __syncthreads();
+    for (int q = 0; q < 3; q++) s[q][t] += v[q];
+    __syncthreads();
+  }
+  int run[3];
+  for (int q = 0; q < 3; q++) run[q] = s[q][t] - sum[q];
+  for (int r = lo; r < hi; r++)
+    for (int q = 0; q < 3; q++) {
+      const int v = counts[3 * r + q];
+      counts[3 * r + q] = run[q];
+      run[q] += v;
+    }
+  if (t == 1023)
+    for (int q = 0; q < 3; q++) totals[q] = s[q][1023];
+}
+
+// Writes level 0: narrow_band over phi's box (1 in the band, 0 elsewhere),
+// the band points in Fortran order, and the outer-layer entries (minus side
+// ascending from outer[0], plus side descending from outer[plus_top]).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_band_write(Field<const T> f, CellMap cm, int ncell, int mode, T width, T width_inner, const int* offsets,
+                 Field<nb_t> nb, int* ix, int* iy, int* iz, int list_base, int* outer, int plus_top) {
+  const int base = (int)blockIdx.x * kChunk;
+  int o[3] = {offsets[3 * blockIdx.x], offsets[3 * blockIdx.x + 1], offsets[3 * blockIdx.x + 2]};
+  for (int r = 0; r < kChunkRounds; r++) {
+    const int x = base + r * 256 + (int)threadIdx.x;
+    bool fl[3] = {false, false, false};
+    int i = 0, j = 0, k = 0;
+    if (x < ncell) {
+      cm.ijk(x, i, j, k);
+      band_flags<T>(mode, f.at(i, j, k), width, width_inner, fl[0], fl[1], fl[2]);
+      nb.at(i, j, k) = fl[0] ? nb_t(1) : nb_t(0);
+    }
+    int rank[3], tot[3];
+    block_rank3(fl, rank, tot);
+    if (fl[0]) {
+      const int p = o[0] + rank[0];
+      ix[p] = i;
+      iy[p] = j;
+      iz[p] = k;
+      if (fl[1]) outer[o[1] + rank[1]] = list_base + p;
+      if (fl[2]) outer[plus_top - (o[2] + rank[2])] = list_base + p;
+    }
+    for (int q = 0; q < 3; q++) o[q] += tot[q];
+  }
+}
+
+// Neighbour q (0..5: i+1, i-1, j+1, j-1, k+1, k-1) of (i, j, k) inside the
+// narrow-band box, lsm_localization3d.f:96-155.
+__device__ __forceinline__ bool neighbour(int q, int i, int j, int k, const Box& b, int& a, int& bb, int& c) {
+  a = i;
+  bb = j;
+  c = k;
+  switch (q) {
+    case 0: if (i >= b.ihi) return false; a = i + 1; break;
+    case 1: if (i <= b.ilo) return false; a = i - 1; break;
+    case 2: if (j >= b.jhi) return false; bb = j + 1; break;
+    case 3: if (j <= b.jlo) return false; bb = j - 1; break;
+    case 4: if (k >= b.khi) return false; c = k + 1; break;
+    default: if (k <= b.klo) return false; c = k - 1; break;
+  }
+  return true;
+}
+
+struct Parents {
+  const int* ix;
+  const int* iy;
+  const int* iz;  // entry 0 = the level's first point
+  int npairs;     // 6 per parent
+};
+
+__global__ void __launch_bounds__(256) k_nbr_bid(Parents P, Field<const nb_t> nb, CellMap cm, int* key) {
+  const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (p >= P.npairs) return;
+  const int m = p / 6, q = p - 6 * m;
+  int a, b, c;
+  if (neighbour(q, P.ix[m], P.iy[m], P.iz[m], cm.b, a, b, c) && nb.at(a, b, c) == 0)
+    atomicMin(&key[cm.lin(a, b, c)], p);
+}
+
+__global__ void __launch_bounds__(256)
+    k_nbr_count(Parents P, Field<const nb_t> nb, CellMap cm, const int* key, unsigned char* win, int* counts) {
+  const int base = (int)blockIdx.x * kChunk;
+  int n = 0, z0 = 0, z1 = 0;
+  for (int r = 0; r < kChunkRounds; r++) {
+    const int p = base + r * 256 + (int)threadIdx.x;
+    if (p < P.npairs) {
+      const int m = p / 6, q = p - 6 * m;
+      int a, b, c;
+      const bool w = neighbour(q, P.ix[m], P.iy[m], P.iz[m], cm.b, a, b, c) && nb.at(a, b, c) == 0 &&
+                     key[cm.lin(a, b, c)] == p;
+      win[p] = w ? 1 : 0;
+      n += w;
+    }
+  }
+  block_sum3(n, z0, z1);
+  if (threadIdx.x == 0) {
+    counts[3 * blockIdx.x] = n;
+    counts[3 * blockIdx.x + 1] = 0;
+    counts[3 * blockIdx.x + 2] = 0;
+  }
+}
+
+// Appends the winners in pair order, marks them, and clears their keys.
+// write == 0 only clears the keys (the list is full: the call fails).
+__global__ void __launch_bounds__(256)
+    k_nbr_write(Parents P, Field<nb_t> nb, CellMap cm, int* key, const unsigned char* win, const int* offsets,
+                int* ox, int* oy, int* oz, nb_t mark, int write) {
+  const int base = (int)blockIdx.x * kChunk;
+  int o = offsets[3 * blockIdx.x];
+  for (int r = 0; r < kChunkRounds; r++) {
+    const int p = base + r * 256 + (int)threadIdx.x;
+    bool fl[3] = {false, false, false};
+    int a = 0, b = 0, c = 0;
+    if (p < P.npairs && win[p]) {
+      const int m = p / 6, q = p - 6 * m;
+      neighbour(q, P.ix[m], P.iy[m], P.iz[m], cm.b, a, b, c);
+      fl[0] = true;
+    }
+    int rank[3], tot[3];
+    block_rank3(fl, rank, tot);
+    if (fl[0]) {
+      key[cm.lin(a, b, c)] = INT_MAX;
+      if (write) {
+        const int pos = o + rank[0];
+        ox[pos] = a;
+        oy[pos] = b;
+        oz[pos] = c;
+        nb.at(a, b, c) = mark;
+      }
+    }
+    o += tot[0];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fill_int(int* p, int64_t n, int v) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+// lsm_localization3d.f:303-378: the six fill-box face planes across the box
+// of narrow_band.  blockIdx.z: face (0..5).
+__global__ void __launch_bounds__(256) k_mark_layer(Field<nb_t> nb, Box gb, Box fb, nb_t mark) {
+  const int face = blockIdx.z, d = face / 2;
+  const int plane = (face & 1) ? (d == 0 ? fb.ihi : d == 1 ? fb.jhi : fb.khi)
+                               : (d == 0 ? fb.ilo : d == 1 ? fb.jlo : fb.klo);
+  const int u = (int)(blockIdx.x * 64 + threadIdx.x % 64), v = (int)(blockIdx.y * 4 + threadIdx.x / 64);
+  int i, j, k;
+  if (d == 0) { i = plane; j = gb.jlo + u; k = gb.klo + v; }
+  else if (d == 1) { i = gb.ilo + u; j = plane; k = gb.klo + v; }
+  else { i = gb.ilo + u; j = gb.jlo + v; k = plane; }
+  if (inside(gb, i, j, k)) nb.at(i, j, k) = mark;
+}
+
+// lsm_localization3d.f:568-654: bit 0 / 1 = a point with phi > 0 / <= 0 seen
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_check_outer(const int* outer, int n, const int* ix, const int* iy, const int* iz, int nlo, int nhi,
+                  Field<const T> phi, Box gphi, int shift, int* bits, int* skipped) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= n) return;
+  const int l = outer[t];
+  if (l < nlo || l > nhi) {
+    atomicAdd(skipped, 1);
+    return;
+  }
+  const int i = ix[l - nlo], j = iy[l - nlo], k = iz[l - nlo];
+  if (!inside(gphi, i, j, k)) {
+    atomicAdd(skipped, 1);
+    return;
+  }
+  atomicOr(bits, (phi.at(i, j, k) > T(0) ? 1 : 2) << shift);
+}
+
+}  // namespace lsm
+
+// ============================================================================
+// Host side
+// ============================================================================
+namespace {
+
+using namespace lsm;
+typedef real_t R;
+
+#define BOXP(p)                                                                                \
+  const int *ilo_##p, const int *ihi_##p, const int *jlo_##p, const int *jhi_##p, const int *klo_##p, \
+      const int *khi_##p
+#define BOX(p) lsm::box_of(ilo_##p, ihi_##p, jlo_##p, jhi_##p, klo_##p, khi_##p)
+
+Box shrink(Box b, int hx, int hy, int hz) {
+  return Box{b.ilo + hx, b.ihi - hx, b.jlo + hy, b.jhi - hy, b.klo + hz, b.khi - hz};
+}
+Box shrink_dir(Box b, int d, int lo, int hi) {
+  if (d == 0) { b.ilo += lo; b.ihi -= hi; }
+  else if (d == 1) { b.jlo += lo; b.jhi -= hi; }
+  else { b.klo += lo; b.khi -= hi; }
+  return b;
+}
+Box isect(Box a, const Box& b) {
+  a.ilo = std::max(a.ilo, b.ilo); a.ihi = std::min(a.ihi, b.ihi);
+  a.jlo = std::max(a.jlo, b.jlo); a.jhi = std::min(a.jhi, b.jhi);
+  a.klo = std::max(a.klo, b.klo); a.khi = std::min(a.khi, b.khi);
+  return a;
+}
+
+Field<const R> none() {
+  Field<const R> f;
+  f.p = nullptr;
+  f.ilo = f.jlo = f.klo = 0;
+  f.sy = f.sz = 0;
+  return f;
+}
+
+int* skipped_counter() {
+  bool fresh = false;
+  int* p = reinterpret_cast<int*>(device_buffer(BUF_LOCAL_FLAGS, 64, &fresh));
+  if (p && fresh) {
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, current_stream(), p, (int64_t)16, 0);
+  }
+  return p;
+}
+
+// The list arguments of one call, staged once with the largest extent used.
+struct ListArgs {
+  const int* ix = nullptr;
+  const int* iy = nullptr;
+  const int* iz = nullptr;
+  int base = 0;  // the nlo the arrays are declared from
+  bool ok = false;
+};
+
+ListArgs stage_lists(Call& c, const int* ix, const int* iy, const int* iz, int base, int last) {
+  ListArgs L;
+  L.base = base;
+  if (last < base) return L;  // nothing is visited
+  const int64_t n = (int64_t)last - base + 1;
+  L.ix = c.in_buf(ix, n);
+  L.iy = c.in_buf(iy, n);
+  L.iz = c.in_buf(iz, n);
+  L.ok = c.ok();
+  return L;
+}
+
+// The mark test of a launch: narrow_band on its box (nb.p == nullptr: none)
+// and the mark; `any` is what every entry touches besides narrow_band.
+struct Marks {
+  Field<const nb_t> nb;
+  Box gnb;
+  int mark;
+};
+
+Marks no_marks() {
+  Marks m;
+  m.nb.p = nullptr;
+  m.nb.ilo = m.nb.jlo = m.nb.klo = 0;
+  m.nb.sy = m.nb.sz = 0;
+  m.gnb = Box{INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX};
+  m.mark = 0;
+  return m;
+}
+
+Entries make_entries(const ListArgs& L, int lo, int hi, const Marks& mk, Box valid, Box any) {
+  Entries e;
+  e.ix = L.ix + (lo - L.base);
+  e.iy = L.iy + (lo - L.base);
+  e.iz = L.iz + (lo - L.base);
+  e.n = hi - lo + 1;
+  e.nb = mk.nb;
+  e.mark = mk.mark;
+  e.any = isect(any, mk.gnb);
+  e.valid = isect(valid, e.any);
+  e.skipped = skipped_counter();
+  return e;
+}
+
+Box everywhere() { return Box{INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX}; }
+
+// Launches op over entries lo..hi of the staged lists.  Unmarked entries
+// need only lie in `any` (and narrow_band's box); marked ones in `valid`.
+template <typename Op>
+void run_entries(Call& c, const char* kname, const ListArgs& L, int lo, int hi, const Marks& mk, Box valid, Op op,
+                 Box any = everywhere()) {
+  if (!c.ok() || !L.ok || hi < lo) return;
+  if (mk.nb.p == nullptr) any = isect(any, valid);
+  const int n = hi - lo + 1;
+  Entries e = make_entries(L, lo, hi, mk, valid, any);
+  if (!c.ok()) return;
+  hipLaunchKernelGGL(k_entries<Op>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream(), e, op);
+  c.launched(kname);
+}
+
+// Runs a list reduction; returns {value, count} (FP64).
+template <typename Op>
+void run_entries_reduce(Call& c, const char* kname, const ListArgs& L, int lo, int hi, const Marks& mk, Box valid,
+                        Op op, int red, double out[2]) {
+  out[0] = (red == RED_LIST_MAX) ? -1.0e308 : 0.0;
+  out[1] = 0.0;
+  if (!c.ok() || !L.ok || hi < lo) return;
+  const int n = hi - lo + 1;
+  const int nblocks = (n + 255) / 256;
+  double* scratch = reinterpret_cast<double*>(device_scratch((size_t)(2 * nblocks + 2) * sizeof(double)));
+  Entries e = make_entries(L, lo, hi, mk, valid, mk.nb.p ? everywhere() : valid);
+  if (!c.ok()) return;
+  hipLaunchKernelGGL(k_entries_reduce<Op>, dim3((unsigned)nblocks), dim3(256), 0, c.stream(), e, op, red,
+                     scratch + 2);
+  c.launched(kname);
+  hipLaunchKernelGGL(k_entries_reduce_final, dim3(1), dim3(256), 0, c.stream(), scratch + 2, nblocks, red,
+                     scratch);
+  c.launched("k_entries_reduce_final");
+  check_hip(hipMemcpyAsync(out, scratch, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream()),
+            "hipMemcpyAsync(reduction result)");
+  check_hip(hipStreamSynchronize(c.stream()), "hipStreamSynchronize");
+}
+
+// Completes a call; in synchronous mode (or when `force`) reports skipped
+// list entries as an error.
+void finish_local(Call& c, bool force = false) {
+  int* sk = skipped_counter();
+  if (!c.ok() || !sk || !(force || sync_on_return())) {
+    c.finish();
+    return;
+  }
+  int h = 0;
+  check_hip(hipMemcpyAsync(&h, sk, sizeof(int), hipMemcpyDeviceToHost, c.stream()), "hipMemcpyAsync(flags)");
+  c.finish();
+  check_hip(hipStreamSynchronize(c.stream()), "hipStreamSynchronize");
+  if (h != 0) {
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, c.stream(), sk, (int64_t)1, 0);
+    (void)hipStreamSynchronize(c.stream());
+    raise_error(std::string(c.name()) + ": " + std::to_string(h) +
+                " list entries lie outside the arrays they index (skipped)");
+  }
+}
+
+Marks marks(Call& c, const unsigned char* nb, const Box& b, const unsigned char* mark) {
+  Marks m;
+  m.nb = c.in(reinterpret_cast<const nb_t*>(nb), b);
+  m.gnb = b;
+  m.mark = (int)(nb_t)*mark;
+  return m;
+}
+
+int blocks_for(int64_t n, int per) { return (int)((n + per - 1) / per); }
+
+// ---------------------------------------------------------------------------
+// Band builders
+// ---------------------------------------------------------------------------
+bool host_array(const char* fn, const void* p, const char* what) {
+  if (p == nullptr || is_device_pointer(p)) {
+    raise_error(std::string(fn) + ": " + what + " must be host memory");
+    return false;
+  }
+  return true;
+}
+
+// lsm3dMarkNarrowBandNeighbors (lsm_localization3d.f:48-168) on the staged
+// device lists; `count` is the next free entry, updated.
+void mark_neighbours(Call& c, const char* fn, Field<nb_t> nb, const Box& bn, int* dix, int* diy, int* diz,
+                     int nlo, int nhi, int* n_lo, int* n_hi, int level) {
+  const int64_t ncell = bn.cells();
+  bool fresh = false;
+  size_t got = 0;
+  int* key = reinterpret_cast<int*>(device_buffer(BUF_BAND_KEY, (size_t)ncell * sizeof(int), &fresh, &got));
+  if (!c.ok()) return;
+  if (fresh) {  // the whole buffer holds INT_MAX between calls
+    const int64_t nk = (int64_t)(got / sizeof(int));
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)blocks_for(nk, 256)), dim3(256), 0, c.stream(), key, nk, INT_MAX);
+  }
+  CellMap cm{bn, bn.nx(), bn.nx() * bn.ny()};
+  Field<const nb_t> nbc = make_field<const nb_t>(nb.p, bn);
+  int count = n_hi[0] + 1;
+  for (int l = 1; l <= level && c.ok(); l++) {
+    const int plo = n_lo[l - 1], phi_ = n_hi[l - 1];
+    int added = 0;
+    if (plo >= nlo && phi_ < count && phi_ >= plo) {
+      const int64_t npairs64 = 6 * ((int64_t)phi_ - plo + 1);
+      if (npairs64 > INT_MAX) {
+        raise_error(std::string(fn) + ": narrow-band level too large");
+        return;
+      }
+      const int npairs = (int)npairs64;
+      const int nb_blocks = blocks_for(npairs, kChunk);
+      unsigned char* work = reinterpret_cast<unsigned char*>(
+          device_buffer(BUF_BAND_WORK, (size_t)npairs + (size_t)(3 * nb_blocks + 4) * sizeof(int) + 256));
+      if (!c.ok()) return;
+      int* counts = reinterpret_cast<int*>(work);
+      int* totals = counts + 3 * nb_blocks;
+      unsigned char* win = reinterpret_cast<unsigned char*>(totals + 4);
+      Parents P{dix + (plo - nlo), diy + (plo - nlo), diz + (plo - nlo), npairs};
+      hipLaunchKernelGGL(k_nbr_bid, dim3((unsigned)blocks_for(npairs, 256)), dim3(256), 0, c.stream(), P, nbc, cm,
+                         key);
+      c.launched("k_nbr_bid");
+      hipLaunchKernelGGL(k_nbr_count, dim3((unsigned)nb_blocks), dim3(256), 0, c.stream(), P, nbc, cm, key, win,
+                         counts);
+      c.launched("k_nbr_count");
+      hipLaunchKernelGGL(k_scan3, dim3(1), dim3(1024), 0, c.stream(), counts, nb_blocks, totals);
+      c.launched("k_scan3");
+      int tot[3] = {0, 0, 0};
+      check_hip(hipMemcpyAsync(tot, totals, 3 * sizeof(int), hipMemcpyDeviceToHost, c.stream()),
+                "hipMemcpyAsync(level size)");
+      check_hip(hipStreamSynchronize(c.stream()), "hipStreamSynchronize");
+      if (!c.ok()) return;
+      const bool fits = (int64_t)count + tot[0] - 1 <= (int64_t)nhi;
+      hipLaunchKernelGGL(k_nbr_write, dim3((unsigned)nb_blocks), dim3(256), 0, c.stream(), P, nb, cm, key, win,
+                         counts, dix + (count - nlo), diy + (count - nlo), diz + (count - nlo), nb_t(l + 1),
+                         fits ? 1 : 0);
+      c.launched("k_nbr_write");
+      if (!fits) {
+        raise_error(std::string(fn) + ": the index arrays are too short for the narrow band");
+        return;
+      }
+      added = tot[0];
+    }
+    count += added;
+    if (count > n_hi[l - 1] + 1) {
+      n_lo[l] = n_hi[l - 1] + 1;
+      n_hi[l] = count - 1;
+    } else {
+      n_lo[l] = -1;
+      n_hi[l] = -1;
+    }
+  }
+}
+
+void determine_band(const char* fn, int mode, const R* phi, Box g, unsigned char* nb_arr, Box bn, int* ix,
+                    int* iy, int* iz, int nlo, int nhi, int* n_lo, int* n_hi, int* outer, int nlo_o, int nhi_o,
+                    int* nlo_plus, int* nhi_plus, int* nlo_minus, int* nhi_minus, R width, R width_inner,
+                    int level) {
+  Call c(fn);
+  if (!host_array(fn, n_lo, "n_lo") || !host_array(fn, n_hi, "n_hi")) return;
+  if (level < 0) {
+    raise_error(std::string(fn) + ": level must be >= 0");
+    return;
+  }
+  require_inside(fn, "narrow_band", bn, g, 0, 0, 0);
+  if (g.cells() <= 0 || g.cells() > INT_MAX || bn.cells() > INT_MAX) {
+    raise_error(std::string(fn) + ": box sizes must be in 1 .. 2^31-1 cells");
+    return;
+  }
+  const int64_t cap = (int64_t)nhi - nlo + 1;
+  auto fphi = c.in(phi, g);
+  Field<nb_t> fnb = c.out(reinterpret_cast<nb_t*>(nb_arr), bn);
+  int* dix = c.out_buf(ix, cap > 0 ? cap : 1);
+  int* diy = c.out_buf(iy, cap > 0 ? cap : 1);
+  int* diz = c.out_buf(iz, cap > 0 ? cap : 1);
+  const int64_t ocap = (int64_t)nhi_o - nlo_o + 1;
+  int* dout = nullptr;
+  if (mode == BAND_PHI) dout = c.out_buf(outer, ocap > 0 ? ocap : 1);
+  if (!c.ok()) return;
+
+  const int ncell = (int)g.cells();
+  const int nblk = blocks_for(ncell, kChunk);
+  int* work = reinterpret_cast<int*>(device_buffer(BUF_BAND_WORK, (size_t)(3 * nblk + 4) * sizeof(int) + 256));
+  if (!c.ok()) return;
+  int* totals = work + 3 * nblk;
+  CellMap cm{g, g.nx(), g.nx() * g.ny()};
+  hipLaunchKernelGGL(k_band_count<R>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), fphi, cm, ncell, mode, width,
+                     width_inner, work);
+  c.launched("k_band_count");
+  hipLaunchKernelGGL(k_scan3, dim3(1), dim3(1024), 0, c.stream(), work, nblk, totals);
+  c.launched("k_scan3");
+  int tot[3] = {0, 0, 0};
+  check_hip(hipMemcpyAsync(tot, totals, 3 * sizeof(int), hipMemcpyDeviceToHost, c.stream()),
+            "hipMemcpyAsync(band size)");
+  check_hip(hipStreamSynchronize(c.stream()), "hipStreamSynchronize");
+  if (!c.ok()) return;
+  if (tot[0] > cap) {
+    raise_error(std::string(fn) + ": the index arrays are too short for the narrow band");
+    return;
+  }
+  if (mode == BAND_PHI && (int64_t)tot[1] + tot[2] > ocap) {
+    raise_error(std::string(fn) + ": index_outer is too short for the outer layer");
+    return;
+  }
+  hipLaunchKernelGGL(k_band_write<R>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), fphi, cm, ncell, mode, width,
+                     width_inner, work, fnb, dix, diy, diz, nlo, dout, (int)(ocap - 1));
+  c.launched("k_band_write");
+  if (!c.ok()) return;
+
+  const int count = nlo + tot[0];
+  n_lo[0] = nlo;
+  if (mode == BAND_PHI) {
+    *nlo_minus = nlo_o;
+    *nhi_plus = nhi_o;
+    if (count > nlo) {
+      n_hi[0] = count - 1;
+      *nhi_minus = nlo_o + tot[1] - 1;
+      *nlo_plus = nhi_o - tot[2] + 1;
+      mark_neighbours(c, fn, fnb, bn, dix, diy, diz, nlo, nhi, n_lo, n_hi, level);
+    } else {
+      n_hi[0] = count;
+      *nhi_minus = nlo_o;
+      *nlo_plus = nhi_o;
+    }
+  } else {
+    n_hi[0] = count - 1;
+    mark_neighbours(c, fn, fnb, bn, dix, diy, diz, nlo, nhi, n_lo, n_hi, level);
+  }
+  c.finish();
+}
+
+// Shared entry for the TVD Runge-Kutta stages on the band.
+void rk_local(const char* fn, int form, R* out, Box go, const R* us, Box gs, const R* uc, Box gc, const R* rhs,
+              Box gr, R dt, const int* ix, const int* iy, const int* iz, int nlo, int nhi,
+              const unsigned char* nb, Box gnb, const unsigned char* mark) {
+  Call c(fn);
+  ListArgs L = stage_lists(c, ix, iy, iz, nlo, nhi);
+  if (L.ok) {
+    Box valid = isect(isect(isect(go, gc), gr), gnb);
+    if (us) valid = isect(valid, gs);
+    OpRk<R> op{form, c.out(out, go), us ? c.in(us, gs) : none(), c.in(uc, gc), c.in(rhs, gr), dt};
+    run_entries(c, "k_entries<rk>", L, nlo, nhi, marks(c, nb, gnb, mark), valid, op);
+  }
+  finish_local(c);
+}
+
+R max3(R a, R b, R c) {
+  R m = a;
+  if (b > m) m = b;
+  if (c > m) m = c;
+  return m;
+}
+
+// lsm3dHJENO1LOCAL / lsm3dHJENO2LOCAL: the D arrays are written as the
+// reference writes them (the last direction's differences remain).
+void hj_eno_local(const char* fn, int order, R* xp, R* yp, R* zp, Box gp, R* xm, R* ym, R* zm, Box gm,
+                  const R* phi, Box gphi, R* D1, Box g1, R* D2, Box g2, R dx, R dy, R dz, const int* ix,
+                  const int* iy, const int* iz, int nlo0, int nhi0, int nhi1, int nhi2, const unsigned char* nb,
+                  Box gnb, const unsigned char* mark_fb, const unsigned char* mark_d1,
+                  const unsigned char* mark_d2) {
+  Call c(fn);
+  const int d1_hi = (order == 1) ? nhi1 : nhi2;  // D1 over levels 0..1 (ENO1) or 0..2 (ENO2)
+  const int last = std::max(nhi0, std::max(d1_hi, order == 2 ? nhi1 : nhi0));
+  ListArgs L = stage_lists(c, ix, iy, iz, nlo0, last);
+  if (L.ok) {
+    Field<R> fxp = c.out(xp, gp), fyp = c.out(yp, gp), fzp = c.out(zp, gp);
+    Field<R> fxm = c.out(xm, gm), fym = c.out(ym, gm), fzm = c.out(zm, gm);
+    Field<const R> fphi = c.in(phi, gphi);
+    Field<R> f1 = c.out(D1, g1);
+    Field<R> f2 = (order == 2) ? c.out(D2, g2) : Field<R>{};
+    const Marks mfb = marks(c, nb, gnb, mark_fb), m1 = marks(c, nb, gnb, mark_d1);
+    const Marks m2 = (order == 2) ? marks(c, nb, gnb, mark_d2) : no_marks();
+    Field<const R> c1 = make_field<const R>(f1.p, g1), c2 = make_field<const R>(f2.p, g2);
+    Field<R> outp[3] = {fxp, fyp, fzp}, outm[3] = {fxm, fym, fzm};
+    const R inv[3] = {R(1.0 / (double)dx), R(1.0 / (double)dy), R(1.0 / (double)dz)};
+    for (int d = 0; d < 3 && c.ok(); d++) {
+      const int e[3] = {d == 0, d == 1, d == 2};
+      // D1(c) = phi(c) - phi(c - e_d) over levels 0..d1_hi (1e10 where unmarked)
+      run_entries(c, "k_entries<dn1>", L, nlo0, d1_hi, m1, isect(g1, shrink_dir(gphi, d, 1, 0)),
+                  OpDn<R>{f1, fphi, e[0], e[1], e[2], R(1)}, g1);
+      if (order == 2) {
+        // D2(c) = -(D1(c) - D1(c + e_d)) over levels 0..1
+        run_entries(c, "k_entries<dn2>", L, nlo0, nhi1, m2, isect(g2, shrink_dir(g1, d, 0, 1)),
+                    OpDn<R>{f2, c1, -e[0], -e[1], -e[2], R(-1)}, g2);
+        Box v = isect(isect(isect(gp, gm), shrink_dir(g2, d, 1, 1)), shrink_dir(g1, d, 0, 1));
+        run_entries(c, "k_entries<eno2>", L, nlo0, nhi0, mfb, v,
+                    OpEno2<R>{outp[d], outm[d], c1, c2, e[0], e[1], e[2], inv[d]});
+      } else {
+        Box v = isect(isect(gp, gm), shrink_dir(g1, d, 0, 1));
+        run_entries(c, "k_entries<eno1>", L, nlo0, nhi0, mfb, v,
+                    OpEno1<R>{outp[d], outm[d], c1, e[0], e[1], e[2], inv[d]});
+      }
+    }
+  }
+  finish_local(c);
+}
+
+}  // namespace
+
+// ============================================================================
+// Entry points (extern "C", reference prototypes; see include/lsm3d_amd.h)
+// ============================================================================
+extern "C" {
+
+// ---- narrow band: lsm_localization3d.h --------------------------------------
+void lsm3ddeterminenarrowband_(const R* phi, BOXP(g), unsigned char* nb, BOXP(nb), int* ix, int* iy, int* iz,
+                               const int* nlo_index, const int* nhi_index, int* n_lo, int* n_hi, int* index_outer,
+                               const int* nlo_index_outer, const int* nhi_index_outer, int* nlo_outer_plus,
+                               int* nhi_outer_plus, int* nlo_outer_minus, int* nhi_outer_minus, const R* width,
+                               const R* width_inner, const int* level) {
+  determine_band("LSM3D_DETERMINE_NARROW_BAND", BAND_PHI, phi, BOX(g), nb, BOX(nb), ix, iy, iz, *nlo_index,
+                 *nhi_index, n_lo, n_hi, index_outer, *nlo_index_outer, *nhi_index_outer, nlo_outer_plus,
+                 nhi_outer_plus, nlo_outer_minus, nhi_outer_minus, *width, *width_inner, *level);
+}
+
+void lsm3ddeterminenarrowbandfrommask_(const R* mask, BOXP(m), unsigned char* nb, BOXP(nb), int* ix, int* iy,
+                                       int* iz, int* nlo_index, int* nhi_index, int* n_lo, int* n_hi,
+                                       const int* level, const int* use_mask_sign) {
+  determine_band("LSM3D_DETERMINE_NARROW_BAND_FROM_MASK",
+                 *use_mask_sign > 0 ? BAND_MASK_NONNEG : BAND_MASK_NONPOS, mask, BOX(m), nb, BOX(nb), ix, iy, iz,
+                 *nlo_index, *nhi_index, n_lo, n_hi, nullptr, 0, -1, nullptr, nullptr, nullptr, nullptr, R(0),
+                 R(0), *level);
+}
+
+void lsm3dmarknarrowbandboundarylayer_(unsigned char* nb, BOXP(gb), BOXP(fb), const unsigned char* mark) {
+  const char* fn = "LSM3D_MARK_NARROW_BAND_BOUNDARY_LAYER";
+  Call c(fn);
+  const Box gb = BOX(gb), fb = BOX(fb);
+  Field<nb_t> f = c.out(reinterpret_cast<nb_t*>(nb), gb);
+  if (c.ok()) {
+    const int n = std::max(gb.nx(), std::max(gb.ny(), gb.nz()));
+    hipLaunchKernelGGL(k_mark_layer, dim3((unsigned)((n + 63) / 64), (unsigned)((n + 3) / 4), 6), dim3(256), 0,
+                       c.stream(), f, gb, fb, (nb_t)*mark);
+    c.launched("k_mark_layer");
+  }
+  c.finish();
+}
+
+void lsm3dmultiplycutofflserhslocal_(const R* phi, R* rhs, BOXP(g), const int* ix, const int* iy, const int* iz,
+                                     const int* nlo, const int* nhi, const unsigned char* nb, BOXP(nb),
+                                     const unsigned char* mark_fb, const R* beta, const R* gamma) {
+  Call c("LSM3D_MULTIPLY_CUT_OFF_LSE_RHS_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  if (L.ok) {
+    const Box g = BOX(g), gnb = BOX(nb);
+    const R b = *beta, gm = *gamma;
+    R c2 = gm - b;
+    c2 = c2 * c2 * c2;
+    OpCutoff<R> op{c.out(rhs, g), c.in(phi, g), b, gm, gm - R(3) * b, c2};
+    run_entries(c, "k_entries<cutoff>", L, *nlo, *nhi, marks(c, nb, gnb, mark_fb), isect(g, gnb), op);
+  }
+  finish_local(c);
+}
+
+void lsm3dcheckouternarrowbandlayer_(int* change_sign, const R* phi, BOXP(g), const int* ix, const int* iy,
+                                     const int* iz, const int* nlo, const int* nhi, const int* index_outer,
+                                     const int* nlo_outer, const int* nhi_outer, const int* nlo_plus,
+                                     const int* nhi_plus, const int* nlo_minus, const int* nhi_minus) {
+  const char* fn = "LSM3D_CHECK_OUTER_NARROW_BAND_LAYER";
+  Call c(fn);
+  if (!host_array(fn, change_sign, "change_sign")) return;
+  const Box g = BOX(g);
+  const int lo[2] = {*nlo_plus, *nlo_minus}, hi[2] = {*nhi_plus, *nhi_minus};
+  for (int s = 0; s < 2; s++)
+    if (hi[s] >= lo[s] && (lo[s] < *nlo_outer || hi[s] > *nhi_outer)) {
+      raise_error(std::string(fn) + ": outer-layer range outside index_outer");
+      return;
+    }
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  const int64_t ocap = (int64_t)*nhi_outer - *nlo_outer + 1;
+  const int* dout = (ocap > 0 && (hi[0] >= lo[0] || hi[1] >= lo[1])) ? c.in_buf(index_outer, ocap) : nullptr;
+  Field<const R> fphi = c.in(phi, g);
+  int* bits = skipped_counter();
+  if (!c.ok()) return;
+  int* dbits = bits + 4;  // bits[4]: the sign bits of this call
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, c.stream(), dbits, (int64_t)1, 0);
+  for (int s = 0; s < 2 && dout && L.ok; s++) {
+    if (hi[s] < lo[s]) continue;
+    const int n = hi[s] - lo[s] + 1;
+    hipLaunchKernelGGL(k_check_outer<R>, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, c.stream(),
+                       dout + (lo[s] - *nlo_outer), n, L.ix, L.iy, L.iz, *nlo, *nhi, fphi, g, 2 * s, dbits, bits);
+    c.launched("k_check_outer");
+  }
+  int h = 0;
+  if (c.ok()) {
+    check_hip(hipMemcpyAsync(&h, dbits, sizeof(int), hipMemcpyDeviceToHost, c.stream()), "hipMemcpyAsync(bits)");
+    check_hip(hipStreamSynchronize(c.stream()), "hipStreamSynchronize");
+  }
+  finish_local(c, true);
+  if (!lsm::error_pending()) *change_sign = ((h & 3) == 3 || ((h >> 2) & 3) == 3) ? 1 : 0;
+}
+
+void lsm3dimposemasklocal_(R* dest, BOXP(d), const R* src, BOXP(s), const R* mask, BOXP(m), const int* ix,
+                           const int* iy, const int* iz, const int* nlo, const int* nhi) {
+  Call c("LSM3D_IMPOSE_MASK_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  if (L.ok) {
+    const Box gd = BOX(d), gs = BOX(s), gm = BOX(m);
+    OpMaskMax<R> op{c.out(dest, gd), c.in(src, gs), c.in(mask, gm)};
+    run_entries(c, "k_entries<mask>", L, *nlo, *nhi, no_marks(), isect(isect(gd, gs), gm), op);
+  }
+  finish_local(c);
+}
+
+void lsm3dcopydatalocal_(R* dest, BOXP(d), const R* src, BOXP(s), const int* ix, const int* iy, const int* iz,
+                         const int* nlo, const int* nhi) {
+  Call c("LSM3D_COPY_DATA_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  if (L.ok) {
+    const Box gd = BOX(d), gs = BOX(s);
+    OpCopy<R> op{c.out(dest, gd), c.in(src, gs)};
+    run_entries(c, "k_entries<copy>", L, *nlo, *nhi, no_marks(), isect(gd, gs), op);
+  }
+  finish_local(c);
+}
+
+// ---- spatial derivatives: lsm_spatial_derivatives3d_local.h -------------------
+void lsm3dhjeno1local_(R* xp, R* yp, R* zp, BOXP(gp), R* xm, R* ym, R* zm, BOXP(gm), const R* phi, BOXP(phi),
+                       R* D1, BOXP(d1), const R* dx, const R* dy, const R* dz, const int* ix, const int* iy,
+                       const int* iz, const int* nlo0, const int* nhi0, const int* nlo1, const int* nhi1,
+                       const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb,
+                       const unsigned char* mark_d1) {
+  (void)nlo1;
+  hj_eno_local("LSM3D_HJ_ENO1_LOCAL", 1, xp, yp, zp, BOX(gp), xm, ym, zm, BOX(gm), phi, BOX(phi), D1, BOX(d1),
+               nullptr, Box{}, *dx, *dy, *dz, ix, iy, iz, *nlo0, *nhi0, *nhi1, *nhi1, nb, BOX(nb), mark_fb,
+               mark_d1, nullptr);
+}
+
+void lsm3dhjeno2local_(R* xp, R* yp, R* zp, BOXP(gp), R* xm, R* ym, R* zm, BOXP(gm), const R* phi, BOXP(phi),
+                       R* D1, BOXP(d1), R* D2, BOXP(d2), const R* dx, const R* dy, const R* dz, const int* ix,
+                       const int* iy, const int* iz, const int* nlo0, const int* nhi0, const int* nlo1,
+                       const int* nhi1, const int* nlo2, const int* nhi2, const unsigned char* nb, BOXP(nb),
+                       const unsigned char* mark_fb, const unsigned char* mark_d1, const unsigned char* mark_d2) {
+  (void)nlo1;
+  (void)nlo2;
+  hj_eno_local("LSM3D_HJ_ENO2_LOCAL", 2, xp, yp, zp, BOX(gp), xm, ym, zm, BOX(gm), phi, BOX(phi), D1, BOX(d1), D2,
+               BOX(d2), *dx, *dy, *dz, ix, iy, iz, *nlo0, *nhi0, *nhi1, *nhi2, nb, BOX(nb), mark_fb, mark_d1,
+               mark_d2);
+}
+
+static void central_local(const char* fn, int order, R* gx, R* gy, R* gz, Box gg, const R* phi, Box gphi, R dx,
+                          R dy, R dz, const int* ix, const int* iy, const int* iz, int nlo, int nhi,
+                          const unsigned char* nb, Box gnb, const unsigned char* mark) {
+  Call c(fn);
+  ListArgs L = stage_lists(c, ix, iy, iz, nlo, nhi);
+  if (L.ok) {
+    const int r = order / 2;
+    Box valid = isect(isect(gg, shrink(gphi, r, r, r)), gnb);
+    Field<R> fx = c.out(gx, gg), fy = c.out(gy, gg), fz = c.out(gz, gg);
+    Field<const R> fp = c.in(phi, gphi);
+    if (order == 2)
+      run_entries(c, "k_entries<central2>", L, nlo, nhi, marks(c, nb, gnb, mark), valid,
+                  OpCentral2<R>{fx, fy, fz, fp, R(0.5 / (double)dx), R(0.5 / (double)dy), R(0.5 / (double)dz)});
+    else
+      run_entries(c, "k_entries<central4>", L, nlo, nhi, marks(c, nb, gnb, mark), valid,
+                  OpCentral4<R>{fx, fy, fz, fp, R(0.0833333333333333333333 / (double)dx),
+                                R(0.0833333333333333333333 / (double)dy), R(0.0833333333333333333333 / (double)dz)});
+  }
+  finish_local(c);
+}
+
+void lsm3dcentralgradorder2local_(R* gx, R* gy, R* gz, BOXP(g), const R* phi, BOXP(phi), const R* dx, const R* dy,
+                                  const R* dz, const int* ix, const int* iy, const int* iz, const int* nlo,
+                                  const int* nhi, const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  central_local("LSM3D_CENTRAL_GRAD_ORDER2_LOCAL", 2, gx, gy, gz, BOX(g), phi, BOX(phi), *dx, *dy, *dz, ix, iy, iz,
+                *nlo, *nhi, nb, BOX(nb), mark_fb);
+}
+
+void lsm3dcentralgradorder4local_(R* gx, R* gy, R* gz, BOXP(g), const R* phi, BOXP(phi), const R* dx, const R* dy,
+                                  const R* dz, const int* ix, const int* iy, const int* iz, const int* nlo,
+                                  const int* nhi, const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  central_local("LSM3D_CENTRAL_GRAD_ORDER4_LOCAL", 4, gx, gy, gz, BOX(g), phi, BOX(phi), *dx, *dy, *dz, ix, iy, iz,
+                *nlo, *nhi, nb, BOX(nb), mark_fb);
+}
+
+void lsm3dlaplacianorder2local_(R* lap, BOXP(l), const R* phi, BOXP(phi), const R* dx, const R* dy, const R* dz,
+                                const int* ix, const int* iy, const int* iz, const int* nlo, const int* nhi,
+                                const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  Call c("LSM3D_LAPLACIAN_ORDER2_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  if (L.ok) {
+    const Box gl = BOX(l), gp = BOX(phi), gnb = BOX(nb);
+    const double x = *dx, y = *dy, z = *dz;
+    OpLaplacian<R> op{c.out(lap, gl), c.in(phi, gp), R(1.0 / x / x), R(1.0 / y / y), R(1.0 / z / z)};
+    run_entries(c, "k_entries<laplacian>", L, *nlo, *nhi, marks(c, nb, gnb, mark_fb),
+                isect(isect(gl, shrink(gp, 1, 1, 1)), gnb), op);
+  }
+  finish_local(c);
+}
+
+void lsm3dcomputeavegradphilocal_(R* ave, const R* phi, BOXP(phi), const R* dx, const R* dy, const R* dz,
+                                  const int* ix, const int* iy, const int* iz, const int* nlo, const int* nhi,
+                                  const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  Call c("LSM3D_COMPUTE_AVE_GRAD_PHI_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  double out[2] = {0.0, 0.0};
+  if (L.ok) {
+    const Box gp = BOX(phi), gnb = BOX(nb);
+    RedAveGrad<R> op{c.in(phi, gp), R(0.5 / (double)*dx), R(0.5 / (double)*dy), R(0.5 / (double)*dz)};
+    run_entries_reduce(c, "k_entries_reduce<avegrad>", L, *nlo, *nhi, marks(c, nb, gnb, mark_fb),
+                       isect(shrink(gp, 1, 1, 1), gnb), op, RED_LIST_SUM, out);
+  }
+  finish_local(c);
+  if (!lsm::error_pending()) *ave = (out[1] > 0.0) ? R(out[0] / out[1]) : R(out[0]);
+}
+
+void lsm3dgradientmagnitudelocal_(const R* gx, const R* gy, const R* gz, R* mag, BOXP(g), const int* ix,
+                                  const int* iy, const int* iz, const int* nlo, const int* nhi,
+                                  const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  Call c("LSM3D_GRADIENT_MAGNITUDE_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  if (L.ok) {
+    const Box g = BOX(g), gnb = BOX(nb);
+    OpGradMag<R> op{c.out(mag, g), c.in(gx, g), c.in(gy, g), c.in(gz, g)};
+    run_entries(c, "k_entries<gradmag>", L, *nlo, *nhi, marks(c, nb, gnb, mark_fb), isect(g, gnb), op);
+  }
+  finish_local(c);
+}
+
+// ---- TVD Runge-Kutta: lsm_tvd_runge_kutta3d_local.h ----------------------------
+void lsm3drk1steplocal_(R* un, BOXP(un), const R* uc, BOXP(uc), const R* rhs, BOXP(r), const R* dt, const int* ix,
+                        const int* iy, const int* iz, const int* nlo, const int* nhi, const unsigned char* nb,
+                        BOXP(nb), const unsigned char* mark_fb) {
+  rk_local("LSM3D_RK1_STEP_LOCAL", RK_EULER, un, BOX(un), nullptr, Box{}, uc, BOX(uc), rhs, BOX(r), *dt, ix, iy,
+           iz, *nlo, *nhi, nb, BOX(nb), mark_fb);
+}
+void lsm3dtvdrk2stage1local_(R* un, BOXP(un), const R* uc, BOXP(uc), const R* rhs, BOXP(r), const R* dt,
+                             const int* ix, const int* iy, const int* iz, const int* nlo, const int* nhi,
+                             const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  rk_local("LSM3D_TVD_RK2_STAGE1_LOCAL", RK_EULER, un, BOX(un), nullptr, Box{}, uc, BOX(uc), rhs, BOX(r), *dt, ix,
+           iy, iz, *nlo, *nhi, nb, BOX(nb), mark_fb);
+}
+void lsm3dtvdrk2stage2local_(R* un, BOXP(un), const R* u1, BOXP(u1), const R* uc, BOXP(uc), const R* rhs, BOXP(r),
+                             const R* dt, const int* ix, const int* iy, const int* iz, const int* nlo,
+                             const int* nhi, const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  rk_local("LSM3D_TVD_RK2_STAGE2_LOCAL", RK2_STAGE2, un, BOX(un), u1, BOX(u1), uc, BOX(uc), rhs, BOX(r), *dt, ix,
+           iy, iz, *nlo, *nhi, nb, BOX(nb), mark_fb);
+}
+void lsm3dtvdrk3stage1local_(R* un, BOXP(un), const R* uc, BOXP(uc), const R* rhs, BOXP(r), const R* dt,
+                             const int* ix, const int* iy, const int* iz, const int* nlo, const int* nhi,
+                             const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  rk_local("LSM3D_TVD_RK3_STAGE1_LOCAL", RK_EULER, un, BOX(un), nullptr, Box{}, uc, BOX(uc), rhs, BOX(r), *dt, ix,
+           iy, iz, *nlo, *nhi, nb, BOX(nb), mark_fb);
+}
+void lsm3dtvdrk3stage2local_(R* un, BOXP(un), const R* u1, BOXP(u1), const R* uc, BOXP(uc), const R* rhs, BOXP(r),
+                             const R* dt, const int* ix, const int* iy, const int* iz, const int* nlo,
+                             const int* nhi, const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  rk_local("LSM3D_TVD_RK3_STAGE2_LOCAL", RK3_STAGE2, un, BOX(un), u1, BOX(u1), uc, BOX(uc), rhs, BOX(r), *dt, ix,
+           iy, iz, *nlo, *nhi, nb, BOX(nb), mark_fb);
+}
+void lsm3dtvdrk3stage3local_(R* un, BOXP(un), const R* u2, BOXP(u2), const R* uc, BOXP(uc), const R* rhs, BOXP(r),
+                             const R* dt, const int* ix, const int* iy, const int* iz, const int* nlo,
+                             const int* nhi, const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  rk_local("LSM3D_TVD_RK3_STAGE3_LOCAL", RK3_STAGE3, un, BOX(un), u2, BOX(u2), uc, BOX(uc), rhs, BOX(r), *dt, ix,
+           iy, iz, *nlo, *nhi, nb, BOX(nb), mark_fb);
+}
+
+// ---- level set equation RHS: lsm_level_set_evolution3d_local.h ------------------
+void lsm3dzerooutlevelseteqnrhslocal_(R* rhs, BOXP(r), const int* ix, const int* iy, const int* iz,
+                                      const int* nlo, const int* nhi) {
+  Call c("LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  if (L.ok) {
+    const Box g = BOX(r);
+    run_entries(c, "k_entries<zero>", L, *nlo, *nhi, no_marks(), g, OpZero<R>{c.out(rhs, g)});
+  }
+  finish_local(c);
+}
+
+void lsm3daddadvectiontermtolserhslocal_(R* rhs, BOXP(r), const R* px, const R* py, const R* pz, BOXP(g),
+                                         const R* vx, const R* vy, const R* vz, BOXP(v), const int* ix,
+                                         const int* iy, const int* iz, const int* nlo, const int* nhi,
+                                         const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  Call c("LSM3D_ADD_ADVECTION_TERM_TO_LSE_RHS_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  if (L.ok) {
+    const Box gr = BOX(r), gg = BOX(g), gv = BOX(v), gnb = BOX(nb);
+    OpAdvection<R> op{c.out(rhs, gr), c.in(px, gg), c.in(py, gg), c.in(pz, gg), c.in(vx, gv), c.in(vy, gv),
+                      c.in(vz, gv)};
+    run_entries(c, "k_entries<advection>", L, *nlo, *nhi, marks(c, nb, gnb, mark_fb),
+                isect(isect(isect(gr, gg), gv), gnb), op);
+  }
+  finish_local(c);
+}
+
+static void normal_vel_local(const char* fn, R* rhs, Box gr, const R* xp, const R* yp, const R* zp, Box gp,
+                             const R* xm, const R* ym, const R* zm, Box gm, const R* vn, Box gv, R vn_const,
+                             const int* ix, const int* iy, const int* iz, int nlo, int nhi,
+                             const unsigned char* nb, Box gnb, const unsigned char* mark) {
+  Call c(fn);
+  ListArgs L = stage_lists(c, ix, iy, iz, nlo, nhi);
+  if (L.ok) {
+    Box valid = isect(isect(isect(gr, gp), gm), gnb);
+    if (vn) valid = isect(valid, gv);
+    OpNormalVel<R> op{c.out(rhs, gr), c.in(xp, gp), c.in(yp, gp), c.in(zp, gp), c.in(xm, gm),
+                      c.in(ym, gm), c.in(zm, gm), vn ? c.in(vn, gv) : none(), vn_const};
+    run_entries(c, "k_entries<normalvel>", L, nlo, nhi, marks(c, nb, gnb, mark), valid, op);
+  }
+  finish_local(c);
+}
+
+void lsm3daddnormalveltermtolserhslocal_(R* rhs, BOXP(r), const R* xp, const R* yp, const R* zp, BOXP(gp),
+                                         const R* xm, const R* ym, const R* zm, BOXP(gm), const R* vn, BOXP(v),
+                                         const int* ix, const int* iy, const int* iz, const int* nlo,
+                                         const int* nhi, const unsigned char* nb, BOXP(nb),
+                                         const unsigned char* mark_fb) {
+  normal_vel_local("LSM3D_ADD_NORMAL_VEL_TERM_TO_LSE_RHS_LOCAL", rhs, BOX(r), xp, yp, zp, BOX(gp), xm, ym, zm,
+                   BOX(gm), vn, BOX(v), R(0), ix, iy, iz, *nlo, *nhi, nb, BOX(nb), mark_fb);
+}
+
+void lsm3daddconstnormalveltermtolserhslocal_(R* rhs, BOXP(r), const R* xp, const R* yp, const R* zp, BOXP(gp),
+                                              const R* xm, const R* ym, const R* zm, BOXP(gm), const R* vn,
+                                              const int* ix, const int* iy, const int* iz, const int* nlo,
+                                              const int* nhi, const unsigned char* nb, BOXP(nb),
+                                              const unsigned char* mark_fb) {
+  normal_vel_local("LSM3D_ADD_CONST_NORMAL_VEL_TERM_TO_LSE_RHS_LOCAL", rhs, BOX(r), xp, yp, zp, BOX(gp), xm, ym,
+                   zm, BOX(gm), nullptr, Box{}, *vn, ix, iy, iz, *nlo, *nhi, nb, BOX(nb), mark_fb);
+}
+
+void lsm3daddconstcurvtermtolserhslocal_(R* rhs, BOXP(r), const R* px, const R* py, const R* pz, BOXP(g1),
+                                         const R* pxx, const R* pxy, const R* pxz, const R* pyy, const R* pyz,
+                                         const R* pzz, BOXP(g2), const R* b, const int* ix, const int* iy,
+                                         const int* iz, const int* nlo, const int* nhi, const unsigned char* nb,
+                                         BOXP(nb), const unsigned char* mark_fb) {
+  Call c("LSM3D_ADD_CONST_CURV_TERM_TO_LSE_RHS_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  if (L.ok) {
+    const Box gr = BOX(r), g1 = BOX(g1), g2 = BOX(g2), gnb = BOX(nb);
+    OpConstCurv<R> op{c.out(rhs, gr), c.in(px, g1),  c.in(py, g1),  c.in(pz, g1),  c.in(pxx, g2),
+                      c.in(pxy, g2),  c.in(pxz, g2), c.in(pyy, g2), c.in(pyz, g2), c.in(pzz, g2), *b};
+    run_entries(c, "k_entries<curv>", L, *nlo, *nhi, marks(c, nb, gnb, mark_fb),
+                isect(isect(isect(gr, g1), g2), gnb), op);
+  }
+  finish_local(c);
+}
+
+void lsm3daddconstprecomputedcurvtermtolserhslocal_(R* rhs, BOXP(r), const R* kappa, BOXP(k), const R* gmag,
+                                                    BOXP(g), const R* b, const int* ix, const int* iy,
+                                                    const int* iz, const int* nlo, const int* nhi,
+                                                    const unsigned char* nb, BOXP(nb),
+                                                    const unsigned char* mark_fb) {
+  Call c("LSM3D_ADD_CONST_PRECOMPUTED_CURV_TERM_TO_LSE_RHS_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  if (L.ok) {
+    const Box gr = BOX(r), gk = BOX(k), gg = BOX(g), gnb = BOX(nb);
+    OpPrecomputedCurv<R> op{c.out(rhs, gr), c.in(kappa, gk), c.in(gmag, gg), *b};
+    run_entries(c, "k_entries<precurv>", L, *nlo, *nhi, marks(c, nb, gnb, mark_fb),
+                isect(isect(isect(gr, gk), gg), gnb), op);
+  }
+  finish_local(c);
+}
+
+void lsm3daddexternalandnormalveltermtolserhslocal_(R* rhs, BOXP(r), const R* xp, const R* yp, const R* zp,
+                                                    BOXP(gp), const R* xm, const R* ym, const R* zm, BOXP(gm),
+                                                    const R* vn, const R* vx, const R* vy, const R* vz, BOXP(v),
+                                                    const int* ix, const int* iy, const int* iz, const int* nlo,
+                                                    const int* nhi, const unsigned char* nb, BOXP(nb),
+                                                    const unsigned char* mark_fb) {
+  Call c("LSM3D_ADD_EXTERNAL_AND_NORMAL_VEL_TERM_TO_LSE_RHS_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  if (L.ok) {
+    const Box gr = BOX(r), gpp = BOX(gp), gmm = BOX(gm), gv = BOX(v), gnb = BOX(nb);
+    OpExtNormal<R> op{c.out(rhs, gr), c.in(xp, gpp), c.in(yp, gpp), c.in(zp, gpp), c.in(xm, gmm), c.in(ym, gmm),
+                      c.in(zm, gmm),  c.in(vn, gv),  c.in(vx, gv),  c.in(vy, gv),  c.in(vz, gv)};
+    run_entries(c, "k_entries<extnormal>", L, *nlo, *nhi, marks(c, nb, gnb, mark_fb),
+                isect(isect(isect(isect(gr, gpp), gmm), gv), gnb), op);
+  }
+  finish_local(c);
+}
+
+// ---- reinitialization: lsm_reinitialization3d_local.h ---------------------------
+void lsm3dcomputereinitializationeqnrhslocal_(R* rhs, BOXP(r), const R* phi, BOXP(phi), const R* phi0, BOXP(p0),
+                                              const R* xp, const R* yp, const R* zp, BOXP(gp), const R* xm,
+                                              const R* ym, const R* zm, BOXP(gm), const R* dx, const R* dy,
+                                              const R* dz, const int* use_phi0, const int* ix, const int* iy,
+                                              const int* iz, const int* nlo, const int* nhi,
+                                              const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  Call c("LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  if (L.ok) {
+    const Box gr = BOX(r), gphi = BOX(phi), g0 = BOX(p0), gpp = BOX(gp), gmm = BOX(gm), gnb = BOX(nb);
+    const bool u0 = (*use_phi0 == 1);
+    const R m = max3(*dx, *dy, *dz);
+    OpReinit<R> op{c.out(rhs, gr), c.in(phi, gphi), u0 ? c.in(phi0, g0) : none(), c.in(xp, gpp), c.in(yp, gpp),
+                   c.in(zp, gpp),  c.in(xm, gmm),   c.in(ym, gmm),  c.in(zm, gmm), m * m, u0 ? 1 : 0};
+    Box valid = isect(isect(isect(isect(gr, gphi), gpp), gmm), gnb);
+    if (u0) valid = isect(valid, g0);
+    run_entries(c, "k_entries<reinit>", L, *nlo, *nhi, marks(c, nb, gnb, mark_fb), valid, op);
+  }
+  finish_local(c);
+}
+
+void lsm3dcomputeorthogonalizationeqnrhslocal_(R* rhs, BOXP(r), const R* xp, const R* yp, const R* zp, BOXP(gp),
+                                               const R* xm, const R* ym, const R* zm, BOXP(gm), const R* psi,
+                                               BOXP(psi), const R* sxp, const R* syp, const R* szp, BOXP(sp),
+                                               const R* sxm, const R* sym, const R* szm, BOXP(sm), const R* dx,
+                                               const R* dy, const R* dz, const int* ix, const int* iy,
+                                               const int* iz, const int* nlo, const int* nhi,
+                                               const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  Call c("LSM3D_COMPUTE_ORTHOGONALIZATION_EQN_RHS_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  if (L.ok) {
+    const Box gr = BOX(r), gpp = BOX(gp), gmm = BOX(gm), gps = BOX(psi), gsp = BOX(sp), gsm = BOX(sm),
+              gnb = BOX(nb);
+    const R m = max3(*dx, *dy, *dz);
+    OpOrtho<R> op{c.out(rhs, gr),  c.in(xp, gpp),  c.in(yp, gpp),  c.in(zp, gpp),  c.in(xm, gmm),
+                  c.in(ym, gmm),   c.in(zm, gmm),  c.in(psi, gps), c.in(sxp, gsp), c.in(syp, gsp),
+                  c.in(szp, gsp),  c.in(sxm, gsm), c.in(sym, gsm), c.in(szm, gsm), m * m};
+    run_entries(c, "k_entries<ortho>", L, *nlo, *nhi, marks(c, nb, gnb, mark_fb),
+                isect(isect(isect(isect(isect(isect(gr, gpp), gmm), gps), gsp), gsm), gnb), op);
+  }
+  finish_local(c);
+}
+
+// ---- reductions: lsm_utilities3d_local.h ---------------------------------------------
+void lsm3dmaxnormdifflocal_(R* out, const R* f1, BOXP(f1), const R* f2, BOXP(f2), const int* ix, const int* iy,
+                            const int* iz, const int* nlo, const int* nhi, const unsigned char* nb, BOXP(nb),
+                            const unsigned char* mark_fb) {
+  Call c("LSM3D_MAX_NORM_DIFF_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  double r[2] = {0.0, 0.0};
+  if (L.ok) {
+    const Box g1 = BOX(f1), g2 = BOX(f2), gnb = BOX(nb);
+    RedMaxDiff<R> op{c.in(f1, g1), c.in(f2, g2)};
+    run_entries_reduce(c, "k_entries_reduce<maxdiff>", L, *nlo, *nhi, marks(c, nb, gnb, mark_fb),
+                       isect(isect(g1, g2), gnb), op, RED_LIST_MAX, r);
+  }
+  finish_local(c);
+  if (!lsm::error_pending()) *out = (r[0] > 0.0) ? R(r[0]) : R(0);  // the reference starts from 0
+}
+
+static void normal_dt_local(const char* fn, R* dt, const R* vn, Box gv, R vn_const, const R* xp, const R* yp,
+                            const R* zp, Box gp, const R* xm, const R* ym, const R* zm, Box gm, const R* cv,
+                            Box gcv, int cv_sgn, R dx, R dy, R dz, R cfl, const int* ix, const int* iy,
+                            const int* iz, int nlo, int nhi, const unsigned char* nb, Box gnb,
+                            const unsigned char* mark) {
+  Call c(fn);
+  ListArgs L = stage_lists(c, ix, iy, iz, nlo, nhi);
+  double r[2] = {-1.0e308, 0.0};
+  if (L.ok) {
+    Box valid = isect(isect(gp, gm), gnb);
+    if (vn) valid = isect(valid, gv);
+    if (cv) valid = isect(valid, gcv);
+    const R mdx = max3(dx, dy, dz);
+    RedNormalDt<R> op{c.in(xp, gp), c.in(yp, gp), c.in(zp, gp), c.in(xm, gm), c.in(ym, gm), c.in(zm, gm),
+                      vn ? c.in(vn, gv) : none(), cv ? c.in(cv, gcv) : none(), R(fabs(vn_const)), mdx * mdx,
+                      R(1.0 / (double)dx), R(1.0 / (double)dy), R(1.0 / (double)dz), cv_sgn};
+    run_entries_reduce(c, "k_entries_reduce<normaldt>", L, nlo, nhi, marks(c, nb, gnb, mark), valid, op,
+                       RED_LIST_MAX, r);
+  }
+  finish_local(c);
+  R mx = R(-1.0);
+  if (r[0] > (double)mx) mx = R(r[0]);
+  if (!lsm::error_pending()) *dt = cfl / (mx + Tol<R>::tiny);
+}
+
+void lsm3dcomputestableadvectiondtlocal_(R* dt, const R* vx, const R* vy, const R* vz, BOXP(v), const R* dx,
+                                         const R* dy, const R* dz, const R* cfl, const int* ix, const int* iy,
+                                         const int* iz, const int* nlo, const int* nhi, const unsigned char* nb,
+                                         BOXP(nb), const unsigned char* mark_fb) {
+  Call c("LSM3D_COMPUTE_STABLE_ADVECTION_DT_LOCAL");
+  ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
+  double r[2] = {-1.0e308, 0.0};
+  if (L.ok) {
+    const Box gv = BOX(v), gnb = BOX(nb);
+    RedAdvDt<R> op{c.in(vx, gv), c.in(vy, gv), c.in(vz, gv), R(1.0 / (double)*dx), R(1.0 / (double)*dy),
+                   R(1.0 / (double)*dz)};
+    run_entries_reduce(c, "k_entries_reduce<advdt>", L, *nlo, *nhi, marks(c, nb, gnb, mark_fb),
+                       isect(gv, gnb), op, RED_LIST_MAX, r);
+  }
+  finish_local(c);
+  R mx = R(-1.0);
+  if (r[0] > (double)mx) mx = R(r[0]);
+  if (!lsm::error_pending()) *dt = *cfl / (mx + Tol<R>::tiny);
+}
+
+void lsm3dcomputestablenormalveldtlocal_(R* dt, const R* vn, BOXP(v), const R* xp, const R* yp, const R* zp,
+                                         BOXP(gp), const R* xm, const R* ym, const R* zm, BOXP(gm), const R* dx,
+                                         const R* dy, const R* dz, const R* cfl, const int* ix, const int* iy,
+                                         const int* iz, const int* nlo, const int* nhi, const unsigned char* nb,
+                                         BOXP(nb), const unsigned char* mark_fb) {
+  normal_dt_local("LSM3D_COMPUTE_STABLE_NORMAL_VEL_DT_LOCAL", dt, vn, BOX(v), R(0), xp, yp, zp, BOX(gp), xm, ym,
+                  zm, BOX(gm), nullptr, Box{}, 0, *dx, *dy, *dz, *cfl, ix, iy, iz, *nlo, *nhi, nb, BOX(nb),
+                  mark_fb);
+}
+
+void lsm3dcomputestablenormalveldtcontrolvolumelocal_(R* dt, const R* vn, BOXP(v), const R* xp, const R* yp,
+                                                      const R* zp, BOXP(gp), const R* xm, const R* ym, const R* zm,
+                                                      BOXP(gm), const R* cv, BOXP(cv), const int* cv_sgn,
+                                                      const R* dx, const R* dy, const R* dz, const R* cfl,
+                                                      const int* ix, const int* iy, const int* iz, const int* nlo,
+                                                      const int* nhi, const unsigned char* nb, BOXP(nb),
+                                                      const unsigned char* mark_fb) {
+  normal_dt_local("LSM3D_COMPUTE_STABLE_NORMAL_VEL_DT_CONTROL_VOLUME_LOCAL", dt, vn, BOX(v), R(0), xp, yp, zp,
+                  BOX(gp), xm, ym, zm, BOX(gm), cv, BOX(cv), *cv_sgn, *dx, *dy, *dz, *cfl, ix, iy, iz, *nlo, *nhi,
+                  nb, BOX(nb), mark_fb);
+}
+
+void lsm3dcomputestableconstnormalveldtlocal_(R* dt, const R* vn, const R* xp, const R* yp, const R* zp, BOXP(gp),
+                                              const R* xm, const R* ym, const R* zm, BOXP(gm), const R* dx,
+                                              const R* dy, const R* dz, const R* cfl, const int* ix, const int* iy,
+                                              const int* iz, const int* nlo, const int* nhi,
+                                              const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  normal_dt_local("LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT_LOCAL", dt, nullptr, Box{}, *vn, xp, yp, zp, BOX(gp),
+                  xm, ym, zm, BOX(gm), nullptr, Box{}, 0, *dx, *dy, *dz, *cfl, ix, iy, iz, *nlo, *nhi, nb, BOX(nb),
+                  mark_fb);
+}
+
+}  // extern "C"

@@ -35,6 +35,12 @@ bool is_device_pointer(const void* p);
 // reused across calls (reductions keep per-block partials here).
 void* device_scratch(size_t bytes);
 
+// Further library-owned device buffers, one per (device, slot), grown on
+// demand like device_scratch.  *fresh is set when the buffer was (re)allocated
+// by this call, so a caller that keeps an invariant in it can re-establish it.
+enum BufferSlot { BUF_BAND_KEY = 0, BUF_BAND_WORK = 1, BUF_LOCAL_FLAGS = 2, BUF_SLOTS = 4 };
+void* device_buffer(int slot, size_t bytes, bool* fresh = nullptr, size_t* size = nullptr);
+
 // Collects the arrays of one entry-point call.  Device pointers pass through;
 // host arrays are copied to a device buffer (whole ghost box) before the
 // launch and, for outputs, copied back by finish().  finish() also
@@ -52,6 +58,15 @@ class Call {
     return make_field<const T>(reinterpret_cast<const T*>(stage(const_cast<T*>(p), gb, sizeof(T), false)),
                                gb);
   }
+  // A flat array of n elements (index lists and the like).
+  template <typename T>
+  T* out_buf(T* p, int64_t n) {
+    return reinterpret_cast<T*>(stage_bytes(p, (size_t)n * sizeof(T), true));
+  }
+  template <typename T>
+  const T* in_buf(const T* p, int64_t n) {
+    return reinterpret_cast<const T*>(stage_bytes(const_cast<T*>(p), (size_t)n * sizeof(T), false));
+  }
   hipStream_t stream() const { return stream_; }
   const char* name() const { return name_; }
   void launched(const char* kernel);  // checks the launch
@@ -60,6 +75,7 @@ class Call {
 
  private:
   void* stage(void* p, const Box& gb, size_t elem, bool is_out);
+  void* stage_bytes(void* p, size_t bytes, bool is_out);
   struct Staged {
     void* host;
     void* dev;

@@ -79,6 +79,33 @@ void* device_scratch(size_t bytes) {
   return g_scratch[dev];
 }
 
+void* device_buffer(int slot, size_t bytes, bool* fresh, size_t* size) {
+  static void* bufs[kMaxDevices][BUF_SLOTS] = {};
+  static size_t sizes[kMaxDevices][BUF_SLOTS] = {};
+  if (fresh) *fresh = false;
+  int dev = 0;
+  check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices || slot < 0 || slot >= BUF_SLOTS) {
+    raise_error("device buffer index out of range");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (sizes[dev][slot] < bytes) {
+    if (bufs[dev][slot]) {
+      check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      check_hip(hipFree(bufs[dev][slot]), "hipFree(buffer)");
+    }
+    size_t want = bytes < (1u << 16) ? (1u << 16) : bytes + bytes / 4;
+    bufs[dev][slot] = nullptr;
+    sizes[dev][slot] = 0;
+    check_hip(hipMalloc(&bufs[dev][slot], want), "hipMalloc(buffer)");
+    if (bufs[dev][slot]) sizes[dev][slot] = want;
+    if (fresh) *fresh = true;
+  }
+  if (size) *size = sizes[dev][slot];
+  return bufs[dev][slot];
+}
+
 Call::Call(const char* name) : name_(name), stream_(tl_stream) {
   tl_error = false;
   tl_message.clear();
@@ -98,13 +125,26 @@ void* Call::stage(void* p, const Box& gb, size_t elem, bool is_out) {
     raise_error(std::string(name_) + ": empty ghost box for an array argument");
     return nullptr;
   }
+  return stage_bytes(p, (size_t)gb.cells() * elem, is_out);
+}
+
+void* Call::stage_bytes(void* p, size_t bytes, bool is_out) {
+  if (error_pending()) return nullptr;
+  if (p == nullptr) {
+    raise_error(std::string(name_) + ": null array argument");
+    return nullptr;
+  }
   if (is_device_pointer(p)) return p;
   for (auto& s : staged_)  // the same host array passed twice shares one buffer
     if (s.host == p) {
+      if (s.bytes < bytes) {
+        raise_error(std::string(name_) + ": one host array passed with two different extents");
+        return nullptr;
+      }
       s.is_out = s.is_out || is_out;
       return s.dev;
     }
-  Staged s{p, nullptr, (size_t)gb.cells() * elem, is_out};
+  Staged s{p, nullptr, bytes, is_out};
   check_hip(hipMalloc(&s.dev, s.bytes), "hipMalloc(staging)");
   if (error_pending()) return nullptr;
   check_hip(hipMemcpyAsync(s.dev, p, s.bytes, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync H2D");

@@ -239,6 +239,17 @@ __device__ __forceinline__ T h_over_dx(T avn, T xp, T yp, T zp, T xm, T ym, T zm
   return avn / ng * (px * ix + py * iy + pz * iz);
 }
 
+// Godunov selection for the combined external + normal velocity,
+// lsm_level_set_evolution3d.f.in:630-650
+template <typename T>
+__device__ __forceinline__ T ext_normal_select(T vc, T vn, T dp, T dm) {
+#pragma clang fp contract(off)  // products and sums rounded separately, as the reference's
+  const T hp = vc + vn * dp, hm = vc + vn * dm;
+  if (hp > T(0)) return (hm > T(0)) ? dm : -(vc / vn);
+  if (hm < T(0)) return dp;
+  return (fabs(hp) > fabs(hm)) ? dp : dm;
+}
+
 // TVD Runge-Kutta stage combinations (lsm_tvd_runge_kutta3d.f), given the
 // current solution uc, the stage input us and the stage right-hand side r.
 // UNBIASED (the FP32 fused stages) writes TVD-RK3 stage 3 as

@@ -315,15 +315,6 @@ __global__ void __launch_bounds__(256)
   LSM_CELL_LOOP_END
 }
 
-// Godunov selection for the combined external + normal velocity, :630-650
-template <typename T>
-__device__ __forceinline__ T ext_normal_select(T vc, T vn, T dp, T dm) {
-  const T hp = vc + vn * dp, hm = vc + vn * dm;
-  if (hp > T(0)) return (hm > T(0)) ? dm : -(vc / vn);
-  if (hm < T(0)) return dp;
-  return (fabs(hp) > fabs(hm)) ? dp : dm;
-}
-
 // :532-719 over the reference's loop box (see the entry point)
 template <typename T>
 __global__ void __launch_bounds__(256)

@@ -11,6 +11,10 @@ forms:
   I  a Python int
   O  omitted from the call; the values are returned (a float, or a tuple
      when the routine has several)
+  X  an int32 array (numpy or torch); N the same, in host memory
+  U  an int8 or uint8 array on its box (numpy or torch)
+  M  a Python int, passed as one byte
+  K  omitted from the call; returned like O, as an int
 
 Arrays hold a box `(ilo..ihi, jlo..jhi, klo..khi)` in Fortran order, i.e. a
 C-contiguous array of shape (nk, nj, ni) with i fastest.
@@ -29,6 +33,29 @@ def box_shape(box):
     """(nk, nj, ni) array shape of a box (ilo, ihi, jlo, jhi, klo, khi)."""
     ilo, ihi, jlo, jhi, klo, khi = box
     return (khi - klo + 1, jhi - jlo + 1, ihi - ilo + 1)
+
+
+def _int_ptr(a, kind):
+    """Data pointer of an int32 (kind X/N) or 8-bit (kind U) array."""
+    ok = (np.int32,) if kind in "XN" else (np.int8, np.uint8)
+    if isinstance(a, np.ndarray):
+        if a.dtype.type not in ok:
+            raise TypeError(f"kind {kind} array must be {'int32' if kind in 'XN' else 'int8/uint8'}, "
+                            f"got {a.dtype}")
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("arrays must be C-contiguous")
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        import torch
+        want = (torch.int32,) if kind in "XN" else (torch.int8, torch.uint8)
+        if a.dtype not in want:
+            raise TypeError(f"kind {kind} tensor has dtype {a.dtype}")
+        if not a.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+        return a.data_ptr()
+    if a is None:
+        return 0
+    raise TypeError(f"unsupported array argument {type(a)!r}")
 
 
 def _data_ptr(a, np_dtype):
@@ -77,8 +104,8 @@ class F77Library:
         cargs, outs, keep = [], [], []
         it = iter(args)
         for k in kinds:
-            if k == "O":
-                o = self._ct(0)
+            if k in "OK":
+                o = self._ct(0) if k == "O" else ctypes.c_int(0)
                 outs.append(o)
                 cargs.append(ctypes.byref(o))
                 continue
@@ -99,6 +126,11 @@ class F77Library:
                 cargs.append(ctypes.byref(self._ct(float(a))))
             elif k == "I":
                 cargs.append(ctypes.byref(ctypes.c_int(int(a))))
+            elif k in "XNU":
+                cargs.append(ctypes.c_void_p(_int_ptr(a, k)))
+                keep.append(a)
+            elif k == "M":
+                cargs.append(ctypes.byref(ctypes.c_ubyte(int(a) & 0xFF)))
             else:  # pragma: no cover - table error
                 raise ValueError(k)
         rest = list(it)

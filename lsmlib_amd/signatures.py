@@ -9,6 +9,13 @@ one prototype as a sequence of argument kinds:
   R  const LSMLIB_REAL* scalar input
   O  LSMLIB_REAL*      scalar output
   I  const int*        integer input
+  X  int*              index list (index_x/y/z, index_outer): device memory
+                       for the product library
+  N  int*              per-level list limits n_lo / n_hi: host memory
+  U  unsigned char*    narrow-band mark array on its box (compared as the
+                       Fortran integer*1, i.e. signed)
+  M  const unsigned char*  one mark value
+  K  int*              integer output
 
 The same table drives the ctypes marshalling for the product library and lets
 tests check that every declared symbol is exported.
@@ -78,6 +85,54 @@ _TABLE = [
     ("lsm3dhomogeneousneumanneno2_", "AB B I", "lsm_boundary_conditions3d.h:186"),
     ("lsm3dhomogeneousneumanneno3_", "AB B I", "lsm_boundary_conditions3d.h:211"),
     ("lsm3dhomogeneousneumannweno5_", "AB B I", "lsm_boundary_conditions3d.h:236"),
+    # narrow band -- lsm_localization3d.h
+    ("lsm3ddeterminenarrowband_", "AB UB XXX II NN X II KKKK RR I", "lsm_localization3d.h:89"),
+    ("lsm3dmarknarrowbandboundarylayer_", "UB B M", "lsm_localization3d.h:138"),
+    ("lsm3ddeterminenarrowbandfrommask_", "AB UB XXX II NN I I", "lsm_localization3d.h:187"),
+    ("lsm3dmultiplycutofflserhslocal_", "AAB XXX II UB M RR", "lsm_localization3d.h:230"),
+    ("lsm3dcheckouternarrowbandlayer_", "K AB XXX II X II II II", "lsm_localization3d.h:270"),
+    ("lsm3dimposemasklocal_", "AB AB AB XXX II", "lsm_localization3d.h:307"),
+    ("lsm3dcopydatalocal_", "AB AB XXX II", "lsm_localization3d.h:347"),
+    # narrow-band kernels -- lsm_*3d_local.h
+    ("lsm3dhjeno1local_", "AAAB AAAB AB AB RRR XXX IIII UB MM", "lsm_spatial_derivatives3d_local.h:78"),
+    ("lsm3dhjeno2local_", "AAAB AAAB AB AB AB RRR XXX IIIIII UB MMM",
+     "lsm_spatial_derivatives3d_local.h:159"),
+    ("lsm3dcentralgradorder2local_", "AAAB AB RRR XXX II UB M", "lsm_spatial_derivatives3d_local.h:240"),
+    ("lsm3dcentralgradorder4local_", "AAAB AB RRR XXX II UB M", "lsm_spatial_derivatives3d_local.h:293"),
+    ("lsm3dlaplacianorder2local_", "AB AB RRR XXX II UB M", "lsm_spatial_derivatives3d_local.h:345"),
+    ("lsm3dcomputeavegradphilocal_", "O AB RRR XXX II UB M", "lsm_spatial_derivatives3d_local.h:397"),
+    ("lsm3dgradientmagnitudelocal_", "AAAAB XXX II UB M", "lsm_spatial_derivatives3d_local.h:438"),
+    ("lsm3drk1steplocal_", "AB AB AB R XXX II UB M", "lsm_tvd_runge_kutta3d_local.h:35"),
+    ("lsm3dtvdrk2stage1local_", "AB AB AB R XXX II UB M", "lsm_tvd_runge_kutta3d_local.h:73"),
+    ("lsm3dtvdrk2stage2local_", "AB AB AB AB R XXX II UB M", "lsm_tvd_runge_kutta3d_local.h:111"),
+    ("lsm3dtvdrk3stage1local_", "AB AB AB R XXX II UB M", "lsm_tvd_runge_kutta3d_local.h:155"),
+    ("lsm3dtvdrk3stage2local_", "AB AB AB AB R XXX II UB M", "lsm_tvd_runge_kutta3d_local.h:192"),
+    ("lsm3dtvdrk3stage3local_", "AB AB AB AB R XXX II UB M", "lsm_tvd_runge_kutta3d_local.h:237"),
+    ("lsm3dzerooutlevelseteqnrhslocal_", "AB XXX II", "lsm_level_set_evolution3d_local.h:67"),
+    ("lsm3daddadvectiontermtolserhslocal_", "AB AAAB AAAB XXX II UB M",
+     "lsm_level_set_evolution3d_local.h:102"),
+    ("lsm3daddnormalveltermtolserhslocal_", "AB AAAB AAAB AB XXX II UB M",
+     "lsm_level_set_evolution3d_local.h:166"),
+    ("lsm3daddconstnormalveltermtolserhslocal_", "AB AAAB AAAB R XXX II UB M",
+     "lsm_level_set_evolution3d_local.h:237"),
+    ("lsm3daddconstcurvtermtolserhslocal_", "AB AAAB AAAAAAB R XXX II UB M",
+     "lsm_level_set_evolution3d_local.h:300"),
+    ("lsm3daddconstprecomputedcurvtermtolserhslocal_", "AB AB AB R XXX II UB M",
+     "lsm_level_set_evolution3d_local.h:374"),
+    ("lsm3daddexternalandnormalveltermtolserhslocal_", "AB AAAB AAAB AAAAB XXX II UB M",
+     "lsm_level_set_evolution3d_local.h:449"),
+    ("lsm3dcomputereinitializationeqnrhslocal_", "AB AB AB AAAB AAAB RRR I XXX II UB M",
+     "lsm_reinitialization3d_local.h:41"),
+    ("lsm3dcomputeorthogonalizationeqnrhslocal_", "AB AAAB AAAB AB AAAB AAAB RRR XXX II UB M",
+     "lsm_reinitialization3d_local.h:99"),
+    ("lsm3dmaxnormdifflocal_", "O AB AB XXX II UB M", "lsm_utilities3d_local.h:65"),
+    ("lsm3dcomputestableadvectiondtlocal_", "O AAAB RRR R XXX II UB M", "lsm_utilities3d_local.h:112"),
+    ("lsm3dcomputestablenormalveldtlocal_", "O AB AAAB AAAB RRR R XXX II UB M",
+     "lsm_utilities3d_local.h:169"),
+    ("lsm3dcomputestablenormalveldtcontrolvolumelocal_", "O AB AAAB AAAB AB I RRR R XXX II UB M",
+     "lsm_utilities3d_local.h:241"),
+    ("lsm3dcomputestableconstnormalveldtlocal_", "O R AAAB AAAB RRR R XXX II UB M",
+     "lsm_utilities3d_local.h:322"),
 ]
 
 SIGNATURES = {name: kinds.replace(" ", "") for name, kinds, _ in _TABLE}
@@ -138,6 +193,46 @@ MACRO_OF = {
     "lsm3dhomogeneousneumanneno2_": "LSM3D_HOMOGENEOUS_NEUMANN_ENO2",
     "lsm3dhomogeneousneumanneno3_": "LSM3D_HOMOGENEOUS_NEUMANN_ENO3",
     "lsm3dhomogeneousneumannweno5_": "LSM3D_HOMOGENEOUS_NEUMANN_WENO5",
+    "lsm3ddeterminenarrowband_": "LSM3D_DETERMINE_NARROW_BAND",
+    "lsm3dmarknarrowbandboundarylayer_": "LSM3D_MARK_NARROW_BAND_BOUNDARY_LAYER",
+    "lsm3ddeterminenarrowbandfrommask_": "LSM3D_DETERMINE_NARROW_BAND_FROM_MASK",
+    "lsm3dmultiplycutofflserhslocal_": "LSM3D_MULTIPLY_CUT_OFF_LSE_RHS_LOCAL",
+    "lsm3dcheckouternarrowbandlayer_": "LSM3D_CHECK_OUTER_NARROW_BAND_LAYER",
+    "lsm3dimposemasklocal_": "LSM3D_IMPOSE_MASK_LOCAL",
+    "lsm3dcopydatalocal_": "LSM3D_COPY_DATA_LOCAL",
+    "lsm3dhjeno1local_": "LSM3D_HJ_ENO1_LOCAL",
+    "lsm3dhjeno2local_": "LSM3D_HJ_ENO2_LOCAL",
+    "lsm3dcentralgradorder2local_": "LSM3D_CENTRAL_GRAD_ORDER2_LOCAL",
+    "lsm3dcentralgradorder4local_": "LSM3D_CENTRAL_GRAD_ORDER4_LOCAL",
+    "lsm3dlaplacianorder2local_": "LSM3D_LAPLACIAN_ORDER2_LOCAL",
+    "lsm3dcomputeavegradphilocal_": "LSM3D_COMPUTE_AVE_GRAD_PHI_LOCAL",
+    "lsm3dgradientmagnitudelocal_": "LSM3D_GRADIENT_MAGNITUDE_LOCAL",
+    "lsm3drk1steplocal_": "LSM3D_RK1_STEP_LOCAL",
+    "lsm3dtvdrk2stage1local_": "LSM3D_TVD_RK2_STAGE1_LOCAL",
+    "lsm3dtvdrk2stage2local_": "LSM3D_TVD_RK2_STAGE2_LOCAL",
+    "lsm3dtvdrk3stage1local_": "LSM3D_TVD_RK3_STAGE1_LOCAL",
+    "lsm3dtvdrk3stage2local_": "LSM3D_TVD_RK3_STAGE2_LOCAL",
+    "lsm3dtvdrk3stage3local_": "LSM3D_TVD_RK3_STAGE3_LOCAL",
+    "lsm3dzerooutlevelseteqnrhslocal_": "LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS_LOCAL",
+    "lsm3daddadvectiontermtolserhslocal_": "LSM3D_ADD_ADVECTION_TERM_TO_LSE_RHS_LOCAL",
+    "lsm3daddnormalveltermtolserhslocal_": "LSM3D_ADD_NORMAL_VEL_TERM_TO_LSE_RHS_LOCAL",
+    "lsm3daddconstnormalveltermtolserhslocal_": "LSM3D_ADD_CONST_NORMAL_VEL_TERM_TO_LSE_RHS_LOCAL",
+    "lsm3daddconstcurvtermtolserhslocal_": "LSM3D_ADD_CONST_CURV_TERM_TO_LSE_RHS_LOCAL",
+    "lsm3daddconstprecomputedcurvtermtolserhslocal_": "LSM3D_ADD_CONST_PRECOMPUTED_CURV_TERM_TO_LSE_RHS_LOCAL",
+    "lsm3daddexternalandnormalveltermtolserhslocal_": "LSM3D_ADD_EXTERNAL_AND_NORMAL_VEL_TERM_TO_LSE_RHS_LOCAL",
+    "lsm3dcomputereinitializationeqnrhslocal_": "LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS_LOCAL",
+    "lsm3dcomputeorthogonalizationeqnrhslocal_": "LSM3D_COMPUTE_ORTHOGONALIZATION_EQN_RHS_LOCAL",
+    "lsm3dmaxnormdifflocal_": "LSM3D_MAX_NORM_DIFF_LOCAL",
+    "lsm3dcomputestableadvectiondtlocal_": "LSM3D_COMPUTE_STABLE_ADVECTION_DT_LOCAL",
+    "lsm3dcomputestablenormalveldtlocal_": "LSM3D_COMPUTE_STABLE_NORMAL_VEL_DT_LOCAL",
+    "lsm3dcomputestablenormalveldtcontrolvolumelocal_":
+        "LSM3D_COMPUTE_STABLE_NORMAL_VEL_DT_CONTROL_VOLUME_LOCAL",
+    "lsm3dcomputestableconstnormalveldtlocal_": "LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT_LOCAL",
 }
+
+# the narrow-band routines (index lists and marks), a subset of SIGNATURES,
+# and the box routines (the rest)
+LOCAL = {n for n, k in SIGNATURES.items() if "X" in k or "U" in k}
+BOX_ROUTINES = sorted(set(SIGNATURES) - LOCAL)
 
 assert set(MACRO_OF) == set(SIGNATURES)

@@ -1305,6 +1305,634 @@ void oracle_lsm3dcomputefieldextensioneqnrhs_(real *rhs, BOXARGS(r), const real 
 }
 
 /* ---------------------------------------------------------------------- */
+/* Narrow band (localization): lsm_localization3d.f and the *LOCAL kernels  */
+/* Index lists index_x/y/z(nlo:nhi) and the integer*1 narrow_band marks     */
+/* (signed in the Fortran; compared as signed char here).                   */
+/* ---------------------------------------------------------------------- */
+typedef signed char nb_t;
+#define LIDX(p, l, nlo) (p)[(l) - (nlo)]
+
+/* lsm_localization3d.f:48-168; an empty previous level (n_lo = n_hi = -1)
+ * has no parents here (the reference would read index -1) */
+static void mark_nb_neighbors(nb_t *nb, box_t bn, int *ix, int *iy, int *iz, int nlo, int *n_lo, int *n_hi,
+                              int level) {
+  int count = n_hi[0] + 1;
+  for (int l = 1; l <= level; l++) {
+    nb_t mark = (nb_t)(l + 1);
+    if (n_lo[l - 1] >= nlo)
+      for (int m = n_lo[l - 1]; m <= n_hi[l - 1]; m++) {
+        int i = LIDX(ix, m, nlo), j = LIDX(iy, m, nlo), k = LIDX(iz, m, nlo);
+        const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, 1, -1};
+        for (int q = 0; q < 6; q++) {
+          int a = i + di[q], b = j + dj[q], c = k + dk[q];
+          int inside = (q == 0) ? (i < bn.ihi) : (q == 1) ? (i > bn.ilo) : (q == 2) ? (j < bn.jhi)
+                     : (q == 3) ? (j > bn.jlo) : (q == 4) ? (k < bn.khi) : (k > bn.klo);
+          if (inside && A(nb, bn, a, b, c) == 0) {
+            LIDX(ix, count, nlo) = a; LIDX(iy, count, nlo) = b; LIDX(iz, count, nlo) = c;
+            A(nb, bn, a, b, c) = mark;
+            count++;
+          }
+        }
+      }
+    if (count > n_hi[l - 1] + 1) { n_lo[l] = n_hi[l - 1] + 1; n_hi[l] = count - 1; }
+    else { n_lo[l] = -1; n_hi[l] = -1; }
+  }
+}
+
+void oracle_lsm3ddeterminenarrowband_(const real *phi, BOXARGS(g), nb_t *nb, BOXARGS(nb), int *ix, int *iy,
+                                      int *iz, const int *nlo_index, const int *nhi_index, int *n_lo, int *n_hi,
+                                      int *index_outer, const int *nlo_index_outer, const int *nhi_index_outer,
+                                      int *nlo_outer_plus, int *nhi_outer_plus, int *nlo_outer_minus,
+                                      int *nhi_outer_minus, const real *width, const real *width_inner,
+                                      const int *level) {
+  /* lsm_localization3d.f:174-298 */
+  box_t bg = MKBOX(g), bn = MKBOX(nb);
+  (void)nhi_index;
+  int nlo = *nlo_index, count = nlo, no = *nlo_index_outer;
+  n_lo[0] = nlo;
+  int cm = *nlo_index_outer, cp = *nhi_index_outer;
+  *nlo_outer_minus = *nlo_index_outer;
+  *nhi_outer_plus = *nhi_index_outer;
+  for (int k = bg.klo; k <= bg.khi; k++)
+    for (int j = bg.jlo; j <= bg.jhi; j++)
+      for (int i = bg.ilo; i <= bg.ihi; i++) {
+        real p = A(phi, bg, i, j, k), a = (real)fabs(p);
+        if (a < *width) {
+          LIDX(ix, count, nlo) = i; LIDX(iy, count, nlo) = j; LIDX(iz, count, nlo) = k;
+          A(nb, bn, i, j, k) = 1;
+          if (a >= *width_inner) {
+            if (p <= 0.0) { index_outer[cm - no] = count; cm++; }
+            else { index_outer[cp - no] = count; cp--; }
+          }
+          count++;
+        } else {
+          A(nb, bn, i, j, k) = 0;
+        }
+      }
+  if (count > nlo) {
+    n_hi[0] = count - 1;
+    *nhi_outer_minus = cm - 1;
+    *nlo_outer_plus = cp + 1;
+    mark_nb_neighbors(nb, bn, ix, iy, iz, nlo, n_lo, n_hi, *level);
+  } else {
+    n_hi[0] = count;
+    *nhi_outer_minus = cm;
+    *nlo_outer_plus = cp;
+  }
+}
+
+void oracle_lsm3ddeterminenarrowbandfrommask_(const real *mask, BOXARGS(g), nb_t *nb, BOXARGS(nb), int *ix,
+                                              int *iy, int *iz, const int *nlo_index, const int *nhi_index,
+                                              int *n_lo, int *n_hi, const int *level, const int *use_mask_sign) {
+  /* lsm_localization3d.f:385-476 */
+  box_t bg = MKBOX(g), bn = MKBOX(nb);
+  (void)nhi_index;
+  int nlo = *nlo_index, count = nlo;
+  n_lo[0] = nlo;
+  for (int k = bg.klo; k <= bg.khi; k++)
+    for (int j = bg.jlo; j <= bg.jhi; j++)
+      for (int i = bg.ilo; i <= bg.ihi; i++) {
+        real m = A(mask, bg, i, j, k);
+        int in = (*use_mask_sign > 0) ? (m >= 0) : (m <= 0);
+        if (in) {
+          LIDX(ix, count, nlo) = i; LIDX(iy, count, nlo) = j; LIDX(iz, count, nlo) = k;
+          A(nb, bn, i, j, k) = 1;
+          count++;
+        } else {
+          A(nb, bn, i, j, k) = 0;
+        }
+      }
+  n_hi[0] = count - 1;
+  mark_nb_neighbors(nb, bn, ix, iy, iz, nlo, n_lo, n_hi, *level);
+}
+
+void oracle_lsm3dmarknarrowbandboundarylayer_(nb_t *nb, BOXARGS(g), BOXARGS(fb), const nb_t *mark) {
+  /* lsm_localization3d.f:303-378: the six fill-box face layers, each across the ghost box */
+  box_t bg = MKBOX(g);
+  for (int k = bg.klo; k <= bg.khi; k++)
+    for (int j = bg.jlo; j <= bg.jhi; j++)
+      for (int i = bg.ilo; i <= bg.ihi; i++)
+        if (i == *ilo_fb || i == *ihi_fb || j == *jlo_fb || j == *jhi_fb || k == *klo_fb || k == *khi_fb)
+          A(nb, bg, i, j, k) = *mark;
+}
+
+/* loop over index_*(nlo..nhi); (i, j, k) of entry l */
+#define FOR_LIST(nlo_p, nhi_p)                                                 \
+  for (int l = *(nlo_p); l <= *(nhi_p); l++) {                                 \
+    int i = LIDX(ix, l, *(nlo_p)), j = LIDX(iy, l, *(nlo_p)), k = LIDX(iz, l, *(nlo_p));
+#define END_LIST }
+#define XYZ const int *ix, const int *iy, const int *iz, const int *nlo_index, const int *nhi_index
+#define NB const nb_t *nb, BOXARGS(nb)
+
+void oracle_lsm3dmultiplycutofflserhslocal_(const real *phi, real *rhs, BOXARGS(g), XYZ, NB, const nb_t *mark_fb,
+                                            const real *beta, const real *gamma) {
+  /* lsm_localization3d.f:482-562 */
+  box_t bg = MKBOX(g), bn = MKBOX(nb);
+  const real b = *beta, gm = *gamma;
+  real c1 = gm - 3 * b;
+  real c2 = (gm - b);
+  c2 = c2 * c2 * c2;
+  FOR_LIST(nlo_index, nhi_index)
+  if (A(nb, bn, i, j, k) <= *mark_fb) {
+    real a = (real)fabs(A(phi, bg, i, j, k)), cut;
+    if (a <= b) cut = 1;
+    else if (a <= gm) {
+      real t = (a - gm);
+      cut = (t * t * (2 * a + c1)) / c2;
+    } else cut = 0;
+    A(rhs, bg, i, j, k) = cut * A(rhs, bg, i, j, k);
+  }
+  END_LIST
+}
+
+void oracle_lsm3dcheckouternarrowbandlayer_(int *change_sign, const real *phi, BOXARGS(g), XYZ,
+                                            const int *index_outer, const int *nlo_outer, const int *nhi_outer,
+                                            const int *nlo_plus, const int *nhi_plus, const int *nlo_minus,
+                                            const int *nhi_minus) {
+  /* lsm_localization3d.f:568-654 */
+  box_t bg = MKBOX(g);
+  (void)nhi_index; (void)nhi_outer;
+  int nlo = *nlo_index, no = *nlo_outer;
+  *change_sign = 0;
+  for (int pass = 0; pass < 2 && !*change_sign; pass++) {
+    int lo = pass ? *nlo_minus : *nlo_plus, hi = pass ? *nhi_minus : *nhi_plus;
+    int sp = 0, sm = 0;
+    for (int m = lo; m <= hi; m++) {
+      int l = index_outer[m - no];
+      real p = A(phi, bg, LIDX(ix, l, nlo), LIDX(iy, l, nlo), LIDX(iz, l, nlo));
+      if (p > 0.0) sp = 1; else sm = 1;
+      if (sp && sm) { *change_sign = 1; break; }
+    }
+  }
+}
+
+void oracle_lsm3dimposemasklocal_(real *dest, BOXARGS(d), const real *src, BOXARGS(s), const real *mask,
+                                  BOXARGS(m), XYZ) {
+  /* lsm_localization3d.f:660-718: max(mask, src) */
+  box_t bd = MKBOX(d), bs = MKBOX(s), bm = MKBOX(m);
+  FOR_LIST(nlo_index, nhi_index)
+  real a = A(mask, bm, i, j, k), b = A(src, bs, i, j, k);
+  A(dest, bd, i, j, k) = (a > b) ? a : b;
+  END_LIST
+}
+
+void oracle_lsm3dcopydatalocal_(real *dest, BOXARGS(d), const real *src, BOXARGS(s), XYZ) {
+  /* lsm_localization3d.f:724-776 */
+  box_t bd = MKBOX(d), bs = MKBOX(s);
+  FOR_LIST(nlo_index, nhi_index)
+  A(dest, bd, i, j, k) = A(src, bs, i, j, k);
+  END_LIST
+}
+
+/* TVD Runge-Kutta on the band (lsm_tvd_runge_kutta3d_local.f) */
+#define RK_LOCAL_1(name)                                                                              \
+  void oracle_##name(real *un, BOXARGS(un), const real *uc, BOXARGS(uc), const real *rhs, BOXARGS(r), \
+                     const real *dt, XYZ, NB, const nb_t *mark_fb) {                                  \
+    box_t bu = MKBOX(un), bc = MKBOX(uc), br = MKBOX(r), bn = MKBOX(nb);                              \
+    FOR_LIST(nlo_index, nhi_index)                                                                    \
+    if (A(nb, bn, i, j, k) <= *mark_fb) A(un, bu, i, j, k) = A(uc, bc, i, j, k) + *dt * A(rhs, br, i, j, k); \
+    END_LIST                                                                                          \
+  }
+RK_LOCAL_1(lsm3drk1steplocal_)          /* :34-131 */
+RK_LOCAL_1(lsm3dtvdrk2stage1local_)     /* :137-237 (calls the RK1 step) */
+RK_LOCAL_1(lsm3dtvdrk3stage1local_)     /* :357-459 */
+
+void oracle_lsm3dtvdrk2stage2local_(real *un, BOXARGS(un), const real *u1, BOXARGS(u1), const real *uc,
+                                    BOXARGS(uc), const real *rhs, BOXARGS(r), const real *dt, XYZ, NB,
+                                    const nb_t *mark_fb) {
+  /* :243-351 */
+  box_t bu = MKBOX(un), b1 = MKBOX(u1), bc = MKBOX(uc), br = MKBOX(r), bn = MKBOX(nb);
+  FOR_LIST(nlo_index, nhi_index)
+  if (A(nb, bn, i, j, k) <= *mark_fb)
+    A(un, bu, i, j, k) = 0.5 * (A(uc, bc, i, j, k) + A(u1, b1, i, j, k) + *dt * A(rhs, br, i, j, k));
+  END_LIST
+}
+
+void oracle_lsm3dtvdrk3stage2local_(real *u2, BOXARGS(u2), const real *u1, BOXARGS(u1), const real *uc,
+                                    BOXARGS(uc), const real *rhs, BOXARGS(r), const real *dt, XYZ, NB,
+                                    const nb_t *mark_fb) {
+  /* :465-569 */
+  box_t bu = MKBOX(u2), b1 = MKBOX(u1), bc = MKBOX(uc), br = MKBOX(r), bn = MKBOX(nb);
+  FOR_LIST(nlo_index, nhi_index)
+  if (A(nb, bn, i, j, k) <= *mark_fb)
+    A(u2, bu, i, j, k) = 0.75 * A(uc, bc, i, j, k) + 0.25 * (A(u1, b1, i, j, k) + *dt * A(rhs, br, i, j, k));
+  END_LIST
+}
+
+void oracle_lsm3dtvdrk3stage3local_(real *un, BOXARGS(un), const real *u2, BOXARGS(u2), const real *uc,
+                                    BOXARGS(uc), const real *rhs, BOXARGS(r), const real *dt, XYZ, NB,
+                                    const nb_t *mark_fb) {
+  /* :575-667; one_third, two_thirds are real variables set from d0 literals */
+  box_t bu = MKBOX(un), b2 = MKBOX(u2), bc = MKBOX(uc), br = MKBOX(r), bn = MKBOX(nb);
+  const real one_third = 1.0 / 3.0, two_thirds = 2.0 / 3.0;
+  FOR_LIST(nlo_index, nhi_index)
+  if (A(nb, bn, i, j, k) <= *mark_fb)
+    A(un, bu, i, j, k) = one_third * A(uc, bc, i, j, k) + two_thirds * (A(u2, b2, i, j, k) + *dt * A(rhs, br, i, j, k));
+  END_LIST
+}
+
+void oracle_lsm3dzerooutlevelseteqnrhslocal_(real *rhs, BOXARGS(r), XYZ) {
+  /* lsm_level_set_evolution3d_local.f.in:16-55 */
+  box_t br = MKBOX(r);
+  FOR_LIST(nlo_index, nhi_index)
+  A(rhs, br, i, j, k) = (real)0.0;
+  END_LIST
+}
+
+void oracle_lsm3daddconstnormalveltermtolserhslocal_(real *rhs, BOXARGS(r), const real *xp, const real *yp,
+                                                     const real *zp, BOXARGS(gp), const real *xm, const real *ym,
+                                                     const real *zm, BOXARGS(gm), const real *vn, XYZ, NB,
+                                                     const nb_t *mark_fb) {
+  /* :282-398 (the Godunov norm is formed for every band cell; the term is
+   * added when |vn| >= zero_tol) */
+  box_t br = MKBOX(r), bp = MKBOX(gp), bm = MKBOX(gm), bn = MKBOX(nb);
+  real v = *vn;
+  FOR_LIST(nlo_index, nhi_index)
+  if (A(nb, bn, i, j, k) <= *mark_fb) {
+    real n2 = godunov_norm_sq(v, A(xp, bp, i, j, k), A(yp, bp, i, j, k), A(zp, bp, i, j, k), A(xm, bm, i, j, k),
+                              A(ym, bm, i, j, k), A(zm, bm, i, j, k));
+    if (fabs(v) >= ZERO_TOL) A(rhs, br, i, j, k) = A(rhs, br, i, j, k) - v * (real)sqrt(n2);
+  }
+  END_LIST
+}
+
+void oracle_lsm3daddconstcurvtermtolserhslocal_(real *rhs, BOXARGS(r), const real *px, const real *py,
+                                                const real *pz, BOXARGS(g1), const real *pxx, const real *pxy,
+                                                const real *pxz, const real *pyy, const real *pyz,
+                                                const real *pzz, BOXARGS(g2), const real *b, XYZ, NB,
+                                                const nb_t *mark_fb) {
+  /* :404-524, the per-cell term of lsm3dAddConstCurvTermToLSERHS */
+  box_t br = MKBOX(r), b1 = MKBOX(g1), b2 = MKBOX(g2), bn = MKBOX(nb);
+  FOR_LIST(nlo_index, nhi_index)
+  if (A(nb, bn, i, j, k) <= *mark_fb) {
+    real x = A(px, b1, i, j, k), y = A(py, b1, i, j, k), z = A(pz, b1, i, j, k);
+    real xx = A(pxx, b2, i, j, k), xy = A(pxy, b2, i, j, k), xz = A(pxz, b2, i, j, k);
+    real yy = A(pyy, b2, i, j, k), yz = A(pyz, b2, i, j, k), zz = A(pzz, b2, i, j, k);
+    real g2 = x * x + y * y + z * z, curv;
+    if (g2 < ZERO_TOL) curv = (real)0.0;
+    else {
+      curv = xx * y * y + yy * x * x - (real)2 * xy * x * y + xx * z * z + zz * x * x -
+             (real)2 * xz * x * z + yy * z * z + zz * y * y - (real)2 * yz * y * z;
+      curv = curv / g2;
+    }
+    A(rhs, br, i, j, k) = A(rhs, br, i, j, k) + *b * curv;
+  }
+  END_LIST
+}
+
+/* lsm3dComputeDnLOCAL, lsm_spatial_derivatives3d_local.f.in:43-164 */
+static void compute_dn_local(real *Dn, box_t bd, const real *Dm, box_t bm, int n, int dir, const int *ix,
+                             const int *iy, const int *iz, int nlo_list, int lo, int hi, const nb_t *nb,
+                             box_t bn, nb_t mark) {
+  int off[3] = {0, 0, 0};
+  real sgn;
+  if (n % 2 == 1) { off[dir] = 1; sgn = (real)1.0; }
+  else { off[dir] = -1; sgn = (real)-1.0; }
+  for (int l = lo; l <= hi; l++) {
+    int i = LIDX(ix, l, nlo_list), j = LIDX(iy, l, nlo_list), k = LIDX(iz, l, nlo_list);
+    if (A(nb, bn, i, j, k) <= mark)
+      A(Dn, bd, i, j, k) = sgn * (A(Dm, bm, i, j, k) - A(Dm, bm, i - off[0], j - off[1], k - off[2]));
+    else
+      A(Dn, bd, i, j, k) = (real)1.e10;
+  }
+}
+
+void oracle_lsm3dhjeno2local_(real *xp, real *yp, real *zp, BOXARGS(gp), real *xm, real *ym, real *zm,
+                              BOXARGS(gm), const real *phi, BOXARGS(phi), real *D1, BOXARGS(d1), real *D2,
+                              BOXARGS(d2), const real *dx, const real *dy, const real *dz, const int *ix,
+                              const int *iy, const int *iz, const int *nlo0, const int *nhi0, const int *nlo1,
+                              const int *nhi1, const int *nlo2, const int *nhi2, NB, const nb_t *mark_fb,
+                              const nb_t *mark_d1, const nb_t *mark_d2) {
+  /* lsm_spatial_derivatives3d_local.f.in:414-757: per direction, D1 on levels
+   * 0-2, D2 on levels 0-1, the ENO2 choice on level 0 */
+  box_t bp = MKBOX(gp), bm = MKBOX(gm), bphi = MKBOX(phi), b1 = MKBOX(d1), b2 = MKBOX(d2), bn = MKBOX(nb);
+  (void)nlo1; (void)nlo2;
+  const real half = (real)0.5;
+  real inv[3];
+  inv[0] = 1.0 / *dx; inv[1] = 1.0 / *dy; inv[2] = 1.0 / *dz;
+  real *outp[3] = {xp, yp, zp}, *outm[3] = {xm, ym, zm};
+  int nlo = *nlo0;
+  for (int d = 0; d < 3; d++) {
+    int a = DI(d), b = DJ(d), c = DK(d);
+    compute_dn_local(D1, b1, phi, bphi, 1, d, ix, iy, iz, nlo, *nlo0, *nhi2, nb, bn, *mark_d1);
+    compute_dn_local(D2, b2, D1, b1, 2, d, ix, iy, iz, nlo, *nlo0, *nhi1, nb, bn, *mark_d2);
+    for (int l = *nlo0; l <= *nhi0; l++) {
+      int i = LIDX(ix, l, nlo), j = LIDX(iy, l, nlo), k = LIDX(iz, l, nlo);
+      if (A(nb, bn, i, j, k) <= *mark_fb) {
+        real d2m = A(D2, b2, i - a, j - b, k - c), d20 = A(D2, b2, i, j, k), d2p = A(D2, b2, i + a, j + b, k + c);
+        if (fabs(d20) < fabs(d2p)) A(outp[d], bp, i, j, k) = (A(D1, b1, i + a, j + b, k + c) - half * d20) * inv[d];
+        else A(outp[d], bp, i, j, k) = (A(D1, b1, i + a, j + b, k + c) - half * d2p) * inv[d];
+        if (fabs(d2m) < fabs(d20)) A(outm[d], bm, i, j, k) = (A(D1, b1, i, j, k) + half * d2m) * inv[d];
+        else A(outm[d], bm, i, j, k) = (A(D1, b1, i, j, k) + half * d20) * inv[d];
+      }
+    }
+  }
+}
+
+void oracle_lsm3dcentralgradorder2local_(real *gx, real *gy, real *gz, BOXARGS(g), const real *phi, BOXARGS(phi),
+                                         const real *dx, const real *dy, const real *dz, XYZ, NB,
+                                         const nb_t *mark_fb) {
+  /* lsm_spatial_derivatives3d_local.f.in:763-862 */
+  box_t bg = MKBOX(g), bp = MKBOX(phi), bn = MKBOX(nb);
+  real fx = 0.5 / *dx, fy = 0.5 / *dy, fz = 0.5 / *dz;
+  FOR_LIST(nlo_index, nhi_index)
+  if (A(nb, bn, i, j, k) <= *mark_fb) {
+    A(gx, bg, i, j, k) = (A(phi, bp, i + 1, j, k) - A(phi, bp, i - 1, j, k)) * fx;
+    A(gy, bg, i, j, k) = (A(phi, bp, i, j + 1, k) - A(phi, bp, i, j - 1, k)) * fy;
+    A(gz, bg, i, j, k) = (A(phi, bp, i, j, k + 1) - A(phi, bp, i, j, k - 1)) * fz;
+  }
+  END_LIST
+}
+
+void oracle_lsm3dcomputeavegradphilocal_(real *ave, const real *phi, BOXARGS(phi), const real *dx, const real *dy,
+                                         const real *dz, XYZ, NB, const nb_t *mark_fb) {
+  /* lsm_spatial_derivatives3d_local.f.in:1089-1180: the sum runs in list order */
+  box_t bp = MKBOX(phi), bn = MKBOX(nb);
+  real fx = 0.5 / *dx, fy = 0.5 / *dy, fz = 0.5 / *dz, s = (real)0.0;
+  int count = 0;
+  FOR_LIST(nlo_index, nhi_index)
+  if (A(nb, bn, i, j, k) <= *mark_fb) {
+    real x = (A(phi, bp, i + 1, j, k) - A(phi, bp, i - 1, j, k)) * fx;
+    real y = (A(phi, bp, i, j + 1, k) - A(phi, bp, i, j - 1, k)) * fy;
+    real z = (A(phi, bp, i, j, k + 1) - A(phi, bp, i, j, k - 1)) * fz;
+    s = s + (real)sqrt(x * x + y * y + z * z);
+    count++;
+  }
+  END_LIST
+  if (count > 0) s = s / (real)count;
+  *ave = s;
+}
+
+void oracle_lsm3dmaxnormdifflocal_(real *out, const real *f1, BOXARGS(f1), const real *f2, BOXARGS(f2), XYZ, NB,
+                                   const nb_t *mark_fb) {
+  /* lsm_utilities3d_local.f.in:16-92 */
+  box_t b1 = MKBOX(f1), b2 = MKBOX(f2), bn = MKBOX(nb);
+  real m = (real)0.0;
+  FOR_LIST(nlo_index, nhi_index)
+  if (A(nb, bn, i, j, k) <= *mark_fb) {
+    real d = (real)fabs(A(f1, b1, i, j, k) - A(f2, b2, i, j, k));
+    if (d > m) m = d;
+  }
+  END_LIST
+  *out = m;
+}
+
+void oracle_lsm3dcomputestableconstnormalveldtlocal_(real *dt, const real *vn, const real *xp, const real *yp,
+                                                     const real *zp, BOXARGS(gp), const real *xm,
+                                                     const real *ym, const real *zm, BOXARGS(gm),
+                                                     const real *dx, const real *dy, const real *dz,
+                                                     const real *cfl, XYZ, NB, const nb_t *mark_fb) {
+  /* lsm_utilities3d_local.f.in:628-753.  The reference never assigns
+   * abs_vel_n and norm_grad_phi here (they are read uninitialised, :108);
+   * this takes the values of lsm3dComputeStableConstNormalVelDt (|vel_n| and
+   * sqrt(phi_x^2 + phi_y^2 + phi_z^2 + max_dx^2), lsm_utilities3d.f.in:520-550). */
+  box_t bp = MKBOX(gp), bm = MKBOX(gm), bn = MKBOX(nb);
+  real m = *dx;
+  if (*dy > m) m = *dy;
+  if (*dz > m) m = *dz;
+  real mdx2 = m * m, mx = (real)-1.0;
+  real ix_ = 1.0 / *dx, iy_ = 1.0 / *dy, iz_ = 1.0 / *dz, avn = (real)fabs(*vn);
+  FOR_LIST(nlo_index, nhi_index)
+  if (A(nb, bn, i, j, k) <= *mark_fb) {
+    real h = normal_h_over_dx(avn, A(xp, bp, i, j, k), A(yp, bp, i, j, k), A(zp, bp, i, j, k),
+                              A(xm, bm, i, j, k), A(ym, bm, i, j, k), A(zm, bm, i, j, k), mdx2, ix_, iy_, iz_);
+    if (h > mx) mx = h;
+  }
+  END_LIST
+  *dt = *cfl / (mx + TINY);
+}
+
+void oracle_lsm3dcomputereinitializationeqnrhslocal_(real *rhs, BOXARGS(r), const real *phi, BOXARGS(phi),
+                                                     const real *phi0, BOXARGS(p0), const real *xp, const real *yp,
+                                                     const real *zp, BOXARGS(gp), const real *xm, const real *ym,
+                                                     const real *zm, BOXARGS(gm), const real *dx, const real *dy,
+                                                     const real *dz, const int *use_phi0, XYZ, NB,
+                                                     const nb_t *mark_fb) {
+  /* lsm_reinitialization3d_local.f.in:50-320, the per-cell rule of
+   * lsm3dComputeReinitializationEqnRHS on the band */
+  box_t br = MKBOX(r), bphi = MKBOX(phi), b0 = MKBOX(p0), bp = MKBOX(gp), bm = MKBOX(gm), bn = MKBOX(nb);
+  real m = *dx;
+  if (*dy > m) m = *dy;
+  if (*dz > m) m = *dz;
+  real dx_sq = m * m;
+  FOR_LIST(nlo_index, nhi_index)
+  if (A(nb, bn, i, j, k) <= *mark_fb) {
+    real pc = (*use_phi0 == 1) ? A(phi0, b0, i, j, k) : A(phi, bphi, i, j, k);
+    real gp3[3] = {A(xp, bp, i, j, k), A(yp, bp, i, j, k), A(zp, bp, i, j, k)};
+    real gm3[3] = {A(xm, bm, i, j, k), A(ym, bm, i, j, k), A(zm, bm, i, j, k)}, gs[3];
+    godunov_star(pc, gp3, gm3, gs);
+    if (fabs(pc) >= ZERO_TOL) {
+      real n2 = gs[0] * gs[0] + gs[1] * gs[1] + gs[2] * gs[2];
+      real sgn = (*use_phi0 == 1) ? pc / (real)sqrt(pc * pc + dx_sq) : pc / (real)sqrt(pc * pc + n2 * dx_sq);
+      A(rhs, br, i, j, k) = sgn * ((real)1.0 - (real)sqrt(n2));
+    } else {
+      A(rhs, br, i, j, k) = (real)0.0;
+    }
+  }
+  END_LIST
+}
+
+/* The remaining *LOCAL kernels.  Where a local routine's per-cell rule is
+ * the box routine's (checked against the reference build), it is applied as
+ * that routine over the one-cell fill box (i,i,j,j,k,k) of each marked entry. */
+#define CELL_BOX &cb[0], &cb[1], &cb[2], &cb[3], &cb[4], &cb[5]
+#define FWD(p) ilo_##p, ihi_##p, jlo_##p, jhi_##p, klo_##p, khi_##p
+#define FOR_MARKED(nlo_p, nhi_p)                                                       \
+  box_t bn_ = MKBOX(nb);                                                               \
+  FOR_LIST(nlo_p, nhi_p)                                                               \
+  if (A(nb, bn_, i, j, k) > *mark_fb) continue;                                        \
+  int cb[6] = {i, i, j, j, k, k};                                                      \
+  (void)cb;
+
+void oracle_lsm3dhjeno1local_(real *xp, real *yp, real *zp, BOXARGS(gp), real *xm, real *ym, real *zm,
+                              BOXARGS(gm), const real *phi, BOXARGS(phi), real *D1, BOXARGS(d1),
+                              const real *dx, const real *dy, const real *dz, const int *ix, const int *iy,
+                              const int *iz, const int *nlo0, const int *nhi0, const int *nlo1,
+                              const int *nhi1, NB, const nb_t *mark_fb, const nb_t *mark_d1) {
+  /* lsm_spatial_derivatives3d_local.f.in:170-404: per direction, D1 on
+   * levels 0-1, then plus = D1(i+1)/dx, minus = D1(i)/dx on level 0 */
+  box_t bp = MKBOX(gp), bm = MKBOX(gm), bphi = MKBOX(phi), b1 = MKBOX(d1), bn = MKBOX(nb);
+  (void)nlo1;
+  real inv[3];
+  inv[0] = 1.0 / *dx; inv[1] = 1.0 / *dy; inv[2] = 1.0 / *dz;
+  real *outp[3] = {xp, yp, zp}, *outm[3] = {xm, ym, zm};
+  int nlo = *nlo0;
+  for (int d = 0; d < 3; d++) {
+    int a = DI(d), b = DJ(d), c = DK(d);
+    compute_dn_local(D1, b1, phi, bphi, 1, d, ix, iy, iz, nlo, *nlo0, *nhi1, nb, bn, *mark_d1);
+    for (int l = *nlo0; l <= *nhi0; l++) {
+      int i = LIDX(ix, l, nlo), j = LIDX(iy, l, nlo), k = LIDX(iz, l, nlo);
+      if (A(nb, bn, i, j, k) <= *mark_fb) {
+        A(outp[d], bp, i, j, k) = A(D1, b1, i + a, j + b, k + c) * inv[d];
+        A(outm[d], bm, i, j, k) = A(D1, b1, i, j, k) * inv[d];
+      }
+    }
+  }
+}
+
+void oracle_lsm3dcentralgradorder4local_(real *gx, real *gy, real *gz, BOXARGS(g), const real *phi,
+                                         BOXARGS(phi), const real *dx, const real *dy, const real *dz, XYZ, NB,
+                                         const nb_t *mark_fb) {
+  /* lsm_spatial_derivatives3d_local.f.in:868-960 */
+  FOR_MARKED(nlo_index, nhi_index)
+  oracle_lsm3dcentralgradorder4_(gx, gy, gz, FWD(g), phi, FWD(phi), CELL_BOX, dx, dy, dz);
+  END_LIST
+}
+
+void oracle_lsm3dlaplacianorder2local_(real *lap, BOXARGS(l), const real *phi, BOXARGS(phi), const real *dx,
+                                       const real *dy, const real *dz, XYZ, NB, const nb_t *mark_fb) {
+  /* lsm_spatial_derivatives3d_local.f.in:966-1060 */
+  FOR_MARKED(nlo_index, nhi_index)
+  oracle_lsm3dlaplacianorder2_(lap, FWD(l), phi, FWD(phi), CELL_BOX, dx, dy, dz);
+  END_LIST
+}
+
+void oracle_lsm3dgradientmagnitudelocal_(const real *gx, const real *gy, const real *gz, real *mag, BOXARGS(g),
+                                         XYZ, NB, const nb_t *mark_fb) {
+  /* lsm_spatial_derivatives3d_local.f.in:1186-1262: like the box routine
+   * but with LSMLIB_ZERO_TOL in place of its 1e-8 */
+  box_t bg = MKBOX(g);
+  FOR_MARKED(nlo_index, nhi_index)
+  real x2 = A(gx, bg, i, j, k) * A(gx, bg, i, j, k);
+  real y2 = A(gy, bg, i, j, k) * A(gy, bg, i, j, k);
+  real z2 = A(gz, bg, i, j, k) * A(gz, bg, i, j, k);
+  if (x2 < ZERO_TOL) x2 = (real)0.0;
+  if (y2 < ZERO_TOL) y2 = (real)0.0;
+  if (z2 < ZERO_TOL) z2 = (real)0.0;
+  real t = (real)sqrt(x2 + y2 + z2);
+  A(mag, bg, i, j, k) = (t < ZERO_TOL) ? (real)0.0 : t;
+  END_LIST
+}
+
+void oracle_lsm3daddadvectiontermtolserhslocal_(real *rhs, BOXARGS(r), const real *px, const real *py,
+                                                const real *pz, BOXARGS(g), const real *vx, const real *vy,
+                                                const real *vz, BOXARGS(vel), XYZ, NB, const nb_t *mark_fb) {
+  /* lsm_level_set_evolution3d_local.f.in:60-146 */
+  FOR_MARKED(nlo_index, nhi_index)
+  oracle_lsm3daddadvectiontermtolserhs_(rhs, FWD(r), px, py, pz, FWD(g), vx, vy, vz, FWD(vel), CELL_BOX);
+  END_LIST
+}
+
+void oracle_lsm3daddnormalveltermtolserhslocal_(real *rhs, BOXARGS(r), const real *xp, const real *yp,
+                                                const real *zp, BOXARGS(gp), const real *xm, const real *ym,
+                                                const real *zm, BOXARGS(gm), const real *vn, BOXARGS(vel), XYZ,
+                                                NB, const nb_t *mark_fb) {
+  /* lsm_level_set_evolution3d_local.f.in:152-276 */
+  FOR_MARKED(nlo_index, nhi_index)
+  oracle_lsm3daddnormalveltermtolserhs_(rhs, FWD(r), xp, yp, zp, FWD(gp), xm, ym, zm, FWD(gm), vn, FWD(vel),
+                                        CELL_BOX);
+  END_LIST
+}
+
+void oracle_lsm3daddconstprecomputedcurvtermtolserhslocal_(real *rhs, BOXARGS(r), const real *kappa,
+                                                           BOXARGS(kap), const real *gmag, BOXARGS(gm),
+                                                           const real *b, XYZ, NB, const nb_t *mark_fb) {
+  /* lsm_level_set_evolution3d_local.f.in:530-606 */
+  box_t br = MKBOX(r), bk = MKBOX(kap), bg = MKBOX(gm);
+  FOR_MARKED(nlo_index, nhi_index)
+  A(rhs, br, i, j, k) = A(rhs, br, i, j, k) + *b * A(kappa, bk, i, j, k) * A(gmag, bg, i, j, k);
+  END_LIST
+}
+
+void oracle_lsm3daddexternalandnormalveltermtolserhslocal_(real *rhs, BOXARGS(r), const real *xp,
+                                                           const real *yp, const real *zp, BOXARGS(gp),
+                                                           const real *xm, const real *ym, const real *zm,
+                                                           BOXARGS(gm), const real *vn, const real *vx,
+                                                           const real *vy, const real *vz, BOXARGS(vel), XYZ,
+                                                           NB, const nb_t *mark_fb) {
+  /* lsm_level_set_evolution3d_local.f.in:612-800 */
+  box_t br = MKBOX(r), bp = MKBOX(gp), bm = MKBOX(gm), bv = MKBOX(vel);
+  FOR_MARKED(nlo_index, nhi_index)
+  real vnc = A(vn, bv, i, j, k), vxc = A(vx, bv, i, j, k);
+  real vyc = A(vy, bv, i, j, k), vzc = A(vz, bv, i, j, k);
+  real px = ext_normal_select(vxc, vnc, A(xp, bp, i, j, k), A(xm, bm, i, j, k));
+  real py = ext_normal_select(vyc, vnc, A(yp, bp, i, j, k), A(ym, bm, i, j, k));
+  real pz = ext_normal_select(vzc, vnc, A(zp, bp, i, j, k), A(zm, bm, i, j, k));
+  real n2 = px * px + py * py + pz * pz;
+  if (fabs(vnc) >= ZERO_TOL && n2 >= ZERO_TOL)
+    A(rhs, br, i, j, k) = A(rhs, br, i, j, k) - vxc * px - vyc * py - vzc * pz - vnc * (real)sqrt(n2);
+  else
+    A(rhs, br, i, j, k) = A(rhs, br, i, j, k) - vxc * px - vyc * py - vzc * pz;
+  END_LIST
+}
+
+void oracle_lsm3dcomputeorthogonalizationeqnrhslocal_(real *rhs, BOXARGS(r), const real *xp, const real *yp,
+                                                      const real *zp, BOXARGS(gp), const real *xm,
+                                                      const real *ym, const real *zm, BOXARGS(gm),
+                                                      const real *psi, BOXARGS(psi), const real *sxp,
+                                                      const real *syp, const real *szp, BOXARGS(sp),
+                                                      const real *sxm, const real *sym, const real *szm,
+                                                      BOXARGS(sm), const real *dx, const real *dy, const real *dz,
+                                                      XYZ, NB, const nb_t *mark_fb) {
+  /* lsm_reinitialization3d_local.f.in:326-470 */
+  FOR_MARKED(nlo_index, nhi_index)
+  oracle_lsm3dcomputeorthogonalizationeqnrhs_(rhs, FWD(r), xp, yp, zp, FWD(gp), xm, ym, zm, FWD(gm), psi,
+                                              FWD(psi), sxp, syp, szp, FWD(sp), sxm, sym, szm, FWD(sm), CELL_BOX,
+                                              dx, dy, dz);
+  END_LIST
+}
+
+void oracle_lsm3dcomputestableadvectiondtlocal_(real *dt, const real *vx, const real *vy, const real *vz,
+                                                BOXARGS(vel), const real *dx, const real *dy, const real *dz,
+                                                const real *cfl, XYZ, NB, const nb_t *mark_fb) {
+  /* lsm_utilities3d_local.f.in:194-285 */
+  box_t bv = MKBOX(vel);
+  real mx = (real)-1.0;
+  real ix_ = 1.0 / *dx, iy_ = 1.0 / *dy, iz_ = 1.0 / *dz;
+  FOR_MARKED(nlo_index, nhi_index)
+  real u = adv_u_over_dx(A(vx, bv, i, j, k), A(vy, bv, i, j, k), A(vz, bv, i, j, k), ix_, iy_, iz_);
+  if (u > mx) mx = u;
+  END_LIST
+  *dt = *cfl / (mx + TINY);
+}
+
+static void normal_dt_local(real *dt, const real *vn, box_t bv, const real *cv, box_t bc, int cv_sgn,
+                            const real *xp, const real *yp, const real *zp, box_t bp, const real *xm,
+                            const real *ym, const real *zm, box_t bm, const real *dx, const real *dy,
+                            const real *dz, const real *cfl, const int *ix, const int *iy, const int *iz,
+                            const int *nlo_index, const int *nhi_index, const nb_t *nb, box_t bn, nb_t mark_fb) {
+  /* lsm_utilities3d_local.f.in:290-426 and :432-622: the H/dX of the box
+   * routine, taken only where sqrt(...) > LSMLIB_ZERO_TOL */
+  real mdx = (real)fmax2(fmax2(*dx, *dy), *dz);
+  real max_dx_sq = mdx * mdx, mx = (real)-1.0;
+  real ix_ = 1.0 / *dx, iy_ = 1.0 / *dy, iz_ = 1.0 / *dz;
+  FOR_LIST(nlo_index, nhi_index)
+  if (A(nb, bn, i, j, k) <= mark_fb) {
+    if (cv) {
+      real c = A(cv, bc, i, j, k);
+      if (!((cv_sgn > 0) ? (c > 0.0) : (c < 0.0))) continue;
+    }
+    real px = (real)fmax2(fabs(A(xp, bp, i, j, k)), fabs(A(xm, bm, i, j, k)));
+    real py = (real)fmax2(fabs(A(yp, bp, i, j, k)), fabs(A(ym, bm, i, j, k)));
+    real pz = (real)fmax2(fabs(A(zp, bp, i, j, k)), fabs(A(zm, bm, i, j, k)));
+    real ng = (real)sqrt(px * px + py * py + pz * pz + max_dx_sq);
+    if (ng > ZERO_TOL) {
+      real h = (real)fabs(A(vn, bv, i, j, k)) / ng * (px * ix_ + py * iy_ + pz * iz_);
+      if (h > mx) mx = h;
+    }
+  }
+  END_LIST
+  *dt = *cfl / (mx + TINY);
+}
+
+void oracle_lsm3dcomputestablenormalveldtlocal_(real *dt, const real *vn, BOXARGS(vel), const real *xp,
+                                                const real *yp, const real *zp, BOXARGS(gp), const real *xm,
+                                                const real *ym, const real *zm, BOXARGS(gm), const real *dx,
+                                                const real *dy, const real *dz, const real *cfl, XYZ, NB,
+                                                const nb_t *mark_fb) {
+  normal_dt_local(dt, vn, MKBOX(vel), NULL, MKBOX(vel), 0, xp, yp, zp, MKBOX(gp), xm, ym, zm, MKBOX(gm), dx, dy,
+                  dz, cfl, ix, iy, iz, nlo_index, nhi_index, nb, MKBOX(nb), *mark_fb);
+}
+
+void oracle_lsm3dcomputestablenormalveldtcontrolvolumelocal_(
+    real *dt, const real *vn, BOXARGS(vel), const real *xp, const real *yp, const real *zp, BOXARGS(gp),
+    const real *xm, const real *ym, const real *zm, BOXARGS(gm), const real *cv, BOXARGS(cv), const int *cv_sgn,
+    const real *dx, const real *dy, const real *dz, const real *cfl, XYZ, NB, const nb_t *mark_fb) {
+  normal_dt_local(dt, vn, MKBOX(vel), cv, MKBOX(cv), *cv_sgn, xp, yp, zp, MKBOX(gp), xm, ym, zm, MKBOX(gm), dx,
+                  dy, dz, cfl, ix, iy, iz, nlo_index, nhi_index, nb, MKBOX(nb), *mark_fb);
+}
+
+/* ---------------------------------------------------------------------- */
 /* Boundary conditions  lsm_boundary_conditions3d.f                         */
 /* bdry_location_idx: 0 x_lo, 1 x_hi, 2 y_lo, 3 y_hi, 4 z_lo, 5 z_hi (:13-20) */
 /* Each face loops over the full ghost box in the two other directions.     */

@@ -5,7 +5,9 @@ oracle/Makefile).
 Inputs are regenerated deterministically by tests/lsm_cases.py (seed 0, fill
 box 6x5x4 with uneven ghost boxes); each fixture stores a SHA-256 of the
 inputs, the output values that changed (flat index + value per array
-argument) and the returned scalars.  Output: tests/golden/toolbox_<real>.npz.
+argument) and the returned scalars.  Output: tests/golden/toolbox_<real>.npz
+(box routines) and tests/golden/local_<real>.npz (narrow-band routines,
+inputs from tests/lsm_local_cases.py).
 
     python tests/golden/make_golden.py
 """
@@ -21,8 +23,17 @@ sys.path.insert(0, str(REPO))
 sys.path.insert(0, str(REPO / "tests"))
 
 from lsm_cases import make_case  # noqa: E402
-from lsmlib_amd.signatures import SIGNATURES  # noqa: E402
+from lsm_local_cases import make_local_case  # noqa: E402
+from lsmlib_amd.signatures import BOX_ROUTINES, LOCAL  # noqa: E402
 from oracle.oracle import ref_lib  # noqa: E402
+
+# the reference reads two uninitialised variables in this routine
+# (lsm_utilities3d_local.f.in:728): its output is no golden value
+UNDEFINED_IN_REFERENCE = {"lsm3dcomputestableconstnormalveldtlocal_"}
+# seeds 2 and 3 reach further branches only in these routines
+LOCAL_SEEDS = {"lsm3ddeterminenarrowband_": (0, 1, 2, 3), "lsm3dcheckouternarrowbandlayer_": (0, 1, 2),
+               "lsm3daddconstnormalveltermtolserhslocal_": (0, 1, 2, 3),
+               "lsm3dmarknarrowbandboundarylayer_": (0, 1, 2)}
 
 GOLDEN_FB = (-2, 3, 1, 5, 3, 6)
 
@@ -46,6 +57,12 @@ def golden_cases(name, real):
     return out
 
 
+def golden_local_cases(name, real):
+    """The narrow-band cases (bands built by the oracle's restatement of
+    LSM3D_DETERMINE_NARROW_BAND, itself checked against these vectors)."""
+    return [make_local_case(name, real, s) for s in LOCAL_SEEDS.get(name, (0, 1))]
+
+
 def input_digest(case):
     h = hashlib.sha256()
     for a in case.args:
@@ -61,23 +78,28 @@ def main():
         lib = ref_lib(real)
         if lib is None:
             raise SystemExit("oracle/_ref is not built: make -C oracle ref")
-        store = {}
-        for name in sorted(SIGNATURES):
-            for ci, case in enumerate(golden_cases(name, real)):
-                key = f"{name}#{ci}"
-                before = case.copy()
-                ret = lib.call(name, *case.args)
-                store[f"{key}/digest"] = np.frombuffer(bytes.fromhex(input_digest(before)), dtype=np.uint8)
-                for (i, b), (_, a) in zip(before.arrays(), case.arrays()):
-                    changed = np.flatnonzero(a.ravel() != b.ravel()).astype(np.int32)
-                    if changed.size:
-                        store[f"{key}/arg{i}/idx"] = changed
-                        store[f"{key}/arg{i}/val"] = a.ravel()[changed]
-                if ret is not None:
-                    store[f"{key}/ret"] = np.atleast_1d(np.asarray(ret, dtype=np.float64))
-        path = HERE / f"toolbox_{real}.npz"
-        np.savez_compressed(path, **store)
-        print(path, path.stat().st_size, "bytes")
+        for fname, names, cases_of in (("toolbox", BOX_ROUTINES, golden_cases),
+                                       ("local", sorted(LOCAL - UNDEFINED_IN_REFERENCE), golden_local_cases)):
+            store = {}
+            for name in names:
+                for ci, case in enumerate(cases_of(name, real)):
+                    record(store, lib, f"{name}#{ci}", name, case)
+            path = HERE / f"{fname}_{real}.npz"
+            np.savez_compressed(path, **store)
+            print(path, path.stat().st_size, "bytes")
+
+
+def record(store, lib, key, name, case):
+    before = case.copy()
+    ret = lib.call(name, *case.args)
+    store[f"{key}/digest"] = np.frombuffer(bytes.fromhex(input_digest(before)), dtype=np.uint8)
+    for (i, b), (_, a) in zip(before.arrays(), case.arrays()):
+        changed = np.flatnonzero(a.ravel() != b.ravel()).astype(np.int32)
+        if changed.size:
+            store[f"{key}/arg{i}/idx"] = changed
+            store[f"{key}/arg{i}/val"] = a.ravel()[changed]
+    if ret is not None:
+        store[f"{key}/ret"] = np.atleast_1d(np.asarray(ret, dtype=np.float64))
 
 
 if __name__ == "__main__":

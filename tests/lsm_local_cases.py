@@ -1,0 +1,227 @@
+"""Seeded argument sets for the narrow-band routines (localization and *LOCAL).
+
+The band is built the way the reference's local curvature example builds it
+(examples/serial/curvature_example/curvature_model3d_local.c:190-229):
+LSM3D_DETERMINE_NARROW_BAND of a sphere's signed distance with width gamma
+and inner width beta, three neighbour levels, then the D2 / D1 / ghost-box
+boundary layers marked 125 / 126 / 127 (fill-box points keep marks <= 124).
+
+`make_local_case(name, real, seed)` returns a `Case` (tests/lsm_cases.py)
+whose arguments follow the reference prototype.  Seed 1 passes the lists
+with a shifted index base (nlo_index = 5, the same entries), so the
+`element l - nlo` addressing is exercised; other seeds vary values and
+branches (signs of the constant speed, use_phi0, control-volume sign,
+sign changes in the outer layer).
+"""
+import zlib
+
+import numpy as np
+
+from lsm_cases import Case, ZERO_TOL, grow
+from lsmlib_amd.f77 import box_shape
+
+GB = (-3, 14, -2, 13, 1, 12)
+DX = (0.1, 0.13, 0.07)
+BETA, GAMMA = 0.12, 0.24
+MARK_FB, MARK_D2, MARK_D1, MARK_GB = 124, 125, 126, 127
+LEVEL = 3
+
+
+def _np(real):
+    return np.float64 if real == "double" else np.float32
+
+
+def sphere(real, rng, box=GB, centre=(0.55, 0.71, 0.43), radius=0.33, noise=0.004):
+    """Signed distance to a sphere on the grid (x = i dx, ...) plus noise."""
+    nk, nj, ni = box_shape(box)
+    k, j, i = np.meshgrid(np.arange(box[4], box[5] + 1), np.arange(box[2], box[3] + 1),
+                          np.arange(box[0], box[1] + 1), indexing="ij")
+    r = np.sqrt((i * DX[0] - centre[0]) ** 2 + (j * DX[1] - centre[1]) ** 2 + (k * DX[2] - centre[2]) ** 2)
+    a = r - radius + noise * rng.standard_normal(r.shape)
+    return np.ascontiguousarray(a.astype(_np(real)))
+
+
+class Band:
+    """The narrow band of a level set on GB, built by a library's
+    LSM3D_DETERMINE_NARROW_BAND and boundary-layer marking."""
+
+    def __init__(self, lib, real, phi, base=0):
+        n = int(np.prod(box_shape(GB)))
+        self.phi = phi
+        self.nb = np.zeros(box_shape(GB), dtype=np.uint8)
+        self.ix = np.full(n, -7, dtype=np.int32)
+        self.iy = np.full(n, -7, dtype=np.int32)
+        self.iz = np.full(n, -7, dtype=np.int32)
+        self.n_lo = np.full(6, -9, dtype=np.int32)
+        self.n_hi = np.full(6, -9, dtype=np.int32)
+        self.outer = np.full(n, -7, dtype=np.int32)
+        self.base, self.cap = base, n
+        r = lib.lsm3ddeterminenarrowband_(phi, GB, self.nb, GB, self.ix, self.iy, self.iz, base, base + n - 1,
+                                          self.n_lo, self.n_hi, self.outer, 0, n - 1, GAMMA, BETA, LEVEL)
+        self.nlo_plus, self.nhi_plus, self.nlo_minus, self.nhi_minus = r
+        for box, mark in ((grow(GB, -2), MARK_D2), (grow(GB, -1), MARK_D1), (GB, MARK_GB)):
+            lib.lsm3dmarknarrowbandboundarylayer_(self.nb, GB, box, mark)
+
+    def lists(self):
+        return [self.ix, self.iy, self.iz]
+
+
+def _field(rng, real, box=GB, scale=1.0):
+    return np.ascontiguousarray((scale * rng.standard_normal(box_shape(box))).astype(_np(real)))
+
+
+def _vel(rng, real, box=GB):
+    a = rng.standard_normal(box_shape(box))
+    m = rng.random(a.shape)
+    a[m < 0.08] = 0.0
+    sel = (m >= 0.08) & (m < 0.14)
+    a[sel] = 0.3 * ZERO_TOL[real] * rng.choice([-1.0, 1.0], size=int(sel.sum()))
+    return np.ascontiguousarray(a.astype(_np(real)))
+
+
+def _small(arrs, rng, v):
+    sel = rng.random(arrs[0].shape) < 0.1
+    for d, a in enumerate(arrs):
+        a[sel] = v if d != 1 else 0.0
+
+
+def _boxes(seed):
+    """Array boxes: GB, or GB grown unevenly (the arrays need not share a box)."""
+    if seed % 2 == 0:
+        return [GB] * 8
+    return [grow(GB, (1, 0, 2), (0, 1, 1)), GB, grow(GB, (0, 2, 1), (1, 0, 0)), grow(GB, 1),
+            GB, grow(GB, (2, 1, 0), (0, 0, 2)), GB, grow(GB, (0, 0, 1), (1, 1, 0))]
+
+
+def make_local_case(name, real="double", seed=0, lib=None):
+    """Arguments of narrow-band routine `name` (reference prototype order).
+    `lib` builds the band (default: the oracle)."""
+    if lib is None:
+        from oracle.oracle import oracle_lib
+        lib = oracle_lib(real)
+    rng = np.random.default_rng(seed + 1000 * (zlib.crc32(name.encode()) % 100003))
+    phi = sphere(real, rng)
+    band = Band(lib, real, phi)
+    shift = 5 if seed == 1 else 0
+    L = band.lists()
+    lo = lambda lev: int(band.n_lo[lev]) + shift  # noqa: E731
+    hi = lambda lev: int(band.n_hi[lev]) + shift  # noqa: E731
+    nb = band.nb.copy()
+    B = _boxes(seed)
+    dx, dy, dz = DX
+    F = lambda b=GB, s=1.0: _field(rng, real, b, s)  # noqa: E731
+    n = band.cap
+
+    if name == "lsm3ddeterminenarrowband_":
+        p = sphere(real, rng, B[1])
+        nbox = grow(B[1], (0, 1, 0), (1, 0, 1))
+        garbage = rng.integers(0, 200, size=box_shape(nbox)).astype(np.uint8)
+        lists = [np.full(n + 8, -7, dtype=np.int32) for _ in range(3)]
+        if seed == 2:
+            p = np.abs(p) + np.asarray(GAMMA, p.dtype)   # no band point: the empty branch
+        lvl = 1 if seed == 3 else LEVEL
+        return Case(name, real, [p, B[1], garbage, nbox, *lists, shift, shift + n + 7,
+                                 np.full(6, -9, np.int32), np.full(6, -9, np.int32),
+                                 np.full(n, -7, np.int32), 2, n - 1, GAMMA, BETA, lvl])
+    if name == "lsm3dmarknarrowbandboundarylayer_":
+        return Case(name, real, [nb, GB, grow(GB, -1 - seed % 3), MARK_D1 - seed % 2])
+    if name == "lsm3ddeterminenarrowbandfrommask_":
+        lists = [np.full(n, -7, dtype=np.int32) for _ in range(3)]
+        return Case(name, real, [phi.copy(), GB, nb, GB, *lists, shift, shift + n - 1,
+                                 np.full(6, -9, np.int32), np.full(6, -9, np.int32), 2, 1 if seed % 2 else -1])
+    if name == "lsm3dmultiplycutofflserhslocal_":
+        return Case(name, real, [phi, F(), GB, *L, lo(0), hi(0), nb, GB, MARK_FB, BETA, GAMMA])
+    if name == "lsm3dcheckouternarrowbandlayer_":
+        p = phi.copy()
+        if seed in (1, 2):   # a sign change in the plus (seed 1) or minus (seed 2) outer layer
+            m = band.nlo_plus if seed == 1 else band.nlo_minus
+            e = int(band.outer[m])
+            p[band.iz[e] - GB[4], band.iy[e] - GB[2], band.ix[e] - GB[0]] *= -1
+        # index_outer holds absolute entry numbers: the lists keep their base here
+        return Case(name, real, [p, GB, *L, int(band.n_lo[0]), int(band.n_hi[2]), band.outer, 0, n - 1,
+                                 band.nlo_plus, band.nhi_plus, band.nlo_minus, band.nhi_minus])
+    if name == "lsm3dimposemasklocal_":
+        return Case(name, real, [F(B[0]), B[0], F(B[1]), B[1], F(B[2]), B[2], *L, lo(0), hi(1)])
+    if name == "lsm3dcopydatalocal_":
+        return Case(name, real, [F(B[0]), B[0], F(B[1]), B[1], *L, lo(0), hi(2)])
+    if name in ("lsm3dhjeno1local_", "lsm3dhjeno2local_"):
+        eno2 = name == "lsm3dhjeno2local_"
+        out = [F(B[0]) for _ in range(3)] + [B[0]] + [F(B[2]) for _ in range(3)] + [B[2]]
+        args = out + [phi, GB, F(B[3]), B[3]] + ([F(B[4]), B[4]] if eno2 else [])
+        args += [dx, dy, dz, *L, lo(0), hi(0), lo(1), hi(1)] + ([lo(2), hi(2)] if eno2 else [])
+        args += [nb, GB, MARK_FB, MARK_D1] + ([MARK_D2] if eno2 else [])
+        return Case(name, real, args, scratch=[])
+    if name in ("lsm3dcentralgradorder2local_", "lsm3dcentralgradorder4local_"):
+        mark = MARK_D1 if name.endswith("2local_") else MARK_D2
+        return Case(name, real, [F(B[0]), F(B[0]), F(B[0]), B[0], phi, GB, dx, dy, dz, *L, lo(0), hi(1), nb, GB,
+                                 mark])
+    if name == "lsm3dlaplacianorder2local_":
+        return Case(name, real, [F(B[0]), B[0], phi, GB, dx, dy, dz, *L, lo(0), hi(1), nb, GB, MARK_D1])
+    if name == "lsm3dcomputeavegradphilocal_":
+        return Case(name, real, [phi, GB, dx, dy, dz, *L, lo(0), hi(0), nb, GB, MARK_FB])
+    if name == "lsm3dgradientmagnitudelocal_":
+        g = [F(B[0], 0.01) for _ in range(3)]
+        _small(g, rng, 1e-7 if real == "double" else 1e-4)
+        return Case(name, real, [*g, F(B[0]), B[0], *L, lo(0), hi(0), nb, GB, MARK_FB])
+    if name in ("lsm3drk1steplocal_", "lsm3dtvdrk2stage1local_", "lsm3dtvdrk3stage1local_"):
+        return Case(name, real, [F(B[0]), B[0], F(B[1]), B[1], F(B[2]), B[2], 0.037, *L, lo(0), hi(0), nb, GB,
+                                 MARK_FB])
+    if name in ("lsm3dtvdrk2stage2local_", "lsm3dtvdrk3stage2local_", "lsm3dtvdrk3stage3local_"):
+        return Case(name, real, [F(B[0]), B[0], F(B[1]), B[1], F(B[2]), B[2], F(B[3]), B[3], 0.037, *L, lo(0),
+                                 hi(0), nb, GB, MARK_FB])
+    if name == "lsm3dzerooutlevelseteqnrhslocal_":
+        return Case(name, real, [F(B[0]), B[0], *L, lo(0), hi(1)])
+    if name == "lsm3daddadvectiontermtolserhslocal_":
+        return Case(name, real, [F(B[0]), B[0], F(B[1]), F(B[1]), F(B[1]), B[1], F(B[2]), F(B[2]), F(B[2]), B[2],
+                                 *L, lo(0), hi(0), nb, GB, MARK_FB])
+    pm = lambda: [F(B[1]), F(B[1]), F(B[1]), B[1], F(B[2]), F(B[2]), F(B[2]), B[2]]  # noqa: E731
+    if name == "lsm3daddnormalveltermtolserhslocal_":
+        return Case(name, real, [F(B[0]), B[0], *pm(), _vel(rng, real, B[3]), B[3], *L, lo(0), hi(0), nb, GB,
+                                 MARK_FB])
+    if name == "lsm3daddconstnormalveltermtolserhslocal_":
+        vn = [-0.8, 0.6, 0.3 * ZERO_TOL[real], 0.0][seed % 4]
+        return Case(name, real, [F(B[0]), B[0], *pm(), vn, *L, lo(0), hi(0), nb, GB, MARK_FB])
+    if name == "lsm3daddconstcurvtermtolserhslocal_":
+        g = [F(B[1]) for _ in range(3)]
+        _small(g, rng, 0.0)
+        return Case(name, real, [F(B[0]), B[0], *g, B[1], *[F(B[2]) for _ in range(6)], B[2], -0.35, *L, lo(0),
+                                 hi(0), nb, GB, MARK_FB])
+    if name == "lsm3daddconstprecomputedcurvtermtolserhslocal_":
+        return Case(name, real, [F(B[0]), B[0], F(B[1]), B[1], F(B[2]), B[2], 0.05, *L, lo(0), hi(0), nb, GB,
+                                 MARK_FB])
+    if name == "lsm3daddexternalandnormalveltermtolserhslocal_":
+        vn = F(B[3])
+        vn[np.abs(vn) < 0.05] = 0.3 * ZERO_TOL[real]
+        return Case(name, real, [F(B[0]), B[0], *pm(), vn, F(B[3]), F(B[3]), F(B[3]), B[3], *L, lo(0), hi(0),
+                                 nb, GB, MARK_FB])
+    if name == "lsm3dcomputereinitializationeqnrhslocal_":
+        p0 = phi.copy()
+        sel = rng.random(p0.shape) < 0.05
+        p0[sel] = 0.0
+        p = phi.copy()
+        p[rng.random(p.shape) < 0.05] = 0.3 * ZERO_TOL[real]
+        return Case(name, real, [F(B[0]), B[0], p, GB, p0, GB, *pm(), dx, dy, dz, seed % 2, *L, lo(0), hi(0), nb,
+                                 GB, MARK_FB])
+    if name == "lsm3dcomputeorthogonalizationeqnrhslocal_":
+        psi = _vel(rng, real, B[3])
+        return Case(name, real, [F(B[0]), B[0], *pm(), psi, B[3], *pm(), dx, dy, dz, *L, lo(0), hi(0), nb, GB,
+                                 MARK_FB])
+    if name == "lsm3dmaxnormdifflocal_":
+        return Case(name, real, [F(B[0]), B[0], F(B[1]), B[1], *L, lo(0), hi(0), nb, GB, MARK_FB])
+    if name == "lsm3dcomputestableadvectiondtlocal_":
+        return Case(name, real, [F(B[0]), F(B[0]), F(B[0]), B[0], dx, dy, dz, 0.5, *L, lo(0), hi(0), nb, GB,
+                                 MARK_FB])
+    if name == "lsm3dcomputestablenormalveldtlocal_":
+        return Case(name, real, [_vel(rng, real, B[0]), B[0], *pm(), dx, dy, dz, 0.5, *L, lo(0), hi(0), nb, GB,
+                                 MARK_FB])
+    if name == "lsm3dcomputestablenormalveldtcontrolvolumelocal_":
+        return Case(name, real, [_vel(rng, real, B[0]), B[0], *pm(), F(B[3]), B[3], 1 if seed % 2 else -1, dx, dy,
+                                 dz, 0.5, *L, lo(0), hi(0), nb, GB, MARK_FB])
+    if name == "lsm3dcomputestableconstnormalveldtlocal_":
+        return Case(name, real, [[-0.8, 0.6][seed % 2], *pm(), dx, dy, dz, 0.5, *L, lo(0), hi(0), nb, GB,
+                                 MARK_FB])
+    raise KeyError(name)
+
+
+def local_cases_for(name, real, seeds=(0, 1, 2, 3)):
+    return [make_local_case(name, real, s) for s in seeds]
