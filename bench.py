@@ -58,7 +58,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (adjoint) measurement")
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the gradient")
     ap.add_argument("--only", choices=["batched", "reinit", "adjoint", "config1", "config5", "throughput",
-                                       "curvature", "curvature_step", "field_extension"],
+                                       "curvature", "curvature_step", "curvature_local", "field_extension"],
                     default=None,
                     help="run one secondary measurement alone and print its JSON (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -392,7 +392,9 @@ def run_throughput(n=1024, steps=5, warmup=1):
             "kernel_ms_avg": r["kernel_ms_avg"], "achieved_gbs": r["achieved_gbs"], "tiling": r["tiling"]}
 
 
-CURV_PUBLISHED_S = {0.04: 105.0, 0.02: 1276.0}   # curvature_example/README:141-144 (regular method)
+CURV_PUBLISHED_S = {0.04: 105.0, 0.02: 1276.0}
+# t_local of the narrow-band example, curvature_example/README:143-144
+LOCAL_PUBLISHED_S = {0.04: 102.0, 0.02: 1086.0}   # curvature_example/README:141-144 (regular method)
 
 
 def run_curvature(dxs=(0.04, 0.02), big_dx=1.5 / 512, big_steps=10, cpu=True, runs=True):
@@ -504,6 +506,109 @@ def run_curvature(dxs=(0.04, 0.02), big_dx=1.5 / 512, big_steps=10, cpu=True, ru
                      "algorithmic_bytes_per_cell": bytes_cell}
         print(f"# curvature step {real} {g.n}: {big[real]['ms_per_step']:.3f} ms/step, stage2 "
               f"{ms['stage2']:.3f} ms ({big[real]['stage2_gbs']:.0f} GB/s)", file=sys.stderr, flush=True)
+        del m, p, phi, mask
+        torch.cuda.empty_cache()
+    out["step_512x512x341"] = big
+    return out
+
+
+def run_curvature_local(dxs=(0.04, 0.02), cpu=True, cpu_intervals=3, big_dx=1.5 / 512, big_steps=5):
+    """The curvature example's narrow-band loop (curvatureModelMedium3dLocalMainLoop,
+    curvature_model3d_local.c:52-728) on its default problem: (1) whole runs to
+    the example's stopping rule at dx 0.04 and 0.02, FP64 and FP32, beside the
+    published t_local (curvature_example/README:135-145); (2) at dx 0.04 the
+    reference Fortran narrow-band routines (oracle/_ref) run the same loop on
+    one host core for the first `cpu_intervals` intervals and the FP64 fields
+    are compared with the GPU's after the same intervals; (3) the localized
+    step at dx = 1.5/512 (512 x 512 x 341 cells), split by HIP events into the
+    band build, the two stages and the reinitialization check."""
+    import numpy as np
+    import torch
+    from lsmlib_amd.curvature_local import LocalCurvatureFlow
+    from lsmlib_amd.workloads import pore_throat
+
+    dev = torch.device("cuda:0")
+    out = {"model": "a=0.4, b=0.05, cfl 0.5, ENO2 / TVD-RK2, mask; band gamma 4dx, beta 2dx, levels 0-3",
+           "published_t_local_s": LOCAL_PUBLISHED_S, "runs": {}}
+    for real, dt_ in (("double", torch.float64), ("float", torch.float32)):
+        for dx in dxs:
+            g, phi, mask = pore_throat(dx, dt_, dev)
+            m = LocalCurvatureFlow(g, real, mask, device=dev)
+            p = phi.clone()
+            m.run(p, max_outer=1)
+            p = phi.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = m.run(p)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            out["runs"][f"{real}_dx{dx}"] = {
+                "grid": list(g.n), "seconds": el, "steps": st["steps"], "intervals": st["outer"],
+                "reinitializations": st["reinits"], "triggers": st["triggers"], "t_final": st["t"],
+                "vol_phi": st["vol_phi"], "mean_band_fraction": float(np.mean(st["band_fraction"])),
+                "ms_per_step": 1e3 * el / max(st["steps"], 1),
+                "speedup_vs_published": LOCAL_PUBLISHED_S[dx] / el if dx in LOCAL_PUBLISHED_S else None}
+            print(f"# curvature local {real} dx={dx} {g.n}: {el:.3f} s, {st['steps']} steps, band fraction "
+                  f"{np.mean(st['band_fraction']):.3f}", file=sys.stderr, flush=True)
+    if cpu:
+        from oracle.oracle import oracle_lib, ref_lib
+        from oracle.steps import LocalCurvatureFlowOracle
+        lib = ref_lib("double")
+        kind = "reference"
+        if lib is None:
+            lib, kind = oracle_lib("double"), "port"
+        g, phi, mask = pore_throat(0.04)
+        pc = phi.numpy().copy()
+        t0 = time.perf_counter()
+        stc = LocalCurvatureFlowOracle(lib, g, mask.numpy().copy(), dt_lib=oracle_lib("double")).run(
+            pc, max_outer=cpu_intervals)
+        el = time.perf_counter() - t0
+        gg, pd, md = pore_throat(0.04, torch.float64, dev)
+        LocalCurvatureFlow(gg, "double", md, device=dev).run(pd, max_outer=cpu_intervals)
+        torch.cuda.synchronize()
+        same = bool(np.array_equal(pd.cpu().numpy(), pc))
+        out["cpu_dx0.04"] = {"kind": kind, "cores": 1, "seconds": el, "steps": stc["steps"],
+                             "sample": f"first {cpu_intervals} intervals (t = 0 .. {0.1 * cpu_intervals:g})",
+                             "identical_field_fp64": same}
+        print(f"# curvature local cpu ({kind}) dx=0.04, {cpu_intervals} intervals: {el:.2f} s, same field: {same}",
+              file=sys.stderr, flush=True)
+    big = {}
+    for real, dt_ in (("float", torch.float32), ("double", torch.float64)):
+        g, phi, mask = pore_throat(big_dx, dt_, dev)
+        m = LocalCurvatureFlow(g, real, mask, device=dev)
+        p = phi.clone()
+        m.box.fill_ghosts(p, masked=False)
+        m.reinitialize_box(p, m.gamma + g.dx[0])
+        m.step(p, 0.0, 1.0)
+        torch.cuda.synchronize()
+        ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+        acc = {"band": 0.0, "step_rest": 0.0, "check": 0.0}
+        t0 = time.perf_counter()
+        for _ in range(big_steps):
+            e = [ev() for _ in range(4)]
+            e[0].record()
+            m.band(p)          # the band build alone (step() builds it again, identically)
+            e[1].record()
+            m.step(p, 0.0, 1.0)
+            e[2].record()
+            m.needs_reinit(p, 0.0, 0.0, 0.0)
+            e[3].record()
+            e[3].synchronize()
+            acc["band"] += e[0].elapsed_time(e[1])
+            acc["step_rest"] += e[1].elapsed_time(e[2])
+            acc["check"] += e[2].elapsed_time(e[3])
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        ms = {k: v / big_steps for k, v in acc.items()}
+        cells = g.n[0] * g.n[1] * g.n[2]
+        frac = float(m.n_hi[2] - m.n_lo[0] + 1) / m.n
+        step_ms = ms["step_rest"] + ms["check"]
+        big[real] = {"grid": list(g.n), "band_points": int(m.n_hi[3] - m.n_lo[0] + 1),
+                     "band_fraction_levels_0_2": frac, "ms_band_build": ms["band"],
+                     "ms_per_step": step_ms, "ms_per_step_wall": 1e3 * el / big_steps - ms["band"],
+                     "cell_steps_per_s": cells / (step_ms * 1e-3), "steps": big_steps}
+        print(f"# curvature local step {real} {g.n}: {step_ms:.3f} ms/step (band build {ms['band']:.3f} ms, "
+              f"fraction {frac:.3f})", file=sys.stderr, flush=True)
         del m, p, phi, mask
         torch.cuda.empty_cache()
     out["step_512x512x341"] = big
@@ -653,6 +758,9 @@ def main():
     if a.only == "curvature":
         print(json.dumps({"curvature_example": run_curvature(cpu=not a.no_cpu_baseline)}), flush=True)
         return
+    if a.only == "curvature_local":
+        print(json.dumps({"curvature_local_example": run_curvature_local(cpu=not a.no_cpu_baseline)}), flush=True)
+        return
     if a.only == "field_extension":
         print(json.dumps({"field_extension_512": run_field_extension()}), flush=True)
         return
@@ -707,6 +815,7 @@ def main():
         torch.cuda.empty_cache()
         extras["curvature_example"] = run_curvature(cpu=not a.no_cpu_baseline)
         torch.cuda.empty_cache()
+        extras["curvature_local_example"] = run_curvature_local(cpu=not a.no_cpu_baseline)
         extras["field_extension_512"] = run_field_extension()
         torch.cuda.empty_cache()
         if not a.no_cpu_baseline:

@@ -286,6 +286,224 @@ class CurvatureFlowOracle:
         return stats
 
 
+class LocalCurvatureFlowOracle(CurvatureFlowOracle):
+    """The curvature example's localized (narrow-band) main loop,
+    curvatureModelMedium3dLocalMainLoop and reinitializeMedium3dLocal
+    (examples/serial/curvature_example/curvature_model3d_local.c:52-728,
+    757-934), composed from the narrow-band routines of `lib`:
+
+      per step:  LSM3D_DETERMINE_NARROW_BAND (width gamma = 4 dx, inner width
+                 beta = 2 dx, levels 0-3), the D2 / D1 / ghost-box boundary
+                 layers marked 125 / 126 / 127; stage 1 and stage 2 as the box
+                 loop but with the *LOCAL kernels on the listed points and the
+                 cut-off function; the ghost fill over the whole box; the mask
+                 over level 0 (IMPOSE_MASK_LOCAL, lsm_macros.h:108-117: entries
+                 n_lo[0] .. n_hi[0] - 1);
+      then:      LSM3D_CHECK_OUTER_NARROW_BAND_LAYER; without a sign change,
+                 LSM3D_COMPUTE_AVE_GRAD_PHI_LOCAL outside [0.9, 1.1], or TPLOT
+                 since the last one, triggers reinitialization on levels 0+1
+                 (the limits shifted down one level) over gamma + 2 dx;
+      interval:  LSM3D_MAX_NORM_DIFF_LOCAL on level 0, the box volume.
+    Before the loop phi is reinitialized over the whole box (gamma + dx).
+    The hyperbolic dt comes from LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT_LOCAL
+    of `dt_lib` (default `lib`): the reference's routine reads two
+    uninitialised variables, so a reference build there gives no meaningful
+    dt; pass the restatement (which takes the box routine's values)."""
+
+    MARK_GB, MARK_D1, MARK_D2, MARK_FB = 127, 126, 125, 124  # curvature_model3d_local.c:43
+    LEVEL = 3
+    AVE_GRAD_PHI_MIN, AVE_GRAD_PHI_MAX = 0.9, 1.1            # :41-42
+    TPLOT = 0.1                                               # curvature_model_top.h:15
+
+    def __init__(self, lib, grid, mask, a=0.4, b=0.05, cfl=0.5, do_mask=True, dt_lib=None):
+        super().__init__(lib, grid, mask, a, b, cfl, do_mask)
+        self.dt_lib = dt_lib or lib
+        n = int(np.prod(grid.shape))
+        self.n = n
+        self.nb = np.zeros(grid.shape, dtype=np.uint8)
+        self.ix, self.iy, self.iz = (np.zeros(n, dtype=np.int32) for _ in range(3))
+        self.n_lo = np.zeros(6, dtype=np.int32)
+        self.n_hi = np.zeros(6, dtype=np.int32)
+        self.outer = np.zeros(n, dtype=np.int32)
+        self.outer_limits = (0, -1, 0, -1)
+        self.beta, self.gamma = 2 * grid.dx[0], 4 * grid.dx[0]   # :154
+
+    def _lists(self, l0, l1):
+        return (self.ix, self.iy, self.iz, int(self.n_lo[l0]), int(self.n_hi[l1]))
+
+    def band(self, phi):
+        o, g = self.o, self.grid
+        self.outer_limits = o.lsm3ddeterminenarrowband_(
+            phi, g.gb, self.nb, g.gb, self.ix, self.iy, self.iz, 0, self.n - 1, self.n_lo, self.n_hi, self.outer,
+            0, self.n - 1, self.gamma, self.beta, self.LEVEL)
+        for box, mark in ((self.fb2, self.MARK_D2), (self.fb1, self.MARK_D1), (g.gb, self.MARK_GB)):
+            o.lsm3dmarknarrowbandboundarylayer_(self.nb, g.gb, box, mark)
+
+    def _eno2_local(self, phi):
+        o, g = self.o, self.grid
+        o.lsm3dhjeno2local_(self.xp, self.yp, self.zp, g.gb, self.xm, self.ym, self.zm, g.gb, phi, g.gb,
+                            self.D1, g.gb, self.D2, g.gb, *g.dx, self.ix, self.iy, self.iz,
+                            int(self.n_lo[0]), int(self.n_hi[0]), int(self.n_lo[1]), int(self.n_hi[1]),
+                            int(self.n_lo[2]), int(self.n_hi[2]), self.nb, g.gb, self.MARK_FB, self.MARK_D1,
+                            self.MARK_D2)
+
+    def _rhs_local(self, phi, stage):
+        """lse_rhs of one stage (:231-404 / :431-565); returns the hyperbolic
+        dt in stage 1 when a > 0."""
+        o, g = self.o, self.grid
+        nb, fb = (self.nb, g.gb), self.MARK_FB
+        o.lsm3dzerooutlevelseteqnrhslocal_(self.rhs, g.gb, *self._lists(0, 0 if stage == 1 else 2))
+        dt = None
+        if self.a > 0:
+            self._eno2_local(phi)
+            o.lsm3daddconstnormalveltermtolserhslocal_(self.rhs, g.gb, self.xp, self.yp, self.zp, g.gb, self.xm,
+                                                       self.ym, self.zm, g.gb, self.a, *self._lists(0, 0), *nb, fb)
+            if stage == 1:
+                dt = self.dt_lib.lsm3dcomputestableconstnormalveldtlocal_(
+                    self.a, self.xp, self.yp, self.zp, g.gb, self.xm, self.ym, self.zm, g.gb, *g.dx, self.cfl,
+                    *self._lists(0, 0), *nb, fb)
+        if self.b > 0:
+            o.lsm3dcentralgradorder2local_(self.px, self.py, self.pz, g.gb, phi, g.gb, *g.dx, *self._lists(0, 1),
+                                           *nb, self.MARK_D1)
+            for src, outs in ((self.px, (self.pxx, self.pxy, self.pxz)), (self.py, (self.pxy, self.pyy, self.pyz)),
+                              (self.pz, (self.pxz, self.pyz, self.pzz))):
+                o.lsm3dcentralgradorder2local_(*outs, g.gb, src, g.gb, *g.dx, *self._lists(0, 0), *nb,
+                                               self.MARK_D2)
+            o.lsm3daddconstcurvtermtolserhslocal_(self.rhs, g.gb, self.px, self.py, self.pz, g.gb, self.pxx,
+                                                  self.pxy, self.pxz, self.pyy, self.pyz, self.pzz, g.gb, self.b,
+                                                  *self._lists(0, 0), *nb, fb)
+        return dt
+
+    def _cutoff(self, phi):
+        self.o.lsm3dmultiplycutofflserhslocal_(phi, self.rhs, self.grid.gb, *self._lists(0, 0), self.nb,
+                                               self.grid.gb, self.MARK_FB, self.beta, self.gamma)
+
+    def _mask_local(self, dest, src):
+        """IMPOSE_MASK_LOCAL (lsm_macros.h:108-117): entries n_lo[0] .. n_hi[0]-1."""
+        g = self.grid
+        self.o.lsm3dimposemasklocal_(dest, g.gb, src, g.gb, self.mask, g.gb, self.ix, self.iy, self.iz,
+                                     int(self.n_lo[0]), int(self.n_hi[0]) - 1)
+
+    def step_local(self, phi, dt_sub, tplot):
+        """One localized TVD-RK2 step (:185-570); returns dt."""
+        o, g = self.o, self.grid
+        nb, fb = (self.nb, g.gb), self.MARK_FB
+        self.band(phi)
+        dt = self._choose_dt(self._rhs_local(phi, 1), dt_sub, tplot)
+        self._cutoff(phi)
+        o.lsm3dtvdrk2stage1local_(self.s1, g.gb, phi, g.gb, self.rhs, g.gb, dt, *self._lists(0, 0), *nb, fb)
+        self._bc(self.s1)
+        if self.do_mask:
+            self._mask_local(self.s1, self.s1)
+        self._rhs_local(self.s1, 2)
+        self._cutoff(self.s1)
+        o.lsm3dtvdrk2stage2local_(self.nxt, g.gb, self.s1, g.gb, phi, g.gb, self.rhs, g.gb, dt, *self._lists(0, 0),
+                                  *nb, fb)
+        self._bc(self.nxt)
+        if self.do_mask:
+            self._mask_local(phi, self.nxt)
+        else:
+            phi[...] = self.nxt
+        return dt
+
+    def needs_reinit(self, phi, t, dt_sub, last_reinit_time):
+        """(:592-652) -> (trigger, reason)"""
+        o, g = self.o, self.grid
+        change = o.lsm3dcheckouternarrowbandlayer_(phi, g.gb, *self._lists(0, 0), self.outer, 0, self.n - 1,
+                                                   *self.outer_limits)
+        if change:
+            return True, "sign"
+        ave = o.lsm3dcomputeavegradphilocal_(phi, g.gb, *g.dx, *self._lists(0, 0), self.nb, g.gb, self.MARK_FB)
+        if ave < self.AVE_GRAD_PHI_MIN or ave > self.AVE_GRAD_PHI_MAX:
+            return True, "grad"
+        return (t + dt_sub - last_reinit_time >= self.TPLOT), "time"
+
+    def reinitialize_local(self, phi, tmax_r):
+        """reinitializeMedium3dLocal (:757-934) on the shifted limits."""
+        o, g = self.o, self.grid
+        nb, fb = (self.nb, g.gb), self.MARK_FB
+        dt_r = self.cfl * g.dx[0]
+        t_r, n = 0.0, 0
+        self.phi0[...] = phi
+        while t_r < tmax_r:
+            for stage in (1, 2):
+                src = phi if stage == 1 else self.s1
+                self._eno2_local(src)
+                o.lsm3dcomputereinitializationeqnrhslocal_(self.rhs, g.gb, src, g.gb, self.phi0, g.gb, self.xp,
+                                                           self.yp, self.zp, g.gb, self.xm, self.ym, self.zm, g.gb,
+                                                           *g.dx, 0, *self._lists(0, 0), *nb, fb)
+                if stage == 1:
+                    o.lsm3dtvdrk2stage1local_(self.s1, g.gb, phi, g.gb, self.rhs, g.gb, dt_r, *self._lists(0, 0),
+                                              *nb, fb)
+                    self._bc(self.s1)
+                else:
+                    o.lsm3dtvdrk2stage2local_(self.nxt, g.gb, self.s1, g.gb, phi, g.gb, self.rhs, g.gb, dt_r,
+                                              *self._lists(0, 0), *nb, fb)
+                    self._bc(self.nxt)
+                    if self.do_mask:
+                        self._mask_local(phi, self.nxt)
+                    else:
+                        phi[...] = self.nxt
+            t_r = t_r + dt_r
+            n += 1
+        return n
+
+    def run(self, phi, tmax=20.0, tplot=0.1, emax_stop=0.05, max_outer=None, max_steps=None):
+        """curvatureModelMedium3dLocalMainLoop on phi (in place).  Returns the
+        run's statistics, with the band sizes and reinitialization triggers."""
+        o, g = self.o, self.grid
+        dx = g.dx
+        eps_stop = emax_stop * dx[0]
+        eps = 1.5 * dx[0]
+        vol_max = o.lsm3dvolumeregionphilessthanzero_(self.mask, g.gb, g.fb, *dx, eps)
+        vol_phi = o.lsm3dvolumeregionphilessthanzero_(phi, g.gb, g.fb, *dx, eps)
+        tplot = tplot if tplot < tmax else tmax
+        stats = {"outer": 0, "steps": 0, "reinits": 0, "reinit_iters": 0, "triggers": {"sign": 0, "grad": 0,
+                 "time": 0}, "dt_min": 100.0, "dt_max": 0.0, "vol_max": float(vol_max), "history": [],
+                 "band_fraction": []}
+        stats["reinit_iters"] += self.reinitialize(phi, self.gamma + dx[0])   # :161
+        t, max_abs_err, last_reinit_time = 0.0, 1000.0, 0.0
+        phi_prev = phi.copy()
+        while t < tmax and max_abs_err > eps_stop and vol_phi > eps_stop:
+            if max_outer is not None and stats["outer"] >= max_outer:
+                break
+            stats["outer"] += 1
+            dt_sub = 0.0
+            phi_prev[...] = phi
+            while dt_sub < tplot:
+                if max_steps is not None and stats["steps"] >= max_steps:
+                    break
+                dt = self.step_local(phi, dt_sub, tplot)
+                stats["dt_max"] = max(stats["dt_max"], dt)
+                stats["dt_min"] = min(stats["dt_min"], dt)
+                trigger, why = self.needs_reinit(phi, t, dt_sub, last_reinit_time)
+                if trigger:
+                    stats["triggers"][why] += 1
+                    stats["reinits"] += 1
+                    last_reinit_time = t
+                    lo, hi = self.n_lo.copy(), self.n_hi.copy()
+                    self.n_hi[0] = self.n_hi[1]
+                    self.n_lo[1], self.n_hi[1] = self.n_lo[2], self.n_hi[2]
+                    self.n_lo[2], self.n_hi[2] = self.n_lo[3], self.n_hi[3]
+                    stats["reinit_iters"] += self.reinitialize_local(phi, self.gamma + 2 * dx[0])
+                    self.n_lo[...], self.n_hi[...] = lo, hi
+                dt_sub = dt_sub + dt
+                stats["steps"] += 1
+            t = t + dt_sub
+            max_abs_err = o.lsm3dmaxnormdifflocal_(phi, g.gb, phi_prev, g.gb, *self._lists(0, 0), self.nb, g.gb,
+                                                   self.MARK_FB)
+            vol_phi = o.lsm3dvolumeregionphilessthanzero_(phi, g.gb, g.gb, *dx, eps)
+            frac = float(self.n_hi[2] - self.n_lo[0] + 1) / self.n
+            stats["band_fraction"].append(frac)
+            stats["history"].append((float(t), float(max_abs_err), float(vol_phi)))
+            if max_steps is not None and stats["steps"] >= max_steps:
+                break
+        stats["t"] = float(t)
+        stats["max_abs_err"] = float(max_abs_err)
+        stats["vol_phi"] = float(vol_phi)
+        return stats
+
+
 class FieldExtensionOracle:
     """FieldExtensionAlgorithm::computeExtensionField on one box, composed from
     the toolbox routines of `lib` (FieldExtensionAlgorithm.cc:195-440,
