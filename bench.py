@@ -58,7 +58,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (adjoint) measurement")
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the gradient")
     ap.add_argument("--only", choices=["batched", "reinit", "adjoint", "config1", "config5", "throughput",
-                                       "curvature", "curvature_step", "curvature_local", "field_extension"],
+                                       "curvature", "curvature_step", "curvature_local", "curvature_local_step",
+                                       "field_extension"],
                     default=None,
                     help="run one secondary measurement alone and print its JSON (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -757,6 +758,10 @@ def main():
         return
     if a.only == "curvature":
         print(json.dumps({"curvature_example": run_curvature(cpu=not a.no_cpu_baseline)}), flush=True)
+        return
+    if a.only == "curvature_local_step":  # the localized 512 x 512 x 341 step alone (profiling runs)
+        print(json.dumps({"curvature_local_example": run_curvature_local(dxs=(), cpu=False, big_steps=a.steps)}),
+              flush=True)
         return
     if a.only == "curvature_local":
         print(json.dumps({"curvature_local_example": run_curvature_local(cpu=not a.no_cpu_baseline)}), flush=True)
