@@ -511,35 +511,54 @@ __global__ void __launch_bounds__(256) k_entries_reduce_final(const double* part
 // ---------------------------------------------------------------------------
 // Band building
 // ---------------------------------------------------------------------------
-constexpr int kChunkRounds = 8;
+constexpr int kChunkRounds = 32;
 constexpr int kChunk = 256 * kChunkRounds;  // cells (or pairs) per block
+constexpr int kBatch = 16;                  // rounds whose loads are issued together
+constexpr int kPairRounds = 8;
+constexpr int kPairChunk = 256 * kPairRounds;  // (parent, direction) pairs per block
 
 enum { BAND_PHI = 0, BAND_MASK_NONNEG = 1, BAND_MASK_NONPOS = 2 };
 
-// lsm_localization3d.f:232-255 and :428-455
+// lsm_localization3d.f:232-255 and :428-455.  Bit 0: a band point; bit 1: an
+// outer-layer point with phi <= 0; bit 2: an outer-layer point with phi > 0.
 template <typename T>
-__device__ __forceinline__ void band_flags(int mode, T v, T width, T width_inner, bool& in, bool& om, bool& op) {
-  om = op = false;
+__device__ __forceinline__ int band_bits(int mode, T v, T width, T width_inner) {
   if (mode == BAND_PHI) {
     const T a = fabs(v);
-    in = a < width;
-    const bool outer = in && a >= width_inner;
-    om = outer && (v <= T(0));
-    op = outer && !(v <= T(0));
-  } else if (mode == BAND_MASK_NONNEG) {
-    in = v >= T(0);
-  } else {
-    in = v <= T(0);
+    const int in = (a < width) ? 1 : 0;
+    const int outer = (in && a >= width_inner) ? 1 : 0;
+    const int neg = (v <= T(0)) ? 1 : 0;
+    return in | ((outer & neg) << 1) | ((outer & (neg ^ 1)) << 2);
   }
+  if (mode == BAND_MASK_NONNEG) return (v >= T(0)) ? 1 : 0;
+  return (v <= T(0)) ? 1 : 0;
 }
+
+// n / d for 0 <= n < 2^31 by a multiply-high and a shift, the invariant-
+// divisor method of Granlund and Montgomery (PLDI 1994): s = ceil(log2 d),
+// m = floor(2^32 (2^s - d) / d) + 1, n / d = (mulhi(m, n) + n) >> s (the sum
+// stays below 2^32 because mulhi(m, n) <= n < 2^31).
+struct FastDiv {
+  uint32_t m, s;
+  __host__ __device__ FastDiv() : m(1), s(0) {}
+  __host__ explicit FastDiv(uint32_t d) {
+    s = 0;
+    while (s < 32 && ((uint64_t)1 << s) < d) s++;
+    m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << s) - d)) / d + 1);
+  }
+  __device__ __forceinline__ int div(int n) const {
+    return (int)((__umulhi(m, (uint32_t)n) + (uint32_t)n) >> s);
+  }
+};
 
 struct CellMap {  // linear cell index of a box, Fortran order, and back
   Box b;
   int nx, nxy;
+  FastDiv dx, dxy;
   __device__ __forceinline__ void ijk(int c, int& i, int& j, int& k) const {
-    const int kk = c / nxy;
+    const int kk = dxy.div(c);
     const int r = c - kk * nxy;
-    const int jj = r / nx;
+    const int jj = dx.div(r);
     i = b.ilo + (r - jj * nx);
     j = b.jlo + jj;
     k = b.klo + kk;
@@ -569,45 +588,54 @@ __device__ __forceinline__ void block_sum3(int& a, int& b, int& c) {
   c = s[2][0] + s[2][1] + s[2][2] + s[2][3];
 }
 
-// Exclusive in-block ranks of three flags for one round of 256 items, in
-// item order; `tot` returns the round's totals.  Ends with a barrier, so the
-// LDS it uses can be reused by the next round.
-__device__ __forceinline__ void block_rank3(const bool f[3], int rank[3], int tot[3]) {
+// Exclusive in-block ranks of the three flag bits of `bits` for one round of
+// 256 items, in item order; `tot` returns the round's totals.  Ends with a
+// barrier, so the LDS it uses can be reused by the next round.
+__device__ __forceinline__ void block_rank3(int bits, int& r0, int& r1, int& r2, int& t0, int& t1, int& t2) {
   __shared__ int s[3][4];
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  unsigned long long bal[3];
-#pragma unroll
-  for (int q = 0; q < 3; q++) {
-    bal[q] = __ballot(f[q]);
-    if (lane == 0) s[q][w] = __popcll(bal[q]);
+  const unsigned long long b0 = __ballot(bits & 1), b1 = __ballot(bits & 2), b2 = __ballot(bits & 4);
+  if (lane == 0) {
+    s[0][w] = __popcll(b0);
+    s[1][w] = __popcll(b1);
+    s[2][w] = __popcll(b2);
   }
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < 3; q++) {
-    int before = 0;
-    for (int x = 0; x < w; x++) before += s[q][x];
-    rank[q] = before + __popcll(bal[q] & lt);
-    tot[q] = s[q][0] + s[q][1] + s[q][2] + s[q][3];
+  int p0 = 0, p1 = 0, p2 = 0;
+  for (int x = 0; x < w; x++) {
+    p0 += s[0][x];
+    p1 += s[1][x];
+    p2 += s[2][x];
   }
+  r0 = p0 + __popcll(b0 & lt);
+  r1 = p1 + __popcll(b1 & lt);
+  r2 = p2 + __popcll(b2 & lt);
+  t0 = s[0][0] + s[0][1] + s[0][2] + s[0][3];
+  t1 = s[1][0] + s[1][1] + s[1][2] + s[1][3];
+  t2 = s[2][0] + s[2][1] + s[2][2] + s[2][3];
   __syncthreads();
 }
 
+// Level 0 scans phi's box in linear (Fortran) order; the field is indexed by
+// the same linear cell number (its Field is made on that box).  Each thread
+// issues the loads of kBatch rounds before using them.
 template <typename T>
 __global__ void __launch_bounds__(256)
-    k_band_count(Field<const T> f, CellMap cm, int ncell, int mode, T width, T width_inner, int* counts) {
+    k_band_count(Field<const T> f, int ncell, int mode, T width, T width_inner, int* counts) {
   const int base = (int)blockIdx.x * kChunk;
   int a = 0, b = 0, c = 0;
-  for (int r = 0; r < kChunkRounds; r++) {
-    const int x = base + r * 256 + (int)threadIdx.x;
-    if (x < ncell) {
-      int i, j, k;
-      cm.ijk(x, i, j, k);
-      bool in, om, op;
-      band_flags<T>(mode, f.at(i, j, k), width, width_inner, in, om, op);
-      a += in;
-      b += om;
-      c += op;
+  for (int r0 = 0; r0 < kChunkRounds; r0 += kBatch) {
+    T v[kBatch];
+#pragma unroll
+    for (int r = 0; r < kBatch; r++) v[r] = f.p[min(base + (r0 + r) * 256 + (int)threadIdx.x, ncell - 1)];
+#pragma unroll
+    for (int r = 0; r < kBatch; r++) {
+      const int x = base + (r0 + r) * 256 + (int)threadIdx.x;
+      const int bits = (x < ncell) ? band_bits<T>(mode, v[r], width, width_inner) : 0;
+      a += bits & 1;
+      b += (bits >> 1) & 1;
+      c += (bits >> 2) & 1;
     }
   }
   block_sum3(a, b, c);
@@ -618,27 +646,36 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Exclusive scan, in place, of `nb` rows of three counts (one block of 1024
-// threads); totals[0..2] receive the sums.
+// Exclusive scan, in place, of `nb` rows of three counts (one workgroup of
+// 1024 threads: each thread sums a run of rows, the runs are scanned with wave
+// shuffles and one LDS step over the 16 wave totals); totals[0..2] receive the
+// sums.
 __global__ void __launch_bounds__(1024) k_scan3(int* counts, int nb, int* totals) {
-  __shared__ int s[3][1024];
-  const int t = threadIdx.x;
+  __shared__ int s_w[3][16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int per = (nb + 1023) / 1024;
-  const int lo = t * per, hi = min(nb, lo + per);
+  const int lo = min(nb, t * per), hi = min(nb, lo + per);
   int sum[3] = {0, 0, 0};
+#pragma unroll 8
   for (int r = lo; r < hi; r++)
     for (int q = 0; q < 3; q++) sum[q] += counts[3 * r + q];
-  for (int q = 0; q < 3; q++) s[q][t] = sum[q];
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan of the thread sums
-    int v[3];
-    for (int q = 0; q < 3; q++) v[q] = (t >= off) ? s[q][t - off] : 0;
-    __syncthreads();
-    for (int q = 0; q < 3; q++) s[q][t] += v[q];
-    __syncthreads();
+  int incl[3];
+  for (int q = 0; q < 3; q++) {
+    int v = sum[q];
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(v, o, 64);
+      if (lane >= o) v += u;
+    }
+    incl[q] = v;
+    if (lane == 63) s_w[q][w] = v;
   }
+  __syncthreads();
   int run[3];
-  for (int q = 0; q < 3; q++) run[q] = s[q][t] - sum[q];
+  for (int q = 0; q < 3; q++) {
+    int before = 0;
+    for (int x = 0; x < w; x++) before += s_w[q][x];
+    run[q] = before + incl[q] - sum[q];
+  }
   for (int r = lo; r < hi; r++)
     for (int q = 0; q < 3; q++) {
       const int v = counts[3 * r + q];
@@ -646,38 +683,56 @@ __global__ void __launch_bounds__(1024) k_scan3(int* counts, int nb, int* totals
       run[q] += v;
     }
   if (t == 1023)
-    for (int q = 0; q < 3; q++) totals[q] = s[q][1023];
+    for (int q = 0; q < 3; q++) {
+      int all = 0;
+      for (int x = 0; x < 16; x++) all += s_w[q][x];
+      totals[q] = all;
+    }
 }
 
 // Writes level 0: narrow_band over phi's box (1 in the band, 0 elsewhere),
 // the band points in Fortran order, and the outer-layer entries (minus side
 // ascending from outer[0], plus side descending from outer[plus_top]).
+// same_box: narrow_band's box is phi's, so it is indexed by the cell number too.
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_band_write(Field<const T> f, CellMap cm, int ncell, int mode, T width, T width_inner, const int* offsets,
-                 Field<nb_t> nb, int* ix, int* iy, int* iz, int list_base, int* outer, int plus_top) {
+                 Field<nb_t> nb, int same_box, int* ix, int* iy, int* iz, int list_base, int* outer,
+                 int plus_top) {
   const int base = (int)blockIdx.x * kChunk;
-  int o[3] = {offsets[3 * blockIdx.x], offsets[3 * blockIdx.x + 1], offsets[3 * blockIdx.x + 2]};
-  for (int r = 0; r < kChunkRounds; r++) {
-    const int x = base + r * 256 + (int)threadIdx.x;
-    bool fl[3] = {false, false, false};
-    int i = 0, j = 0, k = 0;
-    if (x < ncell) {
-      cm.ijk(x, i, j, k);
-      band_flags<T>(mode, f.at(i, j, k), width, width_inner, fl[0], fl[1], fl[2]);
-      nb.at(i, j, k) = fl[0] ? nb_t(1) : nb_t(0);
+  int o0 = offsets[3 * blockIdx.x], o1 = offsets[3 * blockIdx.x + 1], o2 = offsets[3 * blockIdx.x + 2];
+  for (int r0 = 0; r0 < kChunkRounds; r0 += kBatch) {
+    T v[kBatch];
+#pragma unroll
+    for (int r = 0; r < kBatch; r++) v[r] = f.p[min(base + (r0 + r) * 256 + (int)threadIdx.x, ncell - 1)];
+    for (int r = 0; r < kBatch; r++) {
+      const int x = base + (r0 + r) * 256 + (int)threadIdx.x;
+      int bits = 0, i = 0, j = 0, k = 0;
+      if (x < ncell) {
+        bits = band_bits<T>(mode, v[r], width, width_inner);
+        const nb_t mk = (nb_t)(bits & 1);
+        if (same_box) {
+          nb.p[x] = mk;
+        } else {
+          cm.ijk(x, i, j, k);
+          nb.at(i, j, k) = mk;
+        }
+      }
+      int r0_, r1_, r2_, t0_, t1_, t2_;
+      block_rank3(bits, r0_, r1_, r2_, t0_, t1_, t2_);
+      if (bits & 1) {
+        cm.ijk(x, i, j, k);
+        const int p = o0 + r0_;
+        ix[p] = i;
+        iy[p] = j;
+        iz[p] = k;
+        if (bits & 2) outer[o1 + r1_] = list_base + p;
+        if (bits & 4) outer[plus_top - (o2 + r2_)] = list_base + p;
+      }
+      o0 += t0_;
+      o1 += t1_;
+      o2 += t2_;
     }
-    int rank[3], tot[3];
-    block_rank3(fl, rank, tot);
-    if (fl[0]) {
-      const int p = o[0] + rank[0];
-      ix[p] = i;
-      iy[p] = j;
-      iz[p] = k;
-      if (fl[1]) outer[o[1] + rank[1]] = list_base + p;
-      if (fl[2]) outer[plus_top - (o[2] + rank[2])] = list_base + p;
-    }
-    for (int q = 0; q < 3; q++) o[q] += tot[q];
   }
 }
 
@@ -716,9 +771,9 @@ __global__ void __launch_bounds__(256) k_nbr_bid(Parents P, Field<const nb_t> nb
 
 __global__ void __launch_bounds__(256)
     k_nbr_count(Parents P, Field<const nb_t> nb, CellMap cm, const int* key, unsigned char* win, int* counts) {
-  const int base = (int)blockIdx.x * kChunk;
+  const int base = (int)blockIdx.x * kPairChunk;
   int n = 0, z0 = 0, z1 = 0;
-  for (int r = 0; r < kChunkRounds; r++) {
+  for (int r = 0; r < kPairRounds; r++) {
     const int p = base + r * 256 + (int)threadIdx.x;
     if (p < P.npairs) {
       const int m = p / 6, q = p - 6 * m;
@@ -742,30 +797,29 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k_nbr_write(Parents P, Field<nb_t> nb, CellMap cm, int* key, const unsigned char* win, const int* offsets,
                 int* ox, int* oy, int* oz, nb_t mark, int write) {
-  const int base = (int)blockIdx.x * kChunk;
+  const int base = (int)blockIdx.x * kPairChunk;
   int o = offsets[3 * blockIdx.x];
-  for (int r = 0; r < kChunkRounds; r++) {
+  for (int r = 0; r < kPairRounds; r++) {
     const int p = base + r * 256 + (int)threadIdx.x;
-    bool fl[3] = {false, false, false};
-    int a = 0, b = 0, c = 0;
+    int bits = 0, a = 0, b = 0, c = 0;
     if (p < P.npairs && win[p]) {
       const int m = p / 6, q = p - 6 * m;
       neighbour(q, P.ix[m], P.iy[m], P.iz[m], cm.b, a, b, c);
-      fl[0] = true;
+      bits = 1;
     }
-    int rank[3], tot[3];
-    block_rank3(fl, rank, tot);
-    if (fl[0]) {
+    int r0_, r1_, r2_, t0_, t1_, t2_;
+    block_rank3(bits, r0_, r1_, r2_, t0_, t1_, t2_);
+    if (bits) {
       key[cm.lin(a, b, c)] = INT_MAX;
       if (write) {
-        const int pos = o + rank[0];
+        const int pos = o + r0_;
         ox[pos] = a;
         oy[pos] = b;
         oz[pos] = c;
         nb.at(a, b, c) = mark;
       }
     }
-    o += tot[0];
+    o += t0_;
   }
 }
 
@@ -1002,7 +1056,7 @@ void mark_neighbours(Call& c, const char* fn, Field<nb_t> nb, const Box& bn, int
     const int64_t nk = (int64_t)(got / sizeof(int));
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)blocks_for(nk, 256)), dim3(256), 0, c.stream(), key, nk, INT_MAX);
   }
-  CellMap cm{bn, bn.nx(), bn.nx() * bn.ny()};
+  CellMap cm{bn, bn.nx(), bn.nx() * bn.ny(), FastDiv((uint32_t)bn.nx()), FastDiv((uint32_t)(bn.nx() * bn.ny()))};
   Field<const nb_t> nbc = make_field<const nb_t>(nb.p, bn);
   int count = n_hi[0] + 1;
   for (int l = 1; l <= level && c.ok(); l++) {
@@ -1015,7 +1069,7 @@ void mark_neighbours(Call& c, const char* fn, Field<nb_t> nb, const Box& bn, int
         return;
       }
       const int npairs = (int)npairs64;
-      const int nb_blocks = blocks_for(npairs, kChunk);
+      const int nb_blocks = blocks_for(npairs, kPairChunk);
       unsigned char* work = reinterpret_cast<unsigned char*>(
           device_buffer(BUF_BAND_WORK, (size_t)npairs + (size_t)(3 * nb_blocks + 4) * sizeof(int) + 256));
       if (!c.ok()) return;
@@ -1089,8 +1143,8 @@ void determine_band(const char* fn, int mode, const R* phi, Box g, unsigned char
   int* work = reinterpret_cast<int*>(device_buffer(BUF_BAND_WORK, (size_t)(3 * nblk + 4) * sizeof(int) + 256));
   if (!c.ok()) return;
   int* totals = work + 3 * nblk;
-  CellMap cm{g, g.nx(), g.nx() * g.ny()};
-  hipLaunchKernelGGL(k_band_count<R>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), fphi, cm, ncell, mode, width,
+  CellMap cm{g, g.nx(), g.nx() * g.ny(), FastDiv((uint32_t)g.nx()), FastDiv((uint32_t)(g.nx() * g.ny()))};
+  hipLaunchKernelGGL(k_band_count<R>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), fphi, ncell, mode, width,
                      width_inner, work);
   c.launched("k_band_count");
   hipLaunchKernelGGL(k_scan3, dim3(1), dim3(1024), 0, c.stream(), work, nblk, totals);
@@ -1108,8 +1162,10 @@ void determine_band(const char* fn, int mode, const R* phi, Box g, unsigned char
     raise_error(std::string(fn) + ": index_outer is too short for the outer layer");
     return;
   }
+  const int same_box = (bn.ilo == g.ilo && bn.ihi == g.ihi && bn.jlo == g.jlo && bn.jhi == g.jhi &&
+                        bn.klo == g.klo && bn.khi == g.khi) ? 1 : 0;
   hipLaunchKernelGGL(k_band_write<R>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), fphi, cm, ncell, mode, width,
-                     width_inner, work, fnb, dix, diy, diz, nlo, dout, (int)(ocap - 1));
+                     width_inner, work, fnb, same_box, dix, diy, diz, nlo, dout, (int)(ocap - 1));
   c.launched("k_band_write");
   if (!c.ok()) return;
 
