@@ -966,6 +966,42 @@ void lsm3d_amd_const_normal_vel_dt_eno2(LSMLIB_REAL *dt, const LSMLIB_REAL *phi,
 void lsm3d_amd_impose_mask(LSMLIB_REAL *phi_masked, const LSMLIB_REAL *mask, const LSMLIB_REAL *phi,
                            long long num_gridpts);
 
+/* --- the narrow-band curvature example, one call per step ------------------
+ * curvatureModelMedium3dLocalMainLoop / reinitializeMedium3dLocal
+ * (examples/serial/curvature_example/curvature_model3d_local.c:52-728,
+ * 757-934) sequenced natively from the narrow-band routines above, on device
+ * arrays (every pointer below device memory over gb; the index arrays hold
+ * num_gridpts entries; mask may be NULL).  The struct carries the example's
+ * state between calls: the level limits and outer-layer limits of the last
+ * band, written by the step.  The work arrays keep their contents between
+ * calls as the reference's do (the stages read phi_stage1 / phi_next outside
+ * the band). */
+typedef struct LsmLocalCurvatureModel {
+  LSMLIB_REAL *phi_x_plus, *phi_y_plus, *phi_z_plus, *phi_x_minus, *phi_y_minus, *phi_z_minus;
+  LSMLIB_REAL *D1, *D2;
+  LSMLIB_REAL *phi_x, *phi_y, *phi_z, *phi_xx, *phi_xy, *phi_xz, *phi_yy, *phi_yz, *phi_zz;
+  LSMLIB_REAL *lse_rhs, *phi_stage1, *phi_next, *phi0;
+  const LSMLIB_REAL *mask;
+  unsigned char *narrow_band;
+  int *index_x, *index_y, *index_z, *index_outer;
+  int gb[6], fb[6];
+  double dx[3];
+  double a, b, cfl; /* normal speed, curvature coefficient, CFL number */
+  int n_lo[6], n_hi[6];
+  int outer_limits[4]; /* nlo / nhi of the plus and of the minus outer layer */
+} LsmLocalCurvatureModel;
+/* One localized TVD-RK2 step of phi (in place, ghost cells included) at
+ * interval position dt_sub of tplot; returns dt (-1 on error). */
+double lsm3d_amd_local_curvature_step(LsmLocalCurvatureModel *m, LSMLIB_REAL *phi, double dt_sub, double tplot);
+/* The reinitialization trigger after a step (:592-652): 0 none, 1 a sign
+ * change in the outer layer, 2 the average |grad phi| outside [0.9, 1.1],
+ * 3 TPLOT since the last reinitialization; -1 on error. */
+int lsm3d_amd_local_curvature_reinit_trigger(LsmLocalCurvatureModel *m, const LSMLIB_REAL *phi, double t,
+                                             double dt_sub, double last_reinit_time);
+/* reinitializeMedium3dLocal over levels 0+1 for tmax_r; returns the
+ * iteration count (-1 on error). */
+int lsm3d_amd_local_curvature_reinit(LsmLocalCurvatureModel *m, LSMLIB_REAL *phi, double tmax_r);
+
 /* ======================================================================
  * Part 4 - reverse mode (no counterpart in the reference, which has no
  * adjoint; these serve gradient-based callers of the WENO5 / TVD-RK

@@ -27,6 +27,20 @@ def lib_path(real):
     return LIBDIR / f"liblsm3d_amd_{real}.so"
 
 
+class LocalCurvatureModel(ctypes.Structure):
+    """LsmLocalCurvatureModel of include/lsm3d_amd.h (the narrow-band
+    curvature example's arrays and state)."""
+    ARRAYS = ("phi_x_plus", "phi_y_plus", "phi_z_plus", "phi_x_minus", "phi_y_minus", "phi_z_minus", "D1", "D2",
+              "phi_x", "phi_y", "phi_z", "phi_xx", "phi_xy", "phi_xz", "phi_yy", "phi_yz", "phi_zz", "lse_rhs",
+              "phi_stage1", "phi_next", "phi0")
+    _fields_ = ([(n, ctypes.c_void_p) for n in ARRAYS] +
+                [("mask", ctypes.c_void_p), ("narrow_band", ctypes.c_void_p), ("index_x", ctypes.c_void_p),
+                 ("index_y", ctypes.c_void_p), ("index_z", ctypes.c_void_p), ("index_outer", ctypes.c_void_p),
+                 ("gb", ctypes.c_int * 6), ("fb", ctypes.c_int * 6), ("dx", ctypes.c_double * 3),
+                 ("a", ctypes.c_double), ("b", ctypes.c_double), ("cfl", ctypes.c_double),
+                 ("n_lo", ctypes.c_int * 6), ("n_hi", ctypes.c_int * 6), ("outer_limits", ctypes.c_int * 4)])
+
+
 class ToolboxLibrary(F77Library):
     """The product library: LSM3D_* entry points plus runtime control.
 
@@ -92,6 +106,14 @@ class ToolboxLibrary(F77Library):
         L.lsm3d_amd_const_normal_vel_dt_eno2.argtypes = [ctypes.POINTER(R), P, B, B, R, R, R, R, R]
         L.lsm3d_amd_impose_mask.argtypes = [P, P, P, ctypes.c_longlong]
         L.lsm3d_amd_field_extension_velocity.argtypes = [P, P, P, B, P, P, P, B, P, B, B, R, R, R]
+        M = ctypes.POINTER(LocalCurvatureModel)
+        D = ctypes.c_double
+        L.lsm3d_amd_local_curvature_step.argtypes = [M, P, D, D]
+        L.lsm3d_amd_local_curvature_step.restype = D
+        L.lsm3d_amd_local_curvature_reinit_trigger.argtypes = [M, P, D, D, D]
+        L.lsm3d_amd_local_curvature_reinit_trigger.restype = I
+        L.lsm3d_amd_local_curvature_reinit.argtypes = [M, P, D]
+        L.lsm3d_amd_local_curvature_reinit.restype = I
         for f in ("lsm3d_amd_advect_stage", "lsm3d_amd_advect_const_stage", "lsm3d_amd_reinit_stage",
                   "lsm3d_amd_fill_ghosts", "lsm3d_amd_set_stage_tiling", "lsm3d_amd_patch_plan_destroy",
                   "lsm3d_amd_patch_plan_stage", "lsm3d_amd_patch_plan_capture_rk3",

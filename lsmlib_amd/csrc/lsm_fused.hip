@@ -111,7 +111,7 @@ bool valid_form(int form) {
 }
 
 bool device_only(const char* fn, const void* p, const char* what) {
-  if (p && !is_device_pointer(p)) {
+  if (p && !assume_device() && !is_device_pointer(p)) {
     raise_error(std::string(fn) + ": " + what + " must be device memory");
     return false;
   }

@@ -1,0 +1,272 @@
+// The narrow-band curvature example's step sequence as native host code.
+//
+// curvatureModelMedium3dLocalMainLoop and reinitializeMedium3dLocal
+// (examples/serial/curvature_example/curvature_model3d_local.c:52-728,
+// 757-934) call some forty toolbox routines per step with host-side
+// decisions between them (dt, the level limits, the reinitialization
+// trigger).  Sequencing them here, on device arrays and the library stream,
+// leaves the caller one call per step; every routine called is the drop-in
+// entry point of the same name, so results equal the reference's sequence of
+// those routines.
+#include <cmath>
+#include <string>
+
+#include "../../include/lsm3d_amd.h"
+#include "lsm_common.h"
+#include "lsm_runtime.h"
+
+namespace {
+
+typedef real_t R;
+typedef LsmLocalCurvatureModel Model;
+
+constexpr unsigned char kMarkGb = 127, kMarkD1 = 126, kMarkD2 = 125, kMarkFb = 124;  // :43
+constexpr int kLevel = 3;
+constexpr double kDtMinToCorrect = 1e-5, kDtMin = 0.001;   // curvature_model3d_local.c:35-36
+constexpr double kAveGradMin = 0.9, kAveGradMax = 1.1;      // :41-42
+constexpr double kTplot = 0.1;                              // curvature_model_top.h:15
+
+#define GB6(m) &(m)->gb[0], &(m)->gb[1], &(m)->gb[2], &(m)->gb[3], &(m)->gb[4], &(m)->gb[5]
+#define B6(b) &(b)[0], &(b)[1], &(b)[2], &(b)[3], &(b)[4], &(b)[5]
+#define FAIL_IF_ERROR(ret) \
+  if (lsm::error_pending()) return ret
+
+struct Scalars {  // the by-reference scalars of the routines, in the library's precision
+  R dx, dy, dz, a, b, cfl, beta, gamma;
+};
+
+Scalars scalars(const Model* m) {
+  return Scalars{(R)m->dx[0], (R)m->dx[1], (R)m->dx[2], (R)m->a, (R)m->b, (R)m->cfl, (R)(2.0 * m->dx[0]),
+                 (R)(4.0 * m->dx[0])};
+}
+
+int num_points(const Model* m) {
+  return (m->gb[1] - m->gb[0] + 1) * (m->gb[3] - m->gb[2] + 1) * (m->gb[5] - m->gb[4] + 1);
+}
+
+void ghost_fill(Model* m, R* a) {
+  static const int bc[6] = {3, 3, 3, 3, 3, 3};
+  lsm3d_amd_extrapolate_ghosts_masked(a, m->gb, nullptr, m->gb, nullptr, m->gb, m->fb, bc);
+}
+
+// IMPOSE_MASK_LOCAL (src/serial/lsm_macros.h:108-117): entries n_lo[0] .. n_hi[0]-1
+void mask_local(Model* m, R* dest, const R* src) {
+  int lo = m->n_lo[0], hi = m->n_hi[0] - 1;
+  lsm3dimposemasklocal_(dest, GB6(m), src, GB6(m), m->mask, GB6(m), m->index_x, m->index_y, m->index_z, &lo, &hi);
+}
+
+void band(Model* m, const R* phi, const Scalars& s) {
+  const int n = num_points(m);
+  int zero = 0, last = n - 1, level = kLevel;
+  int* ol = m->outer_limits;
+  lsm3ddeterminenarrowband_(phi, GB6(m), m->narrow_band, GB6(m), m->index_x, m->index_y, m->index_z, &zero, &last,
+                            m->n_lo, m->n_hi, m->index_outer, &zero, &last, &ol[0], &ol[1], &ol[2], &ol[3],
+                            &s.gamma, &s.beta, &level);
+  FAIL_IF_ERROR();
+  int d2[6], d1[6];
+  for (int q = 0; q < 6; q++) {
+    d2[q] = m->gb[q] + ((q & 1) ? -2 : 2);
+    d1[q] = m->gb[q] + ((q & 1) ? -1 : 1);
+  }
+  lsm3dmarknarrowbandboundarylayer_(m->narrow_band, GB6(m), B6(d2), &kMarkD2);
+  FAIL_IF_ERROR();
+  lsm3dmarknarrowbandboundarylayer_(m->narrow_band, GB6(m), B6(d1), &kMarkD1);
+  FAIL_IF_ERROR();
+  lsm3dmarknarrowbandboundarylayer_(m->narrow_band, GB6(m), GB6(m), &kMarkGb);
+}
+
+void eno2(Model* m, const R* phi, const Scalars& s) {
+  lsm3dhjeno2local_(m->phi_x_plus, m->phi_y_plus, m->phi_z_plus, GB6(m), m->phi_x_minus, m->phi_y_minus,
+                    m->phi_z_minus, GB6(m), phi, GB6(m), m->D1, GB6(m), m->D2, GB6(m), &s.dx, &s.dy, &s.dz,
+                    m->index_x, m->index_y, m->index_z, &m->n_lo[0], &m->n_hi[0], &m->n_lo[1], &m->n_hi[1],
+                    &m->n_lo[2], &m->n_hi[2], m->narrow_band, GB6(m), &kMarkFb, &kMarkD1, &kMarkD2);
+}
+
+// lse_rhs of one stage (curvature_model3d_local.c:231-404 / :431-565), then the
+// cut-off; returns the hyperbolic dt in stage 1 (when a > 0).
+double stage_rhs(Model* m, const R* phi, int stage, const Scalars& s) {
+  double dt = -1.0;
+  int* lo = m->n_lo;
+  int* hi = m->n_hi;
+  lsm3dzerooutlevelseteqnrhslocal_(m->lse_rhs, GB6(m), m->index_x, m->index_y, m->index_z, &lo[0],
+                                   stage == 1 ? &hi[0] : &hi[2]);
+  FAIL_IF_ERROR(dt);
+  if (m->a > 0) {
+    eno2(m, phi, s);
+    FAIL_IF_ERROR(dt);
+    lsm3daddconstnormalveltermtolserhslocal_(m->lse_rhs, GB6(m), m->phi_x_plus, m->phi_y_plus, m->phi_z_plus,
+                                             GB6(m), m->phi_x_minus, m->phi_y_minus, m->phi_z_minus, GB6(m), &s.a,
+                                             m->index_x, m->index_y, m->index_z, &lo[0], &hi[0], m->narrow_band,
+                                             GB6(m), &kMarkFb);
+    FAIL_IF_ERROR(dt);
+    if (stage == 1) {
+      R d = 0;
+      lsm3dcomputestableconstnormalveldtlocal_(&d, &s.a, m->phi_x_plus, m->phi_y_plus, m->phi_z_plus, GB6(m),
+                                               m->phi_x_minus, m->phi_y_minus, m->phi_z_minus, GB6(m), &s.dx, &s.dy,
+                                               &s.dz, &s.cfl, m->index_x, m->index_y, m->index_z, &lo[0], &hi[0],
+                                               m->narrow_band, GB6(m), &kMarkFb);
+      FAIL_IF_ERROR(dt);
+      dt = (double)d;
+    }
+  }
+  if (m->b > 0) {
+    lsm3dcentralgradorder2local_(m->phi_x, m->phi_y, m->phi_z, GB6(m), phi, GB6(m), &s.dx, &s.dy, &s.dz, m->index_x,
+                                 m->index_y, m->index_z, &lo[0], &hi[1], m->narrow_band, GB6(m), &kMarkD1);
+    FAIL_IF_ERROR(dt);
+    // the second derivatives' last writers: xy = d/dx(phi_y), xz = d/dx(phi_z), yz = d/dy(phi_z)
+    R* outs[3][3] = {{m->phi_xx, m->phi_xy, m->phi_xz}, {m->phi_xy, m->phi_yy, m->phi_yz},
+                     {m->phi_xz, m->phi_yz, m->phi_zz}};
+    const R* srcs[3] = {m->phi_x, m->phi_y, m->phi_z};
+    for (int d = 0; d < 3; d++) {
+      lsm3dcentralgradorder2local_(outs[d][0], outs[d][1], outs[d][2], GB6(m), srcs[d], GB6(m), &s.dx, &s.dy, &s.dz,
+                                   m->index_x, m->index_y, m->index_z, &lo[0], &hi[0], m->narrow_band, GB6(m),
+                                   &kMarkD2);
+      FAIL_IF_ERROR(dt);
+    }
+    lsm3daddconstcurvtermtolserhslocal_(m->lse_rhs, GB6(m), m->phi_x, m->phi_y, m->phi_z, GB6(m), m->phi_xx,
+                                        m->phi_xy, m->phi_xz, m->phi_yy, m->phi_yz, m->phi_zz, GB6(m), &s.b,
+                                        m->index_x, m->index_y, m->index_z, &lo[0], &hi[0], m->narrow_band, GB6(m),
+                                        &kMarkFb);
+    FAIL_IF_ERROR(dt);
+  }
+  lsm3dmultiplycutofflserhslocal_(phi, m->lse_rhs, GB6(m), m->index_x, m->index_y, m->index_z, &lo[0], &hi[0],
+                                  m->narrow_band, GB6(m), &kMarkFb, &s.beta, &s.gamma);
+  return dt;
+}
+
+// curvature_model3d_local.c:331-349 and the clamps of :383-394
+double choose_dt(const Model* m, double dt_hyp, double dt_sub, double tplot) {
+  double dt = (m->a > 0) ? dt_hyp : tplot;
+  if (m->b > 0) {
+    const double dt_corr =
+        (1.0 / (m->dx[0] * m->dx[0]) + 1.0 / (m->dx[1] * m->dx[1]) + 1.0 / (m->dx[2] * m->dx[2])) * (2.0 * m->b);
+    const double max_h = (m->a > 0) ? m->cfl / dt : 0.0;
+    dt = m->cfl / (max_h + dt_corr);
+  }
+  if (dt < kDtMinToCorrect) dt = kDtMin;
+  if (dt_sub + dt > tplot) dt = tplot - dt_sub;
+  return dt;
+}
+
+}  // namespace
+
+extern "C" {
+
+double lsm3d_amd_local_curvature_step(LsmLocalCurvatureModel* m, LSMLIB_REAL* phi, double dt_sub, double tplot) {
+  lsm::AssumeDevice on_device;  // the model's arrays are device memory by contract
+  const Scalars s = scalars(m);
+  band(m, phi, s);
+  FAIL_IF_ERROR(-1.0);
+  const double dt = choose_dt(m, stage_rhs(m, phi, 1, s), dt_sub, tplot);
+  FAIL_IF_ERROR(-1.0);
+  const R dtr = (R)dt;
+  lsm3dtvdrk2stage1local_(m->phi_stage1, GB6(m), phi, GB6(m), m->lse_rhs, GB6(m), &dtr, m->index_x, m->index_y,
+                          m->index_z, &m->n_lo[0], &m->n_hi[0], m->narrow_band, GB6(m), &kMarkFb);
+  FAIL_IF_ERROR(-1.0);
+  ghost_fill(m, m->phi_stage1);
+  FAIL_IF_ERROR(-1.0);
+  if (m->mask) {
+    mask_local(m, m->phi_stage1, m->phi_stage1);
+    FAIL_IF_ERROR(-1.0);
+  }
+  stage_rhs(m, m->phi_stage1, 2, s);
+  FAIL_IF_ERROR(-1.0);
+  lsm3dtvdrk2stage2local_(m->phi_next, GB6(m), m->phi_stage1, GB6(m), phi, GB6(m), m->lse_rhs, GB6(m), &dtr,
+                          m->index_x, m->index_y, m->index_z, &m->n_lo[0], &m->n_hi[0], m->narrow_band, GB6(m),
+                          &kMarkFb);
+  FAIL_IF_ERROR(-1.0);
+  ghost_fill(m, m->phi_next);
+  FAIL_IF_ERROR(-1.0);
+  if (m->mask) {
+    mask_local(m, phi, m->phi_next);
+  } else {
+    const size_t bytes = (size_t)num_points(m) * sizeof(R);
+    lsm::check_hip(hipMemcpyAsync(phi, m->phi_next, bytes, hipMemcpyDeviceToDevice, lsm::current_stream()),
+                   "hipMemcpyAsync(phi <- phi_next)");
+  }
+  FAIL_IF_ERROR(-1.0);
+  return dt;
+}
+
+int lsm3d_amd_local_curvature_reinit_trigger(LsmLocalCurvatureModel* m, const LSMLIB_REAL* phi, double t,
+                                             double dt_sub, double last_reinit_time) {
+  lsm::AssumeDevice on_device;  // the model's arrays are device memory by contract
+  const Scalars s = scalars(m);
+  int change = 0, zero = 0, last = num_points(m) - 1;
+  int* ol = m->outer_limits;
+  lsm3dcheckouternarrowbandlayer_(&change, phi, GB6(m), m->index_x, m->index_y, m->index_z, &m->n_lo[0],
+                                  &m->n_hi[0], m->index_outer, &zero, &last, &ol[0], &ol[1], &ol[2], &ol[3]);
+  FAIL_IF_ERROR(-1);
+  if (change) return 1;
+  R ave = 0;
+  lsm3dcomputeavegradphilocal_(&ave, phi, GB6(m), &s.dx, &s.dy, &s.dz, m->index_x, m->index_y, m->index_z,
+                               &m->n_lo[0], &m->n_hi[0], m->narrow_band, GB6(m), &kMarkFb);
+  FAIL_IF_ERROR(-1);
+  if ((double)ave < kAveGradMin || (double)ave > kAveGradMax) return 2;
+  return (t + dt_sub - last_reinit_time >= kTplot) ? 3 : 0;
+}
+
+int lsm3d_amd_local_curvature_reinit(LsmLocalCurvatureModel* m, LSMLIB_REAL* phi, double tmax_r) {
+  lsm::AssumeDevice on_device;  // the model's arrays are device memory by contract
+  const Scalars s = scalars(m);
+  int lo[6], hi[6];
+  for (int q = 0; q < 6; q++) {
+    lo[q] = m->n_lo[q];
+    hi[q] = m->n_hi[q];
+  }
+  // levels 0 and 1 act as level 0 (curvature_model3d_local.c:661-663)
+  m->n_hi[0] = m->n_hi[1];
+  m->n_lo[1] = m->n_lo[2];
+  m->n_hi[1] = m->n_hi[2];
+  m->n_lo[2] = m->n_lo[3];
+  m->n_hi[2] = m->n_hi[3];
+  const size_t bytes = (size_t)num_points(m) * sizeof(R);
+  lsm::check_hip(hipMemcpyAsync(m->phi0, phi, bytes, hipMemcpyDeviceToDevice, lsm::current_stream()),
+                 "hipMemcpyAsync(phi0 <- phi)");
+  const double dt_r = m->cfl * m->dx[0];
+  const R dtr = (R)dt_r;
+  const int use_phi0 = 0;
+  double t_r = 0.0;
+  int n = 0;
+  while (t_r < tmax_r && !lsm::error_pending()) {
+    for (int stage = 1; stage <= 2 && !lsm::error_pending(); stage++) {
+      const R* src = (stage == 1) ? phi : m->phi_stage1;
+      eno2(m, src, s);
+      if (lsm::error_pending()) break;
+      lsm3dcomputereinitializationeqnrhslocal_(m->lse_rhs, GB6(m), src, GB6(m), m->phi0, GB6(m), m->phi_x_plus,
+                                               m->phi_y_plus, m->phi_z_plus, GB6(m), m->phi_x_minus,
+                                               m->phi_y_minus, m->phi_z_minus, GB6(m), &s.dx, &s.dy, &s.dz,
+                                               &use_phi0, m->index_x, m->index_y, m->index_z, &m->n_lo[0],
+                                               &m->n_hi[0], m->narrow_band, GB6(m), &kMarkFb);
+      if (lsm::error_pending()) break;
+      if (stage == 1) {
+        lsm3dtvdrk2stage1local_(m->phi_stage1, GB6(m), phi, GB6(m), m->lse_rhs, GB6(m), &dtr, m->index_x,
+                                m->index_y, m->index_z, &m->n_lo[0], &m->n_hi[0], m->narrow_band, GB6(m), &kMarkFb);
+        if (lsm::error_pending()) break;
+        ghost_fill(m, m->phi_stage1);
+      } else {
+        lsm3dtvdrk2stage2local_(m->phi_next, GB6(m), m->phi_stage1, GB6(m), phi, GB6(m), m->lse_rhs, GB6(m), &dtr,
+                                m->index_x, m->index_y, m->index_z, &m->n_lo[0], &m->n_hi[0], m->narrow_band,
+                                GB6(m), &kMarkFb);
+        if (lsm::error_pending()) break;
+        ghost_fill(m, m->phi_next);
+        if (lsm::error_pending()) break;
+        if (m->mask) {
+          mask_local(m, phi, m->phi_next);
+        } else {
+          lsm::check_hip(hipMemcpyAsync(phi, m->phi_next, bytes, hipMemcpyDeviceToDevice, lsm::current_stream()),
+                         "hipMemcpyAsync(phi <- phi_next)");
+        }
+      }
+    }
+    t_r = t_r + dt_r;
+    n++;
+  }
+  for (int q = 0; q < 6; q++) {
+    m->n_lo[q] = lo[q];
+    m->n_hi[q] = hi[q];
+  }
+  return lsm::error_pending() ? -1 : n;
+}
+
+}  // extern "C"

@@ -31,6 +31,18 @@ void check_hip(hipError_t e, const char* what);
 // True when p is device (or managed) memory that kernels can address.
 bool is_device_pointer(const void* p);
 
+// While an AssumeDevice object lives on this host thread, entry points take
+// every array argument as device memory without asking the runtime (library
+// code that calls its own entry points on arrays it knows are device memory).
+class AssumeDevice {
+ public:
+  AssumeDevice();
+  ~AssumeDevice();
+ private:
+  bool prev_;
+};
+bool assume_device();
+
 // Device scratch of at least `bytes`, owned by the library per device and
 // reused across calls (reductions keep per-block partials here).
 void* device_scratch(size_t bytes);

@@ -16,6 +16,7 @@ thread_local int tl_sync = 1;
 thread_local int tl_error_mode = 0;  // 0 abort, 1 record
 thread_local bool tl_error = false;
 thread_local std::string tl_message;
+thread_local bool tl_assume_device = false;
 
 std::mutex g_scratch_mu;
 constexpr int kMaxDevices = 64;
@@ -55,6 +56,10 @@ bool is_device_pointer(const void* p) {
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
          attr.type == hipMemoryTypeUnified;
 }
+
+AssumeDevice::AssumeDevice() : prev_(tl_assume_device) { tl_assume_device = true; }
+AssumeDevice::~AssumeDevice() { tl_assume_device = prev_; }
+bool assume_device() { return tl_assume_device; }
 
 void* device_scratch(size_t bytes) {
   int dev = 0;
@@ -134,7 +139,7 @@ void* Call::stage_bytes(void* p, size_t bytes, bool is_out) {
     raise_error(std::string(name_) + ": null array argument");
     return nullptr;
   }
-  if (is_device_pointer(p)) return p;
+  if (tl_assume_device || is_device_pointer(p)) return p;
   for (auto& s : staged_)  // the same host array passed twice shares one buffer
     if (s.host == p) {
       if (s.bytes < bytes) {

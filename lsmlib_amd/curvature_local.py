@@ -23,7 +23,10 @@ the narrow-band stages read them there (the ENO2 stencils of the outer
 levels), so the driver's initial reinitialization writes them exactly as
 reinitializeMedium3d does.
 
-Calls are queued on the library stream without host synchronisation; the
+By default a step is one call of the library's native sequence
+(lsm3d_amd_local_curvature_step, lsm_local_model.hip), which calls the same
+routines in the same order; `run(native=False)` calls them one by one from
+here.  Calls are queued on the library stream without host synchronisation; the
 host reads back only what the loop branches on (the band's level limits, dt,
 the outer-layer sign check, the average gradient, the interval's max
 change).  The hyperbolic dt is LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT_LOCAL
@@ -33,7 +36,9 @@ routine reads two uninitialised variables, lsm_utilities3d_local.f.in:728).
 import numpy as np
 import torch
 
-from ._native import toolbox
+import ctypes
+
+from ._native import LocalCurvatureModel, toolbox
 from .curvature import CurvatureFlow
 from .pylsmlib import _ptr
 
@@ -71,6 +76,33 @@ class LocalCurvatureFlow:
         gb = grid.gb
         self.fb1 = (gb[0] + 1, gb[1] - 1, gb[2] + 1, gb[3] - 1, gb[4] + 1, gb[5] - 1)
         self.fb2 = (gb[0] + 2, gb[1] - 2, gb[2] + 2, gb[3] - 2, gb[4] + 2, gb[5] - 2)
+
+    def model(self):
+        """The LsmLocalCurvatureModel of this driver's arrays (for the native
+        step, lsm3d_amd_local_curvature_step), with the current limits."""
+        m = LocalCurvatureModel()
+        for nme in LocalCurvatureModel.ARRAYS:
+            attr = {"phi_x_plus": "xp", "phi_y_plus": "yp", "phi_z_plus": "zp", "phi_x_minus": "xm",
+                    "phi_y_minus": "ym", "phi_z_minus": "zm", "phi_x": "px", "phi_y": "py", "phi_z": "pz",
+                    "phi_xx": "pxx", "phi_xy": "pxy", "phi_xz": "pxz", "phi_yy": "pyy", "phi_yz": "pyz",
+                    "phi_zz": "pzz", "lse_rhs": "rhs", "phi_stage1": "s1", "phi_next": "nxt"}.get(nme, nme)
+            setattr(m, nme, _ptr(getattr(self, attr)))
+        m.mask = _ptr(self.mask) if self.mask is not None else None
+        m.narrow_band = _ptr(self.nb)
+        m.index_x, m.index_y, m.index_z, m.index_outer = (_ptr(a) for a in (self.ix, self.iy, self.iz, self.outer))
+        m.gb[:] = list(self.grid.gb)
+        m.fb[:] = list(self.grid.fb)
+        m.dx[:] = list(self.grid.dx)
+        m.a, m.b, m.cfl = self.a, self.b, self.cfl
+        m.n_lo[:] = [int(v) for v in self.n_lo]
+        m.n_hi[:] = [int(v) for v in self.n_hi]
+        m.outer_limits[:] = [int(v) for v in self.outer_limits]
+        return m
+
+    def _pull(self, m):
+        self.n_lo[:] = list(m.n_lo)
+        self.n_hi[:] = list(m.n_hi)
+        self.outer_limits = tuple(m.outer_limits)
 
     # -- pieces --------------------------------------------------------------------
     def _lists(self, l0, l1):
@@ -222,11 +254,17 @@ class LocalCurvatureFlow:
         self.lib.set_sync(False)
         self.lib.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
 
-    def run(self, phi, tmax=20.0, tplot=0.1, emax_stop=0.05, max_outer=None, max_steps=None):
+    def run(self, phi, tmax=20.0, tplot=0.1, emax_stop=0.05, max_outer=None, max_steps=None, native=True):
         """curvatureModelMedium3dLocalMainLoop on phi in place (ghost cells
-        valid on entry).  Returns the run's statistics."""
+        valid on entry).  native: one library call per step
+        (lsm3d_amd_local_curvature_step and its trigger / reinitialization
+        companions); False sequences the same routines from Python.  Returns
+        the run's statistics."""
         L, g = self.lib, self.grid
         self._configure()
+        m = self.model() if native else None
+        mref = ctypes.byref(m) if native else None
+        pphi = _ptr(phi)
         dx = g.dx
         eps_stop = emax_stop * dx[0]
         eps = 1.5 * dx[0]
@@ -250,23 +288,35 @@ class LocalCurvatureFlow:
             while dt_sub < tplot:
                 if max_steps is not None and stats["steps"] >= max_steps:
                     break
-                dt = self.step(phi, dt_sub, tplot)
+                if native:
+                    dt = L.fused("lsm3d_amd_local_curvature_step", mref, pphi, dt_sub, tplot)
+                    code = L.fused("lsm3d_amd_local_curvature_reinit_trigger", mref, pphi, t, dt_sub,
+                                   last_reinit_time)
+                    trigger, why = code > 0, ("", "sign", "grad", "time")[code]
+                else:
+                    dt = self.step(phi, dt_sub, tplot)
+                    trigger, why = self.needs_reinit(phi, t, dt_sub, last_reinit_time)
                 stats["dt_max"] = max(stats["dt_max"], dt)
                 stats["dt_min"] = min(stats["dt_min"], dt)
-                trigger, why = self.needs_reinit(phi, t, dt_sub, last_reinit_time)
                 if trigger:
                     stats["triggers"][why] += 1
                     stats["reinits"] += 1
                     last_reinit_time = t
-                    lo, hi = self.n_lo.copy(), self.n_hi.copy()
-                    self.n_hi[0] = self.n_hi[1]
-                    self.n_lo[1], self.n_hi[1] = self.n_lo[2], self.n_hi[2]
-                    self.n_lo[2], self.n_hi[2] = self.n_lo[3], self.n_hi[3]
-                    stats["reinit_iters"] += self.reinitialize_local(phi, self.gamma + 2 * dx[0])
-                    self.n_lo[...], self.n_hi[...] = lo, hi
+                    if native:
+                        stats["reinit_iters"] += L.fused("lsm3d_amd_local_curvature_reinit", mref, pphi,
+                                                         self.gamma + 2 * dx[0])
+                    else:
+                        lo, hi = self.n_lo.copy(), self.n_hi.copy()
+                        self.n_hi[0] = self.n_hi[1]
+                        self.n_lo[1], self.n_hi[1] = self.n_lo[2], self.n_hi[2]
+                        self.n_lo[2], self.n_hi[2] = self.n_lo[3], self.n_hi[3]
+                        stats["reinit_iters"] += self.reinitialize_local(phi, self.gamma + 2 * dx[0])
+                        self.n_lo[...], self.n_hi[...] = lo, hi
                 dt_sub = dt_sub + dt
                 stats["steps"] += 1
             t = t + dt_sub
+            if native:
+                self._pull(m)
             max_abs_err = L.lsm3dmaxnormdifflocal_(phi, g.gb, phi_prev, g.gb, *self._lists(0, 0), self.nb, g.gb,
                                                    self.MARK_FB)
             vol_phi = L.lsm3dvolumeregionphilessthanzero_(phi, g.gb, g.gb, *dx, eps)

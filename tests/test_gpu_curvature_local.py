@@ -20,7 +20,10 @@ from lsmlib_amd.workloads import pore_throat
 
 
 @pytest.mark.gpu
-def test_local_curvature_run_fp64_bit_exact(device):
+@pytest.mark.parametrize("native", [True, False])
+def test_local_curvature_run_fp64_bit_exact(device, native):
+    """native: the step sequenced by lsm3d_amd_local_curvature_step; False:
+    the same routines called one by one from Python."""
     from lsmlib_amd.curvature_local import LocalCurvatureFlow
     from oracle.oracle import oracle_lib
     from oracle.steps import LocalCurvatureFlowOracle
@@ -28,7 +31,7 @@ def test_local_curvature_run_fp64_bit_exact(device):
     pc = phi.numpy().copy()
     want = LocalCurvatureFlowOracle(oracle_lib("double"), g, mask.numpy().copy()).run(pc, max_outer=2)
     gg, pd, md = pore_throat(0.04, torch.float64, device)
-    got = LocalCurvatureFlow(gg, "double", md, device=device).run(pd, max_outer=2)
+    got = LocalCurvatureFlow(gg, "double", md, device=device).run(pd, max_outer=2, native=native)
     torch.cuda.synchronize()
     assert got["steps"] == want["steps"] and got["triggers"] == want["triggers"]
     assert got["reinit_iters"] == want["reinit_iters"]
