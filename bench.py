@@ -126,6 +126,7 @@ def run_gpu(real, n, steps, warmup, tiling, ghosts="synth", zero_tol=None):
         "kernel_ms_by_stage": {str(f): sum(v) / len(v) for f, v in per_form.items()},
         "achieved_gbs": tot_bytes / (tot_ms * 1e-3) / 1e9,
         "algorithmic_bytes_per_launch": tot_bytes / len(events),
+        "cells": cells,
         "steps_total": warmup + steps,
     }
     res["_one_step"] = one_step
@@ -717,6 +718,34 @@ def read_traffic():
     return None
 
 
+# FP32 VALU issue rate of MI355X without packed FP32 (disabled in this build):
+# 256 CUs x 4 SIMDs x 16 lanes per clock x 2.4 GHz
+VALU_PEAK_LANE_INSTR = 256 * 64 * 2.4e9
+
+
+def valu_roofline(r32):
+    """The stage kernel against the VALU issue rate: measured instructions per
+    cell x cells per launch / the live average launch time."""
+    per_cell = read_valu()
+    if per_cell is None or not r32.get("kernel_ms_avg"):
+        return None
+    achieved = per_cell * r32["cells"] / (r32["kernel_ms_avg"] * 1e-3)
+    return {"lane_instructions_per_cell": per_cell, "achieved": achieved, "peak": VALU_PEAK_LANE_INSTR,
+            "unit": "lane-instructions/s", "frac": achieved / VALU_PEAK_LANE_INSTR}
+
+
+def read_valu():
+    """VALU lane-instructions per cell of the FP32 stage kernel (SQ_INSTS_VALU
+    x 64 / cells, profiles/valu_advect_stage_512_fp32.json)."""
+    p = REPO / "profiles" / "valu_advect_stage_512_fp32.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text()).get("valu_lane_instructions_per_cell")
+        except Exception:
+            return None
+    return None
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -864,7 +893,8 @@ def main():
                      "frac": r32["achieved_gbs"] / HBM_PEAK_GBS, "traffic": read_traffic(),
                      "kernel": "k_stage (fused advection stage)",
                      "kernel_ms_avg": r32["kernel_ms_avg"],
-                     "algorithmic_bytes_per_launch": r32["algorithmic_bytes_per_launch"]},
+                     "algorithmic_bytes_per_launch": r32["algorithmic_bytes_per_launch"],
+                     "valu": valu_roofline(r32)},
         "cpu_baseline": cpu,
         "adjoint_s": None if adj is None else adj["adjoint_s"],
         "adjoint": adj,
