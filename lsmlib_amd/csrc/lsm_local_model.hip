@@ -148,12 +148,39 @@ double choose_dt(const Model* m, double dt_hyp, double dt_sub, double tplot) {
   return dt;
 }
 
+// The model's arrays must be device memory (the calls below take them as such
+// without asking the runtime).  Checked when the set of pointers differs from
+// the last set this thread validated.
+bool model_on_device(const Model* m, const void* phi) {
+  const void* ptrs[] = {m->phi_x_plus, m->phi_y_plus, m->phi_z_plus, m->phi_x_minus, m->phi_y_minus,
+                        m->phi_z_minus, m->D1, m->D2, m->phi_x, m->phi_y, m->phi_z, m->phi_xx, m->phi_xy,
+                        m->phi_xz, m->phi_yy, m->phi_yz, m->phi_zz, m->lse_rhs, m->phi_stage1, m->phi_next,
+                        m->phi0, m->narrow_band, m->index_x, m->index_y, m->index_z, m->index_outer, phi,
+                        m->mask};
+  constexpr int n = sizeof(ptrs) / sizeof(ptrs[0]);
+  static thread_local uint64_t validated = 0;
+  uint64_t sig = 1469598103934665603ull;  // FNV-1a over the pointer values
+  for (int q = 0; q < n; q++) sig = (sig ^ (uint64_t)(uintptr_t)ptrs[q]) * 1099511628211ull;
+  if (sig == validated) return true;
+  for (int q = 0; q < n; q++) {
+    if (q == n - 1 && ptrs[q] == nullptr) continue;  // no mask
+    if (ptrs[q] == nullptr || !lsm::is_device_pointer(ptrs[q])) {
+      lsm::raise_error("lsm3d_amd_local_curvature: every array of the model (and phi) must be device memory");
+      return false;
+    }
+  }
+  validated = sig;
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
 
 double lsm3d_amd_local_curvature_step(LsmLocalCurvatureModel* m, LSMLIB_REAL* phi, double dt_sub, double tplot) {
-  lsm::AssumeDevice on_device;  // the model's arrays are device memory by contract
+  lsm::clear_error();
+  if (!model_on_device(m, phi)) return -1.0;
+  lsm::AssumeDevice on_device;
   const Scalars s = scalars(m);
   band(m, phi, s);
   FAIL_IF_ERROR(-1.0);
@@ -190,7 +217,9 @@ double lsm3d_amd_local_curvature_step(LsmLocalCurvatureModel* m, LSMLIB_REAL* ph
 
 int lsm3d_amd_local_curvature_reinit_trigger(LsmLocalCurvatureModel* m, const LSMLIB_REAL* phi, double t,
                                              double dt_sub, double last_reinit_time) {
-  lsm::AssumeDevice on_device;  // the model's arrays are device memory by contract
+  lsm::clear_error();
+  if (!model_on_device(m, phi)) return -1;
+  lsm::AssumeDevice on_device;
   const Scalars s = scalars(m);
   int change = 0, zero = 0, last = num_points(m) - 1;
   int* ol = m->outer_limits;
@@ -207,7 +236,9 @@ int lsm3d_amd_local_curvature_reinit_trigger(LsmLocalCurvatureModel* m, const LS
 }
 
 int lsm3d_amd_local_curvature_reinit(LsmLocalCurvatureModel* m, LSMLIB_REAL* phi, double tmax_r) {
-  lsm::AssumeDevice on_device;  // the model's arrays are device memory by contract
+  lsm::clear_error();
+  if (!model_on_device(m, phi)) return -1;
+  lsm::AssumeDevice on_device;
   const Scalars s = scalars(m);
   int lo[6], hi[6];
   for (int q = 0; q < 6; q++) {

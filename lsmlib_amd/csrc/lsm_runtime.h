@@ -25,6 +25,7 @@ bool sync_on_return();
 // returns without touching its outputs.
 void raise_error(const std::string& msg);
 bool error_pending();
+void clear_error();  // forget a recorded error (start of a call)
 void check_hip(hipError_t e, const char* what);
 
 // --- memory ---------------------------------------------------------------------

@@ -41,6 +41,11 @@ void raise_error(const std::string& msg) {
 
 bool error_pending() { return tl_error; }
 
+void clear_error() {
+  tl_error = false;
+  tl_message.clear();
+}
+
 void check_hip(hipError_t e, const char* what) {
   if (e != hipSuccess) raise_error(std::string(what) + ": " + hipGetErrorString(e));
 }

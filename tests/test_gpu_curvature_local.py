@@ -52,3 +52,22 @@ def test_local_curvature_steps_fp32(device):
     LocalCurvatureFlow(gg, "float", md, device=device).run(pd, max_steps=25)
     torch.cuda.synchronize()
     assert float(np.abs(pd.cpu().numpy() - pc).max()) < 1e-4
+
+
+@pytest.mark.gpu
+def test_native_step_rejects_host_arrays(device):
+    """The native step takes its arrays as device memory; a host array in the
+    model is reported before any kernel runs."""
+    import ctypes
+    from lsmlib_amd._native import LsmError
+    from lsmlib_amd.curvature_local import LocalCurvatureFlow
+    gg, pd, md = pore_throat(0.04, torch.float64, device)
+    flow = LocalCurvatureFlow(gg, "double", md, device=device)
+    m = flow.model()
+    host = np.zeros(flow.grid.shape)
+    m.phi0 = host.ctypes.data
+    with pytest.raises(LsmError, match="device memory"):
+        flow.lib.fused("lsm3d_amd_local_curvature_step", ctypes.byref(m), pd.data_ptr(), 0.0, 0.1)
+    m = flow.model()   # a valid model afterwards steps normally
+    dt = flow.lib.fused("lsm3d_amd_local_curvature_step", ctypes.byref(m), pd.data_ptr(), 0.0, 0.1)
+    assert 0 < dt <= 0.1
