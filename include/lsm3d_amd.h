@@ -362,6 +362,127 @@ void lsm3dvolumeregionphigreaterthanzero_(LSMLIB_REAL *volume, const LSMLIB_REAL
                                           const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
                                           const LSMLIB_REAL *dz, const LSMLIB_REAL *epsilon);
 
+/* LSM3D_AVE_ABS_DIFF, lsm_utilities3d.h:124 */
+void lsm3daveabsdiff_(LSMLIB_REAL *ave_abs_diff, const LSMLIB_REAL *field1, LSM_AMD_BOX(field1_gb),
+                      const LSMLIB_REAL *field2, LSM_AMD_BOX(field2_gb), LSM_AMD_BOX(ib));
+/* LSM3D_VOXEL_COUNT_GREATER_THAN_ZERO, lsm_utilities3d.h:160 */
+void lsm3dvoxelcountgreaterthanzero_(int *count, const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                                     LSM_AMD_BOX(ib));
+/* LSM3D_VOXEL_COUNT_LESS_THAN_ZERO, lsm_utilities3d.h:190 */
+void lsm3dvoxelcountlessthanzero_(int *count, const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                                  LSM_AMD_BOX(ib));
+/* LSM3D_SURFACE_INTEGRAL, lsm_utilities3d.h:477 */
+void lsm3dsurfaceintegral_(LSMLIB_REAL *int_F, const LSMLIB_REAL *F, LSM_AMD_BOX(F_gb),
+                           const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *phi_x,
+                           const LSMLIB_REAL *phi_y, const LSMLIB_REAL *phi_z,
+                           LSM_AMD_BOX(grad_phi_gb), LSM_AMD_BOX(ib), const LSMLIB_REAL *dx,
+                           const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+                           const LSMLIB_REAL *epsilon);
+/* LSM3D_COMPUTE_STABLE_NORMAL_VEL_DT_CONTROL_VOLUME, lsm_utilities3d.h:646 */
+void lsm3dcomputestablenormalveldtcontrolvolume_(
+    LSMLIB_REAL *dt, const LSMLIB_REAL *vel_n, LSM_AMD_BOX(vel_gb), const LSMLIB_REAL *phi_x_plus,
+    const LSMLIB_REAL *phi_y_plus, const LSMLIB_REAL *phi_z_plus, LSM_AMD_BOX(grad_phi_plus_gb),
+    const LSMLIB_REAL *phi_x_minus, const LSMLIB_REAL *phi_y_minus,
+    const LSMLIB_REAL *phi_z_minus, LSM_AMD_BOX(grad_phi_minus_gb),
+    const LSMLIB_REAL *control_vol, LSM_AMD_BOX(control_vol_gb), const int *control_vol_sgn,
+    LSM_AMD_BOX(ib), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+    const LSMLIB_REAL *cfl_number);
+/* LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT_CONTROL_VOLUME, lsm_utilities3d.h:723 */
+void lsm3dcomputestableconstnormalveldtcontrolvolume_(
+    LSMLIB_REAL *dt, const LSMLIB_REAL *vel_n, const LSMLIB_REAL *phi_x_plus,
+    const LSMLIB_REAL *phi_y_plus, const LSMLIB_REAL *phi_z_plus, LSM_AMD_BOX(grad_phi_plus_gb),
+    const LSMLIB_REAL *phi_x_minus, const LSMLIB_REAL *phi_y_minus,
+    const LSMLIB_REAL *phi_z_minus, LSM_AMD_BOX(grad_phi_minus_gb),
+    const LSMLIB_REAL *control_vol, LSM_AMD_BOX(control_vol_gb), const int *control_vol_sgn,
+    LSM_AMD_BOX(ib), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+    const LSMLIB_REAL *cfl_number);
+/* LSM3D_VOLUME_INTEGRAL_PHI_LESS_THAN_ZERO_CONTROL_VOLUME, lsm_utilities3d.h:789 */
+void lsm3dvolumeintegralphilessthanzerocontrolvolume_(
+    LSMLIB_REAL *int_F, const LSMLIB_REAL *F, LSM_AMD_BOX(F_gb), const LSMLIB_REAL *phi,
+    LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *control_vol, LSM_AMD_BOX(control_vol_gb),
+    const int *control_vol_sgn, LSM_AMD_BOX(ib), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+    const LSMLIB_REAL *dz, const LSMLIB_REAL *epsilon);
+/* LSM3D_VOLUME_INTEGRAL_PHI_GREATER_THAN_ZERO_CONTROL_VOLUME, lsm_utilities3d.h:850 */
+void lsm3dvolumeintegralphigreaterthanzerocontrolvolume_(
+    LSMLIB_REAL *int_F, const LSMLIB_REAL *F, LSM_AMD_BOX(F_gb), const LSMLIB_REAL *phi,
+    LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *control_vol, LSM_AMD_BOX(control_vol_gb),
+    const int *control_vol_sgn, LSM_AMD_BOX(ib), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+    const LSMLIB_REAL *dz, const LSMLIB_REAL *epsilon);
+/* LSM3D_SURFACE_INTEGRAL_CONTROL_VOLUME, lsm_utilities3d.h:911 */
+void lsm3dsurfaceintegralcontrolvolume_(
+    LSMLIB_REAL *int_F, const LSMLIB_REAL *F, LSM_AMD_BOX(F_gb), const LSMLIB_REAL *phi,
+    LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
+    const LSMLIB_REAL *phi_z, LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *control_vol,
+    LSM_AMD_BOX(control_vol_gb), const int *control_vol_sgn, LSM_AMD_BOX(ib),
+    const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+    const LSMLIB_REAL *epsilon);
+/* LSM3D_VOXEL_COUNT_GREATER_THAN_ZERO_CONTROL_VOLUME, lsm_utilities3d.h:969.
+ * Counts cells with phi > 0 and control_vol >= 0 (sign +1) or <= 0 (sign
+ * -1): unlike the other control-volume routines, cells with control_vol
+ * equal to zero are included (lsm_utilities3d.f.in:1895-1896). */
+void lsm3dvoxelcountgreaterthanzerocontrolvolume_(int *count, const LSMLIB_REAL *phi,
+                                                  LSM_AMD_BOX(phi_gb),
+                                                  const LSMLIB_REAL *control_vol,
+                                                  LSM_AMD_BOX(control_vol_gb),
+                                                  const int *control_vol_sgn, LSM_AMD_BOX(ib));
+/* LSM3D_VOXEL_COUNT_LESS_THAN_ZERO_CONTROL_VOLUME, lsm_utilities3d.h:1006 (zero
+ * control_vol included as above) */
+void lsm3dvoxelcountlessthanzerocontrolvolume_(int *count, const LSMLIB_REAL *phi,
+                                               LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *control_vol,
+                                               LSM_AMD_BOX(control_vol_gb),
+                                               const int *control_vol_sgn, LSM_AMD_BOX(ib));
+/* LSM3D_SURFACE_AREA_ZERO_LEVEL_SET, lsm_geometry3d.h:256 */
+void lsm3dsurfaceareazerolevelset_(LSMLIB_REAL *area, const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                                   const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
+                                   const LSMLIB_REAL *phi_z, LSM_AMD_BOX(grad_phi_gb),
+                                   LSM_AMD_BOX(ib), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                   const LSMLIB_REAL *dz, const LSMLIB_REAL *epsilon);
+/* LSM3D_SURFACE_AREA_ZERO_LEVEL_SET_DELTA, lsm_geometry3d.h:301 */
+void lsm3dsurfaceareazerolevelsetdelta_(LSMLIB_REAL *area, const LSMLIB_REAL *delta_phi,
+                                        LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *phi_x,
+                                        const LSMLIB_REAL *phi_y, const LSMLIB_REAL *phi_z,
+                                        LSM_AMD_BOX(grad_phi_gb), LSM_AMD_BOX(ib),
+                                        const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                        const LSMLIB_REAL *dz);
+/* LSM3D_VOLUME_REGION_PHI_LESS_THAN_ZERO_CONTROL_VOLUME, lsm_geometry3d.h:351 */
+void lsm3dvolumeregionphilessthanzerocontrolvolume_(
+    LSMLIB_REAL *volume, const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+    const LSMLIB_REAL *control_vol, LSM_AMD_BOX(control_vol_gb), const int *control_vol_sgn,
+    LSM_AMD_BOX(ib), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+    const LSMLIB_REAL *epsilon);
+/* LSM3D_VOLUME_REGION_PHI_GREATER_THAN_ZERO_CONTROL_VOLUME, lsm_geometry3d.h:402 */
+void lsm3dvolumeregionphigreaterthanzerocontrolvolume_(
+    LSMLIB_REAL *volume, const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+    const LSMLIB_REAL *control_vol, LSM_AMD_BOX(control_vol_gb), const int *control_vol_sgn,
+    LSM_AMD_BOX(ib), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+    const LSMLIB_REAL *epsilon);
+/* LSM3D_SURFACE_AREA_ZERO_LEVEL_SET_CONTROL_VOLUME, lsm_geometry3d.h:455 */
+void lsm3dsurfaceareazerolevelsetcontrolvolume_(
+    LSMLIB_REAL *area, const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *phi_x,
+    const LSMLIB_REAL *phi_y, const LSMLIB_REAL *phi_z, LSM_AMD_BOX(grad_phi_gb),
+    const LSMLIB_REAL *control_vol, LSM_AMD_BOX(control_vol_gb), const int *control_vol_sgn,
+    LSM_AMD_BOX(ib), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+    const LSMLIB_REAL *epsilon);
+/* LSM3D_SURFACE_AREA_ZERO_LEVEL_SET_DELTA_CONTROL_VOLUME, lsm_geometry3d.h:514 */
+void lsm3dsurfaceareazerolevelsetdeltacontrolvolume_(
+    LSMLIB_REAL *area, const LSMLIB_REAL *delta_phi, LSM_AMD_BOX(phi_gb),
+    const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y, const LSMLIB_REAL *phi_z,
+    LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *control_vol, LSM_AMD_BOX(control_vol_gb),
+    const int *control_vol_sgn, LSM_AMD_BOX(ib), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+    const LSMLIB_REAL *dz);
+
+/* --- delta function: src/toolbox/utilities/lsm_calculus_toolbox3d.h --- */
+/* LSM3D_DELTA_FUNCTION_ORDER1, lsm_calculus_toolbox3d.h:47.  The reference
+ * header declares delta const although the routine writes it; phi needs one
+ * ghost cell around the fill box (the routine reads its six neighbours).  The
+ * z terms use norm_phi_x and dx as the reference computes them; norm_phi_z and
+ * dz are accepted and unused. */
+void lsm3ddeltafunctionorder1_(const LSMLIB_REAL *phi, LSMLIB_REAL *delta, LSM_AMD_BOX(gb),
+                               const LSMLIB_REAL *norm_phi_x, const LSMLIB_REAL *norm_phi_y,
+                               const LSMLIB_REAL *norm_phi_z, LSM_AMD_BOX(grad_phi_gb),
+                               LSM_AMD_BOX(fb), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                               const LSMLIB_REAL *dz);
+
 /* --- ghost fill: src/toolbox/boundary_conditions/lsm_boundary_conditions3d.h --- */
 /* bdry_location_idx: 0 x_lo, 1 x_hi, 2 y_lo, 3 y_hi, 4 z_lo, 5 z_hi */
 /* LSM3D_LINEAR_EXTRAPOLATION, lsm_boundary_conditions3d.h:71 */

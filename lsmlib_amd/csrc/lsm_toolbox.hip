@@ -114,6 +114,33 @@ __global__ void __launch_bounds__(256)
   LSM_CELL_LOOP_END
 }
 
+// First-order delta function, lsm_calculus_toolbox3d.f.in:15-120: at each
+// cell the sum over the six neighbours across which phi changes sign (or
+// touches zero) of |phi_nb n / (dx (phi_nb - phi))|.  The z terms use the x
+// component of the normal and 1/dx, as the reference writes them (:98-110).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_delta_order1(Field<T> delta, Field<const T> phi, Field<const T> nx, Field<const T> ny, Box fb, T odx,
+                   T ody) {
+#pragma clang fp contract(off)
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T c = phi.at(i, j, k), ax = nx.at(i, j, k), ay = ny.at(i, j, k);
+  const T nb[6] = {phi.at(i + 1, j, k), phi.at(i - 1, j, k), phi.at(i, j + 1, k),
+                   phi.at(i, j - 1, k), phi.at(i, j, k + 1), phi.at(i, j, k - 1)};
+  T sum = T(0);
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    T t = T(0);
+    if (nb[q] * c <= T(0)) {
+      t = nb[q] * (q == 2 || q == 3 ? ay : ax) * (q == 2 || q == 3 ? ody : odx);
+      t = t / (nb[q] - c);
+    }
+    sum = (q == 0) ? fabs(t) : sum + fabs(t);
+  }
+  delta.at(i, j, k) = sum;
+  LSM_CELL_LOOP_END
+}
+
 // :3586-3650 (tol = 1e-8 on the squares and on the magnitude)
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -519,8 +546,10 @@ __device__ __forceinline__ double block_reduce(int op, double v, double ident) {
 // Per-cell terms.  Each returns false when the cell is excluded.
 struct RedArgs {
   int kind;
-  real_t ix, iy, iz, eps, dV, max_dx_sq, vn_const;
+  real_t ix, iy, iz, eps, ooe, dV, max_dx_sq, vn_const;
   int cv_sgn;
+  int cv_zero;     // control-volume cells equal to zero count too (voxel counts)
+  int one_double;  // smoothed delta's (1 + cos) in FP64 (lsm_utilities3d) or LSMLIB_REAL (lsm_geometry3d)
   int ci, cj, ck;  // always-included cell (max-norm initial value)
 };
 enum RedKind {
@@ -531,8 +560,32 @@ enum RedKind {
   RK_VOL_INT_LT = 5,
   RK_VOL_INT_GT = 6,
   RK_VOL_REG_LT = 7,
-  RK_VOL_REG_GT = 8
+  RK_VOL_REG_GT = 8,
+  RK_AVE_ABS_DIFF = 9,
+  RK_VOXEL_GT = 10,
+  RK_VOXEL_LT = 11,
+  RK_SURFACE = 12,
+  RK_SURFACE_DELTA = 13
 };
+
+// Smoothed delta function (1 + cos(pi phi / eps)) / (2 eps) of the surface
+// integrals and areas: the cosine is taken in LSMLIB_REAL, the (1 + cos) sum
+// is FP64 in lsm_utilities3d.f.in (1.d0 + cos) and LSMLIB_REAL in
+// lsm_geometry3d_fort.f.in (1 + cos); the products are FP64 (0.5d0 * ...).
+template <typename T>
+__device__ __forceinline__ T smoothed_delta(T p, T ooe, int one_double) {
+#pragma clang fp contract(off)
+  const T c = cos(T(3.14159265358979323846) * p * ooe);
+  if (one_double) return T(0.5 * double(ooe) * (1.0 + double(c)));
+  return T(0.5 * double(ooe) * double(T(1) + c));
+}
+
+// |grad phi| with the squares and sums rounded in LSMLIB_REAL
+template <typename T>
+__device__ __forceinline__ T norm3(T x, T y, T z) {
+#pragma clang fp contract(off)
+  return T(sqrt(double(x * x + y * y + z * z)));
+}
 
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -550,6 +603,7 @@ __global__ void __launch_bounds__(256)
       if (cv.p) {
         const T c = cv.at(i, j, k);
         inc = (a.cv_sgn > 0) ? (c > T(0)) : (c < T(0));
+        if (a.cv_zero && c == T(0)) inc = true;
         if (a.kind == RK_MAX_DIFF && i == a.ci && j == a.cj && k == a.ck) inc = true;
       }
       if (!inc) continue;
@@ -603,6 +657,33 @@ __global__ void __launch_bounds__(256)
           else if (p > -e)
             v = T(0.5) * (T(1) + x + T(0.31830988618379) * sin(T(3.14159265358979323846) * x)) * T(a.dV);
           else continue;
+          break;
+        }
+        case RK_AVE_ABS_DIFF:  // lsm_utilities3d.f.in:128-137
+          v = fabs(f0.at(i, j, k) - f1.at(i, j, k));
+          break;
+        case RK_VOXEL_GT:  // lsm_utilities3d.f.in:184-190, :1889-1919
+          if (!(f0.at(i, j, k) > T(0))) continue;
+          v = T(1);
+          break;
+        case RK_VOXEL_LT:  // lsm_utilities3d.f.in:233-239, :1971-2001
+          if (!(f0.at(i, j, k) < T(0))) continue;
+          v = T(1);
+          break;
+        case RK_SURFACE: {  // lsm_utilities3d.f.in:826-836, lsm_geometry3d_fort.f.in:511-523
+#pragma clang fp contract(off)
+          const T p = f1.at(i, j, k);
+          if (!(fabs(p) < T(a.eps))) continue;
+          const T d = smoothed_delta<T>(p, T(a.ooe), a.one_double);
+          const T ng = norm3<T>(f2.at(i, j, k), f3.at(i, j, k), f4.at(i, j, k));
+          v = f0.p ? d * ng * f0.at(i, j, k) * T(a.dV) : d * ng * T(a.dV);
+          break;
+        }
+        case RK_SURFACE_DELTA: {  // lsm_geometry3d_fort.f.in:597-604
+#pragma clang fp contract(off)
+          const T d = f0.at(i, j, k);
+          if (!(d > T(0))) continue;
+          v = d * norm3<T>(f2.at(i, j, k), f3.at(i, j, k), f4.at(i, j, k)) * T(a.dV);
           break;
         }
       }
@@ -1278,13 +1359,15 @@ void lsm3dcomputestableadvectiondtcontrolvolume_(R* dt, const R* vx, const R* vy
 }
 
 static void normal_dt_entry(const char* fn, R* dt, const R* vn, Box gv, R vn_const, const R* xp, const R* yp,
-                            const R* zp, Box gp, const R* xm, const R* ym, const R* zm, Box gm, Box ib, R dx,
-                            R dy, R dz, R cfl) {
+                            const R* zp, Box gp, const R* xm, const R* ym, const R* zm, Box gm, const R* cv,
+                            Box gcv, int cv_sgn, Box ib, R dx, R dy, R dz, R cfl) {
   Call c(fn);
   if (vn) require_inside(fn, "vel_n", gv, ib, 0, 0, 0);
   require_inside(fn, "grad_phi_plus", gp, ib, 0, 0, 0);
   require_inside(fn, "grad_phi_minus", gm, ib, 0, 0, 0);
+  if (cv) require_inside(fn, "control_vol", gcv, ib, 0, 0, 0);
   Field<const R> fvn = vn ? c.in(vn, gv) : no_field();
+  Field<const R> fc = cv ? c.in(cv, gcv) : no_field();
   auto a = c.in(xp, gp), b = c.in(yp, gp), d = c.in(zp, gp);
   auto e = c.in(xm, gm), f = c.in(ym, gm), g = c.in(zm, gm);
   RedArgs ra{};
@@ -1295,8 +1378,9 @@ static void normal_dt_entry(const char* fn, R* dt, const R* vn, Box gv, R vn_con
   R mdx = max_of3(dx, dy, dz);
   ra.max_dx_sq = mdx * mdx;
   ra.vn_const = vn_const;
+  ra.cv_sgn = cv_sgn;
   ra.ci = ra.cj = ra.ck = INT32_MIN;
-  double m = run_reduce(c, ra, ib, a, b, d, e, f, g, fvn, no_field());
+  double m = run_reduce(c, ra, ib, a, b, d, e, f, g, fvn, fc);
   c.finish();
   R mx = R(-1.0);
   if (m > (double)mx) mx = R(m);
@@ -1307,54 +1391,252 @@ void lsm3dcomputestablenormalveldt_(R* dt, const R* vn, BOXP(v), const R* xp, co
                                     BOXP(gp), const R* xm, const R* ym, const R* zm, BOXP(gm), BOXP(ib),
                                     const R* dx, const R* dy, const R* dz, const R* cfl) {
   normal_dt_entry("LSM3D_COMPUTE_STABLE_NORMAL_VEL_DT", dt, vn, BOX(v), R(0), xp, yp, zp, BOX(gp), xm, ym, zm,
-                  BOX(gm), BOX(ib), *dx, *dy, *dz, *cfl);
+                  BOX(gm), nullptr, Box{}, 0, BOX(ib), *dx, *dy, *dz, *cfl);
+}
+
+void lsm3dcomputestablenormalveldtcontrolvolume_(R* dt, const R* vn, BOXP(v), const R* xp, const R* yp,
+                                                 const R* zp, BOXP(gp), const R* xm, const R* ym, const R* zm,
+                                                 BOXP(gm), const R* cv, BOXP(cv), const int* cv_sgn, BOXP(ib),
+                                                 const R* dx, const R* dy, const R* dz, const R* cfl) {
+  normal_dt_entry("LSM3D_COMPUTE_STABLE_NORMAL_VEL_DT_CONTROL_VOLUME", dt, vn, BOX(v), R(0), xp, yp, zp, BOX(gp),
+                  xm, ym, zm, BOX(gm), cv, BOX(cv), *cv_sgn, BOX(ib), *dx, *dy, *dz, *cfl);
 }
 
 void lsm3dcomputestableconstnormalveldt_(R* dt, const R* vn, const R* xp, const R* yp, const R* zp, BOXP(gp),
                                          const R* xm, const R* ym, const R* zm, BOXP(gm), BOXP(ib),
                                          const R* dx, const R* dy, const R* dz, const R* cfl) {
   normal_dt_entry("LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT", dt, nullptr, Box{}, *vn, xp, yp, zp, BOX(gp), xm,
-                  ym, zm, BOX(gm), BOX(ib), *dx, *dy, *dz, *cfl);
+                  ym, zm, BOX(gm), nullptr, Box{}, 0, BOX(ib), *dx, *dy, *dz, *cfl);
+}
+
+void lsm3dcomputestableconstnormalveldtcontrolvolume_(R* dt, const R* vn, const R* xp, const R* yp, const R* zp,
+                                                      BOXP(gp), const R* xm, const R* ym, const R* zm, BOXP(gm),
+                                                      const R* cv, BOXP(cv), const int* cv_sgn, BOXP(ib),
+                                                      const R* dx, const R* dy, const R* dz, const R* cfl) {
+  normal_dt_entry("LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT_CONTROL_VOLUME", dt, nullptr, Box{}, *vn, xp, yp, zp,
+                  BOX(gp), xm, ym, zm, BOX(gm), cv, BOX(cv), *cv_sgn, BOX(ib), *dx, *dy, *dz, *cfl);
 }
 
 static void volume_entry(const char* fn, int kind, R* out, const R* F, Box gF, const R* phi, Box gphi,
-                         Box ib, R dx, R dy, R dz, R eps) {
+                         const R* cv, Box gcv, int cv_sgn, Box ib, R dx, R dy, R dz, R eps) {
   Call c(fn);
   if (F) require_inside(fn, "F", gF, ib, 0, 0, 0);
   require_inside(fn, "phi", gphi, ib, 0, 0, 0);
+  if (cv) require_inside(fn, "control_vol", gcv, ib, 0, 0, 0);
   Field<const R> fF = F ? c.in(F, gF) : no_field();
   auto fp = c.in(phi, gphi);
+  Field<const R> fc = cv ? c.in(cv, gcv) : no_field();
   RedArgs ra{};
   ra.kind = kind;
   ra.eps = eps;
   ra.dV = dx * dy * dz;  // dV = dx * dy * dz in LSMLIB_REAL
+  ra.cv_sgn = cv_sgn;
   ra.ci = ra.cj = ra.ck = INT32_MIN;
   double s;
-  if (F) s = run_reduce(c, ra, ib, fF, fp, no_field(), no_field(), no_field(), no_field(), no_field(), no_field());
-  else s = run_reduce(c, ra, ib, fp, no_field(), no_field(), no_field(), no_field(), no_field(), no_field(), no_field());
+  if (F) s = run_reduce(c, ra, ib, fF, fp, no_field(), no_field(), no_field(), no_field(), no_field(), fc);
+  else s = run_reduce(c, ra, ib, fp, no_field(), no_field(), no_field(), no_field(), no_field(), no_field(), fc);
   c.finish();
   if (!lsm::error_pending()) *out = R(s);
 }
 
 void lsm3dvolumeintegralphilessthanzero_(R* out, const R* F, BOXP(F), const R* phi, BOXP(phi), BOXP(ib),
                                          const R* dx, const R* dy, const R* dz, const R* eps) {
-  volume_entry("LSM3D_VOLUME_INTEGRAL_PHI_LESS_THAN_ZERO", RK_VOL_INT_LT, out, F, BOX(F), phi, BOX(phi), BOX(ib),
-               *dx, *dy, *dz, *eps);
+  volume_entry("LSM3D_VOLUME_INTEGRAL_PHI_LESS_THAN_ZERO", RK_VOL_INT_LT, out, F, BOX(F), phi, BOX(phi),
+               nullptr, Box{}, 0, BOX(ib), *dx, *dy, *dz, *eps);
 }
 void lsm3dvolumeintegralphigreaterthanzero_(R* out, const R* F, BOXP(F), const R* phi, BOXP(phi), BOXP(ib),
                                             const R* dx, const R* dy, const R* dz, const R* eps) {
   volume_entry("LSM3D_VOLUME_INTEGRAL_PHI_GREATER_THAN_ZERO", RK_VOL_INT_GT, out, F, BOX(F), phi, BOX(phi),
-               BOX(ib), *dx, *dy, *dz, *eps);
+               nullptr, Box{}, 0, BOX(ib), *dx, *dy, *dz, *eps);
 }
 void lsm3dvolumeregionphilessthanzero_(R* out, const R* phi, BOXP(phi), BOXP(ib), const R* dx, const R* dy,
                                        const R* dz, const R* eps) {
   volume_entry("LSM3D_VOLUME_REGION_PHI_LESS_THAN_ZERO", RK_VOL_REG_LT, out, nullptr, Box{}, phi, BOX(phi),
-               BOX(ib), *dx, *dy, *dz, *eps);
+               nullptr, Box{}, 0, BOX(ib), *dx, *dy, *dz, *eps);
 }
 void lsm3dvolumeregionphigreaterthanzero_(R* out, const R* phi, BOXP(phi), BOXP(ib), const R* dx, const R* dy,
                                           const R* dz, const R* eps) {
   volume_entry("LSM3D_VOLUME_REGION_PHI_GREATER_THAN_ZERO", RK_VOL_REG_GT, out, nullptr, Box{}, phi, BOX(phi),
-               BOX(ib), *dx, *dy, *dz, *eps);
+               nullptr, Box{}, 0, BOX(ib), *dx, *dy, *dz, *eps);
+}
+
+void lsm3daveabsdiff_(R* out, const R* f1, BOXP(f1), const R* f2, BOXP(f2), BOXP(ib)) {
+  const char* fn = "LSM3D_AVE_ABS_DIFF";
+  Call c(fn);
+  Box ib = BOX(ib);
+  require_inside(fn, "field1", BOX(f1), ib, 0, 0, 0);
+  require_inside(fn, "field2", BOX(f2), ib, 0, 0, 0);
+  auto a = c.in(f1, BOX(f1)), b = c.in(f2, BOX(f2));
+  RedArgs ra{};
+  ra.kind = RK_AVE_ABS_DIFF;
+  ra.ci = ra.cj = ra.ck = INT32_MIN;
+  double s = run_reduce(c, ra, ib, a, b, no_field(), no_field(), no_field(), no_field(), no_field(), no_field());
+  c.finish();
+  // lsm_utilities3d.f.in:141-145: the point count is the included box's
+  const double n = (double)ib.cells();
+  if (!lsm::error_pending()) *out = (n > 0) ? R(s / n) : R(0);
+}
+
+static void voxel_entry(const char* fn, int kind, int* count, const R* phi, Box gphi, const R* cv, Box gcv,
+                        int cv_sgn, Box ib) {
+  Call c(fn);
+  require_inside(fn, "phi", gphi, ib, 0, 0, 0);
+  if (cv) require_inside(fn, "control_vol", gcv, ib, 0, 0, 0);
+  auto fp = c.in(phi, gphi);
+  Field<const R> fc = cv ? c.in(cv, gcv) : no_field();
+  RedArgs ra{};
+  ra.kind = kind;
+  ra.cv_sgn = cv_sgn;
+  ra.cv_zero = 1;  // lsm_utilities3d.f.in:1895-1896: control_vol .ge. 0 / .le. 0
+  ra.ci = ra.cj = ra.ck = INT32_MIN;
+  double s = run_reduce(c, ra, ib, fp, no_field(), no_field(), no_field(), no_field(), no_field(), no_field(), fc);
+  c.finish();
+  if (!lsm::error_pending()) *count = (int)s;
+}
+
+void lsm3dvoxelcountgreaterthanzero_(int* count, const R* phi, BOXP(g), BOXP(ib)) {
+  voxel_entry("LSM3D_VOXEL_COUNT_GREATER_THAN_ZERO", RK_VOXEL_GT, count, phi, BOX(g), nullptr, Box{}, 0,
+              BOX(ib));
+}
+void lsm3dvoxelcountlessthanzero_(int* count, const R* phi, BOXP(g), BOXP(ib)) {
+  voxel_entry("LSM3D_VOXEL_COUNT_LESS_THAN_ZERO", RK_VOXEL_LT, count, phi, BOX(g), nullptr, Box{}, 0, BOX(ib));
+}
+void lsm3dvoxelcountgreaterthanzerocontrolvolume_(int* count, const R* phi, BOXP(g), const R* cv, BOXP(cv),
+                                                  const int* cv_sgn, BOXP(ib)) {
+  voxel_entry("LSM3D_VOXEL_COUNT_GREATER_THAN_ZERO_CONTROL_VOLUME", RK_VOXEL_GT, count, phi, BOX(g), cv, BOX(cv),
+              *cv_sgn, BOX(ib));
+}
+void lsm3dvoxelcountlessthanzerocontrolvolume_(int* count, const R* phi, BOXP(g), const R* cv, BOXP(cv),
+                                               const int* cv_sgn, BOXP(ib)) {
+  voxel_entry("LSM3D_VOXEL_COUNT_LESS_THAN_ZERO_CONTROL_VOLUME", RK_VOXEL_LT, count, phi, BOX(g), cv, BOX(cv),
+              *cv_sgn, BOX(ib));
+}
+
+// Surface integrals and areas: F (null for an area) times the smoothed delta
+// function times |grad phi| over |phi| < eps; one_double selects the
+// lsm_utilities3d form of the delta function (see smoothed_delta).
+static void surface_entry(const char* fn, R* out, const R* F, Box gF, const R* phi, Box gphi, const R* px,
+                          const R* py, const R* pz, Box gg, const R* cv, Box gcv, int cv_sgn, Box ib, R dx, R dy,
+                          R dz, R eps, int one_double) {
+  Call c(fn);
+  if (F) require_inside(fn, "F", gF, ib, 0, 0, 0);
+  require_inside(fn, "phi", gphi, ib, 0, 0, 0);
+  require_inside(fn, "grad_phi", gg, ib, 0, 0, 0);
+  if (cv) require_inside(fn, "control_vol", gcv, ib, 0, 0, 0);
+  Field<const R> fF = F ? c.in(F, gF) : no_field();
+  auto fp = c.in(phi, gphi);
+  auto a = c.in(px, gg), b = c.in(py, gg), d = c.in(pz, gg);
+  Field<const R> fc = cv ? c.in(cv, gcv) : no_field();
+  RedArgs ra{};
+  ra.kind = RK_SURFACE;
+  ra.eps = eps;
+  ra.ooe = R(1.0 / (double)eps);  // one_over_epsilon = 1.d0/epsilon
+  ra.dV = dx * dy * dz;
+  ra.cv_sgn = cv_sgn;
+  ra.one_double = one_double;
+  ra.ci = ra.cj = ra.ck = INT32_MIN;
+  double s = run_reduce(c, ra, ib, fF, fp, a, b, d, no_field(), no_field(), fc);
+  c.finish();
+  if (!lsm::error_pending()) *out = R(s);
+}
+
+void lsm3dsurfaceintegral_(R* out, const R* F, BOXP(F), const R* phi, BOXP(phi), const R* px, const R* py,
+                           const R* pz, BOXP(g), BOXP(ib), const R* dx, const R* dy, const R* dz, const R* eps) {
+  surface_entry("LSM3D_SURFACE_INTEGRAL", out, F, BOX(F), phi, BOX(phi), px, py, pz, BOX(g), nullptr, Box{}, 0,
+                BOX(ib), *dx, *dy, *dz, *eps, 1);
+}
+void lsm3dsurfaceintegralcontrolvolume_(R* out, const R* F, BOXP(F), const R* phi, BOXP(phi), const R* px,
+                                        const R* py, const R* pz, BOXP(g), const R* cv, BOXP(cv),
+                                        const int* cv_sgn, BOXP(ib), const R* dx, const R* dy, const R* dz,
+                                        const R* eps) {
+  surface_entry("LSM3D_SURFACE_INTEGRAL_CONTROL_VOLUME", out, F, BOX(F), phi, BOX(phi), px, py, pz, BOX(g), cv,
+                BOX(cv), *cv_sgn, BOX(ib), *dx, *dy, *dz, *eps, 1);
+}
+void lsm3dsurfaceareazerolevelset_(R* out, const R* phi, BOXP(phi), const R* px, const R* py, const R* pz,
+                                   BOXP(g), BOXP(ib), const R* dx, const R* dy, const R* dz, const R* eps) {
+  surface_entry("LSM3D_SURFACE_AREA_ZERO_LEVEL_SET", out, nullptr, Box{}, phi, BOX(phi), px, py, pz, BOX(g),
+                nullptr, Box{}, 0, BOX(ib), *dx, *dy, *dz, *eps, 0);
+}
+void lsm3dsurfaceareazerolevelsetcontrolvolume_(R* out, const R* phi, BOXP(phi), const R* px, const R* py,
+                                                const R* pz, BOXP(g), const R* cv, BOXP(cv), const int* cv_sgn,
+                                                BOXP(ib), const R* dx, const R* dy, const R* dz, const R* eps) {
+  surface_entry("LSM3D_SURFACE_AREA_ZERO_LEVEL_SET_CONTROL_VOLUME", out, nullptr, Box{}, phi, BOX(phi), px, py,
+                pz, BOX(g), cv, BOX(cv), *cv_sgn, BOX(ib), *dx, *dy, *dz, *eps, 0);
+}
+
+static void surface_delta_entry(const char* fn, R* out, const R* dphi, Box gd, const R* px, const R* py,
+                                const R* pz, Box gg, const R* cv, Box gcv, int cv_sgn, Box ib, R dx, R dy, R dz) {
+  Call c(fn);
+  require_inside(fn, "delta_phi", gd, ib, 0, 0, 0);
+  require_inside(fn, "grad_phi", gg, ib, 0, 0, 0);
+  if (cv) require_inside(fn, "control_vol", gcv, ib, 0, 0, 0);
+  auto fd = c.in(dphi, gd);
+  auto a = c.in(px, gg), b = c.in(py, gg), d = c.in(pz, gg);
+  Field<const R> fc = cv ? c.in(cv, gcv) : no_field();
+  RedArgs ra{};
+  ra.kind = RK_SURFACE_DELTA;
+  ra.dV = dx * dy * dz;
+  ra.cv_sgn = cv_sgn;
+  ra.ci = ra.cj = ra.ck = INT32_MIN;
+  double s = run_reduce(c, ra, ib, fd, no_field(), a, b, d, no_field(), no_field(), fc);
+  c.finish();
+  if (!lsm::error_pending()) *out = R(s);
+}
+
+void lsm3dsurfaceareazerolevelsetdelta_(R* out, const R* dphi, BOXP(d), const R* px, const R* py, const R* pz,
+                                        BOXP(g), BOXP(ib), const R* dx, const R* dy, const R* dz) {
+  surface_delta_entry("LSM3D_SURFACE_AREA_ZERO_LEVEL_SET_DELTA", out, dphi, BOX(d), px, py, pz, BOX(g), nullptr,
+                      Box{}, 0, BOX(ib), *dx, *dy, *dz);
+}
+void lsm3dsurfaceareazerolevelsetdeltacontrolvolume_(R* out, const R* dphi, BOXP(d), const R* px, const R* py,
+                                                     const R* pz, BOXP(g), const R* cv, BOXP(cv),
+                                                     const int* cv_sgn, BOXP(ib), const R* dx, const R* dy,
+                                                     const R* dz) {
+  surface_delta_entry("LSM3D_SURFACE_AREA_ZERO_LEVEL_SET_DELTA_CONTROL_VOLUME", out, dphi, BOX(d), px, py, pz,
+                      BOX(g), cv, BOX(cv), *cv_sgn, BOX(ib), *dx, *dy, *dz);
+}
+
+void lsm3ddeltafunctionorder1_(const R* phi, R* delta, BOXP(g), const R* nx, const R* ny, const R* nz, BOXP(n),
+                               BOXP(fb), const R* dx, const R* dy, const R* dz) {
+  const char* fn = "LSM3D_DELTA_FUNCTION_ORDER1";
+  (void)nz;
+  (void)dz;
+  Call c(fn);
+  Box fb = BOX(fb);
+  // the reference reads phi at the six neighbours of every fill-box cell
+  require_inside(fn, "phi", BOX(g), fb, 1, 1, 1);
+  require_inside(fn, "norm_phi", BOX(n), fb, 0, 0, 0);
+  auto fd = c.out(delta, BOX(g));
+  auto fp = c.in(phi, BOX(g));
+  auto fx = c.in(nx, BOX(n)), fy = c.in(ny, BOX(n));
+  launch_cells(c, "k_delta_order1", fb, k_delta_order1<R>, fd, fp, fx, fy, fb, R(1.0 / (double)*dx),
+               R(1.0 / (double)*dy));
+  c.finish();
+}
+
+void lsm3dvolumeintegralphilessthanzerocontrolvolume_(R* out, const R* F, BOXP(F), const R* phi, BOXP(phi),
+                                                      const R* cv, BOXP(cv), const int* cv_sgn, BOXP(ib),
+                                                      const R* dx, const R* dy, const R* dz, const R* eps) {
+  volume_entry("LSM3D_VOLUME_INTEGRAL_PHI_LESS_THAN_ZERO_CONTROL_VOLUME", RK_VOL_INT_LT, out, F, BOX(F), phi,
+               BOX(phi), cv, BOX(cv), *cv_sgn, BOX(ib), *dx, *dy, *dz, *eps);
+}
+void lsm3dvolumeintegralphigreaterthanzerocontrolvolume_(R* out, const R* F, BOXP(F), const R* phi, BOXP(phi),
+                                                         const R* cv, BOXP(cv), const int* cv_sgn, BOXP(ib),
+                                                         const R* dx, const R* dy, const R* dz, const R* eps) {
+  volume_entry("LSM3D_VOLUME_INTEGRAL_PHI_GREATER_THAN_ZERO_CONTROL_VOLUME", RK_VOL_INT_GT, out, F, BOX(F), phi,
+               BOX(phi), cv, BOX(cv), *cv_sgn, BOX(ib), *dx, *dy, *dz, *eps);
+}
+void lsm3dvolumeregionphilessthanzerocontrolvolume_(R* out, const R* phi, BOXP(phi), const R* cv, BOXP(cv),
+                                                    const int* cv_sgn, BOXP(ib), const R* dx, const R* dy,
+                                                    const R* dz, const R* eps) {
+  volume_entry("LSM3D_VOLUME_REGION_PHI_LESS_THAN_ZERO_CONTROL_VOLUME", RK_VOL_REG_LT, out, nullptr, Box{}, phi,
+               BOX(phi), cv, BOX(cv), *cv_sgn, BOX(ib), *dx, *dy, *dz, *eps);
+}
+void lsm3dvolumeregionphigreaterthanzerocontrolvolume_(R* out, const R* phi, BOXP(phi), const R* cv, BOXP(cv),
+                                                       const int* cv_sgn, BOXP(ib), const R* dx, const R* dy,
+                                                       const R* dz, const R* eps) {
+  volume_entry("LSM3D_VOLUME_REGION_PHI_GREATER_THAN_ZERO_CONTROL_VOLUME", RK_VOL_REG_GT, out, nullptr, Box{},
+               phi, BOX(phi), cv, BOX(cv), *cv_sgn, BOX(ib), *dx, *dy, *dz, *eps);
 }
 
 // ---- boundary conditions (lsm_boundary_conditions3d.h) ---------------------------

@@ -31,11 +31,13 @@ typedef double real;
 #define ZERO_TOL ((real)1e-11)
 #define TINY ((real)1e-99)
 #define SIN sin
+#define COS cos
 #else
 typedef float real;
 #define ZERO_TOL ((real)1e-5f)
 #define TINY ((real)1e-35)
 #define SIN sinf
+#define COS cosf
 #endif
 
 /* Fortran MAX/MIN of two reals: first argument wins ties and NaN handling
@@ -1301,6 +1303,302 @@ void oracle_lsm3dcomputefieldextensioneqnrhs_(real *rhs, BOXARGS(r), const real 
                                   A(nz, bn, i, j, k) * A(sz, bs, i, j, k));
         else
           A(rhs, br, i, j, k) = (real)0.0;
+      }
+}
+
+/* ---------------------------------------------------------------------- */
+/* The remaining 3D reductions: lsm_utilities3d.f.in, lsm_geometry3d_fort.f.in */
+/* and lsm_calculus_toolbox3d.f.in                                           */
+/* ---------------------------------------------------------------------- */
+#define CV_IN(c, sgn) (((sgn) > 0) ? ((c) > 0.0) : ((c) < 0.0))
+#define LOOP_IB                                       \
+  for (int k = *klo_ib; k <= *khi_ib; k++)            \
+    for (int j = *jlo_ib; j <= *jhi_ib; j++)          \
+      for (int i = *ilo_ib; i <= *ihi_ib; i++)
+
+void oracle_lsm3daveabsdiff_(real *out, const real *f1, BOXARGS(f1), const real *f2, BOXARGS(f2), BOXARGS(ib)) {
+  /* lsm_utilities3d.f.in:81-149: the sum and the point count are both real */
+  box_t b1 = MKBOX(f1), b2 = MKBOX(f2);
+  real sum = (real)0.0, num = (real)0.0;
+  LOOP_IB {
+    real d = (real)fabs(A(f1, b1, i, j, k) - A(f2, b2, i, j, k));
+    sum = sum + d;
+    num = num + (real)1.0;
+  }
+  *out = (num > (real)0.0) ? sum / num : (real)0.0;
+}
+
+static void voxel_count(int *count, const real *phi, box_t bp, const real *cv, box_t bc, int cv_sgn, int greater,
+                        const int *ilo_ib, const int *ihi_ib, const int *jlo_ib, const int *jhi_ib,
+                        const int *klo_ib, const int *khi_ib) {
+  /* lsm_utilities3d.f.in:154-247; with a control volume :1849-2007, where
+   * the control volume test is >= 0 / <= 0 (unlike the other routines' > / <) */
+  int n = 0;
+  LOOP_IB {
+    real p = A(phi, bp, i, j, k);
+    int in = greater ? (p > 0.0) : (p < 0.0);
+    if (cv) {
+      real c = A(cv, bc, i, j, k);
+      in = in && ((cv_sgn > 0) ? (c >= 0.0) : (c <= 0.0));
+    }
+    if (in) n++;
+  }
+  *count = n;
+}
+
+void oracle_lsm3dvoxelcountgreaterthanzero_(int *count, const real *phi, BOXARGS(g), BOXARGS(ib)) {
+  voxel_count(count, phi, MKBOX(g), NULL, MKBOX(g), 0, 1, ilo_ib, ihi_ib, jlo_ib, jhi_ib, klo_ib, khi_ib);
+}
+void oracle_lsm3dvoxelcountlessthanzero_(int *count, const real *phi, BOXARGS(g), BOXARGS(ib)) {
+  voxel_count(count, phi, MKBOX(g), NULL, MKBOX(g), 0, 0, ilo_ib, ihi_ib, jlo_ib, jhi_ib, klo_ib, khi_ib);
+}
+void oracle_lsm3dvoxelcountgreaterthanzerocontrolvolume_(int *count, const real *phi, BOXARGS(g), const real *cv,
+                                                         BOXARGS(cv), const int *cv_sgn, BOXARGS(ib)) {
+  voxel_count(count, phi, MKBOX(g), cv, MKBOX(cv), *cv_sgn, 1, ilo_ib, ihi_ib, jlo_ib, jhi_ib, klo_ib, khi_ib);
+}
+void oracle_lsm3dvoxelcountlessthanzerocontrolvolume_(int *count, const real *phi, BOXARGS(g), const real *cv,
+                                                      BOXARGS(cv), const int *cv_sgn, BOXARGS(ib)) {
+  voxel_count(count, phi, MKBOX(g), cv, MKBOX(cv), *cv_sgn, 0, ilo_ib, ihi_ib, jlo_ib, jhi_ib, klo_ib, khi_ib);
+}
+
+/* delta_eps(phi) |grad phi| summands.  one_plus: 1 for lsm_utilities3d's
+ * (1.d0 + cos, promoted to double), 0 for lsm_geometry3d's (1 + cos, real). */
+static real smoothed_delta(real p, real ooe, int double_one) {
+  real c = COS(PI_REF * p * ooe);
+  if (double_one) return (real)(0.5 * (double)ooe * (1.0 + (double)c));
+  return (real)(0.5 * (double)ooe * (double)((real)1 + c));
+}
+
+static real surface_sum(const real *F, box_t bF, const real *phi, box_t bp, const real *px, const real *py,
+                        const real *pz, box_t bg, const real *cv, box_t bc, int cv_sgn, const int *ilo_ib,
+                        const int *ihi_ib, const int *jlo_ib, const int *jhi_ib, const int *klo_ib,
+                        const int *khi_ib, real dV, real eps, int double_one) {
+  real acc = (real)0.0, ooe = 1.0 / eps;
+  LOOP_IB {
+    if (cv && !CV_IN(A(cv, bc, i, j, k), cv_sgn)) continue;
+    real p = A(phi, bp, i, j, k);
+    if (fabs(p) < eps) {
+      real delta = smoothed_delta(p, ooe, double_one);
+      real x = A(px, bg, i, j, k), y = A(py, bg, i, j, k), z = A(pz, bg, i, j, k);
+      real ng = (real)sqrt(x * x + y * y + z * z);
+      if (F) acc = acc + delta * ng * A(F, bF, i, j, k) * dV;
+      else acc = acc + delta * ng * dV;
+    }
+  }
+  return acc;
+}
+
+void oracle_lsm3dsurfaceintegral_(real *out, const real *F, BOXARGS(F), const real *phi, BOXARGS(phi),
+                                  const real *px, const real *py, const real *pz, BOXARGS(g), BOXARGS(ib),
+                                  const real *dx, const real *dy, const real *dz, const real *eps) {
+  /* lsm_utilities3d.f.in:745-846 */
+  *out = surface_sum(F, MKBOX(F), phi, MKBOX(phi), px, py, pz, MKBOX(g), NULL, MKBOX(g), 0, ilo_ib, ihi_ib, jlo_ib,
+                     jhi_ib, klo_ib, khi_ib, *dx * *dy * *dz, *eps, 1);
+}
+void oracle_lsm3dsurfaceintegralcontrolvolume_(real *out, const real *F, BOXARGS(F), const real *phi, BOXARGS(phi),
+                                               const real *px, const real *py, const real *pz, BOXARGS(g),
+                                               const real *cv, BOXARGS(cv), const int *cv_sgn, BOXARGS(ib),
+                                               const real *dx, const real *dy, const real *dz, const real *eps) {
+  /* lsm_utilities3d.f.in:1691-1844 */
+  *out = surface_sum(F, MKBOX(F), phi, MKBOX(phi), px, py, pz, MKBOX(g), cv, MKBOX(cv), *cv_sgn, ilo_ib, ihi_ib,
+                     jlo_ib, jhi_ib, klo_ib, khi_ib, *dx * *dy * *dz, *eps, 1);
+}
+void oracle_lsm3dsurfaceareazerolevelset_(real *out, const real *phi, BOXARGS(phi), const real *px, const real *py,
+                                          const real *pz, BOXARGS(g), BOXARGS(ib), const real *dx, const real *dy,
+                                          const real *dz, const real *eps) {
+  /* lsm_geometry3d_fort.f.in:442-533 */
+  *out = surface_sum(NULL, MKBOX(phi), phi, MKBOX(phi), px, py, pz, MKBOX(g), NULL, MKBOX(g), 0, ilo_ib, ihi_ib,
+                     jlo_ib, jhi_ib, klo_ib, khi_ib, *dx * *dy * *dz, *eps, 0);
+}
+void oracle_lsm3dsurfaceareazerolevelsetcontrolvolume_(real *out, const real *phi, BOXARGS(phi), const real *px,
+                                                       const real *py, const real *pz, BOXARGS(g), const real *cv,
+                                                       BOXARGS(cv), const int *cv_sgn, BOXARGS(ib), const real *dx,
+                                                       const real *dy, const real *dz, const real *eps) {
+  /* lsm_geometry3d_fort.f.in:929-1072 */
+  *out = surface_sum(NULL, MKBOX(phi), phi, MKBOX(phi), px, py, pz, MKBOX(g), cv, MKBOX(cv), *cv_sgn, ilo_ib,
+                     ihi_ib, jlo_ib, jhi_ib, klo_ib, khi_ib, *dx * *dy * *dz, *eps, 0);
+}
+
+static real delta_area_sum(const real *dphi, box_t bd, const real *px, const real *py, const real *pz, box_t bg,
+                           const real *cv, box_t bc, int cv_sgn, const int *ilo_ib, const int *ihi_ib,
+                           const int *jlo_ib, const int *jhi_ib, const int *klo_ib, const int *khi_ib, real dV) {
+  /* lsm_geometry3d_fort.f.in:537-613 and :1077-1197 */
+  real area = (real)0.0;
+  LOOP_IB {
+    if (cv && !CV_IN(A(cv, bc, i, j, k), cv_sgn)) continue;
+    real d = A(dphi, bd, i, j, k);
+    if (d > 0.0) {
+      real x = A(px, bg, i, j, k), y = A(py, bg, i, j, k), z = A(pz, bg, i, j, k);
+      real ng = (real)sqrt(x * x + y * y + z * z);
+      area = area + d * ng * dV;
+    }
+  }
+  return area;
+}
+
+void oracle_lsm3dsurfaceareazerolevelsetdelta_(real *out, const real *dphi, BOXARGS(d), const real *px,
+                                               const real *py, const real *pz, BOXARGS(g), BOXARGS(ib),
+                                               const real *dx, const real *dy, const real *dz) {
+  *out = delta_area_sum(dphi, MKBOX(d), px, py, pz, MKBOX(g), NULL, MKBOX(g), 0, ilo_ib, ihi_ib, jlo_ib, jhi_ib,
+                        klo_ib, khi_ib, *dx * *dy * *dz);
+}
+void oracle_lsm3dsurfaceareazerolevelsetdeltacontrolvolume_(real *out, const real *dphi, BOXARGS(d), const real *px,
+                                                            const real *py, const real *pz, BOXARGS(g),
+                                                            const real *cv, BOXARGS(cv), const int *cv_sgn,
+                                                            BOXARGS(ib), const real *dx, const real *dy,
+                                                            const real *dz) {
+  *out = delta_area_sum(dphi, MKBOX(d), px, py, pz, MKBOX(g), cv, MKBOX(cv), *cv_sgn, ilo_ib, ihi_ib, jlo_ib,
+                        jhi_ib, klo_ib, khi_ib, *dx * *dy * *dz);
+}
+
+void oracle_lsm3dcomputestablenormalveldtcontrolvolume_(real *dt, const real *vn, BOXARGS(vel), const real *xp,
+                                                        const real *yp, const real *zp, BOXARGS(gp), const real *xm,
+                                                        const real *ym, const real *zm, BOXARGS(gm), const real *cv,
+                                                        BOXARGS(cv), const int *cv_sgn, BOXARGS(ib),
+                                                        const real *dx, const real *dy, const real *dz,
+                                                        const real *cfl) {
+  /* lsm_utilities3d.f.in:1080-1250 */
+  box_t bv = MKBOX(vel), bp = MKBOX(gp), bm = MKBOX(gm), bc = MKBOX(cv);
+  real mdx = (real)fmax2(fmax2(*dx, *dy), *dz);
+  real max_dx_sq = mdx * mdx, mx = (real)-1.0;
+  real ix = 1.0 / *dx, iy = 1.0 / *dy, iz = 1.0 / *dz;
+  LOOP_IB {
+    if (!CV_IN(A(cv, bc, i, j, k), *cv_sgn)) continue;
+    real h = normal_h_over_dx((real)fabs(A(vn, bv, i, j, k)), A(xp, bp, i, j, k), A(yp, bp, i, j, k),
+                              A(zp, bp, i, j, k), A(xm, bm, i, j, k), A(ym, bm, i, j, k), A(zm, bm, i, j, k),
+                              max_dx_sq, ix, iy, iz);
+    if (h > mx) mx = h;
+  }
+  *dt = *cfl / (mx + TINY);
+}
+
+void oracle_lsm3dcomputestableconstnormalveldtcontrolvolume_(real *dt, const real *vn, const real *xp,
+                                                             const real *yp, const real *zp, BOXARGS(gp),
+                                                             const real *xm, const real *ym, const real *zm,
+                                                             BOXARGS(gm), const real *cv, BOXARGS(cv),
+                                                             const int *cv_sgn, BOXARGS(ib), const real *dx,
+                                                             const real *dy, const real *dz, const real *cfl) {
+  /* lsm_utilities3d.f.in:1255-1415 */
+  box_t bp = MKBOX(gp), bm = MKBOX(gm), bc = MKBOX(cv);
+  real mdx = (real)fmax2(fmax2(*dx, *dy), *dz);
+  real max_dx_sq = mdx * mdx, mx = (real)-1.0;
+  real ix = 1.0 / *dx, iy = 1.0 / *dy, iz = 1.0 / *dz, avn = (real)fabs(*vn);
+  LOOP_IB {
+    if (!CV_IN(A(cv, bc, i, j, k), *cv_sgn)) continue;
+    real h = normal_h_over_dx(avn, A(xp, bp, i, j, k), A(yp, bp, i, j, k), A(zp, bp, i, j, k), A(xm, bm, i, j, k),
+                              A(ym, bm, i, j, k), A(zm, bm, i, j, k), max_dx_sq, ix, iy, iz);
+    if (h > mx) mx = h;
+  }
+  *dt = *cfl / (mx + TINY);
+}
+
+void oracle_lsm3dvolumeintegralphilessthanzerocontrolvolume_(real *out, const real *F, BOXARGS(F), const real *phi,
+                                                             BOXARGS(phi), const real *cv, BOXARGS(cv),
+                                                             const int *cv_sgn, BOXARGS(ib), const real *dx,
+                                                             const real *dy, const real *dz, const real *eps) {
+  /* lsm_utilities3d.f.in:1420-1551 */
+  box_t bF = MKBOX(F), bphi = MKBOX(phi), bc = MKBOX(cv);
+  real dV = *dx * *dy * *dz, acc = (real)0.0, e = *eps;
+  LOOP_IB {
+    if (!CV_IN(A(cv, bc, i, j, k), *cv_sgn)) continue;
+    real p = A(phi, bphi, i, j, k), x = p / e;
+    if (p < -e) {
+      acc = acc + A(F, bF, i, j, k) * dV;
+    } else if (p < e) {
+      real omh = 0.5 * (1.0 - x - ONE_OVER_PI_REF * SIN(PI_REF * x));
+      acc = acc + omh * A(F, bF, i, j, k) * dV;
+    }
+  }
+  *out = acc;
+}
+
+void oracle_lsm3dvolumeintegralphigreaterthanzerocontrolvolume_(real *out, const real *F, BOXARGS(F),
+                                                                const real *phi, BOXARGS(phi), const real *cv,
+                                                                BOXARGS(cv), const int *cv_sgn, BOXARGS(ib),
+                                                                const real *dx, const real *dy, const real *dz,
+                                                                const real *eps) {
+  /* lsm_utilities3d.f.in:1556-1686 */
+  box_t bF = MKBOX(F), bphi = MKBOX(phi), bc = MKBOX(cv);
+  real dV = *dx * *dy * *dz, acc = (real)0.0, e = *eps;
+  LOOP_IB {
+    if (!CV_IN(A(cv, bc, i, j, k), *cv_sgn)) continue;
+    real p = A(phi, bphi, i, j, k), x = p / e;
+    if (p > e) {
+      acc = acc + A(F, bF, i, j, k) * dV;
+    } else if (p > -e) {
+      real H = 0.5 * (1.0 + x + ONE_OVER_PI_REF * SIN(PI_REF * x));
+      acc = acc + H * A(F, bF, i, j, k) * dV;
+    }
+  }
+  *out = acc;
+}
+
+void oracle_lsm3dvolumeregionphilessthanzerocontrolvolume_(real *out, const real *phi, BOXARGS(phi), const real *cv,
+                                                           BOXARGS(cv), const int *cv_sgn, BOXARGS(ib),
+                                                           const real *dx, const real *dy, const real *dz,
+                                                           const real *eps) {
+  /* lsm_geometry3d_fort.f.in:638-760 */
+  box_t bphi = MKBOX(phi), bc = MKBOX(cv);
+  real dV = *dx * *dy * *dz, vol = (real)0.0, e = *eps;
+  LOOP_IB {
+    if (!CV_IN(A(cv, bc, i, j, k), *cv_sgn)) continue;
+    real p = A(phi, bphi, i, j, k), x = p / e;
+    if (p < -e) {
+      vol = vol + dV;
+    } else if (p < e) {
+      real omh = 0.5 * (1 - x - ONE_OVER_PI_REF * SIN(PI_REF * x));
+      vol = vol + omh * dV;
+    }
+  }
+  *out = vol;
+}
+
+void oracle_lsm3dvolumeregionphigreaterthanzerocontrolvolume_(real *out, const real *phi, BOXARGS(phi),
+                                                              const real *cv, BOXARGS(cv), const int *cv_sgn,
+                                                              BOXARGS(ib), const real *dx, const real *dy,
+                                                              const real *dz, const real *eps) {
+  /* lsm_geometry3d_fort.f.in:784-903 */
+  box_t bphi = MKBOX(phi), bc = MKBOX(cv);
+  real dV = *dx * *dy * *dz, vol = (real)0.0, e = *eps;
+  LOOP_IB {
+    if (!CV_IN(A(cv, bc, i, j, k), *cv_sgn)) continue;
+    real p = A(phi, bphi, i, j, k), x = p / e;
+    if (p > e) {
+      vol = vol + dV;
+    } else if (p > -e) {
+      real H = (real)0.5 * (1 + x + ONE_OVER_PI_REF * SIN(PI_REF * x));
+      vol = vol + H * dV;
+    }
+  }
+  *out = vol;
+}
+
+void oracle_lsm3ddeltafunctionorder1_(const real *phi, real *delta, BOXARGS(g), const real *nx, const real *ny,
+                                      const real *nz, BOXARGS(n), BOXARGS(fb), const real *dx, const real *dy,
+                                      const real *dz) {
+  /* lsm_calculus_toolbox3d.f.in:15-120.  The z terms use norm_phi_x and
+   * 1/dx, as the reference writes them (:98-110). */
+  box_t bg = MKBOX(g), bn = MKBOX(n);
+  real odx = 1.0 / *dx, ody = 1.0 / *dy;
+  (void)dz;
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) {
+        real c = A(phi, bg, i, j, k), t[6];
+        const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, 1, -1};
+        for (int q = 0; q < 6; q++) {
+          real nb = A(phi, bg, i + di[q], j + dj[q], k + dk[q]);
+          const real *nf = (q < 2) ? nx : (q < 4) ? ny : nx;
+          real f = (q >= 2 && q < 4) ? ody : odx;
+          if (nb * c <= (real)0.0) {
+            t[q] = nb * A(nf, bn, i, j, k) * f;
+            t[q] = t[q] / (nb - c);
+          } else {
+            t[q] = (real)0.0;
+          }
+        }
+        A(delta, bg, i, j, k) = (real)fabs(t[0]) + (real)fabs(t[1]) + (real)fabs(t[2]) + (real)fabs(t[3]) +
+                                (real)fabs(t[4]) + (real)fabs(t[5]);
       }
 }
 

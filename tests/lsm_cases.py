@@ -135,7 +135,8 @@ def make_case(name, real="double", seed=0, fb=(-3, 8, 2, 12, 5, 14)):
                     box = grow(fb, (1, 2, 0), (2, 1, 1))
             else:
                 box = grow(fb, (gi % 3, (gi + 1) % 3, 1), (1, gi % 2, (gi + 2) % 3))
-                if name.startswith("lsm3dcomputedistance") or "subcellfix" in name:
+                if (name.startswith("lsm3dcomputedistance") or "subcellfix" in name
+                        or name == "lsm3ddeltafunctionorder1_"):
                     box = grow(fb, 1 + gi % 2)
             for a_idx in range(n):
                 kind = _value_kind(name, gi, a_idx)
@@ -162,6 +163,12 @@ def make_case(name, real="double", seed=0, fb=(-3, 8, 2, 12, 5, 14)):
         _small_gradients(case, rng, 0.0, 1)
     if name in ("lsm3dcomputeunitnormal_", "lsm3dcomputesignedunitnormal_"):
         _small_gradients(case, rng, 0.0, 3)
+    if "controlvolume" in name:
+        # cells with control_vol exactly zero: excluded by the > 0 / < 0
+        # tests, included by the voxel counts' >= 0 / <= 0
+        i_sgn = [i for i, a in enumerate(case.args) if isinstance(a, int) and not isinstance(a, bool)][0]
+        cv = case.args[i_sgn - 2]
+        cv[rng.random(cv.shape) < 0.1] = 0.0
     if name in ("lsm3dcomputesignedunitnormal_", "lsm3dphiupwindgradf_"):
         phi = [a for a in case.args if isinstance(a, np.ndarray)][-1]
         phi[rng.random(phi.shape) < 0.05] = 0.0   # phi == 0 branches
@@ -188,10 +195,16 @@ def _value_kind(name, gi, a_idx):
         if name.startswith("lsm3dupwind") and gi == 2:
             return "vel"
         return "rand"
-    if name in ("lsm3dvolumeintegralphilessthanzero_", "lsm3dvolumeintegralphigreaterthanzero_"):
+    if name.startswith("lsm3dvolumeintegralphi"):
         return "near" if gi == 1 else "rand"
     if name.startswith("lsm3dvolumeregion"):
+        return "near" if gi == 0 else "rand"
+    if name.startswith("lsm3dsurfaceintegral") and gi == 1:
         return "near"
+    if name.startswith("lsm3dsurfaceareazerolevelset") and gi == 0:
+        return "near"
+    if name.startswith("lsm3dvoxelcount") or name == "lsm3ddeltafunctionorder1_":
+        return "near" if gi == 0 else "rand"
     if name == "lsm3dcomputefieldextensioneqnrhs_" and gi == 2:
         return "near"   # |phi| around the 3 dx cut-off, sign changes between neighbours
     if name == "lsm3daddnormalveltermtolserhs_" and gi == 3:
@@ -217,11 +230,13 @@ def _scalars(name, dx, dy, dz, eps):
         return [-0.8]
     if "rk" in name:
         return [0.037]
-    if name == "lsm3dcomputestableconstnormalveldt_":
+    if name.startswith("lsm3dcomputestableconstnormalveldt"):
         return [-0.8, dx, dy, dz, 0.5]
     if "stable" in name:
         return [dx, dy, dz, 0.5]
-    if "volume" in name:
+    if "surfaceareazerolevelsetdelta" in name:
+        return [dx, dy, dz]
+    if "volume" in name or "surface" in name:
         return [dx, dy, dz, eps]
     return [dx, dy, dz]
 

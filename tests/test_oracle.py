@@ -285,6 +285,69 @@ def test_smoothed_volume_of_sphere(ora):
     assert ora.lsm3dvolumeintegralphilessthanzero_(F, gb, phi, gb, fb, h, h, h, 1.5 * h) == pytest.approx(vol, rel=1e-14)
 
 
+def test_surface_measures_of_sphere(ora):
+    """Smoothed-delta surface area of a sphere ~ 4 pi r^2 (O(eps^2) bias);
+    the surface integral of F = 1 is that area, of F = z vanishes by
+    symmetry and of F = z^2 is 4 pi r^4 / 3; voxel counts split the box."""
+    n = 40
+    fb = (0, n - 1, 0, n - 1, 0, n - 1)
+    gb = (-3, n + 2, -3, n + 2, -3, n + 2)
+    h = 2.0 / n
+    r = 0.6
+    ax = -1 + (np.arange(-3, n + 3) + 0.5) * h
+    Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")
+    rad = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    phi = rad - r
+    gx, gy, gz = X / rad, Y / rad, Z / rad          # |grad phi| = 1
+    e = 1.5 * h
+    area = ora.lsm3dsurfaceareazerolevelset_(phi, gb, gx, gy, gz, gb, fb, h, h, h, e)
+    assert area == pytest.approx(4 * np.pi * r ** 2, rel=1e-2)
+    one = np.ones_like(phi)
+    s1 = ora.lsm3dsurfaceintegral_(one, gb, phi, gb, gx, gy, gz, gb, fb, h, h, h, e)
+    assert s1 == pytest.approx(area, rel=1e-14)
+    sz = ora.lsm3dsurfaceintegral_(Z.copy(), gb, phi, gb, gx, gy, gz, gb, fb, h, h, h, e)
+    assert abs(sz) < 1e-12 * area
+    sz2 = ora.lsm3dsurfaceintegral_(Z ** 2, gb, phi, gb, gx, gy, gz, gb, fb, h, h, h, e)
+    assert sz2 == pytest.approx(4 * np.pi * r ** 4 / 3, rel=2e-2)
+    # control volume = the half-space z > 0 (sign +1) or z < 0 (sign -1)
+    up = ora.lsm3dsurfaceareazerolevelsetcontrolvolume_(phi, gb, gx, gy, gz, gb, Z.copy(), gb, 1, fb, h, h, h, e)
+    dn = ora.lsm3dsurfaceareazerolevelsetcontrolvolume_(phi, gb, gx, gy, gz, gb, Z.copy(), gb, -1, fb, h, h, h, e)
+    assert up == pytest.approx(area / 2, rel=1e-13) and dn == pytest.approx(area / 2, rel=1e-13)
+    inside = int((phi[3:-3, 3:-3, 3:-3] < 0).sum())
+    assert ora.lsm3dvoxelcountlessthanzero_(phi, gb, fb) == inside
+    assert ora.lsm3dvoxelcountgreaterthanzero_(phi, gb, fb) == n ** 3 - inside
+    # the control-volume voxel counts include cells where control_vol is 0
+    cv = np.zeros_like(phi)
+    assert ora.lsm3dvoxelcountlessthanzerocontrolvolume_(phi, gb, cv, gb, 1, fb) == inside
+    assert ora.lsm3dvoxelcountlessthanzerocontrolvolume_(phi, gb, cv, gb, -1, fb) == inside
+    f2 = phi + 0.25
+    assert ora.lsm3daveabsdiff_(phi, gb, f2, gb, fb) == pytest.approx(0.25, rel=1e-14)
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_delta_function_order1_on_planes(ora, d):
+    """Across a plane normal to axis d, the first-order delta function of a
+    column sums to n_d / dx_d per unit length (Smereka 2006); in z the
+    reference uses norm_phi_x and dx (lsm_calculus_toolbox3d.f.in:98-110),
+    so a z column sums to n_x dz / dx there."""
+    X, Y, Z = coords()
+    c = (0.43, 0.31, 0.5)[d]
+    phi = (X, Y, Z)[d] - c
+    nx, ny, nz = 0.5 * np.ones_like(phi), 0.7 * np.ones_like(phi), np.ones_like(phi)
+    delta = zeros()
+    ora.lsm3ddeltafunctionorder1_(phi, delta, GB, nx, ny, nz, GB, FB, *DX)
+    col = interior(delta).sum(axis=2 - d) * DX[d]
+    want = (0.5, 0.7, 0.5 * DX[2] / DX[0])[d]
+    np.testing.assert_allclose(col, want, rtol=1e-13)
+    # area of the plane inside the box from the delta function and |grad phi| = 1
+    g = [zeros(), zeros(), zeros()]
+    g[d][...] = 1.0
+    area = ora.lsm3dsurfaceareazerolevelsetdelta_(delta, GB, *g, GB, FB, *DX)
+    cells = [FB[1] - FB[0] + 1, FB[3] - FB[2] + 1, FB[5] - FB[4] + 1]
+    side = np.prod([cells[q] * DX[q] for q in range(3) if q != d])
+    assert area == pytest.approx(side * want, rel=1e-13)
+
+
 def test_smoothed_heaviside(ora):
     """lsmHeaviside: 0 below -eps, 1 above eps, 1/2 at 0, H(x) + H(-x) = 1 up
     to the reference's 14-digit 1/pi, nondecreasing."""
