@@ -483,6 +483,56 @@ void lsm3ddeltafunctionorder1_(const LSMLIB_REAL *phi, LSMLIB_REAL *delta, LSM_A
                                LSM_AMD_BOX(fb), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
                                const LSMLIB_REAL *dz);
 
+/* --- curvature: src/toolbox/geometry/lsm_curvature3d.h --- */
+/* Mean curvature (numerator over |grad phi|^3) and Gaussian curvature
+ * (over |grad phi|^4) from phi and its precomputed first derivatives; 0 where
+ * that power of grad_phi_mag is below zero_tol.  Order 2 reads phi one cell
+ * around the fill box (diagonals included); order 4 reads phi two cells out
+ * along each axis and the gradients two cells out (phi_x along y, phi_y along
+ * z, phi_z along x). */
+/* LSM3D_COMPUTE_MEAN_CURVATURE_ORDER2, lsm_curvature3d.h:68 */
+void lsm3dcomputemeancurvatureorder2_(LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb), const LSMLIB_REAL *phi,
+                                      LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *phi_x,
+                                      const LSMLIB_REAL *phi_y, const LSMLIB_REAL *phi_z,
+                                      const LSMLIB_REAL *grad_phi_mag, LSM_AMD_BOX(grad_phi_gb),
+                                      LSM_AMD_BOX(kappa_fb), const LSMLIB_REAL *dx,
+                                      const LSMLIB_REAL *dy, const LSMLIB_REAL *dz);
+/* LSM3D_COMPUTE_GAUSSIAN_CURVATURE_ORDER2, lsm_curvature3d.h:128 */
+void lsm3dcomputegaussiancurvatureorder2_(LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb),
+                                          const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                                          const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
+                                          const LSMLIB_REAL *phi_z, const LSMLIB_REAL *grad_phi_mag,
+                                          LSM_AMD_BOX(grad_phi_gb), LSM_AMD_BOX(kappa_fb),
+                                          const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                          const LSMLIB_REAL *dz);
+/* LSM3D_COMPUTE_MEAN_CURVATURE_ORDER4, lsm_curvature3d.h:183 */
+void lsm3dcomputemeancurvatureorder4_(LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb), const LSMLIB_REAL *phi,
+                                      LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *phi_x,
+                                      const LSMLIB_REAL *phi_y, const LSMLIB_REAL *phi_z,
+                                      const LSMLIB_REAL *grad_phi_mag, LSM_AMD_BOX(grad_phi_gb),
+                                      LSM_AMD_BOX(fb), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                      const LSMLIB_REAL *dz);
+/* LSM3D_COMPUTE_GAUSSIAN_CURVATURE_ORDER4, lsm_curvature3d.h:244 */
+void lsm3dcomputegaussiancurvatureorder4_(LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb),
+                                          const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+                                          const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
+                                          const LSMLIB_REAL *phi_z, const LSMLIB_REAL *grad_phi_mag,
+                                          LSM_AMD_BOX(grad_phi_gb), LSM_AMD_BOX(kappa_fb),
+                                          const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                          const LSMLIB_REAL *dz);
+/* LSM3D_ADD_CONST_PRECOMPUTED_CURV_TERM_TO_LSE_RHS, lsm_level_set_evolution3d.h:334.
+ * lse_rhs += b kappa grad_phi_mag over the cells the reference loops over:
+ * i over the whole rhs ghost box, j and k from the fill box's lower end to
+ * the rhs ghost box's upper end (lsm_level_set_evolution3d.f.in:514-516);
+ * kappa and grad_phi_mag must cover those cells.  The reference header
+ * declares lse_rhs const and kappa non-const; the routine writes lse_rhs
+ * and only reads kappa. */
+void lsm3daddconstprecomputedcurvtermtolserhs_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(rhs_gb),
+                                               const LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb),
+                                               const LSMLIB_REAL *grad_phi_mag,
+                                               LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *b,
+                                               LSM_AMD_BOX(rhs_fb));
+
 /* --- ghost fill: src/toolbox/boundary_conditions/lsm_boundary_conditions3d.h --- */
 /* bdry_location_idx: 0 x_lo, 1 x_hi, 2 y_lo, 3 y_hi, 4 z_lo, 5 z_hi */
 /* LSM3D_LINEAR_EXTRAPOLATION, lsm_boundary_conditions3d.h:71 */
@@ -890,6 +940,43 @@ void lsm3dcomputestableconstnormalveldtlocal_(LSMLIB_REAL *dt, const LSMLIB_REAL
                                               const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
                                               const unsigned char *mark_fb);
 
+/* --- narrow-band kernels: src/toolbox/geometry/lsm_curvature3d_local.h --- */
+/* As the box routines above, at the listed points marked <= mark_fb. */
+/* LSM3D_COMPUTE_MEAN_CURVATURE_ORDER2_LOCAL, lsm_curvature3d_local.h:72 */
+void lsm3dcomputemeancurvatureorder2local_(
+    LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+    const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y, const LSMLIB_REAL *phi_z,
+    const LSMLIB_REAL *grad_phi_mag, LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *dx,
+    const LSMLIB_REAL *dy, const LSMLIB_REAL *dz, const int *index_x, const int *index_y,
+    const int *index_z, const int *nlo_index, const int *nhi_index,
+    const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb), const unsigned char *mark_fb);
+/* LSM3D_COMPUTE_GAUSSIAN_CURVATURE_ORDER2_LOCAL, lsm_curvature3d_local.h:141 */
+void lsm3dcomputegaussiancurvatureorder2local_(
+    LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+    const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y, const LSMLIB_REAL *phi_z,
+    const LSMLIB_REAL *grad_phi_mag, LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *dx,
+    const LSMLIB_REAL *dy, const LSMLIB_REAL *dz, const int *index_x, const int *index_y,
+    const int *index_z, const int *nlo_index, const int *nhi_index,
+    const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb), const unsigned char *mark_fb);
+/* LSM3D_COMPUTE_MEAN_CURVATURE_ORDER4_LOCAL, lsm_curvature3d_local.h:210.
+ * The reference's loop body computes the fourth-order GAUSSIAN curvature
+ * (numerator and |grad phi|^4, lsm_curvature3d_local.f.in:403-417); this
+ * drop-in returns the same. */
+void lsm3dcomputemeancurvatureorder4local_(
+    LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+    const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y, const LSMLIB_REAL *phi_z,
+    const LSMLIB_REAL *grad_phi_mag, LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *dx,
+    const LSMLIB_REAL *dy, const LSMLIB_REAL *dz, const int *index_x, const int *index_y,
+    const int *index_z, const int *nlo_index, const int *nhi_index,
+    const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb), const unsigned char *mark_fb);
+/* LSM3D_COMPUTE_GAUSSIAN_CURVATURE_ORDER4_LOCAL, lsm_curvature3d_local.h:279 */
+void lsm3dcomputegaussiancurvatureorder4local_(
+    LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb), const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb),
+    const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y, const LSMLIB_REAL *phi_z,
+    const LSMLIB_REAL *grad_phi_mag, LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *dx,
+    const LSMLIB_REAL *dy, const LSMLIB_REAL *dz, const int *index_x, const int *index_y,
+    const int *index_z, const int *nlo_index, const int *nhi_index,
+    const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb), const unsigned char *mark_fb);
 
 /* ======================================================================
  * Part 2: runtime control (no counterpart in the reference: the Fortran

@@ -217,6 +217,17 @@ struct OpCentral4 {  // lsm_spatial_derivatives3d_local.f.in:936-955
   }
 };
 
+template <typename T, int ORDER, bool GAUSS>
+struct OpCurvature {  // lsm_curvature3d_local.f.in
+  Field<T> kappa;
+  Field<const T> phi, px, py, pz, gm;
+  CurvCoef<T> c;
+  __device__ void operator()(int i, int j, int k, bool m) const {
+    if (!m) return;
+    kappa.at(i, j, k) = curvature_at<T, ORDER, GAUSS>(phi, px, py, pz, gm, i, j, k, c);
+  }
+};
+
 template <typename T>
 struct OpLaplacian {  // lsm_spatial_derivatives3d_local.f.in:1040-1055
   Field<T> lap;
@@ -1433,6 +1444,53 @@ void lsm3dcentralgradorder4local_(R* gx, R* gy, R* gz, BOXP(g), const R* phi, BO
                                   const int* nhi, const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
   central_local("LSM3D_CENTRAL_GRAD_ORDER4_LOCAL", 4, gx, gy, gz, BOX(g), phi, BOX(phi), *dx, *dy, *dz, ix, iy, iz,
                 *nlo, *nhi, nb, BOX(nb), mark_fb);
+}
+
+}  // extern "C"
+
+namespace {
+
+template <int ORDER, bool GAUSS>
+void curvature_local(const char* fn, R* kappa, Box gk, const R* phi, Box gp, const R* px, const R* py, const R* pz,
+                     const R* gm, Box gg, R dx, R dy, R dz, const int* ix, const int* iy, const int* iz, int nlo,
+                     int nhi, const unsigned char* nb, Box gnb, const unsigned char* mark_fb) {
+  Call c(fn);
+  ListArgs L = stage_lists(c, ix, iy, iz, nlo, nhi);
+  if (L.ok) {
+    constexpr int r = ORDER / 2, rg = ORDER == 4 ? 2 : 0;
+    OpCurvature<R, ORDER, GAUSS> op{c.out(kappa, gk), c.in(phi, gp), c.in(px, gg), c.in(py, gg),
+                                    c.in(pz, gg),     c.in(gm, gg),  curv_coef<ORDER>(dx, dy, dz)};
+    run_entries(c, "k_entries<curvature>", L, nlo, nhi, marks(c, nb, gnb, mark_fb),
+                isect(isect(isect(gk, shrink(gp, r, r, r)), shrink(gg, rg, rg, rg)), gnb), op);
+  }
+  finish_local(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+#define CURV_LOCAL_ARGS                                                                                       \
+  R *kappa, BOXP(k), const R *phi, BOXP(phi), const R *px, const R *py, const R *pz, const R *gm, BOXP(g),    \
+      const R *dx, const R *dy, const R *dz, const int *ix, const int *iy, const int *iz, const int *nlo,     \
+      const int *nhi, const unsigned char *nb, BOXP(nb), const unsigned char *mark_fb
+#define CURV_LOCAL_PASS                                                                                       \
+  kappa, BOX(k), phi, BOX(phi), px, py, pz, gm, BOX(g), *dx, *dy, *dz, ix, iy, iz, *nlo, *nhi, nb, BOX(nb), mark_fb
+
+void lsm3dcomputemeancurvatureorder2local_(CURV_LOCAL_ARGS) {
+  curvature_local<2, false>("LSM3D_COMPUTE_MEAN_CURVATURE_ORDER2_LOCAL", CURV_LOCAL_PASS);
+}
+void lsm3dcomputegaussiancurvatureorder2local_(CURV_LOCAL_ARGS) {
+  curvature_local<2, true>("LSM3D_COMPUTE_GAUSSIAN_CURVATURE_ORDER2_LOCAL", CURV_LOCAL_PASS);
+}
+// The reference's loop body of this routine divides the Gaussian numerator by
+// |grad phi|^4 (lsm_curvature3d_local.f.in:403-417): it returns the
+// fourth-order Gaussian curvature, and so does this drop-in.
+void lsm3dcomputemeancurvatureorder4local_(CURV_LOCAL_ARGS) {
+  curvature_local<4, true>("LSM3D_COMPUTE_MEAN_CURVATURE_ORDER4_LOCAL", CURV_LOCAL_PASS);
+}
+void lsm3dcomputegaussiancurvatureorder4local_(CURV_LOCAL_ARGS) {
+  curvature_local<4, true>("LSM3D_COMPUTE_GAUSSIAN_CURVATURE_ORDER4_LOCAL", CURV_LOCAL_PASS);
 }
 
 void lsm3dlaplacianorder2local_(R* lap, BOXP(l), const R* phi, BOXP(phi), const R* dx, const R* dy, const R* dz,

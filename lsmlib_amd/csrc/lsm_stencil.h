@@ -229,6 +229,105 @@ __device__ __forceinline__ T curvature_term(T x, T y, T z, T xx, T xy, T xz, T y
   return n / g2;
 }
 
+// Mean and Gaussian curvature from phi, its given first derivatives and
+// |grad phi| (lsm_curvature3d.f.in).  Coefficients, computed on the host as
+// FP64 quotients rounded to LSMLIB_REAL as the reference's assignments do:
+//   order 2: f2 = 1/dx_d^2, fm = 0.25/(dx dy), 0.25/(dx dz), 0.25/(dy dz)
+//   order 4: f2 = (1/12)/dx_d^2, fm = (1/12)/dx, (1/12)/dy, (1/12)/dz
+template <typename T>
+struct CurvCoef {
+  T f2[3];
+  T fm[3];
+};
+
+template <int ORDER, typename T>
+inline CurvCoef<T> curv_coef(T dx, T dy, T dz) {
+  const double c12 = 0.0833333333333333333333;  // the reference's 1/12 literal
+  const double d[3] = {double(dx), double(dy), double(dz)};
+  CurvCoef<T> c;
+  for (int q = 0; q < 3; q++) c.f2[q] = T((ORDER == 2 ? 1.0 : c12) / d[q] / d[q]);
+  if (ORDER == 2) {
+    c.fm[0] = T(0.25 / d[0] / d[1]);
+    c.fm[1] = T(0.25 / d[0] / d[2]);
+    c.fm[2] = T(0.25 / d[1] / d[2]);
+  } else {
+    for (int q = 0; q < 3; q++) c.fm[q] = T(c12 / d[q]);
+  }
+  return c;
+}
+
+// Second derivatives at (i,j,k).  Order 2 (:90-101): the pure ones are FP64
+// sums (the 2.0d0 literal promotes them) rounded to LSMLIB_REAL, the mixed ones
+// LSMLIB_REAL.  Order 4 (:330-349): LSMLIB_REAL throughout; the mixed ones
+// differentiate phi_x along y, phi_y along z and phi_z along x.
+template <typename T, int ORDER>
+__device__ __forceinline__ void curv_hessian(const Field<const T>& phi, const Field<const T>& px,
+                                             const Field<const T>& py, const Field<const T>& pz, int i, int j,
+                                             int k, const CurvCoef<T>& c, T& xx, T& yy, T& zz, T& xy, T& xz,
+                                             T& yz) {
+#pragma clang fp contract(off)
+  if (ORDER == 2) {
+    const double p0 = 2.0 * double(phi.at(i, j, k));
+    xx = T(((double(phi.at(i + 1, j, k)) - p0) + double(phi.at(i - 1, j, k))) * double(c.f2[0]));
+    yy = T(((double(phi.at(i, j + 1, k)) - p0) + double(phi.at(i, j - 1, k))) * double(c.f2[1]));
+    zz = T(((double(phi.at(i, j, k + 1)) - p0) + double(phi.at(i, j, k - 1))) * double(c.f2[2]));
+    xy = (((phi.at(i + 1, j + 1, k) - phi.at(i + 1, j - 1, k)) - phi.at(i - 1, j + 1, k)) +
+          phi.at(i - 1, j - 1, k)) * c.fm[0];
+    xz = (((phi.at(i + 1, j, k + 1) - phi.at(i + 1, j, k - 1)) - phi.at(i - 1, j, k + 1)) +
+          phi.at(i - 1, j, k - 1)) * c.fm[1];
+    yz = (((phi.at(i, j + 1, k + 1) - phi.at(i, j + 1, k - 1)) - phi.at(i, j - 1, k + 1)) +
+          phi.at(i, j - 1, k - 1)) * c.fm[2];
+  } else {
+    const T s16 = T(16), s30 = T(30), e8 = T(8), c30 = s30 * phi.at(i, j, k);
+    xx = ((((-phi.at(i + 2, j, k) + s16 * phi.at(i + 1, j, k)) - c30) - phi.at(i - 2, j, k)) +
+          s16 * phi.at(i - 1, j, k)) * c.f2[0];
+    yy = ((((-phi.at(i, j + 2, k) + s16 * phi.at(i, j + 1, k)) - c30) - phi.at(i, j - 2, k)) +
+          s16 * phi.at(i, j - 1, k)) * c.f2[1];
+    zz = ((((-phi.at(i, j, k + 2) + s16 * phi.at(i, j, k + 1)) - c30) - phi.at(i, j, k - 2)) +
+          s16 * phi.at(i, j, k - 1)) * c.f2[2];
+    xy = (((-px.at(i, j + 2, k) + e8 * px.at(i, j + 1, k)) + px.at(i, j - 2, k)) - e8 * px.at(i, j - 1, k)) *
+         c.fm[1];
+    yz = (((-py.at(i, j, k + 2) + e8 * py.at(i, j, k + 1)) + py.at(i, j, k - 2)) - e8 * py.at(i, j, k - 1)) *
+         c.fm[2];
+    xz = (((-pz.at(i + 2, j, k) + e8 * pz.at(i + 1, j, k)) + pz.at(i - 2, j, k)) - e8 * pz.at(i - 1, j, k)) *
+         c.fm[0];
+  }
+}
+
+// kappa at (i,j,k): the mean curvature's numerator over |grad phi|^3
+// (:103-120) or the Gaussian's over |grad phi|^4 (:224-239); 0 where that
+// power is below zero_tol.
+template <typename T, int ORDER, bool GAUSS>
+__device__ __forceinline__ T curvature_at(const Field<const T>& phi, const Field<const T>& px,
+                                          const Field<const T>& py, const Field<const T>& pz,
+                                          const Field<const T>& gm, int i, int j, int k, const CurvCoef<T>& c) {
+#pragma clang fp contract(off)
+  const T g = gm.at(i, j, k);
+  const T den = GAUSS ? ((g * g) * g) * g : (g * g) * g;
+  if (den < Tol<T>::zero) return T(0);
+  T xx, yy, zz, xy, xz, yz;
+  curv_hessian<T, ORDER>(phi, px, py, pz, i, j, k, c, xx, yy, zz, xy, xz, yz);
+  const T x = px.at(i, j, k), y = py.at(i, j, k), z = pz.at(i, j, k);
+  T n;
+  if (GAUSS) {
+    n = (x * x) * (yy * zz - yz * yz) + (y * y) * (xx * zz - xz * xz);
+    n = n + (z * z) * (xx * yy - xy * xy);
+    T m = (x * y) * (xz * yz - xy * zz) + (y * z) * (xy * xz - yz * xx);
+    m = m + (x * z) * (xy * yz - xz * yy);
+    n = n + T(2) * m;
+  } else {
+    n = (xx * y) * y + (yy * x) * x;
+    n = n - ((T(2) * xy) * x) * y;
+    n = n + (xx * z) * z;
+    n = n + (zz * x) * x;
+    n = n - ((T(2) * xz) * x) * z;
+    n = n + (yy * z) * z;
+    n = n + (zz * y) * y;
+    n = n - ((T(2) * yz) * y) * z;
+  }
+  return n / den;
+}
+
 // CFL speed of a normal velocity, lsm_utilities3d.f.in:404-429
 template <typename T>
 __device__ __forceinline__ T h_over_dx(T avn, T xp, T yp, T zp, T xm, T ym, T zm, T mdx2, T ix, T iy,

@@ -141,6 +141,26 @@ __global__ void __launch_bounds__(256)
   LSM_CELL_LOOP_END
 }
 
+// Mean / Gaussian curvature, lsm_curvature3d.f.in (one thread per cell)
+template <typename T, int ORDER, bool GAUSS>
+__global__ void __launch_bounds__(256)
+    k_curvature(Field<T> kappa, Field<const T> phi, Field<const T> px, Field<const T> py, Field<const T> pz,
+                Field<const T> gm, Box fb, CurvCoef<T> c) {
+  LSM_CELL_LOOP_BEGIN(fb)
+  kappa.at(i, j, k) = curvature_at<T, ORDER, GAUSS>(phi, px, py, pz, gm, i, j, k, c);
+  LSM_CELL_LOOP_END
+}
+
+// rhs += b kappa |grad phi|, lsm_level_set_evolution3d.f.in:513-521
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_precomputed_curv(Field<T> rhs, Field<const T> kappa, Field<const T> gm, Box fb, T b) {
+#pragma clang fp contract(off)
+  LSM_CELL_LOOP_BEGIN(fb)
+  rhs.at(i, j, k) = rhs.at(i, j, k) + (b * kappa.at(i, j, k)) * gm.at(i, j, k);
+  LSM_CELL_LOOP_END
+}
+
 // :3586-3650 (tol = 1e-8 on the squares and on the magnitude)
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -1637,6 +1657,75 @@ void lsm3dvolumeregionphigreaterthanzerocontrolvolume_(R* out, const R* phi, BOX
                                                        const R* dz, const R* eps) {
   volume_entry("LSM3D_VOLUME_REGION_PHI_GREATER_THAN_ZERO_CONTROL_VOLUME", RK_VOL_REG_GT, out, nullptr, Box{},
                phi, BOX(phi), cv, BOX(cv), *cv_sgn, BOX(ib), *dx, *dy, *dz, *eps);
+}
+
+// ---- curvature (lsm_curvature3d.h) -------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+template <int ORDER, bool GAUSS>
+void curvature_entry(const char* fn, R* kappa, Box gk, const R* phi, Box gphi, const R* px, const R* py,
+                     const R* pz, const R* gm, Box gg, Box fb, R dx, R dy, R dz) {
+  Call c(fn);
+  require_inside(fn, "kappa", gk, fb, 0, 0, 0);
+  require_inside(fn, "phi", gphi, fb, ORDER / 2, ORDER / 2, ORDER / 2);
+  // order 4 differentiates phi_z along x, phi_x along y and phi_y along z
+  require_inside(fn, "grad_phi", gg, fb, ORDER == 4 ? 2 : 0, ORDER == 4 ? 2 : 0, ORDER == 4 ? 2 : 0);
+  auto fk = c.out(kappa, gk);
+  auto fp = c.in(phi, gphi);
+  auto fx = c.in(px, gg), fy = c.in(py, gg), fz = c.in(pz, gg), fg = c.in(gm, gg);
+  launch_cells(c, "k_curvature", fb, k_curvature<R, ORDER, GAUSS>, fk, fp, fx, fy, fz, fg, fb,
+               curv_coef<ORDER>(dx, dy, dz));
+  c.finish();
+}
+
+}  // namespace
+
+extern "C" {
+
+void lsm3dcomputemeancurvatureorder2_(R* kappa, BOXP(k), const R* phi, BOXP(phi), const R* px, const R* py,
+                                      const R* pz, const R* gm, BOXP(g), BOXP(fb), const R* dx, const R* dy,
+                                      const R* dz) {
+  curvature_entry<2, false>("LSM3D_COMPUTE_MEAN_CURVATURE_ORDER2", kappa, BOX(k), phi, BOX(phi), px, py, pz, gm,
+                            BOX(g), BOX(fb), *dx, *dy, *dz);
+}
+void lsm3dcomputegaussiancurvatureorder2_(R* kappa, BOXP(k), const R* phi, BOXP(phi), const R* px, const R* py,
+                                          const R* pz, const R* gm, BOXP(g), BOXP(fb), const R* dx, const R* dy,
+                                          const R* dz) {
+  curvature_entry<2, true>("LSM3D_COMPUTE_GAUSSIAN_CURVATURE_ORDER2", kappa, BOX(k), phi, BOX(phi), px, py, pz,
+                           gm, BOX(g), BOX(fb), *dx, *dy, *dz);
+}
+void lsm3dcomputemeancurvatureorder4_(R* kappa, BOXP(k), const R* phi, BOXP(phi), const R* px, const R* py,
+                                      const R* pz, const R* gm, BOXP(g), BOXP(fb), const R* dx, const R* dy,
+                                      const R* dz) {
+  curvature_entry<4, false>("LSM3D_COMPUTE_MEAN_CURVATURE_ORDER4", kappa, BOX(k), phi, BOX(phi), px, py, pz, gm,
+                            BOX(g), BOX(fb), *dx, *dy, *dz);
+}
+void lsm3dcomputegaussiancurvatureorder4_(R* kappa, BOXP(k), const R* phi, BOXP(phi), const R* px, const R* py,
+                                          const R* pz, const R* gm, BOXP(g), BOXP(fb), const R* dx, const R* dy,
+                                          const R* dz) {
+  curvature_entry<4, true>("LSM3D_COMPUTE_GAUSSIAN_CURVATURE_ORDER4", kappa, BOX(k), phi, BOX(phi), px, py, pz,
+                           gm, BOX(g), BOX(fb), *dx, *dy, *dz);
+}
+
+void lsm3daddconstprecomputedcurvtermtolserhs_(R* rhs, BOXP(r), const R* kappa, BOXP(k), const R* gm, BOXP(g),
+                                               const R* b, BOXP(fb)) {
+  const char* fn = "LSM3D_ADD_CONST_PRECOMPUTED_CURV_TERM_TO_LSE_RHS";
+  Call c(fn);
+  const Box gr = BOX(r), fb = BOX(fb);
+  // lsm_level_set_evolution3d.f.in:514-516: k from the fill box's klo to the
+  // ghost box's khi, j likewise, i over the whole ghost box
+  const Box d{gr.ilo, gr.ihi, fb.jlo, gr.jhi, fb.klo, gr.khi};
+  if (d.cells() > 0) {
+    require_inside(fn, "lse_rhs", gr, d, 0, 0, 0);
+    require_inside(fn, "kappa", BOX(k), d, 0, 0, 0);
+    require_inside(fn, "grad_phi_mag", BOX(g), d, 0, 0, 0);
+  }
+  auto fr = c.out(rhs, gr);
+  auto fk = c.in(kappa, BOX(k)), fg = c.in(gm, BOX(g));
+  launch_cells(c, "k_precomputed_curv", d, k_precomputed_curv<R>, fr, fk, fg, d, *b);
+  c.finish();
 }
 
 // ---- boundary conditions (lsm_boundary_conditions3d.h) ---------------------------

@@ -1603,6 +1603,121 @@ void oracle_lsm3ddeltafunctionorder1_(const real *phi, real *delta, BOXARGS(g), 
 }
 
 /* ---------------------------------------------------------------------- */
+/* Mean and Gaussian curvature: lsm_curvature3d.f.in                        */
+/* ---------------------------------------------------------------------- */
+/* Second derivatives of one cell.  Order 2 (:78-101): the pure ones are
+ * computed in double (the 2.0d0 literal promotes the sum) and rounded, the
+ * mixed ones in LSMLIB_REAL; the factors 1.d0/dx/dx and 0.25d0/dx/dy are
+ * double quotients rounded to LSMLIB_REAL.  Order 4 (:318-349): sixteen,
+ * thirty and eight are LSMLIB_REAL parameters, the mixed derivatives are
+ * fourth-order differences of the given first derivatives (phi_x along y,
+ * phi_y along z, phi_z along x). */
+typedef struct {
+  real xx, yy, zz, xy, xz, yz;
+} hess_t;
+
+static hess_t hessian(int order, const real *phi, box_t bp, const real *px, const real *py, const real *pz,
+                      box_t bg, int i, int j, int k, double dx, double dy, double dz) {
+  hess_t h;
+#define P(a, b, c) A(phi, bp, a, b, c)
+  if (order == 2) {
+    real fxx = (real)(1.0 / dx / dx), fyy = (real)(1.0 / dy / dy), fzz = (real)(1.0 / dz / dz);
+    real fxy = (real)(0.25 / dx / dy), fxz = (real)(0.25 / dx / dz), fyz = (real)(0.25 / dy / dz);
+    h.xx = (real)(((double)P(i + 1, j, k) - 2.0 * (double)P(i, j, k) + (double)P(i - 1, j, k)) * (double)fxx);
+    h.yy = (real)(((double)P(i, j + 1, k) - 2.0 * (double)P(i, j, k) + (double)P(i, j - 1, k)) * (double)fyy);
+    h.zz = (real)(((double)P(i, j, k + 1) - 2.0 * (double)P(i, j, k) + (double)P(i, j, k - 1)) * (double)fzz);
+    h.xy = (P(i + 1, j + 1, k) - P(i + 1, j - 1, k) - P(i - 1, j + 1, k) + P(i - 1, j - 1, k)) * fxy;
+    h.xz = (P(i + 1, j, k + 1) - P(i + 1, j, k - 1) - P(i - 1, j, k + 1) + P(i - 1, j, k - 1)) * fxz;
+    h.yz = (P(i, j + 1, k + 1) - P(i, j + 1, k - 1) - P(i, j - 1, k + 1) + P(i, j - 1, k - 1)) * fyz;
+  } else {
+    const double c12 = 0.0833333333333333333333;
+    real f2x = (real)(c12 / dx / dx), f2y = (real)(c12 / dy / dy), f2z = (real)(c12 / dz / dz);
+    real f1x = (real)(c12 / dx), f1y = (real)(c12 / dy), f1z = (real)(c12 / dz);
+    const real s16 = (real)16.0, s30 = (real)30.0, e8 = (real)8.0;
+    h.xx = (-P(i + 2, j, k) + s16 * P(i + 1, j, k) - s30 * P(i, j, k) - P(i - 2, j, k) + s16 * P(i - 1, j, k)) * f2x;
+    h.yy = (-P(i, j + 2, k) + s16 * P(i, j + 1, k) - s30 * P(i, j, k) - P(i, j - 2, k) + s16 * P(i, j - 1, k)) * f2y;
+    h.zz = (-P(i, j, k + 2) + s16 * P(i, j, k + 1) - s30 * P(i, j, k) - P(i, j, k - 2) + s16 * P(i, j, k - 1)) * f2z;
+    h.xy = (-A(px, bg, i, j + 2, k) + e8 * A(px, bg, i, j + 1, k) + A(px, bg, i, j - 2, k) -
+            e8 * A(px, bg, i, j - 1, k)) * f1y;
+    h.yz = (-A(py, bg, i, j, k + 2) + e8 * A(py, bg, i, j, k + 1) + A(py, bg, i, j, k - 2) -
+            e8 * A(py, bg, i, j, k - 1)) * f1z;
+    h.xz = (-A(pz, bg, i + 2, j, k) + e8 * A(pz, bg, i + 1, j, k) + A(pz, bg, i - 2, j, k) -
+            e8 * A(pz, bg, i - 1, j, k)) * f1x;
+  }
+#undef P
+  return h;
+}
+
+/* kappa = (sum of the three 2x2 terms) / |grad phi|^3 (mean, :103-120) or
+ * (the cofactor expansion) / |grad phi|^4 (Gaussian, :224-239); 0 where
+ * that power of |grad phi| is below zero_tol. */
+static void curvature(int order, int gauss, real *kappa, box_t bk, const real *phi, box_t bp, const real *px,
+                      const real *py, const real *pz, const real *gm, box_t bg, int ilo, int ihi, int jlo, int jhi,
+                      int klo, int khi, double dx, double dy, double dz) {
+  for (int k = klo; k <= khi; k++)
+    for (int j = jlo; j <= jhi; j++)
+      for (int i = ilo; i <= ihi; i++) {
+        hess_t h = hessian(order, phi, bp, px, py, pz, bg, i, j, k, dx, dy, dz);
+        real x = A(px, bg, i, j, k), y = A(py, bg, i, j, k), z = A(pz, bg, i, j, k), g = A(gm, bg, i, j, k);
+        real den = gauss ? g * g * g * g : g * g * g;
+        if (den < ZERO_TOL) {
+          A(kappa, bk, i, j, k) = (real)0.0;
+          continue;
+        }
+        real t;
+        if (gauss)
+          t = x * x * (h.yy * h.zz - h.yz * h.yz) + y * y * (h.xx * h.zz - h.xz * h.xz) +
+              z * z * (h.xx * h.yy - h.xy * h.xy) +
+              2 * (x * y * (h.xz * h.yz - h.xy * h.zz) + y * z * (h.xy * h.xz - h.yz * h.xx) +
+                   x * z * (h.xy * h.yz - h.xz * h.yy));
+        else
+          t = h.xx * y * y + h.yy * x * x - 2 * h.xy * x * y + h.xx * z * z + h.zz * x * x - 2 * h.xz * x * z +
+              h.yy * z * z + h.zz * y * y - 2 * h.yz * y * z;
+        A(kappa, bk, i, j, k) = t / den;
+      }
+}
+
+#define CURV_ARGS                                                                                        \
+  real *kappa, BOXARGS(k), const real *phi, BOXARGS(phi), const real *px, const real *py, const real *pz, \
+      const real *gm, BOXARGS(g)
+#define CURV_CALL(order, gauss)                                                                             \
+  curvature(order, gauss, kappa, MKBOX(k), phi, MKBOX(phi), px, py, pz, gm, MKBOX(g), *ilo_fb, *ihi_fb, *jlo_fb, \
+            *jhi_fb, *klo_fb, *khi_fb, *dx, *dy, *dz)
+
+void oracle_lsm3dcomputemeancurvatureorder2_(CURV_ARGS, BOXARGS(fb), const real *dx, const real *dy,
+                                             const real *dz) {
+  /* lsm_curvature3d.f.in:15-129 */
+  CURV_CALL(2, 0);
+}
+void oracle_lsm3dcomputegaussiancurvatureorder2_(CURV_ARGS, BOXARGS(fb), const real *dx, const real *dy,
+                                                 const real *dz) {
+  /* lsm_curvature3d.f.in:136-248 */
+  CURV_CALL(2, 1);
+}
+void oracle_lsm3dcomputemeancurvatureorder4_(CURV_ARGS, BOXARGS(fb), const real *dx, const real *dy,
+                                             const real *dz) {
+  /* lsm_curvature3d.f.in:253-376 */
+  CURV_CALL(4, 0);
+}
+void oracle_lsm3dcomputegaussiancurvatureorder4_(CURV_ARGS, BOXARGS(fb), const real *dx, const real *dy,
+                                                 const real *dz) {
+  /* lsm_curvature3d.f.in:383-507 */
+  CURV_CALL(4, 1);
+}
+
+void oracle_lsm3daddconstprecomputedcurvtermtolserhs_(real *rhs, BOXARGS(r), const real *kappa, BOXARGS(k),
+                                                      const real *gm, BOXARGS(g), const real *b, BOXARGS(fb)) {
+  /* lsm_level_set_evolution3d.f.in:465-526.  The loops run k from the fill
+   * box's klo to the ghost box's khi, j likewise, and i over the whole ghost
+   * box (:514-516), as the reference writes them. */
+  box_t br = MKBOX(r), bk = MKBOX(k), bg = MKBOX(g);
+  for (int k = *klo_fb; k <= *khi_r; k++)
+    for (int j = *jlo_fb; j <= *jhi_r; j++)
+      for (int i = *ilo_r; i <= *ihi_r; i++)
+        A(rhs, br, i, j, k) = A(rhs, br, i, j, k) + *b * A(kappa, bk, i, j, k) * A(gm, bg, i, j, k);
+}
+
+/* ---------------------------------------------------------------------- */
 /* Narrow band (localization): lsm_localization3d.f and the *LOCAL kernels  */
 /* Index lists index_x/y/z(nlo:nhi) and the integer*1 narrow_band marks     */
 /* (signed in the Fortran; compared as signed char here).                   */
@@ -2083,6 +2198,35 @@ void oracle_lsm3dlaplacianorder2local_(real *lap, BOXARGS(l), const real *phi, B
   FOR_MARKED(nlo_index, nhi_index)
   oracle_lsm3dlaplacianorder2_(lap, FWD(l), phi, FWD(phi), CELL_BOX, dx, dy, dz);
   END_LIST
+}
+
+#define CURV_LOCAL(order, gauss)                                                                        \
+  FOR_MARKED(nlo_index, nhi_index)                                                                      \
+  curvature(order, gauss, kappa, MKBOX(k), phi, MKBOX(phi), px, py, pz, gm, MKBOX(g), i, i, j, j, k, k, *dx, \
+            *dy, *dz);                                                                                  \
+  END_LIST
+
+void oracle_lsm3dcomputemeancurvatureorder2local_(CURV_ARGS, const real *dx, const real *dy, const real *dz, XYZ,
+                                                  NB, const nb_t *mark_fb) {
+  /* lsm_curvature3d_local.f.in:15-145 */
+  CURV_LOCAL(2, 0)
+}
+void oracle_lsm3dcomputegaussiancurvatureorder2local_(CURV_ARGS, const real *dx, const real *dy, const real *dz,
+                                                      XYZ, NB, const nb_t *mark_fb) {
+  /* lsm_curvature3d_local.f.in:151-280 */
+  CURV_LOCAL(2, 1)
+}
+void oracle_lsm3dcomputemeancurvatureorder4local_(CURV_ARGS, const real *dx, const real *dy, const real *dz, XYZ,
+                                                  NB, const nb_t *mark_fb) {
+  /* lsm_curvature3d_local.f.in:285-426.  Its loop body divides the
+   * Gaussian numerator by |grad phi|^4 (:403-417): it computes the
+   * fourth-order Gaussian curvature, as the reference does. */
+  CURV_LOCAL(4, 1)
+}
+void oracle_lsm3dcomputegaussiancurvatureorder4local_(CURV_ARGS, const real *dx, const real *dy, const real *dz,
+                                                      XYZ, NB, const nb_t *mark_fb) {
+  /* lsm_curvature3d_local.f.in:432-573 */
+  CURV_LOCAL(4, 1)
 }
 
 void oracle_lsm3dgradientmagnitudelocal_(const real *gx, const real *gy, const real *gz, real *mag, BOXARGS(g),

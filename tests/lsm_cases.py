@@ -73,6 +73,8 @@ SPATIAL = {"lsm3dhjeno1_", "lsm3dhjeno2_", "lsm3dhjeno3_", "lsm3dhjweno5_",
            "lsm3dupwindhjweno5_"}
 BC = {n for n in SIGNATURES if "extrapolation" in n or "neumann" in n}
 CENTRAL = {"lsm3dcentralgradorder2_", "lsm3dcentralgradorder4_", "lsm3dlaplacianorder2_"}
+CURVATURE = {"lsm3dcomputemeancurvatureorder2_", "lsm3dcomputegaussiancurvatureorder2_",
+             "lsm3dcomputemeancurvatureorder4_", "lsm3dcomputegaussiancurvatureorder4_"}
 # routines that read phi at the six neighbours: (routine, phi group index)
 PHI_NEIGHBOURS = {"lsm3dphiupwindgradf_": 3, "lsm3dcomputefieldextensioneqnrhs_": 2}
 
@@ -120,6 +122,15 @@ def make_case(name, real="double", seed=0, fb=(-3, 8, 2, 12, 5, 14)):
                 box = grow(fb, (3, 2, 3), (2, 3, 1))
             elif name in PHI_NEIGHBOURS and gi == PHI_NEIGHBOURS[name]:
                 box = grow(fb, (1, 2, 1), (2, 1, 1))
+            elif name in CURVATURE:
+                # kappa: small uneven ghosts; phi: 2-3; the gradients: 2 (order 4
+                # differentiates phi_x along y, phi_y along z, phi_z along x)
+                box = [grow(fb, (1, 0, 2), (0, 2, 1)), grow(fb, (2, 3, 2), (3, 2, 2)),
+                       grow(fb, (2, 2, 3), (3, 2, 2))][gi]
+            elif name == "lsm3daddconstprecomputedcurvtermtolserhs_":
+                # the reference loops over i of the rhs ghost box and j, k from
+                # the fill box's lower end to the ghost box's upper end
+                box = [grow(fb, 1), grow(fb, (2, 0, 1), (1, 2, 1)), grow(fb, (1, 1, 0), (2, 1, 3))][gi]
             elif name in CENTRAL:
                 # outputs: small uneven ghosts; phi: >= 2 ghosts (order 4 reads +-2)
                 box = grow(fb, (1, 0, 2), (0, 2, 1)) if gi == 0 else grow(fb, (2, 3, 2), (3, 2, 2))
@@ -163,6 +174,12 @@ def make_case(name, real="double", seed=0, fb=(-3, 8, 2, 12, 5, 14)):
         _small_gradients(case, rng, 0.0, 1)
     if name in ("lsm3dcomputeunitnormal_", "lsm3dcomputesignedunitnormal_"):
         _small_gradients(case, rng, 0.0, 3)
+    if name in CURVATURE:
+        # |grad phi| in [0.5, ...), and below both zero-tolerance cut-offs on a
+        # tenth of the cells
+        gm = [a for a in case.args if isinstance(a, np.ndarray)][5]
+        gm[...] = np.abs(gm) + gm.dtype.type(0.5)
+        gm[rng.random(gm.shape) < 0.1] = gm.dtype.type(1e-4)
     if "controlvolume" in name:
         # cells with control_vol exactly zero: excluded by the > 0 / < 0
         # tests, included by the voxel counts' >= 0 / <= 0
@@ -205,6 +222,8 @@ def _value_kind(name, gi, a_idx):
         return "near"
     if name.startswith("lsm3dvoxelcount") or name == "lsm3ddeltafunctionorder1_":
         return "near" if gi == 0 else "rand"
+    if name in CURVATURE:
+        return "phi" if gi == 1 else "rand"
     if name == "lsm3dcomputefieldextensioneqnrhs_" and gi == 2:
         return "near"   # |phi| around the 3 dx cut-off, sign changes between neighbours
     if name == "lsm3daddnormalveltermtolserhs_" and gi == 3:
@@ -228,6 +247,8 @@ def _scalars(name, dx, dy, dz, eps):
         return [-0.35]
     if name == "lsm3daddconstnormalveltermtolserhs_":
         return [-0.8]
+    if name == "lsm3daddconstprecomputedcurvtermtolserhs_":
+        return [0.05]
     if "rk" in name:
         return [0.037]
     if name.startswith("lsm3dcomputestableconstnormalveldt"):

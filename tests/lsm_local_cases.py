@@ -85,6 +85,10 @@ def _small(arrs, rng, v):
         a[sel] = v if d != 1 else 0.0
 
 
+def gm_floor(real):
+    return _np(real)(0.5)
+
+
 def _boxes(seed):
     """Array boxes: GB, or GB grown unevenly (the arrays need not share a box)."""
     if seed % 2 == 0:
@@ -186,6 +190,13 @@ def make_local_case(name, real="double", seed=0, lib=None):
         _small(g, rng, 0.0)
         return Case(name, real, [F(B[0]), B[0], *g, B[1], *[F(B[2]) for _ in range(6)], B[2], -0.35, *L, lo(0),
                                  hi(0), nb, GB, MARK_FB])
+    if name.startswith("lsm3dcompute") and "curvatureorder" in name:
+        # order 2 reads phi one point out, order 4 phi and the gradients two
+        mark = MARK_D1 if "order2" in name else MARK_D2
+        g = [F(B[2]) for _ in range(3)]
+        gm = np.abs(F(B[2])) + gm_floor(real)
+        gm[rng.random(gm.shape) < 0.1] = 1e-4
+        return Case(name, real, [F(B[0]), B[0], phi, GB, *g, gm, B[2], dx, dy, dz, *L, lo(0), hi(1), nb, GB, mark])
     if name == "lsm3daddconstprecomputedcurvtermtolserhslocal_":
         return Case(name, real, [F(B[0]), B[0], F(B[1]), B[1], F(B[2]), B[2], 0.05, *L, lo(0), hi(0), nb, GB,
                                  MARK_FB])

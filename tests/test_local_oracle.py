@@ -177,6 +177,13 @@ def _pair_args(name, real, rng):
                                  GB, 0, 0], [0, 1, 2, 4, 5, 6],
                 [*[a.copy() for a in o[:3]], GB, *[a.copy() for a in o[3:]], GB, phi, GB, F(), GB, FB, dx, dy, dz],
                 [0, 1, 2, 4, 5, 6])
+    if "curvatureorder" in name:
+        k, g = F(), [F() for _ in range(3)]
+        gm = np.abs(F()) + np.asarray(0.5, dt)
+        # the reference's order-4 mean local routine computes the Gaussian curvature
+        box = name.replace("local_", "_").replace("meancurvatureorder4", "gaussiancurvatureorder4")
+        return (box, [k, GB, phi, GB, *g, gm, GB, dx, dy, dz, *tail], [0],
+                [k.copy(), GB, phi, GB, *g, gm, GB, FB, dx, dy, dz], [0])
     if name == "lsm3dmaxnormdifflocal_":
         a, b = F(), F()
         return ("lsm3dmaxnormdiff_", [a, GB, b, GB, *tail], None, [a, GB, b, GB, FB], None)
@@ -201,7 +208,8 @@ RELATED = ["lsm3dcentralgradorder2local_", "lsm3dcentralgradorder4local_", "lsm3
            "lsm3daddconstcurvtermtolserhslocal_", "lsm3dcomputereinitializationeqnrhslocal_",
            "lsm3dcomputeorthogonalizationeqnrhslocal_", "lsm3dhjeno1local_", "lsm3dhjeno2local_",
            "lsm3dmaxnormdifflocal_", "lsm3dcomputestableadvectiondtlocal_", "lsm3dcomputestablenormalveldtlocal_",
-           UNDEFINED]
+           UNDEFINED, "lsm3dcomputemeancurvatureorder2local_", "lsm3dcomputegaussiancurvatureorder2local_",
+           "lsm3dcomputemeancurvatureorder4local_", "lsm3dcomputegaussiancurvatureorder4local_"]
 
 
 @pytest.mark.parametrize("real", ["double", "float"])

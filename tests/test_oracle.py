@@ -348,6 +348,104 @@ def test_delta_function_order1_on_planes(ora, d):
     assert area == pytest.approx(side * want, rel=1e-13)
 
 
+def _quadratic(quartic=0.0):
+    """phi = x^T A x / 2 + l . x + quartic x^4 with every Hessian entry
+    different, its exact gradient and Hessian (per cell)."""
+    X, Y, Z = coords()
+    H = np.array([[1.3, -0.7, 0.4], [-0.7, -0.9, 0.25], [0.4, 0.25, 0.6]])
+    lin = np.array([0.8, -0.5, 1.1])
+    phi = (0.5 * (H[0, 0] * X * X + H[1, 1] * Y * Y + H[2, 2] * Z * Z) + H[0, 1] * X * Y + H[0, 2] * X * Z
+           + H[1, 2] * Y * Z + lin[0] * X + lin[1] * Y + lin[2] * Z + quartic * X ** 4)
+    g = [H[d, 0] * X + H[d, 1] * Y + H[d, 2] * Z + lin[d] for d in range(3)]
+    g[0] = g[0] + 4 * quartic * X ** 3
+    Hc = np.broadcast_to(H, X.shape + (3, 3)).copy()
+    Hc[..., 0, 0] += 12 * quartic * X ** 2
+    return phi, g, Hc
+
+
+def _curvatures(g, Hc):
+    """Mean: (|g|^2 tr H - g.Hg) / |g|^3; Gaussian: g.adj(H)g / |g|^4."""
+    gv = np.stack(g, axis=-1)
+    n2 = (gv ** 2).sum(-1)
+    ghg = np.einsum("...i,...ij,...j->...", gv, Hc, gv)
+    mean = (n2 * np.trace(Hc, axis1=-2, axis2=-1) - ghg) / n2 ** 1.5
+    adj = np.linalg.inv(Hc) * np.linalg.det(Hc)[..., None, None]
+    gauss = np.einsum("...i,...ij,...j->...", gv, adj, gv) / n2 ** 2
+    return mean, gauss
+
+
+@pytest.mark.parametrize("order,quartic", [(2, 0.0), (4, 0.0), (4, 0.35)])
+def test_curvatures_exact_on_polynomials(ora, order, quartic):
+    """With exact first derivatives, both orders are exact on quadratics and
+    order 4 also with an x^4 term (its second differences are exact to degree
+    5), where order 2 is not."""
+    phi, g, Hc = _quadratic(quartic)
+    gm = np.sqrt(g[0] ** 2 + g[1] ** 2 + g[2] ** 2)
+    mean, gauss = _curvatures(g, Hc)
+    for kind, want in (("mean", mean), ("gaussian", gauss)):
+        kappa = zeros()
+        getattr(ora, f"lsm3dcompute{kind}curvatureorder{order}_")(kappa, GB, phi, GB, *g, gm, GB, FB, *DX)
+        np.testing.assert_allclose(interior(kappa), interior(want), rtol=1e-9, atol=1e-9)
+    if quartic:
+        kappa = zeros()
+        ora.lsm3dcomputemeancurvatureorder2_(kappa, GB, phi, GB, *g, gm, GB, FB, *DX)
+        assert np.abs(interior(kappa) - interior(mean)).max() > 1e-4
+
+
+def _sphere_curvature_errors(ora, n, order):
+    h = 2.0 / n
+    fb = (0, n - 1, 0, n - 1, 0, n - 1)
+    gb = (-3, n + 2, -3, n + 2, -3, n + 2)
+    ax = -1 + (np.arange(-3, n + 3) + 0.5) * h
+    Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    phi = r - 0.6
+    g = [X / r, Y / r, Z / r]
+    gm = np.ones_like(phi)
+    near = np.abs(phi) < 0.2
+    errs = []
+    for kind, want in (("mean", 2 / r), ("gaussian", 1 / r ** 2)):
+        kappa = np.zeros_like(phi)
+        getattr(ora, f"lsm3dcompute{kind}curvatureorder{order}_")(kappa, gb, phi, gb, *g, gm, gb, fb, h, h, h)
+        errs.append(float(np.abs(kappa - want)[near].max() / np.abs(want[near]).max()))
+    return errs
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_curvature_of_sphere_converges(ora, order):
+    """A sphere's signed distance: mean curvature 2/r and Gaussian 1/r^2,
+    the error falling at least like h^2 when the grid is refined."""
+    coarse, fine = _sphere_curvature_errors(ora, 20, order), _sphere_curvature_errors(ora, 40, order)
+    for c, f in zip(coarse, fine):
+        assert f < 0.01 and c / f > 3.5, (order, c, f)
+
+
+def test_curvature_small_gradients(ora):
+    """0 where |grad phi|^3 (mean) or |grad phi|^4 (Gaussian) is below zero_tol."""
+    phi, g, _ = _quadratic()
+    for v, mean_zero, gauss_zero in ((2e-4, True, True), (1.5e-3, False, True), (0.02, False, False)):
+        gm = np.full_like(phi, v)    # 2e-4^3 = 8e-12; 1.5e-3^3 = 3.4e-9, 1.5e-3^4 = 5.1e-12
+        km, kg = zeros() + 7.0, zeros() + 7.0
+        ora.lsm3dcomputemeancurvatureorder2_(km, GB, phi, GB, *g, gm, GB, FB, *DX)
+        ora.lsm3dcomputegaussiancurvatureorder4_(kg, GB, phi, GB, *g, gm, GB, FB, *DX)
+        assert np.all(interior(km) == 0.0) == mean_zero and np.all(interior(km) != 0.0) != mean_zero
+        assert np.all(interior(kg) == 0.0) == gauss_zero and np.all(interior(kg) != 0.0) != gauss_zero
+
+
+def test_precomputed_curvature_term_loop_range(ora):
+    """rhs += b kappa |grad phi| over i of the whole rhs ghost box and j, k
+    from the fill box's lower end to the ghost box's upper end, as the
+    reference's loops run (lsm_level_set_evolution3d.f.in:514-516)."""
+    rng = np.random.default_rng(3)
+    rhs, kappa, gm = (rng.standard_normal(box_shape(GB)) for _ in range(3))
+    before = rhs.copy()
+    ora.lsm3daddconstprecomputedcurvtermtolserhs_(rhs, GB, kappa, GB, gm, GB, 0.3, FB)
+    want = before.copy()
+    sel = (slice(FB[4] - GB[4], None), slice(FB[2] - GB[2], None), slice(None))
+    want[sel] = before[sel] + 0.3 * kappa[sel] * gm[sel]
+    assert np.array_equal(rhs, want)
+
+
 def test_smoothed_heaviside(ora):
     """lsmHeaviside: 0 below -eps, 1 above eps, 1/2 at 0, H(x) + H(-x) = 1 up
     to the reference's 14-digit 1/pi, nondecreasing."""
