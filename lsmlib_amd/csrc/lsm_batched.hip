@@ -119,6 +119,7 @@ __global__ void __launch_bounds__(TX* TY)
     a.tdx2[1] = tdx2y;
     a.tdx2[2] = tdx2z;
     a.form = form;
+    a.rkb = rk_blend_weight<T>(form);
     a.periodic = 0;
     a.use_phi0 = 0;
     int bx, by, bz;

@@ -231,6 +231,7 @@ void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, co
   a.dx_sq = mdx * mdx;  // lsm_reinitialization3d.f.in:140-141
   a.zero_tol = (R)tl_zero_tol;
   a.form = form;
+  a.rkb = rk_blend_weight<R>(form);
   a.periodic = periodic;
   a.use_phi0 = use_phi0 ? 1 : 0;
   const long long nblk = (long long)a.tg.ntx * a.tg.nty * a.tg.nkz;

@@ -108,6 +108,7 @@ struct StageArgs {
   T zero_tol;    // upwind switch / reinit cut-off (LSMLIB_ZERO_TOL of the build by default)
   T tdx2[3];     // 12 * tiny * dx^2 per axis (the WENO5 epsilon floor in undivided units)
   int form;      // RKForm
+  T rkb;         // rk_blend_weight(form), used by the FP32 (FASTW) stages
   int periodic;  // bit mask: also write phi_out's periodic ghost images
   int use_phi0;  // reinit sign from phi0 (1) or phi_s (0)
 };
@@ -270,6 +271,9 @@ struct Pending {
 
 template <typename T, int TX, int TY, int KIND, bool FASTW, bool SYNTH>
 __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by, int bz, T (*tile)[TX + 7]) {
+  // no implicit contraction: the FP32 arithmetic that fuses writes fma
+  // explicitly, so unrolled copies of a plane round alike
+#pragma clang fp contract(off)
   const int t = threadIdx.x;
   const int tx = t % TX, ty = t / TX;
   const Box& fb = a.fb;
@@ -326,10 +330,10 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
   // z column: zv[q] = phi(k + q), q = 0..3, and zd[q] = phi(k + q - 2) -
   // phi(k + q - 3), q = 0..5 (one new value and difference per plane).  The
   // loads of plane k + 1 are issued before the arithmetic of plane k (one
-  // plane of software pipelining).  The plane loop is not unrolled: every
-  // plane then runs the same compiled arithmetic, so results do not depend
-  // on the z-chunk (unrolled copies may contract multiply-adds differently).
-  constexpr int U = 1;
+  // plane of software pipelining).  FP32 planes are unrolled by two (the
+  // column shifts and the pipelined values then need no register copies);
+  // with no implicit contraction both copies compute identically.
+  constexpr int U = FASTW ? 2 : 1;
   T zv[U + 3], zd[U + 5];
   // the column's value in plane kk, applying the z-face rule outside the fill
   // box (kk is uniform; there the value only feeds the column's own z
@@ -454,9 +458,9 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
           const T gy = upwind_weno5_d<T, FASTW>(dy_, cuy, a.iy, tiny_dx2y, a.zero_tol);
           const T gz = upwind_weno5_d<T, FASTW>(dz_, cuz, a.iz, tiny_dx2z, a.zero_tol);
           // lsm_level_set_evolution3d.f.in:113-116 on a zeroed rhs
-          rhs = -(cux * gx + cuy * gy + cuz * gz);
+          rhs = FASTW ? -fma(cuz, gz, fma(cuy, gy, cux * gx)) : -(cux * gx + cuy * gy + cuz * gz);
         }
-        const T v = rk_combine<T, FASTW>(a.form, uc, us, rhs, a.dt);
+        const T v = FASTW ? rk_blend<T>(uc, us, rhs, a.dt, a.rkb) : rk_combine<T>(a.form, uc, us, rhs, a.dt);
         const bool img_z = (a.periodic & 4) && (k - fb.klo < 3 || fb.khi - k < 3);
         if (img_xy || img_z) store_with_periodic_images<T>(a.out, fb, i, j, k, v, a.periodic, 3);
         else buf_store(v, plane_rsrc(p_out), oo);

@@ -78,21 +78,24 @@ __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
 //   phi_x = p2 + w1 (p1 - p2) + w3 (p3 - p2),
 //   p2 = (-D2 + 5 D3 + 2 D4)/6 = D3 - (b + 2c)/6,
 //   p1 - p2 = (a - 2b + c)/3,  p3 - p2 = (b - 2c + d)/6.
+// The multiply-adds are written as explicit fma so that every compiled copy
+// (unrolled planes, tile shapes) rounds alike.
 template <typename T>
 __device__ __forceinline__ T weno5_fast(T D1, T D2, T D3, T D4, T D5, T tiny_dx2) {
+#pragma clang fp contract(off)
   const T a = D1 - D2, b = D2 - D3, c = D3 - D4, d = D4 - D5;
   const T m = fmax(fmax(fmax(fabs(D1), fabs(D2)), fabs(D3)), fmax(fabs(D4), fabs(D5)));
-  const T ie = fast_rcp(T(0.75e-6) * (m * m) + tiny_dx2 * T(1.0 / 16.0));  // 16 / E
+  const T ie = fast_rcp(fma(T(0.75e-6), m * m, tiny_dx2 * T(1.0 / 16.0)));  // 16 / E
   const T bb = b * b, cc = c * c;
-  const T q1 = a * (a - T(2.75) * b) + T(2.5) * bb;
-  const T q2 = (bb - T(1.25) * (b * c)) + cc;
-  const T q3 = d * (d - T(2.75) * c) + T(2.5) * cc;
-  const T s1 = q1 * ie + T(1), s2 = q2 * ie + T(1), s3 = q3 * ie + T(1);
+  const T q1 = fma(a, fma(T(-2.75), b, a), T(2.5) * bb);
+  const T q2 = fma(T(-1.25), b * c, bb) + cc;
+  const T q3 = fma(d, fma(T(-2.75), c, d), T(2.5) * cc);
+  const T s1 = fma(q1, ie, T(1)), s2 = fma(q2, ie, T(1)), s3 = fma(q3, ie, T(1));
   const T t1 = s1 * s1, t2 = s2 * s2, t3 = s3 * s3;
   const T p23 = t2 * t3, p13 = t1 * t3, p12 = t1 * t2;
-  const T rden = fast_rcp((p23 + T(6) * p13) + T(3) * p12);
-  const T x = (p23 * ((a - T(2) * b) + c)) * T(1.0 / 3.0) + (p12 * ((b - T(2) * c) + d)) * T(0.5);
-  return (D3 - (b + T(2) * c) * T(1.0 / 6.0)) + x * rden;
+  const T rden = fast_rcp(fma(T(3), p12, fma(T(6), p13, p23)));
+  const T x = fma(p23 * (fma(T(-2), b, a) + c), T(1.0 / 3.0), (p12 * (fma(T(-2), c, b) + d)) * T(0.5));
+  return fma(x, rden, fma(fma(T(2), c, b), T(-1.0 / 6.0), D3));
 }
 
 // d1[q] = D1(c+q-2), q = 0..5 (only those the scheme needs are formed).
@@ -358,6 +361,31 @@ __device__ __forceinline__ T ext_normal_select(T vc, T vn, T dp, T dm) {
 // rounded 2/3 multiplies only the step's change.  The other stage weights are
 // exact in binary.
 enum RKForm { RK_EULER = 1, RK2_STAGE2 = 22, RK3_STAGE2 = 32, RK3_STAGE3 = 33 };
+
+// The FP32 fused stages write every form as one branch-free blend,
+// uc + w (us + dt r - uc), with w = 1 (forward Euler / stage 1, where us =
+// uc), 1/2 (TVD-RK2 stage 2), 1/4 (TVD-RK3 stage 2) or 2/3 (TVD-RK3 stage 3,
+// the unbiased form above); the same values as rk_combine up to FP32
+// rounding.  The fused multiply-adds are explicit so every compiled copy
+// rounds alike.
+template <typename T>
+__host__ __device__ inline T rk_blend_weight(int form) {
+  switch (form) {
+    case RK2_STAGE2:
+      return T(0.5);
+    case RK3_STAGE2:
+      return T(0.25);
+    case RK3_STAGE3:
+      return T(2.0 / 3.0);
+    default:
+      return T(1);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T rk_blend(T uc, T us, T r, T dt, T w) {
+  return fma(w, fma(dt, r, us) - uc, uc);
+}
 
 template <typename T, bool UNBIASED = false>
 __device__ __forceinline__ T rk_combine(int form, T uc, T us, T r, T dt) {
