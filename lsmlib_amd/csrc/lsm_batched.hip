@@ -23,7 +23,10 @@
 // lsm3d_amd_patch_plan_fill_ghosts writes the face ghost layers of a pool
 // for callers that want them stored.
 #include <cstdio>
+#include <algorithm>
+#include <cstdint>
 #include <cstring>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -79,17 +82,19 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Workgroup b runs the tiles sched_tile[sched_off[b] .. sched_off[b+1]-1]
+// (a host-built schedule balancing the tiles' costs, see build_schedule).
 template <typename T, int TX, int TY, bool SYNTH>
 __global__ void __launch_bounds__(TX* TY)
     k_patch_stage(const PatchDesc* __restrict__ desc, const int64_t* __restrict__ prefix, int npatch,
-                  int64_t ntiles, T* out_pool, const T* ps_pool, const T* pc_pool, GhostRule ghost, T ix, T iy,
-                  T iz, T dt, T zt, T tdx2x, T tdx2y, T tdx2z, int form) {
+                  const int* __restrict__ sched_off, const int* __restrict__ sched_tile, T* out_pool,
+                  const T* ps_pool, const T* pc_pool, GhostRule ghost, T ix, T iy, T iz, T dt, T zt, T tdx2x,
+                  T tdx2y, T tdx2z, int form) {
   __shared__ T tile[TY + 6][TX + 7];
-  const int G = (int)gridDim.x;  // a multiple of the XCD count
   const int b = (int)blockIdx.x;
-  const int x = b % kNumXcd, local = b / kNumXcd, nloc = G / kNumXcd;
-  const int64_t lo = ntiles * x / kNumXcd, hi = ntiles * (x + 1) / kNumXcd;
-  for (int64_t t = lo + local; t < hi; t += nloc) {
+  const int q0 = sched_off[b], q1 = sched_off[b + 1];
+  for (int q = q0; q < q1; q++) {
+    const int64_t t = sched_tile[q];
     int l = 0, r = npatch - 1;
     while (l < r) {
       const int m = (l + r + 1) >> 1;
@@ -145,6 +150,8 @@ struct lsm3d_amd_patch_plan {
   std::vector<lsm::PatchDesc> host_desc;
   lsm::PatchDesc* desc = nullptr;
   int64_t* prefix = nullptr;  // tiles before each patch
+  int* sched_off = nullptr;   // grid + 1 offsets into sched_tile
+  int* sched_tile = nullptr;  // tile ids, workgroup by workgroup
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
@@ -168,10 +175,12 @@ void launch_patch_t(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const 
   const R tz2 = R(12) * Tol<R>::tiny / (iz * iz);
   if (g.any)
     hipLaunchKernelGGL((k_patch_stage<R, TX, TY, true>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
-                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, zt, tx2, ty2, tz2, form);
+                       p->prefix, p->npatch, p->sched_off, p->sched_tile, out, ps, pc, g, ix, iy, iz, dt, zt, tx2,
+                       ty2, tz2, form);
   else
     hipLaunchKernelGGL((k_patch_stage<R, TX, TY, false>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
-                       p->prefix, p->npatch, p->ntiles, out, ps, pc, g, ix, iy, iz, dt, zt, tx2, ty2, tz2, form);
+                       p->prefix, p->npatch, p->sched_off, p->sched_tile, out, ps, pc, g, ix, iy, iz, dt, zt, tx2,
+                       ty2, tz2, form);
 }
 
 // One stage over every patch.
@@ -204,6 +213,73 @@ int occupancy_blocks(int cfg) {
     n = 2;
   }
   return n;
+}
+
+// Static schedule of the persistent launch.  The tiles, in patch order, are
+// cut into eight contiguous runs of equal summed cost, one per XCD (workgroup
+// b runs on XCD b % 8), so a patch's tiles, which share halo cells, stay in
+// one L2 except where a run ends; then each XCD's tiles, costliest first, go
+// to the least loaded of its workgroups (longest processing time first).  A tile's cost is its planes plus the six planes
+// its march loads before the first, weighted up when its stencils leave the
+// fill box (ghost values synthesised from the rule).  Workgroups list their
+// tiles in that order, so tiles of similar cost from one patch run side by
+// side.
+void build_schedule(const std::vector<PatchDesc>& desc, const std::vector<int64_t>& prefix, int tx, int ty,
+                    int grid, std::vector<int>& off, std::vector<int>& tiles) {
+  const int npatch = (int)desc.size();
+  const int64_t ntiles = prefix[npatch];
+  std::vector<double> tcost(ntiles);
+  double total = 0.0;
+  for (int l = 0; l < npatch; l++) {
+    const PatchDesc& d = desc[l];
+    for (int64_t t = prefix[l]; t < prefix[l + 1]; t++) {
+      const int bx = (int)((t - prefix[l]) % d.tg.ntx);
+      const int rem = (int)((t - prefix[l]) / d.tg.ntx);
+      const int by = rem % d.tg.nty, bz = rem / d.tg.nty;
+      const int i0 = d.fb.ilo + bx * tx, j0 = d.fb.jlo + by * ty, k0 = d.fb.klo + bz * d.tg.kz;
+      const int k1 = std::min(k0 + d.tg.kz - 1, d.fb.khi);
+      const bool interior = i0 - 3 >= d.fb.ilo && i0 + tx + 2 <= d.fb.ihi && j0 - 3 >= d.fb.jlo &&
+                            j0 + ty + 2 <= d.fb.jhi && k0 - 3 >= d.fb.klo && k1 + 3 <= d.fb.khi;
+      tcost[t] = (double)(k1 - k0 + 1 + 6) * (interior ? 1.0 : 1.25);
+      total += tcost[t];
+    }
+  }
+  typedef std::pair<double, int> Load;  // (cost so far, bin)
+  auto lpt = [](const std::vector<int>& items, const std::vector<double>& cost, int nbins,
+                std::vector<std::vector<int>>& bins) {
+    std::vector<int> order(items);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+    for (int k = 0; k < nbins; k++) heap.push(Load(0.0, k));
+    bins.assign(nbins, std::vector<int>());
+    for (int it : order) {
+      Load top = heap.top();
+      heap.pop();
+      bins[top.second].push_back(it);
+      heap.push(Load(top.first + cost[it], top.second));
+    }
+  };
+  std::vector<std::vector<int>> xcd_tiles(kNumXcd);
+  double run = 0.0;
+  for (int64_t t = 0; t < ntiles; t++) {
+    const int x = std::min(kNumXcd - 1, (int)((run + 0.5 * tcost[t]) * kNumXcd / total));
+    xcd_tiles[x].push_back((int)t);
+    run += tcost[t];
+  }
+  const int nloc = grid / kNumXcd;
+  std::vector<std::vector<int>> per_wg(grid);
+  for (int x = 0; x < kNumXcd; x++) {
+    const std::vector<int>& xt = xcd_tiles[x];
+    std::vector<std::vector<int>> wg;
+    lpt(xt, tcost, nloc, wg);
+    for (int w = 0; w < nloc; w++) per_wg[w * kNumXcd + x] = wg[w];
+  }
+  off.assign(grid + 1, 0);
+  tiles.clear();
+  for (int b = 0; b < grid; b++) {
+    tiles.insert(tiles.end(), per_wg[b].begin(), per_wg[b].end());
+    off[b + 1] = (int)tiles.size();
+  }
 }
 
 bool check_plan(const char* fn, const lsm3d_amd_patch_plan* p) {
@@ -272,19 +348,30 @@ lsm3d_amd_patch_plan* lsm3d_amd_patch_plan_create(int npatch, const int* fbs, co
     want = ((want + kNumXcd - 1) / kNumXcd) * kNumXcd;
     if (want < kNumXcd) want = kNumXcd;
     p->grid = (int)want;
-    check_hip(hipMalloc(&p->desc, sizeof(PatchDesc) * npatch), "hipMalloc(patch descriptors)");
-    check_hip(hipMalloc(&p->prefix, sizeof(int64_t) * (npatch + 1)), "hipMalloc(tile prefix)");
+    if (p->ntiles > INT32_MAX) raise_error(std::string(fn) + ": too many tiles for one plan");
+    std::vector<int> off, tiles;
+    if (c.ok()) build_schedule(p->host_desc, prefix, p->tx, p->ty, p->grid, off, tiles);
+    if (c.ok()) check_hip(hipMalloc(&p->desc, sizeof(PatchDesc) * npatch), "hipMalloc(patch descriptors)");
+    if (c.ok()) check_hip(hipMalloc(&p->prefix, sizeof(int64_t) * (npatch + 1)), "hipMalloc(tile prefix)");
+    if (c.ok()) check_hip(hipMalloc(&p->sched_off, sizeof(int) * off.size()), "hipMalloc(schedule offsets)");
+    if (c.ok()) check_hip(hipMalloc(&p->sched_tile, sizeof(int) * tiles.size()), "hipMalloc(schedule tiles)");
     if (c.ok()) {
       check_hip(hipMemcpy(p->desc, p->host_desc.data(), sizeof(PatchDesc) * npatch, hipMemcpyHostToDevice),
                 "hipMemcpy(patch descriptors)");
       check_hip(hipMemcpy(p->prefix, prefix.data(), sizeof(int64_t) * (npatch + 1), hipMemcpyHostToDevice),
                 "hipMemcpy(tile prefix)");
+      check_hip(hipMemcpy(p->sched_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice),
+                "hipMemcpy(schedule offsets)");
+      check_hip(hipMemcpy(p->sched_tile, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice),
+                "hipMemcpy(schedule tiles)");
     }
   }
   c.finish();
   if (!c.ok()) {
     if (p->desc) (void)hipFree(p->desc);
     if (p->prefix) (void)hipFree(p->prefix);
+    if (p->sched_off) (void)hipFree(p->sched_off);
+    if (p->sched_tile) (void)hipFree(p->sched_tile);
     delete p;
     return nullptr;
   }
@@ -296,6 +383,8 @@ void lsm3d_amd_patch_plan_destroy(lsm3d_amd_patch_plan* p) {
   free_graph(p);
   if (p->desc) (void)hipFree(p->desc);
   if (p->prefix) (void)hipFree(p->prefix);
+  if (p->sched_off) (void)hipFree(p->sched_off);
+  if (p->sched_tile) (void)hipFree(p->sched_tile);
   delete p;
 }
 
