@@ -130,9 +130,9 @@ __global__ void __launch_bounds__(TX* TY)
     int bx, by, bz;
     tile_coords(d.tg, (int)(t - prefix[l]), bx, by, bz);
     if (SYNTH && !tile_is_interior<TX, TY>(d.fb, d.tg, bx, by, bz))
-      march_tile<T, TX, TY, STAGE_ADVECT_CONST, (sizeof(T) == 4), true>(a, bx, by, bz, tile);
+      march_tile<T, TX, TY, STAGE_ADVECT_CONST, (sizeof(T) == 4), true, false>(a, bx, by, bz, tile);
     else
-      march_tile<T, TX, TY, STAGE_ADVECT_CONST, (sizeof(T) == 4), false>(a, bx, by, bz, tile);
+      march_tile<T, TX, TY, STAGE_ADVECT_CONST, (sizeof(T) == 4), false, false>(a, bx, by, bz, tile);
   }
 }
 

@@ -269,7 +269,9 @@ struct Pending {
   }
 };
 
-template <typename T, int TX, int TY, int KIND, bool FASTW, bool SYNTH>
+// IMAGES = false compiles out the periodic-image stores (callers whose
+// a.periodic is always 0, such as the patch level).
+template <typename T, int TX, int TY, int KIND, bool FASTW, bool SYNTH, bool IMAGES = true>
 __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by, int bz, T (*tile)[TX + 7]) {
   // no implicit contraction: the FP32 arithmetic that fuses writes fma
   // explicitly, so unrolled copies of a plane round alike
@@ -311,7 +313,7 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
   const uint32_t oo = (uint32_t)((ic - a.out.ilo) + (int)a.out.sy * (jc - a.out.jlo)) * (uint32_t)sizeof(T);
   const T tiny_dx2x = a.tdx2[0], tiny_dx2y = a.tdx2[1], tiny_dx2z = a.tdx2[2];
   // periodic images: only tiles within 3 cells of a periodic x or y face
-  const bool img_xy = ((a.periodic & 1) && (i0 - fb.ilo < 3 || fb.ihi - (i0 + TX - 1) < 3)) ||
+  const bool img_xy = IMAGES && ((a.periodic & 1) && (i0 - fb.ilo < 3 || fb.ihi - (i0 + TX - 1) < 3)) ||
                       ((a.periodic & 2) && (j0 - fb.jlo < 3 || fb.jhi - (j0 + TY - 1) < 3));
 
   // plane pointers, advanced by one plane per step
@@ -461,7 +463,7 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
           rhs = FASTW ? -fma(cuz, gz, fma(cuy, gy, cux * gx)) : -(cux * gx + cuy * gy + cuz * gz);
         }
         const T v = FASTW ? rk_blend<T>(uc, us, rhs, a.dt, a.rkb) : rk_combine<T>(a.form, uc, us, rhs, a.dt);
-        const bool img_z = (a.periodic & 4) && (k - fb.klo < 3 || fb.khi - k < 3);
+        const bool img_z = IMAGES && (a.periodic & 4) && (k - fb.klo < 3 || fb.khi - k < 3);
         if (img_xy || img_z) store_with_periodic_images<T>(a.out, fb, i, j, k, v, a.periodic, 3);
         else buf_store(v, plane_rsrc(p_out), oo);
       }
