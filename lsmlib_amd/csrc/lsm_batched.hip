@@ -193,7 +193,8 @@ void launch_patch(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R*
     case 2: launch_patch_t<32, 8>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
     case 3: launch_patch_t<128, 2>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
     case 4: launch_patch_t<128, 4>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
-    default: launch_patch_t<32, 16>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    case 5: launch_patch_t<32, 16>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    default: launch_patch_t<16, 16>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
   }
 }
 
@@ -206,7 +207,8 @@ int occupancy_blocks(int cfg) {
     case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 8, true>, 256, 0); break;
     case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 2, true>, 256, 0); break;
     case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 4, true>, 512, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 16, true>, 512, 0); break;
+    case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 16, true>, 512, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 16, 16, true>, 256, 0); break;
   }
   if (e != hipSuccess || n < 1) {
     (void)hipGetLastError();

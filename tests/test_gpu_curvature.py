@@ -213,7 +213,7 @@ def test_tilings_give_identical_results(real, device):
     grid, phi, mask = noisy_case(real, n=(70, 13, 23))
     lib = toolbox(real)
     outs = []
-    for cfg, kz in ((0, 32), (1, 5), (2, 8), (3, 1), (4, 64), (5, 3)):
+    for cfg, kz in ((0, 32), (1, 5), (2, 8), (3, 1), (4, 64), (5, 3), (6, 7)):
         m = CurvatureFlow(grid, real, torch.from_numpy(mask).to(device), device=device, tiling=(cfg, kz))
         pd = torch.from_numpy(phi).to(device)
         m.fill_ghosts(pd, masked=False)

@@ -96,7 +96,8 @@ def test_graph_replays_match_eager_stages(real):
 def test_tiling_does_not_change_results():
     desc = random_patch_level(npatch=5, lo=6, hi=20, seed=5, dx=1.0 / 40)
     ref = None
-    for cfg in range(6):
+    from lsmlib_amd import toolbox
+    for cfg in range(toolbox("double").lib.lsm3d_amd_stage_tiling_count()):
         for kz in (3, 16):
             lvl = PatchLevel(desc, "double", tiling=(cfg, kz))
             phi = lvl.initial_phi()
