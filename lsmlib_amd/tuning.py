@@ -51,8 +51,10 @@ def choose_tiling(real, n, kind="advect"):
 
 
 def autotune(reals=("float", "double"), sizes=(128, 256, 512), kind="advect", reps=5, verbose=True, path=None):
-    """Times one stage of `kind` for every (tile shape, kz) with HIP events
-    (median of `reps`) and stores the best in the table at `path`."""
+    """Times one whole TVD-RK3 step (advect: three stage launches, the
+    stages reading phi^n as well as the stage field) or one reinitialization
+    stage for every (tile shape, kz) with HIP events (median of `reps`) and
+    stores the best in the table at `path`."""
     import torch
     from . import toolbox
     from .pylsmlib import RK1, Stepper
@@ -72,12 +74,13 @@ def autotune(reals=("float", "double"), sizes=(128, 256, 512), kind="advect", re
                                                            align=line_elems(dtype))
             st = Stepper(grid, real, bc, tiling=None)  # the sweep sets the tiling
             out = st.empty()
+            s2 = st.empty()
             best = None
             for cfg in range(ncfg):
                 for kz in KZ_CHOICES:
                     lib.lib.lsm3d_amd_set_stage_tiling(cfg, kz)
                     if kind == "advect":
-                        run = lambda: st.advect_stage(RK1, out, phi, None, vel, 1e-3)  # noqa: E731
+                        run = lambda: st.advect_rk3_step(phi, out, s2, vel, 1e-3)  # noqa: E731
                     else:
                         run = lambda: st.reinit_stage(RK1, out, phi, None, None, False, 1e-3)  # noqa: E731
                     run()
@@ -97,9 +100,10 @@ def autotune(reals=("float", "double"), sizes=(128, 256, 512), kind="advect", re
                         best = (ms, cfg, kz)
             table[f"{kind}/{real}/{_size_class(n)}"] = [best[1], best[2]]
             if verbose:
-                print(f"autotune {kind} {real} n={n}: cfg={best[1]} kz={best[2]} {best[0]:.3f} ms/stage",
+                unit = "ms/step" if kind == "advect" else "ms/stage"
+                print(f"autotune {kind} {real} n={n}: cfg={best[1]} kz={best[2]} {best[0]:.3f} {unit}",
                       flush=True)
-            del phi, vel, out
+            del phi, vel, out, s2
             torch.cuda.empty_cache()
     path.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
     return table
