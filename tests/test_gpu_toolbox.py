@@ -30,7 +30,7 @@ SUM_TOL = {"double": 1e-12, "float": 1e-5}
 def is_sum(name):
     """Reductions that add over cells: the reference accumulates in
     LSMLIB_REAL cell by cell, the GPU in FP64 over blocks."""
-    return "volume" in name or "surface" in name or "aveabsdiff" in name
+    return name.startswith(("lsm3dvolume", "lsm3dsurface", "lsm3daveabsdiff"))
 
 
 def compare(name, real, case, got_args, got_ret, want_args, want_ret):
@@ -81,3 +81,59 @@ def test_device_memory_matches_host_memory(name, real, device):
         assert np.array_equal(h, d.cpu().numpy()), f"{name}[{real}] arg {i} differs"
     if r_host is not None:
         assert np.array_equal(np.atleast_1d(r_host), np.atleast_1d(r_dev))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("real", ["double", "float"])
+def test_reductions_at_size_match_oracle(real, device):
+    """Every reduction on a 150 x 140 x 130 box (tens of thousands of
+    workgroups, FP64 partials over many blocks) against the oracle's
+    sequential sums; counts and maxima exactly, sums to the sum tolerance."""
+    import torch
+    lib, ora = toolbox(real), oracle_lib(real)
+    dt = np.float64 if real == "double" else np.float32
+    fb = (0, 149, -5, 134, 3, 132)
+    gb = (-1, 150, -6, 135, 2, 133)
+    shape = (gb[5] - gb[4] + 1, gb[3] - gb[2] + 1, gb[1] - gb[0] + 1)
+    rng = np.random.default_rng(11)
+    k, j, i = np.meshgrid(*(np.arange(s) for s in shape), indexing="ij")
+    h = 1.0 / 128
+    r = np.sqrt(((i - 75) * h) ** 2 + ((j - 70) * h) ** 2 + ((k - 65) * h) ** 2)
+    phi = (r - 0.4 + 1e-3 * rng.standard_normal(shape)).astype(dt)
+    g = [rng.standard_normal(shape).astype(dt) for _ in range(6)]
+    F = rng.standard_normal(shape).astype(dt)
+    cv = rng.standard_normal(shape).astype(dt)
+    cv[rng.random(shape) < 0.1] = 0
+    eps = 1.5 * h
+    calls = [
+        ("lsm3dsurfaceintegral_", [F, gb, phi, gb, *g[:3], gb, fb, h, h, h, eps]),
+        ("lsm3dsurfaceareazerolevelset_", [phi, gb, *g[:3], gb, fb, h, h, h, eps]),
+        ("lsm3dsurfaceareazerolevelsetcontrolvolume_", [phi, gb, *g[:3], gb, cv, gb, -1, fb, h, h, h, eps]),
+        ("lsm3dsurfaceareazerolevelsetdelta_", [np.abs(F), gb, *g[:3], gb, fb, h, h, h]),
+        ("lsm3dvolumeintegralphigreaterthanzerocontrolvolume_", [F, gb, phi, gb, cv, gb, 1, fb, h, h, h, eps]),
+        ("lsm3dvolumeregionphilessthanzerocontrolvolume_", [phi, gb, cv, gb, -1, fb, h, h, h, eps]),
+        ("lsm3daveabsdiff_", [F, gb, phi, gb, fb]),
+        ("lsm3dvoxelcountgreaterthanzero_", [phi, gb, fb]),
+        ("lsm3dvoxelcountlessthanzerocontrolvolume_", [phi, gb, cv, gb, 1, fb]),
+        ("lsm3dcomputestablenormalveldtcontrolvolume_", [F, gb, *g[:3], gb, *g[3:], gb, cv, gb, 1, fb, h, h, h,
+                                                         0.5]),
+        ("lsm3dmaxnormdiffcontrolvolume_", [F, gb, phi, gb, cv, gb, -1, fb]),
+    ]
+    ora64 = oracle_lib("double")
+    for name, args in calls:
+        if is_sum(name) and real == "float":
+            # a sequential FP32 sum over millions of cells loses digits; the FP64
+            # oracle on the same (FP32-valued) inputs is the reference here
+            want = ora64.call(name, *[a.astype(np.float64) if isinstance(a, np.ndarray) else a for a in args])
+        else:
+            want = ora.call(name, *args)
+        dev = [torch.from_numpy(a).to(device) if isinstance(a, np.ndarray) else a for a in args]
+        got = lib.call(name, *dev)
+        if "voxelcount" in name or "maxnorm" in name:
+            tol = 0.0
+        elif is_sum(name):
+            tol = SUM_TOL[real]
+        else:
+            tol = TOL[real]
+        err = abs(float(got) - float(want)) / max(abs(float(want)), 1e-30)
+        assert err <= tol, f"{name}[{real}]: got {got} want {want} (rel {err:.3e})"
