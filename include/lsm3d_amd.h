@@ -533,6 +533,13 @@ void lsm3daddconstprecomputedcurvtermtolserhs_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX
                                                LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *b,
                                                LSM_AMD_BOX(rhs_fb));
 
+/* --- box copy: src/parallel/fortran/lsm_samrai_f77_utilities.h --- */
+/* LSM3D_SAMRAI_UTILITIES_COPY_DATA, lsm_samrai_f77_utilities.h:52: dst = src on
+ * the fill box (the reference header declares src non-const; it is only read) */
+void lsm3dsamraiutilitiescopydata_(LSMLIB_REAL *dst_data, LSM_AMD_BOX(dst_gb),
+                                   const LSMLIB_REAL *src_data, LSM_AMD_BOX(src_gb),
+                                   LSM_AMD_BOX(fb));
+
 /* --- ghost fill: src/toolbox/boundary_conditions/lsm_boundary_conditions3d.h --- */
 /* bdry_location_idx: 0 x_lo, 1 x_hi, 2 y_lo, 3 y_hi, 4 z_lo, 5 z_hi */
 /* LSM3D_LINEAR_EXTRAPOLATION, lsm_boundary_conditions3d.h:71 */

@@ -151,6 +151,14 @@ __global__ void __launch_bounds__(256)
   LSM_CELL_LOOP_END
 }
 
+// dst = src over the fill box, lsm_samrai_f77_utilities.f:159-169
+template <typename T>
+__global__ void __launch_bounds__(256) k_copy_box(Field<T> dst, Field<const T> src, Box fb) {
+  LSM_CELL_LOOP_BEGIN(fb)
+  dst.at(i, j, k) = src.at(i, j, k);
+  LSM_CELL_LOOP_END
+}
+
 // rhs += b kappa |grad phi|, lsm_level_set_evolution3d.f.in:513-521
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -1725,6 +1733,19 @@ void lsm3daddconstprecomputedcurvtermtolserhs_(R* rhs, BOXP(r), const R* kappa, 
   auto fr = c.out(rhs, gr);
   auto fk = c.in(kappa, BOX(k)), fg = c.in(gm, BOX(g));
   launch_cells(c, "k_precomputed_curv", d, k_precomputed_curv<R>, fr, fk, fg, d, *b);
+  c.finish();
+}
+
+// ---- box copy (src/parallel/fortran/lsm_samrai_f77_utilities.h) -------------------
+void lsm3dsamraiutilitiescopydata_(R* dst, BOXP(d), const R* src, BOXP(s), BOXP(fb)) {
+  const char* fn = "LSM3D_SAMRAI_UTILITIES_COPY_DATA";
+  Call c(fn);
+  const Box fb = BOX(fb);
+  require_inside(fn, "dst_data", BOX(d), fb, 0, 0, 0);
+  require_inside(fn, "src_data", BOX(s), fb, 0, 0, 0);
+  auto fd = c.out(dst, BOX(d));
+  auto fs = c.in(src, BOX(s));
+  launch_cells(c, "k_copy_box", fb, k_copy_box<R>, fd, fs, fb);
   c.finish();
 }
 

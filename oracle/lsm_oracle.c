@@ -487,6 +487,14 @@ void oracle_lsm3dzerooutlevelseteqnrhs_(real *rhs, BOXARGS(r)) {
       for (int i = br.ilo; i <= br.ihi; i++) A(rhs, br, i, j, k) = (real)0.0;
 }
 
+void oracle_lsm3dsamraiutilitiescopydata_(real *dst, BOXARGS(d), const real *src, BOXARGS(s), BOXARGS(fb)) {
+  /* lsm_samrai_f77_utilities.f:124-171 (src/parallel/fortran) */
+  box_t bd = MKBOX(d), bs = MKBOX(s);
+  for (int k = *klo_fb; k <= *khi_fb; k++)
+    for (int j = *jlo_fb; j <= *jhi_fb; j++)
+      for (int i = *ilo_fb; i <= *ihi_fb; i++) A(dst, bd, i, j, k) = A(src, bs, i, j, k);
+}
+
 void oracle_lsm3daddadvectiontermtolserhs_(real *rhs, BOXARGS(r), const real *px,
                                            const real *py, const real *pz, BOXARGS(g),
                                            const real *vx, const real *vy, const real *vz,
