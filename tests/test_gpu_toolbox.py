@@ -20,6 +20,7 @@ import pytest
 
 from lsm_cases import cases_for
 from lsmlib_amd import SIGNATURES, toolbox
+from lsmlib_amd.f77 import box_shape
 from lsmlib_amd.signatures import BOX_ROUTINES
 from oracle.oracle import oracle_lib
 
@@ -137,3 +138,46 @@ def test_reductions_at_size_match_oracle(real, device):
             tol = TOL[real]
         err = abs(float(got) - float(want)) / max(abs(float(want)), 1e-30)
         assert err <= tol, f"{name}[{real}]: got {got} want {want} (rel {err:.3e})"
+
+
+@pytest.mark.gpu
+def test_reads_beyond_the_arrays_are_errors(device):
+    """Boxes that would make the reference read outside an array are reported
+    before any launch: the delta function's phi needs one ghost cell, the
+    precomputed curvature term reads kappa over its ghost-box loop range, order-4
+    curvature reads the gradients two cells out."""
+    import torch
+    from lsmlib_amd._native import LsmError
+    lib = toolbox("double")
+    fb = (0, 7, 0, 6, 0, 5)
+    nb = (-1, 8, -1, 7, -1, 6)
+    shp = lambda b: box_shape(b)  # noqa: E731
+    z = lambda b: torch.zeros(shp(b), dtype=torch.float64, device=device)  # noqa: E731
+    with pytest.raises(LsmError, match="phi"):
+        lib.lsm3ddeltafunctionorder1_(z(fb), z(fb), fb, z(fb), z(fb), z(fb), fb, fb, 0.1, 0.1, 0.1)
+    lib.lsm3ddeltafunctionorder1_(z(nb), z(nb), nb, z(fb), z(fb), z(fb), fb, fb, 0.1, 0.1, 0.1)
+    with pytest.raises(LsmError, match="kappa"):
+        lib.lsm3daddconstprecomputedcurvtermtolserhs_(z(nb), nb, z(fb), fb, z(nb), nb, 0.5, fb)
+    with pytest.raises(LsmError, match="grad_phi"):
+        lib.lsm3dcomputemeancurvatureorder4_(z(fb), fb, z((-2, 9, -2, 8, -2, 7)), (-2, 9, -2, 8, -2, 7),
+                                             z(nb), z(nb), z(nb), z(nb), nb, fb, 0.1, 0.1, 0.1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("real", ["double", "float"])
+def test_empty_box_reductions_match_oracle(real, device):
+    """An empty include box: counts 0, sums 0, the average absolute difference 0
+    and the CFL dt from max = -1, as the reference's initial values give."""
+    lib, ora = toolbox(real), oracle_lib(real)
+    dt = np.float64 if real == "double" else np.float32
+    gb = (0, 5, 0, 5, 0, 5)
+    ib = (3, 2, 0, 5, 0, 5)   # ilo > ihi
+    a = np.ones(box_shape(gb), dtype=dt)
+    for name, args in [("lsm3dvoxelcountgreaterthanzero_", [a, gb, ib]),
+                       ("lsm3daveabsdiff_", [a, gb, a, gb, ib]),
+                       ("lsm3dsurfaceareazerolevelset_", [a, gb, a, a, a, gb, ib, 0.1, 0.1, 0.1, 0.2]),
+                       ("lsm3dcomputestablenormalveldtcontrolvolume_",
+                        [a, gb, a, a, a, gb, a, a, a, gb, a, gb, 1, ib, 0.1, 0.1, 0.1, 0.5])]:
+        want = ora.call(name, *[x.copy() if isinstance(x, np.ndarray) else x for x in args])
+        got = lib.call(name, *[x.copy() if isinstance(x, np.ndarray) else x for x in args])
+        assert float(got) == pytest.approx(float(want), rel=1e-6), name
