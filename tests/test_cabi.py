@@ -34,3 +34,23 @@ def test_library_exports_declared_symbols(real):
     lib = ctypes.CDLL(str(p))
     missing = [n for n in sorted(declared_symbols()) if not hasattr(lib, n)]
     assert not missing, f"{p.name} lacks {missing}"
+
+
+def test_every_3d_routine_of_the_reference_headers_is_exported():
+    """Every LSM3D_* macro the reference headers define maps to a symbol of the
+    signature table (and so of the library).  Reads the reference tree where
+    it is mounted; skipped elsewhere."""
+    from lsmlib_amd.signatures import MACRO_OF
+    src = Path("/root/reference/src")
+    if not src.is_dir():
+        pytest.skip("reference tree not mounted")
+    macros = {}
+    for h in src.rglob("*.h"):
+        for m in re.finditer(r"#define\s+(LSM3D_\w+)\s*\\?\s*\n?\s*(lsm3d\w+_)", h.read_text(errors="replace")):
+            macros[m.group(1)] = m.group(2).lower()
+    assert len(macros) > 100
+    have = {v: k for k, v in MACRO_OF.items()}
+    missing = sorted(m for m in macros if m not in have)
+    assert not missing, f"reference 3D routines not exported: {missing}"
+    wrong = sorted(m for m, sym in macros.items() if have.get(m) != sym)
+    assert not wrong, f"macro -> symbol differs from the reference: {wrong}"
