@@ -749,8 +749,9 @@ def read_valu():
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
-              "(the multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the library runs on a "
+              "single GPU (sharding into z-slabs with halo exchange is designed, not implemented)",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
