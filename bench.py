@@ -718,7 +718,7 @@ def read_traffic():
     return None
 
 
-# FP32 VALU issue rate of MI355X without packed FP32 (disabled in this build):
+# FP32 VALU issue rate of MI355X without packed FP32 (the library is compiled without it):
 # 256 CUs x 4 SIMDs x 16 lanes per clock x 2.4 GHz
 VALU_PEAK_LANE_INSTR = 256 * 64 * 2.4e9
 

@@ -1003,7 +1003,7 @@ void lsm_amd_set_error_mode(int mode);
 int lsm_amd_error_pending(void);
 const char *lsm_amd_last_error(void);
 void lsm_amd_clear_error(void);
-/* sizeof(LSMLIB_REAL) of this build */
+/* sizeof(LSMLIB_REAL) of this library (8: double, 4: float) */
 int lsm_amd_real_size(void);
 
 /* ======================================================================

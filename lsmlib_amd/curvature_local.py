@@ -20,7 +20,7 @@ curvature_model3d_local.c:52-728, 757-934) with every array in device memory:
 The stage fields phi_stage1 / phi_next keep, outside the band, what the last
 box reinitialization and earlier steps left there, as the reference's do:
 the narrow-band stages read them there (the ENO2 stencils of the outer
-levels), so the driver's initial reinitialization writes them exactly as
+levels), so this class's initial reinitialization writes them exactly as
 reinitializeMedium3d does.
 
 By default a step is one call of the library's native sequence
