@@ -665,15 +665,26 @@ def fp32_error(grid32, phi32, grid64, phi64):
 
 
 def interior_error(a32, b64, grid64):
+    """Max |a32 - b64| globally and near the interface (|b64| < 3 dx); near the
+    interface also the fraction of cells above 1e-5 dx and the 99.99th
+    percentile (the maximum sits at the few cells where the union of spheres
+    has a kink)."""
     import torch
     a = a32.to(torch.float64)
     b = b64
     err = (a - b).abs()
     h = min(grid64.dx)
     near = b.abs() < 3 * h
-    return {"max_abs_global": float(err.max()),
-            "max_abs_near_interface": float(err[near].max()) if bool(near.any()) else None,
-            "max_near_interface_over_dx": (float(err[near].max()) / h) if bool(near.any()) else None}
+    out = {"max_abs_global": float(err.max()),
+           "max_abs_near_interface": float(err[near].max()) if bool(near.any()) else None,
+           "max_near_interface_over_dx": (float(err[near].max()) / h) if bool(near.any()) else None}
+    if bool(near.any()):
+        e = err[near] / h
+        out["near_cells"] = int(e.numel())
+        out["near_cells_above_1e-5_dx"] = int((e > 1e-5).sum())
+        sample = e if e.numel() <= 4_000_000 else e[torch.randperm(e.numel(), device=e.device)[:4_000_000]]
+        out["near_p9999_over_dx"] = float(torch.quantile(sample, 0.9999))
+    return out
 
 
 def cpu_baseline(seconds):
@@ -819,7 +830,9 @@ def main():
         _, _, phi32b = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts, zero_tol=None)
         err_build_tol = fp32_error(grid32, phi32b, grid64, phi64)
         err_build_tol["after_steps"] = r32["steps_total"]
-        print(f"# fp32 error vs fp64 near interface: {err['max_near_interface_over_dx']:.3e} dx "
+        print(f"# fp32 error vs fp64 near interface: 99.99% of cells within {err['near_p9999_over_dx']:.2e} dx, "
+              f"{err['near_cells_above_1e-5_dx']} of {err['near_cells']} above 1e-5 dx; max "
+              f"{err['max_near_interface_over_dx']:.3e} dx "
               f"(switch 1e-11), {err_build_tol['max_near_interface_over_dx']:.3e} dx (switch 1e-5)",
               file=sys.stderr, flush=True)
         del phi64, phi32b
