@@ -57,16 +57,18 @@ def parse():
                     help="skip the batched-patch and reinitialization measurements")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (adjoint) measurement")
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the gradient")
-    ap.add_argument("--only", choices=["batched", "reinit", "adjoint", "config1", "config5", "throughput",
+    ap.add_argument("--only", choices=["probe", "batched", "reinit", "adjoint", "config1", "config5", "throughput",
                                        "curvature", "curvature_step", "curvature_local", "curvature_local_step",
                                        "field_extension"],
                     default=None,
                     help="run one secondary measurement alone and print its JSON (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--velocity", choices=["taylor_green", "abc"], default="taylor_green",
+                    help="velocity of the timed advection (abc: every component non-zero)")
     return ap.parse_args()
 
 
-def run_gpu(real, n, steps, warmup, tiling, ghosts="synth", zero_tol=None):
+def run_gpu(real, n, steps, warmup, tiling, ghosts="synth", zero_tol=None, velocity="taylor_green"):
     import torch
     from lsmlib_amd import toolbox
     from lsmlib_amd.pylsmlib import RK1, TVDRK3_STAGE2, TVDRK3_STAGE3, Stepper, stable_advection_dt
@@ -75,7 +77,8 @@ def run_gpu(real, n, steps, warmup, tiling, ghosts="synth", zero_tol=None):
 
     dev = torch.device("cuda:0")
     dtype = torch.float32 if real == "float" else torch.float64
-    grid, phi64, vel, bc = sphere_union_taylor_green(n, dtype=torch.float64, device=dev, align=line_elems(dtype))
+    grid, phi64, vel, bc = sphere_union_taylor_green(n, dtype=torch.float64, device=dev, align=line_elems(dtype),
+                                                     velocity=velocity)
     phi = phi64.to(dtype).clone()
     del phi64
     dt = stable_advection_dt(vel, grid, 0.5, "double")  # one dt for both precisions
@@ -131,6 +134,36 @@ def run_gpu(real, n, steps, warmup, tiling, ghosts="synth", zero_tol=None):
     }
     res["_one_step"] = one_step
     return res, grid, phi
+
+
+def run_bandwidth_probe(mib=2048, reps=10):
+    """Streaming HBM rate of memory-only kernels (lsm_amd_bandwidth_probe):
+    copy, 5 reads + 1 write (the fused advection stage's stream count) and
+    reads only, over `mib` MiB per stream, at several grid sizes; the best
+    rate of each pattern is reported (GB/s = 1e9 B/s)."""
+    import ctypes
+    import torch
+    from lsmlib_amd import toolbox
+    lib = toolbox("float")
+    nbytes = mib << 20
+    src = torch.ones(5 * nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    lib.set_stream(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for kind, name, moved in ((0, "copy", 2), (1, "read5_write1", 6), (2, "read", 1)):
+        best = None
+        for wg in (1024, 2048, 4096, 8192, 16384):
+            if kind == 2 and wg * 256 * 4 > nbytes:
+                continue
+            ms = lib.fused("lsm_amd_bandwidth_probe", kind, ctypes.c_void_p(src.data_ptr()),
+                           ctypes.c_void_p(dst.data_ptr()), nbytes, wg, reps)
+            gbs = moved * nbytes / (ms * 1e-3) / 1e9
+            if best is None or gbs > best["gbs"]:
+                best = {"gbs": gbs, "ms": ms, "workgroups": wg, "bytes_moved": moved * nbytes}
+        out[name] = best
+    del src, dst
+    torch.cuda.empty_cache()
+    return out
 
 
 def run_batched(real="float", npatch=512, steps=20, reps=2):
@@ -779,6 +812,9 @@ def main():
                           "launches": 3}), flush=True)
         return
 
+    if a.only == "probe":
+        print(json.dumps({"bandwidth_probe": run_bandwidth_probe()}), flush=True)
+        return
     if a.only == "adjoint":
         print(json.dumps({"adjoint": run_adjoint(fd=not a.no_fd)}), flush=True)
         return
@@ -813,13 +849,14 @@ def main():
     if a.only == "curvature_step":  # the 512 x 512 x 341 model step alone (profiling runs)
         print(json.dumps({"curvature_example": run_curvature(runs=False, big_steps=a.steps)}), flush=True)
         return
-    r32, grid32, phi32 = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts, zero_tol=SWITCH_TOL)
+    r32, grid32, phi32 = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts, zero_tol=SWITCH_TOL,
+                                 velocity=a.velocity)
     print(f"# fp32: {r32['value']:.4e} cell-updates/s, {r32['ms_per_step']:.3f} ms/step, "
           f"kernel {r32['kernel_ms_avg']:.3f} ms, {r32['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     r64 = None
     err = err_build_tol = err_one = None
     if not a.fp32_only:
-        r64, grid64, phi64 = run_gpu("double", a.n, a.steps, a.warmup, a.tiling, a.ghosts)
+        r64, grid64, phi64 = run_gpu("double", a.n, a.steps, a.warmup, a.tiling, a.ghosts, velocity=a.velocity)
         print(f"# fp64: {r64['value']:.4e} cell-updates/s, {r64['ms_per_step']:.3f} ms/step",
               file=sys.stderr, flush=True)
         err = fp32_error(grid32, phi32, grid64, phi64)
@@ -827,7 +864,8 @@ def main():
         err_one = interior_error(r32.pop("_one_step"), r64.pop("_one_step"), grid64)
         del phi32
         # the same FP32 run with the float build's LSMLIB_ZERO_TOL (1e-5) in the upwind switch
-        _, _, phi32b = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts, zero_tol=None)
+        _, _, phi32b = run_gpu("float", a.n, a.steps, a.warmup, a.tiling, a.ghosts, zero_tol=None,
+                               velocity=a.velocity)
         err_build_tol = fp32_error(grid32, phi32b, grid64, phi64)
         err_build_tol["after_steps"] = r32["steps_total"]
         print(f"# fp32 error vs fp64 near interface: 99.99% of cells within {err['near_p9999_over_dx']:.2e} dx, "

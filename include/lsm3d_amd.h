@@ -1005,6 +1005,18 @@ const char *lsm_amd_last_error(void);
 void lsm_amd_clear_error(void);
 /* sizeof(LSMLIB_REAL) of this library (8: double, 4: float) */
 int lsm_amd_real_size(void);
+/* Streaming-bandwidth probe (a measurement utility; no counterpart in the
+ * reference): `reps` launches of a memory-only kernel over device arrays on
+ * the library stream, timed with HIP events; returns milliseconds per launch
+ * (-1 on error).  16-byte non-temporal accesses, `workgroups` x 256 threads.
+ *   kind 0: copy, dst[0:bytes) = src[0:bytes)
+ *   kind 1: 5 reads + 1 write, dst = sum of the five consecutive `bytes`
+ *           slices of src (src holds 5 * bytes)
+ *   kind 2: reads of src[0:bytes); one float per thread is written to dst
+ *           (dst holds workgroups * 256 floats)
+ * bytes must be a positive multiple of 16. */
+double lsm_amd_bandwidth_probe(int kind, const void *src, void *dst, long long bytes, int workgroups,
+                               int reps);
 
 /* ======================================================================
  * Part 3: fused device-resident stages (this library's own API; no
@@ -1062,6 +1074,43 @@ void lsm3d_amd_reinit_stage(int rk_form, LSMLIB_REAL *phi_out, const int out_gb[
                             const LSMLIB_REAL *phi0, const int phi0_gb[6], int use_phi0_for_sgn,
                             const int fb[6], LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz,
                             LSMLIB_REAL dt, const int bc[6], int periodic_mask);
+/* --- Lax-Friedrichs Hamiltonians (no counterpart in the reference, whose
+ * level-set RHS routines are upwind or Godunov only) ----------------------
+ * From the plus/minus one-sided derivatives (e.g. of LSM3D_HJ_WENO5) and a
+ * dissipation coefficient alpha per axis:
+ *   advection:    lse_rhs -= sum_a (0.5 (v_a - alpha_a) phi_a+ + 0.5 (v_a + alpha_a) phi_a-)
+ *   normal speed: lse_rhs -= vel_n |(grad+ + grad-)/2| - sum_a alpha (phi_a+ - phi_a-)/2
+ * alpha_mode 0: local Lax-Friedrichs, alpha_a = |v_a| (|vel_n|) at each cell;
+ * with |v_a| >= LSMLIB_ZERO_TOL this is exactly the upwind term of
+ * LSM3D_UPWIND_HJ_* + LSM3D_ADD_ADVECTION_TERM_TO_LSE_RHS.  alpha_mode 1:
+ * global Lax-Friedrichs with the caller's alpha[3] (alpha), which must bound
+ * |v_a| (|vel_n|) over the box for stability.  vel_n NULL: the constant
+ * vel_n_const.  Work runs over fb; the arrays' ghost boxes must contain it. */
+void lsm3d_amd_add_lf_advection_term(LSMLIB_REAL *lse_rhs, const int rhs_gb[6], const LSMLIB_REAL *phi_x_plus,
+                                     const LSMLIB_REAL *phi_y_plus, const LSMLIB_REAL *phi_z_plus,
+                                     const int plus_gb[6], const LSMLIB_REAL *phi_x_minus,
+                                     const LSMLIB_REAL *phi_y_minus, const LSMLIB_REAL *phi_z_minus,
+                                     const int minus_gb[6], const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y,
+                                     const LSMLIB_REAL *vel_z, const int vel_gb[6], const int fb[6],
+                                     int alpha_mode, const LSMLIB_REAL *alpha);
+void lsm3d_amd_add_lf_normal_vel_term(LSMLIB_REAL *lse_rhs, const int rhs_gb[6], const LSMLIB_REAL *phi_x_plus,
+                                      const LSMLIB_REAL *phi_y_plus, const LSMLIB_REAL *phi_z_plus,
+                                      const int plus_gb[6], const LSMLIB_REAL *phi_x_minus,
+                                      const LSMLIB_REAL *phi_y_minus, const LSMLIB_REAL *phi_z_minus,
+                                      const int minus_gb[6], const LSMLIB_REAL *vel_n, const int vel_n_gb[6],
+                                      LSMLIB_REAL vel_n_const, const int fb[6], int alpha_mode,
+                                      LSMLIB_REAL alpha);
+/* One stage of phi_t + V.grad(phi) = 0 with the Lax-Friedrichs advection
+ * Hamiltonian above on the plus/minus HJ-WENO5 derivatives of phi_stage,
+ * fused with the Runge-Kutta form (arguments as lsm3d_amd_advect_stage).
+ * Equivalent to LSM3D_HJ_WENO5 + LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS +
+ * lsm3d_amd_add_lf_advection_term + LSM3D_TVD_RK*_STAGE* in one pass. */
+void lsm3d_amd_advect_lf_stage(int rk_form, LSMLIB_REAL *phi_out, const int out_gb[6],
+                               const LSMLIB_REAL *phi_stage, const int stage_gb[6], const LSMLIB_REAL *phi_cur,
+                               const int cur_gb[6], const LSMLIB_REAL *vel_x, const LSMLIB_REAL *vel_y,
+                               const LSMLIB_REAL *vel_z, const int vel_gb[6], const int fb[6], LSMLIB_REAL dx,
+                               LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt, const int bc[6], int periodic_mask,
+                               int alpha_mode, const LSMLIB_REAL *alpha);
 /* Fills the ghost cells of phi outside fb, face by face in the order x_lo,
  * x_hi, y_lo, y_hi, z_lo, z_hi (each face over the full ghost box in the
  * other directions, as the LSM3D_*_EXTRAPOLATION routines do).  bc[face]:

@@ -75,6 +75,10 @@ class ToolboxLibrary(F77Library):
         L.lsm3d_amd_advect_const_stage.argtypes = [I, P, B, P, B, P, B, R, R, R, B, R, R, R, R, B, I]
         L.lsm3d_amd_reinit_stage.argtypes = [I, P, B, P, B, P, B, P, B, I, B, R, R, R, R, B, I]
         L.lsm3d_amd_fill_ghosts.argtypes = [P, B, B, B]
+        RP = ctypes.POINTER(R)
+        L.lsm3d_amd_advect_lf_stage.argtypes = [I, P, B, P, B, P, B, P, P, P, B, B, R, R, R, R, B, I, I, RP]
+        L.lsm3d_amd_add_lf_advection_term.argtypes = [P, B, P, P, P, B, P, P, P, B, P, P, P, B, B, I, RP]
+        L.lsm3d_amd_add_lf_normal_vel_term.argtypes = [P, B, P, P, P, B, P, P, P, B, P, B, R, B, I, R]
         L.lsm3d_amd_set_stage_tiling.argtypes = [I, I]
         L.lsm3d_amd_stage_tiling_count.restype = I
         L.lsm3d_amd_stage_tiling_shape.argtypes = [I, B, B]
@@ -91,6 +95,8 @@ class ToolboxLibrary(F77Library):
         L.lsm3d_amd_patch_plan_capture_rk3.argtypes = [P, I, P, P, P, R, R, R, R, B]
         L.lsm3d_amd_patch_plan_replay.argtypes = [P]
         L.lsm3d_amd_patch_plan_fill_ghosts.argtypes = [P, P, B]
+        L.lsm_amd_bandwidth_probe.argtypes = [I, P, P, ctypes.c_longlong, I, I]
+        L.lsm_amd_bandwidth_probe.restype = ctypes.c_double
         L.lsm3d_amd_set_zero_tol.argtypes = [ctypes.c_double]
         L.lsm3d_amd_set_zero_tol.restype = None
         L.lsm3d_amd_zero_tol.argtypes = []
@@ -115,6 +121,7 @@ class ToolboxLibrary(F77Library):
         L.lsm3d_amd_local_curvature_reinit.argtypes = [M, P, D]
         L.lsm3d_amd_local_curvature_reinit.restype = I
         for f in ("lsm3d_amd_advect_stage", "lsm3d_amd_advect_const_stage", "lsm3d_amd_reinit_stage",
+                  "lsm3d_amd_advect_lf_stage", "lsm3d_amd_add_lf_advection_term", "lsm3d_amd_add_lf_normal_vel_term",
                   "lsm3d_amd_fill_ghosts", "lsm3d_amd_set_stage_tiling", "lsm3d_amd_patch_plan_destroy",
                   "lsm3d_amd_patch_plan_stage", "lsm3d_amd_patch_plan_capture_rk3",
                   "lsm3d_amd_patch_plan_replay", "lsm3d_amd_patch_plan_fill_ghosts", "lsm3d_amd_advect_stage_adjoint",

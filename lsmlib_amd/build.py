@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["lsm_runtime.hip", "lsm_toolbox.hip", "lsm_fused.hip", "lsm_batched.hip", "lsm_adjoint.hip",
-           "lsm_curvature.hip", "lsm_local.hip", "lsm_local_model.hip"]
+           "lsm_curvature.hip", "lsm_local.hip", "lsm_local_model.hip", "lsm_probe.hip"]
 
 COMMON_FLAGS = [
     "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-fvisibility=hidden",

@@ -105,6 +105,7 @@ struct AdjArgs {
   T gscale;       // a_s * dt
   T inv[3];       // 1/dx, 1/dy, 1/dz
   T tiny_dx2[3];  // 12 * tiny * dx^2 per axis
+  T zero_tol;     // upwind switch threshold, the forward stages' setting
 };
 
 template <typename T>
@@ -123,7 +124,7 @@ __global__ void __launch_bounds__(256) k_adj_faces(AdjArgs<T> a) {
     T slot[6];
 #pragma unroll
     for (int q = 0; q < 6; q++) slot[q] = T(0);
-    if (fabs(vel) >= Tol<T>::zero) {
+    if (fabs(vel) >= a.zero_tol) {
       T p[7];
       const int l0 = ax == 0 ? li : ax == 1 ? lj : lk;
       const int n = a.n[ax];
@@ -366,6 +367,7 @@ void lsm3d_amd_advect_stage_adjoint(int rk_form, const LSMLIB_REAL* adj_out, con
   a.slots = work;
   a.ncell = ncell;
   a.gscale = as * dt;
+  a.zero_tol = (R)current_zero_tol();  // the switch the forward stage used (lsm3d_amd_set_zero_tol)
   const R h[3] = {dx, dy, dz};
   for (int d = 0; d < 3; d++) {
     a.inv[d] = R(1) / h[d];

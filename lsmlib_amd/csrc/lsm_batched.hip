@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(TX* TY)
                   const int* __restrict__ sched_off, const int* __restrict__ sched_tile, T* out_pool,
                   const T* ps_pool, const T* pc_pool, GhostRule ghost, T ix, T iy, T iz, T dt, T zt, T tdx2x,
                   T tdx2y, T tdx2z, int form) {
-  __shared__ T tile[TY + 6][TX + 7];
+  __shared__ StageTile<T, TX, TY> tile;
   const int b = (int)blockIdx.x;
   const int q0 = sched_off[b], q1 = sched_off[b + 1];
   for (int q = q0; q < q1; q++) {

@@ -31,7 +31,7 @@ namespace lsm {
 // are read.  FP32 uses the rearranged WENO5 form, FP64 the reference order.
 template <typename T, int TX, int TY, int KIND, bool SYNTH>
 __global__ void __launch_bounds__(TX* TY) k_stage(StageArgs<T> a) {
-  __shared__ T tile[TY + 6][TX + 7];
+  __shared__ StageTile<T, TX, TY> tile;
   const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   int bx, by, bz;
   tile_coords(a.tg, b, bx, by, bz);
@@ -178,14 +178,23 @@ void launch_stage_kind(hipStream_t s, const StageArgs<R>& a, int cfg, int nblk) 
 void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, const R* ps,
                const int* ps_gb, const R* pc, const int* pc_gb, const R* vx, const R* vy, const R* vz,
                const int* vel_gb, const R* phi0, const int* phi0_gb, R cvx, R cvy, R cvz,
-               const int* fbp, R dx, R dy, R dz, R dt, int use_phi0, const int* bc, int periodic) {
+               const int* fbp, R dx, R dy, R dz, R dt, int use_phi0, const int* bc, int periodic,
+               int alpha_mode = 0, const R* alpha = nullptr) {
   Call c(fn);
   const Box fb = box6(fbp);
   if (!valid_form(form)) raise_error(std::string(fn) + ": unknown Runge-Kutta form");
+  // tiles read phi_stage's halo while neighbouring tiles write phi_out
+  if (out && (const void*)out == (const void*)ps) raise_error(std::string(fn) + ": phi_out must not alias phi_stage");
   require_inside(fn, "phi_out", box6(out_gb), fb, 0, 0, 0);
   require_inside(fn, "phi_stage", box6(ps_gb), fb, 3, 3, 3);
   if (pc) require_inside(fn, "phi_cur", box6(pc_gb), fb, 0, 0, 0);
-  if (kind == STAGE_ADVECT_FIELD) require_inside(fn, "vel", box6(vel_gb), fb, 0, 0, 0);
+  if (kind == STAGE_ADVECT_FIELD || kind == STAGE_ADVECT_LF) require_inside(fn, "vel", box6(vel_gb), fb, 0, 0, 0);
+  if (kind == STAGE_ADVECT_LF) {
+    if (alpha_mode != 0 && alpha_mode != 1) raise_error(std::string(fn) + ": alpha_mode is 0 (local) or 1 (global)");
+    if (alpha_mode == 1 && !alpha) raise_error(std::string(fn) + ": global alpha needs alpha[3]");
+    if (alpha_mode == 1 && alpha && !(alpha[0] >= 0 && alpha[1] >= 0 && alpha[2] >= 0))
+      raise_error(std::string(fn) + ": alpha must be non-negative");
+  }
   if (kind == STAGE_REINIT && use_phi0) require_inside(fn, "phi0", box6(phi0_gb), fb, 0, 0, 0);
   const GhostRule ghost = make_ghost_rule(fn, bc, fb);
   periodic &= 7;
@@ -209,9 +218,10 @@ void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, co
   a.out = make_field<R>(out, box6(out_gb));
   a.ps = cfield(ps, ps_gb);
   a.pc = cfield(pc, pc_gb);
-  a.vx = cfield(kind == STAGE_ADVECT_FIELD ? vx : nullptr, vel_gb);
-  a.vy = cfield(kind == STAGE_ADVECT_FIELD ? vy : nullptr, vel_gb);
-  a.vz = cfield(kind == STAGE_ADVECT_FIELD ? vz : nullptr, vel_gb);
+  const bool vel_field = kind == STAGE_ADVECT_FIELD || kind == STAGE_ADVECT_LF;
+  a.vx = cfield(vel_field ? vx : nullptr, vel_gb);
+  a.vy = cfield(vel_field ? vy : nullptr, vel_gb);
+  a.vz = cfield(vel_field ? vz : nullptr, vel_gb);
   a.phi0 = cfield((kind == STAGE_REINIT && use_phi0) ? phi0 : nullptr, phi0_gb);
   a.cvx = cvx;
   a.cvy = cvy;
@@ -235,6 +245,8 @@ void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, co
   a.rkb = rk_blend_weight<R>(form);
   a.periodic = periodic;
   a.use_phi0 = use_phi0 ? 1 : 0;
+  a.alpha_mode = (kind == STAGE_ADVECT_LF) ? alpha_mode : 0;
+  for (int d = 0; d < 3; d++) a.alpha[d] = (a.alpha_mode && alpha) ? alpha[d] : R(0);
   const long long nblk = (long long)a.tg.ntx * a.tg.nty * a.tg.nkz;
   if (nblk > 0x7fffffffLL) {
     raise_error(std::string(fn) + ": grid too large for one launch");
@@ -243,6 +255,7 @@ void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, co
   }
   if (kind == STAGE_ADVECT_FIELD) launch_stage_kind<STAGE_ADVECT_FIELD>(c.stream(), a, tl.cfg, (int)nblk);
   else if (kind == STAGE_ADVECT_CONST) launch_stage_kind<STAGE_ADVECT_CONST>(c.stream(), a, tl.cfg, (int)nblk);
+  else if (kind == STAGE_ADVECT_LF) launch_stage_kind<STAGE_ADVECT_LF>(c.stream(), a, tl.cfg, (int)nblk);
   else launch_stage_kind<STAGE_REINIT>(c.stream(), a, tl.cfg, (int)nblk);
   c.launched("k_stage");
   c.finish();
@@ -298,6 +311,17 @@ void lsm3d_amd_advect_const_stage(int rk_form, LSMLIB_REAL* phi_out, const int o
   run_stage("lsm3d_amd_advect_const_stage", STAGE_ADVECT_CONST, rk_form, phi_out, out_gb, phi_stage,
             stage_gb, phi_cur, cur_gb, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, vel_x, vel_y,
             vel_z, fb, dx, dy, dz, dt, 0, bc, periodic_mask);
+}
+
+void lsm3d_amd_advect_lf_stage(int rk_form, LSMLIB_REAL* phi_out, const int out_gb[6],
+                               const LSMLIB_REAL* phi_stage, const int stage_gb[6], const LSMLIB_REAL* phi_cur,
+                               const int cur_gb[6], const LSMLIB_REAL* vel_x, const LSMLIB_REAL* vel_y,
+                               const LSMLIB_REAL* vel_z, const int vel_gb[6], const int fb[6], LSMLIB_REAL dx,
+                               LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt, const int bc[6], int periodic_mask,
+                               int alpha_mode, const LSMLIB_REAL* alpha) {
+  run_stage("lsm3d_amd_advect_lf_stage", STAGE_ADVECT_LF, rk_form, phi_out, out_gb, phi_stage, stage_gb, phi_cur,
+            cur_gb, vel_x, vel_y, vel_z, vel_gb, nullptr, nullptr, 0, 0, 0, fb, dx, dy, dz, dt, 0, bc, periodic_mask,
+            alpha_mode, alpha);
 }
 
 void lsm3d_amd_reinit_stage(int rk_form, LSMLIB_REAL* phi_out, const int out_gb[6],

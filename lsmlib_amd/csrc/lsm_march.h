@@ -91,7 +91,13 @@ __device__ __forceinline__ void store_with_periodic_images(const Field<T>& out, 
 // ---------------------------------------------------------------------------
 // One fused Runge-Kutta stage over one tile.
 // ---------------------------------------------------------------------------
-enum StageKind { STAGE_ADVECT_FIELD = 0, STAGE_ADVECT_CONST = 1, STAGE_REINIT = 2 };
+enum StageKind { STAGE_ADVECT_FIELD = 0, STAGE_ADVECT_CONST = 1, STAGE_REINIT = 2, STAGE_ADVECT_LF = 3 };
+
+// Stage kinds that read a stored velocity field.
+template <int KIND>
+constexpr bool reads_velocity() {
+  return KIND == STAGE_ADVECT_FIELD || KIND == STAGE_ADVECT_LF;
+}
 
 template <typename T>
 struct StageArgs {
@@ -111,6 +117,8 @@ struct StageArgs {
   T rkb;         // rk_blend_weight(form), used by the FP32 (FASTW) stages
   int periodic;  // bit mask: also write phi_out's periodic ghost images
   int use_phi0;  // reinit sign from phi0 (1) or phi_s (0)
+  int alpha_mode;  // Lax-Friedrichs stages: 0 alpha = |v_a| per cell, 1 the global alpha[a]
+  T alpha[3];
 };
 
 // ---------------------------------------------------------------------------
@@ -128,6 +136,32 @@ __device__ __forceinline__ T upwind_weno5_d(const T* d, T vel, T inv, T tiny_dx2
   if (FASTW) r = weno5_fast<T>(D1, D2, D3, D4, D5, tiny_dx2) * inv;
   else r = weno5_ref<T>(D1 * inv, D2 * inv, D3 * inv, D4 * inv, D5 * inv);
   return (fabs(vel) < zero_tol) ? T(0) : r;
+}
+
+// The same selection decided once per wavefront where it can be: when every
+// active lane takes the backward stencil, or every lane the forward one, the
+// WENO5 runs on that side without per-lane selects; when |vel| < zero_tol on
+// every lane the derivative is 0 and no WENO5 is evaluated (the reference
+// evaluates none there either, lsm_spatial_derivatives3d.f.in:2756-2758).
+// Mixed wavefronts take upwind_weno5_d.  Every path computes the same value
+// for a lane as upwind_weno5_d.
+template <typename T, bool FASTW>
+__device__ __forceinline__ T upwind_weno5_wave(const T* d, T vel, T inv, T tiny_dx2, T zero_tol) {
+  const uint64_t live = __builtin_amdgcn_read_exec();
+  const uint64_t nz = __builtin_amdgcn_ballot_w64(!(fabs(vel) < zero_tol));
+  const uint64_t gt = __builtin_amdgcn_ballot_w64(vel > T(0));
+  if (nz == 0) return T(0);
+  if (nz == live) {
+    if ((gt & live) == live) {
+      if (FASTW) return weno5_fast<T>(d[0], d[1], d[2], d[3], d[4], tiny_dx2) * inv;
+      return weno5_ref<T>(d[0] * inv, d[1] * inv, d[2] * inv, d[3] * inv, d[4] * inv);
+    }
+    if ((gt & live) == 0) {
+      if (FASTW) return weno5_fast<T>(d[5], d[4], d[3], d[2], d[1], tiny_dx2) * inv;
+      return weno5_ref<T>(d[5] * inv, d[4] * inv, d[3] * inv, d[2] * inv, d[1] * inv);
+    }
+  }
+  return upwind_weno5_d<T, FASTW>(d, vel, inv, tiny_dx2, zero_tol);
 }
 
 template <typename T, bool FASTW>
@@ -271,8 +305,16 @@ struct Pending {
 
 // IMAGES = false compiles out the periodic-image stores (callers whose
 // a.periodic is always 0, such as the patch level).
+// The LDS plane is double-buffered: plane k is written to buffer k & 1 (of
+// the chunk-relative index), so one barrier per plane suffices -- the reads
+// of a buffer (plane k - 2) all precede the barrier of plane k - 1, which
+// every wave passes before writing the buffer again at plane k.
+template <typename T, int TX, int TY>
+using StageTile = T[2][TY + 6][TX + 7];
+
 template <typename T, int TX, int TY, int KIND, bool FASTW, bool SYNTH, bool IMAGES = true>
-__device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by, int bz, T (*tile)[TX + 7]) {
+__device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by, int bz,
+                                           StageTile<T, TX, TY>& tiles) {
   // no implicit contraction: the FP32 arithmetic that fuses writes fma
   // explicitly, so unrolled copies of a plane round alike
 #pragma clang fp contract(off)
@@ -307,14 +349,14 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
     ry[n] = doy[n] ? load_rule<T, SYNTH>(a.ps, fb, g, ic, yj) : LoadRule{oc, oc, 0, 0};
   }
   const int txl = (tx < 3) ? tx : TX + tx;
-  const uint32_t ov = (KIND == STAGE_ADVECT_FIELD) ? plane_byte_offset<T>(a.vx, ic, jc) : 0u;
+  const uint32_t ov = reads_velocity<KIND>() ? plane_byte_offset<T>(a.vx, ic, jc) : 0u;
   const uint32_t opc = a.pc.p ? plane_byte_offset<T>(a.pc, ic, jc) : 0u;
   const uint32_t op0 = (KIND == STAGE_REINIT && a.use_phi0) ? plane_byte_offset<T>(a.phi0, ic, jc) : 0u;
   const uint32_t oo = (uint32_t)((ic - a.out.ilo) + (int)a.out.sy * (jc - a.out.jlo)) * (uint32_t)sizeof(T);
   const T tiny_dx2x = a.tdx2[0], tiny_dx2y = a.tdx2[1], tiny_dx2z = a.tdx2[2];
   // periodic images: only tiles within 3 cells of a periodic x or y face
-  const bool img_xy = IMAGES && ((a.periodic & 1) && (i0 - fb.ilo < 3 || fb.ihi - (i0 + TX - 1) < 3)) ||
-                      ((a.periodic & 2) && (j0 - fb.jlo < 3 || fb.jhi - (j0 + TY - 1) < 3));
+  const bool img_xy = IMAGES && (((a.periodic & 1) && (i0 - fb.ilo < 3 || fb.ihi - (i0 + TX - 1) < 3)) ||
+                                 ((a.periodic & 2) && (j0 - fb.jlo < 3 || fb.jhi - (j0 + TY - 1) < 3)));
 
   // plane pointers, advanced by one plane per step
   const int64_t s_ps = a.ps.sz * (int64_t)sizeof(T), s_out = a.out.sz * (int64_t)sizeof(T);
@@ -322,9 +364,9 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
   const int64_t s_p0 = a.phi0.sz * (int64_t)sizeof(T);
   const char* p_ps = plane_ptr<T>(a.ps, k0);
   const char* p_top = plane_ptr<T>(a.ps, k0 + 3);
-  const char* p_vx = (KIND == STAGE_ADVECT_FIELD) ? plane_ptr<T>(a.vx, k0) : nullptr;
-  const char* p_vy = (KIND == STAGE_ADVECT_FIELD) ? plane_ptr<T>(a.vy, k0) : nullptr;
-  const char* p_vz = (KIND == STAGE_ADVECT_FIELD) ? plane_ptr<T>(a.vz, k0) : nullptr;
+  const char* p_vx = reads_velocity<KIND>() ? plane_ptr<T>(a.vx, k0) : nullptr;
+  const char* p_vy = reads_velocity<KIND>() ? plane_ptr<T>(a.vy, k0) : nullptr;
+  const char* p_vz = reads_velocity<KIND>() ? plane_ptr<T>(a.vz, k0) : nullptr;
   const char* p_pc = a.pc.p ? plane_ptr<T>(a.pc, k0) : nullptr;
   const char* p_p0 = (KIND == STAGE_REINIT && a.use_phi0) ? plane_ptr<T>(a.phi0, k0) : nullptr;
   char* p_out = reinterpret_cast<char*>(a.out.p) + (int64_t)(k0 - a.out.klo) * s_out;
@@ -375,7 +417,7 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
     phx.load(rps, rx);
 #pragma unroll
     for (int n = 0; n < NR; n++) phy[n].load(rps, ry[n]);
-    if (KIND == STAGE_ADVECT_FIELD) {
+    if (reads_velocity<KIND>()) {
       ux = buf_load<T, kStream>(plane_rsrc(p_vx), ov);
       uy = buf_load<T, kStream>(plane_rsrc(p_vy), ov);
       uz = buf_load<T, kStream>(plane_rsrc(p_vz), ov);
@@ -391,7 +433,7 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
   auto advance = [&]() {
     p_ps += s_ps;
     p_top += s_ps;
-    if (KIND == STAGE_ADVECT_FIELD) {
+    if (reads_velocity<KIND>()) {
       p_vx += s_vel;
       p_vy += s_vel;
       p_vz += s_vel;
@@ -401,17 +443,19 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
   };
   if (k0 <= k1) fetch(k0);
 
+  int par = 0;  // LDS buffer of the plane (U == 1; with U == 2 it is u)
   for (int kg = k0; kg <= k1; kg += U) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int k = kg + u;
       if (k > k1) break;  // uniform
+      T (*tile)[TX + 7] = tiles[U == 2 ? u : par];
+      par ^= 1;
       zv[u + 3] = top_z ? ptop.va : ptop.value(rc);
       const T hx = phx.value(rx);
       T hy[NR];
 #pragma unroll
       for (int n = 0; n < NR; n++) hy[n] = phy[n].value(ry[n]);
-      __syncthreads();  // the previous plane's LDS reads are done
       tile[ty + 3][tx + 3] = zv[u];
       if (do_x) tile[ty + 3][txl] = hx;
 #pragma unroll
@@ -420,8 +464,10 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
       zd[u + 5] = zv[u + 3] - zv[u + 2];
       const T cux = ux, cuy = uy, cuz = uz, cpc = pcv, cp0 = p0v;
       __syncthreads();
-      advance();
-      if (k < k1) fetch(k + 1);  // uniform
+      // the next plane's loads; at the last plane the pointers stay and the
+      // loads re-read plane k1 (unused), so the fetch is unconditional
+      if (k < k1) advance();  // uniform
+      fetch(k < k1 ? k + 1 : k1);
 
       if (active) {
         T dx_[6], dy_[6];
@@ -455,10 +501,20 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
             const T sgn = a.use_phi0 ? cur / sqrt(cur * cur + a.dx_sq) : cur / sqrt(cur * cur + n2 * a.dx_sq);
             rhs = sgn * (T(1) - sqrt(n2));
           }
+        } else if (KIND == STAGE_ADVECT_LF) {
+          // Lax-Friedrichs Hamiltonian on the plus/minus WENO5 derivatives (lf_axis)
+          T xp, xm, yp, ym, zp, zm;
+          hj_weno5_pm_d<T, FASTW>(dx_, a.ix, tiny_dx2x, xp, xm);
+          hj_weno5_pm_d<T, FASTW>(dy_, a.iy, tiny_dx2y, yp, ym);
+          hj_weno5_pm_d<T, FASTW>(dz_, a.iz, tiny_dx2z, zp, zm);
+          const T hx = lf_axis<T>(cux, a.alpha_mode ? a.alpha[0] : fabs(cux), xp, xm);
+          const T hy = lf_axis<T>(cuy, a.alpha_mode ? a.alpha[1] : fabs(cuy), yp, ym);
+          const T hz = lf_axis<T>(cuz, a.alpha_mode ? a.alpha[2] : fabs(cuz), zp, zm);
+          rhs = -((hx + hy) + hz);
         } else {
-          const T gx = upwind_weno5_d<T, FASTW>(dx_, cux, a.ix, tiny_dx2x, a.zero_tol);
-          const T gy = upwind_weno5_d<T, FASTW>(dy_, cuy, a.iy, tiny_dx2y, a.zero_tol);
-          const T gz = upwind_weno5_d<T, FASTW>(dz_, cuz, a.iz, tiny_dx2z, a.zero_tol);
+          const T gx = upwind_weno5_wave<T, FASTW>(dx_, cux, a.ix, tiny_dx2x, a.zero_tol);
+          const T gy = upwind_weno5_wave<T, FASTW>(dy_, cuy, a.iy, tiny_dx2y, a.zero_tol);
+          const T gz = upwind_weno5_wave<T, FASTW>(dz_, cuz, a.iz, tiny_dx2z, a.zero_tol);
           // lsm_level_set_evolution3d.f.in:113-116 on a zeroed rhs
           rhs = FASTW ? -fma(cuz, gz, fma(cuy, gy, cux * gx)) : -(cux * gx + cuy * gy + cuz * gz);
         }

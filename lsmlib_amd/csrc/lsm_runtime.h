@@ -17,6 +17,9 @@ namespace lsm {
 // --- configuration (per host thread) ---------------------------------------
 hipStream_t current_stream();
 bool sync_on_return();
+// Threshold of the fused stages' upwind switch and reinitialization cut-off
+// (lsm3d_amd_set_zero_tol; the build's LSMLIB_ZERO_TOL by default).
+double current_zero_tol();
 
 // --- errors -------------------------------------------------------------------
 // Reports an error.  In abort mode (the default, matching a Fortran library

@@ -27,6 +27,7 @@ template <int S> struct SchemeRadius { static constexpr int R = (S == ENO1) ? 1 
 // (lsm_spatial_derivatives3d.f.in:1292-1326).
 template <typename T>
 __device__ __forceinline__ T weno5_ref(T v1, T v2, T v3, T v4, T v5) {
+#pragma clang fp contract(off)  // products and sums rounded separately, as the reference's
   const T one_third = T(1.0 / 3.0), seven_sixths = T(7.0 / 6.0), eleven_sixths = T(11.0 / 6.0);
   const T one_sixth = T(1.0 / 6.0), five_sixths = T(5.0 / 6.0);
   const T thirteen_twelfths = T(13.0 / 12.0), one_fourth = T(0.25);
@@ -67,7 +68,7 @@ __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
 // 1/dx).  With the second differences a = D1-D2, b = D2-D3, c = D3-D4,
 // d = D4-D5, the Jiang-Shu indicators times 12 are quadratic forms
 //   12 S1 = 13(a-b)^2 + 3(a-3b)^2 = 16 (a (a - 2.75 b) + 2.5 b^2)
-//   12 S2 = 13(b-c)^2 + 3(b+c)^2  = 16 (b^2 - 1.25 bc + c^2)
+//   12 S2 = 13(b-c)^2 + 3(b+c)^2  = 16 (b (b - 1.25 c) + c^2)
 //   12 S3 = 13(c-d)^2 + 3(3c-d)^2 = 16 (d (d - 2.75 c) + 2.5 c^2)
 // and eps = 1e-6 max(v_k^2) + tiny becomes E = 12e-6 max|D|^2 + tiny_dx2
 // (tiny_dx2 = 12 * tiny * dx^2 from the caller).  The weights
@@ -80,15 +81,30 @@ __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
 //   p1 - p2 = (a - 2b + c)/3,  p3 - p2 = (b - 2c + d)/6.
 // The multiply-adds are written as explicit fma so that every compiled copy
 // (unrolled planes, tile shapes) rounds alike.
+// max(|a|, |b|, |c|, |d|, |e|) of finite values in two v_max3_f32 with
+// absolute-value source modifiers.  The compiler's fmax first quiets
+// signalling NaNs of operands that come from another basic block or a loop
+// phi (one v_max per operand); these operands are differences of finite
+// samples, so the quieting is not needed.
+__device__ __forceinline__ float max_abs5(float a, float b, float c, float d, float e) {
+  float t, m;
+  asm("v_max3_f32 %0, |%1|, |%2|, |%3|" : "=v"(t) : "v"(a), "v"(b), "v"(c));
+  asm("v_max3_f32 %0, %1, |%2|, |%3|" : "=v"(m) : "v"(t), "v"(d), "v"(e));
+  return m;
+}
+__device__ __forceinline__ double max_abs5(double a, double b, double c, double d, double e) {
+  return fmax(fmax(fmax(fabs(a), fabs(b)), fabs(c)), fmax(fabs(d), fabs(e)));
+}
+
 template <typename T>
 __device__ __forceinline__ T weno5_fast(T D1, T D2, T D3, T D4, T D5, T tiny_dx2) {
 #pragma clang fp contract(off)
   const T a = D1 - D2, b = D2 - D3, c = D3 - D4, d = D4 - D5;
-  const T m = fmax(fmax(fmax(fabs(D1), fabs(D2)), fabs(D3)), fmax(fabs(D4), fabs(D5)));
+  const T m = max_abs5(D1, D2, D3, D4, D5);
   const T ie = fast_rcp(fma(T(0.75e-6), m * m, tiny_dx2 * T(1.0 / 16.0)));  // 16 / E
   const T bb = b * b, cc = c * c;
   const T q1 = fma(a, fma(T(-2.75), b, a), T(2.5) * bb);
-  const T q2 = fma(T(-1.25), b * c, bb) + cc;
+  const T q2 = fma(b, fma(T(-1.25), c, b), cc);
   const T q3 = fma(d, fma(T(-2.75), c, d), T(2.5) * cc);
   const T s1 = fma(q1, ie, T(1)), s2 = fma(q2, ie, T(1)), s3 = fma(q3, ie, T(1));
   const T t1 = s1 * s1, t2 = s2 * s2, t3 = s3 * s3;
@@ -350,6 +366,35 @@ __device__ __forceinline__ T ext_normal_select(T vc, T vn, T dp, T dm) {
   if (hp > T(0)) return (hm > T(0)) ? dm : -(vc / vn);
   if (hm < T(0)) return dp;
   return (fabs(hp) > fabs(hm)) ? dp : dm;
+}
+
+// Lax-Friedrichs numerical Hamiltonians (no counterpart in the reference,
+// whose level-set RHS routines are upwind or Godunov only).  From the plus
+// and minus one-sided derivatives p, m of one axis and a dissipation
+// coefficient alpha >= |dH/dphi_axis|:
+//   advection, H = V . grad(phi), per axis
+//     H_a = (0.5 (v - alpha)) p + (0.5 (v + alpha)) m
+//   = v (p + m)/2 - alpha (p - m)/2.  With alpha = |v| one of the two
+//   coefficients is exactly 0 and the other exactly v (2v and 0.5 * 2v are
+//   exact), so H_a = v m (v > 0) or v p (v < 0) bit for bit: the upwind term
+//   of LSM3D_ADD_ADVECTION_TERM_TO_LSE_RHS with the upwind selection of
+//   lsm_spatial_derivatives3d.f.in:2756-2810 (where |v| >= zero_tol).  In
+//   general H_a = upwind + 0.5 (|v| - alpha) (p - m).
+//   normal speed, H = vn |grad phi|:
+//     H = vn |(p + m)/2| - sum_a alpha (p_a - m_a)/2,   alpha = |vn|
+//   which on a planar signed distance (p = m, |grad phi| = 1) is vn.
+template <typename T>
+__host__ __device__ inline T lf_axis(T v, T alpha, T p, T m) {
+#pragma clang fp contract(off)
+  return (T(0.5) * (v - alpha)) * p + (T(0.5) * (v + alpha)) * m;
+}
+
+template <typename T>
+__host__ __device__ inline T lf_normal(T vn, T alpha, T xp, T yp, T zp, T xm, T ym, T zm) {
+#pragma clang fp contract(off)
+  const T ax = T(0.5) * (xp + xm), ay = T(0.5) * (yp + ym), az = T(0.5) * (zp + zm);
+  const T visc = (alpha * (T(0.5) * (xp - xm)) + alpha * (T(0.5) * (yp - ym))) + alpha * (T(0.5) * (zp - zm));
+  return vn * sqrt((ax * ax + ay * ay) + az * az) - visc;
 }
 
 // TVD Runge-Kutta stage combinations (lsm_tvd_runge_kutta3d.f), given the

@@ -370,6 +370,40 @@ __global__ void __launch_bounds__(256)
   LSM_CELL_LOOP_END
 }
 
+// Lax-Friedrichs advection term (lf_axis): rhs -= (H_x + H_y) + H_z, alpha
+// per axis = |v_a| at the cell (mode 0) or the given global alpha[a].
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_add_lf_advection(Field<T> rhs, Field<const T> xp, Field<const T> yp, Field<const T> zp,
+                       Field<const T> xm, Field<const T> ym, Field<const T> zm, Field<const T> vx,
+                       Field<const T> vy, Field<const T> vz, Box fb, int mode, T ax, T ay, T az) {
+#pragma clang fp contract(off)
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T u = vx.at(i, j, k), v = vy.at(i, j, k), w = vz.at(i, j, k);
+  const T hx = lf_axis<T>(u, mode ? ax : fabs(u), xp.at(i, j, k), xm.at(i, j, k));
+  const T hy = lf_axis<T>(v, mode ? ay : fabs(v), yp.at(i, j, k), ym.at(i, j, k));
+  const T hz = lf_axis<T>(w, mode ? az : fabs(w), zp.at(i, j, k), zm.at(i, j, k));
+  rhs.at(i, j, k) = rhs.at(i, j, k) - ((hx + hy) + hz);
+  LSM_CELL_LOOP_END
+}
+
+// Lax-Friedrichs normal-speed term (lf_normal): rhs -= H, alpha = |vn| at the
+// cell (mode 0) or the given global alpha; vn from the field, or vn_const
+// when vnf.p is null.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_add_lf_normal(Field<T> rhs, Field<const T> xp, Field<const T> yp, Field<const T> zp, Field<const T> xm,
+                    Field<const T> ym, Field<const T> zm, Field<const T> vnf, T vn_const, Box fb, int mode,
+                    T alpha) {
+#pragma clang fp contract(off)
+  LSM_CELL_LOOP_BEGIN(fb)
+  const T vn = vnf.p ? vnf.at(i, j, k) : vn_const;
+  const T h = lf_normal<T>(vn, mode ? alpha : fabs(vn), xp.at(i, j, k), yp.at(i, j, k), zp.at(i, j, k),
+                           xm.at(i, j, k), ym.at(i, j, k), zm.at(i, j, k));
+  rhs.at(i, j, k) = rhs.at(i, j, k) - h;
+  LSM_CELL_LOOP_END
+}
+
 // :532-719 over the reference's loop box (see the entry point)
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -748,6 +782,8 @@ void launch_cells(Call& c, const char* kname, const Box& b, K kernel, Args... ar
   c.launched(kname);
 }
 
+Box box6(const int* b) { return Box{b[0], b[1], b[2], b[3], b[4], b[5]}; }
+
 Field<const R> no_field() {
   Field<const R> f;
   f.p = nullptr;
@@ -1117,6 +1153,56 @@ void lsm3daddadvectiontermtolserhs_(R* rhs, BOXP(r), const R* px, const R* py, c
   auto fx = c.in(px, BOX(g)), fy = c.in(py, BOX(g)), fz = c.in(pz, BOX(g));
   auto ux = c.in(vx, BOX(v)), uy = c.in(vy, BOX(v)), uz = c.in(vz, BOX(v));
   launch_cells(c, "k_add_advection", fb, k_add_advection<R>, fr, fx, fy, fz, ux, uy, uz, fb);
+  c.finish();
+}
+
+// Part 3: Lax-Friedrichs Hamiltonians (no reference counterpart).
+void lsm3d_amd_add_lf_advection_term(R* rhs, const int rhs_gb[6], const R* xp, const R* yp, const R* zp,
+                                     const int plus_gb[6], const R* xm, const R* ym, const R* zm,
+                                     const int minus_gb[6], const R* vx, const R* vy, const R* vz,
+                                     const int vel_gb[6], const int fbp[6], int alpha_mode, const R* alpha) {
+  const char* fn = "lsm3d_amd_add_lf_advection_term";
+  Call c(fn);
+  const Box fb = box6(fbp);
+  if (alpha_mode != 0 && alpha_mode != 1) raise_error(std::string(fn) + ": alpha_mode is 0 (local) or 1 (global)");
+  if (alpha_mode == 1 && !alpha) raise_error(std::string(fn) + ": global alpha needs alpha[3]");
+  if (alpha_mode == 1 && alpha && !(alpha[0] >= 0 && alpha[1] >= 0 && alpha[2] >= 0))
+    raise_error(std::string(fn) + ": alpha must be non-negative");
+  require_inside(fn, "lse_rhs", box6(rhs_gb), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_plus", box6(plus_gb), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_minus", box6(minus_gb), fb, 0, 0, 0);
+  require_inside(fn, "vel", box6(vel_gb), fb, 0, 0, 0);
+  if (!c.ok()) return;
+  const R ax = alpha_mode ? alpha[0] : R(0), ay = alpha_mode ? alpha[1] : R(0), az = alpha_mode ? alpha[2] : R(0);
+  auto fr = c.out(rhs, box6(rhs_gb));
+  auto a = c.in(xp, box6(plus_gb)), b = c.in(yp, box6(plus_gb)), d = c.in(zp, box6(plus_gb));
+  auto e = c.in(xm, box6(minus_gb)), f = c.in(ym, box6(minus_gb)), g = c.in(zm, box6(minus_gb));
+  auto u = c.in(vx, box6(vel_gb)), v = c.in(vy, box6(vel_gb)), w = c.in(vz, box6(vel_gb));
+  launch_cells(c, "k_add_lf_advection", fb, k_add_lf_advection<R>, fr, a, b, d, e, f, g, u, v, w, fb, alpha_mode,
+               ax, ay, az);
+  c.finish();
+}
+
+void lsm3d_amd_add_lf_normal_vel_term(R* rhs, const int rhs_gb[6], const R* xp, const R* yp, const R* zp,
+                                      const int plus_gb[6], const R* xm, const R* ym, const R* zm,
+                                      const int minus_gb[6], const R* vel_n, const int vel_n_gb[6], R vel_n_const,
+                                      const int fbp[6], int alpha_mode, R alpha) {
+  const char* fn = "lsm3d_amd_add_lf_normal_vel_term";
+  Call c(fn);
+  const Box fb = box6(fbp);
+  if (alpha_mode != 0 && alpha_mode != 1) raise_error(std::string(fn) + ": alpha_mode is 0 (local) or 1 (global)");
+  if (alpha_mode == 1 && !(alpha >= 0)) raise_error(std::string(fn) + ": alpha must be non-negative");
+  require_inside(fn, "lse_rhs", box6(rhs_gb), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_plus", box6(plus_gb), fb, 0, 0, 0);
+  require_inside(fn, "grad_phi_minus", box6(minus_gb), fb, 0, 0, 0);
+  if (vel_n) require_inside(fn, "vel_n", box6(vel_n_gb), fb, 0, 0, 0);
+  if (!c.ok()) return;
+  auto fr = c.out(rhs, box6(rhs_gb));
+  auto a = c.in(xp, box6(plus_gb)), b = c.in(yp, box6(plus_gb)), d = c.in(zp, box6(plus_gb));
+  auto e = c.in(xm, box6(minus_gb)), f = c.in(ym, box6(minus_gb)), g = c.in(zm, box6(minus_gb));
+  Field<const R> n = vel_n ? c.in(vel_n, box6(vel_n_gb)) : no_field();
+  launch_cells(c, "k_add_lf_normal", fb, k_add_lf_normal<R>, fr, a, b, d, e, f, g, n, vel_n_const, fb, alpha_mode,
+               alpha);
   c.finish();
 }
 

@@ -47,9 +47,11 @@ class Grid:
     align (elements, 0 = off): widen the x ghost region so that every row of
     the ghost box, and the first fill-box cell of every row, start on an
     `align`-element boundary (32 for FP32 and 16 for FP64 put them on
-    128-byte cache-line boundaries of a 256-byte aligned allocation).  The
-    kernels still read only `ghost` cells beyond the fill box; the extra
-    cells are padding.
+    128-byte cache-line boundaries of a 256-byte aligned allocation), with
+    an odd number of such blocks per row, so that consecutive rows start in
+    different HBM channels (a plane march over rows of 17 cache lines streams
+    4 % faster than over rows of 18 on MI355X).  The kernels still read only
+    `ghost` cells beyond the fill box; the extra cells are padding.
 
     spacing: the grid spacing per axis when given (the serial package's
     createGridSetDx takes dx itself, lsm_grid.c:50-114); otherwise
@@ -69,7 +71,10 @@ class Grid:
     def xhi(self):  # ghost cells above the fill box along x
         if not self.align:
             return self.ghost
-        return self.ghost + (-(self.n[0] + self.xlo + self.ghost)) % self.align
+        hi = self.ghost + (-(self.n[0] + self.xlo + self.ghost)) % self.align
+        if ((self.xlo + self.n[0] + hi) // self.align) % 2 == 0:
+            hi += self.align  # an odd number of aligned blocks (cache lines) per row
+        return hi
 
     @property
     def fb(self):
@@ -206,6 +211,32 @@ class Stepper:
             self.lib.fused("lsm3d_amd_advect_const_stage", form, _ptr(out), self._gb, _ptr(ps), self._gb,
                            _ptr(pc), self._gb, u, v, w, self._fb, dx, dy, dz, dt, self._stage_bc,
                            self.periodic_mask)
+
+    def advect_lf_stage(self, form, out, ps, pc, vel, dt, alpha=None):
+        """One fused stage with the Lax-Friedrichs advection Hamiltonian on the
+        plus/minus HJ-WENO5 derivatives: alpha None = local (|v_a| per cell),
+        else the global (alpha_x, alpha_y, alpha_z)."""
+        dx, dy, dz = self.grid.dx
+        self._configure("advect")
+        mode, al = self._alpha(alpha)
+        self.lib.fused("lsm3d_amd_advect_lf_stage", form, _ptr(out), self._gb, _ptr(ps), self._gb, _ptr(pc), self._gb,
+                       _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), self._gb, self._fb, dx, dy, dz, dt, self._stage_bc,
+                       self.periodic_mask, mode, al)
+
+    def _alpha(self, alpha):
+        if alpha is None:
+            return 0, None
+        al = (self.lib._ct * 3)(*[float(a) for a in alpha])
+        return 1, al
+
+    def advect_lf_rk3_step(self, phi, s1, s2, vel, dt, alpha=None):
+        """phi <- one WENO5 / TVD-RK3 step with the Lax-Friedrichs Hamiltonian."""
+        self.advect_lf_stage(RK1, s1, phi, None, vel, dt, alpha)
+        self._prefill(s1)
+        self.advect_lf_stage(TVDRK3_STAGE2, s2, s1, phi, vel, dt, alpha)
+        self._prefill(s2)
+        self.advect_lf_stage(TVDRK3_STAGE3, phi, s2, phi, vel, dt, alpha)
+        self._prefill(phi)
 
     def advect_rk3_step(self, phi, s1, s2, vel, dt):
         """phi <- one WENO5 / TVD-RK3 step of phi_t + V.grad(phi) = 0 (in place).

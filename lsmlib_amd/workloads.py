@@ -82,10 +82,25 @@ def sphere_union(grid, n_spheres=64, seed=5, rmin=0.02, rmax=0.1, dtype=torch.fl
     return phi.to(dtype).contiguous()
 
 
-def sphere_union_taylor_green(n=512, n_spheres=64, seed=5, dtype=torch.float64, device="cpu", align=0):
+def abc_flow(grid, dtype=torch.float64, device="cpu"):
+    """Arnold-Beltrami-Childress flow with A = B = C = 1 on the unit cube:
+    u = sin2piz + cos2piy, v = sin2pix + cos2piz, w = sin2piy + cos2pix
+    (divergence-free, every component non-zero almost everywhere)."""
+    X, Y, Z = grid.centers(torch.float64, device)
+    tp = 2 * math.pi
+    u = torch.sin(tp * Z) + torch.cos(tp * Y)
+    v = torch.sin(tp * X) + torch.cos(tp * Z)
+    w = torch.sin(tp * Y) + torch.cos(tp * X)
+    return tuple(a.to(dtype).contiguous() for a in (u, v, w))
+
+
+def sphere_union_taylor_green(n=512, n_spheres=64, seed=5, dtype=torch.float64, device="cpu", align=0,
+                              velocity="taylor_green"):
+    """The large advection case; velocity="abc" replaces the Taylor-Green field
+    (w = 0) by the fully three-dimensional ABC flow."""
     grid = Grid((n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 3, align)
     phi = sphere_union(grid, n_spheres, seed, dtype=dtype, device=device)
-    vel = taylor_green(grid, dtype, device)
+    vel = abc_flow(grid, dtype, device) if velocity == "abc" else taylor_green(grid, dtype, device)
     return grid, phi, vel, (BC_PERIODIC,) * 6
 
 

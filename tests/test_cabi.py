@@ -16,7 +16,8 @@ def declared_symbols():
     for h in (REPO / "include").glob("*.h"):
         text = h.read_text()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"^\s*(?:void|int|const char\s*\*|void\s*\*)\s*\*?\s*(\w+)\s*\(", text, re.M):
+        for m in re.finditer(r"^\s*(?:const\s+)?(?:void|int|double|float|long\s+long|char|LSMLIB_REAL|lsm3d_amd_\w+)"
+                             r"\s*\*?\s*(\w+)\s*\(", text, re.M):
             names.add(m.group(1))
     return names
 
