@@ -47,13 +47,17 @@ class AssumeDevice {
 };
 bool assume_device();
 
-// Device scratch of at least `bytes`, owned by the library per device and
-// reused across calls (reductions keep per-block partials here).
+// Device scratch of at least `bytes`, owned by the library per (device,
+// stream) and reused across calls on that stream (reductions keep
+// per-block partials here).  Calls on different streams use different
+// buffers; calls from several host threads on one stream are ordered by the
+// stream but share the buffer, so such threads must not interleave calls.
 void* device_scratch(size_t bytes);
 
-// Further library-owned device buffers, one per (device, slot), grown on
-// demand like device_scratch.  *fresh is set when the buffer was (re)allocated
-// by this call, so a caller that keeps an invariant in it can re-establish it.
+// Further library-owned device buffers, one per (device, stream, slot), grown
+// on demand like device_scratch.  *fresh is set when the buffer was
+// (re)allocated by this call, so a caller that keeps an invariant in it can
+// re-establish it.
 enum BufferSlot { BUF_BAND_KEY = 0, BUF_BAND_WORK = 1, BUF_LOCAL_FLAGS = 2, BUF_SLOTS = 4 };
 void* device_buffer(int slot, size_t bytes, bool* fresh = nullptr, size_t* size = nullptr);
 

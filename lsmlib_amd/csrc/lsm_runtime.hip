@@ -4,7 +4,9 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 #include "../../include/lsm3d_amd.h"
 
@@ -20,8 +22,47 @@ thread_local bool tl_assume_device = false;
 
 std::mutex g_scratch_mu;
 constexpr int kMaxDevices = 64;
-void* g_scratch[kMaxDevices] = {nullptr};
-size_t g_scratch_bytes[kMaxDevices] = {0};
+
+// Library-owned device buffers, one per (device, stream, slot): calls queued
+// on different streams never share one, and a buffer is freed (to grow it)
+// only after its own stream has drained.  Buffers live until the process
+// ends.
+struct OwnedBuffer {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+std::map<std::tuple<int, uintptr_t, int>, OwnedBuffer> g_buffers;
+
+// The buffer of (current device, current stream, slot), grown to `bytes`
+// (at least `floor`, growth `grow` x the request); *fresh reports a new
+// allocation.
+void* owned_buffer(int slot, size_t bytes, size_t floor, bool grow, bool* fresh, size_t* size) {
+  if (fresh) *fresh = false;
+  int dev = 0;
+  check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) {
+    raise_error("device index out of range");
+    return nullptr;
+  }
+  const hipStream_t s = tl_stream;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  OwnedBuffer& b = g_buffers[std::make_tuple(dev, reinterpret_cast<uintptr_t>(s), slot)];
+  if (b.bytes < bytes) {
+    if (b.p) {
+      // this stream's queued work may still read the old buffer
+      check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+      check_hip(hipFree(b.p), "hipFree(buffer)");
+    }
+    size_t want = bytes < floor ? floor : (grow ? bytes + bytes / 4 : bytes);
+    b.p = nullptr;
+    b.bytes = 0;
+    check_hip(hipMalloc(&b.p, want), "hipMalloc(buffer)");
+    if (b.p) b.bytes = want;
+    if (fresh) *fresh = true;
+  }
+  if (size) *size = b.bytes;
+  return b.p;
+}
 }  // namespace
 
 hipStream_t current_stream() { return tl_stream; }
@@ -66,54 +107,14 @@ AssumeDevice::AssumeDevice() : prev_(tl_assume_device) { tl_assume_device = true
 AssumeDevice::~AssumeDevice() { tl_assume_device = prev_; }
 bool assume_device() { return tl_assume_device; }
 
-void* device_scratch(size_t bytes) {
-  int dev = 0;
-  check_hip(hipGetDevice(&dev), "hipGetDevice");
-  if (dev < 0 || dev >= kMaxDevices) {
-    raise_error("device index out of range");
-    return nullptr;
-  }
-  std::lock_guard<std::mutex> lk(g_scratch_mu);
-  if (g_scratch_bytes[dev] < bytes) {
-    if (g_scratch[dev]) {
-      // another thread's queued work may still read the old buffer
-      check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
-      check_hip(hipFree(g_scratch[dev]), "hipFree(scratch)");
-    }
-    size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
-    g_scratch[dev] = nullptr;
-    g_scratch_bytes[dev] = 0;
-    check_hip(hipMalloc(&g_scratch[dev], want), "hipMalloc(scratch)");
-    if (g_scratch[dev]) g_scratch_bytes[dev] = want;
-  }
-  return g_scratch[dev];
-}
+void* device_scratch(size_t bytes) { return owned_buffer(-1, bytes, 1u << 20, false, nullptr, nullptr); }
 
 void* device_buffer(int slot, size_t bytes, bool* fresh, size_t* size) {
-  static void* bufs[kMaxDevices][BUF_SLOTS] = {};
-  static size_t sizes[kMaxDevices][BUF_SLOTS] = {};
-  if (fresh) *fresh = false;
-  int dev = 0;
-  check_hip(hipGetDevice(&dev), "hipGetDevice");
-  if (dev < 0 || dev >= kMaxDevices || slot < 0 || slot >= BUF_SLOTS) {
+  if (slot < 0 || slot >= BUF_SLOTS) {
     raise_error("device buffer index out of range");
     return nullptr;
   }
-  std::lock_guard<std::mutex> lk(g_scratch_mu);
-  if (sizes[dev][slot] < bytes) {
-    if (bufs[dev][slot]) {
-      check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
-      check_hip(hipFree(bufs[dev][slot]), "hipFree(buffer)");
-    }
-    size_t want = bytes < (1u << 16) ? (1u << 16) : bytes + bytes / 4;
-    bufs[dev][slot] = nullptr;
-    sizes[dev][slot] = 0;
-    check_hip(hipMalloc(&bufs[dev][slot], want), "hipMalloc(buffer)");
-    if (bufs[dev][slot]) sizes[dev][slot] = want;
-    if (fresh) *fresh = true;
-  }
-  if (size) *size = sizes[dev][slot];
-  return bufs[dev][slot];
+  return owned_buffer(slot, bytes, 1u << 16, true, fresh, size);
 }
 
 Call::Call(const char* name) : name_(name), stream_(tl_stream) {
