@@ -1074,6 +1074,22 @@ void lsm3d_amd_reinit_stage(int rk_form, LSMLIB_REAL *phi_out, const int out_gb[
                             const LSMLIB_REAL *phi0, const int phi0_gb[6], int use_phi0_for_sgn,
                             const int fb[6], LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz,
                             LSMLIB_REAL dt, const int bc[6], int periodic_mask);
+/* LSM3D_COMPUTE_STABLE_ADVECTION_DT that allocates nothing.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and result as LSM3D_COMPUTE_STABLE_ADVECTION_DT
+ * (lsm_utilities3d.h, dt = cfl / (max over ib of |u|/dx + |v|/dy + |w|/dz,
+ * at least -1, + tiny)), passed by value, except that dt and the velocity
+ * arrays must be device memory and dt is written on the device, in stream
+ * order.  The partial maxima go to a fixed device array of 65536 doubles
+ * (512 KiB per GPU, kept until the process ends); boxes with
+ * ceil(nx/64) * ceil(ny/4) > 65536 are rejected with an error.  Every call on
+ * one GPU uses that same array: calls that overlap on one GPU (two streams or
+ * two host threads at once) race. */
+void lsm3d_amd_compute_stable_advection_dt_noalloc(LSMLIB_REAL *dt, const LSMLIB_REAL *vel_x,
+                                                   const LSMLIB_REAL *vel_y, const LSMLIB_REAL *vel_z,
+                                                   const int vel_gb[6], const int ib[6], LSMLIB_REAL dx,
+                                                   LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL cfl);
 /* --- Lax-Friedrichs Hamiltonians (no counterpart in the reference, whose
  * level-set RHS routines are upwind or Godunov only) ----------------------
  * From the plus/minus one-sided derivatives (e.g. of LSM3D_HJ_WENO5) and a

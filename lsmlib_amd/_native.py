@@ -76,6 +76,8 @@ class ToolboxLibrary(F77Library):
         L.lsm3d_amd_reinit_stage.argtypes = [I, P, B, P, B, P, B, P, B, I, B, R, R, R, R, B, I]
         L.lsm3d_amd_fill_ghosts.argtypes = [P, B, B, B]
         RP = ctypes.POINTER(R)
+        L.lsm3d_amd_compute_stable_advection_dt_noalloc.argtypes = [P, P, P, P, B, B, R, R, R, R]
+        L.lsm3d_amd_compute_stable_advection_dt_noalloc.restype = None
         L.lsm3d_amd_advect_lf_stage.argtypes = [I, P, B, P, B, P, B, P, P, P, B, B, R, R, R, R, B, I, I, RP]
         L.lsm3d_amd_add_lf_advection_term.argtypes = [P, B, P, P, P, B, P, P, P, B, P, P, P, B, B, I, RP]
         L.lsm3d_amd_add_lf_normal_vel_term.argtypes = [P, B, P, P, P, B, P, P, P, B, P, B, R, B, I, R]

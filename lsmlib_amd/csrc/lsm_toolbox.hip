@@ -12,6 +12,7 @@
 // not written: no caller reads them after a call, and the kernels form the
 // undivided differences in registers instead of re-reading them from HBM.
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../include/lsm3d_amd.h"
@@ -802,15 +803,15 @@ double run_reduce(Call& c, RedArgs a, const Box& ib, Field<const R> f0, Field<co
   dim3 g = cell_grid(ib);
   if (g.z > 64) g.z = 64;
   const int64_t nb = (int64_t)g.x * g.y * g.z;
-  double* scratch = reinterpret_cast<double*>(device_scratch((size_t)(nb + 1) * sizeof(double)));
+  double* part = reinterpret_cast<double*>(device_scratch((size_t)(nb + 1) * sizeof(double)));
   if (!c.ok()) return ident;
   hipLaunchKernelGGL(k_reduce<R>, g, dim3(kBX, kBY, 1), 0, c.stream(), a, ib, f0, f1, f2, f3, f4, f5, f6,
-                     cv, scratch + 1);
+                     cv, part + 1);
   c.launched("k_reduce");
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream(), scratch + 1, nb, op, scratch);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream(), part + 1, nb, op, part);
   c.launched("k_reduce_final");
   double out = ident;
-  check_hip(hipMemcpyAsync(&out, scratch, sizeof(double), hipMemcpyDeviceToHost, c.stream()),
+  check_hip(hipMemcpyAsync(&out, part, sizeof(double), hipMemcpyDeviceToHost, c.stream()),
             "hipMemcpyAsync(reduction result)");
   check_hip(hipStreamSynchronize(c.stream()), "hipStreamSynchronize");
   return out;
@@ -906,6 +907,53 @@ R max_of3(R a, R b, R c) {
 }
 
 }  // namespace
+
+namespace lsm {
+// ---- LSM3D_COMPUTE_STABLE_ADVECTION_DT without allocation -------------------
+// The per-workgroup partial maxima live in a file-scope device array instead
+// of the library's grown scratch: kNoAllocPartials doubles after one slot for
+// the maximum, 512 KiB + 256 B, held by every GPU the library runs on until
+// the process ends.  Its address on each device comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr int64_t kNoAllocPartials = 65536;
+static __device__ __align__(256) double scratch[kNoAllocPartials + 32];
+
+namespace noalloc {
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[64] = {nullptr};
+
+double* noalloc_scratch(const char* fn) {
+  int dev = 0;
+  check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (error_pending()) return nullptr;
+  if (dev < 0 || dev >= 64) {
+    raise_error(std::string(fn) + ": device index out of range");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      raise_error(std::string(fn) + ": hipGetSymbolAddress(scratch): " + hipGetErrorString(e));
+      return nullptr;
+    }
+    g_noalloc_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_noalloc_ptr[dev];
+}
+}  // namespace
+
+// dt = cfl / (max(-1, m) + tiny) from the reduced maximum m (lsm_utilities3d.f.in:297, 323),
+// with m absent for an empty box
+template <typename T>
+__global__ void k_adv_dt_from_max(const double* m, int have, T cfl, T* dt) {
+  T mx = T(-1.0);
+  if (have && m[0] > (double)mx) mx = T(m[0]);
+  dt[0] = cfl / (mx + Tol<T>::tiny);
+}
+
+}  // namespace lsm
 
 extern "C" {
 
@@ -1463,6 +1511,54 @@ void lsm3dcomputestableadvectiondt_(R* dt, const R* vx, const R* vy, const R* vz
                                     const R* dx, const R* dy, const R* dz, const R* cfl) {
   adv_dt_entry("LSM3D_COMPUTE_STABLE_ADVECTION_DT", dt, vx, vy, vz, BOX(v), nullptr, Box{}, 0, BOX(ib), *dx,
                *dy, *dz, *cfl);
+}
+
+void lsm3d_amd_compute_stable_advection_dt_noalloc(R* dt, const R* vx, const R* vy, const R* vz,
+                                                   const int vel_gb[6], const int ibp[6], R dx, R dy, R dz, R cfl) {
+  const char* fn = "lsm3d_amd_compute_stable_advection_dt_noalloc";
+  Call c(fn);
+  const Box ib = box6(ibp), gv = box6(vel_gb);
+  if (!dt || !vx || !vy || !vz) raise_error(std::string(fn) + ": missing array");
+  require_inside(fn, "vel", gv, ib, 0, 0, 0);
+  if (!c.ok()) return;
+  dim3 g = cell_grid(ib);
+  const int64_t plane_blocks = (int64_t)g.x * g.y;
+  if (ib.cells() > 0 && plane_blocks > kNoAllocPartials) {
+    raise_error(std::string(fn) + ": box too large for the fixed scratch: ceil(nx/64) * ceil(ny/4) = " +
+                std::to_string(plane_blocks) + " partial maxima per plane exceed " + std::to_string(kNoAllocPartials));
+    return;
+  }
+  for (const void* q : {(const void*)dt, (const void*)vx, (const void*)vy, (const void*)vz})
+    if (!is_device_pointer(q)) {
+      raise_error(std::string(fn) + ": dt and the velocity arrays must be device memory");
+      return;
+    }
+  double* sc = lsm::noalloc::noalloc_scratch(fn);
+  if (!sc || !c.ok()) return;
+  int64_t nb = 0;
+  if (ib.cells() > 0) {
+    int64_t gz = g.z;
+    if (gz > 64) gz = 64;
+    if (gz > kNoAllocPartials / plane_blocks) gz = kNoAllocPartials / plane_blocks;
+    g.z = (unsigned)gz;
+    nb = plane_blocks * gz;
+    RedArgs ra{};
+    ra.kind = RK_ADV_DT;
+    ra.ix = R(1) / dx;
+    ra.iy = R(1) / dy;
+    ra.iz = R(1) / dz;
+    ra.ci = ra.cj = ra.ck = INT32_MIN;
+    Field<const R> a = make_field<const R>(vx, gv), b = make_field<const R>(vy, gv), d = make_field<const R>(vz, gv);
+    hipLaunchKernelGGL(k_reduce<R>, g, dim3(kBX, kBY, 1), 0, c.stream(), ra, ib, a, b, d, no_field(), no_field(),
+                       no_field(), no_field(), no_field(), sc + 1);
+    c.launched("k_reduce");
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c.stream(), sc + 1, nb, (int)RED_MAX, sc);
+    c.launched("k_reduce_final");
+  }
+  hipLaunchKernelGGL(k_adv_dt_from_max<R>, dim3(1), dim3(1), 0, c.stream(), (const double*)sc, nb > 0 ? 1 : 0, cfl,
+                     dt);
+  c.launched("k_adv_dt_from_max");
+  c.finish();
 }
 
 void lsm3dcomputestableadvectiondtcontrolvolume_(R* dt, const R* vx, const R* vy, const R* vz, BOXP(v),
