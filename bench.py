@@ -51,13 +51,15 @@ def parse():
                     help="synth: stages synthesise periodic ghost values; images: stages write and read "
                          "stored periodic ghost images")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--host-fields", action="store_true",
+                    help="--only batched: build the initial patch pool on the host (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB device-to-device copy (scale for HBM byte counters)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-patch and reinitialization measurements")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (adjoint) measurement")
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the gradient")
-    ap.add_argument("--only", choices=["probe", "batched", "reinit", "adjoint", "config1", "config5", "throughput",
+    ap.add_argument("--only", choices=["probe", "lf", "batched", "reinit", "adjoint", "config1", "config5", "throughput",
                                        "curvature", "curvature_step", "curvature_local", "curvature_local_step",
                                        "field_extension"],
                     default=None,
@@ -152,7 +154,7 @@ def run_bandwidth_probe(mib=2048, reps=10):
     out = {}
     for kind, name, moved in ((0, "copy", 2), (1, "read5_write1", 6), (2, "read", 1)):
         best = None
-        for wg in (1024, 2048, 4096, 8192, 16384):
+        for wg in (2048, 4096, 8192, 16384, 32768):
             if kind == 2 and wg * 256 * 4 > nbytes:
                 continue
             ms = lib.fused("lsm_amd_bandwidth_probe", kind, ctypes.c_void_p(src.data_ptr()),
@@ -166,17 +168,72 @@ def run_bandwidth_probe(mib=2048, reps=10):
     return out
 
 
-def run_batched(real="float", npatch=512, steps=20, reps=2):
+def run_lf(n=512, steps=10, warmup=2):
+    """WENO5 / TVD-RK3 advection with the local Lax-Friedrichs Hamiltonian on
+    the headline workload (n^3 sphere union, Taylor-Green velocity, CFL 0.5),
+    FP32 and FP64: per-stage kernel time from HIP events on the library
+    stream, and the distance to the upwind step (the two agree where
+    |v_a| >= zero_tol, so the difference is only the velocity's zero band)."""
+    import torch
+    from lsmlib_amd.pylsmlib import RK1, TVDRK3_STAGE2, TVDRK3_STAGE3, Stepper, stable_advection_dt
+    from lsmlib_amd.workloads import line_elems, sphere_union_taylor_green
+    dev = torch.device("cuda:0")
+    out = {}
+    for real, dtype in (("float", torch.float32), ("double", torch.float64)):
+        grid, phi64, vel, bc = sphere_union_taylor_green(n, dtype=torch.float64, device=dev, align=line_elems(dtype))
+        dt = stable_advection_dt(vel, grid, 0.5, "double")
+        phi = phi64.to(dtype).clone()
+        del phi64
+        vel = tuple(v.to(dtype).contiguous() for v in vel)
+        st = Stepper(grid, real, bc, zero_tol=SWITCH_TOL if real == "float" else None)
+        st.fill_ghosts(phi)
+        up = phi.clone()
+        s1, s2 = st.empty(), st.empty()
+        for _ in range(warmup):
+            st.advect_lf_rk3_step(phi, s1, s2, vel, dt)
+            st.advect_rk3_step(up, s1, s2, vel, dt)
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream(dev)
+        ev = []
+        for _ in range(steps):
+            for f, o, ps, pc in ((RK1, s1, phi, None), (TVDRK3_STAGE2, s2, s1, phi), (TVDRK3_STAGE3, phi, s2, phi)):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                st.advect_lf_stage(f, o, ps, pc, vel, dt)
+                e1.record(stream)
+                ev.append((f, e0, e1))
+        torch.cuda.synchronize()
+        ms = sum(e0.elapsed_time(e1) for _, e0, e1 in ev) / len(ev)
+        word = 4 if real == "float" else 8
+        alg = sum(WORDS[f] for f, _, _ in ev) / len(ev) * grid.interior_cells * word
+        for _ in range(steps):
+            st.advect_rk3_step(up, s1, s2, vel, dt)
+        torch.cuda.synchronize()
+        diff = float((grid.interior(phi).double() - grid.interior(up).double()).abs().max()) / grid.dx[0]
+        out[real] = {"kernel_ms_avg": ms, "cell_updates_per_s": grid.interior_cells / (3 * ms * 1e-3),
+                     "achieved_gbs": alg / (ms * 1e-3) / 1e9, "steps": warmup + steps,
+                     "max_diff_vs_upwind_over_dx": diff}
+        print(f"# lax-friedrichs {real} {n}^3: stage {ms:.3f} ms, {out[real]['cell_updates_per_s']:.3e} "
+              f"cell-updates/s, |LF - upwind| {diff:.2e} dx", file=sys.stderr, flush=True)
+        del phi, up, s1, s2, vel, st
+        torch.cuda.empty_cache()
+    out["config"] = f"{n}^3 sphere union + Taylor-Green, local Lax-Friedrichs on plus/minus HJ-WENO5, TVD-RK3, CFL 0.5"
+    return out
+
+
+def run_batched(real="float", npatch=512, steps=20, reps=2, tiling="auto", host_fields=False):
     """512 uneven patches (extents U{16..96}, seed 3, dx 1/128, 3 ghosts,
     per-patch constant velocity), `steps` WENO5/TVD-RK3 steps captured into
-    one hipGraph; replays are timed."""
+    one hipGraph; replays are timed.  host_fields builds the initial pool on
+    the host (profiling runs: no small set-up kernels)."""
     import torch
     from lsmlib_amd.patches import PatchLevel, random_patch_level
     from lsmlib_amd.workloads import line_elems
     desc = random_patch_level(npatch=npatch, lo=16, hi=96, seed=3, dx=1.0 / 128)
     dtype = torch.float32 if real == "float" else torch.float64
-    lvl = PatchLevel(desc, real, align=line_elems(dtype))
-    phi = lvl.initial_phi()
+    tl = "auto" if tiling == "auto" else tuple(int(x) for x in tiling.split(","))
+    lvl = PatchLevel(desc, real, align=line_elems(dtype), tiling=tl)
+    phi = lvl.initial_phi(on_host=host_fields)
     s1, s2 = lvl.empty_pool(), lvl.empty_pool()
     dt = lvl.stable_dt(0.5)
     lvl.capture_rk3(steps, phi, s1, s2, dt)
@@ -415,6 +472,10 @@ def run_config5(n=512, steps=200, reinit_every=10, reinit_iters=5):
         del s1, s2, vel
         torch.cuda.empty_cache()
     out["fp32_error_vs_fp64"] = fp32_error(*res["float"], *res["double"])
+    del res
+    torch.cuda.empty_cache()
+    from lsmlib_amd.precision import precision_split
+    out["fp32_error_split"] = precision_split(n, steps, reinit_every, reinit_iters)
     out["config"] = (f"{n}^3 periodic sphere union + Taylor-Green, {steps} WENO5/TVD-RK3 steps with "
                      f"{reinit_iters} reinitialization iterations every {reinit_every} steps")
     return out
@@ -470,8 +531,7 @@ def run_curvature(dxs=(0.04, 0.02), big_dx=1.5 / 512, big_steps=10, cpu=True, ru
                 "grid": list(g.n), "seconds": el, "steps": st["steps"], "intervals": st["outer"],
                 "reinit_iterations": st["reinit_iters"], "t_final": st["t"], "vol_phi": st["vol_phi"],
                 "max_change_last_interval": st["max_abs_err"],
-                "published_reference_s": CURV_PUBLISHED_S.get(dx),
-                "speedup_vs_published": CURV_PUBLISHED_S[dx] / el if dx in CURV_PUBLISHED_S else None}
+                "published_reference_s": CURV_PUBLISHED_S.get(dx)}
             finals[(real, dx)] = (g, p)
             print(f"# curvature {real} dx={dx} {g.n}: {el:.3f} s, {st['steps']} steps", file=sys.stderr, flush=True)
     # FP32 against FP64 after the whole run, near the interface
@@ -581,8 +641,7 @@ def run_curvature_local(dxs=(0.04, 0.02), cpu=True, cpu_intervals=3, big_dx=1.5 
                 "grid": list(g.n), "seconds": el, "steps": st["steps"], "intervals": st["outer"],
                 "reinitializations": st["reinits"], "triggers": st["triggers"], "t_final": st["t"],
                 "vol_phi": st["vol_phi"], "mean_band_fraction": float(np.mean(st["band_fraction"])),
-                "ms_per_step": 1e3 * el / max(st["steps"], 1),
-                "speedup_vs_published": LOCAL_PUBLISHED_S[dx] / el if dx in LOCAL_PUBLISHED_S else None}
+                "ms_per_step": 1e3 * el / max(st["steps"], 1)}
             print(f"# curvature local {real} dx={dx} {g.n}: {el:.3f} s, {st['steps']} steps, band fraction "
                   f"{np.mean(st['band_fraction']):.3f}", file=sys.stderr, flush=True)
     if cpu:
@@ -812,6 +871,9 @@ def main():
                           "launches": 3}), flush=True)
         return
 
+    if a.only == "lf":
+        print(json.dumps({"lax_friedrichs_512": run_lf(a.n)}), flush=True)
+        return
     if a.only == "probe":
         print(json.dumps({"bandwidth_probe": run_bandwidth_probe()}), flush=True)
         return
@@ -819,7 +881,8 @@ def main():
         print(json.dumps({"adjoint": run_adjoint(fd=not a.no_fd)}), flush=True)
         return
     if a.only == "batched":
-        print(json.dumps({"batched_patches_fp32": run_batched("float")}), flush=True)
+        print(json.dumps({"batched_patches_fp32": run_batched("float", tiling=a.tiling, host_fields=a.host_fields)}),
+              flush=True)
         return
     if a.only == "reinit":
         print(json.dumps({"reinit_256": run_reinit()}), flush=True)
@@ -868,14 +931,24 @@ def main():
                                velocity=a.velocity)
         err_build_tol = fp32_error(grid32, phi32b, grid64, phi64)
         err_build_tol["after_steps"] = r32["steps_total"]
-        print(f"# fp32 error vs fp64 near interface: 99.99% of cells within {err['near_p9999_over_dx']:.2e} dx, "
-              f"{err['near_cells_above_1e-5_dx']} of {err['near_cells']} above 1e-5 dx; max "
-              f"{err['max_near_interface_over_dx']:.3e} dx "
-              f"(switch 1e-11), {err_build_tol['max_near_interface_over_dx']:.3e} dx (switch 1e-5)",
+        print(f"# fp32 error vs fp64 near interface: max {err['max_near_interface_over_dx']:.3e} dx "
+              f"(switch 1e-11; {err_build_tol['max_near_interface_over_dx']:.3e} dx with 1e-5), "
+              f"{err['near_cells_above_1e-5_dx']} of {err['near_cells']} cells above 1e-5 dx",
               file=sys.stderr, flush=True)
         del phi64, phi32b
     phi32 = None
     torch.cuda.empty_cache()
+    split = None
+    if not a.fp32_only:
+        from lsmlib_amd.precision import precision_split
+        split = precision_split(a.n, a.warmup + a.steps, velocity=a.velocity)
+        print(f"# fp32 error split after {a.warmup + a.steps} steps (max near interface / dx): fp32 "
+              f"{split['fp32_vs_fp64']['max_over_dx']:.2e}, inputs only {split['inputs_only']['max_over_dx']:.2e}, "
+              f"fp32 storage only {split['storage_only']['max_over_dx']:.2e}, one step "
+              f"{split['one_step']['max_over_dx']:.2e}", file=sys.stderr, flush=True)
+    probe = run_bandwidth_probe()
+    print(f"# measured streaming rates: copy {probe['copy']['gbs']:.0f} GB/s, 5 reads + 1 write "
+          f"{probe['read5_write1']['gbs']:.0f} GB/s, reads {probe['read']['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(a.cpu_seconds)
@@ -887,14 +960,19 @@ def main():
         print(f"# batched: {extras['batched_patches_fp32']['value']:.4e} cell-updates/s", file=sys.stderr,
               flush=True)
         torch.cuda.empty_cache()
+        extras["lax_friedrichs_512"] = run_lf(a.n)
+        torch.cuda.empty_cache()
         extras["reinit_256"] = run_reinit()
         print(f"# reinit: fp64 {extras['reinit_256']['double']['ms_per_iteration']:.2f} ms/it, fp32 "
               f"{extras['reinit_256']['float']['ms_per_iteration']:.2f} ms/it", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
         extras["config5_512_full"] = run_config5()
         c5 = extras["config5_512_full"]
+        sp = c5["fp32_error_split"]
         print(f"# config5: fp32 {c5['float']['seconds']:.2f} s, fp64 {c5['double']['seconds']:.2f} s, near-interface "
-              f"error {c5['fp32_error_vs_fp64']['max_near_interface_over_dx']:.3e} dx", file=sys.stderr, flush=True)
+              f"error {c5['fp32_error_vs_fp64']['max_near_interface_over_dx']:.3e} dx (inputs only "
+              f"{sp['inputs_only']['max_over_dx']:.2e}, fp32 storage only {sp['storage_only']['max_over_dx']:.2e}, "
+              f"one step {sp['one_step']['max_over_dx']:.2e})", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
         extras["throughput_1024_fp32"] = run_throughput()
         print(f"# 1024^3: {extras['throughput_1024_fp32']['value']:.4e} cell-updates/s", file=sys.stderr, flush=True)
@@ -941,11 +1019,14 @@ def main():
         "fp32_error_vs_fp64_one_step": err_one,
         "fp32_error_vs_fp64": err,
         "fp32_error_vs_fp64_switch_1e-5": err_build_tol,
+        "fp32_error_split": split,
         "roofline": {"bound": "hbm", "achieved": r32["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": r32["achieved_gbs"] / HBM_PEAK_GBS, "traffic": read_traffic(),
                      "kernel": "k_stage (fused advection stage)",
                      "kernel_ms_avg": r32["kernel_ms_avg"],
                      "algorithmic_bytes_per_launch": r32["algorithmic_bytes_per_launch"],
+                     "measured_peak_gbs": {k: v["gbs"] for k, v in probe.items()},
+                     "frac_of_measured_read5_write1": r32["achieved_gbs"] / probe["read5_write1"]["gbs"],
                      "valu": valu_roofline(r32)},
         "cpu_baseline": cpu,
         "adjoint_s": None if adj is None else adj["adjoint_s"],

@@ -1156,7 +1156,10 @@ void lsm3d_amd_stage_tiling_shape(int cfg, int *tx, int *ty);
  * persistent launch per Runge-Kutta stage covers all patches (the reference
  * loops over patches, LevelSetMethodToolbox.cc:2446-2592).  Ghost values
  * follow bc[6] as for the fused stages: synthesised per face (codes 1 copy,
- * 2 linear, 3 signed linear) or read as stored (code 0); not periodic. */
+ * 2 linear, 3 signed linear) or read as stored (code 0); not periodic.
+ * Workgroups take tiles from a per-plan device queue that each launch leaves
+ * reset, so the launches of one plan must not overlap (one stream, or
+ * ordered streams). */
 typedef struct lsm3d_amd_patch_plan lsm3d_amd_patch_plan;
 lsm3d_amd_patch_plan *lsm3d_amd_patch_plan_create(int npatch, const int *fbs, const int *gbs,
                                                   const long long *offsets, long long pool_elems,

@@ -9,9 +9,11 @@
 // replayed.
 //
 // Work distribution: tiles are numbered patch by patch; the 8 XCDs each own
-// a contiguous range of the tile list (so a patch's halo re-reads stay in
-// one L2) and that XCD's workgroups stride through it.  A tile finds its
-// patch by binary search over the per-patch tile prefix.
+// a contiguous range of the tile list of about equal cost (so a patch's halo
+// re-reads stay in one L2), ordered costliest first, and that XCD's
+// workgroups take tiles from it through an atomic counter until it is empty
+// (a dynamic queue: the tiles' real costs vary more than any static
+// estimate).  Each queue entry carries its patch, so a tile needs no search.
 //
 // Ghost values: the reference fills every patch's boundary ghost cells
 // before each stage (LevelSetFunctionIntegrator.cc:1319-1510).  Here the
@@ -26,7 +28,6 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
-#include <queue>
 #include <string>
 #include <vector>
 
@@ -82,25 +83,36 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Workgroup b runs the tiles sched_tile[sched_off[b] .. sched_off[b+1]-1]
-// (a host-built schedule balancing the tiles' costs, see build_schedule).
+// The queue of one stage launch: XCD x owns entries off[x] .. off[x+1]-1 of
+// patch[] / tile[] (patch index and tile number within the patch); ctr[x] is
+// its next entry and ctr[8] counts finished workgroups.  The last workgroup
+// to finish resets the counters, so every launch (graph replays included)
+// starts from zero without a memset.
+struct PatchQueue {
+  const int* off;    // kNumXcd + 1
+  const int* patch;  // ntiles
+  const int* tile;   // ntiles
+  int* ctr;          // kNumXcd + 1, zero between launches
+};
+
+// Workgroup b takes tiles from the queue of XCD b % 8 (the hardware hands
+// workgroups to the XCDs round-robin).
 template <typename T, int TX, int TY, bool SYNTH>
 __global__ void __launch_bounds__(TX* TY)
-    k_patch_stage(const PatchDesc* __restrict__ desc, const int64_t* __restrict__ prefix, int npatch,
-                  const int* __restrict__ sched_off, const int* __restrict__ sched_tile, T* out_pool,
-                  const T* ps_pool, const T* pc_pool, GhostRule ghost, T ix, T iy, T iz, T dt, T zt, T tdx2x,
-                  T tdx2y, T tdx2z, int form) {
+    k_patch_stage(const PatchDesc* __restrict__ desc, PatchQueue qu, T* out_pool, const T* ps_pool,
+                  const T* pc_pool, GhostRule ghost, T ix, T iy, T iz, T dt, T zt, T tdx2x, T tdx2y, T tdx2z,
+                  int form) {
   __shared__ StageTile<T, TX, TY> tile;
-  const int b = (int)blockIdx.x;
-  const int q0 = sched_off[b], q1 = sched_off[b + 1];
-  for (int q = q0; q < q1; q++) {
-    const int64_t t = sched_tile[q];
-    int l = 0, r = npatch - 1;
-    while (l < r) {
-      const int m = (l + r + 1) >> 1;
-      if (prefix[m] <= t) l = m;
-      else r = m - 1;
-    }
+  __shared__ int s_next;
+  const int xcd = (int)blockIdx.x % kNumXcd;
+  const int base = qu.off[xcd], count = qu.off[xcd + 1] - base;
+  for (;;) {
+    __syncthreads();  // every lane has read the previous s_next
+    if (threadIdx.x == 0) s_next = atomicAdd(&qu.ctr[xcd], 1);
+    __syncthreads();
+    const int q = s_next;
+    if (q >= count) break;
+    const int l = qu.patch[base + q];
     const PatchDesc d = desc[l];
     StageArgs<T> a;
     a.out = make_field<T>(out_pool + d.offset, d.gb);
@@ -128,11 +140,16 @@ __global__ void __launch_bounds__(TX* TY)
     a.periodic = 0;
     a.use_phi0 = 0;
     int bx, by, bz;
-    tile_coords(d.tg, (int)(t - prefix[l]), bx, by, bz);
+    tile_coords(d.tg, qu.tile[base + q], bx, by, bz);
     if (SYNTH && !tile_is_interior<TX, TY>(d.fb, d.tg, bx, by, bz))
       march_tile<T, TX, TY, STAGE_ADVECT_CONST, (sizeof(T) == 4), true, false>(a, bx, by, bz, tile);
     else
       march_tile<T, TX, TY, STAGE_ADVECT_CONST, (sizeof(T) == 4), false, false>(a, bx, by, bz, tile);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();  // this workgroup's queue reads precede its count
+    if (atomicAdd(&qu.ctr[kNumXcd], 1) == (int)gridDim.x - 1)
+      for (int x = 0; x <= kNumXcd; x++) qu.ctr[x] = 0;
   }
 }
 
@@ -149,9 +166,7 @@ struct lsm3d_amd_patch_plan {
   int device = 0;
   std::vector<lsm::PatchDesc> host_desc;
   lsm::PatchDesc* desc = nullptr;
-  int64_t* prefix = nullptr;  // tiles before each patch
-  int* sched_off = nullptr;   // grid + 1 offsets into sched_tile
-  int* sched_tile = nullptr;  // tile ids, workgroup by workgroup
+  int* queue = nullptr;  // device: off[9], ctr[9], patch[ntiles], tile[ntiles]
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
@@ -173,14 +188,17 @@ void launch_patch_t(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const 
   // 12 * tiny * dx^2 per axis (the WENO5 epsilon floor in undivided units)
   const R tx2 = R(12) * Tol<R>::tiny / (ix * ix), ty2 = R(12) * Tol<R>::tiny / (iy * iy);
   const R tz2 = R(12) * Tol<R>::tiny / (iz * iz);
+  PatchQueue qu;
+  qu.off = p->queue;
+  qu.ctr = p->queue + (kNumXcd + 1);
+  qu.patch = p->queue + 2 * (kNumXcd + 1);
+  qu.tile = qu.patch + p->ntiles;
   if (g.any)
-    hipLaunchKernelGGL((k_patch_stage<R, TX, TY, true>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
-                       p->prefix, p->npatch, p->sched_off, p->sched_tile, out, ps, pc, g, ix, iy, iz, dt, zt, tx2,
-                       ty2, tz2, form);
+    hipLaunchKernelGGL((k_patch_stage<R, TX, TY, true>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc, qu,
+                       out, ps, pc, g, ix, iy, iz, dt, zt, tx2, ty2, tz2, form);
   else
     hipLaunchKernelGGL((k_patch_stage<R, TX, TY, false>), dim3((unsigned)p->grid), dim3(TX * TY), 0, s, p->desc,
-                       p->prefix, p->npatch, p->sched_off, p->sched_tile, out, ps, pc, g, ix, iy, iz, dt, zt, tx2,
-                       ty2, tz2, form);
+                       qu, out, ps, pc, g, ix, iy, iz, dt, zt, tx2, ty2, tz2, form);
 }
 
 // One stage over every patch.
@@ -194,7 +212,9 @@ void launch_patch(hipStream_t s, const lsm3d_amd_patch_plan* p, R* out, const R*
     case 3: launch_patch_t<128, 2>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
     case 4: launch_patch_t<128, 4>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
     case 5: launch_patch_t<32, 16>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
-    default: launch_patch_t<16, 16>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    case 6: launch_patch_t<16, 16>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    case 7: launch_patch_t<16, 8>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
+    default: launch_patch_t<16, 4>(s, p, out, ps, pc, g, ix, iy, iz, dt, form); break;
   }
 }
 
@@ -208,7 +228,9 @@ int occupancy_blocks(int cfg) {
     case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 2, true>, 256, 0); break;
     case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 128, 4, true>, 512, 0); break;
     case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 32, 16, true>, 512, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 16, 16, true>, 256, 0); break;
+    case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 16, 16, true>, 256, 0); break;
+    case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 16, 8, true>, 128, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_patch_stage<R, 16, 4, true>, 64, 0); break;
   }
   if (e != hipSuccess || n < 1) {
     (void)hipGetLastError();
@@ -217,20 +239,21 @@ int occupancy_blocks(int cfg) {
   return n;
 }
 
-// Static schedule of the persistent launch.  The tiles, in patch order, are
-// cut into eight contiguous runs of equal summed cost, one per XCD (workgroup
-// b runs on XCD b % 8), so a patch's tiles, which share halo cells, stay in
-// one L2 except where a run ends; then each XCD's tiles, costliest first, go
-// to the least loaded of its workgroups (longest processing time first).  A tile's cost is its planes plus the six planes
-// its march loads before the first, weighted up when its stencils leave the
-// fill box (ghost values synthesised from the rule).  Workgroups list their
-// tiles in that order, so tiles of similar cost from one patch run side by
-// side.
-void build_schedule(const std::vector<PatchDesc>& desc, const std::vector<int64_t>& prefix, int tx, int ty,
-                    int grid, std::vector<int>& off, std::vector<int>& tiles) {
+// The queue of the persistent launch.  The tiles, in patch order, are cut
+// into eight contiguous runs of equal summed cost, one per XCD, so a patch's
+// tiles, which share halo cells, stay in one L2 except where a run ends; each
+// run is ordered costliest first (longest processing time first), so the
+// dynamic queue ends on the cheapest tiles.  A tile's cost estimate is its
+// planes plus the six planes its march loads before the first, weighted up
+// when its stencils leave the fill box (ghost values synthesised from the
+// rule).  Output: off (kNumXcd + 1), then per entry the patch and the tile
+// number within the patch.
+void build_queue(const std::vector<PatchDesc>& desc, const std::vector<int64_t>& prefix, int tx, int ty,
+                 std::vector<int>& off, std::vector<int>& patch, std::vector<int>& tile) {
   const int npatch = (int)desc.size();
   const int64_t ntiles = prefix[npatch];
   std::vector<double> tcost(ntiles);
+  std::vector<int> owner(ntiles);
   double total = 0.0;
   for (int l = 0; l < npatch; l++) {
     const PatchDesc& d = desc[l];
@@ -243,24 +266,10 @@ void build_schedule(const std::vector<PatchDesc>& desc, const std::vector<int64_
       const bool interior = i0 - 3 >= d.fb.ilo && i0 + tx + 2 <= d.fb.ihi && j0 - 3 >= d.fb.jlo &&
                             j0 + ty + 2 <= d.fb.jhi && k0 - 3 >= d.fb.klo && k1 + 3 <= d.fb.khi;
       tcost[t] = (double)(k1 - k0 + 1 + 6) * (interior ? 1.0 : 1.25);
+      owner[t] = l;
       total += tcost[t];
     }
   }
-  typedef std::pair<double, int> Load;  // (cost so far, bin)
-  auto lpt = [](const std::vector<int>& items, const std::vector<double>& cost, int nbins,
-                std::vector<std::vector<int>>& bins) {
-    std::vector<int> order(items);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
-    for (int k = 0; k < nbins; k++) heap.push(Load(0.0, k));
-    bins.assign(nbins, std::vector<int>());
-    for (int it : order) {
-      Load top = heap.top();
-      heap.pop();
-      bins[top.second].push_back(it);
-      heap.push(Load(top.first + cost[it], top.second));
-    }
-  };
   std::vector<std::vector<int>> xcd_tiles(kNumXcd);
   double run = 0.0;
   for (int64_t t = 0; t < ntiles; t++) {
@@ -268,19 +277,17 @@ void build_schedule(const std::vector<PatchDesc>& desc, const std::vector<int64_
     xcd_tiles[x].push_back((int)t);
     run += tcost[t];
   }
-  const int nloc = grid / kNumXcd;
-  std::vector<std::vector<int>> per_wg(grid);
+  off.assign(kNumXcd + 1, 0);
+  patch.clear();
+  tile.clear();
   for (int x = 0; x < kNumXcd; x++) {
-    const std::vector<int>& xt = xcd_tiles[x];
-    std::vector<std::vector<int>> wg;
-    lpt(xt, tcost, nloc, wg);
-    for (int w = 0; w < nloc; w++) per_wg[w * kNumXcd + x] = wg[w];
-  }
-  off.assign(grid + 1, 0);
-  tiles.clear();
-  for (int b = 0; b < grid; b++) {
-    tiles.insert(tiles.end(), per_wg[b].begin(), per_wg[b].end());
-    off[b + 1] = (int)tiles.size();
+    std::vector<int> order(xcd_tiles[x]);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tcost[a] > tcost[b]; });
+    for (int t : order) {
+      patch.push_back(owner[t]);
+      tile.push_back((int)(t - prefix[owner[t]]));
+    }
+    off[x + 1] = (int)patch.size();
   }
 }
 
@@ -351,29 +358,25 @@ lsm3d_amd_patch_plan* lsm3d_amd_patch_plan_create(int npatch, const int* fbs, co
     if (want < kNumXcd) want = kNumXcd;
     p->grid = (int)want;
     if (p->ntiles > INT32_MAX) raise_error(std::string(fn) + ": too many tiles for one plan");
-    std::vector<int> off, tiles;
-    if (c.ok()) build_schedule(p->host_desc, prefix, p->tx, p->ty, p->grid, off, tiles);
+    std::vector<int> off, patch, tile;
+    if (c.ok()) build_queue(p->host_desc, prefix, p->tx, p->ty, off, patch, tile);
     if (c.ok()) check_hip(hipMalloc(&p->desc, sizeof(PatchDesc) * npatch), "hipMalloc(patch descriptors)");
-    if (c.ok()) check_hip(hipMalloc(&p->prefix, sizeof(int64_t) * (npatch + 1)), "hipMalloc(tile prefix)");
-    if (c.ok()) check_hip(hipMalloc(&p->sched_off, sizeof(int) * off.size()), "hipMalloc(schedule offsets)");
-    if (c.ok()) check_hip(hipMalloc(&p->sched_tile, sizeof(int) * tiles.size()), "hipMalloc(schedule tiles)");
+    const size_t qn = 2 * (kNumXcd + 1) + 2 * (size_t)p->ntiles;
+    if (c.ok()) check_hip(hipMalloc(&p->queue, sizeof(int) * qn), "hipMalloc(tile queue)");
     if (c.ok()) {
+      std::vector<int> q(qn, 0);  // off, zeroed counters, patches, tiles
+      std::copy(off.begin(), off.end(), q.begin());
+      std::copy(patch.begin(), patch.end(), q.begin() + 2 * (kNumXcd + 1));
+      std::copy(tile.begin(), tile.end(), q.begin() + 2 * (kNumXcd + 1) + p->ntiles);
       check_hip(hipMemcpy(p->desc, p->host_desc.data(), sizeof(PatchDesc) * npatch, hipMemcpyHostToDevice),
                 "hipMemcpy(patch descriptors)");
-      check_hip(hipMemcpy(p->prefix, prefix.data(), sizeof(int64_t) * (npatch + 1), hipMemcpyHostToDevice),
-                "hipMemcpy(tile prefix)");
-      check_hip(hipMemcpy(p->sched_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice),
-                "hipMemcpy(schedule offsets)");
-      check_hip(hipMemcpy(p->sched_tile, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice),
-                "hipMemcpy(schedule tiles)");
+      check_hip(hipMemcpy(p->queue, q.data(), sizeof(int) * qn, hipMemcpyHostToDevice), "hipMemcpy(tile queue)");
     }
   }
   c.finish();
   if (!c.ok()) {
     if (p->desc) (void)hipFree(p->desc);
-    if (p->prefix) (void)hipFree(p->prefix);
-    if (p->sched_off) (void)hipFree(p->sched_off);
-    if (p->sched_tile) (void)hipFree(p->sched_tile);
+    if (p->queue) (void)hipFree(p->queue);
     delete p;
     return nullptr;
   }
@@ -384,9 +387,7 @@ void lsm3d_amd_patch_plan_destroy(lsm3d_amd_patch_plan* p) {
   if (!p) return;
   free_graph(p);
   if (p->desc) (void)hipFree(p->desc);
-  if (p->prefix) (void)hipFree(p->prefix);
-  if (p->sched_off) (void)hipFree(p->sched_off);
-  if (p->sched_tile) (void)hipFree(p->sched_tile);
+  if (p->queue) (void)hipFree(p->queue);
   delete p;
 }
 

@@ -360,7 +360,9 @@ void launch_medium_cfg(hipStream_t s, int cfg, int kind, const MediumArgs<R>& a,
     case 3: launch_medium<128, 2>(s, kind, a, nblk); break;
     case 4: launch_medium<128, 4>(s, kind, a, nblk); break;
     case 5: launch_medium<32, 16>(s, kind, a, nblk); break;
-    default: launch_medium<16, 16>(s, kind, a, nblk); break;
+    case 6: launch_medium<16, 16>(s, kind, a, nblk); break;
+    case 7: launch_medium<16, 8>(s, kind, a, nblk); break;
+    default: launch_medium<16, 4>(s, kind, a, nblk); break;
   }
 }
 

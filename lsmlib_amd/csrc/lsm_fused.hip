@@ -92,7 +92,7 @@ namespace lsm {
 
 // Tile configurations compiled in (TX x TY threads).  The autotuner picks
 // one per grid-size class through lsm3d_amd_set_stage_tiling().
-constexpr TileShape kTiles[] = {{64, 4}, {64, 8}, {32, 8}, {128, 2}, {128, 4}, {32, 16}, {16, 16}};
+constexpr TileShape kTiles[] = {{64, 4}, {64, 8}, {32, 8}, {128, 2}, {128, 4}, {32, 16}, {16, 16}, {16, 8}, {16, 4}};
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 thread_local Tiling tl_tiling = {0, 32};
@@ -170,7 +170,9 @@ void launch_stage_kind(hipStream_t s, const StageArgs<R>& a, int cfg, int nblk) 
     case 3: launch_stage_t<128, 2, KIND>(s, a, nblk); break;
     case 4: launch_stage_t<128, 4, KIND>(s, a, nblk); break;
     case 5: launch_stage_t<32, 16, KIND>(s, a, nblk); break;
-    default: launch_stage_t<16, 16, KIND>(s, a, nblk); break;
+    case 6: launch_stage_t<16, 16, KIND>(s, a, nblk); break;
+    case 7: launch_stage_t<16, 8, KIND>(s, a, nblk); break;
+    default: launch_stage_t<16, 4, KIND>(s, a, nblk); break;
   }
 }
 

@@ -177,8 +177,10 @@ __device__ __forceinline__ T upwind_weno5(const T* p, T vel, T inv, T tiny_dx2, 
 template <typename T, bool FASTW>
 __device__ __forceinline__ void hj_weno5_pm_d(const T* d, T inv, T tiny_dx2, T& plus, T& minus) {
   if (FASTW) {
-    plus = weno5_fast<T>(d[5], d[4], d[3], d[2], d[1], tiny_dx2) * inv;
-    minus = weno5_fast<T>(d[0], d[1], d[2], d[3], d[4], tiny_dx2) * inv;
+    T p, m;
+    weno5_fast_pm<T>(d, tiny_dx2, p, m);
+    plus = p * inv;
+    minus = m * inv;
   } else {
     plus = weno5_ref<T>(d[5] * inv, d[4] * inv, d[3] * inv, d[2] * inv, d[1] * inv);
     minus = weno5_ref<T>(d[0] * inv, d[1] * inv, d[2] * inv, d[3] * inv, d[4] * inv);
@@ -497,9 +499,17 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
           const T cur = a.use_phi0 ? cp0 : us;
           rhs = T(0);
           if (fabs(cur) >= a.zero_tol) {
-            const T n2 = godunov_star_sq<T>(cur, xp, yp, zp, xm, ym, zm);
-            const T sgn = a.use_phi0 ? cur / sqrt(cur * cur + a.dx_sq) : cur / sqrt(cur * cur + n2 * a.dx_sq);
-            rhs = sgn * (T(1) - sqrt(n2));
+            const T n2 = godunov_star_sq_max3<T>(cur, xp, yp, zp, xm, ym, zm);
+            if constexpr (FASTW) {
+              // FP32: the smoothed sign through the hardware reciprocal square root
+              // and |grad phi|* through the hardware square root (about 1 ulp each)
+              const T sgn = a.use_phi0 ? cur * __builtin_amdgcn_rsqf(fma(cur, cur, a.dx_sq))
+                                       : cur * __builtin_amdgcn_rsqf(fma(n2, a.dx_sq, cur * cur));
+              rhs = sgn * (T(1) - __builtin_amdgcn_sqrtf(n2));
+            } else {
+              const T sgn = a.use_phi0 ? cur / sqrt(cur * cur + a.dx_sq) : cur / sqrt(cur * cur + n2 * a.dx_sq);
+              rhs = sgn * (T(1) - sqrt(n2));
+            }
           }
         } else if (KIND == STAGE_ADVECT_LF) {
           // Lax-Friedrichs Hamiltonian on the plus/minus WENO5 derivatives (lf_axis)

@@ -114,6 +114,73 @@ __device__ __forceinline__ T weno5_fast(T D1, T D2, T D3, T D4, T D5, T tiny_dx2
   return fma(x, rden, fma(fma(T(2), c, b), T(-1.0 / 6.0), D3));
 }
 
+// Plus and minus WENO5 (weno5_fast) of one axis from the six first
+// differences d[0..5] around the cell, sharing what the two stencils have in
+// common: with e_q = d[q+1] - d[q] the minus stencil's second differences
+// (a, b, c, d) are (-e0, -e1, -e2, -e3) and the plus stencil's
+// (e4, e3, e2, e1); c^2 = e2^2 and 2.5 c^2 are the same in both, and max|D|
+// shares max(|d1|..|d4|).  Negation and the maximum are exact, so both
+// results equal weno5_fast(d0..d4) and weno5_fast(d5..d1) bit for bit.
+// Returns the two undivided derivatives (multiply by 1/dx).
+template <typename T>
+__device__ __forceinline__ void weno5_fast_pm(const T* d, T tiny_dx2, T& plus, T& minus) {
+#pragma clang fp contract(off)
+  const T e0 = d[1] - d[0], e1 = d[2] - d[1], e2 = d[3] - d[2], e3 = d[4] - d[3], e4 = d[5] - d[4];
+  T mid, mm, mp;
+  if constexpr (sizeof(T) == 4) {
+    asm("v_max3_f32 %0, |%1|, |%2|, |%3|" : "=v"(mid) : "v"(d[1]), "v"(d[2]), "v"(d[3]));
+    asm("v_max3_f32 %0, %1, |%2|, |%3|" : "=v"(mm) : "v"(mid), "v"(d[0]), "v"(d[4]));
+    asm("v_max3_f32 %0, %1, |%2|, |%3|" : "=v"(mp) : "v"(mid), "v"(d[4]), "v"(d[5]));
+  } else {
+    mid = fmax(fmax(fabs(d[1]), fabs(d[2])), fabs(d[3]));
+    mm = fmax(fmax(mid, fabs(d[0])), fabs(d[4]));
+    mp = fmax(fmax(mid, fabs(d[4])), fabs(d[5]));
+  }
+  const T tf = tiny_dx2 * T(1.0 / 16.0);
+  const T iem = fast_rcp(fma(T(0.75e-6), mm * mm, tf)), iep = fast_rcp(fma(T(0.75e-6), mp * mp, tf));
+  const T cc = e2 * e2, cc25 = T(2.5) * cc;
+  // minus: (a, b, c, d) = -(e0, e1, e2, e3)
+  {
+    const T q1 = fma(e0, fma(T(-2.75), e1, e0), T(2.5) * (e1 * e1));
+    const T q2 = fma(e1, fma(T(-1.25), e2, e1), cc);
+    const T q3 = fma(e3, fma(T(-2.75), e2, e3), cc25);
+    const T s1 = fma(q1, iem, T(1)), s2 = fma(q2, iem, T(1)), s3 = fma(q3, iem, T(1));
+    const T t1 = s1 * s1, t2 = s2 * s2, t3 = s3 * s3;
+    const T p23 = t2 * t3, p13 = t1 * t3, p12 = t1 * t2;
+    const T rden = fast_rcp(fma(T(3), p12, fma(T(6), p13, p23)));
+    // a - 2b + c = -(e0 - 2 e1 + e2), b - 2c + d = -(e1 - 2 e2 + e3), b + 2c = -(e1 + 2 e2)
+    const T x = fma(p23 * -(fma(T(-2), e1, e0) + e2), T(1.0 / 3.0), (p12 * -(fma(T(-2), e2, e1) + e3)) * T(0.5));
+    minus = fma(x, rden, fma(-fma(T(2), e2, e1), T(-1.0 / 6.0), d[2]));
+  }
+  // plus: (a, b, c, d) = (e4, e3, e2, e1)
+  {
+    const T q1 = fma(e4, fma(T(-2.75), e3, e4), T(2.5) * (e3 * e3));
+    const T q2 = fma(e3, fma(T(-1.25), e2, e3), cc);
+    const T q3 = fma(e1, fma(T(-2.75), e2, e1), cc25);
+    const T s1 = fma(q1, iep, T(1)), s2 = fma(q2, iep, T(1)), s3 = fma(q3, iep, T(1));
+    const T t1 = s1 * s1, t2 = s2 * s2, t3 = s3 * s3;
+    const T p23 = t2 * t3, p13 = t1 * t3, p12 = t1 * t2;
+    const T rden = fast_rcp(fma(T(3), p12, fma(T(6), p13, p23)));
+    const T x = fma(p23 * (fma(T(-2), e3, e4) + e2), T(1.0 / 3.0), (p12 * (fma(T(-2), e2, e3) + e1)) * T(0.5));
+    plus = fma(x, rden, fma(fma(T(2), e2, e3), T(-1.0 / 6.0), d[3]));
+  }
+}
+
+// Godunov |grad phi|*^2 of the reinitialization equation, as godunov_star_sq:
+// per axis max(-p, m, 0) (s > 0) or max(p, -m, 0) (otherwise), which is the
+// reference's max(max(-p, 0), max(m, 0)) exactly (a maximum rounds nothing),
+// in one three-operand maximum per axis.
+template <typename T>
+__device__ __forceinline__ T godunov_star_sq_max3(T s, T xp, T yp, T zp, T xm, T ym, T zm) {
+#pragma clang fp contract(off)
+  const T z = T(0);
+  const bool pos = s > z;
+  const T gx = pos ? fmax(fmax(-xp, xm), z) : fmax(fmax(xp, -xm), z);
+  const T gy = pos ? fmax(fmax(-yp, ym), z) : fmax(fmax(yp, -ym), z);
+  const T gz = pos ? fmax(fmax(-zp, zm), z) : fmax(fmax(zp, -zm), z);
+  return gx * gx + gy * gy + gz * gz;
+}
+
 // d1[q] = D1(c+q-2), q = 0..5 (only those the scheme needs are formed).
 template <typename T, int S>
 __device__ __forceinline__ void first_differences(const T* p, T* d1) {

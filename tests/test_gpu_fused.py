@@ -219,6 +219,7 @@ def test_every_tiling_gives_identical_results(real):
                 s1, s2 = st.empty(), st.empty()
                 st.advect_rk3_step(p, s1, s2, vel, 0.01)
                 st.reinit_rk3_iteration(p, s1, s2, 0.005)
+                st.advect_lf_rk3_step(p, s1, s2, vel, 0.01)
                 torch.cuda.synchronize()
                 out = _np(p)
                 assert np.isfinite(out).all(), f"tiling {cfg}/{kz} left non-finite values"

@@ -1,0 +1,36 @@
+"""FP32 against FP64 on the sphere-union / Taylor-Green workload, with kinks
+(64 spheres) and interleaved reinitialization.
+
+The north-star bar for the FP32 WENO5 / TVD-RK3 step is a near-interface
+(|phi| < 3 dx) error of at most 1e-5 dx against FP64.  A single FP32 step
+meets it with a wide margin, and is asserted here.  Over many steps the
+FP32 error grows, and lsmlib_amd.precision.precision_split shows why: the
+same FP64 arithmetic with every stored stage output rounded to FP32 (FP32
+storage alone) already exceeds 1e-5 dx after a few tens of steps at the
+sphere-union kinks, and with reinitialization even half an FP32 ulp of
+input perturbation, carried in FP64, grows past it.  So the accumulated
+bound is not reachable by any FP32-storage scheme; what is asserted over the
+run is that the FP32 path's error stays within a small factor of that
+FP32-storage floor, i.e. the FP32 arithmetic itself adds little.
+"""
+import pytest
+
+from lsmlib_amd.precision import precision_split
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fp32_error_split_on_the_sphere_union_with_reinitialization():
+    r = precision_split(n=256, steps=40, reinit_every=10, reinit_iters=5)
+    one = r["one_step"]["max_over_dx"]
+    fp32 = r["fp32_same_inputs"]["max_over_dx"]
+    storage = r["storage_only"]["max_over_dx"]
+    inputs = r["inputs_only"]["max_over_dx"]
+    # the FP32 step: within the 1e-5 dx bar by more than an order of magnitude
+    assert one <= 1e-6, r
+    # over the run: FP32 storage rounding dominates the input rounding ...
+    assert storage > inputs, r
+    # ... and the FP32 arithmetic adds at most a factor 3 on top of it
+    assert fp32 <= 3.0 * storage, r
+    # the error against the FP64 reference stays small in absolute terms
+    assert r["fp32_vs_fp64"]["max_over_dx"] <= 2e-4, r
