@@ -128,24 +128,27 @@ void lsm3dlaplacianorder2_(LSMLIB_REAL *laplacian_phi, LSM_AMD_BOX(laplacian_phi
                            const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb), LSM_AMD_BOX(fb),
                            const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz);
 /* LSM3D_GRADIENT_MAGNITUDE, lsm_spatial_derivatives3d.h:939 */
+/* qualifiers as the reference header declares them */
 void lsm3dgradientmagnitude_(const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
-                             const LSMLIB_REAL *phi_z, LSMLIB_REAL *grad_phi_mag,
+                             const LSMLIB_REAL *phi_z, const LSMLIB_REAL *grad_phi_mag,
                              LSM_AMD_BOX(grad_phi_gb), LSM_AMD_BOX(fb));
 
 /* LSM3D_PHI_UPWIND_GRAD_F, lsm_spatial_derivatives3d.h:831 (phi read at fb +- 1;
  * where phi == 0 the outputs are left unchanged, as in the reference) */
+/* qualifiers as the reference header declares them */
 void lsm3dphiupwindgradf_(LSMLIB_REAL *F_x, LSMLIB_REAL *F_y, LSMLIB_REAL *F_z, LSM_AMD_BOX(grad_F_gb),
-                          const LSMLIB_REAL *F_x_plus, const LSMLIB_REAL *F_y_plus,
-                          const LSMLIB_REAL *F_z_plus, LSM_AMD_BOX(grad_F_plus_gb),
-                          const LSMLIB_REAL *F_x_minus, const LSMLIB_REAL *F_y_minus,
-                          const LSMLIB_REAL *F_z_minus, LSM_AMD_BOX(grad_F_minus_gb),
+                          LSMLIB_REAL *F_x_plus, LSMLIB_REAL *F_y_plus,
+                          LSMLIB_REAL *F_z_plus, LSM_AMD_BOX(grad_F_plus_gb),
+                          LSMLIB_REAL *F_x_minus, LSMLIB_REAL *F_y_minus,
+                          LSMLIB_REAL *F_z_minus, LSM_AMD_BOX(grad_F_minus_gb),
                           const LSMLIB_REAL *phi, LSM_AMD_BOX(phi_gb), LSM_AMD_BOX(fb));
 /* LSM3D_AVERAGE_GRAD_PHI, lsm_spatial_derivatives3d.h:893 */
+/* qualifiers as the reference header declares them */
 void lsm3daveragegradphi_(LSMLIB_REAL *phi_x, LSMLIB_REAL *phi_y, LSMLIB_REAL *phi_z,
-                          LSM_AMD_BOX(grad_phi_gb), const LSMLIB_REAL *phi_x_plus,
-                          const LSMLIB_REAL *phi_y_plus, const LSMLIB_REAL *phi_z_plus,
-                          LSM_AMD_BOX(grad_phi_plus_gb), const LSMLIB_REAL *phi_x_minus,
-                          const LSMLIB_REAL *phi_y_minus, const LSMLIB_REAL *phi_z_minus,
+                          LSM_AMD_BOX(grad_phi_gb), LSMLIB_REAL *phi_x_plus,
+                          LSMLIB_REAL *phi_y_plus, LSMLIB_REAL *phi_z_plus,
+                          LSM_AMD_BOX(grad_phi_plus_gb), LSMLIB_REAL *phi_x_minus,
+                          LSMLIB_REAL *phi_y_minus, LSMLIB_REAL *phi_z_minus,
                           LSM_AMD_BOX(grad_phi_minus_gb), LSM_AMD_BOX(fb));
 
 /* --- level set equation RHS: src/toolbox/level_set_evolution/lsm_level_set_evolution3d.h --- */
@@ -182,7 +185,8 @@ void lsm3daddconstnormalveltermtolserhs_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_r
 /* LSM3D_ADD_CONST_CURV_TERM_TO_LSE_RHS, lsm_level_set_evolution3d.h:277:
  * lse_rhs += b * kappa |grad phi| from central first derivatives (grad_phi_gb)
  * and second derivatives (grad2_phi_gb) */
-void lsm3daddconstcurvtermtolserhs_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb),
+/* qualifiers as the reference header declares them */
+void lsm3daddconstcurvtermtolserhs_(const LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_rhs_gb),
                                     const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
                                     const LSMLIB_REAL *phi_z, LSM_AMD_BOX(grad_phi_gb),
                                     const LSMLIB_REAL *phi_xx, const LSMLIB_REAL *phi_xy,
@@ -477,7 +481,8 @@ void lsm3dsurfaceareazerolevelsetdeltacontrolvolume_(
  * ghost cell around the fill box (the routine reads its six neighbours).  The
  * z terms use norm_phi_x and dx as the reference computes them; norm_phi_z and
  * dz are accepted and unused. */
-void lsm3ddeltafunctionorder1_(const LSMLIB_REAL *phi, LSMLIB_REAL *delta, LSM_AMD_BOX(gb),
+/* qualifiers as the reference header declares them */
+void lsm3ddeltafunctionorder1_(const LSMLIB_REAL *phi, const LSMLIB_REAL *delta, LSM_AMD_BOX(gb),
                                const LSMLIB_REAL *norm_phi_x, const LSMLIB_REAL *norm_phi_y,
                                const LSMLIB_REAL *norm_phi_z, LSM_AMD_BOX(grad_phi_gb),
                                LSM_AMD_BOX(fb), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
@@ -527,8 +532,9 @@ void lsm3dcomputegaussiancurvatureorder4_(LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_
  * kappa and grad_phi_mag must cover those cells.  The reference header
  * declares lse_rhs const and kappa non-const; the routine writes lse_rhs
  * and only reads kappa. */
-void lsm3daddconstprecomputedcurvtermtolserhs_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(rhs_gb),
-                                               const LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb),
+/* qualifiers as the reference header declares them */
+void lsm3daddconstprecomputedcurvtermtolserhs_(const LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(rhs_gb),
+                                               LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb),
                                                const LSMLIB_REAL *grad_phi_mag,
                                                LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *b,
                                                LSM_AMD_BOX(rhs_fb));
@@ -604,21 +610,24 @@ void lsm3dmultiplycutofflserhslocal_(const LSMLIB_REAL *phi, LSMLIB_REAL *lse_rh
                                      const unsigned char *mark_fb, const LSMLIB_REAL *beta,
                                      const LSMLIB_REAL *gamma);
 /* LSM3D_CHECK_OUTER_NARROW_BAND_LAYER, lsm_localization3d.h:270 */
+/* qualifiers as the reference header declares them */
 void lsm3dcheckouternarrowbandlayer_(int *change_sign, const LSMLIB_REAL *phi, LSM_AMD_BOX(gb),
-                                     const int *index_x, const int *index_y, const int *index_z,
-                                     const int *nlo_index, const int *nhi_index, const int *index_outer,
+                                     int *index_x, int *index_y, int *index_z,
+                                     int *nlo_index, int *nhi_index, const int *index_outer,
                                      const int *nlo_index_outer, const int *nhi_index_outer,
                                      const int *nlo_index_outer_plus,
                                      const int *nhi_index_outer_plus,
                                      const int *nlo_index_outer_minus,
                                      const int *nhi_index_outer_minus);
 /* LSM3D_IMPOSE_MASK_LOCAL, lsm_localization3d.h:307 */
-void lsm3dimposemasklocal_(LSMLIB_REAL *dest, LSM_AMD_BOX(dest_gb), const LSMLIB_REAL *src,
-                           LSM_AMD_BOX(src_gb), const LSMLIB_REAL *mask, LSM_AMD_BOX(mask_gb),
+/* qualifiers as the reference header declares them */
+void lsm3dimposemasklocal_(LSMLIB_REAL *dest, LSM_AMD_BOX(dest_gb), LSMLIB_REAL *src,
+                           LSM_AMD_BOX(src_gb), LSMLIB_REAL *mask, LSM_AMD_BOX(mask_gb),
                            const int *index_x, const int *index_y, const int *index_z,
                            const int *nlo_index, const int *nhi_index);
 /* LSM3D_COPY_DATA_LOCAL, lsm_localization3d.h:347 */
-void lsm3dcopydatalocal_(LSMLIB_REAL *dest, LSM_AMD_BOX(dest_gb), const LSMLIB_REAL *src,
+/* qualifiers as the reference header declares them */
+void lsm3dcopydatalocal_(const LSMLIB_REAL *dest, LSM_AMD_BOX(dest_gb), const LSMLIB_REAL *src,
                          LSM_AMD_BOX(src_gb), const int *index_x, const int *index_y,
                          const int *index_z, const int *nlo_index, const int *nhi_index);
 
@@ -677,9 +686,10 @@ void lsm3dcomputeavegradphilocal_(LSMLIB_REAL *grad_phi_ave, const LSMLIB_REAL *
                                   const int *index_y, const int *index_z, const int *nlo_index,
                                   const int *nhi_index, const unsigned char *narrow_band,
                                   LSM_AMD_BOX(nb_gb), const unsigned char *mark_fb);
-/* LSM3D_GRADIENT_MAGNITUDE_LOCAL, lsm_spatial_derivatives3d_local.h:438 */
+/* LSM3D_GRADIENT_MAGNITUDE_LOCAL, lsm_spatial_derivatives3d_local.h:438 (the
+ * reference declares the output grad_phi_mag const; kept, and written) */
 void lsm3dgradientmagnitudelocal_(const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y,
-                                  const LSMLIB_REAL *phi_z, LSMLIB_REAL *grad_phi_mag,
+                                  const LSMLIB_REAL *phi_z, const LSMLIB_REAL *grad_phi_mag,
                                   LSM_AMD_BOX(grad_phi_gb), const int *index_x, const int *index_y,
                                   const int *index_z, const int *nlo_index, const int *nhi_index,
                                   const unsigned char *narrow_band, LSM_AMD_BOX(nb_gb),
@@ -794,9 +804,11 @@ void lsm3daddconstcurvtermtolserhslocal_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(lse_r
                                          const int *index_z, const int *nlo_index,
                                          const int *nhi_index, const unsigned char *narrow_band,
                                          LSM_AMD_BOX(nb_gb), const unsigned char *mark_fb);
-/* LSM3D_ADD_CONST_PRECOMPUTED_CURV_TERM_TO_LSE_RHS_LOCAL, lsm_level_set_evolution3d_local.h:374 */
-void lsm3daddconstprecomputedcurvtermtolserhslocal_(LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(rhs_gb),
-                                                    const LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb),
+/* LSM3D_ADD_CONST_PRECOMPUTED_CURV_TERM_TO_LSE_RHS_LOCAL, lsm_level_set_evolution3d_local.h:374
+ * (the reference declares lse_rhs const and kappa non-const; kept, and
+ * lse_rhs is updated) */
+void lsm3daddconstprecomputedcurvtermtolserhslocal_(const LSMLIB_REAL *lse_rhs, LSM_AMD_BOX(rhs_gb),
+                                                    LSMLIB_REAL *kappa, LSM_AMD_BOX(kappa_gb),
                                                     const LSMLIB_REAL *grad_mag_phi,
                                                     LSM_AMD_BOX(phi_gb), const LSMLIB_REAL *b,
                                                     const int *index_x, const int *index_y,

@@ -1325,8 +1325,8 @@ void lsm3dmultiplycutofflserhslocal_(const R* phi, R* rhs, BOXP(g), const int* i
   finish_local(c);
 }
 
-void lsm3dcheckouternarrowbandlayer_(int* change_sign, const R* phi, BOXP(g), const int* ix, const int* iy,
-                                     const int* iz, const int* nlo, const int* nhi, const int* index_outer,
+void lsm3dcheckouternarrowbandlayer_(int* change_sign, const R* phi, BOXP(g), int* ix, int* iy,
+                                     int* iz, int* nlo, int* nhi, const int* index_outer,
                                      const int* nlo_outer, const int* nhi_outer, const int* nlo_plus,
                                      const int* nhi_plus, const int* nlo_minus, const int* nhi_minus) {
   const char* fn = "LSM3D_CHECK_OUTER_NARROW_BAND_LAYER";
@@ -1363,7 +1363,7 @@ void lsm3dcheckouternarrowbandlayer_(int* change_sign, const R* phi, BOXP(g), co
   if (!lsm::error_pending()) *change_sign = ((h & 3) == 3 || ((h >> 2) & 3) == 3) ? 1 : 0;
 }
 
-void lsm3dimposemasklocal_(R* dest, BOXP(d), const R* src, BOXP(s), const R* mask, BOXP(m), const int* ix,
+void lsm3dimposemasklocal_(R* dest, BOXP(d), R* src, BOXP(s), R* mask, BOXP(m), const int* ix,
                            const int* iy, const int* iz, const int* nlo, const int* nhi) {
   Call c("LSM3D_IMPOSE_MASK_LOCAL");
   ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
@@ -1375,8 +1375,9 @@ void lsm3dimposemasklocal_(R* dest, BOXP(d), const R* src, BOXP(s), const R* mas
   finish_local(c);
 }
 
-void lsm3dcopydatalocal_(R* dest, BOXP(d), const R* src, BOXP(s), const int* ix, const int* iy, const int* iz,
-                         const int* nlo, const int* nhi) {
+void lsm3dcopydatalocal_(const R* dest_out, BOXP(d), const R* src, BOXP(s), const int* ix, const int* iy,
+                         const int* iz, const int* nlo, const int* nhi) {
+  R* dest = const_cast<R*>(dest_out);  // output, declared const by the reference
   Call c("LSM3D_COPY_DATA_LOCAL");
   ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
   if (L.ok) {
@@ -1524,9 +1525,13 @@ void lsm3dcomputeavegradphilocal_(R* ave, const R* phi, BOXP(phi), const R* dx, 
   if (!lsm::error_pending()) *ave = (out[1] > 0.0) ? R(out[0] / out[1]) : R(out[0]);
 }
 
-void lsm3dgradientmagnitudelocal_(const R* gx, const R* gy, const R* gz, R* mag, BOXP(g), const int* ix,
+// The reference header declares the output grad_phi_mag const
+// (lsm_spatial_derivatives3d_local.h:442); the prototype keeps that, and the
+// routine writes it as the Fortran does.
+void lsm3dgradientmagnitudelocal_(const R* gx, const R* gy, const R* gz, const R* mag_out, BOXP(g), const int* ix,
                                   const int* iy, const int* iz, const int* nlo, const int* nhi,
                                   const unsigned char* nb, BOXP(nb), const unsigned char* mark_fb) {
+  R* mag = const_cast<R*>(mag_out);
   Call c("LSM3D_GRADIENT_MAGNITUDE_LOCAL");
   ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
   if (L.ok) {
@@ -1654,11 +1659,15 @@ void lsm3daddconstcurvtermtolserhslocal_(R* rhs, BOXP(r), const R* px, const R* 
   finish_local(c);
 }
 
-void lsm3daddconstprecomputedcurvtermtolserhslocal_(R* rhs, BOXP(r), const R* kappa, BOXP(k), const R* gmag,
+// The reference header declares lse_rhs (in/out) const and kappa (in)
+// non-const (lsm_level_set_evolution3d_local.h:375, 382); the prototype keeps
+// those qualifiers, and the routine updates lse_rhs as the Fortran does.
+void lsm3daddconstprecomputedcurvtermtolserhslocal_(const R* rhs_io, BOXP(r), R* kappa, BOXP(k), const R* gmag,
                                                     BOXP(g), const R* b, const int* ix, const int* iy,
                                                     const int* iz, const int* nlo, const int* nhi,
                                                     const unsigned char* nb, BOXP(nb),
                                                     const unsigned char* mark_fb) {
+  R* rhs = const_cast<R*>(rhs_io);
   Call c("LSM3D_ADD_CONST_PRECOMPUTED_CURV_TERM_TO_LSE_RHS_LOCAL");
   ListArgs L = stage_lists(c, ix, iy, iz, *nlo, *nhi);
   if (L.ok) {

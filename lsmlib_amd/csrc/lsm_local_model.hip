@@ -52,7 +52,9 @@ void ghost_fill(Model* m, R* a) {
 // IMPOSE_MASK_LOCAL (src/serial/lsm_macros.h:108-117): entries n_lo[0] .. n_hi[0]-1
 void mask_local(Model* m, R* dest, const R* src) {
   int lo = m->n_lo[0], hi = m->n_hi[0] - 1;
-  lsm3dimposemasklocal_(dest, GB6(m), src, GB6(m), m->mask, GB6(m), m->index_x, m->index_y, m->index_z, &lo, &hi);
+  // the reference prototype takes src and mask non-const; the routine only reads them
+  lsm3dimposemasklocal_(dest, GB6(m), const_cast<R*>(src), GB6(m), const_cast<R*>(m->mask), GB6(m), m->index_x,
+                        m->index_y, m->index_z, &lo, &hi);
 }
 
 void band(Model* m, const R* phi, const Scalars& s) {

@@ -1063,7 +1063,8 @@ void lsm3dlaplacianorder2_(R* lap, BOXP(l), const R* phi, BOXP(phi), BOXP(fb), c
   c.finish();
 }
 
-void lsm3dgradientmagnitude_(const R* gx, const R* gy, const R* gz, R* mag, BOXP(g), BOXP(fb)) {
+void lsm3dgradientmagnitude_(const R* gx, const R* gy, const R* gz, const R* mag_out, BOXP(g), BOXP(fb)) {
+  R* mag = const_cast<R*>(mag_out);  // output, declared const by the reference
   const char* fn = "LSM3D_GRADIENT_MAGNITUDE";
   Call c(fn);
   Box fb = BOX(fb);
@@ -1074,8 +1075,8 @@ void lsm3dgradientmagnitude_(const R* gx, const R* gy, const R* gz, R* mag, BOXP
   c.finish();
 }
 
-void lsm3dphiupwindgradf_(R* fx, R* fy, R* fz, BOXP(g), const R* fxp, const R* fyp, const R* fzp, BOXP(gp),
-                          const R* fxm, const R* fym, const R* fzm, BOXP(gm), const R* phi, BOXP(phi), BOXP(fb)) {
+void lsm3dphiupwindgradf_(R* fx, R* fy, R* fz, BOXP(g), R* fxp, R* fyp, R* fzp, BOXP(gp), R* fxm, R* fym,
+                          R* fzm, BOXP(gm), const R* phi, BOXP(phi), BOXP(fb)) {
   const char* fn = "LSM3D_PHI_UPWIND_GRAD_F";
   Call c(fn);
   Box fb = BOX(fb);
@@ -1091,8 +1092,8 @@ void lsm3dphiupwindgradf_(R* fx, R* fy, R* fz, BOXP(g), const R* fxp, const R* f
   c.finish();
 }
 
-void lsm3daveragegradphi_(R* gx, R* gy, R* gz, BOXP(g), const R* xp, const R* yp, const R* zp, BOXP(gp),
-                          const R* xm, const R* ym, const R* zm, BOXP(gm), BOXP(fb)) {
+void lsm3daveragegradphi_(R* gx, R* gy, R* gz, BOXP(g), R* xp, R* yp, R* zp, BOXP(gp), R* xm, R* ym, R* zm,
+                          BOXP(gm), BOXP(fb)) {
   const char* fn = "LSM3D_AVERAGE_GRAD_PHI";
   Call c(fn);
   Box fb = BOX(fb);
@@ -1290,9 +1291,10 @@ void lsm3daddconstnormalveltermtolserhs_(R* rhs, BOXP(r), const R* xp, const R* 
   c.finish();
 }
 
-void lsm3daddconstcurvtermtolserhs_(R* rhs, BOXP(r), const R* px, const R* py, const R* pz, BOXP(g1),
+void lsm3daddconstcurvtermtolserhs_(const R* rhs_io, BOXP(r), const R* px, const R* py, const R* pz, BOXP(g1),
                                     const R* pxx, const R* pxy, const R* pxz, const R* pyy, const R* pyz,
                                     const R* pzz, BOXP(g2), const R* b, BOXP(fb)) {
+  R* rhs = const_cast<R*>(rhs_io);  // in/out, declared const by the reference
   const char* fn = "LSM3D_ADD_CONST_CURV_TERM_TO_LSE_RHS";
   Call c(fn);
   Box fb = BOX(fb);
@@ -1806,8 +1808,9 @@ void lsm3dsurfaceareazerolevelsetdeltacontrolvolume_(R* out, const R* dphi, BOXP
                       BOX(g), cv, BOX(cv), *cv_sgn, BOX(ib), *dx, *dy, *dz);
 }
 
-void lsm3ddeltafunctionorder1_(const R* phi, R* delta, BOXP(g), const R* nx, const R* ny, const R* nz, BOXP(n),
-                               BOXP(fb), const R* dx, const R* dy, const R* dz) {
+void lsm3ddeltafunctionorder1_(const R* phi, const R* delta_out, BOXP(g), const R* nx, const R* ny, const R* nz,
+                               BOXP(n), BOXP(fb), const R* dx, const R* dy, const R* dz) {
+  R* delta = const_cast<R*>(delta_out);  // output, declared const by the reference
   const char* fn = "LSM3D_DELTA_FUNCTION_ORDER1";
   (void)nz;
   (void)dz;
@@ -1899,8 +1902,9 @@ void lsm3dcomputegaussiancurvatureorder4_(R* kappa, BOXP(k), const R* phi, BOXP(
                            gm, BOX(g), BOX(fb), *dx, *dy, *dz);
 }
 
-void lsm3daddconstprecomputedcurvtermtolserhs_(R* rhs, BOXP(r), const R* kappa, BOXP(k), const R* gm, BOXP(g),
-                                               const R* b, BOXP(fb)) {
+void lsm3daddconstprecomputedcurvtermtolserhs_(const R* rhs_io, BOXP(r), R* kappa, BOXP(k), const R* gm,
+                                               BOXP(g), const R* b, BOXP(fb)) {
+  R* rhs = const_cast<R*>(rhs_io);  // in/out, declared const by the reference
   const char* fn = "LSM3D_ADD_CONST_PRECOMPUTED_CURV_TERM_TO_LSE_RHS";
   Call c(fn);
   const Box gr = BOX(r), fb = BOX(fb);
