@@ -301,7 +301,8 @@ def run_adjoint(n=128, nsteps=40, every=8, reps=3, fd=True):
 
     adjoint_s: wall time of one forward + backward in FP32 (velocity from c,
     40 steps, loss, reverse sweep, chain to c), best of `reps`.  The FP64
-    gradient is the reference for the FP32 one; central differences of the
+    gradient is the reference for the FP32 one (upwind switch 1e-11 in both;
+    fp32_grad_split says where the difference comes from); central differences of the
     FP64 loss check the FP64 gradient along 8 random unit directions in c,
     the 8 coefficients with the largest |dL/dc|, and 2 smooth phi0 bumps."""
     import numpy as np
@@ -325,7 +326,9 @@ def run_adjoint(n=128, nsteps=40, every=8, reps=3, fd=True):
                      f"loss sum (H(phi_T)-H(phi_tgt))^2 eps=1.5dx"}
     grads, losses = {}, {}
     for real, dtype in (("double", torch.float64), ("float", torch.float32)):
-        runner = AdvectionRK3(grid, real, dev, checkpoint_every=every)
+        # the FP32 runner uses the double build's upwind switch (1e-11), so
+        # both gradients see V_a-derivatives on the same cells
+        runner = AdvectionRK3(grid, real, dev, checkpoint_every=every, zero_tol=1e-11 if real == "float" else None)
         with torch.no_grad():
             vt = tuple(v.to(dtype).contiguous() for v in fv.velocity(c0 + delta))
             target, _ = runner.run(phi0.to(dtype), vt, nsteps, dt, keep_checkpoints=False)
@@ -389,6 +392,8 @@ def run_adjoint(n=128, nsteps=40, every=8, reps=3, fd=True):
     out["fp32_grad_rel_err_vs_fp64"] = {
         "c": float((grads["float"][0] - grads["double"][0]).norm() / grads["double"][0].norm()),
         "phi0": float((grads["float"][1] - grads["double"][1]).norm() / grads["double"][1].norm())}
+    from lsmlib_amd.precision import gradient_precision_split
+    out["fp32_grad_split"] = gradient_precision_split(n, nsteps, every, dev)
     out["adjoint_s"] = out["float"]["forward_backward_s"]
     out["dt"] = dt
     return out
@@ -992,6 +997,10 @@ def main():
         adj = run_adjoint(fd=not a.no_fd)
         print(f"# adjoint: fp32 {adj['adjoint_s']:.3f} s fwd+bwd, fp64 {adj['double']['forward_backward_s']:.3f} s, "
               f"fp32 grad err {adj['fp32_grad_rel_err_vs_fp64']}, fd {adj.get('fd_check_fp64', {}).get('max_rel_err')}",
+              file=sys.stderr, flush=True)
+        sp = adj["fp32_grad_split"]
+        print("# fp32 gradient split (rel. err c / phi0): " + ", ".join(
+            f"{k} {v['c']:.2e} / {v['phi0']:.2e}" for k, v in sp.items() if isinstance(v, dict)),
               file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
 

@@ -67,14 +67,20 @@ class AdvectionRK3:
     """Forward and reverse WENO5 / TVD-RK3 advection on one periodic grid.
 
     real: "double" or "float".  The velocity is fixed in time and dt is
-    frozen (a CFL-chosen dt is not differentiable)."""
+    frozen (a CFL-chosen dt is not differentiable).  zero_tol: the upwind
+    switch of the forward stages and of their adjoints (|V_a| < zero_tol: no
+    advection term along a, so no V_a gradient there); None keeps the
+    build's LSMLIB_ZERO_TOL (1e-11 double, 1e-5 float).  A float runner
+    compared with a double one wants 1e-11: with 1e-5 the cells where
+    1e-11 <= |V_a| < 1e-5 contribute to the double V gradient only."""
 
-    def __init__(self, grid, real="double", device="cuda", checkpoint_every=8):
+    def __init__(self, grid, real="double", device="cuda", checkpoint_every=8, zero_tol=None):
         if checkpoint_every < 1:
             raise ValueError("checkpoint_every must be >= 1")
         self.grid = grid
         self.real = real
-        self.stepper = Stepper(grid, real, bc=(BC_PERIODIC,) * 6, device=device, ghosts="images")
+        self.stepper = Stepper(grid, real, bc=(BC_PERIODIC,) * 6, device=device, ghosts="images",
+                               zero_tol=zero_tol)
         self.lib = self.stepper.lib
         self.dtype = self.stepper.dtype
         self.device = self.stepper.device
@@ -114,20 +120,31 @@ class AdvectionRK3:
     def stage_adjoint(self, form, ubar, adj_s, adj_c, accumulate, us, vel, vbar, dt):
         dx, dy, dz = self.grid.dx
         gb = self._gb
+        self.lib.set_zero_tol(self.stepper.zero_tol)  # the forward stages' switch
         self.lib.fused("lsm3d_amd_advect_stage_adjoint", int(form), _ptr(ubar), gb, _ptr(adj_s), gb,
                        _ptr(adj_c), gb, int(accumulate), _ptr(us), gb, _ptr(vel[0]), _ptr(vel[1]),
                        _ptr(vel[2]), gb,
                        _ptr(vbar[0]) if vbar else None, _ptr(vbar[1]) if vbar else None,
                        _ptr(vbar[2]) if vbar else None, gb, _ptr(self.work), self._fb, dx, dy, dz, dt, 7)
 
-    def reverse(self, ckpts, vel, nsteps, dt, phibar_T, want_vel=True):
-        """Adjoints of phi_0 and (optionally) of V given the adjoint of phi_T."""
+    def reverse(self, ckpts, vel, nsteps, dt, phibar_T, want_vel=True, adjoint_runner=None):
+        """Adjoints of phi_0 and (optionally) of V given the adjoint of phi_T.
+
+        adjoint_runner: an AdvectionRK3 of the same grid in another precision
+        whose stage adjoints are applied to this runner's recomputed stage
+        inputs (cast to its precision), for example FP64 adjoints over FP32
+        forward states; the adjoints are then in its precision."""
         st = self.stepper
         st.sync(False)
-        zeros = lambda: torch.zeros(self.grid.shape, dtype=self.dtype, device=self.device)
-        pbar = fold_periodic_ghosts(phibar_T.detach().to(self.dtype).clone().contiguous(), self.grid)
+        ar = self if adjoint_runner is None else adjoint_runner
+        if ar.grid is not self.grid:
+            raise ValueError("adjoint_runner must be built on the same grid")
+        zeros = lambda: torch.zeros(self.grid.shape, dtype=ar.dtype, device=self.device)
+        cast = (lambda t: t) if ar.dtype == self.dtype else (lambda t: t.to(ar.dtype))
+        pbar = fold_periodic_ghosts(phibar_T.detach().to(ar.dtype).clone().contiguous(), self.grid)
         acc, s1b, s2b = zeros(), zeros(), zeros()
         vbar = [zeros(), zeros(), zeros()] if want_vel else None
+        avel = tuple(cast(v).contiguous() for v in vel)
         nseg = (nsteps + self.every - 1) // self.every
         if len(ckpts) != nseg:
             raise ValueError("checkpoints do not match nsteps")
@@ -145,9 +162,9 @@ class AdvectionRK3:
                 if q + 1 < m:
                     st.advect_stage(TVDRK3_STAGE3, phis[q + 1], s2s[q], phis[q], vel, dt)
             for q in reversed(range(m)):
-                self.stage_adjoint(TVDRK3_STAGE3, pbar, s2b, acc, 0, s2s[q], vel, vbar, dt)
-                self.stage_adjoint(TVDRK3_STAGE2, s2b, s1b, acc, _ACC_CUR, s1s[q], vel, vbar, dt)
-                self.stage_adjoint(RK1, s1b, acc, None, _ACC_STAGE, phis[q], vel, vbar, dt)
+                ar.stage_adjoint(TVDRK3_STAGE3, pbar, s2b, acc, 0, cast(s2s[q]), avel, vbar, dt)
+                ar.stage_adjoint(TVDRK3_STAGE2, s2b, s1b, acc, _ACC_CUR, cast(s1s[q]), avel, vbar, dt)
+                ar.stage_adjoint(RK1, s1b, acc, None, _ACC_STAGE, cast(phis[q]), avel, vbar, dt)
                 pbar, acc = acc, pbar
         return pbar, vbar
 

@@ -121,3 +121,79 @@ def precision_split(n=512, steps=25, reinit_every=0, reinit_iters=5, zero_tol=1e
     del runs, ref, vals, start
     torch.cuda.empty_cache()
     return out
+
+
+def _rel(a, b):
+    return float((a - b).norm() / b.norm())
+
+
+def gradient_precision_split(n=128, nsteps=40, every=8, device="cuda"):
+    """Where the FP32 gradient's error against FP64 comes from, on the
+    gradient workload of `workloads.adjoint_case` (n^3 periodic [-1,1]^3,
+    phi0 = |x| - 0.3, 64 x 3 Fourier velocity coefficients c, dt frozen at
+    the CFL-0.5 dt, nsteps WENO5 / TVD-RK3 steps, loss
+    sum (H_eps(phi_T) - H_eps(phi_target))^2 with eps = 1.5 dx and the target
+    advected with c + delta, delta ~ N(0, 0.02^2), seed 9).  Normwise relative
+    errors of dL/dc and dL/dphi0 against the FP64 gradient:
+
+      fp32                      FP32 forward and adjoint, upwind switch 1e-11
+                                (the double build's)
+      fp32_switch_1e-5          the same with the float build's switch 1e-5:
+                                cells with 1e-11 <= |V_a| < 1e-5 then carry no
+                                V_a gradient, which only the FP64 run has
+      fp64_adjoint_fp32_states  FP64 stage adjoints over the FP32 forward
+                                states: what FP32 storage of the forward costs
+      fp64_fp32_inputs          FP64 throughout from phi0, V and dt rounded to
+                                FP32: the problem's own sensitivity
+    """
+    from .adjoint import AdvectionRK3, heaviside_mismatch
+    from .workloads import adjoint_case
+    import numpy as np
+    dev = torch.device(device)
+    grid, phi0, fv, _ = adjoint_case(n, torch.float64, dev)
+    c0 = fv.c0.to(dev)
+    v0 = fv.velocity(c0)
+    dt = stable_advection_dt(tuple(v.contiguous() for v in v0), grid, 0.5, "double")
+    del v0
+    rng = np.random.default_rng(9)
+    delta = torch.tensor(rng.normal(0.0, 0.02, size=tuple(c0.shape)), dtype=torch.float64, device=dev)
+    eps = 1.5 * grid.dx[0]
+    r64 = AdvectionRK3(grid, "double", dev, every)
+
+    def grads(runner, round_inputs=False, adjoint_runner=None):
+        dtype = runner.dtype
+        rnd = (lambda t: t.float().double()) if round_inputs else (lambda t: t)
+        step = float(np.float32(dt)) if round_inputs else dt
+        c = c0.clone().requires_grad_(True)
+        v64 = fv.velocity(c)
+        vel = tuple(rnd(v.detach()).to(dtype).contiguous() for v in v64)
+        with torch.no_grad():
+            vt = tuple(rnd(v).to(dtype).contiguous() for v in fv.velocity(c0 + delta))
+            p0 = rnd(phi0).to(dtype).contiguous()
+            target, _ = runner.run(p0, vt, nsteps, step, keep_checkpoints=False)
+            del vt
+            phi_t, ckpts = runner.run(p0, vel, nsteps, step)
+        pt = phi_t.clone().requires_grad_(True)
+        heaviside_mismatch(pt, target, grid, eps).backward()
+        with torch.no_grad():
+            pbar, vbar = runner.reverse(ckpts, vel, nsteps, step, pt.grad, adjoint_runner=adjoint_runner)
+        torch.autograd.backward(list(v64), [b.double() for b in vbar])  # the chain to c in FP64
+        torch.cuda.synchronize()
+        return c.grad.double(), pbar.double()
+
+    t0 = time.perf_counter()
+    ref = grads(r64)
+    r32 = AdvectionRK3(grid, "float", dev, every, zero_tol=1e-11)
+    runs = {"fp32": lambda: grads(r32),
+            "fp32_switch_1e-5": lambda: grads(AdvectionRK3(grid, "float", dev, every)),
+            "fp64_adjoint_fp32_states": lambda: grads(r32, adjoint_runner=r64),
+            "fp64_fp32_inputs": lambda: grads(r64, round_inputs=True)}
+    out = {}
+    for name, fn in runs.items():
+        gc, gp = fn()
+        out[name] = {"c": _rel(gc, ref[0]), "phi0": _rel(gp, ref[1])}
+    out["workload"] = (f"{n}^3 periodic [-1,1]^3, phi0 = |x| - 0.3, 64 x 3 Fourier modes, {nsteps} WENO5/TVD-RK3 "
+                       f"steps, checkpoint every {every}, Heaviside mismatch loss eps 1.5 dx")
+    out["seconds"] = time.perf_counter() - t0
+    torch.cuda.empty_cache()
+    return out

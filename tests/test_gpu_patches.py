@@ -13,6 +13,7 @@ import torch
 
 from lsmlib_amd.patches import PatchLevel, random_patch_level
 from lsmlib_amd.pylsmlib import BC_SIGNED_LINEAR, Grid
+from lsmlib_amd.workloads import line_elems
 from oracle.oracle import oracle_lib
 from oracle.steps import OracleStepper, fill_ghosts
 
@@ -60,6 +61,27 @@ def test_patch_level_matches_oracle_fp64(align):
     want = oracle_advance(lvl, phi0, dt, 3)
     for q, (a, b) in enumerate(zip(got, want)):
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), f"patch {q}"
+
+
+@pytest.mark.parametrize("real,tiling", [("double", "auto"), ("float", "auto"), ("float", (7, 32))])
+def test_patch_level_bench_distribution_matches_oracle(real, tiling):
+    """16 patches drawn as the 512-patch bench level draws them (extents
+    16..96 per axis, seed 3, dx 1/128): tiles away from every patch face
+    take the stored-ghost march, so these are compared with the oracle too.
+    FP64 1e-12 of max|phi| per patch; FP32 against the FP64 oracle 1e-5."""
+    desc = random_patch_level(npatch=16, lo=16, hi=96, seed=3, dx=1.0 / 128)
+    dtype = torch.float64 if real == "double" else torch.float32
+    lvl = PatchLevel(desc, real, tiling=tiling, align=line_elems(dtype))
+    phi0 = lvl.initial_phi()
+    dt = lvl.stable_dt(0.5)
+    phi, s1, s2 = phi0.clone(), lvl.empty_pool(), lvl.empty_pool()
+    for _ in range(3):
+        lvl.rk3_step(phi, s1, s2, dt)
+    torch.cuda.synchronize()
+    want = oracle_advance(lvl, phi0, dt, 3)
+    tol = 1e-12 if real == "double" else 1e-5
+    for q, (a, b) in enumerate(zip(gpu_interiors(lvl, phi), want)):
+        assert np.abs(a - b).max() <= tol * np.abs(b).max(), f"patch {q} ext {desc['ext'][q]}"
 
 
 def test_patch_level_fp32_vs_fp64_oracle():
