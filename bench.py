@@ -678,36 +678,43 @@ def run_curvature_local(dxs=(0.04, 0.02), cpu=True, cpu_intervals=3, big_dx=1.5 
         p = phi.clone()
         m.box.fill_ghosts(p, masked=False)
         m.reinitialize_box(p, m.gamma + g.dx[0])
-        m.step(p, 0.0, 1.0)
-        torch.cuda.synchronize()
         ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
-        acc = {"band": 0.0, "step_rest": 0.0, "check": 0.0}
-        t0 = time.perf_counter()
-        for _ in range(big_steps):
-            e = [ev() for _ in range(4)]
-            e[0].record()
-            m.band(p)          # the band build alone (step() builds it again, identically)
-            e[1].record()
-            m.step(p, 0.0, 1.0)
-            e[2].record()
-            m.needs_reinit(p, 0.0, 0.0, 0.0)
-            e[3].record()
-            e[3].synchronize()
-            acc["band"] += e[0].elapsed_time(e[1])
-            acc["step_rest"] += e[1].elapsed_time(e[2])
-            acc["check"] += e[2].elapsed_time(e[3])
+        rec = {}
+        # the library's step (band build + fused stages), then the same step
+        # sequenced routine by routine from Python, each after a warm-up step
+        for name, step, check in (("native", m.step_native, m.needs_reinit_native),
+                                  ("composed", m.step, m.needs_reinit)):
+            step(p, 0.0, 1.0)
+            check(p, 0.0, 0.0, 0.0)
+            torch.cuda.synchronize()
+            acc = {"band": 0.0, "step": 0.0, "check": 0.0}
+            for _ in range(big_steps):
+                e = [ev() for _ in range(4)]
+                e[0].record()
+                m.band(p)          # the band build alone (the step builds it again, identically)
+                e[1].record()
+                step(p, 0.0, 1.0)
+                e[2].record()
+                check(p, 0.0, 0.0, 0.0)
+                e[3].record()
+                e[3].synchronize()
+                acc["band"] += e[0].elapsed_time(e[1])
+                acc["step"] += e[1].elapsed_time(e[2])
+                acc["check"] += e[2].elapsed_time(e[3])
+            rec[name] = {k: v / big_steps for k, v in acc.items()}
         torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        ms = {k: v / big_steps for k, v in acc.items()}
         cells = g.n[0] * g.n[1] * g.n[2]
         frac = float(m.n_hi[2] - m.n_lo[0] + 1) / m.n
-        step_ms = ms["step_rest"] + ms["check"]
+        step_ms = rec["native"]["step"] + rec["native"]["check"]
         big[real] = {"grid": list(g.n), "band_points": int(m.n_hi[3] - m.n_lo[0] + 1),
-                     "band_fraction_levels_0_2": frac, "ms_band_build": ms["band"],
-                     "ms_per_step": step_ms, "ms_per_step_wall": 1e3 * el / big_steps - ms["band"],
+                     "band_fraction_levels_0_2": frac, "ms_band_build": rec["native"]["band"],
+                     "ms_per_step": step_ms, "ms_step_only": rec["native"]["step"],
+                     "ms_reinit_check": rec["native"]["check"],
+                     "ms_per_step_composed": rec["composed"]["step"] + rec["composed"]["check"],
                      "cell_steps_per_s": cells / (step_ms * 1e-3), "steps": big_steps}
-        print(f"# curvature local step {real} {g.n}: {step_ms:.3f} ms/step (band build {ms['band']:.3f} ms, "
-              f"fraction {frac:.3f})", file=sys.stderr, flush=True)
+        print(f"# curvature local step {real} {g.n}: {step_ms:.3f} ms/step with the reinitialization check "
+              f"(band build {rec['native']['band']:.3f} ms, fraction {frac:.3f}); routine by routine "
+              f"{big[real]['ms_per_step_composed']:.3f} ms", file=sys.stderr, flush=True)
         del m, p, phi, mask
         torch.cuda.empty_cache()
     out["step_512x512x341"] = big

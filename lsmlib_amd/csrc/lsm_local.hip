@@ -33,6 +33,7 @@
 
 #include "../../include/lsm3d_amd.h"
 #include "lsm_common.h"
+#include "lsm_local_step.h"
 #include "lsm_runtime.h"
 #include "lsm_stencil.h"
 
@@ -873,6 +874,186 @@ __global__ void __launch_bounds__(256)
   atomicOr(bits, (phi.at(i, j, k) > T(0) ? 1 : 2) << shift);
 }
 
+// ---------------------------------------------------------------------------
+// The narrow-band curvature model's stages fused per level-0 point (the
+// routine chain and why the values agree: lsm_local_step.h).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct LocalCurvArgs {
+  const int* ix;
+  const int* iy;
+  const int* iz;
+  int n;  // level-0 entries
+  Field<const nb_t> nb;
+  Field<const T> ps;  // stage input
+  Field<const T> pc;  // u_cur (stage 2)
+  Field<T> rhs;       // stage 1: lse_rhs
+  Field<T> out;       // stage 2: phi_next; RK1: phi_stage1
+  Box gb;             // the arrays' box
+  Box valid;          // gb less two cells: every read of a marked point
+  int mark_fb, mark_d1, mark_d2;
+  int normal, curv;          // the a > 0 and b > 0 branches of the model
+  T vn, avn, b;              // a, |a|, b in the library precision
+  T inv[3];                  // 1/dx: ENO2 (lsm_spatial_derivatives3d_local.f.in:571-601)
+  T half[3];                 // 0.5/dx: central differences (:826-840)
+  T mdx2;                    // max(dx)^2: the CFL H/dx
+  T beta, gamma, c1, c2;     // the cut-off (lsm_localization3d.f:482-562)
+  const T* dt;               // the step's dt on the device
+  double* partials;          // stage 1 with the normal term: per-block max of H/dx
+  int* skipped;
+};
+
+// The stage's right-hand side at a point that passes the fill-box mark; h
+// receives the CFL speed H/dx when the normal term is on.  Returns false when
+// a value the routine chain reads would be left unwritten by it.
+template <typename T>
+__device__ __forceinline__ bool local_curv_rhs(const LocalCurvArgs<T>& a, int i, int j, int k, T& rhs, double& h) {
+#pragma clang fp contract(off)  // every product and sum rounded as the reference's
+  auto P = [&](int di, int dj, int dk) -> T { return a.ps.at(i + di, j + dj, k + dk); };
+  auto NB = [&](int di, int dj, int dk) -> int { return (int)a.nb.at(i + di, j + dj, k + dk); };
+  const T us = P(0, 0, 0);
+  T r = T(0);  // LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS_LOCAL
+  if (a.normal) {
+    T pl[3], mi[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const int ex = (d == 0), ey = (d == 1), ez = (d == 2);
+      // D1 at offsets -1..2 and D2 at -1..1 along the axis (lsm3dComputeDnLOCAL:
+      // D1(c) = phi(c) - phi(c - e), D2(c) = -(D1(c) - D1(c + e)), 1e10 where
+      // narrow_band exceeds the D1 / D2 mark)
+      T d1[4], d2[3];
+#pragma unroll
+      for (int q = -1; q <= 2; q++)
+        d1[q + 1] = (NB(q * ex, q * ey, q * ez) <= a.mark_d1)
+                        ? P(q * ex, q * ey, q * ez) - P((q - 1) * ex, (q - 1) * ey, (q - 1) * ez)
+                        : T(1.e10);
+#pragma unroll
+      for (int q = -1; q <= 1; q++)
+        d2[q + 1] = (NB(q * ex, q * ey, q * ez) <= a.mark_d2) ? -(d1[q + 1] - d1[q + 2]) : T(1.e10);
+      const T hf = T(0.5), iv = a.inv[d];
+      pl[d] = (fabs(d2[1]) < fabs(d2[2])) ? (d1[2] - hf * d2[1]) * iv : (d1[2] - hf * d2[2]) * iv;
+      mi[d] = (fabs(d2[0]) < fabs(d2[1])) ? (d1[1] + hf * d2[0]) * iv : (d1[1] + hf * d2[1]) * iv;
+    }
+    if (fabs(a.vn) >= Tol<T>::zero)  // LSM3D_ADD_CONST_NORMAL_VEL_TERM_TO_LSE_RHS_LOCAL
+      r = r - a.vn * sqrt(godunov_norm_sq<T>(a.vn, pl[0], pl[1], pl[2], mi[0], mi[1], mi[2]));
+    h = (double)h_over_dx<T>(a.avn, pl[0], pl[1], pl[2], mi[0], mi[1], mi[2], a.mdx2, a.inv[0], a.inv[1], a.inv[2]);
+  }
+  if (a.curv) {
+    // first derivatives on levels 0-1 (D1 mark) at the point and its six
+    // neighbours, second derivatives on level 0 (D2 mark) at the point
+    if (NB(0, 0, 0) > a.mark_d2 || NB(1, 0, 0) > a.mark_d1 || NB(-1, 0, 0) > a.mark_d1 ||
+        NB(0, 1, 0) > a.mark_d1 || NB(0, -1, 0) > a.mark_d1 || NB(0, 0, 1) > a.mark_d1 || NB(0, 0, -1) > a.mark_d1)
+      return false;
+    const T hx = a.half[0], hy = a.half[1], hz = a.half[2];
+    const T gx = (P(1, 0, 0) - P(-1, 0, 0)) * hx;
+    const T gy = (P(0, 1, 0) - P(0, -1, 0)) * hy;
+    const T gz = (P(0, 0, 1) - P(0, 0, -1)) * hz;
+    const T gx_ip = (P(2, 0, 0) - us) * hx, gx_im = (us - P(-2, 0, 0)) * hx;
+    const T gy_jp = (P(0, 2, 0) - us) * hy, gy_jm = (us - P(0, -2, 0)) * hy;
+    const T gz_kp = (P(0, 0, 2) - us) * hz, gz_km = (us - P(0, 0, -2)) * hz;
+    const T gy_ip = (P(1, 1, 0) - P(1, -1, 0)) * hy, gy_im = (P(-1, 1, 0) - P(-1, -1, 0)) * hy;
+    const T gz_ip = (P(1, 0, 1) - P(1, 0, -1)) * hz, gz_im = (P(-1, 0, 1) - P(-1, 0, -1)) * hz;
+    const T gz_jp = (P(0, 1, 1) - P(0, 1, -1)) * hz, gz_jm = (P(0, -1, 1) - P(0, -1, -1)) * hz;
+    // the last writer of each second derivative: xx, xy, xz = d/dx of phi_x,
+    // phi_y, phi_z; yy, yz = d/dy of phi_y, phi_z; zz = d/dz of phi_z
+    const T xx = (gx_ip - gx_im) * hx, xy = (gy_ip - gy_im) * hx, xz = (gz_ip - gz_im) * hx;
+    const T yy = (gy_jp - gy_jm) * hy, yz = (gz_jp - gz_jm) * hy, zz = (gz_kp - gz_km) * hz;
+    r = r + a.b * curvature_term<T>(gx, gy, gz, xx, xy, xz, yy, yz, zz, Tol<T>::zero);
+  }
+  const T av = fabs(us);  // LSM3D_MULTIPLY_CUT_OFF_LSE_RHS_LOCAL
+  T cut;
+  if (av <= a.beta) {
+    cut = T(1);
+  } else if (av <= a.gamma) {
+    const T t = av - a.gamma;
+    cut = ((t * t) * (T(2) * av + a.c1)) / a.c2;
+  } else {
+    cut = T(0);
+  }
+  rhs = cut * r;
+  return true;
+}
+
+template <typename T, int STAGE>
+__global__ void __launch_bounds__(256) k_local_curv_stage(LocalCurvArgs<T> a) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  double h = -1.0e308;
+  if (t < a.n) {
+    const int i = a.ix[t], j = a.iy[t], k = a.iz[t];
+    if (!inside(a.gb, i, j, k)) {
+      atomicAdd(a.skipped, 1);
+    } else if ((int)a.nb.at(i, j, k) <= a.mark_fb) {
+      T r;
+      double hh = -1.0e308;
+      if (!inside(a.valid, i, j, k) || !local_curv_rhs<T>(a, i, j, k, r, hh)) {
+        atomicAdd(a.skipped, 1);
+      } else if (STAGE == 1) {
+        a.rhs.at(i, j, k) = r;
+        h = hh;
+      } else {  // LSM3D_TVD_RK2_STAGE2_LOCAL
+        a.out.at(i, j, k) = rk_combine<T>(RK2_STAGE2, a.pc.at(i, j, k), a.ps.at(i, j, k), r, *a.dt);
+      }
+    } else if (STAGE == 1) {
+      a.rhs.at(i, j, k) = T(0);  // zeroed, and no term passes the mark
+    }
+  }
+  if (STAGE == 1 && a.normal) {
+    double c = 0.0;
+    block_reduce2(RED_LIST_MAX, h, c);
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = h;
+  }
+}
+
+// LSM3D_TVD_RK2_STAGE1_LOCAL on level 0 with the device dt.
+template <typename T>
+__global__ void __launch_bounds__(256) k_local_curv_rk1(LocalCurvArgs<T> a) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= a.n) return;
+  const int i = a.ix[t], j = a.iy[t], k = a.iz[t];
+  if (!inside(a.gb, i, j, k)) {
+    atomicAdd(a.skipped, 1);
+    return;
+  }
+  if ((int)a.nb.at(i, j, k) > a.mark_fb) return;
+  const T u = a.ps.at(i, j, k);
+  a.out.at(i, j, k) = rk_combine<T>(RK_EULER, u, u, a.rhs.at(i, j, k), *a.dt);
+}
+
+// The step's dt: the CFL reduction's last step
+// (LSM3D_COMPUTE_STABLE_CONST_NORMAL_VEL_DT_LOCAL: cfl / (max H/dx + tiny) in
+// the library precision), then the model's choice and clamps in double
+// (curvature_model3d_local.c:331-349, 383-394).  dts[0] = dt, dtr[0] = (T) dt.
+struct LocalDtArgs {
+  const double* partials;
+  int nblocks, normal, curv;
+  double a, b, cfl, dx0, dx1, dx2, dt_sub, tplot, dt_min_to_correct, dt_min;
+  double* dts;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_local_curv_dt(LocalDtArgs d, T cfl_r, T* dtr) {
+#pragma clang fp contract(off)
+  double v = -1.0e308, c = 0.0;
+  for (int q = threadIdx.x; q < d.nblocks; q += blockDim.x) v = fmax(v, d.partials[q]);
+  block_reduce2(RED_LIST_MAX, v, c);
+  if (threadIdx.x != 0) return;
+  double dt = d.tplot;
+  if (d.normal) {
+    T mx = T(-1.0);
+    if (v > (double)mx) mx = T(v);
+    dt = (double)(cfl_r / (mx + Tol<T>::tiny));
+  }
+  if (d.curv) {
+    const double corr = (1.0 / (d.dx0 * d.dx0) + 1.0 / (d.dx1 * d.dx1) + 1.0 / (d.dx2 * d.dx2)) * (2.0 * d.b);
+    const double max_h = d.normal ? d.cfl / dt : 0.0;
+    dt = d.cfl / (max_h + corr);
+  }
+  if (dt < d.dt_min_to_correct) dt = d.dt_min;
+  if (d.dt_sub + dt > d.tplot) dt = d.tplot - d.dt_sub;
+  d.dts[0] = dt;
+  dtr[0] = (T)dt;
+}
+
 }  // namespace lsm
 
 // ============================================================================
@@ -1269,6 +1450,132 @@ void hj_eno_local(const char* fn, int order, R* xp, R* yp, R* zp, Box gp, R* xm,
 }
 
 }  // namespace
+
+// ============================================================================
+// The narrow-band curvature model's fused stages (lsm_local_step.h)
+// ============================================================================
+namespace lsm {
+namespace {
+
+constexpr int kLocalDtR = 8;  // element offset (in doubles) of the library-precision dt
+
+// The step's device memory: dt (double) at element 0, the library-precision
+// dt at element kLocalDtR, the per-block CFL maxima from element 16.  Every
+// stage of a step asks for the same size, so the buffer does not move
+// between them.
+double* step_buffer(const LocalCurvStep& s) {
+  const int n = s.hi0 - s.lo0 + 1;
+  const int nblocks = n > 0 ? (n + 255) / 256 : 0;
+  return reinterpret_cast<double*>(device_buffer(BUF_LOCAL_STEP, (size_t)(16 + nblocks) * sizeof(double)));
+}
+
+LocalCurvArgs<R> local_args(Call& c, const LocalCurvStep& s, const R* ps, const R* pc, R* rhs, R* out,
+                            double* buf) {
+  LocalCurvArgs<R> a;
+  const int n = std::max(0, s.hi0 - s.lo0 + 1);
+  a.ix = s.index_x + s.lo0;
+  a.iy = s.index_y + s.lo0;
+  a.iz = s.index_z + s.lo0;
+  a.n = n;
+  a.nb = c.in(reinterpret_cast<const nb_t*>(s.narrow_band), s.gb);
+  a.ps = c.in(ps, s.gb);
+  a.pc = pc ? c.in(pc, s.gb) : none();
+  a.rhs = make_field<R>(rhs, s.gb);
+  a.out = make_field<R>(out, s.gb);
+  a.gb = s.gb;
+  a.valid = shrink(s.gb, 2, 2, 2);
+  a.mark_fb = (int)(nb_t)s.mark_fb;
+  a.mark_d1 = (int)(nb_t)s.mark_d1;
+  a.mark_d2 = (int)(nb_t)s.mark_d2;
+  a.normal = s.a > 0;
+  a.curv = s.b > 0;
+  a.vn = (R)s.a;
+  a.avn = R(fabs(a.vn));
+  a.b = (R)s.b;
+  const R dx = (R)s.dx[0], dy = (R)s.dx[1], dz = (R)s.dx[2];
+  a.inv[0] = R(1.0 / (double)dx);
+  a.inv[1] = R(1.0 / (double)dy);
+  a.inv[2] = R(1.0 / (double)dz);
+  a.half[0] = R(0.5 / (double)dx);
+  a.half[1] = R(0.5 / (double)dy);
+  a.half[2] = R(0.5 / (double)dz);
+  const R mdx = max3(dx, dy, dz);
+  a.mdx2 = mdx * mdx;
+  const R beta = (R)(2.0 * s.dx[0]), gamma = (R)(4.0 * s.dx[0]);  // the model's band widths
+  a.beta = beta;
+  a.gamma = gamma;
+  a.c1 = gamma - R(3) * beta;
+  R c2 = gamma - beta;
+  a.c2 = c2 * c2 * c2;
+  a.dt = reinterpret_cast<const R*>(buf + kLocalDtR);
+  a.partials = buf + 16;
+  a.skipped = skipped_counter();
+  return a;
+}
+
+}  // namespace
+
+void local_curv_stage1(const LocalCurvStep& s, double dt_sub, double tplot) {
+  Call c("lsm3d_amd_local_curvature_step (stage 1)");
+  double* buf = step_buffer(s);
+  if (!buf) raise_error("lsm3d_amd_local_curvature_step: device buffer allocation failed");
+  if (!c.ok()) return;
+  LocalCurvArgs<R> a = local_args(c, s, s.phi, nullptr, s.lse_rhs, s.phi_stage1, buf);
+  const int nblocks = (a.n + 255) / 256;
+  if (c.ok() && a.n > 0) {
+    hipLaunchKernelGGL((k_local_curv_stage<R, 1>), dim3((unsigned)nblocks), dim3(256), 0, c.stream(), a);
+    c.launched("k_local_curv_stage<1>");
+  }
+  if (c.ok()) {
+    LocalDtArgs d{buf + 16, a.normal ? nblocks : 0, a.normal, a.curv, s.a, s.b, s.cfl, s.dx[0], s.dx[1], s.dx[2],
+                  dt_sub, tplot, s.dt_min_to_correct, s.dt_min, buf};
+    hipLaunchKernelGGL(k_local_curv_dt<R>, dim3(1), dim3(256), 0, c.stream(), d, (R)s.cfl,
+                       reinterpret_cast<R*>(buf + kLocalDtR));
+    c.launched("k_local_curv_dt");
+  }
+  finish_local(c);
+}
+
+void local_curv_rk1(const LocalCurvStep& s) {
+  Call c("lsm3d_amd_local_curvature_step (RK2 stage 1)");
+  double* buf = step_buffer(s);
+  if (!buf) raise_error("lsm3d_amd_local_curvature_step: device buffer allocation failed");
+  if (!c.ok()) return;
+  LocalCurvArgs<R> a = local_args(c, s, s.phi, nullptr, s.lse_rhs, s.phi_stage1, buf);
+  if (c.ok() && a.n > 0) {
+    hipLaunchKernelGGL(k_local_curv_rk1<R>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, c.stream(), a);
+    c.launched("k_local_curv_rk1");
+  }
+  finish_local(c);
+}
+
+void local_curv_stage2(const LocalCurvStep& s) {
+  Call c("lsm3d_amd_local_curvature_step (stage 2)");
+  double* buf = step_buffer(s);
+  if (!buf) raise_error("lsm3d_amd_local_curvature_step: device buffer allocation failed");
+  if (!c.ok()) return;
+  LocalCurvArgs<R> a = local_args(c, s, s.phi_stage1, s.phi, s.lse_rhs, s.phi_next, buf);
+  if (c.ok() && a.n > 0) {
+    hipLaunchKernelGGL((k_local_curv_stage<R, 2>), dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0,
+                       c.stream(), a);
+    c.launched("k_local_curv_stage<2>");
+  }
+  finish_local(c);
+}
+
+double local_curv_dt(const LocalCurvStep& s) {
+  Call c("lsm3d_amd_local_curvature_step (dt)");
+  double* buf = step_buffer(s);
+  double dt = -1.0;
+  if (buf && c.ok()) {
+    check_hip(hipMemcpyAsync(&dt, buf, sizeof(double), hipMemcpyDeviceToHost, c.stream()), "hipMemcpyAsync(dt)");
+    check_hip(hipStreamSynchronize(c.stream()), "hipStreamSynchronize");
+  }
+  finish_local(c);
+  return error_pending() ? -1.0 : dt;
+}
+
+}  // namespace lsm
 
 // ============================================================================
 // Entry points (extern "C", reference prototypes; see include/lsm3d_amd.h)

@@ -13,6 +13,7 @@
 
 #include "../../include/lsm3d_amd.h"
 #include "lsm_common.h"
+#include "lsm_local_step.h"
 #include "lsm_runtime.h"
 
 namespace {
@@ -84,70 +85,31 @@ void eno2(Model* m, const R* phi, const Scalars& s) {
                     &m->n_lo[2], &m->n_hi[2], m->narrow_band, GB6(m), &kMarkFb, &kMarkD1, &kMarkD2);
 }
 
-// lse_rhs of one stage (curvature_model3d_local.c:231-404 / :431-565), then the
-// cut-off; returns the hyperbolic dt in stage 1 (when a > 0).
-double stage_rhs(Model* m, const R* phi, int stage, const Scalars& s) {
-  double dt = -1.0;
-  int* lo = m->n_lo;
-  int* hi = m->n_hi;
-  lsm3dzerooutlevelseteqnrhslocal_(m->lse_rhs, GB6(m), m->index_x, m->index_y, m->index_z, &lo[0],
-                                   stage == 1 ? &hi[0] : &hi[2]);
-  FAIL_IF_ERROR(dt);
-  if (m->a > 0) {
-    eno2(m, phi, s);
-    FAIL_IF_ERROR(dt);
-    lsm3daddconstnormalveltermtolserhslocal_(m->lse_rhs, GB6(m), m->phi_x_plus, m->phi_y_plus, m->phi_z_plus,
-                                             GB6(m), m->phi_x_minus, m->phi_y_minus, m->phi_z_minus, GB6(m), &s.a,
-                                             m->index_x, m->index_y, m->index_z, &lo[0], &hi[0], m->narrow_band,
-                                             GB6(m), &kMarkFb);
-    FAIL_IF_ERROR(dt);
-    if (stage == 1) {
-      R d = 0;
-      lsm3dcomputestableconstnormalveldtlocal_(&d, &s.a, m->phi_x_plus, m->phi_y_plus, m->phi_z_plus, GB6(m),
-                                               m->phi_x_minus, m->phi_y_minus, m->phi_z_minus, GB6(m), &s.dx, &s.dy,
-                                               &s.dz, &s.cfl, m->index_x, m->index_y, m->index_z, &lo[0], &hi[0],
-                                               m->narrow_band, GB6(m), &kMarkFb);
-      FAIL_IF_ERROR(dt);
-      dt = (double)d;
-    }
-  }
-  if (m->b > 0) {
-    lsm3dcentralgradorder2local_(m->phi_x, m->phi_y, m->phi_z, GB6(m), phi, GB6(m), &s.dx, &s.dy, &s.dz, m->index_x,
-                                 m->index_y, m->index_z, &lo[0], &hi[1], m->narrow_band, GB6(m), &kMarkD1);
-    FAIL_IF_ERROR(dt);
-    // the second derivatives' last writers: xy = d/dx(phi_y), xz = d/dx(phi_z), yz = d/dy(phi_z)
-    R* outs[3][3] = {{m->phi_xx, m->phi_xy, m->phi_xz}, {m->phi_xy, m->phi_yy, m->phi_yz},
-                     {m->phi_xz, m->phi_yz, m->phi_zz}};
-    const R* srcs[3] = {m->phi_x, m->phi_y, m->phi_z};
-    for (int d = 0; d < 3; d++) {
-      lsm3dcentralgradorder2local_(outs[d][0], outs[d][1], outs[d][2], GB6(m), srcs[d], GB6(m), &s.dx, &s.dy, &s.dz,
-                                   m->index_x, m->index_y, m->index_z, &lo[0], &hi[0], m->narrow_band, GB6(m),
-                                   &kMarkD2);
-      FAIL_IF_ERROR(dt);
-    }
-    lsm3daddconstcurvtermtolserhslocal_(m->lse_rhs, GB6(m), m->phi_x, m->phi_y, m->phi_z, GB6(m), m->phi_xx,
-                                        m->phi_xy, m->phi_xz, m->phi_yy, m->phi_yz, m->phi_zz, GB6(m), &s.b,
-                                        m->index_x, m->index_y, m->index_z, &lo[0], &hi[0], m->narrow_band, GB6(m),
-                                        &kMarkFb);
-    FAIL_IF_ERROR(dt);
-  }
-  lsm3dmultiplycutofflserhslocal_(phi, m->lse_rhs, GB6(m), m->index_x, m->index_y, m->index_z, &lo[0], &hi[0],
-                                  m->narrow_band, GB6(m), &kMarkFb, &s.beta, &s.gamma);
-  return dt;
-}
-
-// curvature_model3d_local.c:331-349 and the clamps of :383-394
-double choose_dt(const Model* m, double dt_hyp, double dt_sub, double tplot) {
-  double dt = (m->a > 0) ? dt_hyp : tplot;
-  if (m->b > 0) {
-    const double dt_corr =
-        (1.0 / (m->dx[0] * m->dx[0]) + 1.0 / (m->dx[1] * m->dx[1]) + 1.0 / (m->dx[2] * m->dx[2])) * (2.0 * m->b);
-    const double max_h = (m->a > 0) ? m->cfl / dt : 0.0;
-    dt = m->cfl / (max_h + dt_corr);
-  }
-  if (dt < kDtMinToCorrect) dt = kDtMin;
-  if (dt_sub + dt > tplot) dt = tplot - dt_sub;
-  return dt;
+// The fused stages' view of the model after band(): level 0 of the lists,
+// the marks of band(), the model's coefficients.
+lsm::LocalCurvStep fused_step(const Model* m, const R* phi) {
+  lsm::LocalCurvStep st;
+  st.phi = phi;
+  st.phi_stage1 = m->phi_stage1;
+  st.phi_next = m->phi_next;
+  st.lse_rhs = m->lse_rhs;
+  st.narrow_band = m->narrow_band;
+  st.index_x = m->index_x;
+  st.index_y = m->index_y;
+  st.index_z = m->index_z;
+  st.lo0 = m->n_lo[0];
+  st.hi0 = m->n_hi[0];
+  st.gb = lsm::Box{m->gb[0], m->gb[1], m->gb[2], m->gb[3], m->gb[4], m->gb[5]};
+  st.mark_fb = kMarkFb;
+  st.mark_d1 = kMarkD1;
+  st.mark_d2 = kMarkD2;
+  st.a = m->a;
+  st.b = m->b;
+  st.cfl = m->cfl;
+  for (int q = 0; q < 3; q++) st.dx[q] = m->dx[q];
+  st.dt_min_to_correct = kDtMinToCorrect;
+  st.dt_min = kDtMin;
+  return st;
 }
 
 // The model's arrays must be device memory (the calls below take them as such
@@ -186,11 +148,12 @@ double lsm3d_amd_local_curvature_step(LsmLocalCurvatureModel* m, LSMLIB_REAL* ph
   const Scalars s = scalars(m);
   band(m, phi, s);
   FAIL_IF_ERROR(-1.0);
-  const double dt = choose_dt(m, stage_rhs(m, phi, 1, s), dt_sub, tplot);
+  // the two stages fused per level-0 point (lsm_local_step.h); dt stays on
+  // the device until the end of the step
+  const lsm::LocalCurvStep st = fused_step(m, phi);
+  lsm::local_curv_stage1(st, dt_sub, tplot);
   FAIL_IF_ERROR(-1.0);
-  const R dtr = (R)dt;
-  lsm3dtvdrk2stage1local_(m->phi_stage1, GB6(m), phi, GB6(m), m->lse_rhs, GB6(m), &dtr, m->index_x, m->index_y,
-                          m->index_z, &m->n_lo[0], &m->n_hi[0], m->narrow_band, GB6(m), &kMarkFb);
+  lsm::local_curv_rk1(st);
   FAIL_IF_ERROR(-1.0);
   ghost_fill(m, m->phi_stage1);
   FAIL_IF_ERROR(-1.0);
@@ -198,11 +161,7 @@ double lsm3d_amd_local_curvature_step(LsmLocalCurvatureModel* m, LSMLIB_REAL* ph
     mask_local(m, m->phi_stage1, m->phi_stage1);
     FAIL_IF_ERROR(-1.0);
   }
-  stage_rhs(m, m->phi_stage1, 2, s);
-  FAIL_IF_ERROR(-1.0);
-  lsm3dtvdrk2stage2local_(m->phi_next, GB6(m), m->phi_stage1, GB6(m), phi, GB6(m), m->lse_rhs, GB6(m), &dtr,
-                          m->index_x, m->index_y, m->index_z, &m->n_lo[0], &m->n_hi[0], m->narrow_band, GB6(m),
-                          &kMarkFb);
+  lsm::local_curv_stage2(st);
   FAIL_IF_ERROR(-1.0);
   ghost_fill(m, m->phi_next);
   FAIL_IF_ERROR(-1.0);
@@ -214,7 +173,7 @@ double lsm3d_amd_local_curvature_step(LsmLocalCurvatureModel* m, LSMLIB_REAL* ph
                    "hipMemcpyAsync(phi <- phi_next)");
   }
   FAIL_IF_ERROR(-1.0);
-  return dt;
+  return lsm::local_curv_dt(st);
 }
 
 int lsm3d_amd_local_curvature_reinit_trigger(LsmLocalCurvatureModel* m, const LSMLIB_REAL* phi, double t,

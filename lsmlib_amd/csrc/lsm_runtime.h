@@ -58,7 +58,7 @@ void* device_scratch(size_t bytes);
 // on demand like device_scratch.  *fresh is set when the buffer was
 // (re)allocated by this call, so a caller that keeps an invariant in it can
 // re-establish it.
-enum BufferSlot { BUF_BAND_KEY = 0, BUF_BAND_WORK = 1, BUF_LOCAL_FLAGS = 2, BUF_SLOTS = 4 };
+enum BufferSlot { BUF_BAND_KEY = 0, BUF_BAND_WORK = 1, BUF_LOCAL_FLAGS = 2, BUF_LOCAL_STEP = 3, BUF_SLOTS = 4 };
 void* device_buffer(int slot, size_t bytes, bool* fresh = nullptr, size_t* size = nullptr);
 
 // Collects the arrays of one entry-point call.  Device pointers pass through;

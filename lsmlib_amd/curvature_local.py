@@ -76,6 +76,7 @@ class LocalCurvatureFlow:
         gb = grid.gb
         self.fb1 = (gb[0] + 1, gb[1] - 1, gb[2] + 1, gb[3] - 1, gb[4] + 1, gb[5] - 1)
         self.fb2 = (gb[0] + 2, gb[1] - 2, gb[2] + 2, gb[3] - 2, gb[4] + 2, gb[5] - 2)
+        self._model = None
 
     def model(self):
         """The LsmLocalCurvatureModel of this driver's arrays (for the native
@@ -184,6 +185,27 @@ class LocalCurvatureFlow:
         else:
             phi.copy_(self.nxt)
         return dt
+
+    def step_native(self, phi, dt_sub=0.0, tplot=0.1):
+        """One step through lsm3d_amd_local_curvature_step (the band build and
+        the two stages, each fused per level-0 point); returns dt."""
+        self._configure()
+        m = self._native_model()
+        dt = self.lib.fused("lsm3d_amd_local_curvature_step", ctypes.byref(m), _ptr(phi), dt_sub, tplot)
+        self._pull(m)
+        return dt
+
+    def needs_reinit_native(self, phi, t, dt_sub, last_reinit_time):
+        """lsm3d_amd_local_curvature_reinit_trigger after step_native."""
+        m = self._native_model()
+        code = self.lib.fused("lsm3d_amd_local_curvature_reinit_trigger", ctypes.byref(m), _ptr(phi), t, dt_sub,
+                              last_reinit_time)
+        return code > 0, ("", "sign", "grad", "time")[code]
+
+    def _native_model(self):
+        if self._model is None:
+            self._model = self.model()
+        return self._model
 
     def needs_reinit(self, phi, t, dt_sub, last_reinit_time):
         L, g = self.lib, self.grid
