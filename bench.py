@@ -15,7 +15,8 @@ reads phi, u, v, w and writes phi1 (5 words); stages 2 and 3 also read phi^n
 
 cpu_baseline: the reference Fortran toolbox (flang build in oracle/_ref) or,
 where it is absent, the C restatement, on one host core, advancing the same
-workload at a smaller grid for a bounded time.
+workload at a smaller grid for a bounded time; all_cores: the same chain on
+k-slabs from one host thread per usable core (harness-level parallelism).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
 """
@@ -818,9 +819,67 @@ def cpu_baseline(seconds):
         el = time.perf_counter() - t0
         if el >= seconds:
             break
-    return {"value": grid.interior_cells * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": kind,
-            "sample": f"{n}^3 periodic sphere-union Taylor-Green advection, FP32 build, WENO5/TVD-RK3 "
-                      f"toolbox chain, {steps} steps in {el:.1f} s"}
+    out = {"value": grid.interior_cells * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+           "sample": f"{n}^3 periodic sphere-union Taylor-Green advection, FP32 build, WENO5/TVD-RK3 "
+                     f"toolbox chain, {steps} steps in {el:.1f} s"}
+    out["all_cores"] = cpu_baseline_slabs(lib, kind, max(4.0, seconds / 2))
+    return out
+
+
+def host_cpu():
+    """(CPU model, usable cores) of this host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return model, cores
+
+
+def cpu_baseline_slabs(lib, kind, seconds, n=128):
+    """The same toolbox chain with every routine called on k-slabs of the fill
+    box from one host thread per usable core (at most OMP_NUM_THREADS, at most
+    16): the harness-level parallelism of the serial reference.  One step is
+    first checked against the serial chain bit for bit."""
+    import numpy as np
+    import torch
+    from oracle.steps import OracleStepper, SlabOracleStepper, fill_ghosts
+    from lsmlib_amd.workloads import sphere_union_taylor_green
+
+    model, cores = host_cpu()
+    threads = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)), 16))
+    grid, phi, vel, bc = sphere_union_taylor_green(n, dtype=torch.float32, device="cpu")
+    ph = phi.numpy().copy()
+    vn = tuple(v.numpy().copy() for v in vel)
+    fill_ghosts(lib, ph, grid, bc)
+    dt = lib.lsm3dcomputestableadvectiondt_(vn[0], vn[1], vn[2], grid.gb, grid.fb, *grid.dx, 0.5)
+    ref = ph.copy()
+    OracleStepper(lib, grid, bc).advect_rk3_step(ref, vn, dt)
+    op = SlabOracleStepper(lib, grid, bc, threads)
+    try:
+        op.advect_rk3_step(ph, vn, dt)
+        same = bool(np.array_equal(ph, ref))
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            op.advect_rk3_step(ph, vn, dt)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= seconds:
+                break
+    finally:
+        op.close()
+    return {"value": grid.interior_cells * steps / el, "unit": "cell-updates/s", "cores": op.threads,
+            "kind": kind + ", harness-parallel", "cpu_model": model, "usable_cores": cores,
+            "identical_to_serial": same,
+            "sample": f"{n}^3, the same chain on {op.threads} k-slabs per routine, {steps} steps in {el:.1f} s"}
 
 
 def read_traffic():
@@ -964,7 +1023,10 @@ def main():
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(a.cpu_seconds)
-        print(f"# cpu: {cpu['value']:.4e} cell-updates/s ({cpu['kind']})", file=sys.stderr, flush=True)
+        print(f"# cpu: {cpu['value']:.4e} cell-updates/s ({cpu['kind']}, 1 core); all cores: "
+              f"{cpu['all_cores']['value']:.4e} on {cpu['all_cores']['cores']} threads "
+              f"({cpu['all_cores']['cpu_model']}, identical {cpu['all_cores']['identical_to_serial']})",
+              file=sys.stderr, flush=True)
 
     extras = {}
     if not a.no_extras:

@@ -101,6 +101,65 @@ class OracleStepper:
         phi[...] = new
 
 
+class SlabOracleStepper:
+    """OracleStepper's WENO5 / TVD-RK3 advection step with the toolbox
+    routines called on k-slabs of the fill box from `threads` host threads:
+    the harness-level parallelism a serial toolbox allows.  Per stage every
+    slab runs LSM3D_UPWIND_HJ_WENO5 (its own D1 work array),
+    LSM3D_ZERO_OUT_LEVEL_SET_EQN_RHS on its planes,
+    LSM3D_ADD_ADVECTION_TERM_TO_LSE_RHS and the TVD-RK3 stage over its fill
+    box; after all slabs the ghost fill runs on one thread.  ctypes releases
+    the interpreter lock around each library call, so the slabs run
+    concurrently.  Each routine writes only its fill box, so the result is
+    the serial stepper's bit for bit."""
+
+    def __init__(self, ora, grid, bc, threads):
+        from concurrent.futures import ThreadPoolExecutor
+        self.ora, self.grid, self.bc = ora, grid, tuple(bc)
+        fb, gb = grid.fb, grid.gb
+        nk = fb[5] - fb[4] + 1
+        self.threads = max(1, min(int(threads), nk))
+        cuts = [fb[4] + (nk * q) // self.threads for q in range(self.threads + 1)]
+        self.slabs = [(cuts[q], cuts[q + 1] - 1) for q in range(self.threads)]
+        z = np.zeros(grid.shape, dtype=ora.dtype)
+        self.gx, self.gy, self.gz, self.rhs = z.copy(), z.copy(), z.copy(), z.copy()
+        self.D1 = [z.copy() for _ in range(self.threads)]
+        self.s1, self.s2, self.new = z.copy(), z.copy(), z.copy()
+        self.pool = ThreadPoolExecutor(self.threads)
+        self._k0 = gb[4]
+
+    def _slab(self, q, form, out, ps, pc, vel, dt):
+        o, g = self.ora, self.grid
+        gb, fb = g.gb, g.fb
+        k0, k1 = self.slabs[q]
+        fbs = (fb[0], fb[1], fb[2], fb[3], k0, k1)
+        o.lsm3dupwindhjweno5_(self.gx, self.gy, self.gz, gb, ps, gb, vel[0], vel[1], vel[2], gb, self.D1[q], gb,
+                              fbs, *g.dx)
+        o.lsm3dzerooutlevelseteqnrhs_(self.rhs[k0 - self._k0:k1 - self._k0 + 1],
+                                      (gb[0], gb[1], gb[2], gb[3], k0, k1))
+        o.lsm3daddadvectiontermtolserhs_(self.rhs, gb, self.gx, self.gy, self.gz, gb, vel[0], vel[1], vel[2], gb,
+                                         fbs)
+        if form == 1:
+            o.lsm3dtvdrk3stage1_(out, gb, ps, gb, self.rhs, gb, fbs, dt)
+        else:
+            getattr(o, "lsm3dtvdrk3stage%d_" % form)(out, gb, ps, gb, pc, gb, self.rhs, gb, fbs, dt)
+
+    def _stage(self, form, out, ps, pc, vel, dt):
+        futs = [self.pool.submit(self._slab, q, form, out, ps, pc, vel, dt) for q in range(self.threads)]
+        for f in futs:
+            f.result()
+        fill_ghosts(self.ora, out, self.grid, self.bc)
+
+    def advect_rk3_step(self, phi, vel, dt):
+        self._stage(1, self.s1, phi, None, vel, dt)
+        self._stage(2, self.s2, self.s1, phi, vel, dt)
+        self._stage(3, self.new, self.s2, phi, vel, dt)
+        phi[...] = self.new
+
+    def close(self):
+        self.pool.shutdown()
+
+
 def impose_mask(out, mask, phi):
     """IMPOSE_MASK over the whole array (src/serial/lsm_macros.h:84-91)."""
     out[...] = np.where(mask > phi, mask, phi)
