@@ -1286,7 +1286,14 @@ typedef struct LsmLocalCurvatureModel {
   int outer_limits[4]; /* nlo / nhi of the plus and of the minus outer layer */
 } LsmLocalCurvatureModel;
 /* One localized TVD-RK2 step of phi (in place, ghost cells included) at
- * interval position dt_sub of tplot; returns dt (-1 on error). */
+ * interval position dt_sub of tplot; returns dt (-1 on error).  The band is
+ * built with the routines above; each stage then runs as one pass over the
+ * level-0 entries that recomputes, per point, what the reference's chain of
+ * *LOCAL routines (curvature_model3d_local.c:231-580) leaves in its work
+ * arrays and reads back, so phi, phi_stage1 and phi_next equal the chain's bit
+ * for bit; dt is chosen on the device and read back once.  The derivative
+ * work arrays (phi_*_plus/minus, D1, D2, phi_x .. phi_zz) are not written by
+ * the step; lse_rhs holds stage 1's right-hand side on level 0. */
 double lsm3d_amd_local_curvature_step(LsmLocalCurvatureModel *m, LSMLIB_REAL *phi, double dt_sub, double tplot);
 /* The reinitialization trigger after a step (:592-652): 0 none, 1 a sign
  * change in the outer layer, 2 the average |grad phi| outside [0.9, 1.1],
