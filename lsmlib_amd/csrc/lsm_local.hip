@@ -634,7 +634,7 @@ __device__ __forceinline__ void block_rank3(int bits, int& r0, int& r1, int& r2,
 // issues the loads of kBatch rounds before using them.
 template <typename T>
 __global__ void __launch_bounds__(256)
-    k_band_count(Field<const T> f, int ncell, int mode, T width, T width_inner, int* counts) {
+    k_band_count(Field<const T> f, int ncell, int mode, T width, T width_inner, int* counts, nb_t* codes) {
   const int base = (int)blockIdx.x * kChunk;
   int a = 0, b = 0, c = 0;
   for (int r0 = 0; r0 < kChunkRounds; r0 += kBatch) {
@@ -645,6 +645,7 @@ __global__ void __launch_bounds__(256)
     for (int r = 0; r < kBatch; r++) {
       const int x = base + (r0 + r) * 256 + (int)threadIdx.x;
       const int bits = (x < ncell) ? band_bits<T>(mode, v[r], width, width_inner) : 0;
+      if (codes && x < ncell) codes[x] = (nb_t)bits;  // the cell's flags, read back by k_band_write_codes
       a += bits & 1;
       b += (bits >> 1) & 1;
       c += (bits >> 2) & 1;
@@ -745,6 +746,89 @@ __global__ void __launch_bounds__(256)
       o1 += t1_;
       o2 += t2_;
     }
+  }
+}
+
+// Level 0 from the flags k_band_count stored in narrow_band (narrow_band on
+// phi's box): each thread takes 32 consecutive cells of the workgroup's
+// chunk, so one block scan per chunk ranks them; the band points are written
+// in cell order as k_band_write writes them, and narrow_band is left 1 on the
+// band and 0 elsewhere (outer-layer flags cleared).  VEC: narrow_band's
+// address is 16-byte aligned (two 16-byte loads per thread).
+template <bool VEC>
+__global__ void __launch_bounds__(256)
+    k_band_write_codes(nb_t* codes, CellMap cm, int ncell, const int* offsets, int* ix, int* iy, int* iz,
+                       int list_base, int* outer, int plus_top) {
+  __shared__ int s_w[3][4];
+  const int t = (int)threadIdx.x, lane = t & 63, w = t >> 6;
+  const int c0 = (int)blockIdx.x * kChunk + t * 32;
+  uint32_t band = 0, oneg = 0, opos = 0;
+  if (VEC && c0 + 32 <= ncell) {
+    const uint4* q = reinterpret_cast<const uint4*>(codes + c0);
+    const uint4 u0 = q[0], u1 = q[1];
+    const uint32_t wv[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+    for (int q8 = 0; q8 < 8; q8++)
+#pragma unroll
+      for (int bq = 0; bq < 4; bq++) {
+        const uint32_t code = (wv[q8] >> (8 * bq)) & 0xffu;
+        const int bit = q8 * 4 + bq;
+        band |= (code & 1u) << bit;
+        oneg |= ((code >> 1) & 1u) << bit;
+        opos |= ((code >> 2) & 1u) << bit;
+      }
+  } else {
+    for (int q = 0; q < 32; q++) {
+      const int x = c0 + q;
+      const uint32_t code = (x < ncell) ? (uint32_t)(uint8_t)codes[x] : 0u;
+      band |= (code & 1u) << q;
+      oneg |= ((code >> 1) & 1u) << q;
+      opos |= ((code >> 2) & 1u) << q;
+    }
+  }
+  // exclusive ranks of the three counts in thread (= cell) order
+  const int n0 = __popc(band), n1 = __popc(oneg), n2 = __popc(opos);
+  int i0 = n0, i1 = n1, i2 = n2;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int a0 = __shfl_up(i0, o, 64), a1 = __shfl_up(i1, o, 64), a2 = __shfl_up(i2, o, 64);
+    if (lane >= o) {
+      i0 += a0;
+      i1 += a1;
+      i2 += a2;
+    }
+  }
+  if (lane == 63) {
+    s_w[0][w] = i0;
+    s_w[1][w] = i1;
+    s_w[2][w] = i2;
+  }
+  __syncthreads();
+  int p0 = offsets[3 * blockIdx.x] + i0 - n0, p1 = offsets[3 * blockIdx.x + 1] + i1 - n1,
+      p2 = offsets[3 * blockIdx.x + 2] + i2 - n2;
+  for (int x = 0; x < w; x++) {
+    p0 += s_w[0][x];
+    p1 += s_w[1][x];
+    p2 += s_w[2][x];
+  }
+  uint32_t m = band;
+  while (m) {
+    const int q = __ffs((int)m) - 1;
+    m &= m - 1;
+    const int x = c0 + q;
+    int i, j, k;
+    cm.ijk(x, i, j, k);
+    ix[p0] = i;
+    iy[p0] = j;
+    iz[p0] = k;
+    if ((oneg >> q) & 1u) {
+      outer[p1++] = list_base + p0;
+      codes[x] = nb_t(1);
+    }
+    if ((opos >> q) & 1u) {
+      outer[plus_top - p2++] = list_base + p0;
+      codes[x] = nb_t(1);
+    }
+    p0++;
   }
 }
 
@@ -1336,8 +1420,13 @@ void determine_band(const char* fn, int mode, const R* phi, Box g, unsigned char
   if (!c.ok()) return;
   int* totals = work + 3 * nblk;
   CellMap cm{g, g.nx(), g.nx() * g.ny(), FastDiv((uint32_t)g.nx()), FastDiv((uint32_t)(g.nx() * g.ny()))};
+  const int same_box = (bn.ilo == g.ilo && bn.ihi == g.ihi && bn.jlo == g.jlo && bn.jhi == g.jhi &&
+                        bn.klo == g.klo && bn.khi == g.khi) ? 1 : 0;
+  // narrow_band on phi's box: the count pass stores each cell's flags there and
+  // the write pass reads them back instead of phi
+  nb_t* codes = same_box ? fnb.p : nullptr;
   hipLaunchKernelGGL(k_band_count<R>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), fphi, ncell, mode, width,
-                     width_inner, work);
+                     width_inner, work, codes);
   c.launched("k_band_count");
   hipLaunchKernelGGL(k_scan3, dim3(1), dim3(1024), 0, c.stream(), work, nblk, totals);
   c.launched("k_scan3");
@@ -1354,11 +1443,19 @@ void determine_band(const char* fn, int mode, const R* phi, Box g, unsigned char
     raise_error(std::string(fn) + ": index_outer is too short for the outer layer");
     return;
   }
-  const int same_box = (bn.ilo == g.ilo && bn.ihi == g.ihi && bn.jlo == g.jlo && bn.jhi == g.jhi &&
-                        bn.klo == g.klo && bn.khi == g.khi) ? 1 : 0;
-  hipLaunchKernelGGL(k_band_write<R>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), fphi, cm, ncell, mode, width,
-                     width_inner, work, fnb, same_box, dix, diy, diz, nlo, dout, (int)(ocap - 1));
-  c.launched("k_band_write");
+  if (codes) {
+    if (((uintptr_t)codes & 15) == 0)
+      hipLaunchKernelGGL(k_band_write_codes<true>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), codes, cm, ncell,
+                         work, dix, diy, diz, nlo, dout, (int)(ocap - 1));
+    else
+      hipLaunchKernelGGL(k_band_write_codes<false>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), codes, cm,
+                         ncell, work, dix, diy, diz, nlo, dout, (int)(ocap - 1));
+    c.launched("k_band_write_codes");
+  } else {
+    hipLaunchKernelGGL(k_band_write<R>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), fphi, cm, ncell, mode,
+                       width, width_inner, work, fnb, same_box, dix, diy, diz, nlo, dout, (int)(ocap - 1));
+    c.launched("k_band_write");
+  }
   if (!c.ok()) return;
 
   const int count = nlo + tot[0];
