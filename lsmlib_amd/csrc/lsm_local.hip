@@ -663,7 +663,16 @@ __global__ void __launch_bounds__(256)
 // 1024 threads: each thread sums a run of rows, the runs are scanned with wave
 // shuffles and one LDS step over the 16 wave totals); totals[0..2] receive the
 // sums.
-__global__ void __launch_bounds__(1024) k_scan3(int* counts, int nb, int* totals) {
+__device__ __forceinline__ void scan3_body(int* counts, int nb, int* totals);
+
+__global__ void __launch_bounds__(1024) k_scan3(int* counts, int nb, int* totals) { scan3_body(counts, nb, totals); }
+
+// The same with the row count read from device memory.
+__global__ void __launch_bounds__(1024) k_scan3_dev(int* counts, const int* nb, int* totals) {
+  scan3_body(counts, *nb, totals);
+}
+
+__device__ __forceinline__ void scan3_body(int* counts, int nb, int* totals) {
   __shared__ int s_w[3][16];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int per = (nb + 1023) / 1024;
@@ -758,8 +767,9 @@ __global__ void __launch_bounds__(256)
 template <bool VEC>
 __global__ void __launch_bounds__(256)
     k_band_write_codes(nb_t* codes, CellMap cm, int ncell, const int* offsets, int* ix, int* iy, int* iz,
-                       int list_base, int* outer, int plus_top) {
+                       int list_base, int* outer, int plus_top, const int* err) {
   __shared__ int s_w[3][4];
+  const bool write = (err == nullptr) || (*err == 0);
   const int t = (int)threadIdx.x, lane = t & 63, w = t >> 6;
   const int c0 = (int)blockIdx.x * kChunk + t * 32;
   uint32_t band = 0, oneg = 0, opos = 0;
@@ -817,15 +827,19 @@ __global__ void __launch_bounds__(256)
     const int x = c0 + q;
     int i, j, k;
     cm.ijk(x, i, j, k);
-    ix[p0] = i;
-    iy[p0] = j;
-    iz[p0] = k;
+    if (write) {
+      ix[p0] = i;
+      iy[p0] = j;
+      iz[p0] = k;
+    }
     if ((oneg >> q) & 1u) {
-      outer[p1++] = list_base + p0;
+      if (write) outer[p1] = list_base + p0;
+      p1++;
       codes[x] = nb_t(1);
     }
     if ((opos >> q) & 1u) {
-      outer[plus_top - p2++] = list_base + p0;
+      if (write) outer[plus_top - p2] = list_base + p0;
+      p2++;
       codes[x] = nb_t(1);
     }
     p0++;
@@ -919,6 +933,152 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Band building with the level sizes kept on the device (the narrow-band
+// model's step): the same kernels' work on the same order of entries, with
+// every size read from a BandState instead of the host, so a band build
+// needs no host synchronisation.  Kernels sized by the state run on fixed
+// grids and stride over their work.
+// ---------------------------------------------------------------------------
+struct BandPrev {  // the level limits a band build leaves alone when level 0 is empty
+  int lo[8], hi[8];
+};
+
+struct BandState {
+  int count;         // next free list entry (absolute index)
+  int err;           // a list or level was too long
+  int nbr;           // level 0 is not empty: the neighbour levels are built
+  int npairs, nchunks, plo, fits;  // the level being built
+  int tot[3];        // level 0: band points, outer minus, outer plus
+  int n_lo[8], n_hi[8];
+  double dt;         // the narrow-band model's step: its dt
+};
+
+__global__ void k_band_level0(BandState* bs, const int* totals, int nlo, int cap, int ocap, int levels,
+                              BandPrev prev) {
+  const int t0 = totals[0], t1 = totals[1], t2 = totals[2];
+  bs->tot[0] = t0;
+  bs->tot[1] = t1;
+  bs->tot[2] = t2;
+  bs->err = (t0 > cap || (int64_t)t1 + t2 > ocap) ? 1 : 0;
+  const int count = nlo + (bs->err ? 0 : t0);
+  bs->count = count;
+  bs->n_lo[0] = nlo;
+  bs->nbr = (count > nlo) && !bs->err;
+  bs->n_hi[0] = bs->nbr ? count - 1 : count;
+  for (int l = 1; l <= levels && l < 8; l++) {  // kept when no neighbour level is built
+    bs->n_lo[l] = prev.lo[l];
+    bs->n_hi[l] = prev.hi[l];
+  }
+}
+
+__global__ void k_band_level_setup(BandState* bs, int l, int nlo) {
+  int np = 0;
+  if (bs->nbr && !bs->err) {
+    const int plo = bs->n_lo[l - 1], phi = bs->n_hi[l - 1];
+    if (plo >= nlo && phi < bs->count && phi >= plo) {
+      const int64_t np64 = 6 * ((int64_t)phi - plo + 1);
+      if (np64 > INT_MAX)
+        bs->err = 1;
+      else
+        np = (int)np64;
+    }
+    bs->plo = plo;
+  }
+  bs->npairs = np;
+  bs->nchunks = (np + kPairChunk - 1) / kPairChunk;
+}
+
+__global__ void __launch_bounds__(256) k_nbr_bid_dev(const BandState* bs, const int* ix, const int* iy,
+                                                     const int* iz, int nlo, Field<const nb_t> nb, CellMap cm,
+                                                     int* key) {
+  const int np = bs->npairs, off = bs->plo - nlo;
+  for (int p = (int)(blockIdx.x * blockDim.x + threadIdx.x); p < np; p += (int)(gridDim.x * blockDim.x)) {
+    const int m = off + p / 6, q = p % 6;
+    int a, b, c;
+    if (neighbour(q, ix[m], iy[m], iz[m], cm.b, a, b, c) && nb.at(a, b, c) == 0) atomicMin(&key[cm.lin(a, b, c)], p);
+  }
+}
+
+// The winners of a level's bids: the pair holding its cell's key.  The test
+// is the same before and during k_nbr_write_dev (only a winner writes its own
+// cell's mark and key).
+__device__ __forceinline__ bool nbr_wins(int p, int off, const int* ix, const int* iy, const int* iz,
+                                         const Field<const nb_t>& nb, const CellMap& cm, const int* key, int& a,
+                                         int& b, int& c) {
+  const int m = off + p / 6, q = p % 6;
+  return neighbour(q, ix[m], iy[m], iz[m], cm.b, a, b, c) && nb.at(a, b, c) == 0 && key[cm.lin(a, b, c)] == p;
+}
+
+__global__ void __launch_bounds__(256) k_nbr_count_dev(const BandState* bs, const int* ix, const int* iy,
+                                                       const int* iz, int nlo, Field<const nb_t> nb, CellMap cm,
+                                                       const int* key, int* counts) {
+  const int np = bs->npairs, nch = bs->nchunks, off = bs->plo - nlo;
+  for (int ch = (int)blockIdx.x; ch < nch; ch += (int)gridDim.x) {  // uniform per workgroup
+    int n = 0, z0 = 0, z1 = 0;
+    for (int r = 0; r < kPairRounds; r++) {
+      const int p = ch * kPairChunk + r * 256 + (int)threadIdx.x;
+      int a, b, c;
+      if (p < np && nbr_wins(p, off, ix, iy, iz, nb, cm, key, a, b, c)) n++;
+    }
+    block_sum3(n, z0, z1);
+    if (threadIdx.x == 0) {
+      counts[3 * ch] = n;
+      counts[3 * ch + 1] = 0;
+      counts[3 * ch + 2] = 0;
+    }
+    __syncthreads();  // block_sum3's LDS is read again by the next chunk
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_nbr_write_dev(BandState* bs, const int* totals, const int* offsets, int* ix, int* iy, int* iz, int nlo,
+                    int nhi, Field<nb_t> nbw, CellMap cm, int* key, nb_t mark) {
+  const int np = bs->npairs, nch = bs->nchunks, off = bs->plo - nlo;
+  const int base_entry = bs->count;  // absolute index of the level's first entry
+  const bool fits = (int64_t)base_entry + totals[0] - 1 <= (int64_t)nhi;
+  const Field<const nb_t> nb = make_field<const nb_t>(nbw.p, cm.b);
+  for (int ch = (int)blockIdx.x; ch < nch; ch += (int)gridDim.x) {
+    int o = offsets[3 * ch];
+    for (int r = 0; r < kPairRounds; r++) {
+      const int p = ch * kPairChunk + r * 256 + (int)threadIdx.x;
+      int a = 0, b = 0, c = 0;
+      const int bits = (p < np && nbr_wins(p, off, ix, iy, iz, nb, cm, key, a, b, c)) ? 1 : 0;
+      int r0_, r1_, r2_, t0_, t1_, t2_;
+      block_rank3(bits, r0_, r1_, r2_, t0_, t1_, t2_);
+      if (bits) {
+        if (fits) {
+          const int pos = base_entry - nlo + o + r0_;
+          ix[pos] = a;
+          iy[pos] = b;
+          iz[pos] = c;
+          nbw.at(a, b, c) = mark;
+        }
+        key[cm.lin(a, b, c)] = INT_MAX;
+      }
+      o += t0_;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) bs->fits = fits ? 1 : 0;
+}
+
+__global__ void k_band_level_finish(BandState* bs, const int* totals, int l) {
+  if (!bs->nbr) return;  // the level limits keep their previous values
+  int added = 0;
+  if (bs->npairs > 0) {
+    if (!bs->fits) bs->err = 1;
+    else added = totals[0];
+  }
+  bs->count += added;
+  if (bs->count > bs->n_hi[l - 1] + 1) {
+    bs->n_lo[l] = bs->n_hi[l - 1] + 1;
+    bs->n_hi[l] = bs->count - 1;
+  } else {
+    bs->n_lo[l] = -1;
+    bs->n_hi[l] = -1;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_fill_int(int* p, int64_t n, int v) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;
@@ -944,18 +1104,22 @@ __global__ void __launch_bounds__(256)
     k_check_outer(const int* outer, int n, const int* ix, const int* iy, const int* iz, int nlo, int nhi,
                   Field<const T> phi, Box gphi, int shift, int* bits, int* skipped) {
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (t >= n) return;
-  const int l = outer[t];
-  if (l < nlo || l > nhi) {
-    atomicAdd(skipped, 1);
-    return;
+  int v = 0;  // 1: phi > 0 seen, 2: phi <= 0 seen
+  if (t < n) {
+    const int l = outer[t];
+    if (l < nlo || l > nhi) {
+      atomicAdd(skipped, 1);
+    } else {
+      const int i = ix[l - nlo], j = iy[l - nlo], k = iz[l - nlo];
+      if (!inside(gphi, i, j, k))
+        atomicAdd(skipped, 1);
+      else
+        v = phi.at(i, j, k) > T(0) ? 1 : 2;
+    }
   }
-  const int i = ix[l - nlo], j = iy[l - nlo], k = iz[l - nlo];
-  if (!inside(gphi, i, j, k)) {
-    atomicAdd(skipped, 1);
-    return;
-  }
-  atomicOr(bits, (phi.at(i, j, k) > T(0) ? 1 : 2) << shift);
+  // one atomic per wave: the OR of the wave's signs
+  const unsigned long long pos = __ballot(v & 1), neg = __ballot(v & 2);
+  if ((threadIdx.x & 63) == 0 && (pos | neg)) atomicOr(bits, ((pos ? 1 : 0) | (neg ? 2 : 0)) << shift);
 }
 
 // ---------------------------------------------------------------------------
@@ -964,10 +1128,10 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 template <typename T>
 struct LocalCurvArgs {
-  const int* ix;
+  const int* ix;  // the band lists, entry l at element l
   const int* iy;
   const int* iz;
-  int n;  // level-0 entries
+  const BandState* bs;  // level 0: entries bs->n_lo[0] .. bs->n_hi[0]
   Field<const nb_t> nb;
   Field<const T> ps;  // stage input
   Field<const T> pc;  // u_cur (stage 2)
@@ -1060,10 +1224,10 @@ __device__ __forceinline__ bool local_curv_rhs(const LocalCurvArgs<T>& a, int i,
 
 template <typename T, int STAGE>
 __global__ void __launch_bounds__(256) k_local_curv_stage(LocalCurvArgs<T> a) {
-  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int lo = a.bs->n_lo[0], n = a.bs->n_hi[0] - lo + 1;
   double h = -1.0e308;
-  if (t < a.n) {
-    const int i = a.ix[t], j = a.iy[t], k = a.iz[t];
+  for (int t = (int)(blockIdx.x * blockDim.x + threadIdx.x); t < n; t += (int)(gridDim.x * blockDim.x)) {
+    const int i = a.ix[lo + t], j = a.iy[lo + t], k = a.iz[lo + t];
     if (!inside(a.gb, i, j, k)) {
       atomicAdd(a.skipped, 1);
     } else if ((int)a.nb.at(i, j, k) <= a.mark_fb) {
@@ -1073,7 +1237,7 @@ __global__ void __launch_bounds__(256) k_local_curv_stage(LocalCurvArgs<T> a) {
         atomicAdd(a.skipped, 1);
       } else if (STAGE == 1) {
         a.rhs.at(i, j, k) = r;
-        h = hh;
+        h = fmax(h, hh);
       } else {  // LSM3D_TVD_RK2_STAGE2_LOCAL
         a.out.at(i, j, k) = rk_combine<T>(RK2_STAGE2, a.pc.at(i, j, k), a.ps.at(i, j, k), r, *a.dt);
       }
@@ -1091,16 +1255,35 @@ __global__ void __launch_bounds__(256) k_local_curv_stage(LocalCurvArgs<T> a) {
 // LSM3D_TVD_RK2_STAGE1_LOCAL on level 0 with the device dt.
 template <typename T>
 __global__ void __launch_bounds__(256) k_local_curv_rk1(LocalCurvArgs<T> a) {
-  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (t >= a.n) return;
-  const int i = a.ix[t], j = a.iy[t], k = a.iz[t];
-  if (!inside(a.gb, i, j, k)) {
-    atomicAdd(a.skipped, 1);
-    return;
+  const int lo = a.bs->n_lo[0], n = a.bs->n_hi[0] - lo + 1;
+  for (int t = (int)(blockIdx.x * blockDim.x + threadIdx.x); t < n; t += (int)(gridDim.x * blockDim.x)) {
+    const int i = a.ix[lo + t], j = a.iy[lo + t], k = a.iz[lo + t];
+    if (!inside(a.gb, i, j, k)) {
+      atomicAdd(a.skipped, 1);
+      continue;
+    }
+    if ((int)a.nb.at(i, j, k) > a.mark_fb) continue;
+    const T u = a.ps.at(i, j, k);
+    a.out.at(i, j, k) = rk_combine<T>(RK_EULER, u, u, a.rhs.at(i, j, k), *a.dt);
   }
-  if ((int)a.nb.at(i, j, k) > a.mark_fb) return;
-  const T u = a.ps.at(i, j, k);
-  a.out.at(i, j, k) = rk_combine<T>(RK_EULER, u, u, a.rhs.at(i, j, k), *a.dt);
+}
+
+// IMPOSE_MASK_LOCAL of the model (entries n_lo[0] .. n_hi[0] - 1,
+// src/serial/lsm_macros.h:108-117): dest = max(mask, src).
+template <typename T>
+__global__ void __launch_bounds__(256) k_local_mask(const BandState* bs, const int* ix, const int* iy,
+                                                    const int* iz, Field<T> dest, Field<const T> src,
+                                                    Field<const T> mask, Box gb, int* skipped) {
+  const int lo = bs->n_lo[0], n = bs->n_hi[0] - lo;
+  for (int t = (int)(blockIdx.x * blockDim.x + threadIdx.x); t < n; t += (int)(gridDim.x * blockDim.x)) {
+    const int i = ix[lo + t], j = iy[lo + t], k = iz[lo + t];
+    if (!inside(gb, i, j, k)) {
+      atomicAdd(skipped, 1);
+      continue;
+    }
+    const T m = mask.at(i, j, k), v = src.at(i, j, k);
+    dest.at(i, j, k) = (m > v) ? m : v;
+  }
 }
 
 // The step's dt: the CFL reduction's last step
@@ -1111,7 +1294,7 @@ struct LocalDtArgs {
   const double* partials;
   int nblocks, normal, curv;
   double a, b, cfl, dx0, dx1, dx2, dt_sub, tplot, dt_min_to_correct, dt_min;
-  double* dts;
+  BandState* bs;  // receives dt
 };
 
 template <typename T>
@@ -1134,7 +1317,7 @@ __global__ void __launch_bounds__(256) k_local_curv_dt(LocalDtArgs d, T cfl_r, T
   }
   if (dt < d.dt_min_to_correct) dt = d.dt_min;
   if (d.dt_sub + dt > d.tplot) dt = d.tplot - d.dt_sub;
-  d.dts[0] = dt;
+  d.bs->dt = dt;
   dtr[0] = (T)dt;
 }
 
@@ -1446,10 +1629,10 @@ void determine_band(const char* fn, int mode, const R* phi, Box g, unsigned char
   if (codes) {
     if (((uintptr_t)codes & 15) == 0)
       hipLaunchKernelGGL(k_band_write_codes<true>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), codes, cm, ncell,
-                         work, dix, diy, diz, nlo, dout, (int)(ocap - 1));
+                         work, dix, diy, diz, nlo, dout, (int)(ocap - 1), (const int*)nullptr);
     else
       hipLaunchKernelGGL(k_band_write_codes<false>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), codes, cm,
-                         ncell, work, dix, diy, diz, nlo, dout, (int)(ocap - 1));
+                         ncell, work, dix, diy, diz, nlo, dout, (int)(ocap - 1), (const int*)nullptr);
     c.launched("k_band_write_codes");
   } else {
     hipLaunchKernelGGL(k_band_write<R>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), fphi, cm, ncell, mode,
@@ -1554,26 +1737,29 @@ void hj_eno_local(const char* fn, int order, R* xp, R* yp, R* zp, Box gp, R* xm,
 namespace lsm {
 namespace {
 
-constexpr int kLocalDtR = 8;  // element offset (in doubles) of the library-precision dt
+// Workgroups of the launches sized by the band state (they stride over
+// their entries, pairs or chunks).
+constexpr int kLocalGrid = 2048;
+constexpr int kStateBytes = 512;  // BandState, then the library-precision dt at byte 256
 
-// The step's device memory: dt (double) at element 0, the library-precision
-// dt at element kLocalDtR, the per-block CFL maxima from element 16.  Every
-// stage of a step asks for the same size, so the buffer does not move
-// between them.
-double* step_buffer(const LocalCurvStep& s) {
-  const int n = s.hi0 - s.lo0 + 1;
-  const int nblocks = n > 0 ? (n + 255) / 256 : 0;
-  return reinterpret_cast<double*>(device_buffer(BUF_LOCAL_STEP, (size_t)(16 + nblocks) * sizeof(double)));
+// The step's device memory: the band state, the library-precision dt, and
+// one CFL maximum per workgroup of the stage-1 launch.  Every call of a
+// step asks for the same size, so the buffer does not move between them.
+char* step_buffer() {
+  return reinterpret_cast<char*>(
+      device_buffer(BUF_LOCAL_STEP, (size_t)kStateBytes + (size_t)kLocalGrid * sizeof(double)));
 }
+BandState* band_state(char* buf) { return reinterpret_cast<BandState*>(buf); }
+R* step_dtr(char* buf) { return reinterpret_cast<R*>(buf + 256); }
+double* step_partials(char* buf) { return reinterpret_cast<double*>(buf + kStateBytes); }
 
 LocalCurvArgs<R> local_args(Call& c, const LocalCurvStep& s, const R* ps, const R* pc, R* rhs, R* out,
-                            double* buf) {
+                            char* buf) {
   LocalCurvArgs<R> a;
-  const int n = std::max(0, s.hi0 - s.lo0 + 1);
-  a.ix = s.index_x + s.lo0;
-  a.iy = s.index_y + s.lo0;
-  a.iz = s.index_z + s.lo0;
-  a.n = n;
+  a.ix = s.index_x;
+  a.iy = s.index_y;
+  a.iz = s.index_z;
+  a.bs = band_state(buf);
   a.nb = c.in(reinterpret_cast<const nb_t*>(s.narrow_band), s.gb);
   a.ps = c.in(ps, s.gb);
   a.pc = pc ? c.in(pc, s.gb) : none();
@@ -1604,72 +1790,230 @@ LocalCurvArgs<R> local_args(Call& c, const LocalCurvStep& s, const R* ps, const 
   a.c1 = gamma - R(3) * beta;
   R c2 = gamma - beta;
   a.c2 = c2 * c2 * c2;
-  a.dt = reinterpret_cast<const R*>(buf + kLocalDtR);
-  a.partials = buf + 16;
+  a.dt = step_dtr(buf);
+  a.partials = step_partials(buf);
   a.skipped = skipped_counter();
   return a;
 }
 
+bool step_ok(Call& c, char* buf, const char* what) {
+  if (!buf) raise_error(std::string("lsm3d_amd_local_curvature_step: ") + what + ": device buffer allocation failed");
+  return c.ok();
+}
+
 }  // namespace
+
+void local_band(const LocalCurvStep& s) {
+  Call c("lsm3d_amd_local_curvature_step (band)");
+  char* buf = step_buffer();
+  if (!step_ok(c, buf, "band")) return;
+  const Box g = s.gb;
+  const int64_t ncell64 = g.cells();
+  if (ncell64 <= 0 || ncell64 > INT_MAX || s.num_points < ncell64 || s.levels < 0 || s.levels > 7) {
+    raise_error("lsm3d_amd_local_curvature_step: the box, the list capacity or the level count is out of range");
+    return;
+  }
+  const int ncell = (int)ncell64;
+  const R width = (R)(4.0 * s.dx[0]), width_inner = (R)(2.0 * s.dx[0]);  // gamma, beta of the model
+  Field<const R> fphi = c.in(s.phi, g);
+  Field<nb_t> fnb = c.out(reinterpret_cast<nb_t*>(s.narrow_band), g);
+  const int nlo = 0, nhi = s.num_points - 1;
+  const int nblk = blocks_for(ncell, kChunk);
+  // level 0's block counts, then the largest neighbour level's chunk counts
+  const int64_t max_chunks = (6 * (int64_t)s.num_points + kPairChunk - 1) / kPairChunk;
+  const int64_t rows = std::max<int64_t>(nblk, max_chunks);
+  int* work = reinterpret_cast<int*>(device_buffer(BUF_BAND_WORK, (size_t)(3 * rows + 8) * sizeof(int) + 256));
+  bool fresh = false;
+  size_t got = 0;
+  int* key = reinterpret_cast<int*>(device_buffer(BUF_BAND_KEY, (size_t)ncell * sizeof(int), &fresh, &got));
+  if (!work || !key) raise_error("lsm3d_amd_local_curvature_step: band buffers: device allocation failed");
+  if (!c.ok()) return;
+  if (fresh) {  // the whole key buffer holds INT_MAX between calls
+    const int64_t nk = (int64_t)(got / sizeof(int));
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)blocks_for(nk, 256)), dim3(256), 0, c.stream(), key, nk, INT_MAX);
+  }
+  int* totals = work + 3 * rows;
+  BandState* bs = band_state(buf);
+  CellMap cm{g, g.nx(), g.nx() * g.ny(), FastDiv((uint32_t)g.nx()), FastDiv((uint32_t)(g.nx() * g.ny()))};
+  // level 0: flags in narrow_band, scan, lists (LSM3D_DETERMINE_NARROW_BAND's scan, lsm_localization3d.f:232-255)
+  hipLaunchKernelGGL(k_band_count<R>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), fphi, ncell, (int)BAND_PHI,
+                     width, width_inner, work, fnb.p);
+  c.launched("k_band_count");
+  hipLaunchKernelGGL(k_scan3, dim3(1), dim3(1024), 0, c.stream(), work, nblk, totals);
+  c.launched("k_scan3");
+  BandPrev prev;
+  for (int l = 0; l < 8; l++) {
+    prev.lo[l] = s.prev_lo[l];
+    prev.hi[l] = s.prev_hi[l];
+  }
+  hipLaunchKernelGGL(k_band_level0, dim3(1), dim3(1), 0, c.stream(), bs, (const int*)totals, nlo, s.num_points,
+                     s.num_points, s.levels, prev);
+  c.launched("k_band_level0");
+  if (((uintptr_t)fnb.p & 15) == 0)
+    hipLaunchKernelGGL(k_band_write_codes<true>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), fnb.p, cm, ncell,
+                       (const int*)work, s.index_x, s.index_y, s.index_z, nlo, s.index_outer, s.num_points - 1,
+                       (const int*)&bs->err);
+  else
+    hipLaunchKernelGGL(k_band_write_codes<false>, dim3((unsigned)nblk), dim3(256), 0, c.stream(), fnb.p, cm, ncell,
+                       (const int*)work, s.index_x, s.index_y, s.index_z, nlo, s.index_outer, s.num_points - 1,
+                       (const int*)&bs->err);
+  c.launched("k_band_write_codes");
+  // levels 1 .. levels: first-discovery bids, winners compacted in pair order
+  // (lsm3dMarkNarrowBandNeighbors, lsm_localization3d.f:48-168)
+  const Field<const nb_t> nbc = make_field<const nb_t>(fnb.p, g);
+  for (int l = 1; l <= s.levels && c.ok(); l++) {
+    hipLaunchKernelGGL(k_band_level_setup, dim3(1), dim3(1), 0, c.stream(), bs, l, nlo);
+    hipLaunchKernelGGL(k_nbr_bid_dev, dim3(4 * kLocalGrid), dim3(256), 0, c.stream(), (const BandState*)bs,
+                       (const int*)s.index_x, (const int*)s.index_y, (const int*)s.index_z, nlo, nbc, cm, key);
+    hipLaunchKernelGGL(k_nbr_count_dev, dim3(kLocalGrid), dim3(256), 0, c.stream(), (const BandState*)bs,
+                       (const int*)s.index_x, (const int*)s.index_y, (const int*)s.index_z, nlo, nbc, cm,
+                       (const int*)key, work);
+    hipLaunchKernelGGL(k_scan3_dev, dim3(1), dim3(1024), 0, c.stream(), work, (const int*)&bs->nchunks, totals);
+    hipLaunchKernelGGL(k_nbr_write_dev, dim3(kLocalGrid), dim3(256), 0, c.stream(), bs, (const int*)totals,
+                       (const int*)work, s.index_x, s.index_y, s.index_z, nlo, nhi, fnb, cm, key, nb_t(l + 1));
+    hipLaunchKernelGGL(k_band_level_finish, dim3(1), dim3(1), 0, c.stream(), bs, (const int*)totals, l);
+    c.launched("neighbour level");
+  }
+  c.finish();
+}
 
 void local_curv_stage1(const LocalCurvStep& s, double dt_sub, double tplot) {
   Call c("lsm3d_amd_local_curvature_step (stage 1)");
-  double* buf = step_buffer(s);
-  if (!buf) raise_error("lsm3d_amd_local_curvature_step: device buffer allocation failed");
-  if (!c.ok()) return;
+  char* buf = step_buffer();
+  if (!step_ok(c, buf, "stage 1")) return;
   LocalCurvArgs<R> a = local_args(c, s, s.phi, nullptr, s.lse_rhs, s.phi_stage1, buf);
-  const int nblocks = (a.n + 255) / 256;
-  if (c.ok() && a.n > 0) {
-    hipLaunchKernelGGL((k_local_curv_stage<R, 1>), dim3((unsigned)nblocks), dim3(256), 0, c.stream(), a);
-    c.launched("k_local_curv_stage<1>");
-  }
-  if (c.ok()) {
-    LocalDtArgs d{buf + 16, a.normal ? nblocks : 0, a.normal, a.curv, s.a, s.b, s.cfl, s.dx[0], s.dx[1], s.dx[2],
-                  dt_sub, tplot, s.dt_min_to_correct, s.dt_min, buf};
-    hipLaunchKernelGGL(k_local_curv_dt<R>, dim3(1), dim3(256), 0, c.stream(), d, (R)s.cfl,
-                       reinterpret_cast<R*>(buf + kLocalDtR));
-    c.launched("k_local_curv_dt");
-  }
+  if (!c.ok()) return;
+  hipLaunchKernelGGL((k_local_curv_stage<R, 1>), dim3(kLocalGrid), dim3(256), 0, c.stream(), a);
+  c.launched("k_local_curv_stage<1>");
+  LocalDtArgs d{step_partials(buf), a.normal ? kLocalGrid : 0, a.normal, a.curv, s.a, s.b, s.cfl, s.dx[0],
+                s.dx[1], s.dx[2], dt_sub, tplot, s.dt_min_to_correct, s.dt_min, band_state(buf)};
+  hipLaunchKernelGGL(k_local_curv_dt<R>, dim3(1), dim3(256), 0, c.stream(), d, (R)s.cfl, step_dtr(buf));
+  c.launched("k_local_curv_dt");
   finish_local(c);
 }
 
 void local_curv_rk1(const LocalCurvStep& s) {
   Call c("lsm3d_amd_local_curvature_step (RK2 stage 1)");
-  double* buf = step_buffer(s);
-  if (!buf) raise_error("lsm3d_amd_local_curvature_step: device buffer allocation failed");
-  if (!c.ok()) return;
+  char* buf = step_buffer();
+  if (!step_ok(c, buf, "RK2 stage 1")) return;
   LocalCurvArgs<R> a = local_args(c, s, s.phi, nullptr, s.lse_rhs, s.phi_stage1, buf);
-  if (c.ok() && a.n > 0) {
-    hipLaunchKernelGGL(k_local_curv_rk1<R>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, c.stream(), a);
-    c.launched("k_local_curv_rk1");
-  }
+  if (!c.ok()) return;
+  hipLaunchKernelGGL(k_local_curv_rk1<R>, dim3(kLocalGrid), dim3(256), 0, c.stream(), a);
+  c.launched("k_local_curv_rk1");
   finish_local(c);
 }
 
 void local_curv_stage2(const LocalCurvStep& s) {
   Call c("lsm3d_amd_local_curvature_step (stage 2)");
-  double* buf = step_buffer(s);
-  if (!buf) raise_error("lsm3d_amd_local_curvature_step: device buffer allocation failed");
-  if (!c.ok()) return;
+  char* buf = step_buffer();
+  if (!step_ok(c, buf, "stage 2")) return;
   LocalCurvArgs<R> a = local_args(c, s, s.phi_stage1, s.phi, s.lse_rhs, s.phi_next, buf);
-  if (c.ok() && a.n > 0) {
-    hipLaunchKernelGGL((k_local_curv_stage<R, 2>), dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0,
-                       c.stream(), a);
-    c.launched("k_local_curv_stage<2>");
-  }
+  if (!c.ok()) return;
+  hipLaunchKernelGGL((k_local_curv_stage<R, 2>), dim3(kLocalGrid), dim3(256), 0, c.stream(), a);
+  c.launched("k_local_curv_stage<2>");
   finish_local(c);
 }
 
-double local_curv_dt(const LocalCurvStep& s) {
-  Call c("lsm3d_amd_local_curvature_step (dt)");
-  double* buf = step_buffer(s);
-  double dt = -1.0;
+void local_mask(const LocalCurvStep& s, R* dest, const R* src) {
+  Call c("lsm3d_amd_local_curvature_step (mask)");
+  char* buf = step_buffer();
+  if (!step_ok(c, buf, "mask")) return;
+  Field<R> fd = c.out(dest, s.gb);
+  Field<const R> fs = c.in(src, s.gb), fm = c.in(s.mask, s.gb);
+  if (!c.ok()) return;
+  hipLaunchKernelGGL(k_local_mask<R>, dim3(kLocalGrid), dim3(256), 0, c.stream(), (const BandState*)band_state(buf),
+                     (const int*)s.index_x, (const int*)s.index_y, (const int*)s.index_z, fd, fs, fm, s.gb,
+                     skipped_counter());
+  c.launched("k_local_mask");
+  finish_local(c);
+}
+
+bool local_trigger_values(const LocalCurvStep& s, int nlo0, int nhi0, const int outer_limits[4], int* change,
+                          R* ave) {
+  Call c("lsm3d_amd_local_curvature_reinit_trigger");
+  // LSM3D_CHECK_OUTER_NARROW_BAND_LAYER over the plus and minus outer lists
+  // (entries resolved through level 0), then LSM3D_COMPUTE_AVE_GRAD_PHI_LOCAL
+  // over level 0 with the fill-box mark: the routines' kernels and launch
+  // shapes, read back together
+  const int lo[2] = {outer_limits[0], outer_limits[2]}, hi[2] = {outer_limits[1], outer_limits[3]};
+  const int last = s.num_points - 1;
+  for (int q = 0; q < 2; q++)
+    if (hi[q] >= lo[q] && (lo[q] < 0 || hi[q] > last)) {
+      raise_error("lsm3d_amd_local_curvature_reinit_trigger: outer-layer range outside index_outer");
+      return false;
+    }
+  ListArgs L = stage_lists(c, s.index_x, s.index_y, s.index_z, nlo0, nhi0);
+  Field<const R> fphi = c.in(s.phi, s.gb);
+  int* bits = skipped_counter();
+  if (!c.ok() || !bits) return false;
+  int* dbits = bits + 4;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, c.stream(), dbits, (int64_t)1, 0);
+  for (int q = 0; q < 2 && L.ok; q++) {
+    if (hi[q] < lo[q]) continue;
+    const int n = hi[q] - lo[q] + 1;
+    hipLaunchKernelGGL(k_check_outer<R>, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, c.stream(),
+                       (const int*)s.index_outer + lo[q], n, L.ix, L.iy, L.iz, nlo0, nhi0, fphi, s.gb, 2 * q, dbits,
+                       bits);
+    c.launched("k_check_outer");
+  }
+  double out[2] = {0.0, 0.0};
+  double* scratch = nullptr;
+  if (L.ok && nhi0 >= nlo0) {
+    const int n = nhi0 - nlo0 + 1, nblocks = (n + 255) / 256;
+    scratch = reinterpret_cast<double*>(device_scratch((size_t)(2 * nblocks + 2) * sizeof(double)));
+    if (!scratch) raise_error("lsm3d_amd_local_curvature_reinit_trigger: device scratch allocation failed");
+    if (!c.ok()) return false;
+    const R dx = (R)s.dx[0], dy = (R)s.dx[1], dz = (R)s.dx[2];
+    RedAveGrad<R> op{fphi, R(0.5 / (double)dx), R(0.5 / (double)dy), R(0.5 / (double)dz)};
+    const unsigned char mfb = (unsigned char)s.mark_fb;
+    const Marks mk = marks(c, s.narrow_band, s.gb, &mfb);
+    Entries e = make_entries(L, nlo0, nhi0, mk, isect(shrink(s.gb, 1, 1, 1), s.gb), everywhere());
+    hipLaunchKernelGGL(k_entries_reduce<RedAveGrad<R>>, dim3((unsigned)nblocks), dim3(256), 0, c.stream(), e, op,
+                       (int)RED_LIST_SUM, scratch + 2);
+    c.launched("k_entries_reduce<avegrad>");
+    hipLaunchKernelGGL(k_entries_reduce_final, dim3(1), dim3(256), 0, c.stream(), (const double*)(scratch + 2),
+                       nblocks, (int)RED_LIST_SUM, scratch);
+    c.launched("k_entries_reduce_final");
+  }
+  int h = 0;
+  if (c.ok()) {
+    check_hip(hipMemcpyAsync(&h, dbits, sizeof(int), hipMemcpyDeviceToHost, c.stream()), "hipMemcpyAsync(bits)");
+    if (scratch)
+      check_hip(hipMemcpyAsync(out, scratch, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream()),
+                "hipMemcpyAsync(average gradient)");
+    check_hip(hipStreamSynchronize(c.stream()), "hipStreamSynchronize");
+  }
+  finish_local(c, true);
+  if (error_pending()) return false;
+  *change = ((h & 3) == 3 || ((h >> 2) & 3) == 3) ? 1 : 0;
+  *ave = (out[1] > 0.0) ? R(out[0] / out[1]) : R(out[0]);
+  return true;
+}
+
+bool local_step_result(const LocalCurvStep& s, LocalStepResult* r) {
+  (void)s;
+  Call c("lsm3d_amd_local_curvature_step (result)");
+  char* buf = step_buffer();
+  if (!buf) raise_error("lsm3d_amd_local_curvature_step: result: device buffer allocation failed");
+  BandState h;
   if (buf && c.ok()) {
-    check_hip(hipMemcpyAsync(&dt, buf, sizeof(double), hipMemcpyDeviceToHost, c.stream()), "hipMemcpyAsync(dt)");
+    check_hip(hipMemcpyAsync(&h, buf, sizeof(BandState), hipMemcpyDeviceToHost, c.stream()),
+              "hipMemcpyAsync(band state)");
     check_hip(hipStreamSynchronize(c.stream()), "hipStreamSynchronize");
   }
   finish_local(c);
-  return error_pending() ? -1.0 : dt;
+  if (error_pending()) return false;
+  if (h.err) {
+    raise_error("lsm3d_amd_local_curvature_step: the index arrays are too short for the narrow band");
+    return false;
+  }
+  for (int l = 0; l < 8; l++) {
+    r->n_lo[l] = h.n_lo[l];
+    r->n_hi[l] = h.n_hi[l];
+  }
+  for (int q = 0; q < 3; q++) r->tot[q] = h.tot[q];
+  r->dt = h.dt;
+  return true;
 }
 
 }  // namespace lsm

@@ -58,14 +58,49 @@ void mask_local(Model* m, R* dest, const R* src) {
                         m->index_y, m->index_z, &lo, &hi);
 }
 
-void band(Model* m, const R* phi, const Scalars& s) {
-  const int n = num_points(m);
-  int zero = 0, last = n - 1, level = kLevel;
-  int* ol = m->outer_limits;
-  lsm3ddeterminenarrowband_(phi, GB6(m), m->narrow_band, GB6(m), m->index_x, m->index_y, m->index_z, &zero, &last,
-                            m->n_lo, m->n_hi, m->index_outer, &zero, &last, &ol[0], &ol[1], &ol[2], &ol[3],
-                            &s.gamma, &s.beta, &level);
-  FAIL_IF_ERROR();
+void eno2(Model* m, const R* phi, const Scalars& s) {
+  lsm3dhjeno2local_(m->phi_x_plus, m->phi_y_plus, m->phi_z_plus, GB6(m), m->phi_x_minus, m->phi_y_minus,
+                    m->phi_z_minus, GB6(m), phi, GB6(m), m->D1, GB6(m), m->D2, GB6(m), &s.dx, &s.dy, &s.dz,
+                    m->index_x, m->index_y, m->index_z, &m->n_lo[0], &m->n_hi[0], &m->n_lo[1], &m->n_hi[1],
+                    &m->n_lo[2], &m->n_hi[2], m->narrow_band, GB6(m), &kMarkFb, &kMarkD1, &kMarkD2);
+}
+
+// The fused step's view of the model: its arrays, the marks of band(), the
+// model's coefficients, and the level limits a band build with an empty
+// level 0 leaves as they are.
+lsm::LocalCurvStep fused_step(const Model* m, const R* phi) {
+  lsm::LocalCurvStep st;
+  st.phi = phi;
+  st.phi_stage1 = m->phi_stage1;
+  st.phi_next = m->phi_next;
+  st.lse_rhs = m->lse_rhs;
+  st.mask = m->mask;
+  st.narrow_band = m->narrow_band;
+  st.index_x = m->index_x;
+  st.index_y = m->index_y;
+  st.index_z = m->index_z;
+  st.index_outer = m->index_outer;
+  st.num_points = num_points(m);
+  st.gb = lsm::Box{m->gb[0], m->gb[1], m->gb[2], m->gb[3], m->gb[4], m->gb[5]};
+  st.mark_fb = kMarkFb;
+  st.mark_d1 = kMarkD1;
+  st.mark_d2 = kMarkD2;
+  st.levels = kLevel;
+  for (int l = 0; l < 8; l++) {
+    st.prev_lo[l] = l < 6 ? m->n_lo[l] : 0;
+    st.prev_hi[l] = l < 6 ? m->n_hi[l] : 0;
+  }
+  st.a = m->a;
+  st.b = m->b;
+  st.cfl = m->cfl;
+  for (int q = 0; q < 3; q++) st.dx[q] = m->dx[q];
+  st.dt_min_to_correct = kDtMinToCorrect;
+  st.dt_min = kDtMin;
+  return st;
+}
+
+// The boundary-layer marks of band() (no host synchronisation).
+void band_marks(Model* m) {
   int d2[6], d1[6];
   for (int q = 0; q < 6; q++) {
     d2[q] = m->gb[q] + ((q & 1) ? -2 : 2);
@@ -76,40 +111,6 @@ void band(Model* m, const R* phi, const Scalars& s) {
   lsm3dmarknarrowbandboundarylayer_(m->narrow_band, GB6(m), B6(d1), &kMarkD1);
   FAIL_IF_ERROR();
   lsm3dmarknarrowbandboundarylayer_(m->narrow_band, GB6(m), GB6(m), &kMarkGb);
-}
-
-void eno2(Model* m, const R* phi, const Scalars& s) {
-  lsm3dhjeno2local_(m->phi_x_plus, m->phi_y_plus, m->phi_z_plus, GB6(m), m->phi_x_minus, m->phi_y_minus,
-                    m->phi_z_minus, GB6(m), phi, GB6(m), m->D1, GB6(m), m->D2, GB6(m), &s.dx, &s.dy, &s.dz,
-                    m->index_x, m->index_y, m->index_z, &m->n_lo[0], &m->n_hi[0], &m->n_lo[1], &m->n_hi[1],
-                    &m->n_lo[2], &m->n_hi[2], m->narrow_band, GB6(m), &kMarkFb, &kMarkD1, &kMarkD2);
-}
-
-// The fused stages' view of the model after band(): level 0 of the lists,
-// the marks of band(), the model's coefficients.
-lsm::LocalCurvStep fused_step(const Model* m, const R* phi) {
-  lsm::LocalCurvStep st;
-  st.phi = phi;
-  st.phi_stage1 = m->phi_stage1;
-  st.phi_next = m->phi_next;
-  st.lse_rhs = m->lse_rhs;
-  st.narrow_band = m->narrow_band;
-  st.index_x = m->index_x;
-  st.index_y = m->index_y;
-  st.index_z = m->index_z;
-  st.lo0 = m->n_lo[0];
-  st.hi0 = m->n_hi[0];
-  st.gb = lsm::Box{m->gb[0], m->gb[1], m->gb[2], m->gb[3], m->gb[4], m->gb[5]};
-  st.mark_fb = kMarkFb;
-  st.mark_d1 = kMarkD1;
-  st.mark_d2 = kMarkD2;
-  st.a = m->a;
-  st.b = m->b;
-  st.cfl = m->cfl;
-  for (int q = 0; q < 3; q++) st.dx[q] = m->dx[q];
-  st.dt_min_to_correct = kDtMinToCorrect;
-  st.dt_min = kDtMin;
-  return st;
 }
 
 // The model's arrays must be device memory (the calls below take them as such
@@ -145,12 +146,13 @@ double lsm3d_amd_local_curvature_step(LsmLocalCurvatureModel* m, LSMLIB_REAL* ph
   lsm::clear_error();
   if (!model_on_device(m, phi)) return -1.0;
   lsm::AssumeDevice on_device;
-  const Scalars s = scalars(m);
-  band(m, phi, s);
-  FAIL_IF_ERROR(-1.0);
-  // the two stages fused per level-0 point (lsm_local_step.h); dt stays on
-  // the device until the end of the step
+  // band(), its limits kept on the device, then the two stages fused per
+  // level-0 point (lsm_local_step.h); one host synchronisation at the end
   const lsm::LocalCurvStep st = fused_step(m, phi);
+  lsm::local_band(st);
+  FAIL_IF_ERROR(-1.0);
+  band_marks(m);
+  FAIL_IF_ERROR(-1.0);
   lsm::local_curv_stage1(st, dt_sub, tplot);
   FAIL_IF_ERROR(-1.0);
   lsm::local_curv_rk1(st);
@@ -158,7 +160,7 @@ double lsm3d_amd_local_curvature_step(LsmLocalCurvatureModel* m, LSMLIB_REAL* ph
   ghost_fill(m, m->phi_stage1);
   FAIL_IF_ERROR(-1.0);
   if (m->mask) {
-    mask_local(m, m->phi_stage1, m->phi_stage1);
+    lsm::local_mask(st, m->phi_stage1, m->phi_stage1);
     FAIL_IF_ERROR(-1.0);
   }
   lsm::local_curv_stage2(st);
@@ -166,14 +168,32 @@ double lsm3d_amd_local_curvature_step(LsmLocalCurvatureModel* m, LSMLIB_REAL* ph
   ghost_fill(m, m->phi_next);
   FAIL_IF_ERROR(-1.0);
   if (m->mask) {
-    mask_local(m, phi, m->phi_next);
+    lsm::local_mask(st, phi, m->phi_next);
   } else {
     const size_t bytes = (size_t)num_points(m) * sizeof(R);
     lsm::check_hip(hipMemcpyAsync(phi, m->phi_next, bytes, hipMemcpyDeviceToDevice, lsm::current_stream()),
                    "hipMemcpyAsync(phi <- phi_next)");
   }
   FAIL_IF_ERROR(-1.0);
-  return lsm::local_curv_dt(st);
+  lsm::LocalStepResult r;
+  if (!lsm::local_step_result(st, &r)) return -1.0;
+  // the limits LSM3D_DETERMINE_NARROW_BAND returns (lsm_localization3d.f:232-298)
+  for (int l = 0; l <= kLevel; l++) {
+    m->n_lo[l] = r.n_lo[l];
+    m->n_hi[l] = r.n_hi[l];
+  }
+  const int last = num_points(m) - 1;
+  int* ol = m->outer_limits;  // nlo_plus, nhi_plus, nlo_minus, nhi_minus
+  ol[2] = 0;
+  ol[1] = last;
+  if (r.tot[0] > 0) {
+    ol[3] = r.tot[1] - 1;
+    ol[0] = last - r.tot[2] + 1;
+  } else {
+    ol[3] = 0;
+    ol[0] = last;
+  }
+  return r.dt;
 }
 
 int lsm3d_amd_local_curvature_reinit_trigger(LsmLocalCurvatureModel* m, const LSMLIB_REAL* phi, double t,
@@ -181,17 +201,14 @@ int lsm3d_amd_local_curvature_reinit_trigger(LsmLocalCurvatureModel* m, const LS
   lsm::clear_error();
   if (!model_on_device(m, phi)) return -1;
   lsm::AssumeDevice on_device;
-  const Scalars s = scalars(m);
-  int change = 0, zero = 0, last = num_points(m) - 1;
-  int* ol = m->outer_limits;
-  lsm3dcheckouternarrowbandlayer_(&change, phi, GB6(m), m->index_x, m->index_y, m->index_z, &m->n_lo[0],
-                                  &m->n_hi[0], m->index_outer, &zero, &last, &ol[0], &ol[1], &ol[2], &ol[3]);
-  FAIL_IF_ERROR(-1);
-  if (change) return 1;
+  // LSM3D_CHECK_OUTER_NARROW_BAND_LAYER, then LSM3D_COMPUTE_AVE_GRAD_PHI_LOCAL
+  // (evaluated together, one read-back; the average is used only without a
+  // sign change, as in the example)
+  int change = 0;
   R ave = 0;
-  lsm3dcomputeavegradphilocal_(&ave, phi, GB6(m), &s.dx, &s.dy, &s.dz, m->index_x, m->index_y, m->index_z,
-                               &m->n_lo[0], &m->n_hi[0], m->narrow_band, GB6(m), &kMarkFb);
-  FAIL_IF_ERROR(-1);
+  if (!lsm::local_trigger_values(fused_step(m, phi), m->n_lo[0], m->n_hi[0], m->outer_limits, &change, &ave))
+    return -1;
+  if (change) return 1;
   if ((double)ave < kAveGradMin || (double)ave > kAveGradMax) return 2;
   return (t + dt_sub - last_reinit_time >= kTplot) ? 3 : 0;
 }
