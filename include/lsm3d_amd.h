@@ -1026,9 +1026,19 @@ int lsm_amd_real_size(void);
  *           slices of src (src holds 5 * bytes)
  *   kind 2: reads of src[0:bytes); one float per thread is written to dst
  *           (dst holds workgroups * 256 floats)
+ *   kind 3: as kind 1 with 4-byte accesses (one float per lane)
  * bytes must be a positive multiple of 16. */
 double lsm_amd_bandwidth_probe(int kind, const void *src, void *dst, long long bytes, int workgroups,
                                int reps);
+/* The stage kernels' tile march with no arithmetic (a measurement utility;
+ * no counterpart in the reference): 32 x 8 column tiles marching over z-chunks
+ * of kz planes in the stage launch's tile order, reading 5 fields and writing
+ * 1 at every interior cell of an n^3 box, non-temporal 4-byte accesses, no
+ * halo.  src holds 5 consecutive fields and dst 1, each px * py * (n + 6)
+ * floats (3 ghost layers per side; interior cell (0,0,0) at row offset 32 of
+ * row 3 of plane 3).  n is a multiple of 64 and of kz, px >= n + 64,
+ * py >= n + 6.  Returns milliseconds per launch (-1 on error). */
+double lsm_amd_march_probe(const void *src, void *dst, int n, int px, int py, int kz, int reps);
 
 /* ======================================================================
  * Part 3: fused device-resident stages (this library's own API; no

@@ -32,9 +32,8 @@ namespace lsm {
 template <typename T, int TX, int TY, int KIND, bool SYNTH>
 __global__ void __launch_bounds__(TX* TY) k_stage(StageArgs<T> a) {
   __shared__ StageTile<T, TX, TY> tile;
-  const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   int bx, by, bz;
-  tile_coords(a.tg, b, bx, by, bz);
+  launch_tile_coords(a.tg, (int)blockIdx.x, (int)gridDim.x, bx, by, bz);
   if (SYNTH && !tile_is_interior<TX, TY>(a.fb, a.tg, bx, by, bz))
     march_tile<T, TX, TY, KIND, (sizeof(T) == 4), true>(a, bx, by, bz, tile);
   else

@@ -8,9 +8,16 @@
 //                          dst[i] = s0[i] + s1[i] + s2[i] + s3[i] + s4[i],
 //                          s_q = src + q * n
 //   kind 2  reads only:    per-thread sums of src, one float per thread to dst
+//   kind 3  5 reads + 1 write as kind 1 with 4-byte accesses (one float per
+//           lane, the width the stage kernels load with)
+// lsm_amd_march_probe: the stage kernels' tile march with the arithmetic
+// taken out -- 32 x 8 column tiles, z-chunks, the stage launch's tile order,
+// 5 fields read and 1 written at every cell, no halo -- the streaming rate the
+// access pattern itself allows.
 #include <string>
 
 #include "../../include/lsm3d_amd.h"
+#include "lsm_march.h"
 #include "lsm_runtime.h"
 
 namespace lsm {
@@ -47,6 +54,40 @@ __global__ void __launch_bounds__(kProbeThreads) k_probe_5r1w(const v4f* __restr
   }
 }
 
+__global__ void __launch_bounds__(kProbeThreads) k_probe_5r1w_b4(const float* __restrict__ s, float* __restrict__ d,
+                                                                long long n) {
+  const long long stride = (long long)gridDim.x * kProbeThreads;
+  for (long long i = (long long)blockIdx.x * kProbeThreads + threadIdx.x; i < n; i += stride) {
+    const float a = __builtin_nontemporal_load(s + i), b = __builtin_nontemporal_load(s + n + i);
+    const float c = __builtin_nontemporal_load(s + 2 * n + i), e = __builtin_nontemporal_load(s + 3 * n + i);
+    const float f = __builtin_nontemporal_load(s + 4 * n + i);
+    __builtin_nontemporal_store(((a + b) + (c + e)) + f, d + i);
+  }
+}
+
+constexpr int kMarchTX = 32, kMarchTY = 8;
+
+// Fields of (n + 6)^3 cells with 3 ghost cells per side, rows of px floats
+// and planes of px * py floats (interior cell (0,0,0) at row offset 32 of
+// row 3 of plane 3, the padded layout of the stage workloads).
+__global__ void __launch_bounds__(kMarchTX* kMarchTY)
+    k_probe_march(const float* __restrict__ s, float* __restrict__ d, long long field, TileGrid tg, int px,
+                  long long plane) {
+  int bx, by, bz;
+  launch_tile_coords(tg, (int)blockIdx.x, (int)gridDim.x, bx, by, bz);
+  const int t = (int)threadIdx.x;
+  const int i = bx * kMarchTX + t % kMarchTX, j = by * kMarchTY + t / kMarchTX;
+  long long o = (long long)(j + 3) * px + (i + 32) + (long long)(bz * tg.kz + 3) * plane;
+  const int nz = tg.nkz * tg.kz;  // n is a multiple of kz
+  const int k1 = min(tg.kz, nz - bz * tg.kz);
+  for (int k = 0; k < k1; k++, o += plane) {
+    const float v = (__builtin_nontemporal_load(s + o) + __builtin_nontemporal_load(s + field + o)) +
+                    (__builtin_nontemporal_load(s + 2 * field + o) + __builtin_nontemporal_load(s + 3 * field + o)) +
+                    __builtin_nontemporal_load(s + 4 * field + o);
+    __builtin_nontemporal_store(v, d + o);
+  }
+}
+
 __global__ void __launch_bounds__(kProbeThreads) k_probe_read(const v4f* __restrict__ s, float* __restrict__ d,
                                                              long long n) {
   const long long stride = (long long)gridDim.x * kProbeThreads;
@@ -70,7 +111,7 @@ extern "C" {
 double lsm_amd_bandwidth_probe(int kind, const void* src, void* dst, long long bytes, int workgroups, int reps) {
   const char* fn = "lsm_amd_bandwidth_probe";
   Call c(fn);
-  if (kind < 0 || kind > 2) raise_error(std::string(fn) + ": kind is 0, 1 or 2");
+  if (kind < 0 || kind > 3) raise_error(std::string(fn) + ": kind is 0, 1, 2 or 3");
   if (bytes <= 0 || bytes % 16 != 0) raise_error(std::string(fn) + ": bytes must be a positive multiple of 16");
   if (workgroups <= 0 || reps <= 0) raise_error(std::string(fn) + ": workgroups and reps must be positive");
   if (!src || !dst) raise_error(std::string(fn) + ": missing array");
@@ -92,12 +133,55 @@ double lsm_amd_bandwidth_probe(int kind, const void* src, void* dst, long long b
     else if (kind == 1)
       hipLaunchKernelGGL(k_probe_5r1w, dim3((unsigned)workgroups), dim3(kProbeThreads), 0, c.stream(), s,
                          static_cast<v4f*>(dst), n);
+    else if (kind == 3)
+      hipLaunchKernelGGL(k_probe_5r1w_b4, dim3((unsigned)workgroups), dim3(kProbeThreads), 0, c.stream(),
+                         static_cast<const float*>(src), static_cast<float*>(dst), bytes / 4);
     else
       hipLaunchKernelGGL(k_probe_read, dim3((unsigned)workgroups), dim3(kProbeThreads), 0, c.stream(), s,
                          static_cast<float*>(dst), n);
     c.launched("k_probe");
   };
   launch();  // warm-up
+  check_hip(hipEventRecord(e0, c.stream()), "hipEventRecord");
+  for (int r = 0; r < reps && c.ok(); r++) launch();
+  check_hip(hipEventRecord(e1, c.stream()), "hipEventRecord");
+  check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+  float ms = 0.f;
+  check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c.finish();
+  return c.ok() ? (double)ms / reps : -1.0;
+}
+
+// src: 5 consecutive fields, dst: 1 field, each of px * py * (n + 6) floats;
+// n a multiple of 32 (x), 64 (y) and kz.  Returns ms per launch (average of
+// reps after one warm-up launch), or -1 on error.
+double lsm_amd_march_probe(const void* src, void* dst, int n, int px, int py, int kz, int reps) {
+  const char* fn = "lsm_amd_march_probe";
+  Call c(fn);
+  if (n <= 0 || n % kMarchTX || n % (kMarchTY * kNumXcd) || kz <= 0 || n % kz || reps <= 0)
+    raise_error(std::string(fn) + ": n must be a positive multiple of 64 and of kz");
+  if (px < n + 64 || py < n + 6) raise_error(std::string(fn) + ": rows of at least n + 64 floats, n + 6 rows");
+  if (!src || !dst) raise_error(std::string(fn) + ": missing array");
+  if (!c.ok()) return -1.0;
+  if (!(is_device_pointer(src) && is_device_pointer(dst))) {
+    raise_error(std::string(fn) + ": src and dst must be device memory");
+    return -1.0;
+  }
+  const long long plane = (long long)px * py, field = plane * (n + 6);
+  const TileGrid tg = make_tile_grid(Box{0, n - 1, 0, n - 1, 0, n - 1}, kMarchTX, kMarchTY, kz);
+  const int nblk = tg.count();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  check_hip(hipEventCreate(&e0), "hipEventCreate");
+  check_hip(hipEventCreate(&e1), "hipEventCreate");
+  if (!c.ok()) return -1.0;
+  auto launch = [&]() {
+    hipLaunchKernelGGL(k_probe_march, dim3((unsigned)nblk), dim3(kMarchTX * kMarchTY), 0, c.stream(),
+                       static_cast<const float*>(src), static_cast<float*>(dst), field, tg, px, plane);
+    c.launched("k_probe_march");
+  };
+  launch();
   check_hip(hipEventRecord(e0, c.stream()), "hipEventRecord");
   for (int r = 0; r < reps && c.ok(); r++) launch();
   check_hip(hipEventRecord(e1, c.stream()), "hipEventRecord");
