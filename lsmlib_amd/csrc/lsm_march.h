@@ -60,6 +60,21 @@ __device__ __forceinline__ T extrapolate(int code, T pb, T p1, int dist) {
   return pb + slope * T(dist);
 }
 
+// extrapolate() without branches, for codes that vary between lanes: code 0
+// (a stored cell or a periodic image resolved to its source) and code 1
+// return pb; codes 2 and 3 the same roundings as extrapolate() (the slope
+// sign(pb) |pb - p1| is copysign(|pb - p1|, pb) exactly).  Selects instead of
+// branches keep the loads of a plane free of control flow, so the next
+// plane's loads stay in flight while a plane is computed.
+template <typename T>
+__device__ __forceinline__ T extrapolate_sel(int code, T pb, T p1, int dist) {
+#pragma clang fp contract(off)
+  const T d = pb - p1;
+  const T slope = (code == 3) ? copysign(fabs(d), pb) : d;
+  const T e = pb + slope * T(dist);
+  return (code >= 2) ? e : pb;
+}
+
 // Writes `val` at (i,j,k) and, for each axis in `periodic` whose image of
 // the cell lies in the g-wide ghost layer, at the periodic images (up to 7
 // more cells).
@@ -282,13 +297,6 @@ __device__ __forceinline__ LoadRule load_rule(const Field<const T>& f, const Box
   return {o, o, 0, 0};
 }
 
-template <typename T, bool SYNTH>
-__device__ __forceinline__ T rule_load(__amdgpu_buffer_rsrc_t r, const LoadRule& lr) {
-  T v = buf_load<T>(r, lr.a);
-  if (SYNTH && lr.code) v = extrapolate<T>(lr.code, v, buf_load<T>(r, lr.b), lr.dist);
-  return v;
-}
-
 // A halo value loaded one plane ahead: both cells of the rule are fetched
 // (b = a when no extrapolation applies) and combined only when consumed, so
 // the loads stay in flight across the previous plane's arithmetic.
@@ -300,9 +308,20 @@ struct Pending {
     if (SYNTH) vb = buf_load<T>(r, lr.b);
   }
   __device__ __forceinline__ T value(const LoadRule& lr) const {
-    if (SYNTH && lr.code) return extrapolate<T>(lr.code, va, vb, lr.dist);
+    if (SYNTH) return extrapolate_sel<T>(lr.code, va, vb, lr.dist);
     return va;
   }
+};
+
+// The planes a column value of plane kk is formed from (uniform): both kk
+// inside the fill box, where the lane's own-column rule applies, or, past a
+// synthesised z face, the face plane and the one next to it (or the periodic
+// source plane) at the lane's own column with the face's code.
+struct PlaneRule {
+  const char* pa;
+  const char* pb;
+  int code, dist;
+  bool face;
 };
 
 // IMAGES = false compiles out the periodic-image stores (callers whose
@@ -384,38 +403,67 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
   // the column's value in plane kk, applying the z-face rule outside the fill
   // box (kk is uniform; there the value only feeds the column's own z
   // derivative, so lanes outside the fill box may read any stored cell)
-  auto zval = [&](int kk) -> T {
+  auto zrule = [&](int kk) -> PlaneRule {
     if (SYNTH && kk < fb.klo && g.code[4]) {
-      if (g.code[4] == 5) return buf_load<T>(plane_rsrc(plane_ptr<T>(a.ps, kk + fb.nz())), oc);
-      return extrapolate<T>(g.code[4], buf_load<T>(plane_rsrc(plane_ptr<T>(a.ps, fb.klo)), oc),
-                            buf_load<T>(plane_rsrc(plane_ptr<T>(a.ps, fb.klo + 1)), oc), fb.klo - kk);
+      if (g.code[4] == 5) {
+        const char* p = plane_ptr<T>(a.ps, kk + fb.nz());
+        return {p, p, 0, 0, true};
+      }
+      return {plane_ptr<T>(a.ps, fb.klo), plane_ptr<T>(a.ps, fb.klo + 1), g.code[4], fb.klo - kk, true};
     }
     if (SYNTH && kk > fb.khi && g.code[5]) {
-      if (g.code[5] == 5) return buf_load<T>(plane_rsrc(plane_ptr<T>(a.ps, kk - fb.nz())), oc);
-      return extrapolate<T>(g.code[5], buf_load<T>(plane_rsrc(plane_ptr<T>(a.ps, fb.khi)), oc),
-                            buf_load<T>(plane_rsrc(plane_ptr<T>(a.ps, fb.khi - 1)), oc), kk - fb.khi);
+      if (g.code[5] == 5) {
+        const char* p = plane_ptr<T>(a.ps, kk - fb.nz());
+        return {p, p, 0, 0, true};
+      }
+      return {plane_ptr<T>(a.ps, fb.khi), plane_ptr<T>(a.ps, fb.khi - 1), g.code[5], kk - fb.khi, true};
     }
-    return rule_load<T, SYNTH>(plane_rsrc(plane_ptr<T>(a.ps, kk)), rc);
+    const char* p = plane_ptr<T>(a.ps, kk);
+    return {p, p, 0, 0, false};
+  };
+  // the column's value of plane kk: both cells loaded, the rule applied when
+  // the value is consumed (SYNTH); otherwise the stored cell
+  struct ColValue {
+    T va, vb;
+    int code, dist;
+  };
+  auto col_load = [&](int kk, ColValue& c) {
+    const PlaneRule z = zrule(kk);
+    c.va = buf_load<T>(plane_rsrc(z.pa), z.face ? oc : rc.a);
+    if (SYNTH) {
+      c.vb = buf_load<T>(plane_rsrc(z.pb), z.face ? oc : rc.b);
+      c.code = z.face ? z.code : rc.code;
+      c.dist = z.face ? z.dist : rc.dist;
+    }
+  };
+  auto col_value = [&](const ColValue& c) -> T {
+    if (SYNTH) return extrapolate_sel<T>(c.code, c.va, c.vb, c.dist);
+    return c.va;
   };
   {
-    T z0[6];
+    ColValue z0[6];
 #pragma unroll
-    for (int q = 0; q < 6; q++) z0[q] = zval(k0 - 3 + q);
+    for (int q = 0; q < 6; q++) col_load(k0 - 3 + q, z0[q]);
+    T z1[6];
 #pragma unroll
-    for (int q = 0; q < 5; q++) zd[q] = z0[q + 1] - z0[q];
+    for (int q = 0; q < 6; q++) z1[q] = col_value(z0[q]);
 #pragma unroll
-    for (int q = 0; q < 3; q++) zv[q] = z0[q + 3];
+    for (int q = 0; q < 5; q++) zd[q] = z1[q + 1] - z1[q];
+#pragma unroll
+    for (int q = 0; q < 3; q++) zv[q] = z1[q + 3];
   }
   // plane values of the next plane to process (halo and top values pending)
-  Pending<T, SYNTH> phx, phy[NR], ptop;
-  bool top_z = false;  // the pending top value came from the z-face rule
+  Pending<T, SYNTH> phx, phy[NR];
+  ColValue ptop;
   T ux = T(0), uy = T(0), uz = T(0), pcv = T(0), p0v = T(0);
   // loads plane kn's values (and the column value 3 planes up)
   auto fetch = [&](int kn) {
     const __amdgpu_buffer_rsrc_t rps = plane_rsrc(p_ps);
-    top_z = SYNTH && kn + 3 > fb.khi;  // uniform
-    if (top_z) ptop.va = zval(kn + 3);
-    else ptop.load(plane_rsrc(p_top), rc);
+    if (SYNTH) {
+      col_load(kn + 3, ptop);
+    } else {
+      ptop.va = buf_load<T>(plane_rsrc(p_top), oc);
+    }
     phx.load(rps, rx);
 #pragma unroll
     for (int n = 0; n < NR; n++) phy[n].load(rps, ry[n]);
@@ -453,7 +501,7 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
       if (k > k1) break;  // uniform
       T (*tile)[TX + 7] = tiles[U == 2 ? u : par];
       par ^= 1;
-      zv[u + 3] = top_z ? ptop.va : ptop.value(rc);
+      zv[u + 3] = col_value(ptop);
       const T hx = phx.value(rx);
       T hy[NR];
 #pragma unroll
@@ -579,20 +627,21 @@ __device__ __forceinline__ void tile_coords(const TileGrid& tg, int b, int& bx, 
 
 // Tile of workgroup b of a one-tile-per-workgroup launch.  The hardware hands
 // workgroup b to XCD b % 8.  When the tile rows split evenly over the 8 XCDs,
-// XCD x owns tile rows [x nty/8, (x+1) nty/8) of every z-chunk, and all XCDs
-// work through the z-chunks together: the tiles in flight then cover one
-// z-slab of the grid (HBM pages of few planes at a time; 4 % faster than
-// contiguous tile ranges per XCD for the 512^3 advection stage), while a
-// tile's x and y neighbours stay in its XCD's L2.  Otherwise each XCD takes a
-// contiguous range of tiles (xcd_remap).  A bijection on [0, count) in both
-// cases.
+// they form 8 slabs of nty/8 rows, and in z-chunk bz XCD x owns slab
+// (x + bz) % 8: all XCDs work through the z-chunks together, so the tiles in
+// flight cover one z-slab of the grid (HBM pages of few planes at a time; 4 %
+// faster than contiguous tile ranges per XCD for the 512^3 advection stage),
+// a tile's x and y neighbours stay in its XCD's L2, and the slabs at the y
+// faces (whose tiles synthesise or write ghost cells) rotate over the XCDs
+// instead of loading two of them.  Otherwise each XCD takes a contiguous
+// range of tiles (xcd_remap).  A bijection on [0, count) in both cases.
 __device__ __forceinline__ void launch_tile_coords(const TileGrid& tg, int b, int nblk, int& bx, int& by, int& bz) {
   if (tg.nty % kNumXcd == 0) {
     const int x = b % kNumXcd, m = b / kNumXcd;
     const int rows = tg.nty / kNumXcd, per = tg.ntx * rows;
     bz = m / per;
     const int r = m - bz * per;
-    by = x * rows + r / tg.ntx;
+    by = ((x + bz) % kNumXcd) * rows + r / tg.ntx;
     bx = r % tg.ntx;
   } else {
     tile_coords(tg, xcd_remap(b, nblk), bx, by, bz);
