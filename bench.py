@@ -341,6 +341,7 @@ def run_adjoint(n=128, nsteps=40, every=8, reps=3, fd=True):
                      f"{nsteps} WENO5/TVD-RK3 steps, dt={dt:.6g} frozen, checkpoint every {every}, "
                      f"loss sum (H(phi_T)-H(phi_tgt))^2 eps=1.5dx"}
     grads, losses = {}, {}
+    fd_dirs = []  # (name, c direction or None, phi0 direction or None, central difference of the FP64 loss)
     for real, dtype in (("double", torch.float64), ("float", torch.float32)):
         # the FP32 runner uses the double build's upwind switch (1e-11), so
         # both gradients see V_a-derivatives on the same cells
@@ -386,25 +387,37 @@ def run_adjoint(n=128, nsteps=40, every=8, reps=3, fd=True):
                 for _ in range(8):
                     e = torch.tensor(rng.normal(size=tuple(c0.shape)), dtype=torch.float64, device=dev)
                     e /= e.norm()
-                    ad = float((gc * e).sum())
-                    checks.append(("c_random", abs(fd_dir(dc=e) - ad) / abs(ad)))
+                    ad, fdv = float((gc * e).sum()), fd_dir(dc=e)
+                    checks.append(("c_random", abs(fdv - ad) / abs(ad)))
+                    fd_dirs.append(("c_random", e, None, fdv))
                 top = torch.argsort(gc.abs().flatten(), descending=True)[:8]
                 for idx in top.tolist():
                     e = torch.zeros_like(c0).flatten()
                     e[idx] = 1.0
                     e = e.reshape(c0.shape)
-                    ad = float(gc.flatten()[idx])
-                    checks.append((f"c[{idx // 3},{idx % 3}]", abs(fd_dir(dc=e) - ad) / abs(ad)))
+                    ad, fdv = float(gc.flatten()[idx]), fd_dir(dc=e)
+                    checks.append((f"c[{idx // 3},{idx % 3}]", abs(fdv - ad) / abs(ad)))
+                    fd_dirs.append((f"c[{idx // 3},{idx % 3}]", e, None, fdv))
                 X, Y, Z = grid.centers(torch.float64, dev)
                 for w in ((1.0, 2.0, 0.5), (3.0, 1.0, 2.0)):
                     e = torch.sin(np.pi * (w[0] * X + w[1] * Y) + w[2] * Z) * torch.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 0.2)
-                    ad = float((gp * e).sum())
-                    checks.append(("phi0_bump", abs(fd_dir(dp=e) - ad) / abs(ad)))
+                    ad, fdv = float((gp * e).sum()), fd_dir(dp=e)
+                    checks.append(("phi0_bump", abs(fdv - ad) / abs(ad)))
+                    fd_dirs.append(("phi0_bump", None, e, fdv))
                 del X, Y, Z
             out["fd_check_fp64"] = {"max_rel_err": max(v for _, v in checks), "n_directions": len(checks),
                                     "h": 1e-4, "per_direction": {f"{k}#{q}": v for q, (k, v) in enumerate(checks)}}
         del runner, target
         torch.cuda.empty_cache()
+    if fd_dirs:
+        # the FP32 gradient (FP32 forward, checkpoints and adjoint) along the
+        # same directions against the same central differences of the FP64 loss
+        g32c, g32p = grads["float"]
+        errs = [(k, abs(fdv - float((g32c * dc).sum() if dc is not None else (g32p * dp).sum())) / abs(fdv))
+                for k, dc, dp, fdv in fd_dirs]
+        out["fd_check_fp32"] = {"max_rel_err": max(v for _, v in errs), "n_directions": len(errs),
+                                "reference": "central differences of the FP64 loss (h 1e-4)",
+                                "per_direction": {f"{k}#{q}": v for q, (k, v) in enumerate(errs)}}
     out["fp32_grad_rel_err_vs_fp64"] = {
         "c": float((grads["float"][0] - grads["double"][0]).norm() / grads["double"][0].norm()),
         "phi0": float((grads["float"][1] - grads["double"][1]).norm() / grads["double"][1].norm())}
