@@ -1062,7 +1062,17 @@ double lsm_amd_march_probe(const void *src, void *dst, int n, int px, int py, in
  * ghost fill is needed between stages.
  * periodic_mask bit d (x=1, y=2, z=4): the stage also writes phi_out's
  * periodic ghost images along axis d (3-cell ghost layer), for callers that
- * read phi_out's ghost cells afterwards. */
+ * read phi_out's ghost cells afterwards.
+ * periodic_mask bit 8 (LSM_AMD_WRITE_FACE_GHOSTS): bc[6] then describes
+ * phi_out's faces instead -- the stage reads phi_stage's stored ghost cells
+ * and writes phi_out's face ghost layers: periodic images (code 5) or the
+ * extrapolation (codes 1-3) the face fill stores for the face layers (edge
+ * and corner ghost cells, which no stage reads, are left as they are), so a
+ * sequence of stages needs one lsm3d_amd_fill_ghosts before the first.  Bits
+ * 0-2 are ignored.  Where the tiling cannot write them (the last tile along x
+ * or y, or the last z-chunk, one cell wide along an extrapolated axis, or
+ * one-plane z-chunks), the stage fills phi_out's ghost box after it instead. */
+#define LSM_AMD_WRITE_FACE_GHOSTS 8
 #define LSM_AMD_RK1 1
 #define LSM_AMD_TVDRK2_STAGE2 22
 #define LSM_AMD_TVDRK3_STAGE2 32

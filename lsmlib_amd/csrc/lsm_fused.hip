@@ -175,6 +175,21 @@ void launch_stage_kind(hipStream_t s, const StageArgs<R>& a, int cfg, int nblk) 
   }
 }
 
+// The ghost box of phi outside the fill box, face by face in the order x, y,
+// z (the reference's fill order; codes already validated).
+void fill_ghost_box(Call& c, R* phi, const int* gb, const int* fb, const int* bc) {
+  const Box g = box6(gb), f = box6(fb);
+  Field<R> fp = make_field<R>(phi, g);
+  const int ext[3] = {g.nx(), g.ny(), g.nz()};
+  for (int d = 0; d < 3; d++) {
+    if (bc[2 * d] == 0 && bc[2 * d + 1] == 0) continue;
+    const int u = (d + 1) % 3, v = (d + 2) % 3;
+    dim3 grid((unsigned)((ext[u] + 63) / 64), (unsigned)((ext[v] + 3) / 4), 2);
+    hipLaunchKernelGGL(k_fill_axis<R>, grid, dim3(64, 4, 1), 0, c.stream(), fp, g, f, d, bc[2 * d], bc[2 * d + 1]);
+    c.launched("k_fill_axis");
+  }
+}
+
 // Shared by the advection and reinitialization stage entry points.
 void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, const R* ps,
                const int* ps_gb, const R* pc, const int* pc_gb, const R* vx, const R* vy, const R* vz,
@@ -197,7 +212,29 @@ void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, co
       raise_error(std::string(fn) + ": alpha must be non-negative");
   }
   if (kind == STAGE_REINIT && use_phi0) require_inside(fn, "phi0", box6(phi0_gb), fb, 0, 0, 0);
-  const GhostRule ghost = make_ghost_rule(fn, bc, fb);
+  // bit 8: bc gives phi_out's face ghost codes; phi_stage's stored ghosts are read
+  const bool face_out = (periodic & 8) != 0;
+  if (face_out && !bc) raise_error(std::string(fn) + ": writing face ghosts (periodic_mask bit 8) needs bc");
+  GhostRule ghost = make_ghost_rule(fn, bc, fb);
+  GhostRule oghost{};
+  bool fill_after = false;  // the tiling cannot write the face ghosts: fill phi_out's ghosts afterwards
+  if (face_out && bc) {
+    periodic = (bc[0] == 5 ? 1 : 0) | (bc[2] == 5 ? 2 : 0) | (bc[4] == 5 ? 4 : 0);
+    const TileShape ts = kTiles[tl_tiling.cfg];
+    for (int f = 0; f < 6; f++) oghost.code[f] = (ghost.code[f] == 5) ? 0 : ghost.code[f];
+    oghost.any = oghost.code[0] | oghost.code[1] | oghost.code[2] | oghost.code[3] | oghost.code[4] | oghost.code[5];
+    // a hi face cell and its inward neighbour must share a tile (x, y) and a
+    // z-chunk (z): the last tile or chunk along an extrapolated axis is not one
+    // cell wide, and z-chunks have two planes at least
+    const int kz = tl_tiling.kz;
+    if ((oghost.code[1] && fb.nx() % ts.tx == 1) || (oghost.code[3] && fb.ny() % ts.ty == 1) ||
+        ((oghost.code[4] || oghost.code[5]) && (kz < 2 || fb.nz() % kz == 1))) {
+      fill_after = oghost.any != 0;
+      oghost = GhostRule{};
+    }
+    ghost = GhostRule{};
+    require_inside(fn, "phi_out", box6(out_gb), fb, 3, 3, 3);
+  }
   periodic &= 7;
   if (periodic) {
     require_inside(fn, "phi_out", box6(out_gb), fb, 3, 3, 3);
@@ -245,6 +282,7 @@ void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, co
   a.form = form;
   a.rkb = rk_blend_weight<R>(form);
   a.periodic = periodic;
+  a.oghost = oghost;
   a.use_phi0 = use_phi0 ? 1 : 0;
   a.alpha_mode = (kind == STAGE_ADVECT_LF) ? alpha_mode : 0;
   for (int d = 0; d < 3; d++) a.alpha[d] = (a.alpha_mode && alpha) ? alpha[d] : R(0);
@@ -259,6 +297,7 @@ void run_stage(const char* fn, int kind, int form, R* out, const int* out_gb, co
   else if (kind == STAGE_ADVECT_LF) launch_stage_kind<STAGE_ADVECT_LF>(c.stream(), a, tl.cfg, (int)nblk);
   else launch_stage_kind<STAGE_REINIT>(c.stream(), a, tl.cfg, (int)nblk);
   c.launched("k_stage");
+  if (fill_after) fill_ghost_box(c, out, out_gb, fbp, bc);
   c.finish();
 }
 
@@ -352,18 +391,7 @@ void lsm3d_amd_fill_ghosts(LSMLIB_REAL* phi, const int gb[6], const int fb[6], c
       if (code == 5 && gw[d][s] > n[d]) raise_error(std::string(fn) + ": periodic ghost wider than the box");
     }
   device_only(fn, phi, "phi");
-  if (c.ok()) {
-    Field<real_t> fp = make_field<real_t>(phi, g);
-    const int ext[3] = {g.nx(), g.ny(), g.nz()};
-    for (int d = 0; d < 3; d++) {  // x faces, then y, then z, as faces 0..5 are applied in order
-      if (bc[2 * d] == 0 && bc[2 * d + 1] == 0) continue;
-      const int u = (d + 1) % 3, v = (d + 2) % 3;
-      dim3 grid((unsigned)((ext[u] + 63) / 64), (unsigned)((ext[v] + 3) / 4), 2);
-      hipLaunchKernelGGL(k_fill_axis<real_t>, grid, dim3(64, 4, 1), 0, c.stream(), fp, g, f, d, bc[2 * d],
-                         bc[2 * d + 1]);
-      c.launched("k_fill_axis");
-    }
-  }
+  if (c.ok()) fill_ghost_box(c, phi, gb, fb, bc);
   c.finish();
 }
 

@@ -75,6 +75,17 @@ __device__ __forceinline__ T extrapolate_sel(int code, T pb, T p1, int dist) {
   return (code >= 2) ? e : pb;
 }
 
+// The three ghost cells beyond a face cell (i,j,k) of `out` in direction
+// (di,dj,dk): extrapolate(code, pb, p1, d) at distance d = 1..3 from the face
+// value pb and its inward neighbour p1, the values the reference's face fill
+// stores there (lsm_boundary_conditions3d.f:66-77, 213-225).
+template <typename T>
+__device__ __forceinline__ void write_face_ghosts(const Field<T>& out, int code, T pb, T p1, int i, int j, int k,
+                                                  int di, int dj, int dk) {
+#pragma unroll
+  for (int d = 1; d <= 3; d++) out.at(i + d * di, j + d * dj, k + d * dk) = extrapolate<T>(code, pb, p1, d);
+}
+
 // Writes `val` at (i,j,k) and, for each axis in `periodic` whose image of
 // the cell lies in the g-wide ghost layer, at the periodic images (up to 7
 // more cells).
@@ -131,6 +142,7 @@ struct StageArgs {
   int form;      // RKForm
   T rkb;         // rk_blend_weight(form), used by the FP32 (FASTW) stages
   int periodic;  // bit mask: also write phi_out's periodic ghost images
+  GhostRule oghost;  // extrapolation codes (1-3) of phi_out's face ghost layers to write (IMAGES)
   int use_phi0;  // reinit sign from phi0 (1) or phi_s (0)
   int alpha_mode;  // Lax-Friedrichs stages: 0 alpha = |v_a| per cell, 1 the global alpha[a]
   T alpha[3];
@@ -378,6 +390,18 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
   // periodic images: only tiles within 3 cells of a periodic x or y face
   const bool img_xy = IMAGES && (((a.periodic & 1) && (i0 - fb.ilo < 3 || fb.ihi - (i0 + TX - 1) < 3)) ||
                                  ((a.periodic & 2) && (j0 - fb.jlo < 3 || fb.jhi - (j0 + TY - 1) < 3)));
+  // extrapolated face ghosts of phi_out: the tiles holding the face cells
+  // write them from the face cell and its inward neighbour, which the host
+  // guarantees lie in the same tile (x, y) and z-chunk (z)
+  const int* oc6 = a.oghost.code;
+  const bool f_xlo = IMAGES && a.oghost.any && oc6[0] && i0 == fb.ilo;
+  const bool f_xhi = IMAGES && a.oghost.any && oc6[1] && i0 + TX - 1 >= fb.ihi;
+  const bool f_ylo = IMAGES && a.oghost.any && oc6[2] && j0 == fb.jlo;
+  const bool f_yhi = IMAGES && a.oghost.any && oc6[3] && j0 + TY - 1 >= fb.jhi;
+  const bool f_xy = f_xlo || f_xhi || f_ylo || f_yhi;
+  const bool f_z = IMAGES && a.oghost.any && ((oc6[4] && k0 == fb.klo) || (oc6[5] && k1 == fb.khi));
+  __shared__ T out_plane[IMAGES ? TY : 1][IMAGES ? TX : 1];
+  T v_prev = T(0);  // phi_out of the column in the previous plane
 
   // plane pointers, advanced by one plane per step
   const int64_t s_ps = a.ps.sz * (int64_t)sizeof(T), s_out = a.out.sz * (int64_t)sizeof(T);
@@ -519,6 +543,7 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
       if (k < k1) advance();  // uniform
       fetch(k < k1 ? k + 1 : k1);
 
+      T v = T(0);
       if (active) {
         T dx_[6], dy_[6];
         {
@@ -576,11 +601,28 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
           // lsm_level_set_evolution3d.f.in:113-116 on a zeroed rhs
           rhs = FASTW ? -fma(cuz, gz, fma(cuy, gy, cux * gx)) : -(cux * gx + cuy * gy + cuz * gz);
         }
-        const T v = FASTW ? rk_blend<T>(uc, us, rhs, a.dt, a.rkb) : rk_combine<T>(a.form, uc, us, rhs, a.dt);
+        v = FASTW ? rk_blend<T>(uc, us, rhs, a.dt, a.rkb) : rk_combine<T>(a.form, uc, us, rhs, a.dt);
         const bool img_z = IMAGES && (a.periodic & 4) && (k - fb.klo < 3 || fb.khi - k < 3);
         if (img_xy || img_z) store_with_periodic_images<T>(a.out, fb, i, j, k, v, a.periodic, 3);
         else buf_store(v, plane_rsrc(p_out), oo);
       }
+      if (f_xy) {  // uniform per tile
+        if (active) out_plane[ty][tx] = v;
+        __syncthreads();
+        if (active) {
+          if (f_xlo && i == fb.ilo) write_face_ghosts<T>(a.out, oc6[0], v, out_plane[ty][tx + 1], i, j, k, -1, 0, 0);
+          if (f_xhi && i == fb.ihi) write_face_ghosts<T>(a.out, oc6[1], v, out_plane[ty][tx - 1], i, j, k, 1, 0, 0);
+          if (f_ylo && j == fb.jlo) write_face_ghosts<T>(a.out, oc6[2], v, out_plane[ty + 1][tx], i, j, k, 0, -1, 0);
+          if (f_yhi && j == fb.jhi) write_face_ghosts<T>(a.out, oc6[3], v, out_plane[ty - 1][tx], i, j, k, 0, 1, 0);
+        }
+      }
+      if (f_z && active) {
+        // below the bottom face from planes klo (kept) and klo + 1; above the
+        // top face from planes khi and khi - 1 (kept)
+        if (oc6[4] && k == fb.klo + 1) write_face_ghosts<T>(a.out, oc6[4], v_prev, v, i, j, fb.klo, 0, 0, -1);
+        if (oc6[5] && k == fb.khi) write_face_ghosts<T>(a.out, oc6[5], v, v_prev, i, j, k, 0, 0, 1);
+      }
+      v_prev = v;
       p_out += s_out;
     }
 #pragma unroll

@@ -128,13 +128,17 @@ class Stepper:
 
     bc: 6 boundary-condition codes (x_lo, x_hi, y_lo, y_hi, z_lo, z_hi).
     ghosts="auto" (default) picks "images" for all-periodic boundaries and
-    "synth" otherwise.  ghosts="synth": each stage synthesises the ghost values it
-    reads from the fill box, so a step is exactly three launches and ghost
-    cells of the outputs are not maintained.  ghosts="fill": a device ghost
-    fill runs after every stage and the stages read stored ghost cells (the
-    reference's fill-then-differentiate order).  ghosts="images" (all faces
-    periodic only): the stages write the periodic ghost images of their
-    output and read stored ghosts.  Results are identical in every mode.
+    "synth" otherwise (synth needs no valid ghost cells on entry).
+    ghosts="images": the stages read stored ghost cells and write the face
+    ghost layers of their output -- periodic images, or the extrapolated
+    layers a face fill stores (edge and corner ghost cells of extrapolated
+    faces, which no stage reads, are not maintained) -- so the input needs
+    valid ghosts once, before the first stage (fill_ghosts).
+    ghosts="synth": each stage synthesises the ghost values it reads from
+    the fill box and ghost cells of the outputs are not maintained.
+    ghosts="fill": a device ghost fill runs after every stage and the stages
+    read stored ghost cells (the reference's fill-then-differentiate order).
+    Results are identical in every mode.
 
     zero_tol: threshold of the upwind switch (|V| < zero_tol: no advection
     term) and of the reinitialization cut-off; None keeps the build's
@@ -148,7 +152,7 @@ class Stepper:
 
     def __init__(self, grid, real="double", bc=(BC_PERIODIC,) * 6, device="cuda", ghosts="auto", zero_tol=None,
                  tiling="auto"):
-        if ghosts == "auto":  # the fastest mode for the boundary conditions
+        if ghosts == "auto":
             ghosts = "images" if all(int(b) == BC_PERIODIC for b in bc) else "synth"
         if ghosts not in ("synth", "fill", "images"):
             raise ValueError(ghosts)
@@ -163,10 +167,9 @@ class Stepper:
         self._fb = box_c(grid.fb)
         self._bc = box_c(self.bc)
         self.all_periodic = all(b == BC_PERIODIC for b in self.bc)
-        if ghosts == "images" and not self.all_periodic:
-            raise ValueError("ghosts='images' needs periodic faces")
-        self._stage_bc = self._bc if ghosts == "synth" else None
-        self.periodic_mask = 7 if ghosts == "images" else 0
+        self._stage_bc = self._bc if ghosts in ("synth", "images") else None
+        # bit 8: bc gives the output's face ghost codes, the stage input's ghosts are stored
+        self.periodic_mask = 8 if ghosts == "images" else 0
         self.zero_tol = zero_tol
         if tiling == "auto":
             from .tuning import choose_tiling

@@ -87,7 +87,7 @@ def test_advect_rk3_periodic_fp64(ghosts):
     assert _rel(got, want) <= FP64_TOL
 
 
-@pytest.mark.parametrize("ghosts", ["synth", "fill"])
+@pytest.mark.parametrize("ghosts", ["synth", "fill", "images"])
 def test_advect_rk3_signed_linear_fp64(ghosts):
     grid, phi, vel, bc = rotating_sphere(20)
     dt = 0.5 * grid.dx[0] / 1.8
@@ -103,16 +103,18 @@ def test_synthesised_ghosts_equal_filled_ghosts(real):
     dt = 0.01
     dt_ = torch.float64 if real == "double" else torch.float32
     outs = []
-    for ghosts in ("synth", "fill"):
+    for ghosts in ("synth", "fill", "images"):
         outs.append(run_both(grid, phi.to(dt_), tuple(v.to(dt_) for v in vel), bc, 2, real, dt, ghosts=ghosts)[0])
     assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[2])
 
 
+@pytest.mark.parametrize("ghosts", ["synth", "images"])
 @pytest.mark.parametrize("bc", [BC_LINEAR, BC_NEUMANN])
-def test_advect_rk3_other_bcs_fp64(bc):
+def test_advect_rk3_other_bcs_fp64(bc, ghosts):
     grid, phi, vel, _ = rotating_sphere(16)
     dt = 0.4 * grid.dx[0]
-    got, want = run_both(grid, phi, vel, (bc,) * 6, 3, "double", dt)
+    got, want = run_both(grid, phi, vel, (bc,) * 6, 3, "double", dt, ghosts=ghosts)
     assert _rel(got, want) <= FP64_TOL
 
 
@@ -129,12 +131,60 @@ def test_advect_const_velocity_fp64():
     assert _rel(got, want) <= FP64_TOL
 
 
+@pytest.mark.parametrize("ghosts", ["synth", "images"])
 @pytest.mark.parametrize("use_phi0", [False, True])
-def test_reinit_rk3_fp64(use_phi0):
+def test_reinit_rk3_fp64(use_phi0, ghosts):
     grid, phi, bc = wavy_sphere_reinit(20)
     got, want = run_both(grid, phi, None, bc, 3, "double", 0.5 * grid.dx[0], mode="reinit",
-                         use_phi0=use_phi0)
+                         use_phi0=use_phi0, ghosts=ghosts)
     assert _rel(got, want) <= FP64_TOL
+
+
+@pytest.mark.parametrize("real", ["double", "float"])
+def test_face_ghost_writes_for_every_tiling(real):
+    """Stages that write their output's extrapolated face ghosts (and read
+    stored ghosts) step the fill box exactly as stages that synthesise the
+    ghost values, for every tile shape and z-chunk -- including the tilings
+    whose last tile or chunk is one cell wide, where the face ghosts come from
+    a ghost fill after the stage -- and the face ghost layers they write equal
+    those of the reference's fill."""
+    dev = torch.device("cuda:0")
+    dt_ = torch.float64 if real == "double" else torch.float32
+    grid = Grid((37, 29, 23), (0.0, 0.0, 0.0), (1.0, 0.8, 0.6), 3)
+    phi = sphere_union(grid, n_spheres=4, seed=7, rmin=0.12, rmax=0.25, dtype=dt_).to(dev)
+    vel = tuple(v.to(dev) for v in taylor_green(grid, dt_))
+    bc = (BC_SIGNED_LINEAR, BC_LINEAR, BC_SIGNED_LINEAR, BC_NEUMANN, BC_LINEAR, BC_SIGNED_LINEAR)
+    lib = toolbox(real)
+    ref_st = Stepper(grid, real, bc, ghosts="synth", tiling=(2, 8))
+    p = phi.clone()
+    s1, s2 = ref_st.empty(), ref_st.empty()
+    ref_st.advect_rk3_step(p, s1, s2, vel, 0.01)
+    ref_st.reinit_rk3_iteration(p, s1, s2, 0.005)
+    ref = grid.interior(_np(p))
+    st = Stepper(grid, real, bc, ghosts="images", tiling=None)  # the loop below sets the tiling
+    try:
+        for cfg in range(lib.lib.lsm3d_amd_stage_tiling_count()):
+            for kz in (1, 7, 64):
+                lib.lib.lsm3d_amd_set_stage_tiling(cfg, kz)
+                p = phi.clone()
+                st.fill_ghosts(p)
+                s1, s2 = st.empty(), st.empty()
+                st.advect_rk3_step(p, s1, s2, vel, 0.01)
+                st.reinit_rk3_iteration(p, s1, s2, 0.005)
+                torch.cuda.synchronize()
+                assert np.array_equal(grid.interior(_np(p)), ref), f"tiling {cfg}/{kz} differs"
+                # the face ghost layers written by the last stage equal a fresh fill's
+                q = p.clone()
+                st.fill_ghosts(q)
+                torch.cuda.synchronize()
+                a, b = _np(p), _np(q)
+                g = 3
+                gi, gj, gk = (g + grid.n[0], g + grid.n[1], g + grid.n[2])
+                assert np.array_equal(a[g:gk, g:gj, :], b[g:gk, g:gj, :]), f"tiling {cfg}/{kz}: x face ghosts"
+                assert np.array_equal(a[g:gk, :, g:gi], b[g:gk, :, g:gi]), f"tiling {cfg}/{kz}: y face ghosts"
+                assert np.array_equal(a[:, g:gj, g:gi], b[:, g:gj, g:gi]), f"tiling {cfg}/{kz}: z face ghosts"
+    finally:
+        lib.lib.lsm3d_amd_set_stage_tiling(0, 32)
 
 
 @pytest.mark.parametrize("ghosts", ["synth", "images"])
