@@ -383,8 +383,8 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
   }
   const int txl = (tx < 3) ? tx : TX + tx;
   const uint32_t ov = reads_velocity<KIND>() ? plane_byte_offset<T>(a.vx, ic, jc) : 0u;
-  const uint32_t opc = a.pc.p ? plane_byte_offset<T>(a.pc, ic, jc) : 0u;
-  const uint32_t op0 = (KIND == STAGE_REINIT && a.use_phi0) ? plane_byte_offset<T>(a.phi0, ic, jc) : 0u;
+  const uint32_t opc = a.pc.p ? plane_byte_offset<T>(a.pc, ic, jc) : oc;
+  const uint32_t op0 = (KIND == STAGE_REINIT && a.use_phi0) ? plane_byte_offset<T>(a.phi0, ic, jc) : oc;
   const uint32_t oo = (uint32_t)((ic - a.out.ilo) + (int)a.out.sy * (jc - a.out.jlo)) * (uint32_t)sizeof(T);
   const T tiny_dx2x = a.tdx2[0], tiny_dx2y = a.tdx2[1], tiny_dx2z = a.tdx2[2];
   // periodic images: only tiles within 3 cells of a periodic x or y face
@@ -407,13 +407,18 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
   const int64_t s_ps = a.ps.sz * (int64_t)sizeof(T), s_out = a.out.sz * (int64_t)sizeof(T);
   const int64_t s_vel = a.vx.sz * (int64_t)sizeof(T), s_pc = a.pc.sz * (int64_t)sizeof(T);
   const int64_t s_p0 = a.phi0.sz * (int64_t)sizeof(T);
+  const int64_t s_pc1 = a.pc.p ? s_pc : s_ps, s_p01 = (KIND == STAGE_REINIT && a.use_phi0) ? s_p0 : s_ps;
   const char* p_ps = plane_ptr<T>(a.ps, k0);
   const char* p_top = plane_ptr<T>(a.ps, k0 + 3);
   const char* p_vx = reads_velocity<KIND>() ? plane_ptr<T>(a.vx, k0) : nullptr;
   const char* p_vy = reads_velocity<KIND>() ? plane_ptr<T>(a.vy, k0) : nullptr;
   const char* p_vz = reads_velocity<KIND>() ? plane_ptr<T>(a.vz, k0) : nullptr;
-  const char* p_pc = a.pc.p ? plane_ptr<T>(a.pc, k0) : nullptr;
-  const char* p_p0 = (KIND == STAGE_REINIT && a.use_phi0) ? plane_ptr<T>(a.phi0, k0) : nullptr;
+  // absent optional fields read the stage field's own plane instead (phi_cur
+  // defaults to phi_stage; an unused phi0 value is discarded), so that every
+  // load is issued unconditionally
+  const bool has_pc = a.pc.p != nullptr, has_p0 = KIND == STAGE_REINIT && a.use_phi0;
+  const char* p_pc = has_pc ? plane_ptr<T>(a.pc, k0) : p_ps;
+  const char* p_p0 = has_p0 ? plane_ptr<T>(a.phi0, k0) : p_ps;
   char* p_out = reinterpret_cast<char*>(a.out.p) + (int64_t)(k0 - a.out.klo) * s_out;
 
   // z column: zv[q] = phi(k + q), q = 0..3, and zd[q] = phi(k + q - 2) -
@@ -499,10 +504,10 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
       ux = a.cvx;
       uy = a.cvy;
       uz = a.cvz;
-    } else if (a.use_phi0) {
+    } else if (KIND == STAGE_REINIT) {
       p0v = buf_load<T, kStream>(plane_rsrc(p_p0), op0);
     }
-    if (a.pc.p) pcv = buf_load<T, kStream>(plane_rsrc(p_pc), opc);
+    pcv = buf_load<T, kStream>(plane_rsrc(p_pc), opc);
   };
   auto advance = [&]() {
     p_ps += s_ps;
@@ -512,8 +517,8 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
       p_vy += s_vel;
       p_vz += s_vel;
     }
-    if (p_pc) p_pc += s_pc;
-    if (p_p0) p_p0 += s_p0;
+    p_pc += s_pc1;
+    if (KIND == STAGE_REINIT) p_p0 += s_p01;
   };
   if (k0 <= k1) fetch(k0);
 
