@@ -29,7 +29,9 @@ namespace lsm {
 
 // SYNTH: ghost values are synthesised per face; otherwise stored ghost cells
 // are read.  FP32 uses the rearranged WENO5 form, FP64 the reference order.
-template <typename T, int TX, int TY, int KIND, bool SYNTH>
+// FACES (with stored ghosts only): the stage writes its output's
+// extrapolated face ghost layers (a.oghost).
+template <typename T, int TX, int TY, int KIND, bool SYNTH, bool FACES = false>
 __global__ void __launch_bounds__(TX* TY) k_stage(StageArgs<T> a) {
   __shared__ StageTile<T, TX, TY> tile;
   int bx, by, bz;
@@ -37,7 +39,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(StageArgs<T> a) {
   if (SYNTH && !tile_is_interior<TX, TY>(a.fb, a.tg, bx, by, bz))
     march_tile<T, TX, TY, KIND, (sizeof(T) == 4), true>(a, bx, by, bz, tile);
   else
-    march_tile<T, TX, TY, KIND, (sizeof(T) == 4), false>(a, bx, by, bz, tile);
+    march_tile<T, TX, TY, KIND, (sizeof(T) == 4), false, true, FACES && !SYNTH>(a, bx, by, bz, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -156,6 +158,8 @@ template <int TX, int TY, int KIND>
 void launch_stage_t(hipStream_t s, const StageArgs<R>& a, int nblk) {
   if (a.ghost.any)
     hipLaunchKernelGGL((k_stage<R, TX, TY, KIND, true>), dim3((unsigned)nblk), dim3(TX * TY), 0, s, a);
+  else if (a.oghost.any)
+    hipLaunchKernelGGL((k_stage<R, TX, TY, KIND, false, true>), dim3((unsigned)nblk), dim3(TX * TY), 0, s, a);
   else
     hipLaunchKernelGGL((k_stage<R, TX, TY, KIND, false>), dim3((unsigned)nblk), dim3(TX * TY), 0, s, a);
 }

@@ -337,7 +337,10 @@ struct PlaneRule {
 };
 
 // IMAGES = false compiles out the periodic-image stores (callers whose
-// a.periodic is always 0, such as the patch level).
+// a.periodic is always 0, such as the patch level); FACES = true compiles in
+// the extrapolated face-ghost writes of a.oghost (stages that read stored
+// ghosts and keep their output's face ghosts), which the other launches do
+// not carry in their registers.
 // The LDS plane is double-buffered: plane k is written to buffer k & 1 (of
 // the chunk-relative index), so one barrier per plane suffices -- the reads
 // of a buffer (plane k - 2) all precede the barrier of plane k - 1, which
@@ -345,7 +348,7 @@ struct PlaneRule {
 template <typename T, int TX, int TY>
 using StageTile = T[2][TY + 6][TX + 7];
 
-template <typename T, int TX, int TY, int KIND, bool FASTW, bool SYNTH, bool IMAGES = true>
+template <typename T, int TX, int TY, int KIND, bool FASTW, bool SYNTH, bool IMAGES = true, bool FACES = false>
 __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by, int bz,
                                            StageTile<T, TX, TY>& tiles) {
   // no implicit contraction: the FP32 arithmetic that fuses writes fma
@@ -394,13 +397,13 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
   // write them from the face cell and its inward neighbour, which the host
   // guarantees lie in the same tile (x, y) and z-chunk (z)
   const int* oc6 = a.oghost.code;
-  const bool f_xlo = IMAGES && a.oghost.any && oc6[0] && i0 == fb.ilo;
-  const bool f_xhi = IMAGES && a.oghost.any && oc6[1] && i0 + TX - 1 >= fb.ihi;
-  const bool f_ylo = IMAGES && a.oghost.any && oc6[2] && j0 == fb.jlo;
-  const bool f_yhi = IMAGES && a.oghost.any && oc6[3] && j0 + TY - 1 >= fb.jhi;
+  const bool f_xlo = FACES && a.oghost.any && oc6[0] && i0 == fb.ilo;
+  const bool f_xhi = FACES && a.oghost.any && oc6[1] && i0 + TX - 1 >= fb.ihi;
+  const bool f_ylo = FACES && a.oghost.any && oc6[2] && j0 == fb.jlo;
+  const bool f_yhi = FACES && a.oghost.any && oc6[3] && j0 + TY - 1 >= fb.jhi;
   const bool f_xy = f_xlo || f_xhi || f_ylo || f_yhi;
-  const bool f_z = IMAGES && a.oghost.any && ((oc6[4] && k0 == fb.klo) || (oc6[5] && k1 == fb.khi));
-  __shared__ T out_plane[IMAGES ? TY : 1][IMAGES ? TX : 1];
+  const bool f_z = FACES && a.oghost.any && ((oc6[4] && k0 == fb.klo) || (oc6[5] && k1 == fb.khi));
+  __shared__ T out_plane[FACES ? TY : 1][FACES ? TX : 1];
   T v_prev = T(0);  // phi_out of the column in the previous plane
 
   // plane pointers, advanced by one plane per step
@@ -482,32 +485,38 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
     for (int q = 0; q < 3; q++) zv[q] = z1[q + 3];
   }
   // plane values of the next plane to process (halo and top values pending)
-  Pending<T, SYNTH> phx, phy[NR];
-  ColValue ptop;
-  T ux = T(0), uy = T(0), uz = T(0), pcv = T(0), p0v = T(0);
-  // loads plane kn's values (and the column value 3 planes up)
-  auto fetch = [&](int kn) {
+  // U sets of pending values: plane k consumes set k % U (chunk-relative)
+  // and fetches plane k + 1 into the other, so that with the planes
+  // unrolled by two every set stays in its registers across the loop and
+  // no copy of an in-flight load (which would wait for it) is needed
+  Pending<T, SYNTH> phx[U], phy[U][NR];
+  ColValue ptop[U];
+  T ux[U], uy[U], uz[U], pcv[U], p0v[U];
+#pragma unroll
+  for (int q = 0; q < U; q++) ux[q] = uy[q] = uz[q] = pcv[q] = p0v[q] = T(0);
+  // loads plane kn's values (and the column value 3 planes up) into set q
+  auto fetch = [&](int kn, int q) {
     const __amdgpu_buffer_rsrc_t rps = plane_rsrc(p_ps);
     if (SYNTH) {
-      col_load(kn + 3, ptop);
+      col_load(kn + 3, ptop[q]);
     } else {
-      ptop.va = buf_load<T>(plane_rsrc(p_top), oc);
+      ptop[q].va = buf_load<T>(plane_rsrc(p_top), oc);
     }
-    phx.load(rps, rx);
+    phx[q].load(rps, rx);
 #pragma unroll
-    for (int n = 0; n < NR; n++) phy[n].load(rps, ry[n]);
+    for (int n = 0; n < NR; n++) phy[q][n].load(rps, ry[n]);
     if (reads_velocity<KIND>()) {
-      ux = buf_load<T, kStream>(plane_rsrc(p_vx), ov);
-      uy = buf_load<T, kStream>(plane_rsrc(p_vy), ov);
-      uz = buf_load<T, kStream>(plane_rsrc(p_vz), ov);
+      ux[q] = buf_load<T, kStream>(plane_rsrc(p_vx), ov);
+      uy[q] = buf_load<T, kStream>(plane_rsrc(p_vy), ov);
+      uz[q] = buf_load<T, kStream>(plane_rsrc(p_vz), ov);
     } else if (KIND == STAGE_ADVECT_CONST) {
-      ux = a.cvx;
-      uy = a.cvy;
-      uz = a.cvz;
+      ux[q] = a.cvx;
+      uy[q] = a.cvy;
+      uz[q] = a.cvz;
     } else if (KIND == STAGE_REINIT) {
-      p0v = buf_load<T, kStream>(plane_rsrc(p_p0), op0);
+      p0v[q] = buf_load<T, kStream>(plane_rsrc(p_p0), op0);
     }
-    pcv = buf_load<T, kStream>(plane_rsrc(p_pc), opc);
+    pcv[q] = buf_load<T, kStream>(plane_rsrc(p_pc), opc);
   };
   auto advance = [&]() {
     p_ps += s_ps;
@@ -520,7 +529,7 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
     p_pc += s_pc1;
     if (KIND == STAGE_REINIT) p_p0 += s_p01;
   };
-  if (k0 <= k1) fetch(k0);
+  if (k0 <= k1) fetch(k0, 0);
 
   int par = 0;  // LDS buffer of the plane (U == 1; with U == 2 it is u)
   for (int kg = k0; kg <= k1; kg += U) {
@@ -530,23 +539,24 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
       if (k > k1) break;  // uniform
       T (*tile)[TX + 7] = tiles[U == 2 ? u : par];
       par ^= 1;
-      zv[u + 3] = col_value(ptop);
-      const T hx = phx.value(rx);
+      const int cs = u % U, ns = (u + 1) % U;  // consumed and refilled sets (static)
+      zv[u + 3] = col_value(ptop[cs]);
+      const T hx = phx[cs].value(rx);
       T hy[NR];
 #pragma unroll
-      for (int n = 0; n < NR; n++) hy[n] = phy[n].value(ry[n]);
+      for (int n = 0; n < NR; n++) hy[n] = phy[cs][n].value(ry[n]);
       tile[ty + 3][tx + 3] = zv[u];
       if (do_x) tile[ty + 3][txl] = hx;
 #pragma unroll
       for (int n = 0; n < NR; n++)
         if (doy[n]) tile[tyr[n]][tx + 3] = hy[n];
       zd[u + 5] = zv[u + 3] - zv[u + 2];
-      const T cux = ux, cuy = uy, cuz = uz, cpc = pcv, cp0 = p0v;
+      const T cux = ux[cs], cuy = uy[cs], cuz = uz[cs], cpc = pcv[cs], cp0 = p0v[cs];
       __syncthreads();
       // the next plane's loads; at the last plane the pointers stay and the
       // loads re-read plane k1 (unused), so the fetch is unconditional
       if (k < k1) advance();  // uniform
-      fetch(k < k1 ? k + 1 : k1);
+      fetch(k < k1 ? k + 1 : k1, ns);
 
       T v = T(0);
       if (active) {
