@@ -416,12 +416,12 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
   const char* p_vx = reads_velocity<KIND>() ? plane_ptr<T>(a.vx, k0) : nullptr;
   const char* p_vy = reads_velocity<KIND>() ? plane_ptr<T>(a.vy, k0) : nullptr;
   const char* p_vz = reads_velocity<KIND>() ? plane_ptr<T>(a.vz, k0) : nullptr;
-  // absent optional fields read the stage field's own plane instead (phi_cur
-  // defaults to phi_stage; an unused phi0 value is discarded), so that every
-  // load is issued unconditionally
+  // absent optional fields load the column's value 3 planes up, the cell
+  // the top load of the same plane reads (a first-level cache hit; the value
+  // is unused), so that every load is issued unconditionally
   const bool has_pc = a.pc.p != nullptr, has_p0 = KIND == STAGE_REINIT && a.use_phi0;
-  const char* p_pc = has_pc ? plane_ptr<T>(a.pc, k0) : p_ps;
-  const char* p_p0 = has_p0 ? plane_ptr<T>(a.phi0, k0) : p_ps;
+  const char* p_pc = has_pc ? plane_ptr<T>(a.pc, k0) : p_top;
+  const char* p_p0 = has_p0 ? plane_ptr<T>(a.phi0, k0) : p_top;
   char* p_out = reinterpret_cast<char*>(a.out.p) + (int64_t)(k0 - a.out.klo) * s_out;
 
   // z column: zv[q] = phi(k + q), q = 0..3, and zd[q] = phi(k + q - 2) -
