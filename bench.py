@@ -147,7 +147,10 @@ def run_bandwidth_probe(mib=2048, reps=10, march_n=512):
     (GB/s = 1e9 B/s).  "march_pattern" is the stage kernels' own tile march
     with the arithmetic and the halo taken out (lsm_amd_march_probe, 32 x 8
     tiles, z-chunks of 16 planes, march_n^3 cells in the padded layout of the
-    stage workloads), counted as 6 words per interior cell."""
+    stage workloads), counted as 6 words per interior cell.  "valu_fma" and
+    "valu_rcp" are the FP32 vector-ALU issue rates of v_fma_f32 and of the
+    transcendental v_rcp_f32 (lsm_amd_valu_probe: eight independent chains
+    per lane, no memory traffic), in lane-instructions/s."""
     import ctypes
     import torch
     from lsmlib_amd import toolbox
@@ -180,6 +183,17 @@ def run_bandwidth_probe(mib=2048, reps=10, march_n=512):
         out["march_pattern"] = {"gbs": 6 * 4 * n ** 3 / (ms * 1e-3) / 1e9, "ms": ms, "grid": n, "tile": [32, 8],
                                 "kz": 16, "bytes_moved": 6 * 4 * n ** 3}
         del src, dst
+    dst = torch.empty(16384 * 256, dtype=torch.float32, device="cuda")
+    for op, name in ((0, "valu_fma"), (1, "valu_rcp")):
+        best = None
+        for wg in (2048, 4096, 16384):
+            trips = (24 if op == 0 else 6) * 1024 * 1024 // wg
+            ms = lib.fused("lsm_amd_valu_probe", ctypes.c_void_p(dst.data_ptr()), op, wg, trips, 5)
+            rate = wg * 256 * trips * 32 / (ms * 1e-3)
+            if best is None or rate > best["lane_instructions_per_s"]:
+                best = {"lane_instructions_per_s": rate, "ms": ms, "workgroups": wg, "trips": trips}
+        out[name] = best
+    del dst
     torch.cuda.empty_cache()
     return out
 
@@ -929,15 +943,17 @@ def read_traffic():
 VALU_PEAK_LANE_INSTR = 256 * 64 * 2.4e9
 
 
-def valu_roofline(r32):
+def valu_roofline(r32, measured=None):
     """The stage kernel against the VALU issue rate: measured instructions per
-    cell x cells per launch / the live average launch time."""
+    cell x cells per launch / the live average launch time, over the nominal
+    rate and over the rate the FMA probe reached in the same run."""
     per_cell = read_valu()
     if per_cell is None or not r32.get("kernel_ms_avg"):
         return None
     achieved = per_cell * r32["cells"] / (r32["kernel_ms_avg"] * 1e-3)
     return {"lane_instructions_per_cell": per_cell, "achieved": achieved, "peak": VALU_PEAK_LANE_INSTR,
-            "unit": "lane-instructions/s", "frac": achieved / VALU_PEAK_LANE_INSTR}
+            "unit": "lane-instructions/s", "frac": achieved / VALU_PEAK_LANE_INSTR,
+            "measured_peak": measured, "frac_of_measured": achieved / measured if measured else None}
 
 
 def read_valu():
@@ -1053,7 +1069,9 @@ def main():
     best5 = max(probe["read5_write1"]["gbs"], probe["read5_write1_4byte"]["gbs"])
     print(f"# measured streaming rates: copy {probe['copy']['gbs']:.0f} GB/s, 5 reads + 1 write "
           f"{best5:.0f} GB/s, reads {probe['read']['gbs']:.0f} GB/s, the stage's tile march without arithmetic "
-          f"{probe.get('march_pattern', {}).get('gbs', float('nan')):.0f} GB/s", file=sys.stderr, flush=True)
+          f"{probe.get('march_pattern', {}).get('gbs', float('nan')):.0f} GB/s; FP32 FMA issue "
+          f"{probe.get('valu_fma', {}).get('lane_instructions_per_s', float('nan')) / 1e12:.1f} T lane-instructions/s",
+          file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(a.cpu_seconds)
@@ -1137,11 +1155,11 @@ def main():
                      "kernel": "k_stage (fused advection stage)",
                      "kernel_ms_avg": r32["kernel_ms_avg"],
                      "algorithmic_bytes_per_launch": r32["algorithmic_bytes_per_launch"],
-                     "measured_peak_gbs": {k: v["gbs"] for k, v in probe.items()},
+                     "measured_peak_gbs": {k: v["gbs"] for k, v in probe.items() if "gbs" in v},
                      "frac_of_measured_read5_write1": r32["achieved_gbs"] / best5,
                      "frac_of_march_pattern": (r32["achieved_gbs"] / probe["march_pattern"]["gbs"]
                                                if "march_pattern" in probe else None),
-                     "valu": valu_roofline(r32)},
+                     "valu": valu_roofline(r32, probe.get("valu_fma", {}).get("lane_instructions_per_s"))},
         "cpu_baseline": cpu,
         "adjoint_s": None if adj is None else adj["adjoint_s"],
         "adjoint": adj,

@@ -1039,6 +1039,13 @@ double lsm_amd_bandwidth_probe(int kind, const void *src, void *dst, long long b
  * row 3 of plane 3).  n is a multiple of 64 and of kz, px >= n + 64,
  * py >= n + 6.  Returns milliseconds per launch (-1 on error). */
 double lsm_amd_march_probe(const void *src, void *dst, int n, int px, int py, int kz, int reps);
+/* FP32 vector-ALU issue-rate probe (a measurement utility; no counterpart in
+ * the reference): `workgroups` x 256 lanes each run trips x 32 instructions in
+ * eight independent chains, v_fma_f32 (op 0) or the transcendental v_rcp_f32
+ * (op 1), no memory traffic in the loop; dst receives one float per lane
+ * (workgroups * 256 floats).  Returns milliseconds per launch (-1 on error);
+ * lane-instructions per second = workgroups * 256 * trips * 32 / time. */
+double lsm_amd_valu_probe(void *dst, int op, int workgroups, int trips, int reps);
 
 /* ======================================================================
  * Part 3: fused device-resident stages (this library's own API; no

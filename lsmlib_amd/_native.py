@@ -101,6 +101,8 @@ class ToolboxLibrary(F77Library):
         L.lsm_amd_bandwidth_probe.restype = ctypes.c_double
         L.lsm_amd_march_probe.argtypes = [P, P, I, I, I, I, I]
         L.lsm_amd_march_probe.restype = ctypes.c_double
+        L.lsm_amd_valu_probe.argtypes = [P, I, I, I, I]
+        L.lsm_amd_valu_probe.restype = ctypes.c_double
         L.lsm3d_amd_set_zero_tol.argtypes = [ctypes.c_double]
         L.lsm3d_amd_set_zero_tol.restype = None
         L.lsm3d_amd_zero_tol.argtypes = []

@@ -14,6 +14,10 @@
 // taken out -- 32 x 8 column tiles, z-chunks, the stage launch's tile order,
 // 5 fields read and 1 written at every cell, no halo -- the streaming rate the
 // access pattern itself allows.
+// lsm_amd_valu_probe: the FP32 vector-ALU issue rate -- eight independent
+// v_fma_f32 (or v_rcp_f32) chains per lane, no memory traffic in the loop --
+// the ceiling the stage kernels' VALU lane-instruction counts are compared
+// with.
 #include <string>
 
 #include "../../include/lsm3d_amd.h"
@@ -101,6 +105,30 @@ __global__ void __launch_bounds__(kProbeThreads) k_probe_read(const v4f* __restr
   d[(long long)blockIdx.x * kProbeThreads + threadIdx.x] = (acc.x + acc.y) + (acc.z + acc.w);
 }
 
+// Eight independent chains per lane, kValuUnroll x 8 instructions per trip:
+// OP 0 v_fma_f32 (a = a * b + c), OP 1 v_rcp_f32 (a = 1 / a, a transcendental
+// instruction).  One float per thread is written so nothing is dead; b and c
+// come from the host, so no chain folds.
+constexpr int kValuChains = 8, kValuUnroll = 4;
+
+template <int OP>
+__global__ void __launch_bounds__(kProbeThreads) k_probe_valu(float* __restrict__ d, int trips, float b, float c) {
+  float a[kValuChains];
+#pragma unroll
+  for (int q = 0; q < kValuChains; q++) a[q] = (float)(threadIdx.x + q) * 1e-3f + 1.f;
+  for (int t = 0; t < trips; t++) {
+#pragma unroll
+    for (int u = 0; u < kValuUnroll; u++) {
+#pragma unroll
+      for (int q = 0; q < kValuChains; q++) a[q] = OP == 0 ? __builtin_fmaf(a[q], b, c) : __builtin_amdgcn_rcpf(a[q]);
+    }
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int q = 0; q < kValuChains; q++) s += a[q];
+  d[(long long)blockIdx.x * kProbeThreads + threadIdx.x] = s;
+}
+
 }  // namespace
 }  // namespace lsm
 
@@ -180,6 +208,46 @@ double lsm_amd_march_probe(const void* src, void* dst, int n, int px, int py, in
     hipLaunchKernelGGL(k_probe_march, dim3((unsigned)nblk), dim3(kMarchTX * kMarchTY), 0, c.stream(),
                        static_cast<const float*>(src), static_cast<float*>(dst), field, tg, px, plane);
     c.launched("k_probe_march");
+  };
+  launch();
+  check_hip(hipEventRecord(e0, c.stream()), "hipEventRecord");
+  for (int r = 0; r < reps && c.ok(); r++) launch();
+  check_hip(hipEventRecord(e1, c.stream()), "hipEventRecord");
+  check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+  float ms = 0.f;
+  check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c.finish();
+  return c.ok() ? (double)ms / reps : -1.0;
+}
+
+// dst: workgroups * 256 floats.  Each launch runs workgroups x 256 lanes of
+// trips x 32 instructions (op 0 FMA, op 1 reciprocal).  Returns ms per launch
+// (average of reps after one warm-up launch), or -1 on error.
+double lsm_amd_valu_probe(void* dst, int op, int workgroups, int trips, int reps) {
+  const char* fn = "lsm_amd_valu_probe";
+  Call c(fn);
+  if (op < 0 || op > 1) raise_error(std::string(fn) + ": op is 0 (FMA) or 1 (reciprocal)");
+  if (workgroups <= 0 || trips <= 0 || reps <= 0) raise_error(std::string(fn) + ": sizes must be positive");
+  if (!dst) raise_error(std::string(fn) + ": missing array");
+  if (!c.ok()) return -1.0;
+  if (!is_device_pointer(dst)) {
+    raise_error(std::string(fn) + ": dst must be device memory");
+    return -1.0;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  check_hip(hipEventCreate(&e0), "hipEventCreate");
+  check_hip(hipEventCreate(&e1), "hipEventCreate");
+  if (!c.ok()) return -1.0;
+  auto launch = [&]() {
+    if (op == 0)
+      hipLaunchKernelGGL(k_probe_valu<0>, dim3((unsigned)workgroups), dim3(kProbeThreads), 0, c.stream(),
+                         static_cast<float*>(dst), trips, 0.999f, 1e-4f);
+    else
+      hipLaunchKernelGGL(k_probe_valu<1>, dim3((unsigned)workgroups), dim3(kProbeThreads), 0, c.stream(),
+                         static_cast<float*>(dst), trips, 0.999f, 1e-4f);
+    c.launched("k_probe_valu");
   };
   launch();
   check_hip(hipEventRecord(e0, c.stream()), "hipEventRecord");
