@@ -75,16 +75,6 @@ __device__ __forceinline__ T extrapolate_sel(int code, T pb, T p1, int dist) {
   return (code >= 2) ? e : pb;
 }
 
-// The three ghost cells beyond a face cell (i,j,k) of `out` in direction
-// (di,dj,dk): extrapolate(code, pb, p1, d) at distance d = 1..3 from the face
-// value pb and its inward neighbour p1, the values the reference's face fill
-// stores there (lsm_boundary_conditions3d.f:66-77, 213-225).
-template <typename T>
-__device__ __forceinline__ void write_face_ghosts(const Field<T>& out, int code, T pb, T p1, int i, int j, int k,
-                                                  int di, int dj, int dk) {
-#pragma unroll
-  for (int d = 1; d <= 3; d++) out.at(i + d * di, j + d * dj, k + d * dk) = extrapolate<T>(code, pb, p1, d);
-}
 
 // Writes `val` at (i,j,k) and, for each axis in `periodic` whose image of
 // the cell lies in the g-wide ghost layer, at the periodic images (up to 7
@@ -268,6 +258,21 @@ __device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t r, ui
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(lsm_u32x2, v), r, (int)off, 0, kStream);
 }
 
+// The three ghost cells beyond a face cell of the output: extrapolate(code,
+// pb, p1, d) at distance d = 1..3 from the face value pb and its inward
+// neighbour p1, the values the reference's face fill stores there
+// (lsm_boundary_conditions3d.f:66-77, 213-225).  Ghost cell d lies in the
+// plane at byte address plane + d * pstep, at byte offset off + d * ostep
+// (x and y faces step the offset, z faces the plane), so the stores need
+// no 64-bit per-lane address arithmetic.
+template <typename T>
+__device__ __forceinline__ void store_face_ghosts(const char* plane, int64_t pstep, uint32_t off, int32_t ostep,
+                                                  int code, T pb, T p1) {
+#pragma unroll
+  for (int d = 1; d <= 3; d++)
+    buf_store(extrapolate<T>(code, pb, p1, d), plane_rsrc(plane + d * pstep), off + (uint32_t)(d * ostep));
+}
+
 template <typename T>
 __device__ __forceinline__ uint32_t plane_byte_offset(const Field<const T>& f, int i, int j) {
   return (uint32_t)((i - f.ilo) + (int)f.sy * (j - f.jlo)) * (uint32_t)sizeof(T);
@@ -403,7 +408,6 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
   const bool f_yhi = FACES && a.oghost.any && oc6[3] && j0 + TY - 1 >= fb.jhi;
   const bool f_xy = f_xlo || f_xhi || f_ylo || f_yhi;
   const bool f_z = FACES && a.oghost.any && ((oc6[4] && k0 == fb.klo) || (oc6[5] && k1 == fb.khi));
-  __shared__ T out_plane[FACES ? TY : 1][FACES ? TX : 1];
   T v_prev = T(0);  // phi_out of the column in the previous plane
 
   // plane pointers, advanced by one plane per step
@@ -621,21 +625,11 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
         if (img_xy || img_z) store_with_periodic_images<T>(a.out, fb, i, j, k, v, a.periodic, 3);
         else buf_store(v, plane_rsrc(p_out), oo);
       }
-      if (f_xy) {  // uniform per tile
-        if (active) out_plane[ty][tx] = v;
-        __syncthreads();
-        if (active) {
-          if (f_xlo && i == fb.ilo) write_face_ghosts<T>(a.out, oc6[0], v, out_plane[ty][tx + 1], i, j, k, -1, 0, 0);
-          if (f_xhi && i == fb.ihi) write_face_ghosts<T>(a.out, oc6[1], v, out_plane[ty][tx - 1], i, j, k, 1, 0, 0);
-          if (f_ylo && j == fb.jlo) write_face_ghosts<T>(a.out, oc6[2], v, out_plane[ty + 1][tx], i, j, k, 0, -1, 0);
-          if (f_yhi && j == fb.jhi) write_face_ghosts<T>(a.out, oc6[3], v, out_plane[ty - 1][tx], i, j, k, 0, 1, 0);
-        }
-      }
       if (f_z && active) {
         // below the bottom face from planes klo (kept) and klo + 1; above the
         // top face from planes khi and khi - 1 (kept)
-        if (oc6[4] && k == fb.klo + 1) write_face_ghosts<T>(a.out, oc6[4], v_prev, v, i, j, fb.klo, 0, 0, -1);
-        if (oc6[5] && k == fb.khi) write_face_ghosts<T>(a.out, oc6[5], v, v_prev, i, j, k, 0, 0, 1);
+        if (oc6[4] && k == fb.klo + 1) store_face_ghosts<T>(p_out - s_out, -s_out, oo, 0, oc6[4], v_prev, v);
+        if (oc6[5] && k == fb.khi) store_face_ghosts<T>(p_out, s_out, oo, 0, oc6[5], v, v_prev);
       }
       v_prev = v;
       p_out += s_out;
@@ -644,6 +638,26 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
     for (int q = 0; q < 3; q++) zv[q] = zv[q + U];
 #pragma unroll
     for (int q = 0; q < 5; q++) zd[q] = zd[q + U];
+  }
+  if (FACES && f_xy && k0 <= k1) {  // uniform per tile
+    // x/y face ghosts of every plane of the chunk, after the march: the face
+    // lane reads back its own and its inward neighbour's output (stored by
+    // this workgroup; the barrier's wait and fence make the stores visible),
+    // so the march itself carries no face state in its registers
+    __syncthreads();
+    if (active) {
+      constexpr int32_t e = (int32_t)sizeof(T);
+      const int32_t row = (int32_t)a.out.sy * e;
+      const char* pl = p_out - (int64_t)(k1 - k0 + 1) * s_out;  // plane k0
+      for (int k = k0; k <= k1; k++, pl += s_out) {
+        const __amdgpu_buffer_rsrc_t r = plane_rsrc(pl);
+        const T own = buf_load<T>(r, oo);
+        if (f_xlo && i == fb.ilo) store_face_ghosts<T>(pl, 0, oo, -e, oc6[0], own, buf_load<T>(r, oo + e));
+        if (f_xhi && i == fb.ihi) store_face_ghosts<T>(pl, 0, oo, e, oc6[1], own, buf_load<T>(r, oo - e));
+        if (f_ylo && j == fb.jlo) store_face_ghosts<T>(pl, 0, oo, -row, oc6[2], own, buf_load<T>(r, oo + row));
+        if (f_yhi && j == fb.jhi) store_face_ghosts<T>(pl, 0, oo, row, oc6[3], own, buf_load<T>(r, oo - row));
+      }
+    }
   }
 }
 
