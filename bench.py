@@ -147,10 +147,13 @@ def run_bandwidth_probe(mib=2048, reps=10, march_n=512):
     (GB/s = 1e9 B/s).  "march_pattern" is the stage kernels' own tile march
     with the arithmetic and the halo taken out (lsm_amd_march_probe, 32 x 8
     tiles, z-chunks of 16 planes, march_n^3 cells in the padded layout of the
-    stage workloads), counted as 6 words per interior cell.  "valu_fma" and
-    "valu_rcp" are the FP32 vector-ALU issue rates of v_fma_f32 and of the
-    transcendental v_rcp_f32 (lsm_amd_valu_probe: eight independent chains
-    per lane, no memory traffic), in lane-instructions/s."""
+    stage workloads), counted as 6 words per interior cell.  "valu_*" are
+    FP32 vector-ALU issue rates (lsm_amd_valu_probe: eight independent chains
+    per lane, no memory traffic), in lane-instructions/s: v_fma_f32, v_mul_f32
+    and v_add_f32 with a scalar-register operand ("_sgpr"), with per-lane
+    operands ("_vgpr", and the accumulating v_fmac_f32), with literal
+    constants, the three-operand v_max3_f32 and the transcendental
+    v_rcp_f32."""
     import ctypes
     import torch
     from lsmlib_amd import toolbox
@@ -184,10 +187,11 @@ def run_bandwidth_probe(mib=2048, reps=10, march_n=512):
                                 "kz": 16, "bytes_moved": 6 * 4 * n ** 3}
         del src, dst
     dst = torch.empty(16384 * 256, dtype=torch.float32, device="cuda")
-    for op, name in ((0, "valu_fma"), (1, "valu_rcp")):
+    for op, name in ((0, "valu_fma_sgpr"), (1, "valu_rcp"), (2, "valu_mul_sgpr"), (3, "valu_add_sgpr"),
+                     (4, "valu_fma_vgpr"), (5, "valu_fmac_vgpr"), (6, "valu_max3_vgpr"), (7, "valu_fma_literal")):
         best = None
         for wg in (2048, 4096, 16384):
-            trips = (24 if op == 0 else 6) * 1024 * 1024 // wg
+            trips = (6 if op == 1 else 24) * 1024 * 1024 // wg
             ms = lib.fused("lsm_amd_valu_probe", ctypes.c_void_p(dst.data_ptr()), op, wg, trips, 5)
             rate = wg * 256 * trips * 32 / (ms * 1e-3)
             if best is None or rate > best["lane_instructions_per_s"]:
@@ -938,9 +942,19 @@ def read_traffic():
     return None
 
 
-# FP32 VALU issue rate of MI355X without packed FP32 (the library is compiled without it):
-# 256 CUs x 4 SIMDs x 16 lanes per clock x 2.4 GHz
-VALU_PEAK_LANE_INSTR = 256 * 64 * 2.4e9
+# FP32 vector-ALU peak of MI355X (157.3 TFLOP/s of FMA): 256 CUs x 128 lane-instructions
+# per clock x 2.4 GHz.  The library is compiled without packed FP32; plain FP32
+# instructions with per-lane or constant operands still issue near this rate on gfx950
+# (lsm_amd_valu_probe), those with a scalar-register operand, three-operand min/max and
+# transcendentals at half of it or less.
+VALU_PEAK_LANE_INSTR = 256 * 128 * 2.4e9
+
+
+def valu_issue_peak(probe):
+    """The best FP32 issue rate of the probe's instruction chains,
+    lane-instructions/s."""
+    rates = [probe[k]["lane_instructions_per_s"] for k in probe if k.startswith("valu_")]
+    return max(rates) if rates else None
 
 
 def valu_roofline(r32, measured=None):
@@ -1069,8 +1083,8 @@ def main():
     best5 = max(probe["read5_write1"]["gbs"], probe["read5_write1_4byte"]["gbs"])
     print(f"# measured streaming rates: copy {probe['copy']['gbs']:.0f} GB/s, 5 reads + 1 write "
           f"{best5:.0f} GB/s, reads {probe['read']['gbs']:.0f} GB/s, the stage's tile march without arithmetic "
-          f"{probe.get('march_pattern', {}).get('gbs', float('nan')):.0f} GB/s; FP32 FMA issue "
-          f"{probe.get('valu_fma', {}).get('lane_instructions_per_s', float('nan')) / 1e12:.1f} T lane-instructions/s",
+          f"{probe.get('march_pattern', {}).get('gbs', float('nan')):.0f} GB/s; FP32 VALU issue "
+          f"{(valu_issue_peak(probe) or float('nan')) / 1e12:.1f} T lane-instructions/s (best plain ALU chain)",
           file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu_baseline:
@@ -1159,7 +1173,7 @@ def main():
                      "frac_of_measured_read5_write1": r32["achieved_gbs"] / best5,
                      "frac_of_march_pattern": (r32["achieved_gbs"] / probe["march_pattern"]["gbs"]
                                                if "march_pattern" in probe else None),
-                     "valu": valu_roofline(r32, probe.get("valu_fma", {}).get("lane_instructions_per_s"))},
+                     "valu": valu_roofline(r32, valu_issue_peak(probe))},
         "cpu_baseline": cpu,
         "adjoint_s": None if adj is None else adj["adjoint_s"],
         "adjoint": adj,

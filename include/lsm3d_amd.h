@@ -1041,10 +1041,14 @@ double lsm_amd_bandwidth_probe(int kind, const void *src, void *dst, long long b
 double lsm_amd_march_probe(const void *src, void *dst, int n, int px, int py, int kz, int reps);
 /* FP32 vector-ALU issue-rate probe (a measurement utility; no counterpart in
  * the reference): `workgroups` x 256 lanes each run trips x 32 instructions in
- * eight independent chains, v_fma_f32 (op 0) or the transcendental v_rcp_f32
- * (op 1), no memory traffic in the loop; dst receives one float per lane
- * (workgroups * 256 floats).  Returns milliseconds per launch (-1 on error);
- * lane-instructions per second = workgroups * 256 * trips * 32 / time. */
+ * eight independent chains, no memory traffic in the loop: op 0 v_fma_f32 with
+ * scalar-register operands, 1 the transcendental v_rcp_f32, 2 v_mul_f32 and
+ * 3 v_add_f32 with a scalar-register operand, 4 v_fma_f32 with per-lane
+ * operands, 5 the accumulating v_fmac_f32, 6 v_max3_f32, 7 v_fmamk_f32 with
+ * literal constants; dst
+ * receives one float per lane (workgroups * 256 floats).  Returns
+ * milliseconds per launch (-1 on error); lane-instructions per second =
+ * workgroups * 256 * trips * 32 / time. */
 double lsm_amd_valu_probe(void *dst, int op, int workgroups, int trips, int reps);
 
 /* ======================================================================

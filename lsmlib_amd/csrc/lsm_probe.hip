@@ -106,9 +106,15 @@ __global__ void __launch_bounds__(kProbeThreads) k_probe_read(const v4f* __restr
 }
 
 // Eight independent chains per lane, kValuUnroll x 8 instructions per trip:
-// OP 0 v_fma_f32 (a = a * b + c), OP 1 v_rcp_f32 (a = 1 / a, a transcendental
-// instruction).  One float per thread is written so nothing is dead; b and c
-// come from the host, so no chain folds.
+// OP 0 v_fma_f32 with scalar b and c (a = a * b + c), OP 1 v_rcp_f32 (a = 1 /
+// a, a transcendental instruction), OP 2 v_mul_f32 (a = a * b), OP 3
+// v_add_f32 (a = a + c), OP 4 v_fma_f32 with per-lane operands, OP 5 the
+// accumulating v_fmac_f32 (a = b' * c' + a, per-lane b', c'), OP 6 the
+// three-operand v_max3_f32 (per-lane operands), OP 7 v_fmamk_f32 (literal
+// constants).  b and c are kernel arguments, so OP 0, 2 and 3 read scalar
+// registers; b', c' are per-lane copies.  One float
+// per thread is written so nothing is dead; b and c come from the host, so no
+// chain folds.
 constexpr int kValuChains = 8, kValuUnroll = 4;
 
 template <int OP>
@@ -116,11 +122,21 @@ __global__ void __launch_bounds__(kProbeThreads) k_probe_valu(float* __restrict_
   float a[kValuChains];
 #pragma unroll
   for (int q = 0; q < kValuChains; q++) a[q] = (float)(threadIdx.x + q) * 1e-3f + 1.f;
+  const float bl = b + (float)threadIdx.x * 1e-9f, cl = c - (float)threadIdx.x * 1e-9f;  // per lane (OP 4)
   for (int t = 0; t < trips; t++) {
 #pragma unroll
     for (int u = 0; u < kValuUnroll; u++) {
 #pragma unroll
-      for (int q = 0; q < kValuChains; q++) a[q] = OP == 0 ? __builtin_fmaf(a[q], b, c) : __builtin_amdgcn_rcpf(a[q]);
+      for (int q = 0; q < kValuChains; q++) {
+        if (OP == 0) a[q] = __builtin_fmaf(a[q], b, c);
+        else if (OP == 1) a[q] = __builtin_amdgcn_rcpf(a[q]);
+        else if (OP == 2) a[q] = a[q] * b;
+        else if (OP == 3) a[q] = a[q] + c;
+        else if (OP == 4) a[q] = __builtin_fmaf(a[q], bl, cl);
+        else if (OP == 5) a[q] = __builtin_fmaf(bl, cl, a[q]);
+        else if (OP == 6) a[q] = __builtin_fmaxf(__builtin_fmaxf(a[q], bl), cl);
+        else a[q] = __builtin_fmaf(a[q], 0.999f, 1e-4f);
+      }
     }
   }
   float s = 0.f;
@@ -223,12 +239,12 @@ double lsm_amd_march_probe(const void* src, void* dst, int n, int px, int py, in
 }
 
 // dst: workgroups * 256 floats.  Each launch runs workgroups x 256 lanes of
-// trips x 32 instructions (op 0 FMA, op 1 reciprocal).  Returns ms per launch
+// trips x 32 instructions (op as k_probe_valu's OP).  Returns ms per launch
 // (average of reps after one warm-up launch), or -1 on error.
 double lsm_amd_valu_probe(void* dst, int op, int workgroups, int trips, int reps) {
   const char* fn = "lsm_amd_valu_probe";
   Call c(fn);
-  if (op < 0 || op > 1) raise_error(std::string(fn) + ": op is 0 (FMA) or 1 (reciprocal)");
+  if (op < 0 || op > 7) raise_error(std::string(fn) + ": op is 0 to 7");
   if (workgroups <= 0 || trips <= 0 || reps <= 0) raise_error(std::string(fn) + ": sizes must be positive");
   if (!dst) raise_error(std::string(fn) + ": missing array");
   if (!c.ok()) return -1.0;
@@ -241,12 +257,11 @@ double lsm_amd_valu_probe(void* dst, int op, int workgroups, int trips, int reps
   check_hip(hipEventCreate(&e1), "hipEventCreate");
   if (!c.ok()) return -1.0;
   auto launch = [&]() {
-    if (op == 0)
-      hipLaunchKernelGGL(k_probe_valu<0>, dim3((unsigned)workgroups), dim3(kProbeThreads), 0, c.stream(),
-                         static_cast<float*>(dst), trips, 0.999f, 1e-4f);
-    else
-      hipLaunchKernelGGL(k_probe_valu<1>, dim3((unsigned)workgroups), dim3(kProbeThreads), 0, c.stream(),
-                         static_cast<float*>(dst), trips, 0.999f, 1e-4f);
+    auto* k = op == 0 ? k_probe_valu<0> : op == 1 ? k_probe_valu<1> : op == 2 ? k_probe_valu<2>
+            : op == 3 ? k_probe_valu<3> : op == 4 ? k_probe_valu<4> : op == 5 ? k_probe_valu<5>
+            : op == 6 ? k_probe_valu<6> : k_probe_valu<7>;
+    hipLaunchKernelGGL(k, dim3((unsigned)workgroups), dim3(kProbeThreads), 0, c.stream(), static_cast<float*>(dst),
+                       trips, 0.999f, 1e-4f);
     c.launched("k_probe_valu");
   };
   launch();
