@@ -191,11 +191,16 @@ class PatchLevel:
         self.stage(TVDRK3_STAGE3, phi, s2, phi, dt)
 
     def capture_rk3(self, nsteps, phi, s1, s2, dt):
+        """Captures `nsteps` TVD-RK3 steps of phi (stage pools s1, s2) into
+        the plan's hipGraph; replay() runs them.  The graph holds the pools'
+        device addresses, so the level keeps the three tensors alive until the
+        next capture."""
         h = self.dx
         self.lib.set_zero_tol(self.zero_tol)
         self.lib.fused("lsm3d_amd_patch_plan_capture_rk3", self.plan, int(nsteps),
                        ctypes.c_void_p(phi.data_ptr()), ctypes.c_void_p(s1.data_ptr()),
                        ctypes.c_void_p(s2.data_ptr()), h, h, h, dt, self.bc)
+        self._graph_pools = (phi, s1, s2)
 
     def replay(self):
         self.lib.fused("lsm3d_amd_patch_plan_replay", self.plan)

@@ -227,3 +227,23 @@ def oracle_advance_patches(level, phi_pool, dt, steps, picks):
             st.advect_rk3_step(ph, vel, dt)
         out.append(grid.interior(ph))
     return out
+
+
+def test_captured_stage_pools_stay_alive():
+    """The captured graph writes the stage pools it was captured with; the
+    level keeps them alive, so pools the caller does not hold are not freed
+    and handed to later allocations (which the replay would overwrite)."""
+    lvl = small_level("float", npatch=9)
+    phi0 = lvl.initial_phi()
+    dt = lvl.stable_dt(0.4)
+    g_phi = phi0.clone()
+    lvl.capture_rk3(2, g_phi, lvl.empty_pool(), lvl.empty_pool(), dt)
+    later = [torch.full_like(g_phi, float("nan")) for _ in range(4)]
+    lvl.replay()
+    torch.cuda.synchronize()
+    e_phi, e1, e2 = phi0.clone(), lvl.empty_pool(), lvl.empty_pool()
+    for _ in range(2):
+        lvl.rk3_step(e_phi, e1, e2, dt)
+    torch.cuda.synchronize()
+    assert torch.equal(g_phi, e_phi)
+    assert all(bool(torch.isnan(t).all()) for t in later)
