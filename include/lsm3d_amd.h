@@ -1221,7 +1221,9 @@ void lsm3d_amd_patch_plan_stage(lsm3d_amd_patch_plan *plan, int rk_form, LSMLIB_
                                 const int bc[6]);
 /* Captures nsteps TVD-RK3 steps (phi advanced in place through s1, s2)
  * into a hipGraph held by the plan; replay launches it on the library
- * stream. */
+ * stream.  The graph holds the three pools' device addresses: they must stay
+ * allocated while the plan may replay it (until the next capture or the
+ * plan's destruction). */
 void lsm3d_amd_patch_plan_capture_rk3(lsm3d_amd_patch_plan *plan, int nsteps, LSMLIB_REAL *phi_pool,
                                       LSMLIB_REAL *s1_pool, LSMLIB_REAL *s2_pool, LSMLIB_REAL dx,
                                       LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL dt,
