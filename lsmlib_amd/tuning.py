@@ -10,7 +10,7 @@ from pathlib import Path
 
 TABLE = Path(__file__).resolve().parent / "tuning_table.json"
 DEFAULT = (0, 32)
-KZ_CHOICES = (8, 16, 24, 32, 48, 64, 128)
+KZ_CHOICES = (8, 16, 18, 20, 24, 32, 48, 64, 128)
 CURVATURE_KZ_CHOICES = (2, 4, 8, 16, 32, 64)
 
 
