@@ -1,7 +1,7 @@
 """FP32 against FP64 on the sphere-union / Taylor-Green workload, with kinks
 (64 spheres) and interleaved reinitialization.
 
-The north-star bar for the FP32 WENO5 / TVD-RK3 step is a near-interface
+The precision target for the FP32 WENO5 / TVD-RK3 step is a near-interface
 (|phi| < 3 dx) error of at most 1e-5 dx against FP64.  A single FP32 step
 meets it with a wide margin, and is asserted here.  Over many steps the
 FP32 error grows, and lsmlib_amd.precision.precision_split shows why: the
