@@ -1077,7 +1077,9 @@ def main():
         split = precision_split(a.n, a.warmup + a.steps, velocity=a.velocity)
         print(f"# fp32 error split after {a.warmup + a.steps} steps (max near interface / dx): fp32 "
               f"{split['fp32_vs_fp64']['max_over_dx']:.2e}, inputs only {split['inputs_only']['max_over_dx']:.2e}, "
-              f"fp32 storage only {split['storage_only']['max_over_dx']:.2e}, one step "
+              f"fp32 storage only {split['storage_only']['max_over_dx']:.2e} (phi alone "
+              f"{split['storage_phi_only']['max_over_dx']:.2e}, s1 and s2 alone "
+              f"{split['storage_stages_only']['max_over_dx']:.2e}), one step "
               f"{split['one_step']['max_over_dx']:.2e}", file=sys.stderr, flush=True)
     probe = run_bandwidth_probe()
     best5 = max(probe["read5_write1"]["gbs"], probe["read5_write1_4byte"]["gbs"])
@@ -1112,7 +1114,9 @@ def main():
         print(f"# config5: fp32 {c5['float']['seconds']:.2f} s, fp64 {c5['double']['seconds']:.2f} s, near-interface "
               f"error {c5['fp32_error_vs_fp64']['max_near_interface_over_dx']:.3e} dx (inputs only "
               f"{sp['inputs_only']['max_over_dx']:.2e}, fp32 storage only {sp['storage_only']['max_over_dx']:.2e}, "
-              f"one step {sp['one_step']['max_over_dx']:.2e})", file=sys.stderr, flush=True)
+              f"phi alone {sp['storage_phi_only']['max_over_dx']:.2e}, s1 and s2 alone "
+              f"{sp['storage_stages_only']['max_over_dx']:.2e}, one step {sp['one_step']['max_over_dx']:.2e})",
+              file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
         extras["throughput_1024_fp32"] = run_throughput()
         print(f"# 1024^3: {extras['throughput_1024_fp32']['value']:.4e} cell-updates/s", file=sys.stderr, flush=True)

@@ -8,6 +8,10 @@ reinitialization with the sign from the current phi):
   in64    FP64 library, inputs rounded once to FP32 (phi0, velocity, dt)
   st64    as in64, and every stage output rounded to FP32: FP32 storage
           with FP64 arithmetic
+  ph64    as in64, with only phi (each step's and each reinitialization
+          iteration's final stage output) rounded to FP32
+  sg64    as in64, with only the intermediate stage fields s1 and s2
+          rounded to FP32
   fp32    FP32 library (FP32 storage and arithmetic), the same FP32 inputs
 
 and reports the near-interface (|phi_ref| < 3 dx) maximum differences in
@@ -18,6 +22,8 @@ units of dx:
                     inputs, grown by the problem itself
   storage_only      |st64 - in64|: what rounding the stored fields to FP32
                     costs with exact FP64 arithmetic
+  storage_phi_only  |ph64 - in64|: rounding phi alone (the RK base state)
+  storage_stages_only  |sg64 - in64|: rounding s1 and s2 alone
   fp32_same_inputs  |fp32 - in64|: storage plus FP32 arithmetic
   one_step          the FP32 step's local error, one step from the final
                     in64 state rounded to FP32
@@ -36,7 +42,7 @@ _FORMS = ((RK1, "s1", "phi", None), (TVDRK3_STAGE2, "s2", "s1", "phi"), (TVDRK3_
 
 
 class _Run:
-    def __init__(self, real, n, round_inputs, round_storage, zero_tol, device, velocity):
+    def __init__(self, real, n, round_inputs, rounded, zero_tol, device, velocity):
         dtype = torch.float32 if real == "float" else torch.float64
         grid, phi, vel, bc = sphere_union_taylor_green(n, dtype=torch.float64, device=device,
                                                        align=line_elems(dtype), velocity=velocity)
@@ -45,7 +51,7 @@ class _Run:
             phi = phi.float().double()
             vel = tuple(v.float().double() for v in vel)
             dt = float(torch.tensor(dt, dtype=torch.float32))
-        self.grid, self.dt, self.round_storage = grid, dt, round_storage
+        self.grid, self.dt, self.rounded = grid, dt, frozenset(rounded)  # fields stored as FP32
         self.f = {"phi": phi.to(dtype).clone()}
         del phi
         self.vel = tuple(v.to(dtype).contiguous() for v in vel)
@@ -53,20 +59,21 @@ class _Run:
         self.st.fill_ghosts(self.f["phi"])
         self.f["s1"], self.f["s2"] = self.st.empty(), self.st.empty()
 
-    def _stored(self, t):
-        if self.round_storage:
+    def _stored(self, name):
+        if name in self.rounded:
+            t = self.f[name]
             t.copy_(t.float().double())
 
     def step(self):
         for form, o, s, c in _FORMS:
             self.st.advect_stage(form, self.f[o], self.f[s], self.f[c] if c else None, self.vel, self.dt)
-            self._stored(self.f[o])
+            self._stored(o)
 
     def reinit(self):
         h = 0.5 * self.grid.dx[0]
         for form, o, s, c in _FORMS:
             self.st.reinit_stage(form, self.f[o], self.f[s], self.f[c] if c else None, None, False, h)
-            self._stored(self.f[o])
+            self._stored(o)
 
     def interior(self):
         return self.grid.interior(self.f["phi"]).double()
@@ -85,10 +92,12 @@ def precision_split(n=512, steps=25, reinit_every=0, reinit_iters=5, zero_tol=1e
     seed 5, Taylor-Green velocity by default), with `reinit_iters`
     reinitialization iterations every `reinit_every` steps (0: none)."""
     t0 = time.perf_counter()
-    runs = {"ref64": _Run("double", n, False, False, None, device, velocity),
-            "in64": _Run("double", n, True, False, None, device, velocity),
-            "st64": _Run("double", n, True, True, None, device, velocity),
-            "fp32": _Run("float", n, False, False, zero_tol, device, velocity)}
+    runs = {"ref64": _Run("double", n, False, (), None, device, velocity),
+            "in64": _Run("double", n, True, (), None, device, velocity),
+            "st64": _Run("double", n, True, ("phi", "s1", "s2"), None, device, velocity),
+            "ph64": _Run("double", n, True, ("phi",), None, device, velocity),
+            "sg64": _Run("double", n, True, ("s1", "s2"), None, device, velocity),
+            "fp32": _Run("float", n, False, (), zero_tol, device, velocity)}
     for s in range(1, steps + 1):
         for r in runs.values():
             r.step()
@@ -103,6 +112,8 @@ def precision_split(n=512, steps=25, reinit_every=0, reinit_iters=5, zero_tol=1e
     out = {"fp32_vs_fp64": _near_max(vals["fp32"], ref, ref, h),
            "inputs_only": _near_max(vals["in64"], ref, ref, h),
            "storage_only": _near_max(vals["st64"], vals["in64"], ref, h),
+           "storage_phi_only": _near_max(vals["ph64"], vals["in64"], ref, h),
+           "storage_stages_only": _near_max(vals["sg64"], vals["in64"], ref, h),
            "fp32_same_inputs": _near_max(vals["fp32"], vals["in64"], ref, h)}
     # the local error of one FP32 step from the final in64 state (rounded to FP32)
     start = vals["in64"].float().double()
