@@ -195,7 +195,7 @@ template <typename T, bool FASTW>
 __device__ __forceinline__ void hj_weno5_pm_d(const T* d, T inv, T tiny_dx2, T& plus, T& minus) {
   if (FASTW) {
     T p, m;
-    weno5_fast_pm<T>(d, tiny_dx2, p, m);
+    weno5_fast_pm<T>(d, tiny_dx2 * T(1.0 / 16.0), p, m);
     plus = p * inv;
     minus = m * inv;
   } else {
@@ -395,6 +395,13 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
   const uint32_t op0 = (KIND == STAGE_REINIT && a.use_phi0) ? plane_byte_offset<T>(a.phi0, ic, jc) : oc;
   const uint32_t oo = (uint32_t)((ic - a.out.ilo) + (int)a.out.sy * (jc - a.out.jlo)) * (uint32_t)sizeof(T);
   const T tiny_dx2x = a.tdx2[0], tiny_dx2y = a.tdx2[1], tiny_dx2z = a.tdx2[2];
+  // FP32 reinitialization: its loop-invariant factors (the WENO weight offsets
+  // and 1/dx) copied to per-lane registers, since a VALU instruction with a
+  // scalar-register operand issues at half the rate of one without
+  T tfx = tiny_dx2x * T(1.0 / 16.0), tfy = tiny_dx2y * T(1.0 / 16.0), tfz = tiny_dx2z * T(1.0 / 16.0);
+  T ivx = a.ix, ivy = a.iy, ivz = a.iz;
+  if constexpr (FASTW && KIND == STAGE_REINIT)
+    asm volatile("" : "+v"(tfx), "+v"(tfy), "+v"(tfz), "+v"(ivx), "+v"(ivy), "+v"(ivz));
   // periodic images: only tiles within 3 cells of a periodic x or y face
   const bool img_xy = IMAGES && (((a.periodic & 1) && (i0 - fb.ilo < 3 || fb.ihi - (i0 + TX - 1) < 3)) ||
                                  ((a.periodic & 2) && (j0 - fb.jlo < 3 || fb.jhi - (j0 + TY - 1) < 3)));
@@ -584,14 +591,21 @@ __device__ __forceinline__ void march_tile(const StageArgs<T>& a, int bx, int by
         T rhs;
         if (KIND == STAGE_REINIT) {
           T xp, xm, yp, ym, zp, zm;
-          hj_weno5_pm_d<T, FASTW>(dx_, a.ix, tiny_dx2x, xp, xm);
-          hj_weno5_pm_d<T, FASTW>(dy_, a.iy, tiny_dx2y, yp, ym);
-          hj_weno5_pm_d<T, FASTW>(dz_, a.iz, tiny_dx2z, zp, zm);
+          if constexpr (FASTW) {  // undivided: 1/dx is applied after the Godunov maxima
+            weno5_fast_pm<T>(dx_, tfx, xp, xm);
+            weno5_fast_pm<T>(dy_, tfy, yp, ym);
+            weno5_fast_pm<T>(dz_, tfz, zp, zm);
+          } else {
+            hj_weno5_pm_d<T, FASTW>(dx_, a.ix, tiny_dx2x, xp, xm);
+            hj_weno5_pm_d<T, FASTW>(dy_, a.iy, tiny_dx2y, yp, ym);
+            hj_weno5_pm_d<T, FASTW>(dz_, a.iz, tiny_dx2z, zp, zm);
+          }
           // lsm_reinitialization3d.f.in:150-268
           const T cur = a.use_phi0 ? cp0 : us;
           rhs = T(0);
           if (fabs(cur) >= a.zero_tol) {
-            const T n2 = godunov_star_sq_max3<T>(cur, xp, yp, zp, xm, ym, zm);
+            const T n2 = FASTW ? godunov_star_sq_undivided<T>(cur, xp, yp, zp, xm, ym, zm, ivx, ivy, ivz)
+                               : godunov_star_sq_max3<T>(cur, xp, yp, zp, xm, ym, zm);
             if constexpr (FASTW) {
               // FP32: the smoothed sign through the hardware reciprocal square root
               // and |grad phi|* through the hardware square root (about 1 ulp each)

@@ -121,9 +121,10 @@ __device__ __forceinline__ T weno5_fast(T D1, T D2, T D3, T D4, T D5, T tiny_dx2
 // (e4, e3, e2, e1); c^2 = e2^2 and 2.5 c^2 are the same in both, and max|D|
 // shares max(|d1|..|d4|).  Negation and the maximum are exact, so both
 // results equal weno5_fast(d0..d4) and weno5_fast(d5..d1) bit for bit.
-// Returns the two undivided derivatives (multiply by 1/dx).
+// Returns the two undivided derivatives (multiply by 1/dx).  tf is the
+// weight offset tiny * dx^2 / 16 (the caller forms it once).
 template <typename T>
-__device__ __forceinline__ void weno5_fast_pm(const T* d, T tiny_dx2, T& plus, T& minus) {
+__device__ __forceinline__ void weno5_fast_pm(const T* d, T tf, T& plus, T& minus) {
 #pragma clang fp contract(off)
   const T e0 = d[1] - d[0], e1 = d[2] - d[1], e2 = d[3] - d[2], e3 = d[4] - d[3], e4 = d[5] - d[4];
   T mid, mm, mp;
@@ -136,7 +137,6 @@ __device__ __forceinline__ void weno5_fast_pm(const T* d, T tiny_dx2, T& plus, T
     mm = fmax(fmax(mid, fabs(d[0])), fabs(d[4]));
     mp = fmax(fmax(mid, fabs(d[4])), fabs(d[5]));
   }
-  const T tf = tiny_dx2 * T(1.0 / 16.0);
   const T iem = fast_rcp(fma(T(0.75e-6), mm * mm, tf)), iep = fast_rcp(fma(T(0.75e-6), mp * mp, tf));
   const T cc = e2 * e2, cc25 = T(2.5) * cc;
   // minus: (a, b, c, d) = -(e0, e1, e2, e3)
@@ -178,6 +178,21 @@ __device__ __forceinline__ T godunov_star_sq_max3(T s, T xp, T yp, T zp, T xm, T
   const T gx = pos ? fmax(fmax(-xp, xm), z) : fmax(fmax(xp, -xm), z);
   const T gy = pos ? fmax(fmax(-yp, ym), z) : fmax(fmax(yp, -ym), z);
   const T gz = pos ? fmax(fmax(-zp, zm), z) : fmax(fmax(zp, -zm), z);
+  return gx * gx + gy * gy + gz * gz;
+}
+
+// The same from undivided one-sided derivatives, each axis's |.|* scaled by its
+// 1/dx after the maxima: multiplying by a positive factor is monotonic and odd
+// under rounding, so max(-a f, b f, 0) = f max(-a, b, 0) exactly and the result
+// equals godunov_star_sq_max3 on the scaled derivatives bit for bit.
+template <typename T>
+__device__ __forceinline__ T godunov_star_sq_undivided(T s, T xp, T yp, T zp, T xm, T ym, T zm, T ix, T iy, T iz) {
+#pragma clang fp contract(off)
+  const T z = T(0);
+  const bool pos = s > z;
+  const T gx = ix * (pos ? fmax(fmax(-xp, xm), z) : fmax(fmax(xp, -xm), z));
+  const T gy = iy * (pos ? fmax(fmax(-yp, ym), z) : fmax(fmax(yp, -ym), z));
+  const T gz = iz * (pos ? fmax(fmax(-zp, zm), z) : fmax(fmax(zp, -zm), z));
   return gx * gx + gy * gy + gz * gz;
 }
 
